@@ -1,0 +1,135 @@
+"""Benchmark of the JPEG 2000 encode hot path on one MI355X.
+
+Workload (the headline configuration): one 4096x4096 interleaved RGB uint8
+image (synthetic, seeded), resident in HBM; one step = the whole encode on
+the device: RGB -> planar + ICT, irreversible 9/7 FP32 DWT with 6 levels and
+scalar quantisation at step 1/64, EBCOT tier-1 on 64x64 code-blocks.
+value = image Mpix/s over the timed steps.
+
+Also reported:
+  roofline      : the level-1 9/7 DWT+quantisation kernel (the DWT's dominant
+                  launch), algorithmic bytes 8 B/px/plane (FP32 in, LL FP32 +
+                  int32 indices out) over its HIP-event-timed launch, vs 8 TB/s.
+  stages_ms     : per-stage times (colour, DWT all levels, tier-1).
+  cpu_baseline  : the CPU restatement (oracle) of the same encode on the host
+                  cores, on the full image.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Encode Mpix/s (4096² RGB, DWT+EBCOT); DWT %HBM peak; batched vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (single-GPU hot path; see DESIGN.md "
+              "for how it would shard); --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from roger_amd import runtime as R
+    from roger_amd import synth
+    from roger_amd._lib import RG_COLOUR_ICT, RG_FILTER_97, lib
+
+    W = H = args.size
+    NC, LEVELS, DELTA, CB = 3, 6, 1.0 / 64, 64
+    print("generating %dx%d RGB input" % (W, H), file=sys.stderr, flush=True)
+    img = synth.rgb(W, H, 1)
+    ctx = R.Context(0)
+    d_src = ctx.upload(img)
+    enc = R.Encoder(ctx)
+    run = lambda: enc.encode_u8(None, levels=LEVELS, reversible=False, cbw=CB, cbh=CB, delta=DELTA,
+                                d_src=d_src, w=W, h=H, ncomp=NC)
+    for _ in range(args.warmup):
+        run()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run()
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    mpix = W * H / 1e6
+    ms_step = dt * 1e3 / args.steps
+    value = mpix * args.steps / dt
+    ncb = lib().rg_encoder_num_codeblocks(enc.handle)
+    _, res, blobs = enc.codeblocks()
+    total_bytes = int(res["len"].sum())
+
+    # ---- per-stage timing with HIP events on the context stream ----
+    step, inv, _, _ = R.band_steps(LEVELS, DELTA)
+    d_pl = ctx.alloc(NC * W * H * 4)
+    d_q = ctx.alloc(NC * W * H * 4)
+    d_s = ctx.alloc(R.scratch_bytes(W, H, NC, RG_FILTER_97))
+    reps = max(5, args.steps)
+
+    def timed(fn):
+        fn()
+        ctx.timer_start()
+        for _ in range(reps):
+            fn()
+        return ctx.timer_stop() / reps
+
+    t_colour = timed(lambda: R.colour_fwd(ctx, d_src, W, H, NC, RG_COLOUR_ICT, 1, d_pl))
+    inv1 = R.band_steps(1, DELTA)[1]
+    t_dwt_l1 = timed(lambda: R.dwt_fwd(ctx, RG_FILTER_97, d_pl, W, H, NC, 1, inv1, 0, d_q, d_s))
+    t_dwt = timed(lambda: R.dwt_fwd(ctx, RG_FILTER_97, d_pl, W, H, NC, LEVELS, inv, 0, d_q, d_s))
+    desc = R.codeblocks(W, H, LEVELS, CB, CB, NC)
+    d_desc = ctx.upload(desc)
+    slot = CB * CB * 3
+    d_out = ctx.alloc(len(desc) * slot)
+    d_res = ctx.alloc(len(desc) * 16)
+    import ctypes
+    t_t1 = timed(lambda: lib().rg_t1_encode(ctx.handle, ctypes.c_void_p(d_q.ptr), W, W * H,
+                                              ctypes.c_void_p(d_desc.ptr), len(desc), ctypes.c_void_p(d_out.ptr),
+                                              slot, ctypes.c_void_p(d_res.ptr)))
+    l1_bytes = 8.0 * W * H * NC
+    achieved = l1_bytes / (t_dwt_l1 * 1e-3) / 1e9
+    colour_gbs = 15.0 * W * H / (t_colour * 1e-3) / 1e9
+
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "%dx%d RGB uint8 -> ICT -> 9/7 FP32 %d levels, step 1/64 -> EBCOT %dx%d "
+                               "(plain per-image path, input resident in HBM)" % (W, H, LEVELS, CB, CB),
+                   "image": [W, H, NC], "levels": LEVELS, "delta": DELTA, "codeblock": [CB, CB],
+                   "codeblocks": int(ncb), "codestream_bytes": total_bytes},
+        "roofline": {"bound": "hbm", "kernel": "fwd_level_kernel<9/7> (level 1, 3 planes)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": l1_bytes, "avg_launch_ms": round(t_dwt_l1, 4)},
+        "stages_ms": {"colour_ict": round(t_colour, 4), "colour_GBps": round(colour_gbs, 1),
+                      "dwt97_quant_6lv": round(t_dwt, 4), "dwt97_level1": round(t_dwt_l1, 4),
+                      "t1_encode": round(t_t1, 4),
+                      "t1_codeblocks_per_s": round(len(desc) / (t_t1 * 1e-3), 1)},
+    }
+
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = min(os.cpu_count() or 1, 16)
+        t = time.perf_counter()
+        O.encode_image(img, levels=LEVELS, reversible=False, cbw=CB, cbh=CB, delta=DELTA, nthreads=threads)
+        cdt = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": round(mpix / cdt, 3), "unit": "Mpix/s", "cores": threads, "kind": "port",
+                               "sample": "full %dx%d RGB image, one encode (ICT, 9/7 6 levels, quantisation, "
+                                         "tier-1), OpenMP over code-blocks" % (W, H)}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

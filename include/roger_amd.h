@@ -1,0 +1,204 @@
+/*
+ * roger_amd — C ABI of the MI355X (gfx950) JPEG 2000 encode/decode hot path.
+ *
+ * Plain C: opaque handles, POD structs, plain pointers and sizes.  No HIP or
+ * torch types appear here; the C++ drop-in classes in include/roger/ (named
+ * after the reference's OCLEncoder, OCLDecoder, OCLDWTForward, OCLDWTRev,
+ * OCLBPC, OCLRGBtoPlanar, OCLMemoryManager) forward to these entry points.
+ *
+ * Return convention: 0 (RG_SUCCESS) on success, a negative RG_ERR_* code on
+ * failure, with a message available from rg_last_error().  This replaces the
+ * reference's cl_int tDeviceRC / DeviceSuccess (src/ocl_platform.h:22-24).
+ * All "d_" pointers are device (HBM) addresses; kernel launches are
+ * asynchronous on the context's stream, rg_context_sync() waits.
+ */
+#ifndef ROGER_AMD_H
+#define ROGER_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RG_SUCCESS 0
+#define RG_ERR_INVALID (-1)     /* bad argument or unsupported shape */
+#define RG_ERR_DEVICE (-2)      /* HIP runtime error */
+#define RG_ERR_OOM (-3)         /* device or pinned allocation failed */
+#define RG_ERR_OVERFLOW (-4)    /* a code-block did not fit its output slot */
+#define RG_ERR_NOT_READY (-5)   /* result requested before the stage ran */
+
+typedef struct rg_context rg_context;
+typedef struct rg_encoder rg_encoder;
+typedef struct rg_decoder rg_decoder;
+
+/* Sample types for planar host components. */
+#define RG_F32 0
+#define RG_I16 1
+#define RG_I32 2
+
+/* Colour transforms. */
+#define RG_COLOUR_NONE 0 /* level shift only, per channel */
+#define RG_COLOUR_RCT 1  /* reversible (integer) component transform */
+#define RG_COLOUR_ICT 2  /* irreversible (FP32) component transform */
+
+/* Wavelet filters. */
+#define RG_FILTER_53 0
+#define RG_FILTER_97 1
+
+/* One code-block: rectangle in the Mallat-packed coefficient plane of
+ * component `comp`, plus its band orientation (0 LL, 1 HL, 2 LH, 3 HH). */
+typedef struct {
+    int32_t x0, y0, w, h;
+    int32_t comp, orient, band, res;
+} rg_cb_desc;
+
+/* Tier-1 output of one code-block. */
+typedef struct {
+    int32_t len;     /* bytes of the MQ codeword (all coding passes) */
+    int32_t npasses; /* 3*nbp-2, or 0 for an all-zero block */
+    int32_t nbp;     /* magnitude bit-planes coded (msb index + 1) */
+    int32_t flags;   /* bit 0: codeword larger than the slot (error) */
+} rg_cb_result;
+
+/* Encoder parameters (T.800 codec choices of the native path). */
+typedef struct {
+    int32_t levels;        /* DWT decomposition levels (0..15) */
+    int32_t filter;        /* RG_FILTER_53 or RG_FILTER_97 */
+    int32_t colour;        /* RG_COLOUR_* (RCT with 5/3, ICT with 9/7) */
+    int32_t cbw, cbh;      /* code-block size: 32 or 64 (w*h <= 4096) */
+    double delta;          /* 9/7 base quantisation step (e.g. 1/64) */
+    int32_t ll_fp16;       /* 9/7: store the inter-level LL band as FP16 */
+    int32_t bitdepth;      /* sample precision (8) */
+} rg_encode_params;
+
+/* ---- errors ---- */
+const char *rg_error_string(int code);
+const char *rg_last_error(void);
+
+/* ---- context: device + stream (replaces ocl_args_d_t, src/OCLUtil.h:31-39,
+ *      and OCLDeviceManager::init/getInfo, src/OCLDeviceManager.h:27-39) ---- */
+int rg_context_create(int device, rg_context **out);
+int rg_context_destroy(rg_context *ctx);
+int rg_context_sync(rg_context *ctx);                 /* OCLEncodeDecode::finish, src/OCLEncodeDecode.cpp:33-36 */
+int rg_context_device_name(rg_context *ctx, char *buf, int cap);
+
+/* ---- device memory plumbing ---- */
+int rg_malloc(rg_context *ctx, size_t bytes, void **d_ptr);
+int rg_free(rg_context *ctx, void *d_ptr);
+int rg_memcpy_h2d(rg_context *ctx, void *d_dst, const void *src, size_t bytes);   /* stream-ordered, waits */
+int rg_memcpy_d2h(rg_context *ctx, void *dst, const void *d_src, size_t bytes);   /* stream-ordered, waits */
+int rg_memset(rg_context *ctx, void *d_dst, int value, size_t bytes);
+
+/* ---- timing on the context stream (HIP events) ---- */
+int rg_timer_start(rg_context *ctx);
+int rg_timer_stop(rg_context *ctx, float *ms);        /* waits for the stop event */
+
+/* ---- stage kernels (device pointers, asynchronous) ---- */
+
+/* Interleaved uint8 (ncomp channels, w*h pixels) -> level-shifted planes.
+ * colour RG_COLOUR_NONE: each channel shifted by -128 into its plane;
+ * RCT: int32 planes Y,U,V; ICT: float planes Y,Cb,Cr.  out_float selects
+ * float (1) or int32 (0) planes for NONE.  Planes are contiguous w*h each.
+ * Replaces the host level shift + interleave of src/OCLTest.cpp:81-90 and
+ * src/OCLMemoryManager.cpp:41-54 (the reference has no RCT/ICT). */
+int rg_colour_fwd(rg_context *ctx, const uint8_t *d_src, int w, int h, int ncomp,
+                  int colour, int out_float, void *d_planes);
+/* Planes -> interleaved uint8 with inverse transform, +128 and clamping. */
+int rg_colour_inv(rg_context *ctx, const void *d_planes, int w, int h, int ncomp,
+                  int colour, int in_float, uint8_t *d_dst);
+
+/* Interleaved int16 coefficient image (ncomp channels) -> ncomp int16 planes.
+ * Replaces OCLRGBtoPlanar::run (src/OCLRGBtoPlanar.cpp:36-80,
+ * src/CL/oclplanar.cl:23-39). */
+int rg_planar_split_i16(rg_context *ctx, const int16_t *d_src, int w, int h, int ncomp,
+                        int16_t *d_planes);
+
+/* Scratch bytes the multi-level DWT entry points need for a w x h plane
+ * (nplanes planes transformed together). */
+size_t rg_dwt_scratch_bytes(int w, int h, int nplanes, int filter);
+
+/* Forward multi-level DWT of nplanes planes (each w*h, contiguous, plane
+ * stride w*h) into Mallat-packed int32 coefficient planes d_q.
+ *  5/3: d_in int32 samples, d_q receives the integer coefficients.
+ *  9/7: d_in float samples, d_q receives quantisation indices
+ *       sign*floor(|y| * inv_step[band]); inv_step has 3*levels+1 entries
+ *       in band order (LL_N, then HL, LH, HH for level N..1).
+ * Replaces OCLDWTForward::run (src/OCLDWTForward.cpp:124-137) with its
+ * per-level kernels src/CL/ocldwt53.cl:167-292 and
+ * src/CL/ocldwt97.cl:487-717. */
+int rg_dwt_fwd(rg_context *ctx, int filter, const void *d_in, int w, int h, int nplanes,
+               int levels, const float *inv_step, int ll_fp16, int32_t *d_q, void *d_scratch);
+/* 9/7 only: forward DWT without quantisation, float Mallat output
+ * (the reference's outputDwt path, src/CL/ocldwt97.cl:196-420). */
+int rg_dwt97_fwd_float(rg_context *ctx, const float *d_in, int w, int h, int nplanes, int levels,
+                       float *d_out, void *d_scratch);
+/* Inverse multi-level DWT (the decoder side of OCLDWTRev, src/OCLDWTRev.cpp:39-59).
+ *  5/3: d_q int32 coefficients -> int32 samples in d_out.
+ *  9/7: d_q quantisation indices, dequantised as sign*(|q|+0.5)*step[band]
+ *       (step == NULL: d_q is read as float coefficients), -> float samples. */
+int rg_dwt_inv(rg_context *ctx, int filter, const void *d_q, int w, int h, int nplanes,
+               int levels, const float *step, void *d_out, void *d_scratch);
+
+/* Code-block list for a w x h image with `levels` levels and cbw x cbh
+ * blocks, per component in resolution / band / raster order.  Returns the
+ * count per component; fills at most cap descriptors (host memory) for
+ * component 0 (callers replicate per component with comp set). */
+int rg_codeblocks(int w, int h, int levels, int cbw, int cbh, rg_cb_desc *out, int cap);
+
+/* EBCOT tier-1 encode of ncb code-blocks (descriptors in device memory)
+ * over int32 coefficient planes (plane c at d_q + c*plane_stride, row
+ * pitch `pitch`).  Each block's codeword goes to d_out + i*slot_bytes.
+ * Replaces OCLBPC::run (src/OCLBPC.cpp:38-51, src/CL/oclbpc.cl:279-746). */
+int rg_t1_encode(rg_context *ctx, const int32_t *d_q, int pitch, size_t plane_stride,
+                 const rg_cb_desc *d_desc, int ncb, uint8_t *d_out, int slot_bytes,
+                 rg_cb_result *d_res);
+
+/* ---- encoder: owns device buffers (OCLMemoryManager) and runs the plain
+ *      per-image path (OCLEncoder, src/OCLEncoder.h:27-37) ---- */
+int rg_encoder_create(rg_context *ctx, rg_encoder **out);
+int rg_encoder_destroy(rg_encoder *enc);
+/* Native path: interleaved uint8 image (device pointer if src_on_device,
+ * else host) -> colour -> DWT (+quantisation) -> tier-1.  Asynchronous
+ * when the source is on the device. */
+int rg_encoder_encode_u8(rg_encoder *enc, const uint8_t *src, int src_on_device, int w, int h,
+                         int ncomp, const rg_encode_params *p);
+/* Drop-in path of OCLEncoder<T>::run(components, w, h, levels, precision):
+ * ncomp planar host components of sample type RG_F32/RG_I16/RG_I32, already
+ * level shifted by the caller (src/OCLTest.cpp:87).  lossy selects 9/7
+ * (+ quantisation unless output_dwt) else 5/3; code-blocks cbw x cbh. */
+int rg_encoder_run_planar(rg_encoder *enc, const void *const *components, int ncomp, int sample_type,
+                          int w, int h, int levels, int precision, int lossy, int output_dwt,
+                          int cbw, int cbh, double delta);
+/* Device pointers of the last run (valid until the next run / destroy). */
+int rg_encoder_coefficients(rg_encoder *enc, int32_t **d_q, int *ncomp, int *w, int *h);
+int rg_encoder_float_coefficients(rg_encoder *enc, float **d_coeff);
+/* Number of code-blocks of the last run (all components). */
+int rg_encoder_num_codeblocks(rg_encoder *enc);
+/* Copies per-block results (ncb) and codewords to host; data gets the
+ * concatenated codewords (cap bytes), offsets[i] their start.  Waits. */
+int rg_encoder_get_codeblocks(rg_encoder *enc, rg_cb_desc *desc, rg_cb_result *res, long *offsets,
+                              uint8_t *data, long cap, long *total);
+/* mapDWTOut / unmapDWTOut (src/OCLEncodeDecode.cpp:42-48): host view of the
+ * coefficients, interleaved by component — int16 saturated quantisation
+ * indices / 5/3 coefficients, or float when output_dwt on the lossy path. */
+int rg_encoder_map_dwt_out(rg_encoder *enc, void **host_ptr);
+int rg_encoder_unmap_dwt_out(rg_encoder *enc, void *host_ptr);
+
+/* ---- decoder: inverse DWT of planar coefficients (OCLDecoder,
+ *      src/OCLDecoder.h:25-34) ---- */
+int rg_decoder_create(rg_context *ctx, rg_decoder **out);
+int rg_decoder_destroy(rg_decoder *dec);
+/* components: ncomp Mallat-packed coefficient planes (host) of sample type
+ * RG_F32 (lossy: 9/7 float coefficients) or RG_I16/RG_I32 (5/3). */
+int rg_decoder_run_planar(rg_decoder *dec, const void *const *components, int ncomp, int sample_type,
+                          int w, int h, int levels, int precision, int lossy);
+/* Host view of the reconstructed samples, interleaved by component (float
+ * for lossy, int16 for lossless), still level shifted. */
+int rg_decoder_map_out(rg_decoder *dec, void **host_ptr);
+int rg_decoder_unmap_out(rg_decoder *dec, void *host_ptr);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,121 @@
+"""ctypes binding of libroger_amd.so (the C ABI declared in include/roger_amd.h).
+
+The library must have been built (``python -m roger_amd.build`` or
+``__graft_entry__.build()``); there is no fallback path: a missing library or
+a missing GPU raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libroger_amd.so")
+
+RG_SUCCESS = 0
+RG_ERR_INVALID = -1
+RG_ERR_DEVICE = -2
+RG_ERR_OOM = -3
+RG_ERR_OVERFLOW = -4
+RG_ERR_NOT_READY = -5
+
+RG_F32, RG_I16, RG_I32 = 0, 1, 2
+RG_COLOUR_NONE, RG_COLOUR_RCT, RG_COLOUR_ICT = 0, 1, 2
+RG_FILTER_53, RG_FILTER_97 = 0, 1
+
+
+class RogerError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (_ERRNAMES.get(code, "error"), code, msg))
+        self.code = code
+
+
+_ERRNAMES = {RG_ERR_INVALID: "RG_ERR_INVALID", RG_ERR_DEVICE: "RG_ERR_DEVICE", RG_ERR_OOM: "RG_ERR_OOM",
+             RG_ERR_OVERFLOW: "RG_ERR_OVERFLOW", RG_ERR_NOT_READY: "RG_ERR_NOT_READY"}
+
+
+class CbDesc(ctypes.Structure):
+    _fields_ = [("x0", ctypes.c_int32), ("y0", ctypes.c_int32), ("w", ctypes.c_int32), ("h", ctypes.c_int32),
+                ("comp", ctypes.c_int32), ("orient", ctypes.c_int32), ("band", ctypes.c_int32),
+                ("res", ctypes.c_int32)]
+
+
+class CbResult(ctypes.Structure):
+    _fields_ = [("len", ctypes.c_int32), ("npasses", ctypes.c_int32), ("nbp", ctypes.c_int32),
+                ("flags", ctypes.c_int32)]
+
+
+class EncodeParams(ctypes.Structure):
+    _fields_ = [("levels", ctypes.c_int32), ("filter", ctypes.c_int32), ("colour", ctypes.c_int32),
+                ("cbw", ctypes.c_int32), ("cbh", ctypes.c_int32), ("delta", ctypes.c_double),
+                ("ll_fp16", ctypes.c_int32), ("bitdepth", ctypes.c_int32)]
+
+
+P = ctypes.c_void_p
+I = ctypes.c_int
+SZ = ctypes.c_size_t
+L = ctypes.c_long
+
+# name -> (restype, argtypes); the list is the exported surface of include/roger_amd.h
+SIGNATURES = {
+    "rg_error_string": (ctypes.c_char_p, [I]),
+    "rg_last_error": (ctypes.c_char_p, []),
+    "rg_context_create": (I, [I, P]),
+    "rg_context_destroy": (I, [P]),
+    "rg_context_sync": (I, [P]),
+    "rg_context_device_name": (I, [P, ctypes.c_char_p, I]),
+    "rg_malloc": (I, [P, SZ, P]),
+    "rg_free": (I, [P, P]),
+    "rg_memcpy_h2d": (I, [P, P, P, SZ]),
+    "rg_memcpy_d2h": (I, [P, P, P, SZ]),
+    "rg_memset": (I, [P, P, I, SZ]),
+    "rg_timer_start": (I, [P]),
+    "rg_timer_stop": (I, [P, P]),
+    "rg_colour_fwd": (I, [P, P, I, I, I, I, I, P]),
+    "rg_colour_inv": (I, [P, P, I, I, I, I, I, P]),
+    "rg_planar_split_i16": (I, [P, P, I, I, I, P]),
+    "rg_dwt_scratch_bytes": (SZ, [I, I, I, I]),
+    "rg_dwt_fwd": (I, [P, I, P, I, I, I, I, P, I, P, P]),
+    "rg_dwt97_fwd_float": (I, [P, P, I, I, I, I, P, P]),
+    "rg_dwt_inv": (I, [P, I, P, I, I, I, I, P, P, P]),
+    "rg_codeblocks": (I, [I, I, I, I, I, P, I]),
+    "rg_t1_encode": (I, [P, P, I, SZ, P, I, P, I, P]),
+    "rg_encoder_create": (I, [P, P]),
+    "rg_encoder_destroy": (I, [P]),
+    "rg_encoder_encode_u8": (I, [P, P, I, I, I, I, P]),
+    "rg_encoder_run_planar": (I, [P, P, I, I, I, I, I, I, I, I, I, I, ctypes.c_double]),
+    "rg_encoder_coefficients": (I, [P, P, P, P, P]),
+    "rg_encoder_float_coefficients": (I, [P, P]),
+    "rg_encoder_num_codeblocks": (I, [P]),
+    "rg_encoder_get_codeblocks": (I, [P, P, P, P, P, L, P]),
+    "rg_encoder_map_dwt_out": (I, [P, P]),
+    "rg_encoder_unmap_dwt_out": (I, [P, P]),
+    "rg_decoder_create": (I, [P, P]),
+    "rg_decoder_destroy": (I, [P]),
+    "rg_decoder_run_planar": (I, [P, P, I, I, I, I, I, I, I]),
+    "rg_decoder_map_out": (I, [P, P]),
+    "rg_decoder_unmap_out": (I, [P, P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libroger_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libroger_amd.so is not built: run __graft_entry__.build() "
+                               "(or python -m roger_amd.build)")
+        L_ = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L_, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L_
+    return _lib
+
+
+def check(rc):
+    if rc < 0:
+        msg = lib().rg_last_error()
+        raise RogerError(rc, msg.decode() if msg else "")
+    return rc

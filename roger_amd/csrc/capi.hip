@@ -1,0 +1,759 @@
+// C ABI of libroger_amd: contexts, device memory, stage entry points, the
+// multi-level DWT drivers and the encoder / decoder objects that own their
+// device buffers (the role of the reference's OCLMemoryManager,
+// src/OCLMemoryManager.cpp:56-148: buffers are reallocated only when the
+// image shape grows, so repeated runs keep everything resident in HBM).
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <cmath>
+#include <vector>
+#include <algorithm>
+#include "rg_internal.h"
+
+// ------------------------------------------------------------ errors
+static thread_local char g_err[512] = "";
+
+int rg_fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+extern "C" const char *rg_last_error(void) { return g_err; }
+
+extern "C" const char *rg_error_string(int code)
+{
+    switch (code) {
+    case RG_SUCCESS: return "success";
+    case RG_ERR_INVALID: return "invalid argument";
+    case RG_ERR_DEVICE: return "device (HIP) error";
+    case RG_ERR_OOM: return "out of memory";
+    case RG_ERR_OVERFLOW: return "code-block output overflow";
+    case RG_ERR_NOT_READY: return "no result available";
+    default: return "unknown error";
+    }
+}
+
+// ------------------------------------------------------------ context
+extern "C" int rg_context_create(int device, rg_context **out)
+{
+    RG_CHECK(out, "rg_context_create: null output");
+    int n = 0;
+    RG_HIP(hipGetDeviceCount(&n));
+    RG_CHECK(device >= 0 && device < n, "rg_context_create: device %d out of range (%d devices)", device, n);
+    RG_HIP(hipSetDevice(device));
+    rg_context *c = new rg_context();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev_start);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev_stop);
+    if (e != hipSuccess) {
+        delete c;
+        return rg_fail(RG_ERR_DEVICE, "rg_context_create: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_context_destroy(rg_context *c)
+{
+    if (!c) return RG_SUCCESS;
+    hipSetDevice(c->device);
+    if (c->stream) { hipStreamSynchronize(c->stream); hipStreamDestroy(c->stream); }
+    if (c->ev_start) hipEventDestroy(c->ev_start);
+    if (c->ev_stop) hipEventDestroy(c->ev_stop);
+    delete c;
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_context_sync(rg_context *c)
+{
+    RG_CHECK(c, "null context");
+    RG_HIP(hipStreamSynchronize(c->stream));
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_context_device_name(rg_context *c, char *buf, int cap)
+{
+    RG_CHECK(c && buf && cap > 0, "bad arguments");
+    hipDeviceProp_t p;
+    RG_HIP(hipGetDeviceProperties(&p, c->device));
+    snprintf(buf, cap, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    return RG_SUCCESS;
+}
+
+// ------------------------------------------------------------ memory
+extern "C" int rg_malloc(rg_context *c, size_t bytes, void **p)
+{
+    RG_CHECK(c && p, "bad arguments");
+    RG_HIP(hipSetDevice(c->device));
+    if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) {
+        (void)hipGetLastError();
+        return rg_fail(RG_ERR_OOM, "hipMalloc(%zu) failed", bytes);
+    }
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_free(rg_context *c, void *p)
+{
+    RG_CHECK(c, "null context");
+    if (p) RG_HIP(hipFree(p));
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_memcpy_h2d(rg_context *c, void *dst, const void *src, size_t bytes)
+{
+    RG_CHECK(c, "null context");
+    RG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    RG_HIP(hipStreamSynchronize(c->stream));
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_memcpy_d2h(rg_context *c, void *dst, const void *src, size_t bytes)
+{
+    RG_CHECK(c, "null context");
+    RG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    RG_HIP(hipStreamSynchronize(c->stream));
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_memset(rg_context *c, void *dst, int value, size_t bytes)
+{
+    RG_CHECK(c, "null context");
+    RG_HIP(hipMemsetAsync(dst, value, bytes, c->stream));
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_timer_start(rg_context *c)
+{
+    RG_CHECK(c, "null context");
+    RG_HIP(hipEventRecord(c->ev_start, c->stream));
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_timer_stop(rg_context *c, float *ms)
+{
+    RG_CHECK(c && ms, "bad arguments");
+    RG_HIP(hipEventRecord(c->ev_stop, c->stream));
+    RG_HIP(hipEventSynchronize(c->ev_stop));
+    RG_HIP(hipEventElapsedTime(ms, c->ev_start, c->ev_stop));
+    return RG_SUCCESS;
+}
+
+// ------------------------------------------------------------ stages
+extern "C" int rg_colour_fwd(rg_context *c, const uint8_t *src, int w, int h, int ncomp, int colour,
+                             int out_float, void *planes)
+{
+    RG_CHECK(c && src && planes, "rg_colour_fwd: null argument");
+    RG_CHECK(w > 0 && h > 0 && ncomp >= 1 && ncomp <= 4, "rg_colour_fwd: bad shape %dx%dx%d", w, h, ncomp);
+    RG_CHECK(colour >= 0 && colour <= 2, "rg_colour_fwd: bad colour mode %d", colour);
+    RG_CHECK(((uintptr_t)src & 15) == 0 && ((uintptr_t)planes & 15) == 0, "rg_colour_fwd: buffers must be 16-byte aligned");
+    if (colour == RG_COLOUR_RCT) out_float = 0;
+    if (colour == RG_COLOUR_ICT) out_float = 1;
+    return rg_launch_colour_fwd(c->stream, src, w, h, ncomp, colour, out_float, planes);
+}
+
+extern "C" int rg_colour_inv(rg_context *c, const void *planes, int w, int h, int ncomp, int colour,
+                             int in_float, uint8_t *dst)
+{
+    RG_CHECK(c && dst && planes, "rg_colour_inv: null argument");
+    RG_CHECK(w > 0 && h > 0 && ncomp >= 1 && ncomp <= 4, "rg_colour_inv: bad shape %dx%dx%d", w, h, ncomp);
+    RG_CHECK(colour >= 0 && colour <= 2, "rg_colour_inv: bad colour mode %d", colour);
+    RG_CHECK(((uintptr_t)dst & 15) == 0 && ((uintptr_t)planes & 15) == 0, "rg_colour_inv: buffers must be 16-byte aligned");
+    if (colour == RG_COLOUR_RCT) in_float = 0;
+    if (colour == RG_COLOUR_ICT) in_float = 1;
+    return rg_launch_colour_inv(c->stream, planes, w, h, ncomp, colour, in_float, dst);
+}
+
+extern "C" int rg_planar_split_i16(rg_context *c, const int16_t *src, int w, int h, int ncomp, int16_t *planes)
+{
+    RG_CHECK(c && src && planes, "rg_planar_split_i16: null argument");
+    RG_CHECK(w > 0 && h > 0 && ncomp >= 1 && ncomp <= 4, "rg_planar_split_i16: bad shape");
+    return rg_launch_planar_split_i16(c->stream, src, w, h, ncomp, planes);
+}
+
+static int check_levels(int w, int h, int levels)
+{
+    RG_CHECK(w > 0 && h > 0, "bad image size %dx%d", w, h);
+    RG_CHECK(levels >= 0 && levels <= 32, "levels %d out of range 0..32", levels);
+    return RG_SUCCESS;
+}
+
+extern "C" size_t rg_dwt_scratch_bytes(int w, int h, int nplanes, int filter)
+{
+    (void)filter;
+    size_t w1 = (w + 1) / 2, h1 = (h + 1) / 2, w2 = (w1 + 1) / 2, h2 = (h1 + 1) / 2;
+    return (w1 * h1 + w2 * h2) * (size_t)nplanes * 4 + 512;
+}
+
+// Level-0 (no transform) copy / quantisation: plane in -> q.
+int rg_launch_copy_quant(hipStream_t s, int filter, const void *in, long n, int nplanes, float inv, int q_float,
+                         void *q);
+int rg_launch_copy_dequant(hipStream_t s, int filter, const void *q, long n, int nplanes, float step, int q_float,
+                           void *out);
+
+static int dwt_fwd_impl(rg_context *c, int filter, const void *in, int w, int h, int nplanes, int levels,
+                        const float *inv_step, int ll_fp16, void *dq, int q_float, void *scratch)
+{
+    int rc = check_levels(w, h, levels);
+    if (rc) return rc;
+    RG_CHECK(filter == RG_FILTER_53 || filter == RG_FILTER_97, "bad filter %d", filter);
+    RG_CHECK(in && dq && nplanes >= 1, "null buffer");
+    RG_CHECK(filter == RG_FILTER_53 || q_float || inv_step, "9/7 quantisation needs inv_step");
+    if (levels == 0)
+        return rg_launch_copy_quant(c->stream, filter, in, (long)w * h, nplanes,
+                                    (filter == RG_FILTER_97 && !q_float) ? inv_step[0] : 1.0f, q_float, dq);
+    RG_CHECK(scratch || levels == 1, "scratch buffer required");
+    rg_level_dims d = rg_dims(w, h, levels);
+    size_t sizeA = (size_t)d.w[1] * d.h[1] * nplanes;
+    char *bufA = reinterpret_cast<char *>(scratch);
+    char *bufB = bufA ? bufA + ((sizeA * 4 + 255) / 256) * 256 : nullptr;
+    const bool half = filter == RG_FILTER_97 && ll_fp16;
+    for (int l = 1; l <= levels; ++l) {
+        rg_fwd_level_args a{};
+        a.cw = d.w[l - 1]; a.ch = d.h[l - 1]; a.nplanes = nplanes;
+        if (l == 1) {
+            a.in = in; a.in_pitch = w; a.in_ps = (long)w * h; a.in_half = 0;
+        } else {
+            a.in = (l - 1) & 1 ? bufA : bufB;
+            a.in_pitch = d.w[l - 1]; a.in_ps = (long)d.w[l - 1] * d.h[l - 1]; a.in_half = half;
+        }
+        if (l < levels) {
+            a.ll_out = l & 1 ? bufA : bufB;
+            a.ll_pitch = d.w[l]; a.ll_ps = (long)d.w[l] * d.h[l]; a.ll_half = half;
+        } else {
+            a.ll_out = nullptr;
+        }
+        a.q = dq; a.qp = w; a.q_ps = (long)w * h; a.q_float = q_float;
+        if (filter == RG_FILTER_97 && !q_float) {
+            const int b = 1 + 3 * (levels - l);  // HL of level l
+            a.inv_hl = inv_step[b]; a.inv_lh = inv_step[b + 1]; a.inv_hh = inv_step[b + 2];
+            a.inv_ll = inv_step[0];
+        }
+        rc = filter == RG_FILTER_53 ? rg_launch_fwd53_level(c->stream, a) : rg_launch_fwd97_level(c->stream, a);
+        if (rc) return rc;
+    }
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_dwt_fwd(rg_context *c, int filter, const void *in, int w, int h, int nplanes, int levels,
+                          const float *inv_step, int ll_fp16, int32_t *dq, void *scratch)
+{
+    RG_CHECK(c, "null context");
+    return dwt_fwd_impl(c, filter, in, w, h, nplanes, levels, inv_step, ll_fp16, dq, 0, scratch);
+}
+
+extern "C" int rg_dwt97_fwd_float(rg_context *c, const float *in, int w, int h, int nplanes, int levels, float *out,
+                                  void *scratch)
+{
+    RG_CHECK(c, "null context");
+    return dwt_fwd_impl(c, RG_FILTER_97, in, w, h, nplanes, levels, nullptr, 0, out, 1, scratch);
+}
+
+extern "C" int rg_dwt_inv(rg_context *c, int filter, const void *dq, int w, int h, int nplanes, int levels,
+                          const float *step, void *out, void *scratch)
+{
+    RG_CHECK(c, "null context");
+    int rc = check_levels(w, h, levels);
+    if (rc) return rc;
+    RG_CHECK(filter == RG_FILTER_53 || filter == RG_FILTER_97, "bad filter %d", filter);
+    RG_CHECK(dq && out && nplanes >= 1, "null buffer");
+    const int q_float = filter == RG_FILTER_97 && step == nullptr;
+    if (levels == 0)
+        return rg_launch_copy_dequant(c->stream, filter, dq, (long)w * h, nplanes, step ? step[0] : 1.0f, q_float, out);
+    RG_CHECK(scratch || levels == 1, "scratch buffer required");
+    rg_level_dims d = rg_dims(w, h, levels);
+    size_t sizeA = (size_t)d.w[1] * d.h[1] * nplanes;
+    char *bufA = reinterpret_cast<char *>(scratch);
+    char *bufB = bufA ? bufA + ((sizeA * 4 + 255) / 256) * 256 : nullptr;
+    for (int l = levels; l >= 1; --l) {
+        rg_inv_level_args a{};
+        a.cw = d.w[l - 1]; a.ch = d.h[l - 1]; a.nplanes = nplanes;
+        if (l == levels) {
+            a.ll_is_q = 1;
+        } else {
+            a.ll_is_q = 0;
+            a.ll_in = (l + 1) % 2 == 0 ? bufA : bufB;   // output of level l+1
+            a.ll_pitch = d.w[l]; a.ll_ps = (long)d.w[l] * d.h[l];
+        }
+        a.q = dq; a.qp = w; a.q_ps = (long)w * h; a.q_float = q_float;
+        if (filter == RG_FILTER_97 && step) {
+            const int b = 1 + 3 * (levels - l);
+            a.s_ll = step[0]; a.s_hl = step[b]; a.s_lh = step[b + 1]; a.s_hh = step[b + 2];
+        }
+        if (l > 1) {
+            a.out = l % 2 == 0 ? bufA : bufB;
+            a.out_pitch = d.w[l - 1]; a.out_ps = (long)d.w[l - 1] * d.h[l - 1];
+        } else {
+            a.out = out; a.out_pitch = w; a.out_ps = (long)w * h;
+        }
+        rc = filter == RG_FILTER_53 ? rg_launch_inv53_level(c->stream, a) : rg_launch_inv97_level(c->stream, a);
+        if (rc) return rc;
+    }
+    return RG_SUCCESS;
+}
+
+// ------------------------------------------------------------ code-blocks
+extern "C" int rg_codeblocks(int w, int h, int levels, int cbw, int cbh, rg_cb_desc *out, int cap)
+{
+    if (check_levels(w, h, levels)) return RG_ERR_INVALID;
+    if (cbw < 4 || cbh < 4 || cbw > 64 || cbh > 64 || cbw * cbh > 4096 || (cbw & (cbw - 1)) || (cbh & (cbh - 1)))
+        return rg_fail(RG_ERR_INVALID, "code-block size %dx%d not supported", cbw, cbh);
+    rg_level_dims d = rg_dims(w, h, levels);
+    int n = 0;
+    auto add_band = [&](int x0, int y0, int bw, int bh, int orient, int band, int res) {
+        if (bw <= 0 || bh <= 0) return;
+        for (int j = 0; j < (bh + cbh - 1) / cbh; ++j)
+            for (int i = 0; i < (bw + cbw - 1) / cbw; ++i) {
+                if (out && n < cap) {
+                    rg_cb_desc &c = out[n];
+                    c.x0 = x0 + i * cbw; c.y0 = y0 + j * cbh;
+                    c.w = std::min(cbw, bw - i * cbw); c.h = std::min(cbh, bh - j * cbh);
+                    c.comp = 0; c.orient = orient; c.band = band; c.res = res;
+                }
+                ++n;
+            }
+    };
+    add_band(0, 0, d.w[levels], d.h[levels], 0, 0, 0);
+    int band = 1;
+    for (int l = levels; l >= 1; --l) {
+        int wl = d.w[l], hl = d.h[l], wp = d.w[l - 1], hp = d.h[l - 1], res = levels - l + 1;
+        add_band(wl, 0, wp - wl, hl, 1, band++, res);
+        add_band(0, hl, wl, hp - hl, 2, band++, res);
+        add_band(wl, hl, wp - wl, hp - hl, 3, band++, res);
+    }
+    return n;
+}
+
+extern "C" int rg_t1_encode(rg_context *c, const int32_t *dq, int pitch, size_t plane_stride, const rg_cb_desc *desc,
+                            int ncb, uint8_t *out, int slot_bytes, rg_cb_result *res)
+{
+    RG_CHECK(c && dq && desc && out && res, "rg_t1_encode: null argument");
+    RG_CHECK(ncb >= 0 && slot_bytes > 0 && pitch > 0, "rg_t1_encode: bad sizes");
+    return rg_launch_t1_encode(c->stream, dq, pitch, plane_stride, desc, ncb, out, slot_bytes, res);
+}
+
+// ------------------------------------------------------------ buffers
+namespace {
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes)
+    {
+        if (bytes <= cap && p) return RG_SUCCESS;
+        if (p) { hipFree(p); p = nullptr; cap = 0; }
+        if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            return rg_fail(RG_ERR_OOM, "hipMalloc(%zu) failed", bytes);
+        }
+        cap = bytes;
+        return RG_SUCCESS;
+    }
+    void release() { if (p) hipFree(p); p = nullptr; cap = 0; }
+};
+
+struct HostBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes)
+    {
+        if (bytes <= cap && p) return RG_SUCCESS;
+        if (p) { hipHostFree(p); p = nullptr; cap = 0; }
+        if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            return rg_fail(RG_ERR_OOM, "hipHostMalloc(%zu) failed", bytes);
+        }
+        cap = bytes;
+        return RG_SUCCESS;
+    }
+    void release() { if (p) hipHostFree(p); p = nullptr; cap = 0; }
+};
+
+// Per-band (3*levels+1) quantisation steps: Delta_b = 2^(R_b - eps_b)(1 + mu_b/2^11)
+// with (eps_b, mu_b) the T.800 E.1 exponent/mantissa closest to `delta`.
+void band_steps(int levels, int bitdepth, double delta, float *step, float *inv)
+{
+    const int nb = 3 * levels + 1;
+    for (int b = 0; b < nb; ++b) {
+        int orient = b == 0 ? 0 : 1 + (b - 1) % 3;
+        int Rb = bitdepth + (orient == 0 ? 0 : (orient == 3 ? 2 : 1));
+        int e;
+        double m = frexp(delta, &e);
+        m *= 2.0; e -= 1;
+        int mu = (int)floor((m - 1.0) * 2048.0 + 0.5);
+        if (mu >= 2048) { mu = 0; e += 1; }
+        int eps = std::max(0, std::min(31, Rb - e));
+        double s = ldexp(1.0 + mu / 2048.0, Rb - eps);
+        step[b] = (float)s;
+        inv[b] = (float)(1.0 / s);
+    }
+}
+}  // namespace
+
+// ------------------------------------------------------------ encoder
+struct rg_encoder {
+    rg_context *ctx = nullptr;
+    int w = 0, h = 0, ncomp = 0, levels = 0, cbw = 0, cbh = 0, filter = 0;
+    int lossy = 0, output_dwt = 0, ran = 0, t1_ran = 0;
+    int ncb_comp = 0, ncb = 0, slot_bytes = 0;
+    float step[97], inv[97];
+    DevBuf input, planes, q, fcoeff, scratch, desc, res, slots;
+    HostBuf staging, map;
+    std::vector<rg_cb_desc> h_desc;
+    int desc_key[6] = {0, 0, 0, 0, 0, 0};
+};
+
+extern "C" int rg_encoder_create(rg_context *c, rg_encoder **out)
+{
+    RG_CHECK(c && out, "rg_encoder_create: null argument");
+    rg_encoder *e = new rg_encoder();
+    e->ctx = c;
+    *out = e;
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_encoder_destroy(rg_encoder *e)
+{
+    if (!e) return RG_SUCCESS;
+    hipSetDevice(e->ctx->device);
+    hipStreamSynchronize(e->ctx->stream);
+    for (DevBuf *b : {&e->input, &e->planes, &e->q, &e->fcoeff, &e->scratch, &e->desc, &e->res, &e->slots}) b->release();
+    e->staging.release();
+    e->map.release();
+    delete e;
+    return RG_SUCCESS;
+}
+
+static int prepare_codeblocks(rg_encoder *e)
+{
+    int key[6] = {e->w, e->h, e->levels, e->cbw, e->cbh, e->ncomp};
+    if (memcmp(key, e->desc_key, sizeof(key)) == 0 && e->ncb > 0) return RG_SUCCESS;
+    int n = rg_codeblocks(e->w, e->h, e->levels, e->cbw, e->cbh, nullptr, 0);
+    if (n < 0) return n;
+    e->h_desc.resize((size_t)n * e->ncomp);
+    rg_codeblocks(e->w, e->h, e->levels, e->cbw, e->cbh, e->h_desc.data(), n);
+    for (int c = 1; c < e->ncomp; ++c)
+        for (int i = 0; i < n; ++i) { e->h_desc[(size_t)c * n + i] = e->h_desc[i]; e->h_desc[(size_t)c * n + i].comp = c; }
+    e->ncb_comp = n;
+    e->ncb = n * e->ncomp;
+    int rc = e->desc.ensure(sizeof(rg_cb_desc) * e->h_desc.size());
+    if (rc) return rc;
+    RG_HIP(hipMemcpyAsync(e->desc.p, e->h_desc.data(), sizeof(rg_cb_desc) * e->h_desc.size(), hipMemcpyHostToDevice,
+                          e->ctx->stream));
+    RG_HIP(hipStreamSynchronize(e->ctx->stream));
+    memcpy(e->desc_key, key, sizeof(key));
+    return RG_SUCCESS;
+}
+
+static int run_t1(rg_encoder *e)
+{
+    int rc = prepare_codeblocks(e);
+    if (rc) return rc;
+    e->slot_bytes = std::max(256, e->cbw * e->cbh * 3);
+    if ((rc = e->res.ensure(sizeof(rg_cb_result) * e->ncb))) return rc;
+    if ((rc = e->slots.ensure((size_t)e->slot_bytes * e->ncb))) return rc;
+    rc = rg_launch_t1_encode(e->ctx->stream, (const int32_t *)e->q.p, e->w, (size_t)e->w * e->h,
+                             (const rg_cb_desc *)e->desc.p, e->ncb, (uint8_t *)e->slots.p, e->slot_bytes,
+                             (rg_cb_result *)e->res.p);
+    if (rc == RG_SUCCESS) e->t1_ran = 1;
+    return rc;
+}
+
+static int check_params(const rg_encode_params *p, int w, int h, int ncomp)
+{
+    RG_CHECK(p, "null params");
+    RG_CHECK(w > 0 && h > 0 && ncomp >= 1 && ncomp <= 4, "bad image shape %dx%dx%d", w, h, ncomp);
+    RG_CHECK(p->levels >= 0 && p->levels <= 15, "levels %d out of range 0..15", p->levels);
+    RG_CHECK(p->filter == RG_FILTER_53 || p->filter == RG_FILTER_97, "bad filter %d", p->filter);
+    RG_CHECK(p->colour == RG_COLOUR_NONE || ncomp == 3, "colour transform needs 3 components");
+    RG_CHECK(p->colour != RG_COLOUR_RCT || p->filter == RG_FILTER_53, "RCT pairs with the 5/3 filter");
+    RG_CHECK(p->colour != RG_COLOUR_ICT || p->filter == RG_FILTER_97, "ICT pairs with the 9/7 filter");
+    RG_CHECK(p->filter == RG_FILTER_53 || (p->delta > 0 && std::isfinite(p->delta)), "bad quantisation step");
+    RG_CHECK(p->bitdepth >= 1 && p->bitdepth <= 16, "bad bitdepth %d", p->bitdepth);
+    if (rg_codeblocks(w, h, p->levels, p->cbw, p->cbh, nullptr, 0) < 0) return RG_ERR_INVALID;
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_encoder_encode_u8(rg_encoder *e, const uint8_t *src, int on_device, int w, int h, int ncomp,
+                                    const rg_encode_params *p)
+{
+    RG_CHECK(e && src, "rg_encoder_encode_u8: null argument");
+    int rc = check_params(p, w, h, ncomp);
+    if (rc) return rc;
+    RG_HIP(hipSetDevice(e->ctx->device));
+    const size_t npix = (size_t)w * h;
+    e->w = w; e->h = h; e->ncomp = ncomp; e->levels = p->levels; e->cbw = p->cbw; e->cbh = p->cbh;
+    e->filter = p->filter; e->lossy = p->filter == RG_FILTER_97; e->output_dwt = 0;
+    e->ran = 0; e->t1_ran = 0;
+    band_steps(p->levels, p->bitdepth, p->delta, e->step, e->inv);
+    const uint8_t *d_src = src;
+    if (!on_device) {
+        if ((rc = e->input.ensure(npix * ncomp))) return rc;
+        RG_HIP(hipMemcpyAsync(e->input.p, src, npix * ncomp, hipMemcpyHostToDevice, e->ctx->stream));
+        d_src = (const uint8_t *)e->input.p;
+    }
+    if ((rc = e->planes.ensure(npix * ncomp * 4))) return rc;
+    if ((rc = e->q.ensure(npix * ncomp * 4))) return rc;
+    if ((rc = e->scratch.ensure(rg_dwt_scratch_bytes(w, h, ncomp, p->filter)))) return rc;
+    rc = rg_launch_colour_fwd(e->ctx->stream, d_src, w, h, ncomp, p->colour, p->filter == RG_FILTER_97, e->planes.p);
+    if (rc) return rc;
+    rc = dwt_fwd_impl(e->ctx, p->filter, e->planes.p, w, h, ncomp, p->levels, e->inv, p->ll_fp16, e->q.p, 0,
+                      e->scratch.p);
+    if (rc) return rc;
+    e->ran = 1;
+    return run_t1(e);
+}
+
+extern "C" int rg_encoder_run_planar(rg_encoder *e, const void *const *comps, int ncomp, int type, int w, int h,
+                                     int levels, int precision, int lossy, int output_dwt, int cbw, int cbh,
+                                     double delta)
+{
+    RG_CHECK(e && comps, "rg_encoder_run_planar: null argument");
+    RG_CHECK(type == RG_F32 || type == RG_I16 || type == RG_I32, "bad sample type %d", type);
+    rg_encode_params p{};
+    p.levels = levels; p.filter = lossy ? RG_FILTER_97 : RG_FILTER_53; p.colour = RG_COLOUR_NONE;
+    p.cbw = cbw; p.cbh = cbh; p.delta = delta; p.ll_fp16 = 0; p.bitdepth = precision > 0 ? precision : 8;
+    int rc = check_params(&p, w, h, ncomp);
+    if (rc) return rc;
+    for (int c = 0; c < ncomp; ++c) RG_CHECK(comps[c], "component %d is null", c);
+    RG_HIP(hipSetDevice(e->ctx->device));
+    const size_t npix = (size_t)w * h;
+    e->w = w; e->h = h; e->ncomp = ncomp; e->levels = levels; e->cbw = cbw; e->cbh = cbh;
+    e->filter = p.filter; e->lossy = lossy; e->output_dwt = lossy && output_dwt; e->ran = 0; e->t1_ran = 0;
+    band_steps(levels, p.bitdepth, delta, e->step, e->inv);
+    // host conversion of the T components into the kernels' sample type
+    if ((rc = e->staging.ensure(npix * ncomp * 4))) return rc;
+    RG_HIP(hipStreamSynchronize(e->ctx->stream));  // staging may still feed a previous copy
+    for (int c = 0; c < ncomp; ++c) {
+        if (lossy) {
+            float *dst = (float *)e->staging.p + c * npix;
+            if (type == RG_F32) memcpy(dst, comps[c], npix * 4);
+            else if (type == RG_I16) for (size_t i = 0; i < npix; ++i) dst[i] = ((const int16_t *)comps[c])[i];
+            else for (size_t i = 0; i < npix; ++i) dst[i] = (float)((const int32_t *)comps[c])[i];
+        } else {
+            int32_t *dst = (int32_t *)e->staging.p + c * npix;
+            if (type == RG_F32) for (size_t i = 0; i < npix; ++i) dst[i] = (int32_t)lrintf(((const float *)comps[c])[i]);
+            else if (type == RG_I16) for (size_t i = 0; i < npix; ++i) dst[i] = ((const int16_t *)comps[c])[i];
+            else memcpy(dst, comps[c], npix * 4);
+        }
+    }
+    if ((rc = e->planes.ensure(npix * ncomp * 4))) return rc;
+    if ((rc = e->q.ensure(npix * ncomp * 4))) return rc;
+    if ((rc = e->scratch.ensure(rg_dwt_scratch_bytes(w, h, ncomp, p.filter)))) return rc;
+    RG_HIP(hipMemcpyAsync(e->planes.p, e->staging.p, npix * ncomp * 4, hipMemcpyHostToDevice, e->ctx->stream));
+    if (e->output_dwt) {
+        if ((rc = e->fcoeff.ensure(npix * ncomp * 4))) return rc;
+        rc = dwt_fwd_impl(e->ctx, RG_FILTER_97, e->planes.p, w, h, ncomp, levels, nullptr, 0, e->fcoeff.p, 1,
+                          e->scratch.p);
+        if (rc) return rc;
+        e->ran = 1;
+        return RG_SUCCESS;   // outputDwt: the reference skips the bit-plane coder (src/OCLEncoder.cpp:46-50)
+    }
+    rc = dwt_fwd_impl(e->ctx, p.filter, e->planes.p, w, h, ncomp, levels, e->inv, 0, e->q.p, 0, e->scratch.p);
+    if (rc) return rc;
+    e->ran = 1;
+    return run_t1(e);
+}
+
+extern "C" int rg_encoder_coefficients(rg_encoder *e, int32_t **dq, int *ncomp, int *w, int *h)
+{
+    RG_CHECK(e, "null encoder");
+    if (!e->ran) return rg_fail(RG_ERR_NOT_READY, "encoder has not run");
+    if (dq) *dq = (int32_t *)e->q.p;
+    if (ncomp) *ncomp = e->ncomp;
+    if (w) *w = e->w;
+    if (h) *h = e->h;
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_encoder_float_coefficients(rg_encoder *e, float **d)
+{
+    RG_CHECK(e && d, "null argument");
+    if (!e->ran || !e->output_dwt) return rg_fail(RG_ERR_NOT_READY, "no float coefficients (output_dwt not set)");
+    *d = (float *)e->fcoeff.p;
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_encoder_num_codeblocks(rg_encoder *e)
+{
+    RG_CHECK(e, "null encoder");
+    return e->t1_ran ? e->ncb : 0;
+}
+
+extern "C" int rg_encoder_get_codeblocks(rg_encoder *e, rg_cb_desc *desc, rg_cb_result *res, long *offsets,
+                                         uint8_t *data, long cap, long *total)
+{
+    RG_CHECK(e, "null encoder");
+    if (!e->t1_ran) return rg_fail(RG_ERR_NOT_READY, "tier-1 has not run");
+    std::vector<rg_cb_result> r(e->ncb);
+    RG_HIP(hipMemcpyAsync(r.data(), e->res.p, sizeof(rg_cb_result) * e->ncb, hipMemcpyDeviceToHost, e->ctx->stream));
+    RG_HIP(hipStreamSynchronize(e->ctx->stream));
+    long sum = 0;
+    for (int i = 0; i < e->ncb; ++i) {
+        if (r[i].flags & 1)
+            return rg_fail(RG_ERR_OVERFLOW, "code-block %d needs more than %d bytes", i, e->slot_bytes);
+        sum += r[i].len;
+    }
+    if (total) *total = sum;
+    if (desc) memcpy(desc, e->h_desc.data(), sizeof(rg_cb_desc) * e->ncb);
+    if (res) memcpy(res, r.data(), sizeof(rg_cb_result) * e->ncb);
+    if (data) {
+        RG_CHECK(cap >= sum, "data buffer too small (%ld < %ld)", cap, sum);
+        std::vector<uint8_t> all((size_t)e->slot_bytes * e->ncb);
+        RG_HIP(hipMemcpyAsync(all.data(), e->slots.p, all.size(), hipMemcpyDeviceToHost, e->ctx->stream));
+        RG_HIP(hipStreamSynchronize(e->ctx->stream));
+        long o = 0;
+        for (int i = 0; i < e->ncb; ++i) {
+            if (offsets) offsets[i] = o;
+            memcpy(data + o, all.data() + (size_t)i * e->slot_bytes, r[i].len);
+            o += r[i].len;
+        }
+    } else if (offsets) {
+        long o = 0;
+        for (int i = 0; i < e->ncb; ++i) { offsets[i] = o; o += r[i].len; }
+    }
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_encoder_map_dwt_out(rg_encoder *e, void **host)
+{
+    RG_CHECK(e && host, "null argument");
+    if (!e->ran) return rg_fail(RG_ERR_NOT_READY, "encoder has not run");
+    const size_t npix = (size_t)e->w * e->h, nc = e->ncomp;
+    std::vector<uint8_t> tmp(npix * nc * 4);
+    const void *src = e->output_dwt ? e->fcoeff.p : e->q.p;
+    RG_HIP(hipMemcpyAsync(tmp.data(), src, tmp.size(), hipMemcpyDeviceToHost, e->ctx->stream));
+    RG_HIP(hipStreamSynchronize(e->ctx->stream));
+    int rc;
+    if (e->output_dwt) {
+        if ((rc = e->map.ensure(npix * nc * 4))) return rc;
+        const float *f = (const float *)tmp.data();
+        float *o = (float *)e->map.p;
+        for (size_t i = 0; i < npix; ++i)
+            for (size_t c = 0; c < nc; ++c) o[i * nc + c] = f[c * npix + i];
+    } else {
+        if ((rc = e->map.ensure(npix * nc * 2))) return rc;
+        const int32_t *q = (const int32_t *)tmp.data();
+        int16_t *o = (int16_t *)e->map.p;
+        for (size_t i = 0; i < npix; ++i)
+            for (size_t c = 0; c < nc; ++c) {
+                int32_t v = q[c * npix + i];
+                o[i * nc + c] = (int16_t)std::max(-32768, std::min(32767, v));
+            }
+    }
+    *host = e->map.p;
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_encoder_unmap_dwt_out(rg_encoder *e, void *host)
+{
+    RG_CHECK(e, "null encoder");
+    RG_CHECK(host == e->map.p, "pointer was not mapped by this encoder");
+    return RG_SUCCESS;
+}
+
+// ------------------------------------------------------------ decoder
+struct rg_decoder {
+    rg_context *ctx = nullptr;
+    int w = 0, h = 0, ncomp = 0, lossy = 0, ran = 0;
+    DevBuf coeff, out, scratch;
+    HostBuf staging, map;
+};
+
+extern "C" int rg_decoder_create(rg_context *c, rg_decoder **out)
+{
+    RG_CHECK(c && out, "null argument");
+    rg_decoder *d = new rg_decoder();
+    d->ctx = c;
+    *out = d;
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_decoder_destroy(rg_decoder *d)
+{
+    if (!d) return RG_SUCCESS;
+    hipSetDevice(d->ctx->device);
+    hipStreamSynchronize(d->ctx->stream);
+    d->coeff.release(); d->out.release(); d->scratch.release();
+    d->staging.release(); d->map.release();
+    delete d;
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_decoder_run_planar(rg_decoder *d, const void *const *comps, int ncomp, int type, int w, int h,
+                                     int levels, int precision, int lossy)
+{
+    (void)precision;
+    RG_CHECK(d && comps, "null argument");
+    RG_CHECK(type == RG_F32 || type == RG_I16 || type == RG_I32, "bad sample type %d", type);
+    RG_CHECK(w > 0 && h > 0 && ncomp >= 1 && ncomp <= 4, "bad image shape");
+    RG_CHECK(levels >= 0 && levels <= 15, "levels out of range");
+    for (int c = 0; c < ncomp; ++c) RG_CHECK(comps[c], "component %d is null", c);
+    RG_HIP(hipSetDevice(d->ctx->device));
+    const size_t npix = (size_t)w * h;
+    int rc;
+    if ((rc = d->staging.ensure(npix * ncomp * 4))) return rc;
+    RG_HIP(hipStreamSynchronize(d->ctx->stream));
+    for (int c = 0; c < ncomp; ++c) {
+        if (lossy) {
+            float *dst = (float *)d->staging.p + c * npix;
+            if (type == RG_F32) memcpy(dst, comps[c], npix * 4);
+            else if (type == RG_I16) for (size_t i = 0; i < npix; ++i) dst[i] = ((const int16_t *)comps[c])[i];
+            else for (size_t i = 0; i < npix; ++i) dst[i] = (float)((const int32_t *)comps[c])[i];
+        } else {
+            int32_t *dst = (int32_t *)d->staging.p + c * npix;
+            if (type == RG_F32) for (size_t i = 0; i < npix; ++i) dst[i] = (int32_t)lrintf(((const float *)comps[c])[i]);
+            else if (type == RG_I16) for (size_t i = 0; i < npix; ++i) dst[i] = ((const int16_t *)comps[c])[i];
+            else memcpy(dst, comps[c], npix * 4);
+        }
+    }
+    if ((rc = d->coeff.ensure(npix * ncomp * 4))) return rc;
+    if ((rc = d->out.ensure(npix * ncomp * 4))) return rc;
+    if ((rc = d->scratch.ensure(rg_dwt_scratch_bytes(w, h, ncomp, lossy)))) return rc;
+    RG_HIP(hipMemcpyAsync(d->coeff.p, d->staging.p, npix * ncomp * 4, hipMemcpyHostToDevice, d->ctx->stream));
+    rc = rg_dwt_inv(d->ctx, lossy ? RG_FILTER_97 : RG_FILTER_53, d->coeff.p, w, h, ncomp, levels, nullptr, d->out.p,
+                    d->scratch.p);
+    if (rc) return rc;
+    d->w = w; d->h = h; d->ncomp = ncomp; d->lossy = lossy; d->ran = 1;
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_decoder_map_out(rg_decoder *d, void **host)
+{
+    RG_CHECK(d && host, "null argument");
+    if (!d->ran) return rg_fail(RG_ERR_NOT_READY, "decoder has not run");
+    const size_t npix = (size_t)d->w * d->h, nc = d->ncomp;
+    std::vector<uint8_t> tmp(npix * nc * 4);
+    RG_HIP(hipMemcpyAsync(tmp.data(), d->out.p, tmp.size(), hipMemcpyDeviceToHost, d->ctx->stream));
+    RG_HIP(hipStreamSynchronize(d->ctx->stream));
+    int rc;
+    if (d->lossy) {
+        if ((rc = d->map.ensure(npix * nc * 4))) return rc;
+        const float *f = (const float *)tmp.data();
+        float *o = (float *)d->map.p;
+        for (size_t i = 0; i < npix; ++i)
+            for (size_t c = 0; c < nc; ++c) o[i * nc + c] = f[c * npix + i];
+    } else {
+        if ((rc = d->map.ensure(npix * nc * 2))) return rc;
+        const int32_t *q = (const int32_t *)tmp.data();
+        int16_t *o = (int16_t *)d->map.p;
+        for (size_t i = 0; i < npix; ++i)
+            for (size_t c = 0; c < nc; ++c)
+                o[i * nc + c] = (int16_t)std::max(-32768, std::min(32767, q[c * npix + i]));
+    }
+    *host = d->map.p;
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_decoder_unmap_out(rg_decoder *d, void *host)
+{
+    RG_CHECK(d, "null decoder");
+    RG_CHECK(host == d->map.p, "pointer was not mapped by this decoder");
+    return RG_SUCCESS;
+}

@@ -1,0 +1,254 @@
+// Colour stage: interleaved uint8 <-> level-shifted planes with RCT / ICT.
+//
+// Replaces the host-side level shift and interleave of the reference
+// (src/OCLTest.cpp:81-90, src/OCLMemoryManager.cpp:41-54) and its post-DWT
+// de-interleave kernel (src/CL/oclplanar.cl:23-39).  T.800 G.2 (RCT) and
+// G.3 (ICT); FP32 operation order identical to the CPU oracle (fmaf chains).
+//
+// Streaming design: each thread owns 16 consecutive pixels, so a wavefront
+// reads one contiguous 1 KiB (gray) / 3 KiB (RGB) span with 16-byte loads and
+// writes 4 KiB per plane with 16-byte stores; no LDS.
+#include "rg_internal.h"
+
+namespace {
+
+constexpr int PX = 16;  // pixels per thread
+
+__device__ __forceinline__ float ict_y(float r, float g, float b)
+{
+    return __builtin_fmaf((float)0.114, b, __builtin_fmaf((float)0.587, g, (float)0.299 * r));
+}
+__device__ __forceinline__ float ict_cb(float r, float g, float b)
+{
+    return __builtin_fmaf((float)0.5, b, __builtin_fmaf((float)-0.33126, g, (float)-0.16875 * r));
+}
+__device__ __forceinline__ float ict_cr(float r, float g, float b)
+{
+    return __builtin_fmaf((float)-0.08131, b, __builtin_fmaf((float)-0.41869, g, (float)0.5 * r));
+}
+
+__device__ __forceinline__ uint8_t clamp_u8(int x) { return (uint8_t)(x < 0 ? 0 : (x > 255 ? 255 : x)); }
+__device__ __forceinline__ uint8_t out_u8f(float x)
+{
+    float s = x + 128.0f;
+    return clamp_u8((int)floorf(s + 0.5f));
+}
+
+// MODE: 0 shift only, 1 RCT, 2 ICT.  FLT: float planes (else int32).
+template <int NC, int MODE, bool FLT>
+__global__ void __launch_bounds__(256) colour_fwd_kernel(const uint8_t *__restrict__ src, long npix,
+                                                          void *__restrict__ planes)
+{
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long p0 = g * PX;
+    if (p0 >= npix) return;
+    int n = (int)min((long)PX, npix - p0);
+    uint8_t b[PX * NC];
+    if (n == PX) {
+        const uint4 *s4 = reinterpret_cast<const uint4 *>(src + p0 * NC);
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            uint4 v = s4[k];
+            *reinterpret_cast<uint4 *>(&b[16 * k]) = v;
+        }
+    } else {
+        for (int i = 0; i < n * NC; ++i) b[i] = src[p0 * NC + i];
+    }
+    if (FLT) {
+        float *out = reinterpret_cast<float *>(planes);
+        float o[NC][PX];
+#pragma unroll
+        for (int i = 0; i < PX; ++i) {
+            if (MODE == 2) {
+                float r = (float)((int)b[3 * i] - 128), gg = (float)((int)b[3 * i + 1] - 128),
+                      bb = (float)((int)b[3 * i + 2] - 128);
+                o[0][i] = ict_y(r, gg, bb);
+                o[1][i] = ict_cb(r, gg, bb);
+                o[2][i] = ict_cr(r, gg, bb);
+            } else {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) o[c][i] = (float)((int)b[NC * i + c] - 128);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            float *dst = out + (long)c * npix + p0;
+            if (n == PX) {
+#pragma unroll
+                for (int k = 0; k < PX / 4; ++k)
+                    reinterpret_cast<float4 *>(dst)[k] = make_float4(o[c][4 * k], o[c][4 * k + 1], o[c][4 * k + 2], o[c][4 * k + 3]);
+            } else {
+                for (int i = 0; i < n; ++i) dst[i] = o[c][i];
+            }
+        }
+    } else {
+        int32_t *out = reinterpret_cast<int32_t *>(planes);
+        int o[NC][PX];
+#pragma unroll
+        for (int i = 0; i < PX; ++i) {
+            if (MODE == 1) {
+                int r = (int)b[3 * i] - 128, gg = (int)b[3 * i + 1] - 128, bb = (int)b[3 * i + 2] - 128;
+                o[0][i] = (r + 2 * gg + bb) >> 2;
+                o[1][i] = bb - gg;
+                o[2][i] = r - gg;
+            } else {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) o[c][i] = (int)b[NC * i + c] - 128;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            int32_t *dst = out + (long)c * npix + p0;
+            if (n == PX) {
+#pragma unroll
+                for (int k = 0; k < PX / 4; ++k)
+                    reinterpret_cast<int4 *>(dst)[k] = make_int4(o[c][4 * k], o[c][4 * k + 1], o[c][4 * k + 2], o[c][4 * k + 3]);
+            } else {
+                for (int i = 0; i < n; ++i) dst[i] = o[c][i];
+            }
+        }
+    }
+}
+
+template <int NC, int MODE, bool FLT>
+__global__ void __launch_bounds__(256) colour_inv_kernel(const void *__restrict__ planes, long npix,
+                                                          uint8_t *__restrict__ dst)
+{
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long p0 = g * PX;
+    if (p0 >= npix) return;
+    int n = (int)min((long)PX, npix - p0);
+    uint8_t b[PX * NC];
+#pragma unroll
+    for (int i = 0; i < PX; ++i) {
+        if (i >= n) break;
+        if (FLT) {
+            const float *in = reinterpret_cast<const float *>(planes);
+            if (MODE == 2) {
+                float y = in[p0 + i], cb = in[npix + p0 + i], cr = in[2 * npix + p0 + i];
+                float r = __builtin_fmaf((float)1.402, cr, y);
+                float gg = __builtin_fmaf((float)-0.71414, cr, __builtin_fmaf((float)-0.34413, cb, y));
+                float bb = __builtin_fmaf((float)1.772, cb, y);
+                b[3 * i] = out_u8f(r); b[3 * i + 1] = out_u8f(gg); b[3 * i + 2] = out_u8f(bb);
+            } else {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) b[NC * i + c] = out_u8f(in[(long)c * npix + p0 + i]);
+            }
+        } else {
+            const int32_t *in = reinterpret_cast<const int32_t *>(planes);
+            if (MODE == 1) {
+                int y = in[p0 + i], u = in[npix + p0 + i], v = in[2 * npix + p0 + i];
+                int gg = y - ((u + v) >> 2);
+                b[3 * i] = clamp_u8(v + gg + 128); b[3 * i + 1] = clamp_u8(gg + 128); b[3 * i + 2] = clamp_u8(u + gg + 128);
+            } else {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) b[NC * i + c] = clamp_u8(in[(long)c * npix + p0 + i] + 128);
+            }
+        }
+    }
+    if (n == PX) {
+        uint4 *d4 = reinterpret_cast<uint4 *>(dst + p0 * NC);
+#pragma unroll
+        for (int k = 0; k < NC; ++k) d4[k] = *reinterpret_cast<const uint4 *>(&b[16 * k]);
+    } else {
+        for (int i = 0; i < n * NC; ++i) dst[p0 * NC + i] = b[i];
+    }
+}
+
+// Generic per-pixel path for other channel counts (shift only).
+template <bool FLT>
+__global__ void colour_fwd_generic(const uint8_t *__restrict__ src, long npix, int nc, void *__restrict__ planes)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix) return;
+    for (int c = 0; c < nc; ++c) {
+        int v = (int)src[i * nc + c] - 128;
+        if (FLT) reinterpret_cast<float *>(planes)[(long)c * npix + i] = (float)v;
+        else reinterpret_cast<int32_t *>(planes)[(long)c * npix + i] = v;
+    }
+}
+
+template <bool FLT>
+__global__ void colour_inv_generic(const void *__restrict__ planes, long npix, int nc, uint8_t *__restrict__ dst)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix) return;
+    for (int c = 0; c < nc; ++c) {
+        if (FLT) dst[i * nc + c] = out_u8f(reinterpret_cast<const float *>(planes)[(long)c * npix + i]);
+        else dst[i * nc + c] = clamp_u8(reinterpret_cast<const int32_t *>(planes)[(long)c * npix + i] + 128);
+    }
+}
+
+__global__ void planar_split_i16_kernel(const int16_t *__restrict__ src, long npix, int nc, int16_t *__restrict__ dst)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix) return;
+    for (int c = 0; c < nc; ++c) dst[(long)c * npix + i] = src[i * nc + c];
+}
+
+}  // namespace
+
+int rg_launch_colour_fwd(hipStream_t s, const uint8_t *src, int w, int h, int ncomp, int colour,
+                         int out_float, void *planes)
+{
+    long npix = (long)w * h;
+    if (npix == 0) return RG_SUCCESS;
+    long nthr = (npix + PX - 1) / PX;
+    dim3 grid((unsigned)((nthr + 255) / 256)), block(256);
+    if (ncomp == 3 && colour == RG_COLOUR_RCT)
+        colour_fwd_kernel<3, 1, false><<<grid, block, 0, s>>>(src, npix, planes);
+    else if (ncomp == 3 && colour == RG_COLOUR_ICT)
+        colour_fwd_kernel<3, 2, true><<<grid, block, 0, s>>>(src, npix, planes);
+    else if (colour != RG_COLOUR_NONE)
+        return rg_fail(RG_ERR_INVALID, "colour transform %d needs 3 components (got %d)", colour, ncomp);
+    else if (ncomp == 1)
+        out_float ? colour_fwd_kernel<1, 0, true><<<grid, block, 0, s>>>(src, npix, planes)
+                  : colour_fwd_kernel<1, 0, false><<<grid, block, 0, s>>>(src, npix, planes);
+    else if (ncomp == 3)
+        out_float ? colour_fwd_kernel<3, 0, true><<<grid, block, 0, s>>>(src, npix, planes)
+                  : colour_fwd_kernel<3, 0, false><<<grid, block, 0, s>>>(src, npix, planes);
+    else {
+        dim3 g2((unsigned)((npix + 255) / 256));
+        out_float ? colour_fwd_generic<true><<<g2, block, 0, s>>>(src, npix, ncomp, planes)
+                  : colour_fwd_generic<false><<<g2, block, 0, s>>>(src, npix, ncomp, planes);
+    }
+    RG_LAUNCH_CHECK();
+    return RG_SUCCESS;
+}
+
+int rg_launch_colour_inv(hipStream_t s, const void *planes, int w, int h, int ncomp, int colour,
+                         int in_float, uint8_t *dst)
+{
+    long npix = (long)w * h;
+    if (npix == 0) return RG_SUCCESS;
+    long nthr = (npix + PX - 1) / PX;
+    dim3 grid((unsigned)((nthr + 255) / 256)), block(256);
+    if (ncomp == 3 && colour == RG_COLOUR_RCT)
+        colour_inv_kernel<3, 1, false><<<grid, block, 0, s>>>(planes, npix, dst);
+    else if (ncomp == 3 && colour == RG_COLOUR_ICT)
+        colour_inv_kernel<3, 2, true><<<grid, block, 0, s>>>(planes, npix, dst);
+    else if (colour != RG_COLOUR_NONE)
+        return rg_fail(RG_ERR_INVALID, "colour transform %d needs 3 components (got %d)", colour, ncomp);
+    else if (ncomp == 1)
+        in_float ? colour_inv_kernel<1, 0, true><<<grid, block, 0, s>>>(planes, npix, dst)
+                 : colour_inv_kernel<1, 0, false><<<grid, block, 0, s>>>(planes, npix, dst);
+    else if (ncomp == 3)
+        in_float ? colour_inv_kernel<3, 0, true><<<grid, block, 0, s>>>(planes, npix, dst)
+                 : colour_inv_kernel<3, 0, false><<<grid, block, 0, s>>>(planes, npix, dst);
+    else {
+        dim3 g2((unsigned)((npix + 255) / 256));
+        in_float ? colour_inv_generic<true><<<g2, block, 0, s>>>(planes, npix, ncomp, dst)
+                 : colour_inv_generic<false><<<g2, block, 0, s>>>(planes, npix, ncomp, dst);
+    }
+    RG_LAUNCH_CHECK();
+    return RG_SUCCESS;
+}
+
+int rg_launch_planar_split_i16(hipStream_t s, const int16_t *src, int w, int h, int ncomp, int16_t *dst)
+{
+    long npix = (long)w * h;
+    if (npix == 0) return RG_SUCCESS;
+    planar_split_i16_kernel<<<dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, s>>>(src, npix, ncomp, dst);
+    RG_LAUNCH_CHECK();
+    return RG_SUCCESS;
+}

@@ -1,0 +1,101 @@
+// Internal declarations shared by the HIP sources of libroger_amd.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include "../../include/roger_amd.h"
+
+struct rg_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+};
+
+// Thread-local last-error message; set by rg_fail().
+int rg_fail(int code, const char *fmt, ...);
+
+#define RG_HIP(call)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            return rg_fail(RG_ERR_DEVICE, "%s failed: %s (%s:%d)", #call,            \
+                           hipGetErrorString(e_), __FILE__, __LINE__);               \
+    } while (0)
+
+#define RG_CHECK(cond, ...)                                                          \
+    do {                                                                             \
+        if (!(cond)) return rg_fail(RG_ERR_INVALID, __VA_ARGS__);                    \
+    } while (0)
+
+#define RG_LAUNCH_CHECK() RG_HIP(hipGetLastError())
+
+// Whole-sample symmetric extension index for a signal of length n >= 1.
+__host__ __device__ inline int rg_mirror(int i, int n)
+{
+    if (n == 1) return 0;
+    int p = 2 * (n - 1);
+    i %= p;
+    if (i < 0) i += p;
+    return i >= n ? p - i : i;
+}
+
+// Band geometry of the Mallat layout (origin 0): level l region sizes.
+struct rg_level_dims {
+    int w[33], h[33];
+};
+inline rg_level_dims rg_dims(int w, int h, int levels)
+{
+    rg_level_dims d;
+    d.w[0] = w; d.h[0] = h;
+    for (int l = 1; l <= levels && l < 33; ++l) {
+        d.w[l] = (d.w[l - 1] + 1) / 2;
+        d.h[l] = (d.h[l - 1] + 1) / 2;
+    }
+    return d;
+}
+
+// Kernel launchers (defined in the .hip files).
+int rg_launch_colour_fwd(hipStream_t s, const uint8_t *src, int w, int h, int ncomp, int colour,
+                         int out_float, void *planes);
+int rg_launch_colour_inv(hipStream_t s, const void *planes, int w, int h, int ncomp, int colour,
+                         int in_float, uint8_t *dst);
+int rg_launch_planar_split_i16(hipStream_t s, const int16_t *src, int w, int h, int ncomp, int16_t *dst);
+
+// One forward level: input region cw x ch (pitch in_pitch, plane stride in_ps),
+// LL to ll_out (pitch ll_pitch, plane stride ll_ps), high bands into the
+// Mallat plane q (pitch qp, plane stride q_ps) at offsets (wl, hl).
+struct rg_fwd_level_args {
+    const void *in;
+    long in_pitch, in_ps;
+    int in_half;            // input LL stored as fp16 (9/7)
+    void *ll_out;           // next-level LL (float / half / int32); nullptr: LL goes to q
+    long ll_pitch, ll_ps;
+    int ll_half;
+    void *q;                // int32 indices (or float coefficients when q_float)
+    long qp, q_ps;
+    int q_float;            // write float coefficients (no quantisation)
+    int cw, ch, nplanes;
+    float inv_ll, inv_hl, inv_lh, inv_hh;  // quantisation (9/7 when !q_float)
+};
+int rg_launch_fwd53_level(hipStream_t s, const rg_fwd_level_args &a);
+int rg_launch_fwd97_level(hipStream_t s, const rg_fwd_level_args &a);
+
+// One inverse level: reads the four bands of a cw x ch region (LL from ll_in,
+// pitch ll_pitch; high bands from the Mallat plane q), writes the region to out.
+struct rg_inv_level_args {
+    const void *ll_in;
+    long ll_pitch, ll_ps;
+    int ll_is_q;            // LL read from q (coarsest level) with dequantisation
+    const void *q;
+    long qp, q_ps;
+    int q_float;            // q holds float coefficients
+    void *out;
+    long out_pitch, out_ps;
+    int cw, ch, nplanes;
+    float s_ll, s_hl, s_lh, s_hh; // dequantisation steps (9/7 with int q)
+};
+int rg_launch_inv53_level(hipStream_t s, const rg_inv_level_args &a);
+int rg_launch_inv97_level(hipStream_t s, const rg_inv_level_args &a);
+
+int rg_launch_t1_encode(hipStream_t s, const int32_t *q, int pitch, size_t plane_stride,
+                        const rg_cb_desc *desc, int ncb, uint8_t *out, int slot_bytes,
+                        rg_cb_result *res);

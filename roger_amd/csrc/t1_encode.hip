@@ -1,0 +1,409 @@
+// EBCOT tier-1 encoder (T.800 Annex D) with the MQ coder (Annex C).
+//
+// Replaces the reference's bit-plane coder skeleton src/CL/oclbpc.cl:279-746
+// (launched per channel by src/OCLBPC.cpp:38-51), which stops after the
+// per-block MSB and has no context formation, MQ coder or output.
+//
+// Design (gfx950, wave64): one 64-lane workgroup per code-block (<= 64x64).
+// Lane j owns column j; every per-sample state (significance, visited,
+// refined, sign, current magnitude bit-plane) is a 64-bit row mask in
+// registers, so the 8-neighbourhood of a 4-row stripe is a 6-bit window of
+// the lane's own masks plus the windows of lanes j-1 / j+1 (one cross-lane
+// shuffle each).  Per stripe and pass:
+//   * significance propagation: a sample joins the pass if any neighbour is
+//     significant at its turn in scan order.  The only cross-lane dependency
+//     is "left column became significant earlier in this stripe"; it is
+//     resolved by a wave-wide fix-point (recompute with the left lane's
+//     newly-significant set until no lane changes, checked with __any), which
+//     converges in (longest left-to-right chain + 1) rounds, usually 1-2.
+//   * magnitude refinement: no intra-pass dependency.
+//   * cleanup: newly significant samples are known up front (bit == 1), so
+//     contexts and the run-length test come straight from the windows.
+// Each lane emits its column's (context, bit) symbols in scan order; a wave
+// prefix sum of the per-lane counts gives contiguous offsets into an LDS
+// symbol buffer, which the MQ coder then consumes in order as wave-uniform
+// (scalar) code.  The codeword is written to the block's output slot.
+#include "rg_internal.h"
+
+namespace {
+
+// MQ probability table (T.800 Table C.2): Qe | NMPS << 16 | NLPS << 22 | SWITCH << 28.
+#define MQE(q, nm, nl, sw) ((uint32_t)(q) | ((uint32_t)(nm) << 16) | ((uint32_t)(nl) << 22) | ((uint32_t)(sw) << 28))
+__constant__ uint32_t MQT[47] = {
+    MQE(0x5601, 1, 1, 1),   MQE(0x3401, 2, 6, 0),   MQE(0x1801, 3, 9, 0),   MQE(0x0AC1, 4, 12, 0),
+    MQE(0x0521, 5, 29, 0),  MQE(0x0221, 38, 33, 0), MQE(0x5601, 7, 6, 1),   MQE(0x5401, 8, 14, 0),
+    MQE(0x4801, 9, 14, 0),  MQE(0x3801, 10, 14, 0), MQE(0x3001, 11, 17, 0), MQE(0x2401, 12, 18, 0),
+    MQE(0x1C01, 13, 20, 0), MQE(0x1601, 29, 21, 0), MQE(0x5601, 15, 14, 1), MQE(0x5401, 16, 14, 0),
+    MQE(0x5101, 17, 15, 0), MQE(0x4801, 18, 16, 0), MQE(0x3801, 19, 17, 0), MQE(0x3401, 20, 18, 0),
+    MQE(0x3001, 21, 19, 0), MQE(0x2801, 22, 19, 0), MQE(0x2401, 23, 20, 0), MQE(0x2201, 24, 21, 0),
+    MQE(0x1C01, 25, 22, 0), MQE(0x1801, 26, 23, 0), MQE(0x1601, 27, 24, 0), MQE(0x1401, 28, 25, 0),
+    MQE(0x1201, 29, 26, 0), MQE(0x1101, 30, 27, 0), MQE(0x0AC1, 31, 28, 0), MQE(0x09C1, 32, 29, 0),
+    MQE(0x08A1, 33, 30, 0), MQE(0x0521, 34, 31, 0), MQE(0x0441, 35, 32, 0), MQE(0x02A1, 36, 33, 0),
+    MQE(0x0221, 37, 34, 0), MQE(0x0141, 38, 35, 0), MQE(0x0111, 39, 36, 0), MQE(0x0085, 40, 37, 0),
+    MQE(0x0049, 41, 38, 0), MQE(0x0025, 42, 39, 0), MQE(0x0015, 43, 40, 0), MQE(0x0009, 44, 41, 0),
+    MQE(0x0005, 45, 42, 0), MQE(0x0001, 45, 43, 0), MQE(0x5601, 46, 46, 0)};
+
+constexpr int CTX_RL = 17, CTX_UNI = 18, NCTX = 19;
+constexpr int SYMBUF = 64 * 10 + 64;
+
+// ZC context (Table D.1), same rule as the oracle.
+__device__ __forceinline__ int zc_ctx(int orient, int h, int v, int d)
+{
+    if (orient == 1) { int t = h; h = v; v = t; }
+    if (orient == 3) {
+        int hv = h + v;
+        if (d >= 3) return 8;
+        if (d == 2) return hv >= 1 ? 7 : 6;
+        if (d == 1) return hv >= 2 ? 5 : (hv == 1 ? 4 : 3);
+        return hv >= 2 ? 2 : hv;
+    }
+    if (h == 2) return 8;
+    if (h == 1) return v >= 1 ? 7 : (d >= 1 ? 6 : 5);
+    if (v == 2) return 4;
+    if (v == 1) return 3;
+    return d >= 2 ? 2 : d;
+}
+
+// Sign context (Table D.3): returns ctx << 1 | xorbit.
+__device__ __forceinline__ int sc_ctx(int H, int V)
+{
+    int x = 0;
+    if (H < 0 || (H == 0 && V < 0)) { x = 1; H = -H; V = -V; }
+    int c = (H == 1) ? (V == 1 ? 13 : (V == 0 ? 12 : 11)) : (V == 1 ? 10 : 9);
+    return (c << 1) | x;
+}
+
+__device__ __forceinline__ uint32_t win6(uint64_t m, int r0)
+{
+    // bits r0-1 .. r0+4 of m (bit 0 of the window = row r0-1)
+    return r0 == 0 ? (uint32_t)(m << 1) & 0x3Fu : (uint32_t)(m >> (r0 - 1)) & 0x3Fu;
+}
+
+__device__ __forceinline__ uint32_t b1(uint32_t w, int i) { return (w >> i) & 1u; }
+
+struct SymList {
+    uint64_t lo = 0, hi = 0;
+    int n = 0;
+    __device__ __forceinline__ void push(int cx, int bit)
+    {
+        uint64_t s = (uint64_t)((cx << 1) | bit);
+        if (n < 8) lo |= s << (8 * n); else hi |= s << (8 * (n - 8));
+        ++n;
+    }
+    __device__ __forceinline__ int get(int i) const
+    {
+        return (int)(((i < 8) ? (lo >> (8 * i)) : (hi >> (8 * (i - 8)))) & 0xFF);
+    }
+};
+
+// Sign symbol for window row i with neighbour windows: own column (cur state
+// above, pre state below), left (Lw), right (rw); sign windows negw/neglw/negrw.
+__device__ __forceinline__ void push_sign(SymList &sl, int i, uint32_t cur, uint32_t sw, uint32_t Lw, uint32_t rw,
+                                          uint32_t negw, uint32_t neglw, uint32_t negrw)
+{
+    int H = (b1(Lw, i) ? (b1(neglw, i) ? -1 : 1) : 0) + (b1(rw, i) ? (b1(negrw, i) ? -1 : 1) : 0);
+    int V = (b1(cur, i - 1) ? (b1(negw, i - 1) ? -1 : 1) : 0) + (b1(sw, i + 1) ? (b1(negw, i + 1) ? -1 : 1) : 0);
+    H = H > 1 ? 1 : (H < -1 ? -1 : H);
+    V = V > 1 ? 1 : (V < -1 ? -1 : V);
+    int sc = sc_ctx(H, V);
+    sl.push(sc >> 1, (int)b1(negw, i) ^ (sc & 1));
+}
+
+__device__ __forceinline__ int zc_at(int orient, int i, uint32_t cur, uint32_t sw, uint32_t Lw, uint32_t rw)
+{
+    int h = (int)(b1(Lw, i) + b1(rw, i));
+    int v = (int)(b1(cur, i - 1) + b1(sw, i + 1));
+    int d = (int)(b1(Lw, i - 1) + b1(Lw, i + 1) + b1(rw, i - 1) + b1(rw, i + 1));
+    return zc_ctx(orient, h, v, d);
+}
+
+__device__ __forceinline__ int wave_incl_scan(int v, int lane)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+struct MQEnc {
+    uint32_t A, C, B;
+    int CT, bp;       // bp: index of the pending byte B (-1: the fake byte before the codeword)
+    int cap, overflow;
+    uint8_t *out;
+};
+
+__device__ __forceinline__ void mq_store(MQEnc &m, int lane)
+{
+    if (m.bp >= 0) {
+        if (m.bp < m.cap) { if (lane == 0) m.out[m.bp] = (uint8_t)m.B; }
+        else m.overflow = 1;
+    }
+}
+
+__device__ __forceinline__ void mq_byteout(MQEnc &m, int lane)
+{
+    if (m.B == 0xFF) {
+        mq_store(m, lane); m.bp++; m.B = m.C >> 20; m.C &= 0xFFFFF; m.CT = 7;
+    } else if (m.C < 0x8000000u) {
+        mq_store(m, lane); m.bp++; m.B = m.C >> 19; m.C &= 0x7FFFF; m.CT = 8;
+    } else {
+        m.B++;
+        if (m.B == 0xFF) {
+            m.C &= 0x7FFFFFF;
+            mq_store(m, lane); m.bp++; m.B = m.C >> 20; m.C &= 0xFFFFF; m.CT = 7;
+        } else {
+            mq_store(m, lane); m.bp++; m.B = m.C >> 19; m.C &= 0x7FFFF; m.CT = 8;
+        }
+    }
+}
+
+__device__ __forceinline__ void mq_renorm(MQEnc &m, int lane)
+{
+    // shift until bit 15 of A is set, emitting a byte whenever CT runs out
+    int n = __clz(m.A) - 16;
+    while (n > 0) {
+        int s = n < m.CT ? n : m.CT;
+        m.A <<= s; m.C <<= s; m.CT -= s; n -= s;
+        if (m.CT == 0) mq_byteout(m, lane);
+    }
+}
+
+// Encodes the `total` symbols of sbuf in order; ctx holds I << 1 | MPS per context.
+__device__ __forceinline__ void mq_run(MQEnc &m, const uint8_t *sbuf, uint8_t *ctx, int total, int lane)
+{
+    for (int i = 0; i < total; ++i) {
+        int sym = __builtin_amdgcn_readfirstlane((int)sbuf[i]);
+        int cx = sym >> 1, d = sym & 1;
+        int st = __builtin_amdgcn_readfirstlane((int)ctx[cx]);
+        int idx = st >> 1, mps = st & 1;
+        uint32_t t = MQT[idx];
+        uint32_t qe = t & 0xFFFF;
+        m.A -= qe;
+        if (d == mps) {
+            if (m.A & 0x8000) {
+                m.C += qe;
+            } else {
+                if (m.A < qe) m.A = qe; else m.C += qe;
+                if (lane == 0) ctx[cx] = (uint8_t)((((t >> 16) & 63) << 1) | mps);
+                mq_renorm(m, lane);
+            }
+        } else {
+            if (m.A < qe) m.C += qe; else m.A = qe;
+            mps ^= (int)(t >> 28);
+            if (lane == 0) ctx[cx] = (uint8_t)((((t >> 22) & 63) << 1) | mps);
+            mq_renorm(m, lane);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+__global__ void __launch_bounds__(64) t1_encode_kernel(const int32_t *__restrict__ q, int pitch, size_t plane_stride,
+                                                       const rg_cb_desc *__restrict__ desc, uint8_t *__restrict__ out,
+                                                       int slot_bytes, rg_cb_result *__restrict__ res)
+{
+    __shared__ uint8_t sbuf[SYMBUF];
+    __shared__ uint8_t ctx[32];
+
+    const int lane = threadIdx.x;
+    const rg_cb_desc d = desc[blockIdx.x];
+    const int w = d.w, h = d.h, orient = d.orient;
+    const int32_t *src = q + (size_t)d.comp * plane_stride + (size_t)d.y0 * pitch + d.x0;
+
+    // ---- load column `lane`: magnitudes in registers, sign as a row mask ----
+    uint32_t mag[64];
+    uint64_t NEG = 0;
+    uint32_t orm = 0;
+    const bool colok = lane < w;
+#pragma unroll
+    for (int r = 0; r < 64; ++r) {
+        int v = (colok && r < h) ? src[(size_t)r * pitch + lane] : 0;
+        uint32_t m = (uint32_t)(v < 0 ? -v : v);
+        mag[r] = m;
+        NEG |= (uint64_t)(v < 0) << r;
+        orm |= m;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) orm |= __shfl_xor(orm, o, 64);
+    const int nbp = orm ? 32 - __clz(orm) : 0;
+    if (nbp == 0) {
+        if (lane == 0) res[blockIdx.x] = rg_cb_result{0, 0, 0, 0};
+        return;
+    }
+
+    if (lane < NCTX) ctx[lane] = (uint8_t)(lane == 0 ? (4 << 1) : lane == CTX_RL ? (3 << 1) : lane == CTX_UNI ? (46 << 1) : 0);
+    __syncthreads();
+
+    const uint64_t rowmask = h >= 64 ? ~0ull : ((1ull << h) - 1);
+    const uint64_t NEGL = lane == 0 ? 0 : [&] { uint64_t t = __shfl_up(NEG, 1, 64); return t; }();
+    const uint64_t NEGR = __shfl_down(NEG, 1, 64);
+    const uint64_t negl = lane == 0 ? 0 : NEGL;
+    const uint64_t negr = lane == 63 ? 0 : NEGR;
+
+    MQEnc m;
+    m.A = 0x8000; m.C = 0; m.B = 0; m.CT = 12; m.bp = -1;
+    m.cap = slot_bytes; m.overflow = 0;
+    m.out = out + (size_t)blockIdx.x * slot_bytes;
+
+    uint64_t S = 0, V = 0, R = 0;
+    const int nstripes = (h + 3) >> 2;
+
+    for (int p = nbp - 1; p >= 0; --p) {
+        uint32_t lo = 0, hi = 0;
+#pragma unroll
+        for (int r = 0; r < 32; ++r) lo |= ((mag[r] >> p) & 1u) << r;
+#pragma unroll
+        for (int r = 0; r < 32; ++r) hi |= ((mag[r + 32] >> p) & 1u) << r;
+        const uint64_t B = ((uint64_t)hi << 32) | lo;
+
+        for (int pass = (p == nbp - 1) ? 2 : 0; pass < 3; ++pass) {
+            for (int s = 0; s < nstripes; ++s) {
+                const int r0 = 4 * s;
+                const uint32_t vrow = colok ? (uint32_t)(rowmask >> r0) & 0xFu : 0u;
+                const uint32_t sw = win6(S, r0);
+                uint32_t lw = __shfl_up(sw, 1, 64), rw = __shfl_down(sw, 1, 64);
+                if (lane == 0) lw = 0;
+                if (lane == 63) rw = 0;
+                const uint32_t bw = win6(B, r0), negw = win6(NEG, r0);
+                const uint32_t neglw = win6(negl, r0), negrw = win6(negr, r0);
+                SymList sl;
+
+                if (pass == 0) {
+                    // ---- significance propagation ----
+                    uint32_t N = 0, P = 0, Lw = lw;
+                    bool again = true;
+                    // monotone fix-point: converges within 65 rounds (chain across <= 64 lanes)
+                    for (int round = 0; again && round < 80; ++round) {
+                        uint32_t NL = __shfl_up(N, 1, 64);
+                        if (lane == 0) NL = 0;
+                        Lw = lw | (NL << 1);
+                        uint32_t cur = sw, Nn = 0;
+                        P = 0;
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            const int i = k + 1;
+                            if (!((vrow >> k) & 1u) || b1(cur, i)) continue;
+                            uint32_t nb = b1(cur, i - 1) | b1(sw, i + 1) | ((Lw >> (i - 1)) & 7u) | ((rw >> (i - 1)) & 7u);
+                            if (nb) {
+                                P |= 1u << k;
+                                if (b1(bw, i)) { Nn |= 1u << k; cur |= 1u << i; }
+                            }
+                        }
+                        again = __any(Nn != N);
+                        N = Nn;
+                    }
+                    uint32_t cur = sw;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const int i = k + 1;
+                        if (!((P >> k) & 1u)) continue;
+                        const int bit = (int)b1(bw, i);
+                        sl.push(zc_at(orient, i, cur, sw, Lw, rw), bit);
+                        if (bit) {
+                            push_sign(sl, i, cur, sw, Lw, rw, negw, neglw, negrw);
+                            cur |= 1u << i;
+                        }
+                    }
+                    S |= (uint64_t)N << r0;
+                    V |= (uint64_t)P << r0;
+                } else if (pass == 1) {
+                    // ---- magnitude refinement ----
+                    const uint32_t vw = (uint32_t)(V >> r0) & 0xFu;
+                    const uint32_t rf = (uint32_t)(R >> r0) & 0xFu;
+                    const uint32_t M = (sw >> 1) & ~vw & vrow;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const int i = k + 1;
+                        if (!((M >> k) & 1u)) continue;
+                        int cx;
+                        if ((rf >> k) & 1u) cx = 16;
+                        else {
+                            uint32_t nb = b1(sw, i - 1) | b1(sw, i + 1) | ((lw >> (i - 1)) & 7u) | ((rw >> (i - 1)) & 7u);
+                            cx = nb ? 15 : 14;
+                        }
+                        sl.push(cx, (int)b1(bw, i));
+                    }
+                    R |= (uint64_t)M << r0;
+                } else {
+                    // ---- cleanup ----
+                    const uint32_t vw = (uint32_t)(V >> r0) & 0xFu;
+                    const uint32_t Cm = ~(sw >> 1) & ~vw & vrow & 0xFu;
+                    const uint32_t N = Cm & (bw >> 1);
+                    uint32_t NL = __shfl_up(N, 1, 64);
+                    if (lane == 0) NL = 0;
+                    const uint32_t Lw = lw | (NL << 1);
+                    const bool full = r0 + 4 <= h;
+                    const bool rl = full && Cm == 0xFu && (Lw & 0x3Fu) == 0 && (rw & 0x3Fu) == 0 && (sw & 0x21u) == 0;
+                    uint32_t cur = sw;
+                    int kstart = 0;
+                    bool done = false;
+                    if (rl) {
+                        if (N == 0) {
+                            sl.push(CTX_RL, 0);
+                            done = true;
+                        } else {
+                            const int k = __builtin_ctz(N);
+                            sl.push(CTX_RL, 1);
+                            sl.push(CTX_UNI, k >> 1);
+                            sl.push(CTX_UNI, k & 1);
+                            push_sign(sl, k + 1, cur, sw, Lw, rw, negw, neglw, negrw);
+                            cur |= 1u << (k + 1);
+                            kstart = k + 1;
+                        }
+                    }
+                    if (!done) {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            const int i = k + 1;
+                            if (k < kstart || !((Cm >> k) & 1u)) continue;
+                            const int bit = (int)((N >> k) & 1u);
+                            sl.push(zc_at(orient, i, cur, sw, Lw, rw), bit);
+                            if (bit) {
+                                push_sign(sl, i, cur, sw, Lw, rw, negw, neglw, negrw);
+                                cur |= 1u << i;
+                            }
+                        }
+                    }
+                    S |= (uint64_t)N << r0;
+                }
+
+                // ---- compact this stripe's symbols (scan order) and MQ-code them ----
+                const int incl = wave_incl_scan(sl.n, lane);
+                const int total = __shfl(incl, 63, 64);
+                if (total == 0) continue;
+                const int off = incl - sl.n;
+                for (int t = 0; t < sl.n; ++t) sbuf[off + t] = (uint8_t)sl.get(t);
+                __syncthreads();
+                mq_run(m, sbuf, ctx, total, lane);
+                __syncthreads();
+            }
+            if (pass == 2) V = 0;
+        }
+    }
+
+    // ---- FLUSH (C.2.9) ----
+    {
+        uint32_t tmp = m.C + m.A;
+        m.C |= 0xFFFF;
+        if (m.C >= tmp) m.C -= 0x8000;
+        m.C <<= m.CT; mq_byteout(m, lane);
+        m.C <<= m.CT; mq_byteout(m, lane);
+        int len;
+        if (m.B != 0xFF) { mq_store(m, lane); len = m.bp + 1; }
+        else len = m.bp;
+        if (len > slot_bytes) m.overflow = 1;
+        if (lane == 0) res[blockIdx.x] = rg_cb_result{len, 3 * nbp - 2, nbp, m.overflow ? 1 : 0};
+    }
+}
+
+}  // namespace
+
+int rg_launch_t1_encode(hipStream_t s, const int32_t *q, int pitch, size_t plane_stride,
+                        const rg_cb_desc *desc, int ncb, uint8_t *out, int slot_bytes, rg_cb_result *res)
+{
+    if (ncb <= 0) return RG_SUCCESS;
+    t1_encode_kernel<<<dim3((unsigned)ncb), dim3(64), 0, s>>>(q, pitch, plane_stride, desc, out, slot_bytes, res);
+    RG_LAUNCH_CHECK();
+    return RG_SUCCESS;
+}

@@ -1,0 +1,250 @@
+"""Python host side over the C ABI: device context, device buffers and the
+stage entry points, taking numpy arrays on the host and DeviceBuffer objects
+on the device.  No torch: device memory comes from rg_malloc."""
+import ctypes
+
+import numpy as np
+
+from ._lib import (CbDesc, CbResult, EncodeParams, RG_COLOUR_ICT, RG_COLOUR_NONE, RG_COLOUR_RCT, RG_FILTER_53,
+                   RG_FILTER_97, check, lib)
+
+
+def _vp(x):
+    return ctypes.c_void_p(x)
+
+
+class Context:
+    """A device + HIP stream (the reference's ocl_args_d_t)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        check(lib().rg_context_create(device, ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def sync(self):
+        check(lib().rg_context_sync(self.handle))
+
+    def device_name(self):
+        buf = ctypes.create_string_buffer(256)
+        check(lib().rg_context_device_name(self.handle, buf, 256))
+        return buf.value.decode()
+
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        b = DeviceBuffer(self, arr.nbytes)
+        b.write(arr)
+        return b
+
+    def timer_start(self):
+        check(lib().rg_timer_start(self.handle))
+
+    def timer_stop(self):
+        ms = ctypes.c_float()
+        check(lib().rg_timer_stop(self.handle, ctypes.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self.handle:
+            lib().rg_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    """Device memory owned by Python (rg_malloc / rg_free)."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        check(lib().rg_malloc(ctx.handle, self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    def write(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes, "host array larger than the device buffer"
+        check(lib().rg_memcpy_h2d(self.ctx.handle, _vp(self.ptr), arr.ctypes.data_as(ctypes.c_void_p), arr.nbytes))
+
+    def read(self, dtype, shape):
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes, "read larger than the device buffer"
+        check(lib().rg_memcpy_d2h(self.ctx.handle, out.ctypes.data_as(ctypes.c_void_p), _vp(self.ptr), out.nbytes))
+        return out
+
+    def zero(self):
+        check(lib().rg_memset(self.ctx.handle, _vp(self.ptr), 0, self.nbytes))
+
+    def free(self):
+        if self.ptr:
+            lib().rg_free(self.ctx.handle, _vp(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- band steps
+def band_steps(levels, delta, bitdepth=8):
+    """Per-band quantisation steps (T.800 E.1 exponent/mantissa closest to delta):
+    returns (step float32[3L+1], inv_step float32[3L+1], eps int[], mu int[])."""
+    nb = 3 * levels + 1
+    step = np.empty(nb, np.float32)
+    inv = np.empty(nb, np.float32)
+    eps = np.empty(nb, np.int32)
+    mu = np.empty(nb, np.int32)
+    for b in range(nb):
+        orient = 0 if b == 0 else 1 + (b - 1) % 3
+        Rb = bitdepth + (0 if orient == 0 else (2 if orient == 3 else 1))
+        m, e = np.frexp(delta)
+        m, e = m * 2.0, int(e) - 1
+        u = int(np.floor((m - 1.0) * 2048.0 + 0.5))
+        if u >= 2048:
+            u, e = 0, e + 1
+        ep = max(0, min(31, Rb - e))
+        s = np.ldexp(1.0 + u / 2048.0, Rb - ep)
+        step[b], inv[b], eps[b], mu[b] = np.float32(s), np.float32(1.0 / s), ep, u
+    return step, inv, eps, mu
+
+
+# ---------------------------------------------------------------- stages
+def colour_fwd(ctx, d_src, w, h, ncomp, colour, out_float, d_planes):
+    check(lib().rg_colour_fwd(ctx.handle, _vp(d_src.ptr), w, h, ncomp, colour, int(out_float), _vp(d_planes.ptr)))
+
+
+def colour_inv(ctx, d_planes, w, h, ncomp, colour, in_float, d_dst):
+    check(lib().rg_colour_inv(ctx.handle, _vp(d_planes.ptr), w, h, ncomp, colour, int(in_float), _vp(d_dst.ptr)))
+
+
+def planar_split_i16(ctx, d_src, w, h, ncomp, d_planes):
+    check(lib().rg_planar_split_i16(ctx.handle, _vp(d_src.ptr), w, h, ncomp, _vp(d_planes.ptr)))
+
+
+def scratch_bytes(w, h, nplanes, filt):
+    return lib().rg_dwt_scratch_bytes(w, h, nplanes, filt)
+
+
+def dwt_fwd(ctx, filt, d_in, w, h, nplanes, levels, inv_step, ll_fp16, d_q, d_scratch):
+    inv = None if inv_step is None else np.ascontiguousarray(inv_step, np.float32)
+    check(lib().rg_dwt_fwd(ctx.handle, filt, _vp(d_in.ptr), w, h, nplanes, levels,
+                           None if inv is None else inv.ctypes.data_as(ctypes.c_void_p), int(ll_fp16),
+                           _vp(d_q.ptr), _vp(d_scratch.ptr) if d_scratch else None))
+
+
+def dwt97_fwd_float(ctx, d_in, w, h, nplanes, levels, d_out, d_scratch):
+    check(lib().rg_dwt97_fwd_float(ctx.handle, _vp(d_in.ptr), w, h, nplanes, levels, _vp(d_out.ptr),
+                                   _vp(d_scratch.ptr) if d_scratch else None))
+
+
+def dwt_inv(ctx, filt, d_q, w, h, nplanes, levels, step, d_out, d_scratch):
+    st = None if step is None else np.ascontiguousarray(step, np.float32)
+    check(lib().rg_dwt_inv(ctx.handle, filt, _vp(d_q.ptr), w, h, nplanes, levels,
+                           None if st is None else st.ctypes.data_as(ctypes.c_void_p), _vp(d_out.ptr),
+                           _vp(d_scratch.ptr) if d_scratch else None))
+
+
+def codeblocks(w, h, levels, cbw, cbh, ncomp=1):
+    n = check(lib().rg_codeblocks(w, h, levels, cbw, cbh, None, 0))
+    arr = (CbDesc * max(n, 1))()
+    check(lib().rg_codeblocks(w, h, levels, cbw, cbh, ctypes.cast(arr, ctypes.c_void_p), n))
+    one = np.ctypeslib.as_array(arr)[:n].copy()
+    out = np.concatenate([one] * ncomp) if ncomp > 1 else one
+    if ncomp > 1:
+        out["comp"] = np.repeat(np.arange(ncomp, dtype=np.int32), n)
+    return out
+
+
+CB_DESC_DTYPE = np.dtype([("x0", "<i4"), ("y0", "<i4"), ("w", "<i4"), ("h", "<i4"), ("comp", "<i4"),
+                          ("orient", "<i4"), ("band", "<i4"), ("res", "<i4")])
+CB_RES_DTYPE = np.dtype([("len", "<i4"), ("npasses", "<i4"), ("nbp", "<i4"), ("flags", "<i4")])
+
+
+def t1_encode(ctx, d_q, pitch, plane_stride, descs, slot_bytes):
+    """Tier-1 encodes the blocks `descs` (structured array); returns (results, list of bytes)."""
+    descs = np.ascontiguousarray(descs).view(CB_DESC_DTYPE)
+    n = len(descs)
+    d_desc = ctx.upload(descs)
+    d_out = ctx.alloc(n * slot_bytes)
+    d_res = ctx.alloc(n * 16)
+    check(lib().rg_t1_encode(ctx.handle, _vp(d_q.ptr), pitch, plane_stride, _vp(d_desc.ptr), n, _vp(d_out.ptr),
+                             slot_bytes, _vp(d_res.ptr)))
+    res = d_res.read(CB_RES_DTYPE, (n,))
+    raw = d_out.read(np.uint8, (n, slot_bytes))
+    if (res["flags"] != 0).any():
+        raise RuntimeError("tier-1 output overflow")
+    data = [bytes(raw[i, :res["len"][i]]) for i in range(n)]
+    return res, data
+
+
+class Encoder:
+    """The native encode path (rg_encoder): colour -> DWT -> quantisation -> tier-1."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        check(lib().rg_encoder_create(ctx.handle, ctypes.byref(h)))
+        self.handle = h
+
+    def encode_u8(self, img, levels=5, reversible=True, cbw=64, cbh=64, delta=1.0 / 64, ll_fp16=False,
+                  d_src=None, w=None, h=None, ncomp=None):
+        if d_src is None:
+            img = np.ascontiguousarray(img, np.uint8)
+            h, w = img.shape[:2]
+            ncomp = 1 if img.ndim == 2 else img.shape[2]
+            src, on_dev = img.ctypes.data_as(ctypes.c_void_p), 0
+        else:
+            src, on_dev = _vp(d_src.ptr), 1
+        p = EncodeParams()
+        p.levels = levels
+        p.filter = RG_FILTER_53 if reversible else RG_FILTER_97
+        p.colour = RG_COLOUR_NONE if ncomp != 3 else (RG_COLOUR_RCT if reversible else RG_COLOUR_ICT)
+        p.cbw, p.cbh, p.delta, p.ll_fp16, p.bitdepth = cbw, cbh, float(delta), int(ll_fp16), 8
+        self.shape = (w, h, ncomp)
+        check(lib().rg_encoder_encode_u8(self.handle, src, on_dev, w, h, ncomp, ctypes.byref(p)))
+
+    def coefficients(self):
+        dq = ctypes.c_void_p()
+        nc, w, h = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().rg_encoder_coefficients(self.handle, ctypes.byref(dq), ctypes.byref(nc), ctypes.byref(w),
+                                            ctypes.byref(h)))
+        out = np.empty((nc.value, h.value, w.value), np.int32)
+        self.ctx.sync()
+        check(lib().rg_memcpy_d2h(self.ctx.handle, out.ctypes.data_as(ctypes.c_void_p), dq, out.nbytes))
+        return out
+
+    def codeblocks(self):
+        n = check(lib().rg_encoder_num_codeblocks(self.handle))
+        desc = np.empty(n, CB_DESC_DTYPE)
+        res = np.empty(n, CB_RES_DTYPE)
+        offs = np.empty(n, np.int64)
+        total = ctypes.c_long()
+        check(lib().rg_encoder_get_codeblocks(self.handle, desc.ctypes.data_as(ctypes.c_void_p),
+                                              res.ctypes.data_as(ctypes.c_void_p), None, None, 0,
+                                              ctypes.byref(total)))
+        data = np.empty(max(total.value, 1), np.uint8)
+        check(lib().rg_encoder_get_codeblocks(self.handle, None, None, offs.ctypes.data_as(ctypes.c_void_p),
+                                              data.ctypes.data_as(ctypes.c_void_p), data.size, None))
+        blobs = [bytes(data[offs[i]:offs[i] + res["len"][i]]) for i in range(n)]
+        return desc, res, blobs
+
+    def close(self):
+        if self.handle:
+            lib().rg_encoder_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

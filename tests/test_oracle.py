@@ -1,0 +1,218 @@
+"""CPU tests that pin the oracle (no GPU).
+
+Each check is independent of the oracle's own formulas: literal transcriptions
+of T.800's context tables, published 9/7 filter taps (convolution form instead
+of lifting), exhaustive RCT inversion, closed forms, and an independent
+JPEG 2000 decoder (OpenJPEG through Pillow) decoding the oracle's codestreams.
+"""
+import io
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from roger_amd import synth
+
+# ---------------------------------------------------------------- context tables
+# T.800 Table D.1 rows for LL/LH: (sum_h, sum_v, sum_d) -> context; 'x' = any.
+_D1_LL = [((2, None, None), 8), ((1, ">=1", None), 7), ((1, 0, ">=1"), 6), ((1, 0, 0), 5),
+          ((0, 2, None), 4), ((0, 1, None), 3), ((0, 0, ">=2"), 2), ((0, 0, 1), 1), ((0, 0, 0), 0)]
+# Table D.1 HH rows: (sum_hv, sum_d) -> context
+_D1_HH = [((None, ">=3"), 8), ((">=1", 2), 7), ((0, 2), 6), ((">=2", 1), 5), ((1, 1), 4), ((0, 1), 3),
+          ((">=2", 0), 2), ((1, 0), 1), ((0, 0), 0)]
+
+
+def _match(spec, v):
+    if spec is None:
+        return True
+    if isinstance(spec, str):
+        return v >= int(spec[2:])
+    return v == spec
+
+
+def _d1(orient, h, v, d):
+    if orient == 3:
+        for (shv, sd), c in _D1_HH:
+            if _match(shv, h + v) and _match(sd, d):
+                return c
+    if orient == 1:
+        h, v = v, h
+    for (sh, sv, sd), c in _D1_LL:
+        if _match(sh, h) and _match(sv, v) and _match(sd, d):
+            return c
+    raise AssertionError("no row")
+
+
+def test_zc_table_d1():
+    for orient in range(4):
+        for h in range(3):
+            for v in range(3):
+                for d in range(5):
+                    assert O.zc_context(orient, h, v, d) == _d1(orient, h, v, d), (orient, h, v, d)
+
+
+# Table D.3: (H, V) -> (context, XORbit)
+_D3 = {(1, 1): (13, 0), (1, 0): (12, 0), (1, -1): (11, 0), (0, 1): (10, 0), (0, 0): (9, 0),
+       (0, -1): (10, 1), (-1, 1): (11, 1), (-1, 0): (12, 1), (-1, -1): (13, 1)}
+
+
+def test_sc_table_d3():
+    for (H, V), want in _D3.items():
+        assert O.sc_context(H, V) == want
+
+
+# ---------------------------------------------------------------- colour
+def test_rct_exhaustive_inverse():
+    v = np.arange(1 << 24, dtype=np.uint32)
+    rgb = np.stack([(v >> 16) & 255, (v >> 8) & 255, v & 255], axis=-1).astype(np.uint8)
+    planes = O.rct_fwd(rgb)
+    np.testing.assert_array_equal(O.rct_inv(planes), rgb)
+    # closed form on a few triples (G.2)
+    r, g, b = rgb[123456].astype(int) - 128
+    assert planes[0][123456] == (r + 2 * g + b) // 4 and planes[1][123456] == b - g and planes[2][123456] == r - g
+
+
+def test_ict_against_matrix():
+    rng = np.random.default_rng(1)
+    rgb = rng.integers(0, 256, size=(4096, 3), dtype=np.uint8)
+    M = np.array([[0.299, 0.587, 0.114], [-0.16875, -0.33126, 0.5], [0.5, -0.41869, -0.08131]])
+    ref = (rgb.astype(np.float64) - 128) @ M.T
+    got = O.ict_fwd(rgb).T
+    assert np.abs(got - ref).max() < 1e-4
+    back = O.ict_inv_f32(O.ict_fwd(rgb))
+    assert np.abs(back.astype(int) - rgb.astype(int)).max() <= 1
+
+
+# ---------------------------------------------------------------- DWT
+def _mirror(i, n):
+    if n == 1:
+        return 0
+    p = 2 * (n - 1)
+    i %= p
+    return p - i if i >= n else i
+
+
+# CDF 9/7 analysis taps (T.800 Table F.4 filter form; low band DC gain 1,
+# high band Nyquist gain 2), centred.
+H0 = {0: 0.6029490182363579, 1: 0.2668641184428723, 2: -0.07822326652898785, 3: -0.01686411844287495,
+      4: 0.02674875741080976}
+H1 = {0: 1.115087052456994, 1: -0.5912717631142470, 2: -0.05754352622849957, 3: 0.09127176311424948}
+
+
+@pytest.mark.parametrize("n", list(range(2, 34)) + [64, 101])
+def test_dwt97_lifting_equals_convolution(n):
+    rng = np.random.default_rng(n)
+    x = rng.normal(0, 50, n)
+    got = O.dwt97_fwd_1d_f64(x)
+    nl = (n + 1) // 2
+    lo = [sum(c * (x[_mirror(2 * k + t, n)] + (x[_mirror(2 * k - t, n)] if t else 0)) for t, c in H0.items())
+          for k in range(nl)]
+    hi = [sum(c * (x[_mirror(2 * k + 1 + t, n)] + (x[_mirror(2 * k + 1 - t, n)] if t else 0)) for t, c in H1.items())
+          for k in range(n // 2)]
+    np.testing.assert_allclose(got[:nl], lo, atol=1e-9)
+    np.testing.assert_allclose(got[nl:], hi, atol=1e-9)
+
+
+@pytest.mark.parametrize("n", range(1, 34))
+def test_dwt53_1d_closed_forms(n):
+    c = np.full(n, 37, np.int32)
+    out = O.dwt53_fwd_1d(c)
+    nl = (n + 1) // 2
+    assert (out[:nl] == 37).all() and (out[nl:] == 0).all()       # DC: low = c, high = 0
+    ramp = (np.arange(n) * 3).astype(np.int32)
+    out = O.dwt53_fwd_1d(ramp)
+    # the predict step is exact on a linear ramp away from the right border
+    assert (out[nl:nl + max(0, n // 2 - 1)] == 0).all()
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (2, 1), (1, 5), (7, 3), (33, 17), (64, 64), (100, 37)])
+def test_dwt_perfect_reconstruction(w, h):
+    rng = np.random.default_rng(w * h)
+    p = rng.integers(-300, 300, size=(h, w), dtype=np.int32)
+    for levels in (0, 1, 2, 5):
+        np.testing.assert_array_equal(O.dwt53_inv(O.dwt53_fwd(p, levels), levels), p)
+        f = p.astype(np.float64)
+        assert np.abs(O.dwt97_inv(O.dwt97_fwd(f, levels, np.float64), levels, np.float64) - f).max() < 1e-9
+        f32 = p.astype(np.float32)
+        assert np.abs(O.dwt97_inv(O.dwt97_fwd(f32, levels), levels) - f32).max() < 2e-3
+
+
+def test_fp16_rounding():
+    for v in [0.0, 1.0, 1.0009765625, 65504.0, 65519.0, 6.0e-5, 3e-8, -2.5, 1 / 3]:
+        assert O.round_f16(v) == float(np.float32(np.float16(np.float32(v))))
+
+
+# ---------------------------------------------------------------- quantiser
+def test_quantiser_closed_form():
+    step = O.step_from_expmant(8, 14, 0)
+    assert step == 1.0 / 64
+    assert O.expmant_for_step(10, 1.0 / 64) == (16, 0)
+    e, m = O.expmant_for_step(8, 0.709869)
+    assert abs(O.step_from_expmant(8, e, m) - 0.709869) < 0.709869 / 2048
+    y = np.array([[0.0, -0.0, 0.015, -0.016, 1.0, -1.0, 7.99, -123.4]], np.float32)
+    q = O.quantize(y, 0, np.array([64.0], np.float32))
+    np.testing.assert_array_equal(q, [[0, 0, 0, -1, 64, -64, 511, -7897]])
+    r = O.dequantize(q, 0, np.array([1 / 64], np.float32))
+    np.testing.assert_allclose(r, [[0, 0, 0, -1.5 / 64, 64.5 / 64, -64.5 / 64, 511.5 / 64, -7897.5 / 64]])
+
+
+# ---------------------------------------------------------------- tier-1
+@pytest.mark.parametrize("w,h", [(1, 1), (4, 4), (64, 64), (32, 32), (37, 13), (64, 3), (5, 64)])
+def test_t1_roundtrip(w, h):
+    rng = np.random.default_rng(w + 100 * h)
+    for orient in range(4):
+        for scale in (0, 1, 3, 200, 40000):
+            b = (rng.laplace(0, scale, size=(h, w))).astype(np.int32) if scale else np.zeros((h, w), np.int32)
+            data, npass, nbp = O.t1_encode(b, orient)
+            if not b.any():
+                assert (npass, nbp, data) == (0, 0, b"")
+                continue
+            assert nbp == int(np.abs(b).max()).bit_length() and npass == 3 * nbp - 2
+            np.testing.assert_array_equal(O.t1_decode(data, w, h, orient, nbp, npass), b)
+
+
+def test_codeblock_enumeration():
+    cbs = O.codeblocks(512, 512, 5, 64, 64)
+    assert len(cbs) == 70  # 16^2 LL + 3*(16^2 + 32^2 + 64^2 + 128^2 + 256^2) px in 64x64 blocks
+    area = sum(c["w"] * c["h"] for c in cbs)
+    assert area == 512 * 512
+    cbs = O.codeblocks(97, 61, 3, 32, 32)
+    assert sum(c["w"] * c["h"] for c in cbs) == 97 * 61
+
+
+# ---------------------------------------------------------------- end to end, independent decoder
+def _decode(cs):
+    from PIL import Image
+    return np.array(Image.open(io.BytesIO(cs)))
+
+
+@pytest.mark.parametrize("w,h,nc,levels,cb", [(512, 512, 1, 5, 64), (97, 61, 1, 3, 64), (256, 200, 3, 5, 64),
+                                              (33, 70, 3, 2, 32), (1, 1, 1, 0, 64), (8, 3, 3, 4, 64)])
+def test_openjpeg_decodes_lossless(w, h, nc, levels, cb):
+    img = synth.image(w, h, nc, 3)
+    cs = O.encode_j2k(img, levels=levels, reversible=True, cbw=cb, cbh=cb)[0]
+    np.testing.assert_array_equal(_decode(cs), img)
+
+
+@pytest.mark.parametrize("w,h,nc,levels,delta", [(512, 512, 1, 5, 1.0), (300, 257, 3, 6, 2.0), (129, 65, 1, 3, 8.0),
+                                                 (128, 128, 3, 6, 1.0 / 64)])
+def test_openjpeg_decodes_lossy_like_oracle(w, h, nc, levels, delta):
+    img = synth.image(w, h, nc, 3)
+    cs, rs, data, q = O.encode_j2k(img, levels=levels, reversible=False, delta=delta)
+    dec = _decode(cs)
+    # own reconstruction: tier-1 decode -> dequantise -> inverse 9/7 -> inverse ICT
+    steps = np.array([s[2] for s in O.band_steps(w, h, levels, delta)], np.float32)
+    cbs = O.codeblocks(w, h, levels, 64, 64)
+    planes = []
+    for c in range(nc):
+        qq = np.zeros((h, w), np.int32)
+        for i, d in enumerate(cbs):
+            k = c * len(cbs) + i
+            qq[d["y0"]:d["y0"] + d["h"], d["x0"]:d["x0"] + d["w"]] = O.t1_decode(
+                data[k], d["w"], d["h"], d["orient"], rs[k][2], rs[k][1])
+        np.testing.assert_array_equal(qq, q[c])
+        planes.append(O.dwt97_inv(O.dequantize(qq, levels, steps), levels))
+    mine = O.ict_inv_f32(np.stack(planes)) if nc == 3 else O.unshift(planes[0])
+    assert np.abs(mine.astype(int) - dec.astype(int)).max() <= 1
+    mse = ((dec.astype(float) - img) ** 2).mean()
+    assert 10 * np.log10(255 ** 2 / max(mse, 1e-12)) > 30
