@@ -29,6 +29,55 @@ METRIC = "Encode Mpix/s (4096² RGB, DWT+EBCOT); DWT %HBM peak; batched vs plain
 HBM_PEAK_GBS = 8000.0
 
 
+def bench_batched(ctx, args):
+    """Mixed-size batch (sides log-uniform 256..4096, half gray / half RGB,
+    half 5/3 / half 9/7, 5 levels): batched persistent + graph path vs the plain
+    per-image path, both on device-resident inputs."""
+    from roger_amd import autotune
+    from roger_amd import runtime as R
+    from roger_amd import synth
+
+    specs = synth.batch_specs(args.batch_images)
+    log = lambda s: print(s, file=sys.stderr, flush=True)
+    log("batch: generating %d images on the device" % len(specs))
+    images = []
+    for s in specs:
+        buf = synth.device_image(ctx, s["w"], s["h"], s["ncomp"], s["seed"])
+        images.append(dict(s, buf=buf))
+    mpix = sum(m["w"] * m["h"] for m in images) / 1e6
+    batch = R.Batch(ctx)
+    tuned = autotune.tune(ctx, batch, images, log=log)
+    cb = tuned["cb"]
+    steps = max(2, min(5, args.steps))
+    batch.run()
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(steps):
+        batch.run()
+    ctx.sync()
+    t_batch = (time.perf_counter() - t) / steps
+    ncb = batch.num_codeblocks(-1)
+    # plain per-image path: one resident encoder per image, same code-block size
+    encs = [R.Encoder(ctx) for _ in images]
+    run_plain = lambda: [e.encode_u8(None, levels=m["levels"], reversible=m["reversible"], cbw=cb, cbh=cb,
+                                     delta=m["delta"], d_src=m["buf"], w=m["w"], h=m["h"], ncomp=m["ncomp"])
+                         for e, m in zip(encs, images)]
+    run_plain()
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(steps):
+        run_plain()
+    ctx.sync()
+    t_plain = (time.perf_counter() - t) / steps
+    for e in encs:
+        e.close()
+    batch.close()
+    return {"images": len(images), "mpix": round(mpix, 2), "codeblocks": ncb, "autotune": tuned,
+            "batched_mpix_s": round(mpix / t_batch, 1), "plain_mpix_s": round(mpix / t_plain, 1),
+            "batched_ms": round(t_batch * 1e3, 3), "plain_ms": round(t_plain * 1e3, 3),
+            "speedup": round(t_plain / t_batch, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -36,6 +85,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain measurement")
+    ap.add_argument("--batch-images", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (single-GPU hot path; see DESIGN.md "
@@ -118,6 +169,9 @@ def main():
                       "t1_encode": round(t_t1, 4),
                       "t1_codeblocks_per_s": round(len(desc) / (t_t1 * 1e-3), 1)},
     }
+
+    if not args.no_batch:
+        out["batched"] = bench_batched(ctx, args)
 
     if not args.no_cpu:
         from oracle import oracle as O

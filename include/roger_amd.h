@@ -170,6 +170,21 @@ int rg_encoder_encode_u8(rg_encoder *enc, const uint8_t *src, int src_on_device,
 int rg_encoder_run_planar(rg_encoder *enc, const void *const *components, int ncomp, int sample_type,
                           int w, int h, int levels, int precision, int lossy, int output_dwt,
                           int cbw, int cbh, double delta);
+/* The same path as separate stages, for the stage classes:
+ *  load_planar  = OCLMemoryManager<T>::init + hostToDWTIn
+ *                 (src/OCLMemoryManager.cpp:56-160): host components -> HBM;
+ *  dwt          = OCLDWTForward<T>::run over all levels
+ *                 (src/OCLDWTForward.cpp:124-137), quantised unless output_dwt;
+ *  planar_i16   = OCLRGBtoPlanar<T>::run (src/OCLRGBtoPlanar.cpp:36-44): the
+ *                 per-channel int16 planes (OCLMemoryManager::getDWTOutByChannel,
+ *                 src/OCLMemoryManager.h:50-54), saturated from the int32 planes;
+ *  bpc          = OCLBPC<T>::run(cbx, cby) (src/OCLBPC.cpp:38-51). */
+int rg_encoder_load_planar(rg_encoder *enc, const void *const *components, int ncomp, int sample_type,
+                           int w, int h, int precision, int lossy);
+int rg_encoder_dwt(rg_encoder *enc, int levels, int output_dwt, double delta);
+int rg_encoder_planar_i16(rg_encoder *enc);
+int rg_encoder_channel_i16(rg_encoder *enc, int channel, int16_t **d_plane);
+int rg_encoder_bpc(rg_encoder *enc, int cbw, int cbh);
 /* Device pointers of the last run (valid until the next run / destroy). */
 int rg_encoder_coefficients(rg_encoder *enc, int32_t **d_q, int *ncomp, int *w, int *h);
 int rg_encoder_float_coefficients(rg_encoder *enc, float **d_coeff);
@@ -193,10 +208,45 @@ int rg_decoder_destroy(rg_decoder *dec);
  * RG_F32 (lossy: 9/7 float coefficients) or RG_I16/RG_I32 (5/3). */
 int rg_decoder_run_planar(rg_decoder *dec, const void *const *components, int ncomp, int sample_type,
                           int w, int h, int levels, int precision, int lossy);
+/* Separate stages: load (OCLMemoryManager<T>::init) and the inverse DWT
+ * (OCLDWTRev<T>::run, src/OCLDWTRev.cpp:39-59). */
+int rg_decoder_load_planar(rg_decoder *dec, const void *const *components, int ncomp, int sample_type,
+                           int w, int h, int precision, int lossy);
+int rg_decoder_idwt(rg_decoder *dec, int levels);
 /* Host view of the reconstructed samples, interleaved by component (float
  * for lossy, int16 for lossless), still level shifted. */
 int rg_decoder_map_out(rg_decoder *dec, void **host_ptr);
 int rg_decoder_unmap_out(rg_decoder *dec, void *host_ptr);
+
+/* ---- batched encode of many images (no counterpart in the reference, which
+ *      encodes one image per OCLEncoder::run call) ---- */
+typedef struct rg_batch rg_batch;
+typedef struct {
+    const uint8_t *d_src;  /* interleaved uint8 image in device memory, 16-byte aligned */
+    int32_t w, h, ncomp;   /* ncomp 1 or 3 */
+    int32_t filter, colour, levels;
+    double delta;          /* 9/7 base quantisation step */
+} rg_batch_image;
+int rg_batch_create(rg_context *ctx, rg_batch **out);
+int rg_batch_destroy(rg_batch *b);
+/* Lays out all buffers and job tables; with use_graph the pipeline (colour,
+ * every DWT level, tier-1) is captured once into a HIP graph.  waves_per_cu
+ * sizes the persistent tier-1 grid. */
+int rg_batch_plan(rg_batch *b, const rg_batch_image *imgs, int n, int cbw, int cbh, int use_graph, int waves_per_cu);
+/* Runs (replays) the plan asynchronously on the context stream. */
+int rg_batch_run(rg_batch *b);
+/* Code-blocks of one image (image = -1: the whole batch). */
+int rg_batch_num_codeblocks(rg_batch *b, int image);
+int rg_batch_coefficients(rg_batch *b, int image, int32_t **d_q);
+int rg_batch_get_codeblocks(rg_batch *b, int image, rg_cb_desc *desc, rg_cb_result *res, long *offsets,
+                            uint8_t *data, long cap, long *total);
+
+/* ---- synthetic test images generated on the device (the seeded recipe of
+ *      roger_amd/synth.py: params are 16 doubles per channel = 4 x (cos theta,
+ *      sin theta, period, phase); noise_base one counter base per channel;
+ *      walk_x[w] / walk_y[h] the luminance walks or both NULL) ---- */
+int rg_synth_u8(rg_context *ctx, uint8_t *d_dst, int w, int h, int ncomp, const double *params,
+                const uint64_t *noise_base, const double *walk_x, const double *walk_y);
 
 #ifdef __cplusplus
 }

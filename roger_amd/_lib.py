@@ -43,6 +43,12 @@ class CbResult(ctypes.Structure):
                 ("flags", ctypes.c_int32)]
 
 
+class BatchImage(ctypes.Structure):
+    _fields_ = [("d_src", ctypes.c_void_p), ("w", ctypes.c_int32), ("h", ctypes.c_int32), ("ncomp", ctypes.c_int32),
+                ("filter", ctypes.c_int32), ("colour", ctypes.c_int32), ("levels", ctypes.c_int32),
+                ("delta", ctypes.c_double)]
+
+
 class EncodeParams(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_int32), ("filter", ctypes.c_int32), ("colour", ctypes.c_int32),
                 ("cbw", ctypes.c_int32), ("cbh", ctypes.c_int32), ("delta", ctypes.c_double),
@@ -82,6 +88,11 @@ SIGNATURES = {
     "rg_encoder_destroy": (I, [P]),
     "rg_encoder_encode_u8": (I, [P, P, I, I, I, I, P]),
     "rg_encoder_run_planar": (I, [P, P, I, I, I, I, I, I, I, I, I, I, ctypes.c_double]),
+    "rg_encoder_load_planar": (I, [P, P, I, I, I, I, I, I]),
+    "rg_encoder_dwt": (I, [P, I, I, ctypes.c_double]),
+    "rg_encoder_planar_i16": (I, [P]),
+    "rg_encoder_channel_i16": (I, [P, I, P]),
+    "rg_encoder_bpc": (I, [P, I, I]),
     "rg_encoder_coefficients": (I, [P, P, P, P, P]),
     "rg_encoder_float_coefficients": (I, [P, P]),
     "rg_encoder_num_codeblocks": (I, [P]),
@@ -91,7 +102,17 @@ SIGNATURES = {
     "rg_decoder_create": (I, [P, P]),
     "rg_decoder_destroy": (I, [P]),
     "rg_decoder_run_planar": (I, [P, P, I, I, I, I, I, I, I]),
+    "rg_decoder_load_planar": (I, [P, P, I, I, I, I, I, I]),
+    "rg_decoder_idwt": (I, [P, I]),
     "rg_decoder_map_out": (I, [P, P]),
+    "rg_batch_create": (I, [P, P]),
+    "rg_batch_destroy": (I, [P]),
+    "rg_batch_plan": (I, [P, P, I, I, I, I, I]),
+    "rg_batch_run": (I, [P]),
+    "rg_batch_num_codeblocks": (I, [P, I]),
+    "rg_batch_coefficients": (I, [P, I, P]),
+    "rg_batch_get_codeblocks": (I, [P, I, P, P, P, P, L, P]),
+    "rg_synth_u8": (I, [P, P, I, I, I, P, P, P, P]),
     "rg_decoder_unmap_out": (I, [P, P]),
 }
 

@@ -215,6 +215,7 @@ static int dwt_fwd_impl(rg_context *c, int filter, const void *in, int w, int h,
     const bool half = filter == RG_FILTER_97 && ll_fp16;
     for (int l = 1; l <= levels; ++l) {
         rg_fwd_level_args a{};
+        a.filter = filter;
         a.cw = d.w[l - 1]; a.ch = d.h[l - 1]; a.nplanes = nplanes;
         if (l == 1) {
             a.in = in; a.in_pitch = w; a.in_ps = (long)w * h; a.in_half = 0;
@@ -272,6 +273,7 @@ extern "C" int rg_dwt_inv(rg_context *c, int filter, const void *dq, int w, int 
     char *bufB = bufA ? bufA + ((sizeA * 4 + 255) / 256) * 256 : nullptr;
     for (int l = levels; l >= 1; --l) {
         rg_inv_level_args a{};
+        a.filter = filter;
         a.cw = d.w[l - 1]; a.ch = d.h[l - 1]; a.nplanes = nplanes;
         if (l == levels) {
             a.ll_is_q = 1;
@@ -400,10 +402,10 @@ void band_steps(int levels, int bitdepth, double delta, float *step, float *inv)
 struct rg_encoder {
     rg_context *ctx = nullptr;
     int w = 0, h = 0, ncomp = 0, levels = 0, cbw = 0, cbh = 0, filter = 0;
-    int lossy = 0, output_dwt = 0, ran = 0, t1_ran = 0;
+    int lossy = 0, output_dwt = 0, ran = 0, t1_ran = 0, loaded = 0, i16_ran = 0, bitdepth = 8;
     int ncb_comp = 0, ncb = 0, slot_bytes = 0;
     float step[97], inv[97];
-    DevBuf input, planes, q, fcoeff, scratch, desc, res, slots;
+    DevBuf input, planes, q, fcoeff, scratch, desc, res, slots, chan16;
     HostBuf staging, map;
     std::vector<rg_cb_desc> h_desc;
     int desc_key[6] = {0, 0, 0, 0, 0, 0};
@@ -423,7 +425,8 @@ extern "C" int rg_encoder_destroy(rg_encoder *e)
     if (!e) return RG_SUCCESS;
     hipSetDevice(e->ctx->device);
     hipStreamSynchronize(e->ctx->stream);
-    for (DevBuf *b : {&e->input, &e->planes, &e->q, &e->fcoeff, &e->scratch, &e->desc, &e->res, &e->slots}) b->release();
+    for (DevBuf *b : {&e->input, &e->planes, &e->q, &e->fcoeff, &e->scratch, &e->desc, &e->res, &e->slots, &e->chan16})
+        b->release();
     e->staging.release();
     e->map.release();
     delete e;
@@ -489,8 +492,8 @@ extern "C" int rg_encoder_encode_u8(rg_encoder *e, const uint8_t *src, int on_de
     RG_HIP(hipSetDevice(e->ctx->device));
     const size_t npix = (size_t)w * h;
     e->w = w; e->h = h; e->ncomp = ncomp; e->levels = p->levels; e->cbw = p->cbw; e->cbh = p->cbh;
-    e->filter = p->filter; e->lossy = p->filter == RG_FILTER_97; e->output_dwt = 0;
-    e->ran = 0; e->t1_ran = 0;
+    e->filter = p->filter; e->lossy = p->filter == RG_FILTER_97; e->output_dwt = 0; e->bitdepth = p->bitdepth;
+    e->ran = 0; e->t1_ran = 0; e->loaded = 1; e->i16_ran = 0;
     band_steps(p->levels, p->bitdepth, p->delta, e->step, e->inv);
     const uint8_t *d_src = src;
     if (!on_device) {
@@ -510,55 +513,120 @@ extern "C" int rg_encoder_encode_u8(rg_encoder *e, const uint8_t *src, int on_de
     return run_t1(e);
 }
 
-extern "C" int rg_encoder_run_planar(rg_encoder *e, const void *const *comps, int ncomp, int type, int w, int h,
-                                     int levels, int precision, int lossy, int output_dwt, int cbw, int cbh,
-                                     double delta)
+// Host conversion of T components into the kernels' sample type (float for
+// 9/7, int32 for 5/3) in pinned staging memory.
+static int stage_components(HostBuf &staging, hipStream_t s, const void *const *comps, int ncomp, int type,
+                            size_t npix, int lossy)
 {
-    RG_CHECK(e && comps, "rg_encoder_run_planar: null argument");
-    RG_CHECK(type == RG_F32 || type == RG_I16 || type == RG_I32, "bad sample type %d", type);
-    rg_encode_params p{};
-    p.levels = levels; p.filter = lossy ? RG_FILTER_97 : RG_FILTER_53; p.colour = RG_COLOUR_NONE;
-    p.cbw = cbw; p.cbh = cbh; p.delta = delta; p.ll_fp16 = 0; p.bitdepth = precision > 0 ? precision : 8;
-    int rc = check_params(&p, w, h, ncomp);
-    if (rc) return rc;
-    for (int c = 0; c < ncomp; ++c) RG_CHECK(comps[c], "component %d is null", c);
-    RG_HIP(hipSetDevice(e->ctx->device));
-    const size_t npix = (size_t)w * h;
-    e->w = w; e->h = h; e->ncomp = ncomp; e->levels = levels; e->cbw = cbw; e->cbh = cbh;
-    e->filter = p.filter; e->lossy = lossy; e->output_dwt = lossy && output_dwt; e->ran = 0; e->t1_ran = 0;
-    band_steps(levels, p.bitdepth, delta, e->step, e->inv);
-    // host conversion of the T components into the kernels' sample type
-    if ((rc = e->staging.ensure(npix * ncomp * 4))) return rc;
-    RG_HIP(hipStreamSynchronize(e->ctx->stream));  // staging may still feed a previous copy
+    int rc;
+    if ((rc = staging.ensure(npix * ncomp * 4))) return rc;
+    RG_HIP(hipStreamSynchronize(s));  // staging may still feed a previous copy
     for (int c = 0; c < ncomp; ++c) {
         if (lossy) {
-            float *dst = (float *)e->staging.p + c * npix;
+            float *dst = (float *)staging.p + c * npix;
             if (type == RG_F32) memcpy(dst, comps[c], npix * 4);
             else if (type == RG_I16) for (size_t i = 0; i < npix; ++i) dst[i] = ((const int16_t *)comps[c])[i];
             else for (size_t i = 0; i < npix; ++i) dst[i] = (float)((const int32_t *)comps[c])[i];
         } else {
-            int32_t *dst = (int32_t *)e->staging.p + c * npix;
+            int32_t *dst = (int32_t *)staging.p + c * npix;
             if (type == RG_F32) for (size_t i = 0; i < npix; ++i) dst[i] = (int32_t)lrintf(((const float *)comps[c])[i]);
             else if (type == RG_I16) for (size_t i = 0; i < npix; ++i) dst[i] = ((const int16_t *)comps[c])[i];
             else memcpy(dst, comps[c], npix * 4);
         }
     }
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_encoder_load_planar(rg_encoder *e, const void *const *comps, int ncomp, int type, int w, int h,
+                                      int precision, int lossy)
+{
+    RG_CHECK(e && comps, "rg_encoder_load_planar: null argument");
+    RG_CHECK(type == RG_F32 || type == RG_I16 || type == RG_I32, "bad sample type %d", type);
+    RG_CHECK(w > 0 && h > 0 && ncomp >= 1 && ncomp <= 4, "bad image shape %dx%dx%d", w, h, ncomp);
+    RG_CHECK(precision >= 0 && precision <= 16, "bad precision %d", precision);
+    for (int c = 0; c < ncomp; ++c) RG_CHECK(comps[c], "component %d is null", c);
+    RG_HIP(hipSetDevice(e->ctx->device));
+    const size_t npix = (size_t)w * h;
+    int rc = stage_components(e->staging, e->ctx->stream, comps, ncomp, type, npix, lossy);
+    if (rc) return rc;
     if ((rc = e->planes.ensure(npix * ncomp * 4))) return rc;
-    if ((rc = e->q.ensure(npix * ncomp * 4))) return rc;
-    if ((rc = e->scratch.ensure(rg_dwt_scratch_bytes(w, h, ncomp, p.filter)))) return rc;
     RG_HIP(hipMemcpyAsync(e->planes.p, e->staging.p, npix * ncomp * 4, hipMemcpyHostToDevice, e->ctx->stream));
+    e->w = w; e->h = h; e->ncomp = ncomp; e->lossy = lossy; e->bitdepth = precision > 0 ? precision : 8;
+    e->filter = lossy ? RG_FILTER_97 : RG_FILTER_53;
+    e->loaded = 1; e->ran = 0; e->t1_ran = 0; e->i16_ran = 0;
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_encoder_dwt(rg_encoder *e, int levels, int output_dwt, double delta)
+{
+    RG_CHECK(e, "null encoder");
+    if (!e->loaded) return rg_fail(RG_ERR_NOT_READY, "no image loaded");
+    RG_CHECK(levels >= 0 && levels <= 15, "levels %d out of range 0..15", levels);
+    RG_CHECK(!e->lossy || (delta > 0 && std::isfinite(delta)), "bad quantisation step");
+    const size_t npix = (size_t)e->w * e->h;
+    int rc;
+    if ((rc = e->q.ensure(npix * e->ncomp * 4))) return rc;
+    if ((rc = e->scratch.ensure(rg_dwt_scratch_bytes(e->w, e->h, e->ncomp, e->filter)))) return rc;
+    e->levels = levels;
+    e->output_dwt = e->lossy && output_dwt;
+    e->t1_ran = 0; e->i16_ran = 0;
+    band_steps(levels, e->bitdepth, delta, e->step, e->inv);
     if (e->output_dwt) {
-        if ((rc = e->fcoeff.ensure(npix * ncomp * 4))) return rc;
-        rc = dwt_fwd_impl(e->ctx, RG_FILTER_97, e->planes.p, w, h, ncomp, levels, nullptr, 0, e->fcoeff.p, 1,
+        if ((rc = e->fcoeff.ensure(npix * e->ncomp * 4))) return rc;
+        rc = dwt_fwd_impl(e->ctx, RG_FILTER_97, e->planes.p, e->w, e->h, e->ncomp, levels, nullptr, 0, e->fcoeff.p, 1,
                           e->scratch.p);
-        if (rc) return rc;
-        e->ran = 1;
-        return RG_SUCCESS;   // outputDwt: the reference skips the bit-plane coder (src/OCLEncoder.cpp:46-50)
+    } else {
+        rc = dwt_fwd_impl(e->ctx, e->filter, e->planes.p, e->w, e->h, e->ncomp, levels, e->inv, 0, e->q.p, 0,
+                          e->scratch.p);
     }
-    rc = dwt_fwd_impl(e->ctx, p.filter, e->planes.p, w, h, ncomp, levels, e->inv, 0, e->q.p, 0, e->scratch.p);
     if (rc) return rc;
     e->ran = 1;
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_encoder_bpc(rg_encoder *e, int cbw, int cbh)
+{
+    RG_CHECK(e, "null encoder");
+    if (!e->ran || e->output_dwt) return rg_fail(RG_ERR_NOT_READY, "no quantised coefficients to code");
+    if (rg_codeblocks(e->w, e->h, e->levels, cbw, cbh, nullptr, 0) < 0) return RG_ERR_INVALID;
+    e->cbw = cbw; e->cbh = cbh;
     return run_t1(e);
+}
+
+int rg_launch_sat_i16(hipStream_t s, const int32_t *src, long n, int16_t *dst);
+
+extern "C" int rg_encoder_planar_i16(rg_encoder *e)
+{
+    RG_CHECK(e, "null encoder");
+    if (!e->ran || e->output_dwt) return rg_fail(RG_ERR_NOT_READY, "no quantised coefficients");
+    const size_t n = (size_t)e->w * e->h * e->ncomp;
+    int rc;
+    if ((rc = e->chan16.ensure(n * 2))) return rc;
+    rc = rg_launch_sat_i16(e->ctx->stream, (const int32_t *)e->q.p, (long)n, (int16_t *)e->chan16.p);
+    if (rc) return rc;
+    e->i16_ran = 1;
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_encoder_channel_i16(rg_encoder *e, int c, int16_t **d_plane)
+{
+    RG_CHECK(e && d_plane, "null argument");
+    if (!e->i16_ran) return rg_fail(RG_ERR_NOT_READY, "planar int16 channels not produced");
+    RG_CHECK(c >= 0 && c < e->ncomp, "channel %d out of range", c);
+    *d_plane = (int16_t *)e->chan16.p + (size_t)c * e->w * e->h;
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_encoder_run_planar(rg_encoder *e, const void *const *comps, int ncomp, int type, int w, int h,
+                                     int levels, int precision, int lossy, int output_dwt, int cbw, int cbh,
+                                     double delta)
+{
+    int rc = rg_encoder_load_planar(e, comps, ncomp, type, w, h, precision, lossy);
+    if (rc) return rc;
+    if ((rc = rg_encoder_dwt(e, levels, output_dwt, delta))) return rc;
+    if (e->output_dwt) return RG_SUCCESS;  // outputDwt: no bit-plane coder (src/OCLEncoder.cpp:46-50)
+    if ((rc = rg_encoder_planar_i16(e))) return rc;
+    return rg_encoder_bpc(e, cbw, cbh);
 }
 
 extern "C" int rg_encoder_coefficients(rg_encoder *e, int32_t **dq, int *ncomp, int *w, int *h)
@@ -661,7 +729,7 @@ extern "C" int rg_encoder_unmap_dwt_out(rg_encoder *e, void *host)
 // ------------------------------------------------------------ decoder
 struct rg_decoder {
     rg_context *ctx = nullptr;
-    int w = 0, h = 0, ncomp = 0, lossy = 0, ran = 0;
+    int w = 0, h = 0, ncomp = 0, lossy = 0, ran = 0, loaded = 0;
     DevBuf coeff, out, scratch;
     HostBuf staging, map;
 };
@@ -686,42 +754,46 @@ extern "C" int rg_decoder_destroy(rg_decoder *d)
     return RG_SUCCESS;
 }
 
-extern "C" int rg_decoder_run_planar(rg_decoder *d, const void *const *comps, int ncomp, int type, int w, int h,
-                                     int levels, int precision, int lossy)
+extern "C" int rg_decoder_load_planar(rg_decoder *d, const void *const *comps, int ncomp, int type, int w, int h,
+                                      int precision, int lossy)
 {
     (void)precision;
     RG_CHECK(d && comps, "null argument");
     RG_CHECK(type == RG_F32 || type == RG_I16 || type == RG_I32, "bad sample type %d", type);
     RG_CHECK(w > 0 && h > 0 && ncomp >= 1 && ncomp <= 4, "bad image shape");
-    RG_CHECK(levels >= 0 && levels <= 15, "levels out of range");
     for (int c = 0; c < ncomp; ++c) RG_CHECK(comps[c], "component %d is null", c);
     RG_HIP(hipSetDevice(d->ctx->device));
     const size_t npix = (size_t)w * h;
-    int rc;
-    if ((rc = d->staging.ensure(npix * ncomp * 4))) return rc;
-    RG_HIP(hipStreamSynchronize(d->ctx->stream));
-    for (int c = 0; c < ncomp; ++c) {
-        if (lossy) {
-            float *dst = (float *)d->staging.p + c * npix;
-            if (type == RG_F32) memcpy(dst, comps[c], npix * 4);
-            else if (type == RG_I16) for (size_t i = 0; i < npix; ++i) dst[i] = ((const int16_t *)comps[c])[i];
-            else for (size_t i = 0; i < npix; ++i) dst[i] = (float)((const int32_t *)comps[c])[i];
-        } else {
-            int32_t *dst = (int32_t *)d->staging.p + c * npix;
-            if (type == RG_F32) for (size_t i = 0; i < npix; ++i) dst[i] = (int32_t)lrintf(((const float *)comps[c])[i]);
-            else if (type == RG_I16) for (size_t i = 0; i < npix; ++i) dst[i] = ((const int16_t *)comps[c])[i];
-            else memcpy(dst, comps[c], npix * 4);
-        }
-    }
-    if ((rc = d->coeff.ensure(npix * ncomp * 4))) return rc;
-    if ((rc = d->out.ensure(npix * ncomp * 4))) return rc;
-    if ((rc = d->scratch.ensure(rg_dwt_scratch_bytes(w, h, ncomp, lossy)))) return rc;
-    RG_HIP(hipMemcpyAsync(d->coeff.p, d->staging.p, npix * ncomp * 4, hipMemcpyHostToDevice, d->ctx->stream));
-    rc = rg_dwt_inv(d->ctx, lossy ? RG_FILTER_97 : RG_FILTER_53, d->coeff.p, w, h, ncomp, levels, nullptr, d->out.p,
-                    d->scratch.p);
+    int rc = stage_components(d->staging, d->ctx->stream, comps, ncomp, type, npix, lossy);
     if (rc) return rc;
-    d->w = w; d->h = h; d->ncomp = ncomp; d->lossy = lossy; d->ran = 1;
+    if ((rc = d->coeff.ensure(npix * ncomp * 4))) return rc;
+    RG_HIP(hipMemcpyAsync(d->coeff.p, d->staging.p, npix * ncomp * 4, hipMemcpyHostToDevice, d->ctx->stream));
+    d->w = w; d->h = h; d->ncomp = ncomp; d->lossy = lossy; d->loaded = 1; d->ran = 0;
     return RG_SUCCESS;
+}
+
+extern "C" int rg_decoder_idwt(rg_decoder *d, int levels)
+{
+    RG_CHECK(d, "null decoder");
+    if (!d->loaded) return rg_fail(RG_ERR_NOT_READY, "no coefficients loaded");
+    RG_CHECK(levels >= 0 && levels <= 15, "levels out of range");
+    const size_t npix = (size_t)d->w * d->h;
+    int rc;
+    if ((rc = d->out.ensure(npix * d->ncomp * 4))) return rc;
+    if ((rc = d->scratch.ensure(rg_dwt_scratch_bytes(d->w, d->h, d->ncomp, d->lossy)))) return rc;
+    rc = rg_dwt_inv(d->ctx, d->lossy ? RG_FILTER_97 : RG_FILTER_53, d->coeff.p, d->w, d->h, d->ncomp, levels, nullptr,
+                    d->out.p, d->scratch.p);
+    if (rc) return rc;
+    d->ran = 1;
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_decoder_run_planar(rg_decoder *d, const void *const *comps, int ncomp, int type, int w, int h,
+                                     int levels, int precision, int lossy)
+{
+    int rc = rg_decoder_load_planar(d, comps, ncomp, type, w, h, precision, lossy);
+    if (rc) return rc;
+    return rg_decoder_idwt(d, levels);
 }
 
 extern "C" int rg_decoder_map_out(rg_decoder *d, void **host)

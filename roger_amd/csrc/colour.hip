@@ -36,10 +36,9 @@ __device__ __forceinline__ uint8_t out_u8f(float x)
 
 // MODE: 0 shift only, 1 RCT, 2 ICT.  FLT: float planes (else int32).
 template <int NC, int MODE, bool FLT>
-__global__ void __launch_bounds__(256) colour_fwd_kernel(const uint8_t *__restrict__ src, long npix,
-                                                          void *__restrict__ planes)
+__device__ __forceinline__ void colour_fwd_body(const uint8_t *__restrict__ src, long npix, void *__restrict__ planes,
+                                                long g)
 {
-    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long p0 = g * PX;
     if (p0 >= npix) return;
     int n = (int)min((long)PX, npix - p0);
@@ -108,6 +107,13 @@ __global__ void __launch_bounds__(256) colour_fwd_kernel(const uint8_t *__restri
             }
         }
     }
+}
+
+template <int NC, int MODE, bool FLT>
+__global__ void __launch_bounds__(256) colour_fwd_kernel(const uint8_t *__restrict__ src, long npix,
+                                                          void *__restrict__ planes)
+{
+    colour_fwd_body<NC, MODE, FLT>(src, npix, planes, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 template <int NC, int MODE, bool FLT>
@@ -250,5 +256,121 @@ int rg_launch_planar_split_i16(hipStream_t s, const int16_t *src, int w, int h, 
     if (npix == 0) return RG_SUCCESS;
     planar_split_i16_kernel<<<dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, s>>>(src, npix, ncomp, dst);
     RG_LAUNCH_CHECK();
+    return RG_SUCCESS;
+}
+
+// int32 coefficient planes -> saturated int16 planes (the per-channel int16
+// planes the reference's planar stage leaves for its bit-plane coder).
+namespace {
+__global__ void sat_i16_kernel(const int32_t *__restrict__ src, long n, int16_t *__restrict__ dst)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int v = src[i];
+    dst[i] = (int16_t)(v < -32768 ? -32768 : (v > 32767 ? 32767 : v));
+}
+}  // namespace
+
+int rg_launch_sat_i16(hipStream_t s, const int32_t *src, long n, int16_t *dst)
+{
+    if (n == 0) return RG_SUCCESS;
+    sat_i16_kernel<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(src, n, dst);
+    RG_LAUNCH_CHECK();
+    return RG_SUCCESS;
+}
+
+// ---- batched colour: one workgroup per job {image, first 16-px chunk} ----
+namespace {
+__global__ void __launch_bounds__(256) colour_fwd_batch_kernel(const rg_colour_job_params *__restrict__ P,
+                                                               const int2 *__restrict__ jobs)
+{
+    const int2 j = jobs[blockIdx.x];
+    const rg_colour_job_params p = P[j.x];
+    const long g = (long)j.y + threadIdx.x;
+    if (p.ncomp == 3 && p.colour == RG_COLOUR_ICT) colour_fwd_body<3, 2, true>(p.src, p.npix, p.planes, g);
+    else if (p.ncomp == 3 && p.colour == RG_COLOUR_RCT) colour_fwd_body<3, 1, false>(p.src, p.npix, p.planes, g);
+    else if (p.ncomp == 3) {
+        if (p.out_float) colour_fwd_body<3, 0, true>(p.src, p.npix, p.planes, g);
+        else colour_fwd_body<3, 0, false>(p.src, p.npix, p.planes, g);
+    } else {
+        if (p.out_float) colour_fwd_body<1, 0, true>(p.src, p.npix, p.planes, g);
+        else colour_fwd_body<1, 0, false>(p.src, p.npix, p.planes, g);
+    }
+}
+}  // namespace
+
+int rg_launch_colour_fwd_batch(hipStream_t s, const rg_colour_job_params *params, const int2 *jobs, int njobs)
+{
+    if (njobs <= 0) return RG_SUCCESS;
+    colour_fwd_batch_kernel<<<dim3((unsigned)njobs), dim3(256), 0, s>>>(params, jobs);
+    RG_LAUNCH_CHECK();
+    return RG_SUCCESS;
+}
+
+// ---- synthetic test images on the device ----
+// Same recipe and FP64 operation order as roger_amd/synth.py: 128 + sum of 4
+// sinusoids (amplitude 24) + integer noise -8..8 (+ a luminance walk for RGB),
+// rounded half up and clamped.  Per channel: 12 params (period, theta, phi) x4
+// already expanded to (cos(theta), sin(theta), period, phi) by the host.
+namespace {
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ void synth_kernel(uint8_t *__restrict__ dst, int w, int h, int ncomp, const double *__restrict__ par,
+                             const uint64_t *__restrict__ noise_base, const double *__restrict__ walk_x,
+                             const double *__restrict__ walk_y)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)w * h) return;
+    const int x = (int)(i % w), y = (int)(i / w);
+    const double twopi = 2.0 * 3.141592653589793;
+    for (int c = 0; c < ncomp; ++c) {
+        const double *p = par + 16 * c;
+        double f = 128.0;
+        for (int k = 0; k < 4; ++k) {
+            double t = (double)x * p[4 * k] + (double)y * p[4 * k + 1];
+            double arg = twopi * t / p[4 * k + 2] + p[4 * k + 3];
+            f += 24.0 * sin(arg);
+        }
+        uint64_t z = mix64(noise_base[c] * (1ull << 20) + (uint64_t)i);
+        double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+        f = f + (floor(u * 17.0) - 8.0);
+        if (walk_x) f = f + (walk_x[x] + walk_y[y]);
+        double r = floor(f + 0.5);
+        dst[i * ncomp + c] = (uint8_t)(r < 0.0 ? 0.0 : (r > 255.0 ? 255.0 : r));
+    }
+}
+}  // namespace
+
+extern "C" int rg_synth_u8(rg_context *ctx, uint8_t *d_dst, int w, int h, int ncomp, const double *params,
+                           const uint64_t *noise_base, const double *walk_x, const double *walk_y)
+{
+    RG_CHECK(ctx && d_dst && params && noise_base, "rg_synth_u8: null argument");
+    RG_CHECK(w > 0 && h > 0 && ncomp >= 1 && ncomp <= 4, "rg_synth_u8: bad shape");
+    RG_CHECK((walk_x == nullptr) == (walk_y == nullptr), "rg_synth_u8: give both walks or neither");
+    // small host arrays -> device (stream-ordered, freed after the launch completes)
+    size_t npar = 16 * (size_t)ncomp, bytes = npar * 8 + ncomp * 8 + (walk_x ? ((size_t)w + h) * 8 : 0);
+    char *tmp = nullptr;
+    RG_HIP(hipMallocAsync((void **)&tmp, bytes, ctx->stream));
+    RG_HIP(hipMemcpyAsync(tmp, params, npar * 8, hipMemcpyHostToDevice, ctx->stream));
+    RG_HIP(hipMemcpyAsync(tmp + npar * 8, noise_base, ncomp * 8, hipMemcpyHostToDevice, ctx->stream));
+    double *wx = nullptr, *wy = nullptr;
+    if (walk_x) {
+        wx = (double *)(tmp + npar * 8 + ncomp * 8);
+        wy = wx + w;
+        RG_HIP(hipMemcpyAsync(wx, walk_x, (size_t)w * 8, hipMemcpyHostToDevice, ctx->stream));
+        RG_HIP(hipMemcpyAsync(wy, walk_y, (size_t)h * 8, hipMemcpyHostToDevice, ctx->stream));
+    }
+    long n = (long)w * h;
+    synth_kernel<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream>>>(
+        d_dst, w, h, ncomp, (const double *)tmp, (const uint64_t *)(tmp + npar * 8), wx, wy);
+    RG_LAUNCH_CHECK();
+    RG_HIP(hipFreeAsync(tmp, ctx->stream));
+    RG_HIP(hipStreamSynchronize(ctx->stream));
     return RG_SUCCESS;
 }

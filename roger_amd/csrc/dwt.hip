@@ -121,18 +121,21 @@ __device__ __forceinline__ float dequant(int q, float s)
 }
 
 // ============================ forward ============================
+constexpr int FWD_LDS_BYTES = TILE * (TILE + 2 * 4 + 1) * 4;   // largest (9/7) tile
+constexpr int INV_LDS_BYTES = (TILE + 2 * 4) * (TILE + 1) * 4;
+
+// One 64x64 tile (bx, by) of plane `plane`; `lds` holds FWD_LDS_BYTES.
 template <int F>
-__global__ void __launch_bounds__(256) fwd_level_kernel(rg_fwd_level_args a)
+__device__ __forceinline__ void fwd_tile(const rg_fwd_level_args &a, int plane, int bx, int by, char *lds)
 {
     using T = typename Filt<F>::T;
     constexpr int H = Filt<F>::H;
     constexpr int TW = TILE + 2 * H;          // LDS columns (with halo)
     constexpr int LP = TW + 1;                // padded pitch
     constexpr int N = SEG + 2 * H;            // register window
-    __shared__ T tile[TILE * LP];
+    T *tile = reinterpret_cast<T *>(lds);
 
-    const int plane = blockIdx.z;
-    const int x0 = blockIdx.x * TILE, y0 = blockIdx.y * TILE;
+    const int x0 = bx * TILE, y0 = by * TILE;
     const int cw = a.cw, ch = a.ch;
     const void *in = reinterpret_cast<const char *>(a.in) +
                      (long)plane * a.in_ps * (F == RG_FILTER_53 ? 4 : (a.in_half ? 2 : 4));
@@ -227,17 +230,16 @@ __device__ __forceinline__ typename Filt<F>::T load_band(const rg_inv_level_args
 }
 
 template <int F>
-__global__ void __launch_bounds__(256) inv_level_kernel(rg_inv_level_args a)
+__device__ __forceinline__ void inv_tile(const rg_inv_level_args &a, int plane, int bx, int by, char *lds)
 {
     using T = typename Filt<F>::T;
     constexpr int H = Filt<F>::H;
     constexpr int TH = TILE + 2 * H;          // LDS rows (with halo)
     constexpr int LP = TILE + 1;
     constexpr int N = SEG + 2 * H;
-    __shared__ T tile[TH * LP];
+    T *tile = reinterpret_cast<T *>(lds);
 
-    const int plane = blockIdx.z;
-    const int x0 = blockIdx.x * TILE, y0 = blockIdx.y * TILE;
+    const int x0 = bx * TILE, y0 = by * TILE;
     const int cw = a.cw, ch = a.ch;
     const int wl = (cw + 1) >> 1, hl = (ch + 1) >> 1;
     const bool vlift = ch > 1, hlift = cw > 1;
@@ -277,7 +279,59 @@ __global__ void __launch_bounds__(256) inv_level_kernel(rg_inv_level_args a)
     }
 }
 
+template <int F>
+__global__ void __launch_bounds__(256) fwd_level_kernel(rg_fwd_level_args a)
+{
+    __shared__ __attribute__((aligned(16))) char lds[FWD_LDS_BYTES];
+    fwd_tile<F>(a, blockIdx.z, blockIdx.x, blockIdx.y, lds);
+}
+
+template <int F>
+__global__ void __launch_bounds__(256) inv_level_kernel(rg_inv_level_args a)
+{
+    __shared__ __attribute__((aligned(16))) char lds[INV_LDS_BYTES];
+    inv_tile<F>(a, blockIdx.z, blockIdx.x, blockIdx.y, lds);
+}
+
+// Batched: one workgroup per job {image args index, plane, tile x, tile y};
+// images of both filters share the launch (the filter is uniform per job).
+__global__ void __launch_bounds__(256) fwd_level_batch_kernel(const rg_fwd_level_args *__restrict__ args,
+                                                              const int4 *__restrict__ jobs)
+{
+    __shared__ __attribute__((aligned(16))) char lds[FWD_LDS_BYTES];
+    const int4 j = jobs[blockIdx.x];
+    const rg_fwd_level_args a = args[j.x];
+    if (a.filter == RG_FILTER_53) fwd_tile<RG_FILTER_53>(a, j.y, j.z, j.w, lds);
+    else fwd_tile<RG_FILTER_97>(a, j.y, j.z, j.w, lds);
+}
+
+__global__ void __launch_bounds__(256) inv_level_batch_kernel(const rg_inv_level_args *__restrict__ args,
+                                                              const int4 *__restrict__ jobs)
+{
+    __shared__ __attribute__((aligned(16))) char lds[INV_LDS_BYTES];
+    const int4 j = jobs[blockIdx.x];
+    const rg_inv_level_args a = args[j.x];
+    if (a.filter == RG_FILTER_53) inv_tile<RG_FILTER_53>(a, j.y, j.z, j.w, lds);
+    else inv_tile<RG_FILTER_97>(a, j.y, j.z, j.w, lds);
+}
+
 }  // namespace
+
+int rg_launch_fwd_level_batch(hipStream_t s, const rg_fwd_level_args *args, const int4 *jobs, int njobs)
+{
+    if (njobs <= 0) return RG_SUCCESS;
+    fwd_level_batch_kernel<<<dim3((unsigned)njobs), dim3(256), 0, s>>>(args, jobs);
+    RG_LAUNCH_CHECK();
+    return RG_SUCCESS;
+}
+
+int rg_launch_inv_level_batch(hipStream_t s, const rg_inv_level_args *args, const int4 *jobs, int njobs)
+{
+    if (njobs <= 0) return RG_SUCCESS;
+    inv_level_batch_kernel<<<dim3((unsigned)njobs), dim3(256), 0, s>>>(args, jobs);
+    RG_LAUNCH_CHECK();
+    return RG_SUCCESS;
+}
 
 int rg_launch_fwd53_level(hipStream_t s, const rg_fwd_level_args &a)
 {

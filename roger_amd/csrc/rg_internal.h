@@ -64,6 +64,7 @@ int rg_launch_planar_split_i16(hipStream_t s, const int16_t *src, int w, int h, 
 // LL to ll_out (pitch ll_pitch, plane stride ll_ps), high bands into the
 // Mallat plane q (pitch qp, plane stride q_ps) at offsets (wl, hl).
 struct rg_fwd_level_args {
+    int filter;             // RG_FILTER_53 / RG_FILTER_97 (read by the batched kernel)
     const void *in;
     long in_pitch, in_ps;
     int in_half;            // input LL stored as fp16 (9/7)
@@ -82,6 +83,7 @@ int rg_launch_fwd97_level(hipStream_t s, const rg_fwd_level_args &a);
 // One inverse level: reads the four bands of a cw x ch region (LL from ll_in,
 // pitch ll_pitch; high bands from the Mallat plane q), writes the region to out.
 struct rg_inv_level_args {
+    int filter;
     const void *ll_in;
     long ll_pitch, ll_ps;
     int ll_is_q;            // LL read from q (coarsest level) with dequantisation
@@ -96,6 +98,25 @@ struct rg_inv_level_args {
 int rg_launch_inv53_level(hipStream_t s, const rg_inv_level_args &a);
 int rg_launch_inv97_level(hipStream_t s, const rg_inv_level_args &a);
 
+int rg_launch_fwd_level_batch(hipStream_t s, const rg_fwd_level_args *args, const int4 *jobs, int njobs);
+int rg_launch_inv_level_batch(hipStream_t s, const rg_inv_level_args *args, const int4 *jobs, int njobs);
+
 int rg_launch_t1_encode(hipStream_t s, const int32_t *q, int pitch, size_t plane_stride,
                         const rg_cb_desc *desc, int ncb, uint8_t *out, int slot_bytes,
                         rg_cb_result *res);
+
+// Batched tier-1: job records built on the host with rg_t1_job_fill.
+int rg_t1_job_bytes();
+void rg_t1_job_fill(void *dst, const int32_t *src, int pitch, int w, int h, int orient, uint8_t *out, int cap,
+                    rg_cb_result *res);
+int rg_launch_t1_queue(hipStream_t s, const void *jobs, int njobs, int *counter, int nwaves);
+
+// Batched colour: per-image parameters; jobs {image, first chunk} cover 256
+// chunks of 16 pixels each.
+struct rg_colour_job_params {
+    const uint8_t *src;
+    long npix;
+    int ncomp, colour, out_float;
+    void *planes;
+};
+int rg_launch_colour_fwd_batch(hipStream_t s, const rg_colour_job_params *params, const int2 *jobs, int njobs);

@@ -199,17 +199,20 @@ __device__ __forceinline__ void mq_run(MQEnc &m, const uint8_t *sbuf, uint8_t *c
     }
 }
 
-__global__ void __launch_bounds__(64) t1_encode_kernel(const int32_t *__restrict__ q, int pitch, size_t plane_stride,
-                                                       const rg_cb_desc *__restrict__ desc, uint8_t *__restrict__ out,
-                                                       int slot_bytes, rg_cb_result *__restrict__ res)
-{
-    __shared__ uint8_t sbuf[SYMBUF];
-    __shared__ uint8_t ctx[32];
+// One code-block job: coefficients (row pitch), size, orientation, output slot.
+struct T1Job {
+    const int32_t *src;
+    int pitch, w, h, orient, cap;
+    uint8_t *out;
+    rg_cb_result *res;
+};
 
+// Encodes one code-block with the calling wavefront (64 lanes).
+__device__ void t1_block(const T1Job &job, uint8_t *sbuf, uint8_t *ctx)
+{
     const int lane = threadIdx.x;
-    const rg_cb_desc d = desc[blockIdx.x];
-    const int w = d.w, h = d.h, orient = d.orient;
-    const int32_t *src = q + (size_t)d.comp * plane_stride + (size_t)d.y0 * pitch + d.x0;
+    const int w = job.w, h = job.h, orient = job.orient, pitch = job.pitch;
+    const int32_t *src = job.src;
 
     // ---- load column `lane`: magnitudes in registers, sign as a row mask ----
     uint32_t mag[64];
@@ -228,7 +231,7 @@ __global__ void __launch_bounds__(64) t1_encode_kernel(const int32_t *__restrict
     for (int o = 32; o >= 1; o >>= 1) orm |= __shfl_xor(orm, o, 64);
     const int nbp = orm ? 32 - __clz(orm) : 0;
     if (nbp == 0) {
-        if (lane == 0) res[blockIdx.x] = rg_cb_result{0, 0, 0, 0};
+        if (lane == 0) *job.res = rg_cb_result{0, 0, 0, 0};
         return;
     }
 
@@ -236,15 +239,15 @@ __global__ void __launch_bounds__(64) t1_encode_kernel(const int32_t *__restrict
     __syncthreads();
 
     const uint64_t rowmask = h >= 64 ? ~0ull : ((1ull << h) - 1);
-    const uint64_t NEGL = lane == 0 ? 0 : [&] { uint64_t t = __shfl_up(NEG, 1, 64); return t; }();
+    const uint64_t NEGL = __shfl_up(NEG, 1, 64);
     const uint64_t NEGR = __shfl_down(NEG, 1, 64);
     const uint64_t negl = lane == 0 ? 0 : NEGL;
     const uint64_t negr = lane == 63 ? 0 : NEGR;
 
     MQEnc m;
     m.A = 0x8000; m.C = 0; m.B = 0; m.CT = 12; m.bp = -1;
-    m.cap = slot_bytes; m.overflow = 0;
-    m.out = out + (size_t)blockIdx.x * slot_bytes;
+    m.cap = job.cap; m.overflow = 0;
+    m.out = job.out;
 
     uint64_t S = 0, V = 0, R = 0;
     const int nstripes = (h + 3) >> 2;
@@ -392,12 +395,69 @@ __global__ void __launch_bounds__(64) t1_encode_kernel(const int32_t *__restrict
         int len;
         if (m.B != 0xFF) { mq_store(m, lane); len = m.bp + 1; }
         else len = m.bp;
-        if (len > slot_bytes) m.overflow = 1;
-        if (lane == 0) res[blockIdx.x] = rg_cb_result{len, 3 * nbp - 2, nbp, m.overflow ? 1 : 0};
+        if (len > job.cap) m.overflow = 1;
+        if (lane == 0) *job.res = rg_cb_result{len, 3 * nbp - 2, nbp, m.overflow ? 1 : 0};
     }
 }
 
+// Plain path: one workgroup (one wavefront) per descriptor.
+__global__ void __launch_bounds__(64) t1_encode_kernel(const int32_t *__restrict__ q, int pitch, size_t plane_stride,
+                                                       const rg_cb_desc *__restrict__ desc, uint8_t *__restrict__ out,
+                                                       int slot_bytes, rg_cb_result *__restrict__ res)
+{
+    __shared__ uint8_t sbuf[SYMBUF];
+    __shared__ uint8_t ctx[32];
+    const rg_cb_desc d = desc[blockIdx.x];
+    T1Job job;
+    job.src = q + (size_t)d.comp * plane_stride + (size_t)d.y0 * pitch + d.x0;
+    job.pitch = pitch; job.w = d.w; job.h = d.h; job.orient = d.orient; job.cap = slot_bytes;
+    job.out = out + (size_t)blockIdx.x * slot_bytes;
+    job.res = res + blockIdx.x;
+    t1_block(job, sbuf, ctx);
+}
+
+// Batched path: a persistent grid of wavefronts pulls job indices from a
+// device counter (jobs are pre-sorted, largest first) until the list is done.
+__global__ void __launch_bounds__(64) t1_encode_queue_kernel(const T1Job *__restrict__ jobs, int njobs,
+                                                             int *__restrict__ counter)
+{
+    __shared__ uint8_t sbuf[SYMBUF];
+    __shared__ uint8_t ctx[32];
+    for (;;) {
+        int i = 0;
+        if (threadIdx.x == 0) i = atomicAdd(counter, 1);
+        i = __shfl(i, 0, 64);
+        if (i >= njobs) break;
+        const T1Job job = jobs[i];
+        t1_block(job, sbuf, ctx);
+        __syncthreads();
+    }
+}
+
+__global__ void reset_counter_kernel(int *counter) { *counter = 0; }
+
 }  // namespace
+
+int rg_t1_job_bytes() { return (int)sizeof(T1Job); }
+
+int rg_launch_t1_queue(hipStream_t s, const void *jobs, int njobs, int *counter, int nwaves)
+{
+    if (njobs <= 0) return RG_SUCCESS;
+    reset_counter_kernel<<<1, 1, 0, s>>>(counter);
+    RG_LAUNCH_CHECK();
+    t1_encode_queue_kernel<<<dim3((unsigned)nwaves), dim3(64), 0, s>>>((const T1Job *)jobs, njobs, counter);
+    RG_LAUNCH_CHECK();
+    return RG_SUCCESS;
+}
+
+// Host-side fill of one job record (the struct layout stays private to this file).
+void rg_t1_job_fill(void *dst, const int32_t *src, int pitch, int w, int h, int orient, uint8_t *out, int cap,
+                    rg_cb_result *res)
+{
+    T1Job j;
+    j.src = src; j.pitch = pitch; j.w = w; j.h = h; j.orient = orient; j.cap = cap; j.out = out; j.res = res;
+    *reinterpret_cast<T1Job *>(dst) = j;
+}
 
 int rg_launch_t1_encode(hipStream_t s, const int32_t *q, int pitch, size_t plane_stride,
                         const rg_cb_desc *desc, int ncb, uint8_t *out, int slot_bytes, rg_cb_result *res)

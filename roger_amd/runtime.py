@@ -187,6 +187,73 @@ def t1_encode(ctx, d_q, pitch, plane_stride, descs, slot_bytes):
     return res, data
 
 
+class Batch:
+    """Batched encode of many images (rg_batch): one launch per stage for the
+    whole batch, optionally replayed from a captured HIP graph."""
+
+    def __init__(self, ctx):
+        from ._lib import BatchImage
+        self._BatchImage = BatchImage
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        check(lib().rg_batch_create(ctx.handle, ctypes.byref(h)))
+        self.handle = h
+
+    def plan(self, images, cbw=64, cbh=64, use_graph=True, waves_per_cu=12):
+        """images: list of dicts {buf: DeviceBuffer, w, h, ncomp, reversible, levels, delta}."""
+        arr = (self._BatchImage * len(images))()
+        for a, m in zip(arr, images):
+            a.d_src = m["buf"].ptr
+            a.w, a.h, a.ncomp = m["w"], m["h"], m["ncomp"]
+            rev = m["reversible"]
+            a.filter = RG_FILTER_53 if rev else RG_FILTER_97
+            a.colour = RG_COLOUR_NONE if m["ncomp"] != 3 else (RG_COLOUR_RCT if rev else RG_COLOUR_ICT)
+            a.levels, a.delta = m["levels"], float(m.get("delta", 1.0 / 64))
+        self.images = images
+        check(lib().rg_batch_plan(self.handle, ctypes.cast(arr, ctypes.c_void_p), len(images), cbw, cbh,
+                                  int(use_graph), waves_per_cu))
+
+    def run(self):
+        check(lib().rg_batch_run(self.handle))
+
+    def num_codeblocks(self, image=-1):
+        return check(lib().rg_batch_num_codeblocks(self.handle, image))
+
+    def coefficients(self, image):
+        m = self.images[image]
+        dq = ctypes.c_void_p()
+        check(lib().rg_batch_coefficients(self.handle, image, ctypes.byref(dq)))
+        out = np.empty((m["ncomp"], m["h"], m["w"]), np.int32)
+        self.ctx.sync()
+        check(lib().rg_memcpy_d2h(self.ctx.handle, out.ctypes.data_as(ctypes.c_void_p), dq, out.nbytes))
+        return out
+
+    def codeblocks(self, image):
+        n = self.num_codeblocks(image)
+        desc = np.empty(n, CB_DESC_DTYPE)
+        res = np.empty(n, CB_RES_DTYPE)
+        offs = np.empty(n, np.int64)
+        total = ctypes.c_long()
+        self.ctx.sync()
+        check(lib().rg_batch_get_codeblocks(self.handle, image, desc.ctypes.data_as(ctypes.c_void_p),
+                                            res.ctypes.data_as(ctypes.c_void_p), None, None, 0, ctypes.byref(total)))
+        data = np.empty(max(total.value, 1), np.uint8)
+        check(lib().rg_batch_get_codeblocks(self.handle, image, None, None, offs.ctypes.data_as(ctypes.c_void_p),
+                                            data.ctypes.data_as(ctypes.c_void_p), data.size, None))
+        return desc, res, [bytes(data[offs[i]:offs[i] + res["len"][i]]) for i in range(n)]
+
+    def close(self):
+        if self.handle:
+            lib().rg_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Encoder:
     """The native encode path (rg_encoder): colour -> DWT -> quantisation -> tier-1."""
 

@@ -79,3 +79,48 @@ def rgb(w, h, seed=1):
 
 def image(w, h, ncomp, seed):
     return gray(w, h, seed) if ncomp == 1 else rgb(w, h, seed)
+
+
+def batch_specs(n=256, seed=7, lo=256, hi=4096, levels=5):
+    """Mixed-size batch: sides independently log-uniform on [lo, hi]; image i is
+    grayscale when i is even, RGB when odd; 5/3 lossless when i//2 is even,
+    9/7 otherwise; image seed 1000 + i."""
+    u = _u01(seed, 5, np.arange(2 * n))
+    sides = np.floor(lo * np.exp(u * np.log(hi / lo)) + 0.5).astype(int)
+    return [dict(w=int(sides[2 * i]), h=int(sides[2 * i + 1]), ncomp=1 if i % 2 == 0 else 3,
+                 reversible=(i // 2) % 2 == 0, levels=levels, delta=1.0 / 64, seed=1000 + i) for i in range(n)]
+
+
+# ---- the same recipe generated on the device (rg_synth_u8) ----
+def channel_params(seed):
+    """16 doubles: 4 x (cos theta, sin theta, period, phase), as base_field uses them."""
+    r = _u01(seed, 1, np.arange(12))
+    out = []
+    for k in range(4):
+        period = 8.0 * np.exp(r[3 * k] * np.log(128.0 / 8.0))
+        theta = np.pi * r[3 * k + 1]
+        phi = 2.0 * np.pi * r[3 * k + 2]
+        out += [np.cos(theta), np.sin(theta), period, phi]
+    return np.array(out, np.float64)
+
+
+def noise_base(seed):
+    return np.uint64((int(seed) * 0x100000001B3 + 2 * 0x9E37) & 0xFFFFFFFFFFFFFFFF)
+
+
+def device_image(ctx, w, h, ncomp, seed):
+    """Generates image(w, h, ncomp, seed) directly in device memory."""
+    import ctypes
+    from ._lib import check, lib
+    seeds = [seed] if ncomp == 1 else [seed + c for c in range(3)]
+    par = np.concatenate([channel_params(s) for s in seeds])
+    nb = np.array([noise_base(s) for s in seeds], np.uint64)
+    buf = ctx.alloc(w * h * ncomp)
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    if ncomp == 3:
+        wx = np.ascontiguousarray(_walk(w, seed + 3, 3))
+        wy = np.ascontiguousarray(_walk(h, seed + 3, 4))
+        check(lib().rg_synth_u8(ctx.handle, ctypes.c_void_p(buf.ptr), w, h, 3, vp(par), vp(nb), vp(wx), vp(wy)))
+    else:
+        check(lib().rg_synth_u8(ctx.handle, ctypes.c_void_p(buf.ptr), w, h, 1, vp(par), vp(nb), None, None))
+    return buf
