@@ -194,6 +194,10 @@ int rg_encoder_num_codeblocks(rg_encoder *enc);
  * concatenated codewords (cap bytes), offsets[i] their start.  Waits. */
 int rg_encoder_get_codeblocks(rg_encoder *enc, rg_cb_desc *desc, rg_cb_result *res, long *offsets,
                               uint8_t *data, long cap, long *total);
+/* Tier-2: the JPEG 2000 codestream (.j2k: one tile, one layer, LRCP) of the
+ * last encode.  out == NULL only reports the size.  The reference has no
+ * tier-2 (it stops after its bit-plane coder, src/OCLEncoder.cpp:43-52). */
+int rg_encoder_write_j2k(rg_encoder *enc, uint8_t *out, long cap, long *size);
 /* mapDWTOut / unmapDWTOut (src/OCLEncodeDecode.cpp:42-48): host view of the
  * coefficients, interleaved by component — int16 saturated quantisation
  * indices / 5/3 coefficients, or float when output_dwt on the lossy path. */
@@ -218,6 +222,22 @@ int rg_decoder_idwt(rg_decoder *dec, int levels);
 int rg_decoder_map_out(rg_decoder *dec, void **host_ptr);
 int rg_decoder_unmap_out(rg_decoder *dec, void *host_ptr);
 
+/* EBCOT tier-1 decode of ncb code-blocks (device memory): codeword i at
+ * d_data + d_offsets[i] with results d_res[i] (len, npasses, nbp); signed
+ * coefficients written into the Mallat planes like rg_t1_encode reads them.
+ * No counterpart in the reference (its bit-plane coder has no output and
+ * its decoder never decodes bits, src/OCLDecoder.cpp:35-38). */
+int rg_t1_decode(rg_context *ctx, const uint8_t *d_data, const long *d_offsets, const rg_cb_result *d_res,
+                 const rg_cb_desc *d_desc, int ncb, int32_t *d_q, int pitch, size_t plane_stride);
+/* Full native decode of one image from its code-blocks (host arrays, the
+ * layout rg_encoder_get_codeblocks returns): tier-1 decode -> dequantisation
+ * + inverse DWT -> inverse RCT/ICT -> interleaved uint8 on the device. */
+int rg_decoder_decode_codeblocks(rg_decoder *dec, const rg_cb_desc *desc, const rg_cb_result *res,
+                                 const long *offsets, const uint8_t *data, long total, int ncb, int w, int h,
+                                 int ncomp, const rg_encode_params *p);
+int rg_decoder_image_u8(rg_decoder *dec, uint8_t **d_img);
+int rg_decoder_coefficients(rg_decoder *dec, int32_t **d_q);
+
 /* ---- batched encode of many images (no counterpart in the reference, which
  *      encodes one image per OCLEncoder::run call) ---- */
 typedef struct rg_batch rg_batch;
@@ -240,6 +260,7 @@ int rg_batch_num_codeblocks(rg_batch *b, int image);
 int rg_batch_coefficients(rg_batch *b, int image, int32_t **d_q);
 int rg_batch_get_codeblocks(rg_batch *b, int image, rg_cb_desc *desc, rg_cb_result *res, long *offsets,
                             uint8_t *data, long cap, long *total);
+int rg_batch_write_j2k(rg_batch *b, int image, uint8_t *out, long cap, long *size);
 
 /* ---- synthetic test images generated on the device (the seeded recipe of
  *      roger_amd/synth.py: params are 16 doubles per channel = 4 x (cos theta,

@@ -97,6 +97,7 @@ SIGNATURES = {
     "rg_encoder_float_coefficients": (I, [P, P]),
     "rg_encoder_num_codeblocks": (I, [P]),
     "rg_encoder_get_codeblocks": (I, [P, P, P, P, P, L, P]),
+    "rg_encoder_write_j2k": (I, [P, P, L, P]),
     "rg_encoder_map_dwt_out": (I, [P, P]),
     "rg_encoder_unmap_dwt_out": (I, [P, P]),
     "rg_decoder_create": (I, [P, P]),
@@ -105,6 +106,10 @@ SIGNATURES = {
     "rg_decoder_load_planar": (I, [P, P, I, I, I, I, I, I]),
     "rg_decoder_idwt": (I, [P, I]),
     "rg_decoder_map_out": (I, [P, P]),
+    "rg_t1_decode": (I, [P, P, P, P, P, I, P, I, SZ]),
+    "rg_decoder_decode_codeblocks": (I, [P, P, P, P, P, L, I, I, I, I, P]),
+    "rg_decoder_image_u8": (I, [P, P]),
+    "rg_decoder_coefficients": (I, [P, P]),
     "rg_batch_create": (I, [P, P]),
     "rg_batch_destroy": (I, [P]),
     "rg_batch_plan": (I, [P, P, I, I, I, I, I]),
@@ -112,6 +117,7 @@ SIGNATURES = {
     "rg_batch_num_codeblocks": (I, [P, I]),
     "rg_batch_coefficients": (I, [P, I, P]),
     "rg_batch_get_codeblocks": (I, [P, I, P, P, P, P, L, P]),
+    "rg_batch_write_j2k": (I, [P, I, P, L, P]),
     "rg_synth_u8": (I, [P, P, I, I, I, P, P, P, P]),
     "rg_decoder_unmap_out": (I, [P, P]),
 }

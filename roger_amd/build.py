@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libroger_amd.so")
-SOURCES = ["capi.hip", "colour.hip", "dwt.hip", "t1_encode.hip", "batch.hip"]
+SOURCES = ["capi.hip", "colour.hip", "dwt.hip", "t1_encode.hip", "t1_decode.hip", "batch.hip", "t2_write.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -16,6 +16,7 @@
 #include <numeric>
 #include <vector>
 #include "rg_internal.h"
+#include "rg_t2.h"
 
 namespace {
 struct Buf {
@@ -322,6 +323,36 @@ extern "C" int rg_batch_get_codeblocks(rg_batch *b, int image, rg_cb_desc *desc,
         if (offsets) offsets[i] = o;
         if (data) memcpy(data + o, all.data() + (size_t)i * b->slot, b->h_res[first + i].len);
         o += b->h_res[first + i].len;
+    }
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_batch_write_j2k(rg_batch *b, int image, uint8_t *out, long cap, long *size)
+{
+    RG_CHECK(b && b->planned, "batch not planned");
+    RG_CHECK(image >= 0 && image < (int)b->img.size(), "image index out of range");
+    long total = 0;
+    int rc = rg_batch_get_codeblocks(b, image, nullptr, nullptr, nullptr, nullptr, 0, &total);
+    if (rc) return rc;
+    const int cnt = b->cb_count[image];
+    std::vector<rg_cb_result> res(cnt);
+    std::vector<long> offs(cnt);
+    std::vector<uint8_t> data(total > 0 ? total : 1);
+    if ((rc = rg_batch_get_codeblocks(b, image, nullptr, res.data(), offs.data(), data.data(), (long)data.size(),
+                                      nullptr)))
+        return rc;
+    std::vector<const uint8_t *> ptr(cnt);
+    for (int i = 0; i < cnt; ++i) ptr[i] = data.data() + offs[i];
+    const rg_batch_image &m = b->img[image];
+    rg_t2_image im{m.w, m.h, m.ncomp, m.levels, b->cbw, b->cbh, m.filter == RG_FILTER_53,
+                   m.ncomp == 3 && m.colour != RG_COLOUR_NONE, 2, 8, m.delta};
+    std::vector<uint8_t> cs;
+    if ((rc = rg_j2k_assemble(im, b->desc.data() + b->cb_first[image], res.data(), ptr.data(), cnt / m.ncomp, cs)))
+        return rc;
+    if (size) *size = (long)cs.size();
+    if (out) {
+        RG_CHECK(cap >= (long)cs.size(), "output buffer too small (%ld < %zu)", cap, cs.size());
+        memcpy(out, cs.data(), cs.size());
     }
     return RG_SUCCESS;
 }

@@ -10,6 +10,7 @@
 #include <vector>
 #include <algorithm>
 #include "rg_internal.h"
+#include "rg_t2.h"
 
 // ------------------------------------------------------------ errors
 static thread_local char g_err[512] = "";
@@ -403,7 +404,8 @@ struct rg_encoder {
     rg_context *ctx = nullptr;
     int w = 0, h = 0, ncomp = 0, levels = 0, cbw = 0, cbh = 0, filter = 0;
     int lossy = 0, output_dwt = 0, ran = 0, t1_ran = 0, loaded = 0, i16_ran = 0, bitdepth = 8;
-    int ncb_comp = 0, ncb = 0, slot_bytes = 0;
+    int ncb_comp = 0, ncb = 0, slot_bytes = 0, colour = 0;
+    double delta = 1.0 / 64;
     float step[97], inv[97];
     DevBuf input, planes, q, fcoeff, scratch, desc, res, slots, chan16;
     HostBuf staging, map;
@@ -493,6 +495,7 @@ extern "C" int rg_encoder_encode_u8(rg_encoder *e, const uint8_t *src, int on_de
     const size_t npix = (size_t)w * h;
     e->w = w; e->h = h; e->ncomp = ncomp; e->levels = p->levels; e->cbw = p->cbw; e->cbh = p->cbh;
     e->filter = p->filter; e->lossy = p->filter == RG_FILTER_97; e->output_dwt = 0; e->bitdepth = p->bitdepth;
+    e->colour = p->colour; e->delta = p->delta;
     e->ran = 0; e->t1_ran = 0; e->loaded = 1; e->i16_ran = 0;
     band_steps(p->levels, p->bitdepth, p->delta, e->step, e->inv);
     const uint8_t *d_src = src;
@@ -553,6 +556,7 @@ extern "C" int rg_encoder_load_planar(rg_encoder *e, const void *const *comps, i
     RG_HIP(hipMemcpyAsync(e->planes.p, e->staging.p, npix * ncomp * 4, hipMemcpyHostToDevice, e->ctx->stream));
     e->w = w; e->h = h; e->ncomp = ncomp; e->lossy = lossy; e->bitdepth = precision > 0 ? precision : 8;
     e->filter = lossy ? RG_FILTER_97 : RG_FILTER_53;
+    e->colour = RG_COLOUR_NONE;
     e->loaded = 1; e->ran = 0; e->t1_ran = 0; e->i16_ran = 0;
     return RG_SUCCESS;
 }
@@ -571,6 +575,7 @@ extern "C" int rg_encoder_dwt(rg_encoder *e, int levels, int output_dwt, double 
     e->output_dwt = e->lossy && output_dwt;
     e->t1_ran = 0; e->i16_ran = 0;
     band_steps(levels, e->bitdepth, delta, e->step, e->inv);
+    e->delta = delta;
     if (e->output_dwt) {
         if ((rc = e->fcoeff.ensure(npix * e->ncomp * 4))) return rc;
         rc = dwt_fwd_impl(e->ctx, RG_FILTER_97, e->planes.p, e->w, e->h, e->ncomp, levels, nullptr, 0, e->fcoeff.p, 1,
@@ -689,6 +694,32 @@ extern "C" int rg_encoder_get_codeblocks(rg_encoder *e, rg_cb_desc *desc, rg_cb_
     return RG_SUCCESS;
 }
 
+extern "C" int rg_encoder_write_j2k(rg_encoder *e, uint8_t *out, long cap, long *size)
+{
+    RG_CHECK(e, "null encoder");
+    if (!e->t1_ran) return rg_fail(RG_ERR_NOT_READY, "tier-1 has not run");
+    long total = 0;
+    int rc = rg_encoder_get_codeblocks(e, nullptr, nullptr, nullptr, nullptr, 0, &total);
+    if (rc) return rc;
+    std::vector<rg_cb_result> res(e->ncb);
+    std::vector<long> offs(e->ncb);
+    std::vector<uint8_t> data(total > 0 ? total : 1);
+    if ((rc = rg_encoder_get_codeblocks(e, nullptr, res.data(), offs.data(), data.data(), (long)data.size(), nullptr)))
+        return rc;
+    std::vector<const uint8_t *> ptr(e->ncb);
+    for (int i = 0; i < e->ncb; ++i) ptr[i] = data.data() + offs[i];
+    rg_t2_image im{e->w, e->h, e->ncomp, e->levels, e->cbw, e->cbh, e->filter == RG_FILTER_53,
+                   e->ncomp == 3 && e->colour != RG_COLOUR_NONE, 2, e->bitdepth, e->delta};
+    std::vector<uint8_t> cs;
+    if ((rc = rg_j2k_assemble(im, e->h_desc.data(), res.data(), ptr.data(), e->ncb_comp, cs))) return rc;
+    if (size) *size = (long)cs.size();
+    if (out) {
+        RG_CHECK(cap >= (long)cs.size(), "output buffer too small (%ld < %zu)", cap, cs.size());
+        memcpy(out, cs.data(), cs.size());
+    }
+    return RG_SUCCESS;
+}
+
 extern "C" int rg_encoder_map_dwt_out(rg_encoder *e, void **host)
 {
     RG_CHECK(e && host, "null argument");
@@ -729,10 +760,83 @@ extern "C" int rg_encoder_unmap_dwt_out(rg_encoder *e, void *host)
 // ------------------------------------------------------------ decoder
 struct rg_decoder {
     rg_context *ctx = nullptr;
-    int w = 0, h = 0, ncomp = 0, lossy = 0, ran = 0, loaded = 0;
-    DevBuf coeff, out, scratch;
+    int w = 0, h = 0, ncomp = 0, lossy = 0, ran = 0, loaded = 0, u8_ran = 0;
+    DevBuf coeff, out, scratch, cbdata, cboffs, cbres, cbdesc, u8;
     HostBuf staging, map;
 };
+
+int rg_launch_t1_decode(hipStream_t s, const uint8_t *data, const long *offs, const rg_cb_result *res,
+                        const rg_cb_desc *desc, int ncb, int32_t *q, int pitch, size_t plane_stride);
+
+extern "C" int rg_t1_decode(rg_context *c, const uint8_t *d_data, const long *d_offsets, const rg_cb_result *d_res,
+                            const rg_cb_desc *d_desc, int ncb, int32_t *d_q, int pitch, size_t plane_stride)
+{
+    RG_CHECK(c && d_data && d_offsets && d_res && d_desc && d_q, "rg_t1_decode: null argument");
+    RG_CHECK(ncb >= 0 && pitch > 0, "rg_t1_decode: bad sizes");
+    return rg_launch_t1_decode(c->stream, d_data, d_offsets, d_res, d_desc, ncb, d_q, pitch, plane_stride);
+}
+
+extern "C" int rg_decoder_decode_codeblocks(rg_decoder *d, const rg_cb_desc *desc, const rg_cb_result *res,
+                                            const long *offsets, const uint8_t *data, long total, int ncb, int w,
+                                            int h, int ncomp, const rg_encode_params *p)
+{
+    RG_CHECK(d && desc && res && offsets && (data || total == 0), "rg_decoder_decode_codeblocks: null argument");
+    int rc = check_params(p, w, h, ncomp);
+    if (rc) return rc;
+    const int per = rg_codeblocks(w, h, p->levels, p->cbw, p->cbh, nullptr, 0);
+    RG_CHECK(per * ncomp == ncb, "expected %d code-blocks, got %d", per * ncomp, ncb);
+    for (int i = 0; i < ncb; ++i) {
+        RG_CHECK(desc[i].comp >= 0 && desc[i].comp < ncomp && desc[i].w <= p->cbw && desc[i].h <= p->cbh &&
+                     desc[i].x0 >= 0 && desc[i].y0 >= 0 && desc[i].x0 + desc[i].w <= w && desc[i].y0 + desc[i].h <= h,
+                 "code-block %d lies outside the image", i);
+        RG_CHECK(res[i].len >= 0 && res[i].len <= 12288 && offsets[i] >= 0 && offsets[i] + res[i].len <= total,
+                 "code-block %d: bad codeword extent", i);
+        RG_CHECK(res[i].nbp >= 0 && res[i].nbp <= 31 && res[i].npasses >= 0 && res[i].npasses <= 3 * res[i].nbp,
+                 "code-block %d: bad pass counts", i);
+    }
+    RG_HIP(hipSetDevice(d->ctx->device));
+    const size_t npix = (size_t)w * h;
+    if ((rc = d->cbdata.ensure(total + 16)) || (rc = d->cboffs.ensure(sizeof(long) * ncb)) ||
+        (rc = d->cbres.ensure(sizeof(rg_cb_result) * ncb)) || (rc = d->cbdesc.ensure(sizeof(rg_cb_desc) * ncb)) ||
+        (rc = d->coeff.ensure(npix * ncomp * 4)) || (rc = d->out.ensure(npix * ncomp * 4)) ||
+        (rc = d->scratch.ensure(rg_dwt_scratch_bytes(w, h, ncomp, p->filter))) || (rc = d->u8.ensure(npix * ncomp)))
+        return rc;
+    hipStream_t s = d->ctx->stream;
+    RG_HIP(hipMemcpyAsync(d->cbdata.p, data, total, hipMemcpyHostToDevice, s));
+    RG_HIP(hipMemcpyAsync(d->cboffs.p, offsets, sizeof(long) * ncb, hipMemcpyHostToDevice, s));
+    RG_HIP(hipMemcpyAsync(d->cbres.p, res, sizeof(rg_cb_result) * ncb, hipMemcpyHostToDevice, s));
+    RG_HIP(hipMemcpyAsync(d->cbdesc.p, desc, sizeof(rg_cb_desc) * ncb, hipMemcpyHostToDevice, s));
+    if ((rc = rg_launch_t1_decode(s, (const uint8_t *)d->cbdata.p, (const long *)d->cboffs.p,
+                                  (const rg_cb_result *)d->cbres.p, (const rg_cb_desc *)d->cbdesc.p, ncb,
+                                  (int32_t *)d->coeff.p, w, npix)))
+        return rc;
+    float step[97], inv[97];
+    band_steps(p->levels, p->bitdepth, p->delta, step, inv);
+    if ((rc = rg_dwt_inv(d->ctx, p->filter, d->coeff.p, w, h, ncomp, p->levels,
+                         p->filter == RG_FILTER_97 ? step : nullptr, d->out.p, d->scratch.p)))
+        return rc;
+    if ((rc = rg_launch_colour_inv(s, d->out.p, w, h, ncomp, p->colour, p->filter == RG_FILTER_97, (uint8_t *)d->u8.p)))
+        return rc;
+    RG_HIP(hipStreamSynchronize(s));  // the host arrays above must outlive their copies
+    d->w = w; d->h = h; d->ncomp = ncomp; d->lossy = p->filter == RG_FILTER_97; d->ran = 1; d->u8_ran = 1;
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_decoder_image_u8(rg_decoder *d, uint8_t **d_img)
+{
+    RG_CHECK(d && d_img, "null argument");
+    if (!d->u8_ran) return rg_fail(RG_ERR_NOT_READY, "no decoded image");
+    *d_img = (uint8_t *)d->u8.p;
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_decoder_coefficients(rg_decoder *d, int32_t **d_q)
+{
+    RG_CHECK(d && d_q, "null argument");
+    if (!d->u8_ran) return rg_fail(RG_ERR_NOT_READY, "no decoded code-blocks");
+    *d_q = (int32_t *)d->coeff.p;
+    return RG_SUCCESS;
+}
 
 extern "C" int rg_decoder_create(rg_context *c, rg_decoder **out)
 {
@@ -748,7 +852,8 @@ extern "C" int rg_decoder_destroy(rg_decoder *d)
     if (!d) return RG_SUCCESS;
     hipSetDevice(d->ctx->device);
     hipStreamSynchronize(d->ctx->stream);
-    d->coeff.release(); d->out.release(); d->scratch.release();
+    for (DevBuf *b : {&d->coeff, &d->out, &d->scratch, &d->cbdata, &d->cboffs, &d->cbres, &d->cbdesc, &d->u8})
+        b->release();
     d->staging.release(); d->map.release();
     delete d;
     return RG_SUCCESS;

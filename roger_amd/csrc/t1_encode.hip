@@ -19,10 +19,11 @@
 //   * magnitude refinement: no intra-pass dependency.
 //   * cleanup: newly significant samples are known up front (bit == 1), so
 //     contexts and the run-length test come straight from the windows.
-// Each lane emits its column's (context, bit) symbols in scan order; a wave
-// prefix sum of the per-lane counts gives contiguous offsets into an LDS
-// symbol buffer, which the MQ coder then consumes in order as wave-uniform
-// (scalar) code.  The codeword is written to the block's output slot.
+// Each lane keeps its column's (context, bit) symbols for the stripe in two
+// 64-bit registers; the MQ coder then walks the lanes in scan order (ballot
+// mask, readlane) as wave-uniform scalar code whose context states and
+// probability table sit in VGPR lanes, so the coder touches no memory except
+// the codeword bytes it writes to the block's output slot.
 #include "rg_internal.h"
 
 namespace {
@@ -43,8 +44,7 @@ __constant__ uint32_t MQT[47] = {
     MQE(0x0049, 41, 38, 0), MQE(0x0025, 42, 39, 0), MQE(0x0015, 43, 40, 0), MQE(0x0009, 44, 41, 0),
     MQE(0x0005, 45, 42, 0), MQE(0x0001, 45, 43, 0), MQE(0x5601, 46, 46, 0)};
 
-constexpr int CTX_RL = 17, CTX_UNI = 18, NCTX = 19;
-constexpr int SYMBUF = 64 * 10 + 64;
+constexpr int CTX_RL = 17, CTX_UNI = 18;
 
 // ZC context (Table D.1), same rule as the oracle.
 __device__ __forceinline__ int zc_ctx(int orient, int h, int v, int d)
@@ -90,10 +90,6 @@ struct SymList {
         if (n < 8) lo |= s << (8 * n); else hi |= s << (8 * (n - 8));
         ++n;
     }
-    __device__ __forceinline__ int get(int i) const
-    {
-        return (int)(((i < 8) ? (lo >> (8 * i)) : (hi >> (8 * (i - 8)))) & 0xFF);
-    }
 };
 
 // Sign symbol for window row i with neighbour windows: own column (cur state
@@ -115,16 +111,6 @@ __device__ __forceinline__ int zc_at(int orient, int i, uint32_t cur, uint32_t s
     int v = (int)(b1(cur, i - 1) + b1(sw, i + 1));
     int d = (int)(b1(Lw, i - 1) + b1(Lw, i + 1) + b1(rw, i - 1) + b1(rw, i + 1));
     return zc_ctx(orient, h, v, d);
-}
-
-__device__ __forceinline__ int wave_incl_scan(int v, int lane)
-{
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        int t = __shfl_up(v, o, 64);
-        if (lane >= o) v += t;
-    }
-    return v;
 }
 
 struct MQEnc {
@@ -170,32 +156,56 @@ __device__ __forceinline__ void mq_renorm(MQEnc &m, int lane)
     }
 }
 
-// Encodes the `total` symbols of sbuf in order; ctx holds I << 1 | MPS per context.
-__device__ __forceinline__ void mq_run(MQEnc &m, const uint8_t *sbuf, uint8_t *ctx, int total, int lane)
+// ENCODE (C.2.2-C.2.6) as wave-uniform scalar code.  The 19 context states
+// (I << 1 | MPS) live in lanes 0..18 of the VGPR `ctxv` and the probability
+// table in lanes 0..46 of `mqtv`; readlane with a uniform index (and a lane
+// select to write a state back) replaces every memory access of the coder.
+__device__ __forceinline__ void mq_encode(MQEnc &m, int &ctxv, int mqtv, int cx, int d, int lane)
 {
-    for (int i = 0; i < total; ++i) {
-        int sym = __builtin_amdgcn_readfirstlane((int)sbuf[i]);
-        int cx = sym >> 1, d = sym & 1;
-        int st = __builtin_amdgcn_readfirstlane((int)ctx[cx]);
-        int idx = st >> 1, mps = st & 1;
-        uint32_t t = MQT[idx];
-        uint32_t qe = t & 0xFFFF;
-        m.A -= qe;
-        if (d == mps) {
-            if (m.A & 0x8000) {
-                m.C += qe;
-            } else {
-                if (m.A < qe) m.A = qe; else m.C += qe;
-                if (lane == 0) ctx[cx] = (uint8_t)((((t >> 16) & 63) << 1) | mps);
-                mq_renorm(m, lane);
-            }
-        } else {
-            if (m.A < qe) m.C += qe; else m.A = qe;
-            mps ^= (int)(t >> 28);
-            if (lane == 0) ctx[cx] = (uint8_t)((((t >> 22) & 63) << 1) | mps);
-            mq_renorm(m, lane);
+    const int st = __builtin_amdgcn_readlane(ctxv, cx);
+    const int mps = st & 1;
+    const uint32_t t = (uint32_t)__builtin_amdgcn_readlane(mqtv, st >> 1);
+    const uint32_t qe = t & 0xFFFF;
+    m.A -= qe;
+    if (d == mps) {
+        if (m.A & 0x8000) {
+            m.C += qe;
+            return;
         }
-        __builtin_amdgcn_wave_barrier();
+        if (m.A < qe) m.A = qe; else m.C += qe;
+        ctxv = lane == cx ? (int)((((t >> 16) & 63) << 1) | mps) : ctxv;
+    } else {
+        if (m.A < qe) m.C += qe; else m.A = qe;
+        const int nm = mps ^ (int)(t >> 28);
+        ctxv = lane == cx ? (int)((((t >> 22) & 63) << 1) | nm) : ctxv;
+    }
+    mq_renorm(m, lane);
+}
+
+// Codes one stripe's symbols in scan order: columns (lanes) left to right,
+// each lane's own list in order.  Lanes without symbols are skipped via the
+// ballot mask; a lane's list (<= 16 symbols) is read with four readlanes.
+__device__ __forceinline__ void mq_stripe(MQEnc &m, int &ctxv, int mqtv, const SymList &sl, int lane)
+{
+    uint64_t mask = __ballot(sl.n > 0);
+    const int lo0 = (int)(uint32_t)sl.lo, lo1 = (int)(uint32_t)(sl.lo >> 32);
+    const int hi0 = (int)(uint32_t)sl.hi, hi1 = (int)(uint32_t)(sl.hi >> 32);
+    while (mask) {
+        const int j = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        const int n = __builtin_amdgcn_readlane(sl.n, j);
+        uint64_t a = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(lo1, j) << 32) |
+                     (uint32_t)__builtin_amdgcn_readlane(lo0, j);
+        uint64_t b = 0;
+        if (n > 8)
+            b = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(hi1, j) << 32) |
+                (uint32_t)__builtin_amdgcn_readlane(hi0, j);
+        for (int t = 0; t < n; ++t) {
+            const int sym = (int)(a & 0xFF);
+            a = (a >> 8) | (b << 56);
+            b >>= 8;
+            mq_encode(m, ctxv, mqtv, sym >> 1, sym & 1, lane);
+        }
     }
 }
 
@@ -208,7 +218,7 @@ struct T1Job {
 };
 
 // Encodes one code-block with the calling wavefront (64 lanes).
-__device__ void t1_block(const T1Job &job, uint8_t *sbuf, uint8_t *ctx)
+__device__ void t1_block(const T1Job &job)
 {
     const int lane = threadIdx.x;
     const int w = job.w, h = job.h, orient = job.orient, pitch = job.pitch;
@@ -235,8 +245,9 @@ __device__ void t1_block(const T1Job &job, uint8_t *sbuf, uint8_t *ctx)
         return;
     }
 
-    if (lane < NCTX) ctx[lane] = (uint8_t)(lane == 0 ? (4 << 1) : lane == CTX_RL ? (3 << 1) : lane == CTX_UNI ? (46 << 1) : 0);
-    __syncthreads();
+    // initial context states (Table D.7): ZC ctx 0 -> state 4, RL -> 3, UNIFORM -> 46, others 0
+    int ctxv = lane == 0 ? (4 << 1) : lane == CTX_RL ? (3 << 1) : lane == CTX_UNI ? (46 << 1) : 0;
+    const int mqtv = lane < 47 ? (int)MQT[lane] : 0;
 
     const uint64_t rowmask = h >= 64 ? ~0ull : ((1ull << h) - 1);
     const uint64_t NEGL = __shfl_up(NEG, 1, 64);
@@ -371,15 +382,8 @@ __device__ void t1_block(const T1Job &job, uint8_t *sbuf, uint8_t *ctx)
                     S |= (uint64_t)N << r0;
                 }
 
-                // ---- compact this stripe's symbols (scan order) and MQ-code them ----
-                const int incl = wave_incl_scan(sl.n, lane);
-                const int total = __shfl(incl, 63, 64);
-                if (total == 0) continue;
-                const int off = incl - sl.n;
-                for (int t = 0; t < sl.n; ++t) sbuf[off + t] = (uint8_t)sl.get(t);
-                __syncthreads();
-                mq_run(m, sbuf, ctx, total, lane);
-                __syncthreads();
+                // ---- MQ-code this stripe's symbols in scan order ----
+                mq_stripe(m, ctxv, mqtv, sl, lane);
             }
             if (pass == 2) V = 0;
         }
@@ -405,15 +409,13 @@ __global__ void __launch_bounds__(64) t1_encode_kernel(const int32_t *__restrict
                                                        const rg_cb_desc *__restrict__ desc, uint8_t *__restrict__ out,
                                                        int slot_bytes, rg_cb_result *__restrict__ res)
 {
-    __shared__ uint8_t sbuf[SYMBUF];
-    __shared__ uint8_t ctx[32];
     const rg_cb_desc d = desc[blockIdx.x];
     T1Job job;
     job.src = q + (size_t)d.comp * plane_stride + (size_t)d.y0 * pitch + d.x0;
     job.pitch = pitch; job.w = d.w; job.h = d.h; job.orient = d.orient; job.cap = slot_bytes;
     job.out = out + (size_t)blockIdx.x * slot_bytes;
     job.res = res + blockIdx.x;
-    t1_block(job, sbuf, ctx);
+    t1_block(job);
 }
 
 // Batched path: a persistent grid of wavefronts pulls job indices from a
@@ -421,16 +423,13 @@ __global__ void __launch_bounds__(64) t1_encode_kernel(const int32_t *__restrict
 __global__ void __launch_bounds__(64) t1_encode_queue_kernel(const T1Job *__restrict__ jobs, int njobs,
                                                              int *__restrict__ counter)
 {
-    __shared__ uint8_t sbuf[SYMBUF];
-    __shared__ uint8_t ctx[32];
     for (;;) {
         int i = 0;
         if (threadIdx.x == 0) i = atomicAdd(counter, 1);
         i = __shfl(i, 0, 64);
         if (i >= njobs) break;
         const T1Job job = jobs[i];
-        t1_block(job, sbuf, ctx);
-        __syncthreads();
+        t1_block(job);
     }
 }
 

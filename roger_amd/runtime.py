@@ -242,9 +242,88 @@ class Batch:
                                             data.ctypes.data_as(ctypes.c_void_p), data.size, None))
         return desc, res, [bytes(data[offs[i]:offs[i] + res["len"][i]]) for i in range(n)]
 
+    def j2k(self, image):
+        size = ctypes.c_long()
+        self.ctx.sync()
+        check(lib().rg_batch_write_j2k(self.handle, image, None, 0, ctypes.byref(size)))
+        out = np.empty(size.value, np.uint8)
+        check(lib().rg_batch_write_j2k(self.handle, image, out.ctypes.data_as(ctypes.c_void_p), out.size, None))
+        return out.tobytes()
+
     def close(self):
         if self.handle:
             lib().rg_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def t1_decode(ctx, descs, results, blobs, q_shape):
+    """Tier-1 decodes code-blocks into zeroed int32 planes of shape (ncomp, h, w)."""
+    descs = np.ascontiguousarray(descs).view(CB_DESC_DTYPE)
+    res = np.ascontiguousarray(results).view(CB_RES_DTYPE)
+    n = len(descs)
+    offs = np.zeros(n, np.int64)
+    pos = 0
+    for i, b in enumerate(blobs):
+        offs[i] = pos
+        pos += len(b)
+    data = np.frombuffer(b"".join(blobs) + b"\0" * 16, np.uint8)
+    d_data, d_offs, d_res, d_desc = ctx.upload(data), ctx.upload(offs), ctx.upload(res), ctx.upload(descs)
+    nc, h, w = q_shape
+    d_q = ctx.alloc(nc * h * w * 4)
+    d_q.zero()
+    check(lib().rg_t1_decode(ctx.handle, _vp(d_data.ptr), _vp(d_offs.ptr), _vp(d_res.ptr), _vp(d_desc.ptr), n,
+                             _vp(d_q.ptr), w, h * w))
+    return d_q.read(np.int32, q_shape)
+
+
+class Decoder:
+    """The native decode path (rg_decoder): tier-1 decode -> dequantisation +
+    inverse DWT -> inverse colour -> uint8."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        check(lib().rg_decoder_create(ctx.handle, ctypes.byref(h)))
+        self.handle = h
+
+    def decode(self, descs, results, blobs, w, h, ncomp, levels=5, reversible=True, cbw=64, cbh=64,
+               delta=1.0 / 64):
+        descs = np.ascontiguousarray(descs).view(CB_DESC_DTYPE)
+        res = np.ascontiguousarray(results).view(CB_RES_DTYPE)
+        offs = np.zeros(len(blobs), np.int64)
+        pos = 0
+        for i, b in enumerate(blobs):
+            offs[i] = pos
+            pos += len(b)
+        data = np.frombuffer(b"".join(blobs) + b"\0", np.uint8)
+        p = EncodeParams()
+        p.levels = levels
+        p.filter = RG_FILTER_53 if reversible else RG_FILTER_97
+        p.colour = RG_COLOUR_NONE if ncomp != 3 else (RG_COLOUR_RCT if reversible else RG_COLOUR_ICT)
+        p.cbw, p.cbh, p.delta, p.ll_fp16, p.bitdepth = cbw, cbh, float(delta), 0, 8
+        check(lib().rg_decoder_decode_codeblocks(
+            self.handle, descs.ctypes.data_as(ctypes.c_void_p), res.ctypes.data_as(ctypes.c_void_p),
+            offs.ctypes.data_as(ctypes.c_void_p), data.ctypes.data_as(ctypes.c_void_p), pos, len(blobs), w, h,
+            ncomp, ctypes.byref(p)))
+        self.shape = (h, w) if ncomp == 1 else (h, w, ncomp)
+
+    def image(self):
+        dp = ctypes.c_void_p()
+        check(lib().rg_decoder_image_u8(self.handle, ctypes.byref(dp)))
+        out = np.empty(self.shape, np.uint8)
+        self.ctx.sync()
+        check(lib().rg_memcpy_d2h(self.ctx.handle, out.ctypes.data_as(ctypes.c_void_p), dp, out.nbytes))
+        return out
+
+    def close(self):
+        if self.handle:
+            lib().rg_decoder_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -304,6 +383,15 @@ class Encoder:
                                               data.ctypes.data_as(ctypes.c_void_p), data.size, None))
         blobs = [bytes(data[offs[i]:offs[i] + res["len"][i]]) for i in range(n)]
         return desc, res, blobs
+
+    def j2k(self):
+        """The JPEG 2000 codestream of the last encode (bytes)."""
+        size = ctypes.c_long()
+        self.ctx.sync()
+        check(lib().rg_encoder_write_j2k(self.handle, None, 0, ctypes.byref(size)))
+        out = np.empty(size.value, np.uint8)
+        check(lib().rg_encoder_write_j2k(self.handle, out.ctypes.data_as(ctypes.c_void_p), out.size, None))
+        return out.tobytes()
 
     def close(self):
         if self.handle:

@@ -93,6 +93,8 @@ def test_encoder_vs_oracle_and_openjpeg(ctx, w, h, nc, rev, levels, cb):
     params = O.j2k_params(w, h, nc, levels, rev, cb, cb, 1.0 / 64)
     gres = np.stack([res["len"], res["npasses"], res["nbp"], res["flags"]], axis=1)
     cs = O.t2_write(params, gres, blobs)
+    # the library's own tier-2 writer produces the same codestream
+    assert enc.j2k() == cs
     dec = np.array(Image.open(io.BytesIO(cs)))
     if rev:
         np.testing.assert_array_equal(dec, img)
