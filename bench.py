@@ -78,6 +78,33 @@ def bench_batched(ctx, args):
             "speedup": round(t_plain / t_batch, 3)}
 
 
+def bench_serving(ctx, args):
+    """Serving loop: Poisson arrivals of mixed encode/decode requests (sides
+    log-uniform 128..8192, 70% 9/7 / 30% 5/3) at ~70% of the plain path's
+    measured capacity; continuous batching (4 slots) vs the plain path (1 slot)."""
+    from roger_amd import serve
+    log = lambda s: print(s, file=sys.stderr, flush=True)
+    reqs = serve.make_requests(ctx, args.serve_requests, seed=42)
+    log("serving: %d requests, %.1f Mpix" % (len(reqs), sum(r.mpix for r in reqs)))
+    out = {}
+    plain = serve.Server(slots=1, hbm_budget=96 << 30)
+    plain.warm(reqs)
+    for r in reqs:
+        r.arrival = 0.0
+    cap = plain.run(reqs)
+    rate = 0.7 * len(reqs) / cap["span_s"]
+    serve.poisson_arrivals(reqs, rate, seed=42)
+    out["plain"] = plain.run(reqs)
+    plain.close()
+    cb = serve.Server(slots=4, hbm_budget=96 << 30)
+    cb.warm(reqs)
+    out["continuous"] = cb.run(reqs)
+    cb.close()
+    out["arrival_rate_per_s"] = round(rate, 2)
+    out["plain_capacity_req_s"] = round(len(reqs) / cap["span_s"], 2)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +114,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain measurement")
     ap.add_argument("--batch-images", type=int, default=256)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--serve-requests", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (single-GPU hot path; see DESIGN.md "
@@ -172,6 +201,8 @@ def main():
 
     if not args.no_batch:
         out["batched"] = bench_batched(ctx, args)
+    if not args.no_serve:
+        out["serving"] = bench_serving(ctx, args)
 
     if not args.no_cpu:
         from oracle import oracle as O

@@ -82,6 +82,8 @@ int rg_context_create(int device, rg_context **out);
 int rg_context_destroy(rg_context *ctx);
 int rg_context_sync(rg_context *ctx);                 /* OCLEncodeDecode::finish, src/OCLEncodeDecode.cpp:33-36 */
 int rg_context_device_name(rg_context *ctx, char *buf, int cap);
+/* 1 when all work queued on the context's stream has finished, 0 if not. */
+int rg_context_query(rg_context *ctx);
 
 /* ---- device memory plumbing ---- */
 int rg_malloc(rg_context *ctx, size_t bytes, void **d_ptr);

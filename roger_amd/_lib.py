@@ -68,6 +68,7 @@ SIGNATURES = {
     "rg_context_destroy": (I, [P]),
     "rg_context_sync": (I, [P]),
     "rg_context_device_name": (I, [P, ctypes.c_char_p, I]),
+    "rg_context_query": (I, [P]),
     "rg_malloc": (I, [P, SZ, P]),
     "rg_free": (I, [P, P]),
     "rg_memcpy_h2d": (I, [P, P, P, SZ]),

@@ -78,6 +78,15 @@ extern "C" int rg_context_sync(rg_context *c)
     return RG_SUCCESS;
 }
 
+extern "C" int rg_context_query(rg_context *c)
+{
+    RG_CHECK(c, "null context");
+    hipError_t e = hipStreamQuery(c->stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    return rg_fail(RG_ERR_DEVICE, "hipStreamQuery: %s", hipGetErrorString(e));
+}
+
 extern "C" int rg_context_device_name(rg_context *c, char *buf, int cap)
 {
     RG_CHECK(c && buf && cap > 0, "bad arguments");
@@ -802,10 +811,20 @@ extern "C" int rg_decoder_decode_codeblocks(rg_decoder *d, const rg_cb_desc *des
         (rc = d->scratch.ensure(rg_dwt_scratch_bytes(w, h, ncomp, p->filter))) || (rc = d->u8.ensure(npix * ncomp)))
         return rc;
     hipStream_t s = d->ctx->stream;
-    RG_HIP(hipMemcpyAsync(d->cbdata.p, data, total, hipMemcpyHostToDevice, s));
-    RG_HIP(hipMemcpyAsync(d->cboffs.p, offsets, sizeof(long) * ncb, hipMemcpyHostToDevice, s));
-    RG_HIP(hipMemcpyAsync(d->cbres.p, res, sizeof(rg_cb_result) * ncb, hipMemcpyHostToDevice, s));
-    RG_HIP(hipMemcpyAsync(d->cbdesc.p, desc, sizeof(rg_cb_desc) * ncb, hipMemcpyHostToDevice, s));
+    // stage the host arrays in pinned memory so the uploads are truly
+    // asynchronous; the previous call's copies must have drained first
+    RG_HIP(hipStreamSynchronize(s));
+    const size_t o1 = (total + 15) / 16 * 16, o2 = o1 + sizeof(long) * ncb, o3 = o2 + sizeof(rg_cb_result) * ncb;
+    if ((rc = d->staging.ensure(o3 + sizeof(rg_cb_desc) * ncb))) return rc;
+    char *st = (char *)d->staging.p;
+    if (total) memcpy(st, data, total);
+    memcpy(st + o1, offsets, sizeof(long) * ncb);
+    memcpy(st + o2, res, sizeof(rg_cb_result) * ncb);
+    memcpy(st + o3, desc, sizeof(rg_cb_desc) * ncb);
+    RG_HIP(hipMemcpyAsync(d->cbdata.p, st, total, hipMemcpyHostToDevice, s));
+    RG_HIP(hipMemcpyAsync(d->cboffs.p, st + o1, sizeof(long) * ncb, hipMemcpyHostToDevice, s));
+    RG_HIP(hipMemcpyAsync(d->cbres.p, st + o2, sizeof(rg_cb_result) * ncb, hipMemcpyHostToDevice, s));
+    RG_HIP(hipMemcpyAsync(d->cbdesc.p, st + o3, sizeof(rg_cb_desc) * ncb, hipMemcpyHostToDevice, s));
     if ((rc = rg_launch_t1_decode(s, (const uint8_t *)d->cbdata.p, (const long *)d->cboffs.p,
                                   (const rg_cb_result *)d->cbres.p, (const rg_cb_desc *)d->cbdesc.p, ncb,
                                   (int32_t *)d->coeff.p, w, npix)))
@@ -817,7 +836,6 @@ extern "C" int rg_decoder_decode_codeblocks(rg_decoder *d, const rg_cb_desc *des
         return rc;
     if ((rc = rg_launch_colour_inv(s, d->out.p, w, h, ncomp, p->colour, p->filter == RG_FILTER_97, (uint8_t *)d->u8.p)))
         return rc;
-    RG_HIP(hipStreamSynchronize(s));  // the host arrays above must outlive their copies
     d->w = w; d->h = h; d->ncomp = ncomp; d->lossy = p->filter == RG_FILTER_97; d->ran = 1; d->u8_ran = 1;
     return RG_SUCCESS;
 }

@@ -25,6 +25,10 @@ class Context:
     def sync(self):
         check(lib().rg_context_sync(self.handle))
 
+    def query(self):
+        """True when all work queued on this context's stream has finished."""
+        return check(lib().rg_context_query(self.handle)) == 1
+
     def device_name(self):
         buf = ctypes.create_string_buffer(256)
         check(lib().rg_context_device_name(self.handle, buf, 256))
@@ -262,6 +266,16 @@ class Batch:
             pass
 
 
+def pack_codewords(blobs):
+    """List of codewords -> (concatenated uint8 array, int64 offsets)."""
+    offs = np.zeros(len(blobs), np.int64)
+    pos = 0
+    for i, b in enumerate(blobs):
+        offs[i] = pos
+        pos += len(b)
+    return np.frombuffer(b"".join(blobs) + b"\0", np.uint8), offs
+
+
 def t1_decode(ctx, descs, results, blobs, q_shape):
     """Tier-1 decodes code-blocks into zeroed int32 planes of shape (ncomp, h, w)."""
     descs = np.ascontiguousarray(descs).view(CB_DESC_DTYPE)
@@ -296,12 +310,8 @@ class Decoder:
                delta=1.0 / 64):
         descs = np.ascontiguousarray(descs).view(CB_DESC_DTYPE)
         res = np.ascontiguousarray(results).view(CB_RES_DTYPE)
-        offs = np.zeros(len(blobs), np.int64)
-        pos = 0
-        for i, b in enumerate(blobs):
-            offs[i] = pos
-            pos += len(b)
-        data = np.frombuffer(b"".join(blobs) + b"\0", np.uint8)
+        data, offs = pack_codewords(blobs) if isinstance(blobs, list) else blobs
+        pos = int(offs[-1] + res["len"][-1]) if len(offs) else 0
         p = EncodeParams()
         p.levels = levels
         p.filter = RG_FILTER_53 if reversible else RG_FILTER_97
