@@ -239,7 +239,9 @@ __device__ void t1_block(const T1Job &job)
     }
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) orm |= __shfl_xor(orm, o, 64);
-    const int nbp = orm ? 32 - __clz(orm) : 0;
+    // the butterfly leaves the same value in every lane; readfirstlane tells the
+    // compiler so, which keeps the whole bit-plane loop (and the MQ coder) scalar
+    const int nbp = __builtin_amdgcn_readfirstlane(orm ? 32 - __clz(orm) : 0);
     if (nbp == 0) {
         if (lane == 0) *job.res = rg_cb_result{0, 0, 0, 0};
         return;
@@ -285,6 +287,8 @@ __device__ void t1_block(const T1Job &job)
 
                 if (pass == 0) {
                     // ---- significance propagation ----
+                    // no significant sample in any column's 6-row window: nobody joins the pass
+                    if (!__any((sw & 0x3Fu) != 0)) continue;
                     uint32_t N = 0, P = 0, Lw = lw;
                     bool again = true;
                     // monotone fix-point: converges within 65 rounds (chain across <= 64 lanes)
@@ -326,6 +330,7 @@ __device__ void t1_block(const T1Job &job)
                     const uint32_t vw = (uint32_t)(V >> r0) & 0xFu;
                     const uint32_t rf = (uint32_t)(R >> r0) & 0xFu;
                     const uint32_t M = (sw >> 1) & ~vw & vrow;
+                    if (!__any(M != 0)) continue;
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
                         const int i = k + 1;
@@ -426,7 +431,7 @@ __global__ void __launch_bounds__(64) t1_encode_queue_kernel(const T1Job *__rest
     for (;;) {
         int i = 0;
         if (threadIdx.x == 0) i = atomicAdd(counter, 1);
-        i = __shfl(i, 0, 64);
+        i = __builtin_amdgcn_readfirstlane(i);   // uniform: job fields load as scalars
         if (i >= njobs) break;
         const T1Job job = jobs[i];
         t1_block(job);

@@ -62,8 +62,11 @@ def test_precision_study_8192(ctx):
     p64 = _psnr(rec64, img)
     report["psnr_fp64"] = p64
     for mode in ("fp32", "fp16"):
-        report["psnr_" + mode] = _psnr(recon[mode], img)
-        report["psnr_delta_" + mode] = report["psnr_" + mode] - p64
+        p = _psnr(recon[mode], img)
+        report["psnr_" + mode] = p
+        # both lossless (infinite PSNR) counts as no difference
+        report["psnr_delta_" + mode] = 0.0 if (np.isinf(p) and np.isinf(p64)) else p - p64
+        report["pixels_differing_from_fp64_" + mode] = int((recon[mode] != rec64).sum())
     # inter-level LL traffic: written once and read once per level below the top
     ll_px = sum(((W + (1 << l) - 1) >> l) * ((H + (1 << l) - 1) >> l) for l in range(1, L))
     report["interlevel_bytes_fp32"] = 8 * ll_px
