@@ -177,6 +177,13 @@ def main():
     t_t1 = timed(lambda: lib().rg_t1_encode(ctx.handle, ctypes.c_void_p(d_q.ptr), W, W * H,
                                               ctypes.c_void_p(d_desc.ptr), len(desc), ctypes.c_void_p(d_out.ptr),
                                               slot, ctypes.c_void_p(d_res.ptr)))
+    # measured device-to-device copy ceiling (read + write of the same bytes as the level-1 DWT)
+    cp_n = 4 * W * H * NC
+    d_cp = ctx.alloc(cp_n)
+    t_copy = timed(lambda: lib().rg_memcpy_d2d(ctx.handle, ctypes.c_void_p(d_cp.ptr), ctypes.c_void_p(d_pl.ptr),
+                                                 cp_n))
+    copy_gbs = 2.0 * cp_n / (t_copy * 1e-3) / 1e9
+    d_cp.free()
     l1_bytes = 8.0 * W * H * NC
     achieved = l1_bytes / (t_dwt_l1 * 1e-3) / 1e9
     colour_gbs = 15.0 * W * H / (t_colour * 1e-3) / 1e9
@@ -193,6 +200,7 @@ def main():
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes": l1_bytes, "avg_launch_ms": round(t_dwt_l1, 4)},
+        "hbm_copy_ceiling_GBps": round(copy_gbs, 1),
         "stages_ms": {"colour_ict": round(t_colour, 4), "colour_GBps": round(colour_gbs, 1),
                       "dwt97_quant_6lv": round(t_dwt, 4), "dwt97_level1": round(t_dwt_l1, 4),
                       "t1_encode": round(t_t1, 4),

@@ -91,6 +91,7 @@ int rg_free(rg_context *ctx, void *d_ptr);
 int rg_memcpy_h2d(rg_context *ctx, void *d_dst, const void *src, size_t bytes);   /* stream-ordered, waits */
 int rg_memcpy_d2h(rg_context *ctx, void *dst, const void *d_src, size_t bytes);   /* stream-ordered, waits */
 int rg_memset(rg_context *ctx, void *d_dst, int value, size_t bytes);
+int rg_memcpy_d2d(rg_context *ctx, void *d_dst, const void *d_src, size_t bytes);  /* asynchronous */
 
 /* ---- timing on the context stream (HIP events) ---- */
 int rg_timer_start(rg_context *ctx);

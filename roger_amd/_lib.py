@@ -74,6 +74,7 @@ SIGNATURES = {
     "rg_memcpy_h2d": (I, [P, P, P, SZ]),
     "rg_memcpy_d2h": (I, [P, P, P, SZ]),
     "rg_memset": (I, [P, P, I, SZ]),
+    "rg_memcpy_d2d": (I, [P, P, P, SZ]),
     "rg_timer_start": (I, [P]),
     "rg_timer_stop": (I, [P, P]),
     "rg_colour_fwd": (I, [P, P, I, I, I, I, I, P]),

@@ -62,7 +62,7 @@ struct rg_batch {
     std::vector<int> cb_first, cb_count;            // per image, into the block list
     std::vector<rg_cb_desc> desc;                    // all blocks, image-major
     std::vector<int> desc_img;
-    Buf planes, q, scratch, slots, res, t1jobs, counter, cparams, cjobs;
+    Buf planes, q, scratch, slots, res, t1jobs, counter, cparams, cjobs, syms, cm;
     Buf fargs[16], fjobs[16];
     int nfjobs[16] = {0}, ncjobs = 0, ncb = 0;
     hipGraph_t graph = nullptr;
@@ -91,7 +91,8 @@ extern "C" int rg_batch_destroy(rg_batch *b)
     hipSetDevice(b->ctx->device);
     hipStreamSynchronize(b->ctx->stream);
     drop_graph(b);
-    for (Buf *x : {&b->planes, &b->q, &b->scratch, &b->slots, &b->res, &b->t1jobs, &b->counter, &b->cparams, &b->cjobs})
+    for (Buf *x : {&b->planes, &b->q, &b->scratch, &b->slots, &b->res, &b->t1jobs, &b->counter, &b->cparams, &b->cjobs,
+                   &b->syms, &b->cm})
         x->release();
     for (int l = 0; l < 16; ++l) { b->fargs[l].release(); b->fjobs[l].release(); }
     delete b;
@@ -109,7 +110,8 @@ static int issue(rg_batch *b)
         if ((rc = rg_launch_fwd_level_batch(s, (const rg_fwd_level_args *)b->fargs[l].p, (const int4 *)b->fjobs[l].p,
                                             b->nfjobs[l])))
             return rc;
-    return rg_launch_t1_queue(s, b->t1jobs.p, b->ncb, (int *)b->counter.p, b->nwaves);
+    if (rg_t1_inline()) return rg_launch_t1_queue(s, b->t1jobs.p, b->ncb, (int *)b->counter.p, b->nwaves);
+    return rg_launch_t1_queue_split(s, b->t1jobs.p, b->ncb, (int *)b->counter.p, b->nwaves);
 }
 
 extern "C" int rg_batch_plan(rg_batch *b, const rg_batch_image *imgs, int n, int cbw, int cbh, int use_graph,
@@ -223,8 +225,10 @@ extern "C" int rg_batch_plan(rg_batch *b, const rg_batch_image *imgs, int n, int
     }
     b->ncb = (int)b->desc.size();
     b->slot = std::max(256, cbw * cbh * 3);
+    const int symcap = rg_t1_symcap(cbw, cbh);
     if ((rc = b->slots.ensure((size_t)b->slot * b->ncb)) || (rc = b->res.ensure(sizeof(rg_cb_result) * b->ncb)) ||
-        (rc = b->counter.ensure(256)))
+        (rc = b->counter.ensure(256)) || (rc = b->cm.ensure(sizeof(int4) * b->ncb)) ||
+        (!rg_t1_inline() && (rc = b->syms.ensure((size_t)symcap * b->ncb))))
         return rc;
     std::vector<int> order(b->ncb);
     std::iota(order.begin(), order.end(), 0);
@@ -239,7 +243,8 @@ extern "C" int rg_batch_plan(rg_batch *b, const rg_batch_image *imgs, int n, int
         const int32_t *base = (const int32_t *)b->q.p + b->plane_off[b->desc_img[idx]] +
                               (size_t)d.comp * m.w * m.h + (size_t)d.y0 * m.w + d.x0;
         rg_t1_job_fill(jobs.data() + (size_t)k * jb, base, m.w, d.w, d.h, d.orient,
-                       (uint8_t *)b->slots.p + (size_t)idx * b->slot, b->slot, (rg_cb_result *)b->res.p + idx);
+                       (uint8_t *)b->slots.p + (size_t)idx * b->slot, b->slot, (rg_cb_result *)b->res.p + idx,
+                       b->syms.p ? (uint8_t *)b->syms.p + (size_t)k * symcap : nullptr, symcap, (int4 *)b->cm.p + k);
     }
     if ((rc = b->t1jobs.ensure(jobs.size()))) return rc;
     RG_HIP(hipMemcpy(b->t1jobs.p, jobs.data(), jobs.size(), hipMemcpyHostToDevice));

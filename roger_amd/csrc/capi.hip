@@ -131,6 +131,13 @@ extern "C" int rg_memcpy_d2h(rg_context *c, void *dst, const void *src, size_t b
     return RG_SUCCESS;
 }
 
+extern "C" int rg_memcpy_d2d(rg_context *c, void *dst, const void *src, size_t bytes)
+{
+    RG_CHECK(c, "null context");
+    RG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    return RG_SUCCESS;
+}
+
 extern "C" int rg_memset(rg_context *c, void *dst, int value, size_t bytes)
 {
     RG_CHECK(c, "null context");
@@ -346,7 +353,18 @@ extern "C" int rg_t1_encode(rg_context *c, const int32_t *dq, int pitch, size_t 
 {
     RG_CHECK(c && dq && desc && out && res, "rg_t1_encode: null argument");
     RG_CHECK(ncb >= 0 && slot_bytes > 0 && pitch > 0, "rg_t1_encode: bad sizes");
-    return rg_launch_t1_encode(c->stream, dq, pitch, plane_stride, desc, ncb, out, slot_bytes, res);
+    if (rg_t1_inline() || ncb == 0)
+        return rg_launch_t1_encode(c->stream, dq, pitch, plane_stride, desc, ncb, out, slot_bytes, res);
+    // stream-ordered scratch for the symbol streams (cbw, cbh <= 64)
+    const int symcap = rg_t1_symcap(64, 64);
+    void *syms = nullptr, *cm = nullptr;
+    RG_HIP(hipMallocAsync(&syms, (size_t)symcap * ncb, c->stream));
+    RG_HIP(hipMallocAsync(&cm, sizeof(int4) * ncb, c->stream));
+    int rc = rg_launch_t1_split(c->stream, dq, pitch, plane_stride, desc, ncb, (uint8_t *)syms, symcap, (int4 *)cm, out,
+                                slot_bytes, res);
+    hipFreeAsync(syms, c->stream);
+    hipFreeAsync(cm, c->stream);
+    return rc;
 }
 
 // ------------------------------------------------------------ buffers
@@ -416,7 +434,7 @@ struct rg_encoder {
     int ncb_comp = 0, ncb = 0, slot_bytes = 0, colour = 0;
     double delta = 1.0 / 64;
     float step[97], inv[97];
-    DevBuf input, planes, q, fcoeff, scratch, desc, res, slots, chan16;
+    DevBuf input, planes, q, fcoeff, scratch, desc, res, slots, chan16, syms, cm;
     HostBuf staging, map;
     std::vector<rg_cb_desc> h_desc;
     int desc_key[6] = {0, 0, 0, 0, 0, 0};
@@ -436,7 +454,8 @@ extern "C" int rg_encoder_destroy(rg_encoder *e)
     if (!e) return RG_SUCCESS;
     hipSetDevice(e->ctx->device);
     hipStreamSynchronize(e->ctx->stream);
-    for (DevBuf *b : {&e->input, &e->planes, &e->q, &e->fcoeff, &e->scratch, &e->desc, &e->res, &e->slots, &e->chan16})
+    for (DevBuf *b : {&e->input, &e->planes, &e->q, &e->fcoeff, &e->scratch, &e->desc, &e->res, &e->slots, &e->chan16,
+                      &e->syms, &e->cm})
         b->release();
     e->staging.release();
     e->map.release();
@@ -472,6 +491,15 @@ static int run_t1(rg_encoder *e)
     e->slot_bytes = std::max(256, e->cbw * e->cbh * 3);
     if ((rc = e->res.ensure(sizeof(rg_cb_result) * e->ncb))) return rc;
     if ((rc = e->slots.ensure((size_t)e->slot_bytes * e->ncb))) return rc;
+    if (!rg_t1_inline()) {
+        const int symcap = rg_t1_symcap(e->cbw, e->cbh);
+        if ((rc = e->syms.ensure((size_t)symcap * e->ncb)) || (rc = e->cm.ensure(sizeof(int4) * e->ncb))) return rc;
+        rc = rg_launch_t1_split(e->ctx->stream, (const int32_t *)e->q.p, e->w, (size_t)e->w * e->h,
+                                (const rg_cb_desc *)e->desc.p, e->ncb, (uint8_t *)e->syms.p, symcap, (int4 *)e->cm.p,
+                                (uint8_t *)e->slots.p, e->slot_bytes, (rg_cb_result *)e->res.p);
+        if (rc == RG_SUCCESS) e->t1_ran = 1;
+        return rc;
+    }
     rc = rg_launch_t1_encode(e->ctx->stream, (const int32_t *)e->q.p, e->w, (size_t)e->w * e->h,
                              (const rg_cb_desc *)e->desc.p, e->ncb, (uint8_t *)e->slots.p, e->slot_bytes,
                              (rg_cb_result *)e->res.p);

@@ -124,6 +124,16 @@ __device__ __forceinline__ float dequant(int q, float s)
 constexpr int FWD_LDS_BYTES = TILE * (TILE + 2 * 4 + 1) * 4;   // largest (9/7) tile
 constexpr int INV_LDS_BYTES = (TILE + 2 * 4) * (TILE + 1) * 4;
 
+// Symmetric-extension index: one reflection then a clamp.  Exact for every
+// position an output inside the signal depends on (overshoot <= halo < n);
+// positions further out only feed outputs outside the signal, and the clamp
+// keeps their loads in bounds.  Levels shorter than halo+1 use rg_mirror.
+__device__ __forceinline__ int mirror1(int i, int n)
+{
+    i = i < 0 ? -i : (i >= n ? 2 * n - 2 - i : i);
+    return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+
 // One 64x64 tile (bx, by) of plane `plane`; `lds` holds FWD_LDS_BYTES.
 template <int F>
 __device__ __forceinline__ void fwd_tile(const rg_fwd_level_args &a, int plane, int bx, int by, char *lds)
@@ -140,15 +150,21 @@ __device__ __forceinline__ void fwd_tile(const rg_fwd_level_args &a, int plane, 
     const void *in = reinterpret_cast<const char *>(a.in) +
                      (long)plane * a.in_ps * (F == RG_FILTER_53 ? 4 : (a.in_half ? 2 : 4));
     const bool vlift = ch > 1, hlift = cw > 1;
+    const bool simple = cw > H && ch > H;      // one reflection suffices
 
-    // ---- vertical pass: (column c of TW, segment k) ----
+    // ---- vertical pass straight from HBM: (column c of TW, segment k) ----
     for (int item = threadIdx.x; item < TW * NSEG; item += blockDim.x) {
         const int c = item % TW, k = item / TW;
-        const int x = rg_mirror(x0 - H + c, cw);
+        const int x = simple ? mirror1(x0 - H + c, cw) : rg_mirror(x0 - H + c, cw);
         const int ybase = y0 + k * SEG - H;
         T e[N];
+        if (simple) {
 #pragma unroll
-        for (int j = 0; j < N; ++j) e[j] = load_in<F>(in, (long)rg_mirror(ybase + j, ch) * a.in_pitch + x, a.in_half);
+            for (int j = 0; j < N; ++j) e[j] = load_in<F>(in, (long)mirror1(ybase + j, ch) * a.in_pitch + x, a.in_half);
+        } else {
+#pragma unroll
+            for (int j = 0; j < N; ++j) e[j] = load_in<F>(in, (long)rg_mirror(ybase + j, ch) * a.in_pitch + x, a.in_half);
+        }
         if (vlift) {
             if constexpr (F == RG_FILTER_53) fwd53(e); else fwd97(e, true);
         }
@@ -157,48 +173,52 @@ __device__ __forceinline__ void fwd_tile(const rg_fwd_level_args &a, int plane, 
     }
     __syncthreads();
 
-    // ---- horizontal pass: (segment k, row r), k fastest ----
-    const int wl = (cw + 1) >> 1, hl = (ch + 1) >> 1;
-    char *qbase = reinterpret_cast<char *>(a.q) + (long)plane * a.q_ps * 4;
-    for (int item = threadIdx.x; item < TILE * NSEG; item += blockDim.x) {
-        const int k = item % NSEG, r = item / NSEG;
-        const int y = y0 + r;
-        if (y >= ch) continue;
+    // ---- horizontal pass from LDS: (segment k, row r), k fastest ----
+    T o[SEG];
+    const int kk = threadIdx.x % NSEG, rr = threadIdx.x / NSEG;   // TILE * NSEG == blockDim.x
+    {
         T e[N];
 #pragma unroll
-        for (int j = 0; j < N; ++j) e[j] = tile[r * LP + k * SEG + j];
+        for (int j = 0; j < N; ++j) e[j] = tile[rr * LP + kk * SEG + j];
         if (hlift) {
             if constexpr (F == RG_FILTER_53) fwd53(e); else fwd97(e, true);
         }
-        const int xs = x0 + k * SEG;          // first output column (even)
-        const bool lowrow = !(y & 1);
-        const int orow = lowrow ? (y >> 1) : hl + (y >> 1);
 #pragma unroll
-        for (int j = 0; j < SEG; ++j) {
-            const int x = xs + j;
-            if (x >= cw) break;
-            const T v = e[H + j];
-            const bool lowcol = !(x & 1);
-            if (lowrow && lowcol && a.ll_out) {
-                // LL handed to the next level
-                long o = (long)plane * a.ll_ps + (long)(y >> 1) * a.ll_pitch + (x >> 1);
-                if constexpr (F == RG_FILTER_53) reinterpret_cast<int *>(a.ll_out)[o] = v;
-                else if (a.ll_half) reinterpret_cast<__half *>(a.ll_out)[o] = __float2half_rn(v);
-                else reinterpret_cast<float *>(a.ll_out)[o] = v;
-                continue;
-            }
-            const int ocol = lowcol ? (x >> 1) : wl + (x >> 1);
-            long o = (long)orow * a.qp + ocol;
-            if constexpr (F == RG_FILTER_53) {
-                reinterpret_cast<int *>(qbase)[o] = v;
-            } else {
-                if (a.q_float) {
-                    reinterpret_cast<float *>(qbase)[o] = v;
-                } else {
-                    const float inv = lowrow ? (lowcol ? a.inv_ll : a.inv_hl) : (lowcol ? a.inv_lh : a.inv_hh);
-                    reinterpret_cast<int *>(qbase)[o] = quant(v, inv);
-                }
-            }
+        for (int j = 0; j < SEG; ++j) o[j] = e[H + j];
+    }
+    __syncthreads();
+    // deinterleave the row into LDS: low outputs to [0, 32), high to [32, 64)
+#pragma unroll
+    for (int j = 0; j < SEG; ++j) tile[rr * LP + (j & 1) * (TILE / 2) + kk * (SEG / 2) + (j >> 1)] = o[j];
+    __syncthreads();
+
+    // ---- write out: each wavefront stores one row: 32 low + 32 high values ----
+    const int wl = (cw + 1) >> 1, hl = (ch + 1) >> 1;
+    char *qbase = reinterpret_cast<char *>(a.q) + (long)plane * a.q_ps * 4;
+    for (int idx = threadIdx.x; idx < TILE * TILE; idx += blockDim.x) {
+        const int r = idx / TILE, c = idx % TILE;
+        const int y = y0 + r;
+        const bool high = c >= TILE / 2;
+        const int xo = (x0 >> 1) + (c & (TILE / 2 - 1));       // column inside its half band
+        const int x = 2 * xo + (high ? 1 : 0);                 // source column
+        if (y >= ch || x >= cw) continue;
+        const T v = tile[r * LP + c];
+        const bool lowrow = !(y & 1);
+        if (lowrow && !high && a.ll_out) {
+            const long off = (long)plane * a.ll_ps + (long)(y >> 1) * a.ll_pitch + xo;
+            if constexpr (F == RG_FILTER_53) reinterpret_cast<int *>(a.ll_out)[off] = v;
+            else if (a.ll_half) reinterpret_cast<__half *>(a.ll_out)[off] = __float2half_rn(v);
+            else reinterpret_cast<float *>(a.ll_out)[off] = v;
+            continue;
+        }
+        const long off = (long)(lowrow ? (y >> 1) : hl + (y >> 1)) * a.qp + (high ? wl + xo : xo);
+        if constexpr (F == RG_FILTER_53) {
+            reinterpret_cast<int *>(qbase)[off] = v;
+        } else if (a.q_float) {
+            reinterpret_cast<float *>(qbase)[off] = v;
+        } else {
+            const float inv = lowrow ? (high ? a.inv_hl : a.inv_ll) : (high ? a.inv_hh : a.inv_lh);
+            reinterpret_cast<int *>(qbase)[off] = quant(v, inv);
         }
     }
 }
@@ -247,11 +267,13 @@ __device__ __forceinline__ void inv_tile(const rg_inv_level_args &a, int plane, 
     // ---- horizontal synthesis: (segment k, row rr of TH), k fastest ----
     for (int item = threadIdx.x; item < TH * NSEG; item += blockDim.x) {
         const int k = item % NSEG, rr = item / NSEG;
-        const int y = rg_mirror(y0 - H + rr, ch);
+        const bool simple = cw > H && ch > H;
+        const int y = simple ? mirror1(y0 - H + rr, ch) : rg_mirror(y0 - H + rr, ch);
         const int xb = x0 + k * SEG - H;
         T e[N];
 #pragma unroll
-        for (int j = 0; j < N; ++j) e[j] = load_band<F>(a, plane, y, rg_mirror(xb + j, cw), wl, hl);
+        for (int j = 0; j < N; ++j)
+            e[j] = load_band<F>(a, plane, y, simple ? mirror1(xb + j, cw) : rg_mirror(xb + j, cw), wl, hl);
         if constexpr (F == RG_FILTER_53) { if (hlift) inv53(e); }
         else inv97(e, hlift);
 #pragma unroll
@@ -300,7 +322,7 @@ __global__ void __launch_bounds__(256) fwd_level_batch_kernel(const rg_fwd_level
 {
     __shared__ __attribute__((aligned(16))) char lds[FWD_LDS_BYTES];
     const int4 j = jobs[blockIdx.x];
-    const rg_fwd_level_args a = args[j.x];
+    const rg_fwd_level_args &a = args[j.x];
     if (a.filter == RG_FILTER_53) fwd_tile<RG_FILTER_53>(a, j.y, j.z, j.w, lds);
     else fwd_tile<RG_FILTER_97>(a, j.y, j.z, j.w, lds);
 }
@@ -310,7 +332,7 @@ __global__ void __launch_bounds__(256) inv_level_batch_kernel(const rg_inv_level
 {
     __shared__ __attribute__((aligned(16))) char lds[INV_LDS_BYTES];
     const int4 j = jobs[blockIdx.x];
-    const rg_inv_level_args a = args[j.x];
+    const rg_inv_level_args &a = args[j.x];
     if (a.filter == RG_FILTER_53) inv_tile<RG_FILTER_53>(a, j.y, j.z, j.w, lds);
     else inv_tile<RG_FILTER_97>(a, j.y, j.z, j.w, lds);
 }

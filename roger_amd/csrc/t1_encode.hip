@@ -24,6 +24,7 @@
 // mask, readlane) as wave-uniform scalar code whose context states and
 // probability table sit in VGPR lanes, so the coder touches no memory except
 // the codeword bytes it writes to the block's output slot.
+#include <cstdlib>
 #include "rg_internal.h"
 
 namespace {
@@ -215,9 +216,42 @@ struct T1Job {
     int pitch, w, h, orient, cap;
     uint8_t *out;
     rg_cb_result *res;
+    uint8_t *sym;   // split path: this block's symbol stream (symcap bytes)
+    int symcap;
+    int4 *cm;       // split path: {symbols, passes, bit-planes, overflow}
 };
 
-// Encodes one code-block with the calling wavefront (64 lanes).
+__device__ __forceinline__ int wave_incl_scan(int v, int lane)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Split path: appends one stripe's symbols (scan order) to the block's stream.
+__device__ __forceinline__ void emit_stripe(const SymList &sl, const T1Job &job, int &pos, int &ovf, int lane)
+{
+    const int incl = wave_incl_scan(sl.n, lane);
+    const int total = __builtin_amdgcn_readlane(incl, 63);
+    if (total == 0) return;
+    if (pos + total > job.symcap) { ovf = 1; return; }
+    const int off = pos + incl - sl.n;
+    uint64_t a = sl.lo, b = sl.hi;
+    for (int t = 0; t < 16; ++t) {
+        if (!__any(t < sl.n)) break;
+        if (t < sl.n) job.sym[off + t] = (uint8_t)(a & 0xFF);
+        a = (a >> 8) | (b << 56);
+        b >>= 8;
+    }
+    pos += total;
+}
+
+// Encodes one code-block with the calling wavefront (64 lanes).  SPLIT:
+// context modelling only, the symbols go to job.sym for t1_mq_kernel.
+template <bool SPLIT>
 __device__ void t1_block(const T1Job &job)
 {
     const int lane = threadIdx.x;
@@ -243,9 +277,13 @@ __device__ void t1_block(const T1Job &job)
     // compiler so, which keeps the whole bit-plane loop (and the MQ coder) scalar
     const int nbp = __builtin_amdgcn_readfirstlane(orm ? 32 - __clz(orm) : 0);
     if (nbp == 0) {
-        if (lane == 0) *job.res = rg_cb_result{0, 0, 0, 0};
+        if (lane == 0) {
+            if (SPLIT) *job.cm = make_int4(0, 0, 0, 0);
+            else *job.res = rg_cb_result{0, 0, 0, 0};
+        }
         return;
     }
+    int spos = 0, sovf = 0;   // split path: stream position / overflow (uniform)
 
     // initial context states (Table D.7): ZC ctx 0 -> state 4, RL -> 3, UNIFORM -> 46, others 0
     int ctxv = lane == 0 ? (4 << 1) : lane == CTX_RL ? (3 << 1) : lane == CTX_UNI ? (46 << 1) : 0;
@@ -387,13 +425,18 @@ __device__ void t1_block(const T1Job &job)
                     S |= (uint64_t)N << r0;
                 }
 
-                // ---- MQ-code this stripe's symbols in scan order ----
-                mq_stripe(m, ctxv, mqtv, sl, lane);
+                // ---- MQ-code this stripe's symbols in scan order (or hand them on) ----
+                if constexpr (SPLIT) emit_stripe(sl, job, spos, sovf, lane);
+                else mq_stripe(m, ctxv, mqtv, sl, lane);
             }
             if (pass == 2) V = 0;
         }
     }
 
+    if constexpr (SPLIT) {
+        if (lane == 0) *job.cm = make_int4(spos, 3 * nbp - 2, nbp, sovf);
+        return;
+    }
     // ---- FLUSH (C.2.9) ----
     {
         uint32_t tmp = m.C + m.A;
@@ -420,7 +463,7 @@ __global__ void __launch_bounds__(64) t1_encode_kernel(const int32_t *__restrict
     job.pitch = pitch; job.w = d.w; job.h = d.h; job.orient = d.orient; job.cap = slot_bytes;
     job.out = out + (size_t)blockIdx.x * slot_bytes;
     job.res = res + blockIdx.x;
-    t1_block(job);
+    t1_block<false>(job);
 }
 
 // Batched path: a persistent grid of wavefronts pulls job indices from a
@@ -434,13 +477,209 @@ __global__ void __launch_bounds__(64) t1_encode_queue_kernel(const T1Job *__rest
         i = __builtin_amdgcn_readfirstlane(i);   // uniform: job fields load as scalars
         if (i >= njobs) break;
         const T1Job job = jobs[i];
-        t1_block(job);
+        t1_block<false>(job);
     }
 }
 
 __global__ void reset_counter_kernel(int *counter) { *counter = 0; }
 
+// ------------------------------------------------------------------------
+// Split path.  Context modelling runs one wavefront per code-block and writes
+// the block's (context, bit) stream; the MQ coder then runs one LANE per
+// code-block, so 64 blocks advance their serial coder chains together in
+// vector registers instead of one block per wavefront in scalar registers.
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(64) t1_cm_kernel(const int32_t *__restrict__ q, int pitch, size_t plane_stride,
+                                                   const rg_cb_desc *__restrict__ desc, uint8_t *__restrict__ syms,
+                                                   int symcap, int4 *__restrict__ cm)
+{
+    const rg_cb_desc d = desc[blockIdx.x];
+    T1Job job;
+    job.src = q + (size_t)d.comp * plane_stride + (size_t)d.y0 * pitch + d.x0;
+    job.pitch = pitch; job.w = d.w; job.h = d.h; job.orient = d.orient;
+    job.sym = syms + (size_t)blockIdx.x * symcap;
+    job.symcap = symcap;
+    job.cm = cm + blockIdx.x;
+    t1_block<true>(job);
+}
+
+__global__ void __launch_bounds__(64) t1_cm_queue_kernel(const T1Job *__restrict__ jobs, int njobs,
+                                                         int *__restrict__ counter)
+{
+    for (;;) {
+        int i = 0;
+        if (threadIdx.x == 0) i = atomicAdd(counter, 1);
+        i = __builtin_amdgcn_readfirstlane(i);
+        if (i >= njobs) break;
+        const T1Job job = jobs[i];
+        t1_block<true>(job);
+    }
+}
+
+struct LaneMQ {
+    uint32_t A, C, B;
+    int CT, bp, ovf;
+};
+
+__device__ __forceinline__ void lane_byteout(LaneMQ &m, uint8_t *out, int cap)
+{
+    // BYTEOUT (C.2.7) without branches on the byte value: a carry into B,
+    // then either 8 or 7 (after 0xFF, bit stuffing) bits move out of C.
+    const bool carry = m.B != 0xFF && m.C >= 0x8000000u;
+    const uint32_t b2 = m.B + (carry ? 1u : 0u);
+    if (carry) m.C &= 0x7FFFFFF;
+    const bool big = b2 == 0xFF;
+    if (m.bp >= 0) {
+        if (m.bp < cap) out[m.bp] = (uint8_t)b2;
+        else m.ovf = 1;
+    }
+    m.bp++;
+    m.B = big ? (m.C >> 20) : (m.C >> 19);
+    m.C &= big ? 0xFFFFFu : 0x7FFFFu;
+    m.CT = big ? 7 : 8;
+}
+
+// One ENCODE step (C.2.2-C.2.6) for this lane's block.  The 19 context states
+// are packed 4 per register (8 bits: I << 1 | MPS); the probability table is
+// read from LDS.
+__device__ __forceinline__ void lane_encode(LaneMQ &m, uint32_t (&cs)[5], const uint32_t *tab, uint32_t sym,
+                                            uint8_t *out, int cap)
+{
+    const uint32_t cx = sym >> 1, d = sym & 1u;
+    const uint32_t wsel = cx >> 2, sh = (cx & 3u) * 8u;
+    uint32_t cw = cs[0];
+    cw = wsel == 1 ? cs[1] : cw;
+    cw = wsel == 2 ? cs[2] : cw;
+    cw = wsel == 3 ? cs[3] : cw;
+    cw = wsel == 4 ? cs[4] : cw;
+    const uint32_t st = (cw >> sh) & 0xFFu, mps = st & 1u;
+    const uint32_t t = tab[st >> 1];
+    const uint32_t qe = t & 0xFFFFu;
+    m.A -= qe;
+    const bool lps = d != mps;
+    if (!lps && (m.A & 0x8000u)) {
+        m.C += qe;
+        return;
+    }
+    const bool small = m.A < qe;
+    if (lps ? small : !small) m.C += qe;
+    else m.A = qe;
+    const uint32_t ns = lps ? ((((t >> 22) & 63u) << 1) | (mps ^ (t >> 28))) : ((((t >> 16) & 63u) << 1) | mps);
+    cw = (cw & ~(0xFFu << sh)) | (ns << sh);
+    cs[0] = wsel == 0 ? cw : cs[0];
+    cs[1] = wsel == 1 ? cw : cs[1];
+    cs[2] = wsel == 2 ? cw : cs[2];
+    cs[3] = wsel == 3 ? cw : cs[3];
+    cs[4] = wsel == 4 ? cw : cs[4];
+    int n = __clz(m.A) - 16;
+    while (n > 0) {
+        const int sft = n < m.CT ? n : m.CT;
+        m.A <<= sft; m.C <<= sft; m.CT -= sft; n -= sft;
+        if (m.CT == 0) lane_byteout(m, out, cap);
+    }
+}
+
+// MQ-codes one block per lane from the streams t1_cm_kernel wrote.
+template <bool QUEUE>
+__device__ __forceinline__ void t1_mq_lanes(const T1Job *jobs, int i, int njobs, const uint8_t *syms, int symcap,
+                                            const int4 *cm, uint8_t *slots, int slot, rg_cb_result *res)
+{
+    __shared__ uint32_t tab[48];
+    if (threadIdx.x < 47) tab[threadIdx.x] = MQT[threadIdx.x];
+    __syncthreads();
+    const bool active = i < njobs;
+    int4 c = make_int4(0, 0, 0, 0);
+    const uint8_t *src = nullptr;
+    uint8_t *out = nullptr;
+    rg_cb_result *dst = nullptr;
+    int cap = 0;
+    if (active) {
+        if (QUEUE) {
+            const T1Job &j = jobs[i];
+            c = *j.cm; src = j.sym; out = j.out; cap = j.cap; dst = j.res;
+        } else {
+            c = cm[i]; src = syms + (size_t)i * symcap; out = slots + (size_t)i * slot; cap = slot; dst = res + i;
+        }
+    }
+    const int n = c.x;
+    int nmax = n;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o, 64));
+    LaneMQ m;
+    m.A = 0x8000; m.C = 0; m.B = 0; m.CT = 12; m.bp = -1; m.ovf = 0;
+    // initial states (Table D.7): ctx 0 -> 4, RL (17) -> 3, UNIFORM (18) -> 46
+    uint32_t cs[5] = {4u << 1, 0, 0, 0, (3u << 1) << 8 | (46u << 1) << 16};
+    const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+    for (int base = 0; base < nmax; base += 16) {
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (base < n) v = s4[base >> 4];
+        const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if (base + k < n) lane_encode(m, cs, tab, (w4[k >> 2] >> (8 * (k & 3))) & 0xFFu, out, cap);
+    }
+    if (!active) return;
+    if (c.z == 0) {
+        *dst = rg_cb_result{0, 0, 0, 0};
+        return;
+    }
+    // FLUSH (C.2.9)
+    const uint32_t tmp = m.C + m.A;
+    m.C |= 0xFFFF;
+    if (m.C >= tmp) m.C -= 0x8000;
+    m.C <<= m.CT; lane_byteout(m, out, cap);
+    m.C <<= m.CT; lane_byteout(m, out, cap);
+    int len;
+    if (m.B != 0xFF) {
+        if (m.bp >= 0 && m.bp < cap) out[m.bp] = (uint8_t)m.B;
+        else m.ovf = 1;
+        len = m.bp + 1;
+    } else {
+        len = m.bp;
+    }
+    if (len > cap) m.ovf = 1;
+    *dst = rg_cb_result{len, c.y, c.z, (m.ovf || c.w) ? 1 : 0};
+}
+
+__global__ void __launch_bounds__(64) t1_mq_kernel(const uint8_t *__restrict__ syms, int symcap,
+                                                   const int4 *__restrict__ cm, int ncb, uint8_t *__restrict__ slots,
+                                                   int slot, rg_cb_result *__restrict__ res)
+{
+    t1_mq_lanes<false>(nullptr, blockIdx.x * 64 + threadIdx.x, ncb, syms, symcap, cm, slots, slot, res);
+}
+
+__global__ void __launch_bounds__(64) t1_mq_queue_kernel(const T1Job *__restrict__ jobs, int njobs)
+{
+    t1_mq_lanes<true>(jobs, blockIdx.x * 64 + threadIdx.x, njobs, nullptr, 0, nullptr, nullptr, 0, nullptr);
+}
+
 }  // namespace
+
+// Symbol-stream capacity per code-block sample for the split path.
+int rg_t1_symcap(int cbw, int cbh) { return ((cbw * cbh * 24) + 15) / 16 * 16; }
+
+int rg_launch_t1_split(hipStream_t s, const int32_t *q, int pitch, size_t plane_stride, const rg_cb_desc *desc, int ncb,
+                       uint8_t *syms, int symcap, int4 *cm, uint8_t *out, int slot_bytes, rg_cb_result *res)
+{
+    if (ncb <= 0) return RG_SUCCESS;
+    t1_cm_kernel<<<dim3((unsigned)ncb), dim3(64), 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm);
+    RG_LAUNCH_CHECK();
+    t1_mq_kernel<<<dim3((unsigned)((ncb + 63) / 64)), dim3(64), 0, s>>>(syms, symcap, cm, ncb, out, slot_bytes, res);
+    RG_LAUNCH_CHECK();
+    return RG_SUCCESS;
+}
+
+int rg_launch_t1_queue_split(hipStream_t s, const void *jobs, int njobs, int *counter, int nwaves)
+{
+    if (njobs <= 0) return RG_SUCCESS;
+    reset_counter_kernel<<<1, 1, 0, s>>>(counter);
+    RG_LAUNCH_CHECK();
+    t1_cm_queue_kernel<<<dim3((unsigned)nwaves), dim3(64), 0, s>>>((const T1Job *)jobs, njobs, counter);
+    RG_LAUNCH_CHECK();
+    t1_mq_queue_kernel<<<dim3((unsigned)((njobs + 63) / 64)), dim3(64), 0, s>>>((const T1Job *)jobs, njobs);
+    RG_LAUNCH_CHECK();
+    return RG_SUCCESS;
+}
 
 int rg_t1_job_bytes() { return (int)sizeof(T1Job); }
 
@@ -456,11 +695,24 @@ int rg_launch_t1_queue(hipStream_t s, const void *jobs, int njobs, int *counter,
 
 // Host-side fill of one job record (the struct layout stays private to this file).
 void rg_t1_job_fill(void *dst, const int32_t *src, int pitch, int w, int h, int orient, uint8_t *out, int cap,
-                    rg_cb_result *res)
+                    rg_cb_result *res, uint8_t *sym, int symcap, int4 *cm)
 {
     T1Job j;
     j.src = src; j.pitch = pitch; j.w = w; j.h = h; j.orient = orient; j.cap = cap; j.out = out; j.res = res;
+    j.sym = sym; j.symcap = symcap; j.cm = cm;
     *reinterpret_cast<T1Job *>(dst) = j;
+}
+
+// Tier-1 variant: the split context-model + lane-MQ path unless RG_T1_INLINE=1
+// selects the single-kernel path (one wavefront per block, scalar MQ coder).
+bool rg_t1_inline()
+{
+    static int v = -1;
+    if (v < 0) {
+        const char *e = getenv("RG_T1_INLINE");
+        v = (e && e[0] == '1') ? 1 : 0;
+    }
+    return v == 1;
 }
 
 int rg_launch_t1_encode(hipStream_t s, const int32_t *q, int pitch, size_t plane_stride,

@@ -319,7 +319,7 @@ class Decoder:
         p.cbw, p.cbh, p.delta, p.ll_fp16, p.bitdepth = cbw, cbh, float(delta), 0, 8
         check(lib().rg_decoder_decode_codeblocks(
             self.handle, descs.ctypes.data_as(ctypes.c_void_p), res.ctypes.data_as(ctypes.c_void_p),
-            offs.ctypes.data_as(ctypes.c_void_p), data.ctypes.data_as(ctypes.c_void_p), pos, len(blobs), w, h,
+            offs.ctypes.data_as(ctypes.c_void_p), data.ctypes.data_as(ctypes.c_void_p), pos, len(res), w, h,
             ncomp, ctypes.byref(p)))
         self.shape = (h, w) if ncomp == 1 else (h, w, ncomp)
 
