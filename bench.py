@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 
 def bench_batched(ctx, args):
     """Mixed-size batch (sides log-uniform 256..4096, half gray / half RGB,
-    half 5/3 / half 9/7, 5 levels): batched persistent + graph path vs the plain
+    half 5/3 / half 9/7, 5 levels): batched job-table + graph path vs the plain
     per-image path, both on device-resident inputs."""
     from roger_amd import autotune
     from roger_amd import runtime as R
@@ -180,8 +180,8 @@ def main():
     # measured device-to-device copy ceiling (read + write of the same bytes as the level-1 DWT)
     cp_n = 4 * W * H * NC
     d_cp = ctx.alloc(cp_n)
-    t_copy = timed(lambda: lib().rg_memcpy_d2d(ctx.handle, ctypes.c_void_p(d_cp.ptr), ctypes.c_void_p(d_pl.ptr),
-                                                 cp_n))
+    t_copy = timed(lambda: lib().rg_stream_copy(ctx.handle, ctypes.c_void_p(d_cp.ptr), ctypes.c_void_p(d_pl.ptr),
+                                                  cp_n))
     copy_gbs = 2.0 * cp_n / (t_copy * 1e-3) / 1e9
     d_cp.free()
     l1_bytes = 8.0 * W * H * NC

@@ -92,6 +92,8 @@ int rg_memcpy_h2d(rg_context *ctx, void *d_dst, const void *src, size_t bytes); 
 int rg_memcpy_d2h(rg_context *ctx, void *dst, const void *d_src, size_t bytes);   /* stream-ordered, waits */
 int rg_memset(rg_context *ctx, void *d_dst, int value, size_t bytes);
 int rg_memcpy_d2d(rg_context *ctx, void *d_dst, const void *d_src, size_t bytes);  /* asynchronous */
+/* Streaming device copy kernel (16-byte accesses): the measured HBM ceiling. */
+int rg_stream_copy(rg_context *ctx, void *d_dst, const void *d_src, size_t bytes);
 
 /* ---- timing on the context stream (HIP events) ---- */
 int rg_timer_start(rg_context *ctx);
@@ -252,10 +254,10 @@ typedef struct {
 } rg_batch_image;
 int rg_batch_create(rg_context *ctx, rg_batch **out);
 int rg_batch_destroy(rg_batch *b);
-/* Lays out all buffers and job tables; with use_graph the pipeline (colour,
- * every DWT level, tier-1) is captured once into a HIP graph.  waves_per_cu
- * sizes the persistent tier-1 grid. */
-int rg_batch_plan(rg_batch *b, const rg_batch_image *imgs, int n, int cbw, int cbh, int use_graph, int waves_per_cu);
+/* Lays out all buffers and job tables (tier-1 jobs sorted largest block
+ * first); with use_graph the pipeline (colour, every DWT level, tier-1) is
+ * captured once into a HIP graph. */
+int rg_batch_plan(rg_batch *b, const rg_batch_image *imgs, int n, int cbw, int cbh, int use_graph);
 /* Runs (replays) the plan asynchronously on the context stream. */
 int rg_batch_run(rg_batch *b);
 /* Code-blocks of one image (image = -1: the whole batch). */

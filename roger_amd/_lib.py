@@ -75,6 +75,7 @@ SIGNATURES = {
     "rg_memcpy_d2h": (I, [P, P, P, SZ]),
     "rg_memset": (I, [P, P, I, SZ]),
     "rg_memcpy_d2d": (I, [P, P, P, SZ]),
+    "rg_stream_copy": (I, [P, P, P, SZ]),
     "rg_timer_start": (I, [P]),
     "rg_timer_stop": (I, [P, P]),
     "rg_colour_fwd": (I, [P, P, I, I, I, I, I, P]),
@@ -114,7 +115,7 @@ SIGNATURES = {
     "rg_decoder_coefficients": (I, [P, P]),
     "rg_batch_create": (I, [P, P]),
     "rg_batch_destroy": (I, [P]),
-    "rg_batch_plan": (I, [P, P, I, I, I, I, I]),
+    "rg_batch_plan": (I, [P, P, I, I, I, I]),
     "rg_batch_run": (I, [P]),
     "rg_batch_num_codeblocks": (I, [P, I]),
     "rg_batch_coefficients": (I, [P, I, P]),

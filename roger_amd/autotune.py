@@ -1,14 +1,14 @@
 """Autotuner for the batched encode: picks the code-block size (64x64 or
-32x32) and the persistent tier-1 grid (waves per CU) by timing the actual
-batch, and caches the choice in a JSON file keyed by the batch's shape
-signature (image count, total megapixels, share of 9/7 and RGB images)."""
+32x32) by timing the actual batch, and caches the choice in a JSON file keyed
+by the batch's shape signature (image count, total megapixels, share of 9/7
+and RGB images)."""
 import json
 import os
 import time
 
 CACHE = os.environ.get("RG_AUTOTUNE_CACHE", os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                           ".autotune.json"))
-CANDIDATES = [(64, 4), (64, 8), (64, 12), (32, 8), (32, 16)]
+CANDIDATES = [64, 32]
 
 
 def signature(images):
@@ -27,17 +27,17 @@ def _load():
 
 
 def tune(ctx, batch, images, candidates=CANDIDATES, reps=2, log=None):
-    """Plans `batch` with each candidate, times `reps` runs, leaves the batch
-    planned with the best one and returns {cb, waves_per_cu, ms, table}."""
+    """Plans `batch` with each candidate code-block size, times `reps` runs,
+    leaves the batch planned with the best one and returns {cb, ms, table}."""
     key = signature(images)
     cache = _load()
-    if key in cache:
+    if key in cache and "cb" in cache[key]:
         best = cache[key]
-        batch.plan(images, cbw=best["cb"], cbh=best["cb"], use_graph=True, waves_per_cu=best["waves_per_cu"])
+        batch.plan(images, cbw=best["cb"], cbh=best["cb"], use_graph=True)
         return dict(best, cached=True)
     table = []
-    for cb, wpc in candidates:
-        batch.plan(images, cbw=cb, cbh=cb, use_graph=True, waves_per_cu=wpc)
+    for cb in candidates:
+        batch.plan(images, cbw=cb, cbh=cb, use_graph=True)
         batch.run()
         ctx.sync()
         t = time.perf_counter()
@@ -45,15 +45,15 @@ def tune(ctx, batch, images, candidates=CANDIDATES, reps=2, log=None):
             batch.run()
         ctx.sync()
         ms = (time.perf_counter() - t) * 1e3 / reps
-        table.append({"cb": cb, "waves_per_cu": wpc, "ms": round(ms, 3)})
+        table.append({"cb": cb, "ms": round(ms, 3)})
         if log:
-            log("autotune cb=%d waves/CU=%d: %.2f ms" % (cb, wpc, ms))
+            log("autotune cb=%d: %.2f ms" % (cb, ms))
     best = min(table, key=lambda r: r["ms"])
-    cache[key] = {"cb": best["cb"], "waves_per_cu": best["waves_per_cu"], "ms": best["ms"]}
+    cache[key] = {"cb": best["cb"], "ms": best["ms"]}
     try:
         with open(CACHE, "w") as f:
             json.dump(cache, f, indent=1)
     except OSError:
         pass
-    batch.plan(images, cbw=best["cb"], cbh=best["cb"], use_graph=True, waves_per_cu=best["waves_per_cu"])
+    batch.plan(images, cbw=best["cb"], cbh=best["cb"], use_graph=True)
     return dict(best, cached=False, table=table)

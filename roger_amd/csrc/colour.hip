@@ -8,6 +8,7 @@
 // Streaming design: each thread owns 16 consecutive pixels, so a wavefront
 // reads one contiguous 1 KiB (gray) / 3 KiB (RGB) span with 16-byte loads and
 // writes 4 KiB per plane with 16-byte stores; no LDS.
+#include <algorithm>
 #include "rg_internal.h"
 
 namespace {
@@ -372,5 +373,29 @@ extern "C" int rg_synth_u8(rg_context *ctx, uint8_t *d_dst, int w, int h, int nc
     RG_LAUNCH_CHECK();
     RG_HIP(hipFreeAsync(tmp, ctx->stream));
     RG_HIP(hipStreamSynchronize(ctx->stream));
+    return RG_SUCCESS;
+}
+
+// ---- streaming copy: the measured HBM ceiling the bandwidth-bound kernels are
+//      compared with (16-byte loads/stores, grid-stride) ----
+namespace {
+__global__ void __launch_bounds__(256) stream_copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" int rg_stream_copy(rg_context *ctx, void *d_dst, const void *d_src, size_t bytes)
+{
+    RG_CHECK(ctx && d_dst && d_src, "rg_stream_copy: null argument");
+    RG_CHECK(bytes % 16 == 0 && ((uintptr_t)d_dst & 15) == 0 && ((uintptr_t)d_src & 15) == 0,
+             "rg_stream_copy: 16-byte aligned sizes only");
+    const long n = (long)(bytes / 16);
+    if (n == 0) return RG_SUCCESS;
+    hipDeviceProp_t p;
+    RG_HIP(hipGetDeviceProperties(&p, ctx->device));
+    const long blocks = std::min<long>((n + 255) / 256, (long)p.multiProcessorCount * 16);
+    stream_copy_kernel<<<dim3((unsigned)blocks), dim3(256), 0, ctx->stream>>>((const uint4 *)d_src, (uint4 *)d_dst, n);
+    RG_LAUNCH_CHECK();
     return RG_SUCCESS;
 }

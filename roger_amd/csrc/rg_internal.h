@@ -109,14 +109,14 @@ int rg_launch_t1_encode(hipStream_t s, const int32_t *q, int pitch, size_t plane
 int rg_t1_job_bytes();
 void rg_t1_job_fill(void *dst, const int32_t *src, int pitch, int w, int h, int orient, uint8_t *out, int cap,
                     rg_cb_result *res, uint8_t *sym, int symcap, int4 *cm);
-int rg_launch_t1_queue(hipStream_t s, const void *jobs, int njobs, int *counter, int nwaves);
+int rg_launch_t1_jobs(hipStream_t s, const void *jobs, int njobs);
 // Split tier-1: context modelling writes per-block symbol streams (symcap
 // bytes each), then one lane per block runs the MQ coder.
 bool rg_t1_inline();
 int rg_t1_symcap(int cbw, int cbh);
 int rg_launch_t1_split(hipStream_t s, const int32_t *q, int pitch, size_t plane_stride, const rg_cb_desc *desc, int ncb,
                        uint8_t *syms, int symcap, int4 *cm, uint8_t *out, int slot_bytes, rg_cb_result *res);
-int rg_launch_t1_queue_split(hipStream_t s, const void *jobs, int njobs, int *counter, int nwaves);
+int rg_launch_t1_jobs_split(hipStream_t s, const void *jobs, int njobs);
 
 // Batched colour: per-image parameters; jobs {image, first chunk} cover 256
 // chunks of 16 pixels each.

@@ -1,0 +1,590 @@
+// EBCOT tier-1 encoder cores (T.800 Annex D) with the MQ coder (Annex C).
+//
+// Replaces the reference's bit-plane coder skeleton src/CL/oclbpc.cl:279-746
+// (launched per channel by src/OCLBPC.cpp:38-51), which stops after the
+// per-block MSB and has no context formation, MQ coder or output.
+//
+// Written against the wavefront primitives of wave_hip.h (RG_DEV, wv_*), so
+// the kernels in t1_encode.hip and the host lane emulator build the same code.
+//
+// Design (gfx950, wave64): one 64-lane wavefront per code-block (<= 64x64).
+// Lane j owns column j; every per-sample state (significance, visited,
+// refined, sign, current magnitude bit-plane) is a 64-bit row mask in
+// registers, so the 8-neighbourhood of a 4-row stripe is a 6-bit window of
+// the lane's own masks plus the windows of lanes j-1 / j+1 (one cross-lane
+// shuffle each).  Per stripe and pass:
+//   * significance propagation: a sample joins the pass if any neighbour is
+//     significant at its turn in scan order.  The only cross-lane dependency
+//     is "left column became significant earlier in this stripe"; it is
+//     resolved by a wave-wide fix-point (recompute with the left lane's
+//     newly-significant set until no lane changes, checked with wv_any), which
+//     converges in (longest left-to-right chain + 1) rounds, usually 1-2.
+//   * magnitude refinement: no intra-pass dependency.
+//   * cleanup: newly significant samples are known up front (bit == 1), so
+//     contexts and the run-length test come straight from the windows.
+// Each lane keeps its column's (context, bit) symbols for the stripe in two
+// 64-bit registers.  Two ways to MQ-code them:
+//   * inline: the coder walks the lanes in scan order (ballot mask, readlane)
+//     as wave-uniform scalar code whose context states and probability table
+//     sit in VGPR lanes;
+//   * split: the stripe's symbols are appended to a per-block stream, and a
+//     second kernel runs one MQ coder per LANE (64 blocks per wavefront).
+#pragma once
+#include <stdint.h>
+#include "../../include/roger_amd.h"
+
+namespace rg_t1 {
+
+// MQ probability table (T.800 Table C.2): Qe | NMPS << 16 | NLPS << 22 | SWITCH << 28.
+#define MQE(q, nm, nl, sw) ((uint32_t)(q) | ((uint32_t)(nm) << 16) | ((uint32_t)(nl) << 22) | ((uint32_t)(sw) << 28))
+RG_CONST uint32_t MQT[47] = {
+    MQE(0x5601, 1, 1, 1),   MQE(0x3401, 2, 6, 0),   MQE(0x1801, 3, 9, 0),   MQE(0x0AC1, 4, 12, 0),
+    MQE(0x0521, 5, 29, 0),  MQE(0x0221, 38, 33, 0), MQE(0x5601, 7, 6, 1),   MQE(0x5401, 8, 14, 0),
+    MQE(0x4801, 9, 14, 0),  MQE(0x3801, 10, 14, 0), MQE(0x3001, 11, 17, 0), MQE(0x2401, 12, 18, 0),
+    MQE(0x1C01, 13, 20, 0), MQE(0x1601, 29, 21, 0), MQE(0x5601, 15, 14, 1), MQE(0x5401, 16, 14, 0),
+    MQE(0x5101, 17, 15, 0), MQE(0x4801, 18, 16, 0), MQE(0x3801, 19, 17, 0), MQE(0x3401, 20, 18, 0),
+    MQE(0x3001, 21, 19, 0), MQE(0x2801, 22, 19, 0), MQE(0x2401, 23, 20, 0), MQE(0x2201, 24, 21, 0),
+    MQE(0x1C01, 25, 22, 0), MQE(0x1801, 26, 23, 0), MQE(0x1601, 27, 24, 0), MQE(0x1401, 28, 25, 0),
+    MQE(0x1201, 29, 26, 0), MQE(0x1101, 30, 27, 0), MQE(0x0AC1, 31, 28, 0), MQE(0x09C1, 32, 29, 0),
+    MQE(0x08A1, 33, 30, 0), MQE(0x0521, 34, 31, 0), MQE(0x0441, 35, 32, 0), MQE(0x02A1, 36, 33, 0),
+    MQE(0x0221, 37, 34, 0), MQE(0x0141, 38, 35, 0), MQE(0x0111, 39, 36, 0), MQE(0x0085, 40, 37, 0),
+    MQE(0x0049, 41, 38, 0), MQE(0x0025, 42, 39, 0), MQE(0x0015, 43, 40, 0), MQE(0x0009, 44, 41, 0),
+    MQE(0x0005, 45, 42, 0), MQE(0x0001, 45, 43, 0), MQE(0x5601, 46, 46, 0)};
+#undef MQE
+
+constexpr int CTX_RL = 17, CTX_UNI = 18;
+
+// ZC context (Table D.1), same rule as the oracle.
+RG_DEV int zc_ctx(int orient, int h, int v, int d)
+{
+    if (orient == 1) { int t = h; h = v; v = t; }
+    if (orient == 3) {
+        int hv = h + v;
+        if (d >= 3) return 8;
+        if (d == 2) return hv >= 1 ? 7 : 6;
+        if (d == 1) return hv >= 2 ? 5 : (hv == 1 ? 4 : 3);
+        return hv >= 2 ? 2 : hv;
+    }
+    if (h == 2) return 8;
+    if (h == 1) return v >= 1 ? 7 : (d >= 1 ? 6 : 5);
+    if (v == 2) return 4;
+    if (v == 1) return 3;
+    return d >= 2 ? 2 : d;
+}
+
+// Sign context (Table D.3): returns ctx << 1 | xorbit.
+RG_DEV int sc_ctx(int H, int V)
+{
+    int x = 0;
+    if (H < 0 || (H == 0 && V < 0)) { x = 1; H = -H; V = -V; }
+    int c = (H == 1) ? (V == 1 ? 13 : (V == 0 ? 12 : 11)) : (V == 1 ? 10 : 9);
+    return (c << 1) | x;
+}
+
+RG_DEV uint32_t win6(uint64_t m, int r0)
+{
+    // bits r0-1 .. r0+4 of m (bit 0 of the window = row r0-1)
+    return r0 == 0 ? (uint32_t)(m << 1) & 0x3Fu : (uint32_t)(m >> (r0 - 1)) & 0x3Fu;
+}
+
+RG_DEV uint32_t b1(uint32_t w, int i) { return (w >> i) & 1u; }
+
+struct SymList {
+    uint64_t lo = 0, hi = 0;
+    int n = 0;
+    RG_DEV void push(int cx, int bit)
+    {
+        uint64_t s = (uint64_t)((cx << 1) | bit);
+        if (n < 8) lo |= s << (8 * n); else hi |= s << (8 * (n - 8));
+        ++n;
+    }
+};
+
+// Sign symbol for window row i with neighbour windows: own column (cur state
+// above, pre state below), left (Lw), right (rw); sign windows negw/neglw/negrw.
+RG_DEV void push_sign(SymList &sl, int i, uint32_t cur, uint32_t sw, uint32_t Lw, uint32_t rw, uint32_t negw,
+                      uint32_t neglw, uint32_t negrw)
+{
+    int H = (b1(Lw, i) ? (b1(neglw, i) ? -1 : 1) : 0) + (b1(rw, i) ? (b1(negrw, i) ? -1 : 1) : 0);
+    int V = (b1(cur, i - 1) ? (b1(negw, i - 1) ? -1 : 1) : 0) + (b1(sw, i + 1) ? (b1(negw, i + 1) ? -1 : 1) : 0);
+    H = H > 1 ? 1 : (H < -1 ? -1 : H);
+    V = V > 1 ? 1 : (V < -1 ? -1 : V);
+    int sc = sc_ctx(H, V);
+    sl.push(sc >> 1, (int)b1(negw, i) ^ (sc & 1));
+}
+
+RG_DEV int zc_at(int orient, int i, uint32_t cur, uint32_t sw, uint32_t Lw, uint32_t rw)
+{
+    int h = (int)(b1(Lw, i) + b1(rw, i));
+    int v = (int)(b1(cur, i - 1) + b1(sw, i + 1));
+    int d = (int)(b1(Lw, i - 1) + b1(Lw, i + 1) + b1(rw, i - 1) + b1(rw, i + 1));
+    return zc_ctx(orient, h, v, d);
+}
+
+struct MQEnc {
+    uint32_t A, C, B;
+    int CT, bp;       // bp: index of the pending byte B (-1: the fake byte before the codeword)
+    int cap, overflow;
+    uint8_t *out;
+};
+
+RG_DEV void mq_store(MQEnc &m, int lane)
+{
+    if (m.bp >= 0) {
+        if (m.bp < m.cap) { if (lane == 0) m.out[m.bp] = (uint8_t)m.B; }
+        else m.overflow = 1;
+    }
+}
+
+// BYTEOUT (C.2.7).  B always holds one byte: a carry out of C (bit 27) is
+// added to B and then cleared from C before the next byte is taken.
+RG_DEV void mq_byteout(MQEnc &m, int lane)
+{
+    if (m.B == 0xFF) {
+        mq_store(m, lane); m.bp++; m.B = m.C >> 20; m.C &= 0xFFFFF; m.CT = 7;
+    } else if (m.C < 0x8000000u) {
+        mq_store(m, lane); m.bp++; m.B = m.C >> 19; m.C &= 0x7FFFF; m.CT = 8;
+    } else {
+        m.B++;
+        m.C &= 0x7FFFFFF;
+        if (m.B == 0xFF) {
+            mq_store(m, lane); m.bp++; m.B = m.C >> 20; m.C &= 0xFFFFF; m.CT = 7;
+        } else {
+            mq_store(m, lane); m.bp++; m.B = m.C >> 19; m.C &= 0x7FFFF; m.CT = 8;
+        }
+    }
+}
+
+RG_DEV void mq_renorm(MQEnc &m, int lane)
+{
+    // shift until bit 15 of A is set, emitting a byte whenever CT runs out
+    int n = wv_clz(m.A) - 16;
+    while (n > 0) {
+        int s = n < m.CT ? n : m.CT;
+        m.A <<= s; m.C <<= s; m.CT -= s; n -= s;
+        if (m.CT == 0) mq_byteout(m, lane);
+    }
+}
+
+// ENCODE (C.2.2-C.2.6) as wave-uniform scalar code.  The 19 context states
+// (I << 1 | MPS) live in lanes 0..18 of the VGPR `ctxv` and the probability
+// table in lanes 0..46 of `mqtv`; readlane with a uniform index (and a lane
+// select to write a state back) replaces every memory access of the coder.
+RG_DEV void mq_encode(MQEnc &m, int &ctxv, int mqtv, int cx, int d, int lane)
+{
+    const int st = wv_readlane(ctxv, cx);
+    const int mps = st & 1;
+    const uint32_t t = (uint32_t)wv_readlane(mqtv, st >> 1);
+    const uint32_t qe = t & 0xFFFF;
+    m.A -= qe;
+    if (d == mps) {
+        if (m.A & 0x8000) {
+            m.C += qe;
+            return;
+        }
+        if (m.A < qe) m.A = qe; else m.C += qe;
+        ctxv = lane == cx ? (int)((((t >> 16) & 63) << 1) | mps) : ctxv;
+    } else {
+        if (m.A < qe) m.C += qe; else m.A = qe;
+        const int nm = mps ^ (int)(t >> 28);
+        ctxv = lane == cx ? (int)((((t >> 22) & 63) << 1) | nm) : ctxv;
+    }
+    mq_renorm(m, lane);
+}
+
+// Codes one stripe's symbols in scan order: columns (lanes) left to right,
+// each lane's own list in order.  Lanes without symbols are skipped via the
+// ballot mask; a lane's list (<= 16 symbols) is read with four readlanes.
+RG_DEV void mq_stripe(MQEnc &m, int &ctxv, int mqtv, const SymList &sl, int lane)
+{
+    uint64_t mask = wv_ballot(sl.n > 0);
+    const int lo0 = (int)(uint32_t)sl.lo, lo1 = (int)(uint32_t)(sl.lo >> 32);
+    const int hi0 = (int)(uint32_t)sl.hi, hi1 = (int)(uint32_t)(sl.hi >> 32);
+    while (mask) {
+        const int j = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        const int n = wv_readlane(sl.n, j);
+        uint64_t a = ((uint64_t)(uint32_t)wv_readlane(lo1, j) << 32) | (uint32_t)wv_readlane(lo0, j);
+        uint64_t b = 0;
+        if (n > 8) b = ((uint64_t)(uint32_t)wv_readlane(hi1, j) << 32) | (uint32_t)wv_readlane(hi0, j);
+        for (int t = 0; t < n; ++t) {
+            const int sym = (int)(a & 0xFF);
+            a = (a >> 8) | (b << 56);
+            b >>= 8;
+            mq_encode(m, ctxv, mqtv, sym >> 1, sym & 1, lane);
+        }
+    }
+}
+
+// One code-block job: coefficients (row pitch), size, orientation, output slot.
+struct T1Job {
+    const int32_t *src;
+    int pitch, w, h, orient, cap;
+    uint8_t *out;
+    rg_cb_result *res;
+    uint8_t *sym;   // split path: this block's symbol stream (symcap bytes, 16-byte aligned)
+    int symcap;
+    i32x4 *cm;      // split path: {symbols, passes, bit-planes, overflow}
+};
+
+RG_DEV int wave_incl_scan(int v, int lane)
+{
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = wv_shfl_up(v, o);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Split path: appends one stripe's symbols (scan order) to the block's stream.
+RG_DEV void emit_stripe(const SymList &sl, const T1Job &job, int &pos, int &ovf, int lane)
+{
+    const int incl = wave_incl_scan(sl.n, lane);
+    const int total = wv_readlane(incl, 63);
+    if (total == 0) return;
+    if (pos + total > job.symcap) { ovf = 1; return; }
+    const int off = pos + incl - sl.n;
+    uint64_t a = sl.lo, b = sl.hi;
+    for (int t = 0; t < 16; ++t) {
+        if (!wv_any(t < sl.n)) break;
+        if (t < sl.n) job.sym[off + t] = (uint8_t)(a & 0xFF);
+        a = (a >> 8) | (b << 56);
+        b >>= 8;
+    }
+    pos += total;
+}
+
+// Encodes one code-block with the calling wavefront (64 lanes).  SPLIT:
+// context modelling only, the symbols go to job.sym for the lane MQ coder.
+template <bool SPLIT>
+RG_DEV void t1_block(const T1Job &job)
+{
+    const int lane = wv_lane();
+    const int w = job.w, h = job.h, orient = job.orient, pitch = job.pitch;
+    const int32_t *src = job.src;
+
+    // ---- load column `lane`: magnitudes in registers, sign as a row mask ----
+    uint32_t mag[64];
+    uint64_t NEG = 0;
+    uint32_t orm = 0;
+    const bool colok = lane < w;
+#pragma unroll
+    for (int r = 0; r < 64; ++r) {
+        int v = (colok && r < h) ? src[(size_t)r * pitch + lane] : 0;
+        uint32_t m = (uint32_t)(v < 0 ? -v : v);
+        mag[r] = m;
+        NEG |= (uint64_t)(v < 0) << r;
+        orm |= m;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) orm |= wv_shfl_xor(orm, o);
+    // the butterfly leaves the same value in every lane; saying so keeps the
+    // whole bit-plane loop (and the MQ coder) in scalar registers
+    const int nbp = wv_uniform(orm ? 32 - wv_clz(orm) : 0);
+    if (nbp == 0) {
+        if (lane == 0) {
+            if (SPLIT) *job.cm = mk_i32x4(0, 0, 0, 0);
+            else *job.res = rg_cb_result{0, 0, 0, 0};
+        }
+        return;
+    }
+    int spos = 0, sovf = 0;   // split path: stream position / overflow (uniform)
+
+    // initial context states (Table D.7): ZC ctx 0 -> state 4, RL -> 3, UNIFORM -> 46, others 0
+    int ctxv = lane == 0 ? (4 << 1) : lane == CTX_RL ? (3 << 1) : lane == CTX_UNI ? (46 << 1) : 0;
+    const int mqtv = lane < 47 ? (int)MQT[lane] : 0;
+
+    const uint64_t rowmask = h >= 64 ? ~0ull : ((1ull << h) - 1);
+    const uint64_t NEGL = wv_shfl_up(NEG, 1);
+    const uint64_t NEGR = wv_shfl_down(NEG, 1);
+    const uint64_t negl = lane == 0 ? 0 : NEGL;
+    const uint64_t negr = lane == 63 ? 0 : NEGR;
+
+    MQEnc m;
+    m.A = 0x8000; m.C = 0; m.B = 0; m.CT = 12; m.bp = -1;
+    m.cap = job.cap; m.overflow = 0;
+    m.out = job.out;
+
+    uint64_t S = 0, V = 0, R = 0;
+    const int nstripes = (h + 3) >> 2;
+
+    for (int p = nbp - 1; p >= 0; --p) {
+        uint32_t lo = 0, hi = 0;
+#pragma unroll
+        for (int r = 0; r < 32; ++r) lo |= ((mag[r] >> p) & 1u) << r;
+#pragma unroll
+        for (int r = 0; r < 32; ++r) hi |= ((mag[r + 32] >> p) & 1u) << r;
+        const uint64_t B = ((uint64_t)hi << 32) | lo;
+
+        for (int pass = (p == nbp - 1) ? 2 : 0; pass < 3; ++pass) {
+            for (int s = 0; s < nstripes; ++s) {
+                const int r0 = 4 * s;
+                const uint32_t vrow = colok ? (uint32_t)(rowmask >> r0) & 0xFu : 0u;
+                const uint32_t sw = win6(S, r0);
+                uint32_t lw = wv_shfl_up(sw, 1), rw = wv_shfl_down(sw, 1);
+                if (lane == 0) lw = 0;
+                if (lane == 63) rw = 0;
+                const uint32_t bw = win6(B, r0), negw = win6(NEG, r0);
+                const uint32_t neglw = win6(negl, r0), negrw = win6(negr, r0);
+                SymList sl;
+
+                if (pass == 0) {
+                    // ---- significance propagation ----
+                    // no significant sample in any column's 6-row window: nobody joins the pass
+                    if (!wv_any((sw & 0x3Fu) != 0)) continue;
+                    uint32_t N = 0, P = 0, Lw = lw;
+                    bool again = true;
+                    // monotone fix-point: converges within 65 rounds (chain across <= 64 lanes)
+                    for (int round = 0; again && round < 80; ++round) {
+                        uint32_t NL = wv_shfl_up(N, 1);
+                        if (lane == 0) NL = 0;
+                        Lw = lw | (NL << 1);
+                        uint32_t cur = sw, Nn = 0;
+                        P = 0;
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            const int i = k + 1;
+                            if (!((vrow >> k) & 1u) || b1(cur, i)) continue;
+                            uint32_t nb = b1(cur, i - 1) | b1(sw, i + 1) | ((Lw >> (i - 1)) & 7u) | ((rw >> (i - 1)) & 7u);
+                            if (nb) {
+                                P |= 1u << k;
+                                if (b1(bw, i)) { Nn |= 1u << k; cur |= 1u << i; }
+                            }
+                        }
+                        again = wv_any(Nn != N);
+                        N = Nn;
+                    }
+                    uint32_t cur = sw;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const int i = k + 1;
+                        if (!((P >> k) & 1u)) continue;
+                        const int bit = (int)b1(bw, i);
+                        sl.push(zc_at(orient, i, cur, sw, Lw, rw), bit);
+                        if (bit) {
+                            push_sign(sl, i, cur, sw, Lw, rw, negw, neglw, negrw);
+                            cur |= 1u << i;
+                        }
+                    }
+                    S |= (uint64_t)N << r0;
+                    V |= (uint64_t)P << r0;
+                } else if (pass == 1) {
+                    // ---- magnitude refinement ----
+                    const uint32_t vw = (uint32_t)(V >> r0) & 0xFu;
+                    const uint32_t rf = (uint32_t)(R >> r0) & 0xFu;
+                    const uint32_t M = (sw >> 1) & ~vw & vrow;
+                    if (!wv_any(M != 0)) continue;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const int i = k + 1;
+                        if (!((M >> k) & 1u)) continue;
+                        int cx;
+                        if ((rf >> k) & 1u) cx = 16;
+                        else {
+                            uint32_t nb = b1(sw, i - 1) | b1(sw, i + 1) | ((lw >> (i - 1)) & 7u) | ((rw >> (i - 1)) & 7u);
+                            cx = nb ? 15 : 14;
+                        }
+                        sl.push(cx, (int)b1(bw, i));
+                    }
+                    R |= (uint64_t)M << r0;
+                } else {
+                    // ---- cleanup ----
+                    const uint32_t vw = (uint32_t)(V >> r0) & 0xFu;
+                    const uint32_t Cm = ~(sw >> 1) & ~vw & vrow & 0xFu;
+                    const uint32_t N = Cm & (bw >> 1);
+                    uint32_t NL = wv_shfl_up(N, 1);
+                    if (lane == 0) NL = 0;
+                    const uint32_t Lw = lw | (NL << 1);
+                    const bool full = r0 + 4 <= h;
+                    const bool rl = full && Cm == 0xFu && (Lw & 0x3Fu) == 0 && (rw & 0x3Fu) == 0 && (sw & 0x21u) == 0;
+                    uint32_t cur = sw;
+                    int kstart = 0;
+                    bool done = false;
+                    if (rl) {
+                        if (N == 0) {
+                            sl.push(CTX_RL, 0);
+                            done = true;
+                        } else {
+                            const int k = __builtin_ctz(N);
+                            sl.push(CTX_RL, 1);
+                            sl.push(CTX_UNI, k >> 1);
+                            sl.push(CTX_UNI, k & 1);
+                            push_sign(sl, k + 1, cur, sw, Lw, rw, negw, neglw, negrw);
+                            cur |= 1u << (k + 1);
+                            kstart = k + 1;
+                        }
+                    }
+                    if (!done) {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            const int i = k + 1;
+                            if (k < kstart || !((Cm >> k) & 1u)) continue;
+                            const int bit = (int)((N >> k) & 1u);
+                            sl.push(zc_at(orient, i, cur, sw, Lw, rw), bit);
+                            if (bit) {
+                                push_sign(sl, i, cur, sw, Lw, rw, negw, neglw, negrw);
+                                cur |= 1u << i;
+                            }
+                        }
+                    }
+                    S |= (uint64_t)N << r0;
+                }
+
+                // ---- MQ-code this stripe's symbols in scan order (or hand them on) ----
+                if constexpr (SPLIT) emit_stripe(sl, job, spos, sovf, lane);
+                else mq_stripe(m, ctxv, mqtv, sl, lane);
+            }
+            if (pass == 2) V = 0;
+        }
+    }
+
+    if constexpr (SPLIT) {
+        if (lane == 0) *job.cm = mk_i32x4(spos, 3 * nbp - 2, nbp, sovf);
+        return;
+    }
+    // ---- FLUSH (C.2.9) ----
+    {
+        uint32_t tmp = m.C + m.A;
+        m.C |= 0xFFFF;
+        if (m.C >= tmp) m.C -= 0x8000;
+        m.C <<= m.CT; mq_byteout(m, lane);
+        m.C <<= m.CT; mq_byteout(m, lane);
+        int len;
+        if (m.B != 0xFF) { mq_store(m, lane); len = m.bp + 1; }
+        else len = m.bp;
+        if (len > job.cap) m.overflow = 1;
+        if (lane == 0) *job.res = rg_cb_result{len, 3 * nbp - 2, nbp, m.overflow ? 1 : 0};
+    }
+}
+
+// ---- split path, second half: one MQ coder per lane ----
+struct LaneMQ {
+    uint32_t A, C, B;
+    int CT, bp, ovf;
+};
+
+RG_DEV void lane_byteout(LaneMQ &m, uint8_t *out, int cap)
+{
+    // BYTEOUT (C.2.7) without branches on the byte value: a carry into B,
+    // then either 8 or 7 (after 0xFF, bit stuffing) bits move out of C.
+    const bool carry = m.B != 0xFF && m.C >= 0x8000000u;
+    const uint32_t b2 = m.B + (carry ? 1u : 0u);
+    if (carry) m.C &= 0x7FFFFFF;
+    const bool big = b2 == 0xFF;
+    if (m.bp >= 0) {
+        if (m.bp < cap) out[m.bp] = (uint8_t)b2;
+        else m.ovf = 1;
+    }
+    m.bp++;
+    m.B = big ? (m.C >> 20) : (m.C >> 19);
+    m.C &= big ? 0xFFFFFu : 0x7FFFFu;
+    m.CT = big ? 7 : 8;
+}
+
+// One ENCODE step (C.2.2-C.2.6) for this lane's block.  The 19 context states
+// are packed 4 per register (8 bits: I << 1 | MPS); the probability table is
+// read from `tab` (LDS).
+RG_DEV void lane_encode(LaneMQ &m, uint32_t (&cs)[5], const uint32_t *tab, uint32_t sym, uint8_t *out, int cap)
+{
+    const uint32_t cx = sym >> 1, d = sym & 1u;
+    const uint32_t wsel = cx >> 2, sh = (cx & 3u) * 8u;
+    uint32_t cw = cs[0];
+    cw = wsel == 1 ? cs[1] : cw;
+    cw = wsel == 2 ? cs[2] : cw;
+    cw = wsel == 3 ? cs[3] : cw;
+    cw = wsel == 4 ? cs[4] : cw;
+    const uint32_t st = (cw >> sh) & 0xFFu, mps = st & 1u;
+    const uint32_t t = tab[st >> 1];
+    const uint32_t qe = t & 0xFFFFu;
+    m.A -= qe;
+    const bool lps = d != mps;
+    if (!lps && (m.A & 0x8000u)) {
+        m.C += qe;
+        return;
+    }
+    const bool small = m.A < qe;
+    if (lps ? small : !small) m.C += qe;
+    else m.A = qe;
+    const uint32_t ns = lps ? ((((t >> 22) & 63u) << 1) | (mps ^ (t >> 28))) : ((((t >> 16) & 63u) << 1) | mps);
+    cw = (cw & ~(0xFFu << sh)) | (ns << sh);
+    cs[0] = wsel == 0 ? cw : cs[0];
+    cs[1] = wsel == 1 ? cw : cs[1];
+    cs[2] = wsel == 2 ? cw : cs[2];
+    cs[3] = wsel == 3 ? cw : cs[3];
+    cs[4] = wsel == 4 ? cw : cs[4];
+    int n = wv_clz(m.A) - 16;
+    while (n > 0) {
+        const int sft = n < m.CT ? n : m.CT;
+        m.A <<= sft; m.C <<= sft; m.CT -= sft; n -= sft;
+        if (m.CT == 0) lane_byteout(m, out, cap);
+    }
+}
+
+// MQ-codes one block per lane from the streams the context-modelling pass
+// wrote.  Block `i` (i >= njobs: idle lane) is either jobs[i] (QUEUE) or
+// entry i of the flat arrays.  `tab` is 48 words of workgroup-shared memory.
+template <bool QUEUE>
+RG_DEV void t1_mq_lanes(uint32_t *tab, const T1Job *jobs, int i, int njobs, const uint8_t *syms, int symcap,
+                        const i32x4 *cm, uint8_t *slots, int slot, rg_cb_result *res)
+{
+    const int lane = wv_lane();
+    if (lane < 47) tab[lane] = MQT[lane];
+    wv_sync();
+    const bool active = i < njobs;
+    i32x4 c = mk_i32x4(0, 0, 0, 0);
+    const uint8_t *src = nullptr;
+    uint8_t *out = nullptr;
+    rg_cb_result *dst = nullptr;
+    int cap = 0;
+    if (active) {
+        if (QUEUE) {
+            const T1Job &j = jobs[i];
+            c = *j.cm; src = j.sym; out = j.out; cap = j.cap; dst = j.res;
+        } else {
+            c = cm[i]; src = syms + (size_t)i * symcap; out = slots + (size_t)i * slot; cap = slot; dst = res + i;
+        }
+    }
+    const int n = c.x;
+    int nmax = n;
+    for (int o = 32; o >= 1; o >>= 1) {
+        const int t = wv_shfl_xor(nmax, o);
+        nmax = nmax > t ? nmax : t;
+    }
+    LaneMQ m;
+    m.A = 0x8000; m.C = 0; m.B = 0; m.CT = 12; m.bp = -1; m.ovf = 0;
+    // initial states (Table D.7): ctx 0 -> 4, RL (17) -> 3, UNIFORM (18) -> 46
+    uint32_t cs[5] = {4u << 1, 0, 0, 0, (3u << 1) << 8 | (46u << 1) << 16};
+    const u32x4 *s4 = reinterpret_cast<const u32x4 *>(src);
+    for (int base = 0; base < nmax; base += 16) {
+        u32x4 v;
+        v.x = v.y = v.z = v.w = 0;
+        if (base < n) v = s4[base >> 4];
+        const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if (base + k < n) lane_encode(m, cs, tab, (w4[k >> 2] >> (8 * (k & 3))) & 0xFFu, out, cap);
+    }
+    if (!active) return;
+    if (c.z == 0) {
+        *dst = rg_cb_result{0, 0, 0, 0};
+        return;
+    }
+    // FLUSH (C.2.9)
+    const uint32_t tmp = m.C + m.A;
+    m.C |= 0xFFFF;
+    if (m.C >= tmp) m.C -= 0x8000;
+    m.C <<= m.CT; lane_byteout(m, out, cap);
+    m.C <<= m.CT; lane_byteout(m, out, cap);
+    int len;
+    if (m.B != 0xFF) {
+        if (m.bp < cap) out[m.bp] = (uint8_t)m.B;
+        else m.ovf = 1;
+        len = m.bp + 1;
+    } else {
+        len = m.bp;
+    }
+    if (len > cap) m.ovf = 1;
+    *dst = rg_cb_result{len, c.y, c.z, (m.ovf || c.w) ? 1 : 0};
+}
+
+}  // namespace rg_t1

@@ -1,0 +1,31 @@
+// Wavefront primitives used by the tier-1 cores (t1_core.h), mapped onto the
+// gfx950 intrinsics.  The cores are written against these names only, so the
+// same source also builds for the host against a lock-step lane emulator
+// (tests/emu/wave_emu.h) that checks the kernels' logic on the CPU.
+//
+// Contract shared by both mappings: one wavefront of 64 lanes per workgroup
+// (the tier-1 kernels launch 64 threads), and every cross-lane call is made
+// by all 64 lanes at the same program point (wave-uniform control flow).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define RG_DEV __device__ __forceinline__
+#define RG_CONST __constant__
+
+using i32x4 = int4;
+using u32x4 = uint4;
+
+RG_DEV i32x4 mk_i32x4(int x, int y, int z, int w) { return make_int4(x, y, z, w); }
+RG_DEV int wv_lane() { return (int)threadIdx.x; }
+RG_DEV void wv_sync() { __syncthreads(); }
+RG_DEV int wv_clz(uint32_t x) { return __clz((int)x); }
+RG_DEV bool wv_any(bool p) { return __any(p); }
+RG_DEV uint64_t wv_ballot(bool p) { return __ballot(p); }
+// value of lane j (j uniform)
+RG_DEV int wv_readlane(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
+// a value the caller knows to be wave-uniform, moved to a scalar register
+RG_DEV int wv_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+template <class T> RG_DEV T wv_shfl_up(T v, int d) { return __shfl_up(v, d, 64); }
+template <class T> RG_DEV T wv_shfl_down(T v, int d) { return __shfl_down(v, d, 64); }
+template <class T> RG_DEV T wv_shfl_xor(T v, int m) { return __shfl_xor(v, m, 64); }

@@ -1,0 +1,235 @@
+// Runs the tier-1 encoder cores (roger_amd/csrc/t1_core.h) on the CPU under
+// the lock-step wavefront emulator of wave_emu.h, so the kernels' own code
+// (not a restatement) can be checked against the oracle without a GPU.
+//
+//   t1_emu MODE DIR PITCH PLANE_STRIDE SLOT
+//     MODE  inline | split | queue_split
+//     DIR   holds q.bin (int32 coefficient planes) and desc.bin (rg_cb_desc
+//           records); res.bin (rg_cb_result per block) and out.bin (SLOT
+//           bytes per block) are written there.
+// The three modes drive the cores the way t1_encode.hip's kernels do:
+//   inline       t1_block<false>, one wave per block
+//   split        t1_block<true> per block, then t1_mq_lanes<false> per 64 blocks
+//   queue_split  the same over a job list (t1_mq_lanes<true>)
+// Exit status 3: an emulator check failed (message on stderr).
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "wave_emu.h"
+#include "../../roger_amd/csrc/t1_core.h"
+
+using namespace rg_t1;
+
+// ---- fiber switch (x86-64 System V: callee-saved registers on the stack) ----
+extern "C" void emu_swap(void **save_sp, void *to_sp);
+asm(R"(
+    .text
+    .globl emu_swap
+    .type emu_swap, @function
+emu_swap:
+    pushq %rbp
+    pushq %rbx
+    pushq %r12
+    pushq %r13
+    pushq %r14
+    pushq %r15
+    movq %rsp, (%rdi)
+    movq %rsi, %rsp
+    popq %r15
+    popq %r14
+    popq %r13
+    popq %r12
+    popq %rbx
+    popq %rbp
+    ret
+    .size emu_swap, .-emu_swap
+)");
+
+namespace emu {
+namespace {
+constexpr size_t STACK = 1 << 18;
+void *sched_sp;
+void *lane_sp[64];
+char *stacks[64];
+int cur = 0;
+bool done[64];
+uint64_t buf[2][64];
+int phase[64];
+void *site[64];
+int want_uniform[64];
+void (*body)(void *) = nullptr;
+void *body_arg = nullptr;
+
+[[noreturn]] void fail(const char *msg)
+{
+    fprintf(stderr, "wave emulator: %s\n", msg);
+    exit(3);
+}
+
+void entry()
+{
+    body(body_arg);
+    done[cur] = true;
+    emu_swap(&lane_sp[cur], sched_sp);
+    fail("finished lane resumed");
+}
+}  // namespace
+
+int lane() { return cur; }
+
+__attribute__((noinline)) const uint64_t *exchange(uint64_t v, int check_uniform)
+{
+    const int L = cur, ph = phase[L];
+    buf[ph][L] = v;
+    phase[L] ^= 1;
+    site[L] = __builtin_return_address(0);
+    want_uniform[L] = check_uniform;
+    emu_swap(&lane_sp[L], sched_sp);
+    return buf[ph];
+}
+
+// Runs fn(arg) as one 64-lane wavefront to completion.
+void run(void (*fn)(void *), void *arg)
+{
+    body = fn;
+    body_arg = arg;
+    for (int l = 0; l < 64; ++l) {
+        if (!stacks[l]) stacks[l] = static_cast<char *>(aligned_alloc(16, STACK));
+        const uintptr_t top = reinterpret_cast<uintptr_t>(stacks[l] + STACK) & ~uintptr_t(15);
+        void **sp = reinterpret_cast<void **>(top - 64);
+        for (int k = 0; k < 8; ++k) sp[k] = nullptr;
+        sp[6] = reinterpret_cast<void *>(&entry);   // popped by `ret` after the six registers
+        lane_sp[l] = sp;
+        done[l] = false;
+        phase[l] = 0;
+    }
+    for (;;) {
+        for (int l = 0; l < 64; ++l) {
+            if (done[l]) continue;
+            cur = l;
+            emu_swap(&sched_sp, lane_sp[l]);
+        }
+        int ndone = 0;
+        for (int l = 0; l < 64; ++l) ndone += done[l];
+        if (ndone == 64) return;
+        if (ndone) fail("a lane left the kernel while the others wait at a cross-lane call");
+        for (int l = 1; l < 64; ++l)
+            if (site[l] != site[0]) fail("lanes reached different cross-lane call sites (non-uniform control flow)");
+        if (want_uniform[0]) {
+            const int ph = phase[0] ^ 1;
+            for (int l = 1; l < 64; ++l)
+                if (buf[ph][l] != buf[ph][0]) fail("a value declared wave-uniform differs between lanes");
+        }
+    }
+}
+}  // namespace emu
+
+namespace {
+uint32_t g_tab[48];
+
+void inline_body(void *a) { t1_block<false>(*static_cast<T1Job *>(a)); }
+void cm_body(void *a) { t1_block<true>(*static_cast<T1Job *>(a)); }
+
+struct MQFlat {
+    int i0, ncb, symcap, slot;
+    const uint8_t *syms;
+    const i32x4 *cm;
+    uint8_t *slots;
+    rg_cb_result *res;
+};
+void mq_flat_body(void *a)
+{
+    const MQFlat &m = *static_cast<MQFlat *>(a);
+    t1_mq_lanes<false>(g_tab, nullptr, m.i0 + wv_lane(), m.ncb, m.syms, m.symcap, m.cm, m.slots, m.slot, m.res);
+}
+
+struct MQQueue {
+    const T1Job *jobs;
+    int i0, njobs;
+};
+void mq_queue_body(void *a)
+{
+    const MQQueue &m = *static_cast<MQQueue *>(a);
+    t1_mq_lanes<true>(g_tab, m.jobs, m.i0 + wv_lane(), m.njobs, nullptr, 0, nullptr, nullptr, 0, nullptr);
+}
+
+std::vector<char> slurp(const std::string &path)
+{
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) { fprintf(stderr, "cannot open %s\n", path.c_str()); exit(2); }
+    fseek(f, 0, SEEK_END);
+    const long n = ftell(f);
+    fseek(f, 0, SEEK_SET);
+    std::vector<char> v((size_t)n);
+    if (n && fread(v.data(), 1, (size_t)n, f) != (size_t)n) { fprintf(stderr, "short read %s\n", path.c_str()); exit(2); }
+    fclose(f);
+    return v;
+}
+
+void spill(const std::string &path, const void *p, size_t n)
+{
+    FILE *f = fopen(path.c_str(), "wb");
+    if (!f || fwrite(p, 1, n, f) != n) { fprintf(stderr, "cannot write %s\n", path.c_str()); exit(2); }
+    fclose(f);
+}
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    if (argc != 6) {
+        fprintf(stderr, "usage: t1_emu inline|split|queue_split DIR PITCH PLANE_STRIDE SLOT\n");
+        return 2;
+    }
+    const std::string mode = argv[1], dir = argv[2];
+    const int pitch = atoi(argv[3]);
+    const long plane_stride = atol(argv[4]);
+    const int slot = atoi(argv[5]);
+    std::vector<char> qraw = slurp(dir + "/q.bin"), draw = slurp(dir + "/desc.bin");
+    const int32_t *q = reinterpret_cast<const int32_t *>(qraw.data());
+    const int ncb = (int)(draw.size() / sizeof(rg_cb_desc));
+    const rg_cb_desc *desc = reinterpret_cast<const rg_cb_desc *>(draw.data());
+
+    const int symcap = ((64 * 64 * 24) + 15) / 16 * 16;
+    std::vector<uint8_t> slots((size_t)slot * ncb, 0);
+    std::vector<rg_cb_result> res(ncb);
+    uint8_t *syms = static_cast<uint8_t *>(aligned_alloc(16, (size_t)symcap * (ncb ? ncb : 1)));
+    std::vector<i32x4> cm(ncb);
+    std::vector<T1Job> jobs(ncb);
+    for (int k = 0; k < ncb; ++k) {
+        const rg_cb_desc &d = desc[k];
+        T1Job &j = jobs[k];
+        j.src = q + (size_t)d.comp * plane_stride + (size_t)d.y0 * pitch + d.x0;
+        j.pitch = pitch; j.w = d.w; j.h = d.h; j.orient = d.orient; j.cap = slot;
+        j.out = slots.data() + (size_t)k * slot;
+        j.res = &res[k];
+        j.sym = syms + (size_t)k * symcap;
+        j.symcap = symcap;
+        j.cm = &cm[k];
+    }
+    if (mode == "inline") {
+        for (int k = 0; k < ncb; ++k) emu::run(inline_body, &jobs[k]);
+    } else if (mode == "split" || mode == "queue_split") {
+        for (int k = 0; k < ncb; ++k) emu::run(cm_body, &jobs[k]);
+        for (int i0 = 0; i0 < ncb; i0 += 64) {
+            if (mode == "split") {
+                MQFlat m{i0, ncb, symcap, slot, syms, cm.data(), slots.data(), res.data()};
+                emu::run(mq_flat_body, &m);
+            } else {
+                MQQueue m{jobs.data(), i0, ncb};
+                emu::run(mq_queue_body, &m);
+            }
+        }
+    } else {
+        fprintf(stderr, "unknown mode %s\n", mode.c_str());
+        return 2;
+    }
+    spill(dir + "/res.bin", res.data(), sizeof(rg_cb_result) * ncb);
+    spill(dir + "/out.bin", slots.data(), slots.size());
+    free(syms);
+    return 0;
+}

@@ -1,0 +1,72 @@
+// Host lock-step emulator of one 64-lane wavefront, for running the tier-1
+// cores (roger_amd/csrc/t1_core.h) on the CPU.
+//
+// Every lane is a fiber on its own stack.  A cross-lane call (shuffle, any,
+// ballot, readlane, barrier) deposits the lane's value and yields; once all
+// 64 lanes have arrived the scheduler resumes them in order and each reads the
+// deposited values.  Deposits alternate between two buffers, so a lane that
+// runs ahead to its next cross-lane call never overwrites values that slower
+// lanes still have to read.
+//
+// Checks made on every cross-lane call (any failure aborts with a message):
+//   * all live lanes reach the same call site (wave-uniform control flow),
+//   * no lane leaves the kernel while others wait at a cross-lane call,
+//   * a value passed to wv_uniform is the same in every lane.
+#pragma once
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#define RG_DEV inline
+#define RG_CONST static const
+
+struct i32x4 {
+    int x, y, z, w;
+};
+struct alignas(16) u32x4 {
+    uint32_t x, y, z, w;
+};
+static inline i32x4 mk_i32x4(int x, int y, int z, int w) { return i32x4{x, y, z, w}; }
+
+namespace emu {
+int lane();
+// deposits v for the current lane, waits for the wave; returns the 64 deposits
+const uint64_t *exchange(uint64_t v, int check_uniform);
+}  // namespace emu
+
+static inline int wv_lane() { return emu::lane(); }
+static inline void wv_sync() { emu::exchange(0, 0); }
+static inline int wv_clz(uint32_t x) { return x ? __builtin_clz(x) : 32; }
+static inline bool wv_any(bool p)
+{
+    const uint64_t *v = emu::exchange(p, 0);
+    for (int l = 0; l < 64; ++l)
+        if (v[l]) return true;
+    return false;
+}
+static inline uint64_t wv_ballot(bool p)
+{
+    const uint64_t *v = emu::exchange(p, 0);
+    uint64_t m = 0;
+    for (int l = 0; l < 64; ++l) m |= (uint64_t)(v[l] != 0) << l;
+    return m;
+}
+static inline int wv_readlane(int x, int j) { return (int)(uint32_t)emu::exchange((uint32_t)x, 0)[j]; }
+static inline int wv_uniform(int x) { return (int)(uint32_t)emu::exchange((uint32_t)x, 1)[0]; }
+template <class T> static inline T wv_shfl_up(T x, int d)
+{
+    const uint64_t *v = emu::exchange((uint64_t)x, 0);
+    const int s = emu::lane() - d;
+    return s >= 0 ? (T)v[s] : x;
+}
+template <class T> static inline T wv_shfl_down(T x, int d)
+{
+    const uint64_t *v = emu::exchange((uint64_t)x, 0);
+    const int s = emu::lane() + d;
+    return s < 64 ? (T)v[s] : x;
+}
+template <class T> static inline T wv_shfl_xor(T x, int m)
+{
+    const uint64_t *v = emu::exchange((uint64_t)x, 0);
+    return (T)v[emu::lane() ^ m];
+}

@@ -1,5 +1,5 @@
 """GPU: the batched encode path (one launch per stage over many images of
-uneven sizes, HIP-graph replay, persistent tier-1 work queue) produces, for
+uneven sizes, HIP-graph replay, tier-1 job list sorted largest first) produces, for
 every image, exactly the code-blocks the CPU oracle produces; and the device
 image generator reproduces the numpy generator bit-exactly."""
 import numpy as np
@@ -34,7 +34,7 @@ def test_batch_matches_oracle(ctx, use_graph, cb):
         imgs.append(dict(buf=buf, w=w, h=h, ncomp=nc, reversible=rev, levels=lv, delta=1.0 / 64,
                          host=synth.image(w, h, nc, 50 + i)))
     b = R.Batch(ctx)
-    b.plan(imgs, cbw=cb, cbh=cb, use_graph=use_graph, waves_per_cu=4)
+    b.plan(imgs, cbw=cb, cbh=cb, use_graph=use_graph)
     for rep in range(2):  # the second run replays the plan (counter reset inside)
         b.run()
         ctx.sync()

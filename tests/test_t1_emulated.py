@@ -1,0 +1,101 @@
+"""CPU: the tier-1 kernels' own code (roger_amd/csrc/t1_core.h, built for the
+host against the lock-step wavefront emulator in tests/emu/) produces exactly
+the oracle's code-block bytes, pass counts and bit-plane counts, in all three
+ways the kernels drive it (inline MQ, split with flat arrays, split over a job
+list).  The emulator also fails a run whose cross-lane calls are not
+wave-uniform or whose declared-uniform values differ between lanes."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from roger_amd import synth
+from roger_amd.runtime import CB_DESC_DTYPE, CB_RES_DTYPE
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EMU_SRC = os.path.join(HERE, "emu", "t1_emu.cpp")
+MODES = ["inline", "split", "queue_split"]
+
+
+@pytest.fixture(scope="module")
+def emu(tmp_path_factory):
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    exe = str(tmp_path_factory.mktemp("emu") / "t1_emu")
+    subprocess.run([gxx, "-O2", "-std=c++17", "-ffp-contract=off", "-Wno-unknown-pragmas", EMU_SRC, "-o", exe],
+                   check=True)
+    return exe
+
+
+def _run(emu, tmp_path, mode, q, desc, pitch, plane_stride, slot):
+    d = str(tmp_path)
+    np.ascontiguousarray(q, np.int32).tofile(os.path.join(d, "q.bin"))
+    np.ascontiguousarray(desc, CB_DESC_DTYPE).tofile(os.path.join(d, "desc.bin"))
+    r = subprocess.run([emu, mode, d, str(pitch), str(plane_stride), str(slot)], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    res = np.fromfile(os.path.join(d, "res.bin"), CB_RES_DTYPE)
+    out = np.fromfile(os.path.join(d, "out.bin"), np.uint8).reshape(len(desc), slot)
+    return res, [bytes(out[i, :res["len"][i]]) for i in range(len(desc))]
+
+
+def _image_case(w, h, nc, rev, levels, seed, cb):
+    img = synth.image(w, h, nc, seed)
+    rs, data, q = O.encode_image(img, levels=levels, reversible=rev, cbw=cb, cbh=cb, delta=1.0 / 64)
+    one = O.codeblocks(w, h, levels, cb, cb)
+    desc = np.zeros(len(one) * nc, CB_DESC_DTYPE)
+    k = 0
+    for c in range(nc):
+        for d in one:
+            desc[k] = (d["x0"], d["y0"], d["w"], d["h"], c, d["orient"], d["band"], d["res"])
+            k += 1
+    return q, desc, rs, data
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("w,h,nc,rev,levels,seed,cb", [
+    (97, 61, 1, False, 3, 52, 64),    # 9/7, ragged bands
+    (33, 1, 1, False, 1, 56, 64),     # one-row image: empty high bands, 1-row blocks
+    (64, 64, 3, False, 2, 54, 32),    # RGB, 32x32 blocks
+    (129, 57, 1, True, 4, 55, 64),    # 5/3, odd sizes
+])
+def test_image_blocks_match_oracle(emu, tmp_path, mode, w, h, nc, rev, levels, seed, cb):
+    q, desc, rs, data = _image_case(w, h, nc, rev, levels, seed, cb)
+    res, blobs = _run(emu, tmp_path, mode, q, desc, w, w * h, max(256, cb * cb * 3))
+    assert (res["flags"] == 0).all()
+    np.testing.assert_array_equal(res["nbp"], rs[:, 2])
+    np.testing.assert_array_equal(res["npasses"], rs[:, 1])
+    assert blobs == data
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_dense_blocks_match_oracle(emu, tmp_path, mode):
+    """Full 64x64 blocks of wide-range Laplacian-like values (many bit-planes,
+    long codewords, frequent carries into the byte register) plus sparse and
+    single-sample blocks, all four orientations."""
+    rng = np.random.default_rng(11)
+    blocks = []
+    for k in range(6):
+        scale = [3, 40, 900, 20000, 5, 250][k]
+        b = np.round(rng.laplace(0, scale, (64, 64))).astype(np.int32)
+        if k == 4:
+            b[rng.random((64, 64)) < 0.97] = 0          # sparse: run-length paths
+        blocks.append(b)
+    one = np.zeros((64, 64), np.int32)
+    one[37, 21] = -5
+    blocks.append(one)
+    plane = np.concatenate(blocks, axis=1)               # 64 x (64 * nblocks)
+    n = len(blocks)
+    desc = np.zeros(n, CB_DESC_DTYPE)
+    for i in range(n):
+        desc[i] = (64 * i, 0, 64, 64, 0, i % 4, 0, 0)
+    res, blobs = _run(emu, tmp_path, mode, plane, desc, plane.shape[1], plane.size, 64 * 64 * 4)
+    for i, b in enumerate(blocks):
+        data, npasses, nbp = O.t1_encode(b, i % 4)
+        assert res["flags"][i] == 0
+        assert (res["nbp"][i], res["npasses"][i]) == (nbp, npasses), i
+        assert blobs[i] == data, i
