@@ -154,6 +154,9 @@ int rg_codeblocks(int w, int h, int levels, int cbw, int cbh, rg_cb_desc *out, i
 /* EBCOT tier-1 encode of ncb code-blocks (descriptors in device memory)
  * over int32 coefficient planes (plane c at d_q + c*plane_stride, row
  * pitch `pitch`).  Each block's codeword goes to d_out + i*slot_bytes.
+ * With d_out 16-byte aligned and slot_bytes a multiple of 16 the split
+ * tier-1 (lane-parallel MQ coder) runs, otherwise the single-kernel one;
+ * both give the same bytes.
  * Replaces OCLBPC::run (src/OCLBPC.cpp:38-51, src/CL/oclbpc.cl:279-746). */
 int rg_t1_encode(rg_context *ctx, const int32_t *d_q, int pitch, size_t plane_stride,
                  const rg_cb_desc *d_desc, int ncb, uint8_t *d_out, int slot_bytes,

@@ -353,7 +353,8 @@ extern "C" int rg_t1_encode(rg_context *c, const int32_t *dq, int pitch, size_t 
 {
     RG_CHECK(c && dq && desc && out && res, "rg_t1_encode: null argument");
     RG_CHECK(ncb >= 0 && slot_bytes > 0 && pitch > 0, "rg_t1_encode: bad sizes");
-    if (rg_t1_inline() || ncb == 0)
+    // the lane MQ coder writes 16-byte groups: it needs 16-byte aligned slots
+    if (rg_t1_inline() || ncb == 0 || (slot_bytes & 15) != 0 || ((uintptr_t)out & 15) != 0)
         return rg_launch_t1_encode(c->stream, dq, pitch, plane_stride, desc, ncb, out, slot_bytes, res);
     // stream-ordered scratch for the symbol streams (cbw, cbh <= 64)
     const int symcap = rg_t1_symcap(64, 64);

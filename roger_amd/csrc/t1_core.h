@@ -458,10 +458,38 @@ RG_DEV void t1_block(const T1Job &job)
 }
 
 // ---- split path, second half: one MQ coder per lane ----
+//
+// Memory traffic is kept off the coder's dependency chain: symbols arrive 16
+// at a time from an unconditional (index-clamped) 16-byte load issued one
+// chunk ahead, and codeword bytes collect in four registers that leave as one
+// 16-byte store per 16 bytes.  (A byte store per output byte would make every
+// later wait on the symbol load also wait for that store.)  Output slots must
+// be 16-byte aligned with a capacity that is a multiple of 16.
 struct LaneMQ {
     uint32_t A, C, B;
     int CT, bp, ovf;
+    uint32_t o0, o1, o2, o3;   // codeword bytes [bp & ~15, bp) not yet stored
 };
+
+// Places byte v at codeword position pos (bytes at or beyond cap only set the
+// overflow flag); a completed 16-byte group is stored.
+RG_DEV void lane_put(LaneMQ &m, uint32_t v, int pos, uint8_t *out, int cap)
+{
+    if (pos >= cap) {
+        m.ovf = 1;
+        return;
+    }
+    const uint32_t x = v << ((pos & 3) * 8);
+    const int w = (pos >> 2) & 3;
+    m.o0 |= w == 0 ? x : 0u;
+    m.o1 |= w == 1 ? x : 0u;
+    m.o2 |= w == 2 ? x : 0u;
+    m.o3 |= w == 3 ? x : 0u;
+    if ((pos & 15) == 15) {
+        wv_store16(out + (pos & ~15), m.o0, m.o1, m.o2, m.o3);
+        m.o0 = m.o1 = m.o2 = m.o3 = 0;
+    }
+}
 
 RG_DEV void lane_byteout(LaneMQ &m, uint8_t *out, int cap)
 {
@@ -471,19 +499,16 @@ RG_DEV void lane_byteout(LaneMQ &m, uint8_t *out, int cap)
     const uint32_t b2 = m.B + (carry ? 1u : 0u);
     if (carry) m.C &= 0x7FFFFFF;
     const bool big = b2 == 0xFF;
-    if (m.bp >= 0) {
-        if (m.bp < cap) out[m.bp] = (uint8_t)b2;
-        else m.ovf = 1;
-    }
+    if (m.bp >= 0) lane_put(m, b2, m.bp, out, cap);
     m.bp++;
     m.B = big ? (m.C >> 20) : (m.C >> 19);
     m.C &= big ? 0xFFFFFu : 0x7FFFFu;
     m.CT = big ? 7 : 8;
 }
 
-// One ENCODE step (C.2.2-C.2.6) for this lane's block.  The 19 context states
-// are packed 4 per register (8 bits: I << 1 | MPS); the probability table is
-// read from `tab` (LDS).
+// One ENCODE step (C.2.2-C.2.6) for this lane's block, branch-free except for
+// the renormalisation loop.  The 19 context states are packed 4 per register
+// (8 bits: I << 1 | MPS); the probability table is read from `tab` (LDS).
 RG_DEV void lane_encode(LaneMQ &m, uint32_t (&cs)[5], const uint32_t *tab, uint32_t sym, uint8_t *out, int cap)
 {
     const uint32_t cx = sym >> 1, d = sym & 1u;
@@ -496,17 +521,17 @@ RG_DEV void lane_encode(LaneMQ &m, uint32_t (&cs)[5], const uint32_t *tab, uint3
     const uint32_t st = (cw >> sh) & 0xFFu, mps = st & 1u;
     const uint32_t t = tab[st >> 1];
     const uint32_t qe = t & 0xFFFFu;
-    m.A -= qe;
+    const uint32_t a = m.A - qe;
     const bool lps = d != mps;
-    if (!lps && (m.A & 0x8000u)) {
-        m.C += qe;
-        return;
-    }
-    const bool small = m.A < qe;
-    if (lps ? small : !small) m.C += qe;
-    else m.A = qe;
-    const uint32_t ns = lps ? ((((t >> 22) & 63u) << 1) | (mps ^ (t >> 28))) : ((((t >> 16) & 63u) << 1) | mps);
-    cw = (cw & ~(0xFFu << sh)) | (ns << sh);
+    const bool renorm = lps || !(a & 0x8000u);
+    // C.2.4 / C.2.5: the interval kept is the upper one (C += Qe) for an MPS
+    // unless A < Qe (conditional exchange), and the reverse for an LPS
+    const bool small = a < qe;
+    const bool upper = !renorm || (lps ? small : !small);
+    m.C += upper ? qe : 0u;
+    m.A = upper ? a : qe;
+    const uint32_t nst = lps ? ((((t >> 22) & 63u) << 1) | (mps ^ (t >> 28))) : ((((t >> 16) & 63u) << 1) | mps);
+    cw = (cw & ~(0xFFu << sh)) | ((renorm ? nst : st) << sh);
     cs[0] = wsel == 0 ? cw : cs[0];
     cs[1] = wsel == 1 ? cw : cs[1];
     cs[2] = wsel == 2 ? cw : cs[2];
@@ -532,7 +557,8 @@ RG_DEV void t1_mq_lanes(uint32_t *tab, const T1Job *jobs, int i, int njobs, cons
     wv_sync();
     const bool active = i < njobs;
     i32x4 c = mk_i32x4(0, 0, 0, 0);
-    const uint8_t *src = nullptr;
+    // idle lanes read block 0's stream (always allocated) and store nothing
+    const uint8_t *src = QUEUE ? jobs[0].sym : syms;
     uint8_t *out = nullptr;
     rg_cb_result *dst = nullptr;
     int cap = 0;
@@ -552,14 +578,16 @@ RG_DEV void t1_mq_lanes(uint32_t *tab, const T1Job *jobs, int i, int njobs, cons
     }
     LaneMQ m;
     m.A = 0x8000; m.C = 0; m.B = 0; m.CT = 12; m.bp = -1; m.ovf = 0;
+    m.o0 = m.o1 = m.o2 = m.o3 = 0;
     // initial states (Table D.7): ctx 0 -> 4, RL (17) -> 3, UNIFORM (18) -> 46
     uint32_t cs[5] = {4u << 1, 0, 0, 0, (3u << 1) << 8 | (46u << 1) << 16};
-    const u32x4 *s4 = reinterpret_cast<const u32x4 *>(src);
+    const int last = n > 0 ? (n - 1) >> 4 : 0;   // last chunk index with symbols
+    u32x4 nxt = wv_load16(src);
     for (int base = 0; base < nmax; base += 16) {
-        u32x4 v;
-        v.x = v.y = v.z = v.w = 0;
-        if (base < n) v = s4[base >> 4];
-        const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+        const u32x4 cur = nxt;
+        const int k1 = (base >> 4) + 1;
+        nxt = wv_load16(src + 16 * (k1 < last ? k1 : last));
+        const uint32_t w4[4] = {cur.x, cur.y, cur.z, cur.w};
 #pragma unroll
         for (int k = 0; k < 16; ++k)
             if (base + k < n) lane_encode(m, cs, tab, (w4[k >> 2] >> (8 * (k & 3))) & 0xFFu, out, cap);
@@ -577,12 +605,12 @@ RG_DEV void t1_mq_lanes(uint32_t *tab, const T1Job *jobs, int i, int njobs, cons
     m.C <<= m.CT; lane_byteout(m, out, cap);
     int len;
     if (m.B != 0xFF) {
-        if (m.bp < cap) out[m.bp] = (uint8_t)m.B;
-        else m.ovf = 1;
+        lane_put(m, m.B, m.bp, out, cap);
         len = m.bp + 1;
     } else {
         len = m.bp;
     }
+    if ((len & 15) != 0 && (len & ~15) < cap) wv_store16(out + (len & ~15), m.o0, m.o1, m.o2, m.o3);
     if (len > cap) m.ovf = 1;
     *dst = rg_cb_result{len, c.y, c.z, (m.ovf || c.w) ? 1 : 0};
 }

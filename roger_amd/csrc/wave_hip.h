@@ -29,3 +29,19 @@ RG_DEV int wv_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 template <class T> RG_DEV T wv_shfl_up(T v, int d) { return __shfl_up(v, d, 64); }
 template <class T> RG_DEV T wv_shfl_down(T v, int d) { return __shfl_down(v, d, 64); }
 template <class T> RG_DEV T wv_shfl_xor(T v, int m) { return __shfl_xor(v, m, 64); }
+
+// 16-byte global-memory load / store (p 16-byte aligned).  The explicit global
+// address space gives global_load/store (vector-memory counter only) where a
+// generic pointer would give flat instructions, whose completion every later
+// LDS wait would also have to cover.
+typedef unsigned int rg_v4u __attribute__((ext_vector_type(4)));
+RG_DEV u32x4 wv_load16(const uint8_t *p)
+{
+    const rg_v4u v = *(const __attribute__((address_space(1))) rg_v4u *)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+RG_DEV void wv_store16(uint8_t *p, uint32_t a, uint32_t b, uint32_t c, uint32_t d)
+{
+    const rg_v4u v = {a, b, c, d};
+    *(__attribute__((address_space(1))) rg_v4u *)p = v;
+}

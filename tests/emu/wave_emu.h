@@ -16,6 +16,7 @@
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
 #define RG_DEV inline
 #define RG_CONST static const
@@ -27,6 +28,19 @@ struct alignas(16) u32x4 {
     uint32_t x, y, z, w;
 };
 static inline i32x4 mk_i32x4(int x, int y, int z, int w) { return i32x4{x, y, z, w}; }
+static inline u32x4 wv_load16(const uint8_t *p)
+{
+    if (((uintptr_t)p & 15) != 0) { fprintf(stderr, "wave emulator: misaligned 16-byte load\n"); exit(3); }
+    u32x4 v;
+    memcpy(&v, p, 16);
+    return v;
+}
+static inline void wv_store16(uint8_t *p, uint32_t a, uint32_t b, uint32_t c, uint32_t d)
+{
+    if (((uintptr_t)p & 15) != 0) { fprintf(stderr, "wave emulator: misaligned 16-byte store\n"); exit(3); }
+    const u32x4 v{a, b, c, d};
+    memcpy(p, &v, 16);
+}
 
 namespace emu {
 int lane();

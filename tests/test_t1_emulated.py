@@ -99,3 +99,25 @@ def test_dense_blocks_match_oracle(emu, tmp_path, mode):
         assert res["flags"][i] == 0
         assert (res["nbp"][i], res["npasses"][i]) == (nbp, npasses), i
         assert blobs[i] == data, i
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_small_slots_flag_overflow(emu, tmp_path, mode):
+    """Slots smaller than the codeword: the block is flagged, nothing is
+    written past its slot, and blocks that fit are still exact."""
+    rng = np.random.default_rng(5)
+    blocks = [np.round(rng.laplace(0, s, (64, 64))).astype(np.int32) for s in (0.4, 1.0, 30, 2000)]
+    plane = np.concatenate(blocks, axis=1)
+    desc = np.zeros(len(blocks), CB_DESC_DTYPE)
+    for i in range(len(blocks)):
+        desc[i] = (64 * i, 0, 64, 64, 0, 0, 0, 0)
+    slot = 160
+    res, blobs = _run(emu, tmp_path, mode, plane, desc, plane.shape[1], plane.size, slot)
+    raw = np.fromfile(os.path.join(str(tmp_path), "out.bin"), np.uint8).reshape(len(blocks), slot)
+    for i, b in enumerate(blocks):
+        data, npasses, nbp = O.t1_encode(b, 0)
+        assert res["flags"][i] == (1 if len(data) > slot else 0), i
+        if len(data) <= slot:
+            assert blobs[i] == data
+        else:
+            assert bytes(raw[i]) == data[:slot]   # the slot holds the codeword's first bytes
