@@ -93,14 +93,26 @@ typedef struct {
     int32_t flags;   /* bit0: output did not fit the slot */
 } orc_cb_result;
 
+/* Code-block style bits (T.800 Table A.19) the oracle implements. */
+#define ORC_STYLE_RESET   0x02  /* contexts reset after every coding pass */
+#define ORC_STYLE_TERMALL 0x04  /* codeword terminated after every coding pass */
+#define ORC_MAX_PASSES    96    /* >= 3*32-2: stride of per-pass length arrays */
+
 /* Encodes one code-block of w x h (<= 64 x 64, w*h <= 4096) signed
  * coefficients (q, row stride `stride`) of orientation `orient`.  Writes at
  * most `cap` bytes to out. */
 void orc_t1_encode(const int32_t *q, int w, int h, int stride, int orient,
                    uint8_t *out, int cap, orc_cb_result *res);
+/* Same with a code-block style; with TERMALL, passlen[i] receives the length
+ * of pass i's codeword segment (the codeword is their concatenation). */
+void orc_t1_encode_style(const int32_t *q, int w, int h, int stride, int orient, int style,
+                         uint8_t *out, int cap, orc_cb_result *res, int32_t *passlen);
 /* Decodes all `npasses` passes of a code-block back into q (row stride). */
 int orc_t1_decode(const uint8_t *data, int len, int w, int h, int orient,
                   int nbp, int npasses, int32_t *q, int stride);
+/* Same with a code-block style (passlen: segment lengths when TERMALL). */
+int orc_t1_decode_style(const uint8_t *data, int len, int w, int h, int orient, int nbp, int npasses,
+                        int style, const int32_t *passlen, int32_t *q, int stride);
 /* ZC context lookup used by both coders (exported for table tests). */
 int orc_zc_context(int orient, int h, int v, int d);
 /* Sign context (9..13) and xor bit for H, V contributions in -1..1. */
@@ -124,16 +136,18 @@ typedef struct {
     int guard_bits;
     int bitdepth;
     int eps[97], mu[97]; /* per band */
+    int cblk_style;      /* code-block style byte of COD (ORC_STYLE_*) */
 } orc_j2k_params;
 /* Nominal dynamic range R_b of a band (bitdepth + log2 gain). */
 int orc_band_range(int bitdepth, int orient);
 /* Number of magnitude bit-planes M_b = G + eps_b - 1. */
 int orc_band_mb(const orc_j2k_params *p, int band);
 /* cbres/cbdata are per component in orc_codeblocks order; cbdata holds each
- * block's bytes at cboff[i].  Returns the codestream size or -1. */
+ * block's bytes at cboff[i]; with TERMALL, passlen holds each block's segment
+ * lengths (ORC_MAX_PASSES per block).  Returns the codestream size or -1. */
 long orc_t2_write(const orc_j2k_params *p, const orc_cb_result *cbres,
-                  const long *cboff, const uint8_t *cbdata, int ncb_per_comp,
-                  uint8_t *out, long cap);
+                  const long *cboff, const uint8_t *cbdata, const int32_t *passlen,
+                  int ncb_per_comp, uint8_t *out, long cap);
 
 /* ---------------- synthetic inputs ---------------- */
 uint64_t orc_splitmix64(uint64_t *state);
@@ -143,12 +157,13 @@ typedef struct {
     int levels, cbw, cbh, reversible, ncomp;
     double delta;  /* 9/7 base quantisation step */
     int nthreads;  /* OpenMP threads (<= 0: library default) */
+    int style;     /* code-block style (ORC_STYLE_*) */
 } orc_encode_cfg;
 /* Encodes an interleaved uint8 image; writes per-code-block results and
  * bytes (slot_bytes each, component-major).  Returns #code-blocks or <0. */
 int orc_encode_image(const uint8_t *img, int w, int h, const orc_encode_cfg *cfg,
                      orc_cb_result *res, uint8_t *slots, int slot_bytes, int ncb_cap,
-                     int32_t *qplanes_out);
+                     int32_t *qplanes_out, int32_t *passlen_out);
 
 #ifdef __cplusplus
 }

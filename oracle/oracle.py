@@ -54,13 +54,13 @@ class J2KParams(ctypes.Structure):
     _fields_ = [("w", ctypes.c_int), ("h", ctypes.c_int), ("ncomp", ctypes.c_int), ("levels", ctypes.c_int),
                 ("cbw_log2", ctypes.c_int), ("cbh_log2", ctypes.c_int), ("reversible", ctypes.c_int),
                 ("mct", ctypes.c_int), ("guard_bits", ctypes.c_int), ("bitdepth", ctypes.c_int),
-                ("eps", ctypes.c_int * 97), ("mu", ctypes.c_int * 97)]
+                ("eps", ctypes.c_int * 97), ("mu", ctypes.c_int * 97), ("cblk_style", ctypes.c_int)]
 
 
 class EncodeCfg(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_int), ("cbw", ctypes.c_int), ("cbh", ctypes.c_int),
                 ("reversible", ctypes.c_int), ("ncomp", ctypes.c_int), ("delta", ctypes.c_double),
-                ("nthreads", ctypes.c_int)]
+                ("nthreads", ctypes.c_int), ("style", ctypes.c_int)]
 
 
 P = ctypes.c_void_p
@@ -83,11 +83,13 @@ def _declare(L):
         "orc_expmant_for_step": [I, ctypes.c_double, P, P],
         "orc_t1_encode": [P, I, I, I, I, P, I, P],
         "orc_t1_decode": [P, I, I, I, I, I, I, P, I],
+        "orc_t1_encode_style": [P, I, I, I, I, I, P, I, P, P],
+        "orc_t1_decode_style": [P, I, I, I, I, I, I, I, P, P, I],
         "orc_zc_context": [I, I, I, I], "orc_sc_context": [I, I, P],
         "orc_codeblocks": [I, I, I, I, I, P, I],
         "orc_band_range": [I, I],
-        "orc_t2_write": [P, P, P, P, I, P, ctypes.c_long],
-        "orc_encode_image": [P, I, I, P, P, P, I, I, P],
+        "orc_t2_write": [P, P, P, P, P, I, P, ctypes.c_long],
+        "orc_encode_image": [P, I, I, P, P, P, I, I, P, P],
         "orc_round_f16": [ctypes.c_float],
     }
     for name, args in sig.items():
@@ -272,21 +274,38 @@ def sc_context(H, V):
     return c, x.value
 
 
-def t1_encode(block, orient, cap=1 << 17):
+STYLE_RESET = 0x02     # T.800 Table A.19: contexts reset after every pass
+STYLE_TERMALL = 0x04   # codeword terminated after every pass
+STYLE_PARALLEL = STYLE_RESET | STYLE_TERMALL
+MAX_PASSES = 96        # stride of the oracle's per-pass length arrays
+
+
+def t1_encode(block, orient, cap=1 << 17, style=0):
+    """(codeword bytes, npasses, nbp); with style, also the per-pass segment
+    lengths (meaningful with STYLE_TERMALL) as a 4th element."""
     b = np.ascontiguousarray(block, dtype=np.int32)
     h, w = b.shape
     out = np.zeros(cap, np.uint8)
     r = CbResult()
-    lib().orc_t1_encode(_p(b), w, h, w, orient, _p(out), cap, ctypes.byref(r))
+    passlen = np.zeros(MAX_PASSES, np.int32)
+    lib().orc_t1_encode_style(_p(b), w, h, w, orient, style, _p(out), cap, ctypes.byref(r), _p(passlen))
     if r.flags:
         raise RuntimeError("oracle t1 output overflow")
+    if style:
+        return bytes(out[:r.len]), r.npasses, r.nbp, passlen[:r.npasses].copy()
     return bytes(out[:r.len]), r.npasses, r.nbp
 
 
-def t1_decode(data, w, h, orient, nbp, npasses):
+def t1_decode(data, w, h, orient, nbp, npasses, style=0, passlen=None):
     buf = np.frombuffer(bytes(data) + b"\0", np.uint8).copy()
     q = np.empty((h, w), np.int32)
-    lib().orc_t1_decode(_p(buf), len(data), w, h, orient, nbp, npasses, _p(q), w)
+    if style:
+        pl = np.zeros(MAX_PASSES, np.int32)
+        if passlen is not None:
+            pl[:len(passlen)] = passlen
+        lib().orc_t1_decode_style(_p(buf), len(data), w, h, orient, nbp, npasses, style, _p(pl), _p(q), w)
+    else:
+        lib().orc_t1_decode(_p(buf), len(data), w, h, orient, nbp, npasses, _p(q), w)
     return q
 
 
@@ -300,8 +319,10 @@ def codeblocks(w, h, levels, cbw, cbh):
 
 # ---------------- whole image + tier-2 ----------------
 def encode_image(img, levels=5, reversible=True, cbw=64, cbh=64, delta=1.0 / 64, nthreads=0,
-                 slot_bytes=1 << 15):
-    """Returns (results[ncomp*ncb] as (len, npasses, nbp, flags), list of bytes, qplanes)."""
+                 slot_bytes=1 << 15, style=0, return_passlen=False):
+    """Returns (results[ncomp*ncb] as (len, npasses, nbp, flags), list of bytes,
+    qplanes) and, with return_passlen, the per-pass segment lengths
+    [ncomp*ncb, MAX_PASSES] as a 4th element."""
     img = np.ascontiguousarray(img, dtype=np.uint8)
     h, w = img.shape[:2]
     nc = 1 if img.ndim == 2 else img.shape[2]
@@ -309,20 +330,25 @@ def encode_image(img, levels=5, reversible=True, cbw=64, cbh=64, delta=1.0 / 64,
     res = (CbResult * ncb)()
     slots = np.zeros((ncb, slot_bytes), np.uint8)
     q = np.empty((nc, h, w), np.int32)
-    cfg = EncodeCfg(levels, cbw, cbh, int(reversible), nc, float(delta), int(nthreads))
+    passlen = np.zeros((ncb, MAX_PASSES), np.int32)
+    cfg = EncodeCfg(levels, cbw, cbh, int(reversible), nc, float(delta), int(nthreads), int(style))
     n = lib().orc_encode_image(_p(img), w, h, ctypes.byref(cfg), ctypes.cast(res, P), _p(slots),
-                               slot_bytes, ncb, _p(q))
+                               slot_bytes, ncb, _p(q), _p(passlen))
     if n != ncb:
         raise RuntimeError("oracle encode failed: %d" % n)
     rs = np.array([(r.len, r.npasses, r.nbp, r.flags) for r in res], np.int32).reshape(-1, 4)
     if (rs[:, 3] != 0).any():
         raise RuntimeError("oracle code-block overflow")
     data = [bytes(slots[i, :rs[i, 0]]) for i in range(ncb)]
+    if return_passlen:
+        return rs, data, q, passlen
     return rs, data, q
 
 
-def j2k_params(w, h, ncomp, levels, reversible, cbw=64, cbh=64, delta=1.0 / 64, guard_bits=2, bitdepth=8):
+def j2k_params(w, h, ncomp, levels, reversible, cbw=64, cbh=64, delta=1.0 / 64, guard_bits=2, bitdepth=8,
+               style=0):
     p = J2KParams()
+    p.cblk_style = int(style)
     p.w, p.h, p.ncomp, p.levels = w, h, ncomp, levels
     p.cbw_log2, p.cbh_log2 = int(cbw).bit_length() - 1, int(cbh).bit_length() - 1
     p.reversible, p.mct = int(reversible), int(ncomp == 3)
@@ -338,8 +364,9 @@ def j2k_params(w, h, ncomp, levels, reversible, cbw=64, cbh=64, delta=1.0 / 64, 
     return p
 
 
-def t2_write(params, results, datas):
-    """results: int array [ncomp*ncb, 4]; datas: list of bytes (same order)."""
+def t2_write(params, results, datas, passlen=None):
+    """results: int array [ncomp*ncb, 4]; datas: list of bytes (same order);
+    passlen: [ncomp*ncb, MAX_PASSES] segment lengths (TERMALL style)."""
     n = len(datas)
     ncb = n // params.ncomp
     res = (CbResult * n)()
@@ -352,17 +379,18 @@ def t2_write(params, results, datas):
     blob_np = np.frombuffer(bytes(blob) + b"\0", np.uint8).copy()
     cap = len(blob) + 65536 + 64 * n
     out = np.zeros(cap, np.uint8)
-    sz = lib().orc_t2_write(ctypes.byref(params), ctypes.cast(res, P), _p(offs), _p(blob_np), ncb,
-                            _p(out), cap)
+    pl = None if passlen is None else np.ascontiguousarray(passlen, np.int32)
+    sz = lib().orc_t2_write(ctypes.byref(params), ctypes.cast(res, P), _p(offs), _p(blob_np),
+                            None if pl is None else _p(pl), ncb, _p(out), cap)
     if sz < 0:
         raise RuntimeError("t2 write failed")
     return bytes(out[:sz])
 
 
-def encode_j2k(img, levels=5, reversible=True, cbw=64, cbh=64, delta=1.0 / 64):
+def encode_j2k(img, levels=5, reversible=True, cbw=64, cbh=64, delta=1.0 / 64, style=0):
     img = np.ascontiguousarray(img, dtype=np.uint8)
     h, w = img.shape[:2]
     nc = 1 if img.ndim == 2 else img.shape[2]
-    rs, data, q = encode_image(img, levels, reversible, cbw, cbh, delta)
-    params = j2k_params(w, h, nc, levels, reversible, cbw, cbh, delta)
-    return t2_write(params, rs, data), rs, data, q
+    rs, data, q, passlen = encode_image(img, levels, reversible, cbw, cbh, delta, style=style, return_passlen=True)
+    params = j2k_params(w, h, nc, levels, reversible, cbw, cbh, delta, style=style)
+    return t2_write(params, rs, data, passlen if style & STYLE_TERMALL else None), rs, data, q

@@ -24,7 +24,7 @@ uint64_t orc_splitmix64(uint64_t *state)
 
 int orc_encode_image(const uint8_t *img, int w, int h, const orc_encode_cfg *cfg,
                      orc_cb_result *res, uint8_t *slots, int slot_bytes, int ncb_cap,
-                     int32_t *qplanes_out)
+                     int32_t *qplanes_out, int32_t *passlen_out)
 {
     int nc = cfg->ncomp, L = cfg->levels;
     size_t npix = (size_t)w * h;
@@ -65,8 +65,9 @@ int orc_encode_image(const uint8_t *img, int w, int h, const orc_encode_cfg *cfg
     for (int i = 0; i < ncb * nc; ++i) {
         int c = i / ncb, k = i % ncb;
         const int32_t *src = q + c * npix + (size_t)d[k].y0 * w + d[k].x0;
-        orc_t1_encode(src, d[k].w, d[k].h, w, d[k].orient,
-                      slots + (size_t)i * slot_bytes, slot_bytes, &res[i]);
+        orc_t1_encode_style(src, d[k].w, d[k].h, w, d[k].orient, cfg->style,
+                            slots + (size_t)i * slot_bytes, slot_bytes, &res[i],
+                            passlen_out ? passlen_out + (size_t)i * ORC_MAX_PASSES : NULL);
     }
     free(d);
     if (!qplanes_out) free(q);

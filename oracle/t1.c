@@ -67,6 +67,16 @@ static void mqe_init(mq_enc *e, uint8_t *buf, long cap)
     mq_reset_ctx(e->I, e->MPS);
 }
 
+/* Re-initialises the coder registers for a new codeword segment (after a
+ * termination) and keeps the context states (C.2.8 INITENC without the
+ * context reset; the byte before the segment is never 0xFF, see mqe_flush). */
+static void mqe_restart(mq_enc *e, uint8_t *buf, long cap)
+{
+    e->A = 0x8000; e->C = 0; e->CT = 12;
+    e->buf = buf; e->bp = 0; e->cap = cap; e->overflow = 0;
+    buf[0] = 0;
+}
+
 static inline void mqe_put(mq_enc *e, uint32_t v)
 {
     if (e->bp + 1 >= e->cap) { e->overflow = 1; return; }
@@ -153,12 +163,18 @@ static void mqd_bytein(mq_dec *m)
     }
 }
 
-static void mqd_init(mq_dec *m, const uint8_t *d, long len)
+/* INITDEC (C.3.5) on a codeword segment; contexts are left untouched. */
+static void mqd_start(mq_dec *m, const uint8_t *d, long len)
 {
     m->d = d; m->len = len; m->bp = 0;
     m->C = mqd_byte(m, 0) << 16;
     mqd_bytein(m);
     m->C <<= 7; m->CT -= 7; m->A = 0x8000;
+}
+
+static void mqd_init(mq_dec *m, const uint8_t *d, long len)
+{
+    mqd_start(m, d, len);
     mq_reset_ctx(m->I, m->MPS);
 }
 
@@ -336,8 +352,13 @@ static void enc_cup(cbstate *s, mq_enc *e, int p)
         }
 }
 
-void orc_t1_encode(const int32_t *q, int w, int h, int stride, int orient,
-                   uint8_t *out, int cap, orc_cb_result *res)
+/* Code-block styles (Table A.19): RESET resets every context to its initial
+ * state after each coding pass; TERMALL terminates the codeword (C.2.9) after
+ * each pass and restarts the coder (C.2.8) for the next one, so every pass is
+ * its own codeword segment.  Together they make the passes of a block
+ * independent MQ chains. */
+void orc_t1_encode_style(const int32_t *q, int w, int h, int stride, int orient, int style,
+                         uint8_t *out, int cap, orc_cb_result *res, int32_t *passlen)
 {
     cbstate s;
     s.w = w; s.h = h; s.fs = w + 2; s.orient = orient;
@@ -363,18 +384,39 @@ void orc_t1_encode(const int32_t *q, int w, int h, int stride, int orient,
     }
     long icap = (long)w * h * (nbp + 2) + 1024;
     uint8_t *buf = (uint8_t *)malloc((size_t)icap);
+    uint8_t *code = (uint8_t *)malloc((size_t)icap);   /* concatenated segments */
+    long total = 0;
+    int overflow = 0, pass = 0;
     mq_enc e;
     mqe_init(&e, buf, icap);
-    for (int p = nbp - 1; p >= 0; --p) {
-        if (p != nbp - 1) { enc_spp(&s, &e, p); enc_mrp(&s, &e, p); }
-        enc_cup(&s, &e, p);
-    }
-    long len = mqe_flush(&e);
+    for (int p = nbp - 1; p >= 0; --p)
+        for (int t = (p == nbp - 1) ? 2 : 0; t < 3; ++t) {
+            if (t == 0) enc_spp(&s, &e, p);
+            else if (t == 1) enc_mrp(&s, &e, p);
+            else enc_cup(&s, &e, p);
+            const int last = (p == 0 && t == 2);
+            if ((style & ORC_STYLE_TERMALL) || last) {
+                long len = mqe_flush(&e);
+                overflow |= e.overflow;
+                memcpy(code + total, buf + 1, (size_t)len);
+                total += len;
+                if (passlen) passlen[pass] = (int32_t)len;
+                if (!last) mqe_restart(&e, buf, icap);
+            }
+            if (style & ORC_STYLE_RESET) mq_reset_ctx(e.I, e.MPS);
+            ++pass;
+        }
     res->npasses = 3 * nbp - 2;
-    res->len = (int32_t)len;
-    if (e.overflow || len > cap) res->flags |= 1;
-    else memcpy(out, buf + 1, (size_t)len);
-    free(buf); free(s.f); free(s.m);
+    res->len = (int32_t)total;
+    if (overflow || total > cap) res->flags |= 1;
+    else memcpy(out, code, (size_t)total);
+    free(code); free(buf); free(s.f); free(s.m);
+}
+
+void orc_t1_encode(const int32_t *q, int w, int h, int stride, int orient,
+                   uint8_t *out, int cap, orc_cb_result *res)
+{
+    orc_t1_encode_style(q, w, h, stride, orient, 0, out, cap, res, NULL);
 }
 
 /* ---------------- decoder passes ---------------- */
@@ -446,24 +488,33 @@ static void dec_cup(cbstate *s, mq_dec *m, int p)
         }
 }
 
-int orc_t1_decode(const uint8_t *data, int len, int w, int h, int orient,
-                  int nbp, int npasses, int32_t *q, int stride)
+int orc_t1_decode_style(const uint8_t *data, int len, int w, int h, int orient, int nbp, int npasses,
+                        int style, const int32_t *passlen, int32_t *q, int stride)
 {
     cbstate s;
     s.w = w; s.h = h; s.fs = w + 2; s.orient = orient;
     s.f = (uint8_t *)calloc((size_t)(w + 2) * (h + 2), 1);
     s.m = (uint32_t *)calloc((size_t)w * h + 4, sizeof(uint32_t));
     if (nbp > 0 && npasses > 0) {
+        const int termall = (style & ORC_STYLE_TERMALL) && passlen;
         mq_dec m;
-        mqd_init(&m, data, len);
+        long off = 0;
+        mqd_init(&m, data, termall ? passlen[0] : len);
         int pass = 0;
-        for (int p = nbp - 1; p >= 0 && pass < npasses; --p) {
-            if (p != nbp - 1) {
-                dec_spp(&s, &m, p); if (++pass >= npasses) break;
-                dec_mrp(&s, &m, p); if (++pass >= npasses) break;
+        for (int p = nbp - 1; p >= 0 && pass < npasses; --p)
+            for (int t = (p == nbp - 1) ? 2 : 0; t < 3 && pass < npasses; ++t) {
+                if (pass > 0) {
+                    if (style & ORC_STYLE_RESET) mq_reset_ctx(m.I, m.MPS);
+                    if (termall) {
+                        off += passlen[pass - 1];
+                        mqd_start(&m, data + off, passlen[pass]);
+                    }
+                }
+                if (t == 0) dec_spp(&s, &m, p);
+                else if (t == 1) dec_mrp(&s, &m, p);
+                else dec_cup(&s, &m, p);
+                ++pass;
             }
-            dec_cup(&s, &m, p); ++pass;
-        }
     }
     for (int y = 0; y < h; ++y)
         for (int x = 0; x < w; ++x) {
@@ -472,6 +523,12 @@ int orc_t1_decode(const uint8_t *data, int len, int w, int h, int orient,
         }
     free(s.f); free(s.m);
     return 0;
+}
+
+int orc_t1_decode(const uint8_t *data, int len, int w, int h, int orient,
+                  int nbp, int npasses, int32_t *q, int stride)
+{
+    return orc_t1_decode_style(data, len, w, h, orient, nbp, npasses, 0, NULL, q, stride);
 }
 
 /* ---------------- code-block enumeration ---------------- */

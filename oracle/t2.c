@@ -131,8 +131,8 @@ static int floorlog2(int v) { int l = -1; while (v) { v >>= 1; ++l; } return l; 
 int orc_band_mb(const orc_j2k_params *p, int band) { return p->guard_bits + p->eps[band] - 1; }
 
 long orc_t2_write(const orc_j2k_params *p, const orc_cb_result *cbres,
-                  const long *cboff, const uint8_t *cbdata, int ncb_per_comp,
-                  uint8_t *out, long cap)
+                  const long *cboff, const uint8_t *cbdata, const int32_t *passlen,
+                  int ncb_per_comp, uint8_t *out, long cap)
 {
     wbuf w = { out, 0, cap, 0 };
     int nbands = 3 * p->levels + 1;
@@ -146,7 +146,7 @@ long orc_t2_write(const orc_j2k_params *p, const orc_cb_result *cbres,
     /* COD */
     put16(&w, 0xFF52); put16(&w, 12); put8(&w, 0);
     put8(&w, 0); put16(&w, 1); put8(&w, p->mct);
-    put8(&w, p->levels); put8(&w, p->cbw_log2 - 2); put8(&w, p->cbh_log2 - 2); put8(&w, 0);
+    put8(&w, p->levels); put8(&w, p->cbw_log2 - 2); put8(&w, p->cbh_log2 - 2); put8(&w, p->cblk_style);
     put8(&w, p->reversible ? 1 : 0);
     /* QCD */
     put16(&w, 0xFF5C);
@@ -180,6 +180,7 @@ long orc_t2_write(const orc_j2k_params *p, const orc_cb_result *cbres,
         for (int c = 0; c < p->ncomp; ++c) {
             const orc_cb_result *cr = cbres + (long)c * ncb_per_comp;
             const long *co = cboff + (long)c * ncb_per_comp;
+            const int32_t *pl = passlen ? passlen + (long)c * ncb_per_comp * ORC_MAX_PASSES : NULL;
             int any = 0;
             for (int k = b0; k < b1; ++k)
                 for (int i = 0; i < bnx[k] * bny[k]; ++i) if (cr[bstart[k] + i].npasses > 0) any = 1;
@@ -206,13 +207,24 @@ long orc_t2_write(const orc_j2k_params *p, const orc_cb_result *cbres,
                         if (q->npasses == 0) continue;
                         tt_encode(&imsb, &b, i, j, 999);
                         put_npasses(&b, q->npasses);
-                        int lblock = 3, nb = floorlog2(q->len) + 1;
-                        int have = lblock + floorlog2(q->npasses);
-                        int inc = nb > have ? nb - have : 0;
+                        /* B.10.7: one Lblock increment, then the length of each
+                         * codeword segment (every pass with TERMALL, else one
+                         * segment of npasses passes) in Lblock + log2(passes) bits */
+                        const int32_t *segs = pl ? pl + (long)(bstart[k] + j * bnx[k] + i) * ORC_MAX_PASSES : NULL;
+                        const int termall = (p->cblk_style & ORC_STYLE_TERMALL) && segs;
+                        const int nseg = termall ? q->npasses : 1;
+                        const int seg_passes_log2 = termall ? 0 : floorlog2(q->npasses);
+                        int lblock = 3, inc = 0;
+                        for (int sgi = 0; sgi < nseg; ++sgi) {
+                            int len = termall ? segs[sgi] : q->len;
+                            int nb = floorlog2(len) + 1, have = lblock + seg_passes_log2;
+                            if (nb - have > inc) inc = nb - have;
+                        }
                         for (int t = 0; t < inc; ++t) bw_bit(&b, 1);
                         bw_bit(&b, 0);
                         lblock += inc;
-                        bw_bits(&b, (uint32_t)q->len, lblock + floorlog2(q->npasses));
+                        for (int sgi = 0; sgi < nseg; ++sgi)
+                            bw_bits(&b, (uint32_t)(termall ? segs[sgi] : q->len), lblock + seg_passes_log2);
                     }
                 tt_free(&incl); tt_free(&imsb);
             }

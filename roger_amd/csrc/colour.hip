@@ -5,15 +5,20 @@
 // de-interleave kernel (src/CL/oclplanar.cl:23-39).  T.800 G.2 (RCT) and
 // G.3 (ICT); FP32 operation order identical to the CPU oracle (fmaf chains).
 //
-// Streaming design: each thread owns 16 consecutive pixels, so a wavefront
-// reads one contiguous 1 KiB (gray) / 3 KiB (RGB) span with 16-byte loads and
-// writes 4 KiB per plane with 16-byte stores; no LDS.
+// Streaming design, forward: a thread handles 4-pixel quads strided by the
+// workgroup, so lane k of a wavefront reads bytes [12k, 12k+12) (RGB) of one
+// contiguous span and writes bytes [16k, 16k+16) of each plane's span: every
+// memory instruction is a fully coalesced, contiguous wavefront access.
+// Inverse: each thread owns 16 consecutive pixels.  No LDS.
 #include <algorithm>
+#include <type_traits>
 #include "rg_internal.h"
 
 namespace {
 
-constexpr int PX = 16;  // pixels per thread
+constexpr int PX = 16;               // inverse: pixels per thread
+constexpr int QPT = 4;               // forward: 4-pixel quads per thread
+constexpr int SPAN = 256 * QPT * 4;  // forward: pixels per 256-thread workgroup
 
 __device__ __forceinline__ float ict_y(float r, float g, float b)
 {
@@ -36,77 +41,80 @@ __device__ __forceinline__ uint8_t out_u8f(float x)
 }
 
 // MODE: 0 shift only, 1 RCT, 2 ICT.  FLT: float planes (else int32).
-template <int NC, int MODE, bool FLT>
-__device__ __forceinline__ void colour_fwd_body(const uint8_t *__restrict__ src, long npix, void *__restrict__ planes,
-                                                long g)
+// Forward transform of one level-shifted pixel v[0..NC) into o[0..NC).
+template <int NC, int MODE, bool FLT, class T>
+__device__ __forceinline__ void px_fwd(const int (&v)[NC], T (&o)[NC])
 {
-    long p0 = g * PX;
-    if (p0 >= npix) return;
-    int n = (int)min((long)PX, npix - p0);
-    uint8_t b[PX * NC];
-    if (n == PX) {
-        const uint4 *s4 = reinterpret_cast<const uint4 *>(src + p0 * NC);
-#pragma unroll
-        for (int k = 0; k < NC; ++k) {
-            uint4 v = s4[k];
-            *reinterpret_cast<uint4 *>(&b[16 * k]) = v;
-        }
+    if constexpr (MODE == 2) {
+        const float r = (float)v[0], g = (float)v[1], b = (float)v[2];
+        o[0] = ict_y(r, g, b);
+        o[1] = ict_cb(r, g, b);
+        o[2] = ict_cr(r, g, b);
+    } else if constexpr (MODE == 1) {
+        o[0] = (T)((v[0] + 2 * v[1] + v[2]) >> 2);
+        o[1] = (T)(v[2] - v[1]);
+        o[2] = (T)(v[0] - v[1]);
     } else {
-        for (int i = 0; i < n * NC; ++i) b[i] = src[p0 * NC + i];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) o[c] = (T)v[c];
     }
-    if (FLT) {
-        float *out = reinterpret_cast<float *>(planes);
-        float o[NC][PX];
+}
+
+// One quad: pixels [p, p+4) (p < npix, p a multiple of 4).  A full quad reads
+// NC aligned dwords (4 pixels x NC bytes) and writes one 16-byte vector per
+// plane; a quad cut by the image end goes pixel by pixel.
+template <int NC, int MODE, bool FLT>
+__device__ __forceinline__ void colour_fwd_quad(const uint8_t *__restrict__ src, long npix, void *__restrict__ planes,
+                                                long p)
+{
+    using T = typename std::conditional<FLT, float, int32_t>::type;
+    T *out = reinterpret_cast<T *>(planes);
+    if (p + 4 <= npix) {
+        const uint32_t *s32 = reinterpret_cast<const uint32_t *>(src + p * NC);
+        uint32_t w[NC];
 #pragma unroll
-        for (int i = 0; i < PX; ++i) {
-            if (MODE == 2) {
-                float r = (float)((int)b[3 * i] - 128), gg = (float)((int)b[3 * i + 1] - 128),
-                      bb = (float)((int)b[3 * i + 2] - 128);
-                o[0][i] = ict_y(r, gg, bb);
-                o[1][i] = ict_cb(r, gg, bb);
-                o[2][i] = ict_cr(r, gg, bb);
-            } else {
+        for (int k = 0; k < NC; ++k) w[k] = s32[k];
+        T o[NC][4];
 #pragma unroll
-                for (int c = 0; c < NC; ++c) o[c][i] = (float)((int)b[NC * i + c] - 128);
-            }
+        for (int i = 0; i < 4; ++i) {
+            int v[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) v[c] = (int)((w[(NC * i + c) >> 2] >> (((NC * i + c) & 3) * 8)) & 0xFFu) - 128;
+            T t[NC];
+            px_fwd<NC, MODE, FLT>(v, t);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) o[c][i] = t[c];
         }
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-            float *dst = out + (long)c * npix + p0;
-            if (n == PX) {
-#pragma unroll
-                for (int k = 0; k < PX / 4; ++k)
-                    reinterpret_cast<float4 *>(dst)[k] = make_float4(o[c][4 * k], o[c][4 * k + 1], o[c][4 * k + 2], o[c][4 * k + 3]);
-            } else {
-                for (int i = 0; i < n; ++i) dst[i] = o[c][i];
-            }
+            T *d = out + (long)c * npix + p;
+            if constexpr (FLT) *reinterpret_cast<float4 *>(d) = make_float4(o[c][0], o[c][1], o[c][2], o[c][3]);
+            else *reinterpret_cast<int4 *>(d) = make_int4(o[c][0], o[c][1], o[c][2], o[c][3]);
         }
     } else {
-        int32_t *out = reinterpret_cast<int32_t *>(planes);
-        int o[NC][PX];
+        for (long q = p; q < npix; ++q) {
+            int v[NC];
 #pragma unroll
-        for (int i = 0; i < PX; ++i) {
-            if (MODE == 1) {
-                int r = (int)b[3 * i] - 128, gg = (int)b[3 * i + 1] - 128, bb = (int)b[3 * i + 2] - 128;
-                o[0][i] = (r + 2 * gg + bb) >> 2;
-                o[1][i] = bb - gg;
-                o[2][i] = r - gg;
-            } else {
+            for (int c = 0; c < NC; ++c) v[c] = (int)src[q * NC + c] - 128;
+            T t[NC];
+            px_fwd<NC, MODE, FLT>(v, t);
 #pragma unroll
-                for (int c = 0; c < NC; ++c) o[c][i] = (int)b[NC * i + c] - 128;
-            }
+            for (int c = 0; c < NC; ++c) out[(long)c * npix + q] = t[c];
         }
+    }
+}
+
+// A 256-thread workgroup covers SPAN pixels from p_wg: thread t takes quads
+// t, t+256, ... so each load / store instruction of the wavefront touches one
+// contiguous range (12 B per lane in, 16 B per lane per plane out for RGB).
+template <int NC, int MODE, bool FLT>
+__device__ __forceinline__ void colour_fwd_span(const uint8_t *__restrict__ src, long npix, void *__restrict__ planes,
+                                                long p_wg)
+{
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            int32_t *dst = out + (long)c * npix + p0;
-            if (n == PX) {
-#pragma unroll
-                for (int k = 0; k < PX / 4; ++k)
-                    reinterpret_cast<int4 *>(dst)[k] = make_int4(o[c][4 * k], o[c][4 * k + 1], o[c][4 * k + 2], o[c][4 * k + 3]);
-            } else {
-                for (int i = 0; i < n; ++i) dst[i] = o[c][i];
-            }
-        }
+    for (int u = 0; u < QPT; ++u) {
+        const long p = p_wg + 4 * ((long)u * 256 + threadIdx.x);
+        if (p < npix) colour_fwd_quad<NC, MODE, FLT>(src, npix, planes, p);
     }
 }
 
@@ -114,7 +122,7 @@ template <int NC, int MODE, bool FLT>
 __global__ void __launch_bounds__(256) colour_fwd_kernel(const uint8_t *__restrict__ src, long npix,
                                                           void *__restrict__ planes)
 {
-    colour_fwd_body<NC, MODE, FLT>(src, npix, planes, (long)blockIdx.x * blockDim.x + threadIdx.x);
+    colour_fwd_span<NC, MODE, FLT>(src, npix, planes, (long)blockIdx.x * SPAN);
 }
 
 template <int NC, int MODE, bool FLT>
@@ -200,8 +208,8 @@ int rg_launch_colour_fwd(hipStream_t s, const uint8_t *src, int w, int h, int nc
 {
     long npix = (long)w * h;
     if (npix == 0) return RG_SUCCESS;
-    long nthr = (npix + PX - 1) / PX;
-    dim3 grid((unsigned)((nthr + 255) / 256)), block(256);
+    RG_CHECK(((uintptr_t)src & 3) == 0, "colour: source must be 4-byte aligned");
+    dim3 grid((unsigned)((npix + SPAN - 1) / SPAN)), block(256);
     if (ncomp == 3 && colour == RG_COLOUR_RCT)
         colour_fwd_kernel<3, 1, false><<<grid, block, 0, s>>>(src, npix, planes);
     else if (ncomp == 3 && colour == RG_COLOUR_ICT)
@@ -287,15 +295,15 @@ __global__ void __launch_bounds__(256) colour_fwd_batch_kernel(const rg_colour_j
 {
     const int2 j = jobs[blockIdx.x];
     const rg_colour_job_params p = P[j.x];
-    const long g = (long)j.y + threadIdx.x;
-    if (p.ncomp == 3 && p.colour == RG_COLOUR_ICT) colour_fwd_body<3, 2, true>(p.src, p.npix, p.planes, g);
-    else if (p.ncomp == 3 && p.colour == RG_COLOUR_RCT) colour_fwd_body<3, 1, false>(p.src, p.npix, p.planes, g);
+    const long p0 = (long)j.y * 16;   // jobs count 16-pixel chunks; a workgroup spans 256 of them
+    if (p.ncomp == 3 && p.colour == RG_COLOUR_ICT) colour_fwd_span<3, 2, true>(p.src, p.npix, p.planes, p0);
+    else if (p.ncomp == 3 && p.colour == RG_COLOUR_RCT) colour_fwd_span<3, 1, false>(p.src, p.npix, p.planes, p0);
     else if (p.ncomp == 3) {
-        if (p.out_float) colour_fwd_body<3, 0, true>(p.src, p.npix, p.planes, g);
-        else colour_fwd_body<3, 0, false>(p.src, p.npix, p.planes, g);
+        if (p.out_float) colour_fwd_span<3, 0, true>(p.src, p.npix, p.planes, p0);
+        else colour_fwd_span<3, 0, false>(p.src, p.npix, p.planes, p0);
     } else {
-        if (p.out_float) colour_fwd_body<1, 0, true>(p.src, p.npix, p.planes, g);
-        else colour_fwd_body<1, 0, false>(p.src, p.npix, p.planes, g);
+        if (p.out_float) colour_fwd_span<1, 0, true>(p.src, p.npix, p.planes, p0);
+        else colour_fwd_span<1, 0, false>(p.src, p.npix, p.planes, p0);
     }
 }
 }  // namespace

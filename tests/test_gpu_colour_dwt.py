@@ -56,6 +56,19 @@ def test_colour_gray_shift(ctx, w, h):
     np.testing.assert_array_equal(d_back.read(np.uint8, (h, w)), img)
 
 
+@pytest.mark.parametrize("w,h", [(7, 3), (257, 300)])
+def test_colour_rgb_shift_only(ctx, w, h):
+    """Three components without a colour transform: plain level shift."""
+    img = rand_img(w, h, 3, w * 3 + h)
+    d_src = ctx.upload(img)
+    d_pl = ctx.alloc(3 * w * h * 4)
+    want = np.moveaxis(img.astype(np.int32) - 128, 2, 0)
+    R.colour_fwd(ctx, d_src, w, h, 3, RG_COLOUR_NONE, 0, d_pl)
+    np.testing.assert_array_equal(d_pl.read(np.int32, (3, h, w)), want)
+    R.colour_fwd(ctx, d_src, w, h, 3, RG_COLOUR_NONE, 1, d_pl)
+    np.testing.assert_array_equal(d_pl.read(np.float32, (3, h, w)), want.astype(np.float32))
+
+
 def _levels_for(w, h):
     return [0, 1, 3, 6]
 

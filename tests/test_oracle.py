@@ -216,3 +216,51 @@ def test_openjpeg_decodes_lossy_like_oracle(w, h, nc, levels, delta):
     assert np.abs(mine.astype(int) - dec.astype(int)).max() <= 1
     mse = ((dec.astype(float) - img) ** 2).mean()
     assert 10 * np.log10(255 ** 2 / max(mse, 1e-12)) > 30
+
+
+# ---------------------------------------------------------------- code-block styles (RESET / TERMALL)
+@pytest.mark.parametrize("style", [O.STYLE_RESET, O.STYLE_TERMALL, O.STYLE_PARALLEL])
+def test_t1_style_roundtrip(style):
+    rng = np.random.default_rng(style)
+    for (w, h) in [(64, 64), (37, 13), (1, 1), (5, 64)]:
+        for orient in range(4):
+            b = rng.laplace(0, 300, size=(h, w)).astype(np.int32)
+            if not b.any():
+                continue
+            data, npass, nbp, passlen = O.t1_encode(b, orient, style=style)
+            if style & O.STYLE_TERMALL:
+                assert len(passlen) == npass and int(passlen.sum()) == len(data)
+                assert (passlen > 0).all()
+            np.testing.assert_array_equal(O.t1_decode(data, w, h, orient, nbp, npass, style, passlen), b)
+
+
+def test_t1_style_changes_the_codeword():
+    """Each flag changes the bytes (the styles are really applied), while the
+    standard style is what t1_encode without a style produces."""
+    b = np.random.default_rng(1).laplace(0, 50, size=(64, 64)).astype(np.int32)
+    plain = O.t1_encode(b, 0)[0]
+    assert O.t1_encode(b, 0, style=0)[0] == plain
+    outs = {s: O.t1_encode(b, 0, style=s)[0] for s in (O.STYLE_RESET, O.STYLE_TERMALL, O.STYLE_PARALLEL)}
+    assert len({plain, *outs.values()}) == 4
+    # a terminated segment never ends in 0xFF (so a restarted coder starts with CT = 12)
+    data, npass, nbp, passlen = O.t1_encode(b, 0, style=O.STYLE_PARALLEL)
+    ends = np.cumsum(passlen) - 1
+    assert all(data[e] != 0xFF for e in ends)
+
+
+@pytest.mark.parametrize("style", [O.STYLE_RESET, O.STYLE_TERMALL, O.STYLE_PARALLEL])
+@pytest.mark.parametrize("w,h,nc,levels,cb", [(256, 200, 3, 5, 64), (97, 61, 1, 3, 32), (8, 3, 3, 4, 64)])
+def test_openjpeg_decodes_styles_lossless(style, w, h, nc, levels, cb):
+    """OpenJPEG (an independent decoder) reads the COD style byte and the
+    per-segment lengths of the packet headers and reconstructs exactly."""
+    img = synth.image(w, h, nc, 9)
+    cs = O.encode_j2k(img, levels=levels, reversible=True, cbw=cb, cbh=cb, style=style)[0]
+    np.testing.assert_array_equal(_decode(cs), img)
+
+
+def test_openjpeg_decodes_parallel_style_lossy():
+    img = synth.image(300, 257, 3, 4)
+    cs_std = O.encode_j2k(img, levels=6, reversible=False, delta=1.0 / 64)[0]
+    cs_par = O.encode_j2k(img, levels=6, reversible=False, delta=1.0 / 64, style=O.STYLE_PARALLEL)[0]
+    np.testing.assert_array_equal(_decode(cs_par), _decode(cs_std))   # same coefficients, same image
+    assert len(cs_par) > len(cs_std)                                     # termination costs bytes
