@@ -3,14 +3,20 @@
 Workload (the headline configuration): one 4096x4096 interleaved RGB uint8
 image (synthetic, seeded), resident in HBM; one step = the whole encode on
 the device: RGB -> planar + ICT, irreversible 9/7 FP32 DWT with 6 levels and
-scalar quantisation at step 1/64, EBCOT tier-1 on 64x64 code-blocks.
+scalar quantisation at step 1/64, EBCOT tier-1 on 64x64 code-blocks in the
+parallel code-block style (RESET|TERMALL: every coding pass its own MQ
+codeword segment; --style standard for one segment per block).
 value = image Mpix/s over the timed steps.
 
 Also reported:
   roofline      : the level-1 9/7 DWT+quantisation kernel (the DWT's dominant
                   launch), algorithmic bytes 8 B/px/plane (FP32 in, LL FP32 +
                   int32 indices out) over its HIP-event-timed launch, vs 8 TB/s.
-  stages_ms     : per-stage times (colour, DWT all levels, tier-1).
+  stages_ms     : per-stage times (colour, DWT all levels, tier-1 in both
+                  styles).
+  standard_style: the same encode with single-segment code-blocks.
+  decode        : the reverse path on the device-resident codewords (tier-1
+                  decode, dequantisation + inverse 9/7, inverse ICT).
   cpu_baseline  : the CPU restatement (oracle) of the same encode on the host
                   cores, on the full image.
 """
@@ -29,10 +35,10 @@ METRIC = "Encode Mpix/s (4096² RGB, DWT+EBCOT); DWT %HBM peak; batched vs plain
 HBM_PEAK_GBS = 8000.0
 
 
-def bench_batched(ctx, args):
+def bench_batched(ctx, args, style):
     """Mixed-size batch (sides log-uniform 256..4096, half gray / half RGB,
     half 5/3 / half 9/7, 5 levels): batched job-table + graph path vs the plain
-    per-image path, both on device-resident inputs."""
+    per-image path, both on device-resident inputs, both in `style`."""
     from roger_amd import autotune
     from roger_amd import runtime as R
     from roger_amd import synth
@@ -46,7 +52,7 @@ def bench_batched(ctx, args):
         images.append(dict(s, buf=buf))
     mpix = sum(m["w"] * m["h"] for m in images) / 1e6
     batch = R.Batch(ctx)
-    tuned = autotune.tune(ctx, batch, images, log=log)
+    tuned = autotune.tune(ctx, batch, images, log=log, style=style)
     cb = tuned["cb"]
     steps = max(2, min(5, args.steps))
     batch.run()
@@ -57,11 +63,14 @@ def bench_batched(ctx, args):
     ctx.sync()
     t_batch = (time.perf_counter() - t) / steps
     ncb = batch.num_codeblocks(-1)
-    # plain per-image path: one resident encoder per image, same code-block size
-    encs = [R.Encoder(ctx) for _ in images]
-    run_plain = lambda: [e.encode_u8(None, levels=m["levels"], reversible=m["reversible"], cbw=cb, cbh=cb,
-                                     delta=m["delta"], d_src=m["buf"], w=m["w"], h=m["h"], ncomp=m["ncomp"])
-                         for e, m in zip(encs, images)]
+    # plain per-image path: one encoder over the images in turn (its buffers
+    # grow to the largest image during the warm-up pass, then stay resident)
+    enc1 = R.Encoder(ctx)
+    encs = [enc1]
+    run_plain = lambda: [enc1.encode_u8(None, levels=m["levels"], reversible=m["reversible"], cbw=cb, cbh=cb,
+                                        delta=m["delta"], d_src=m["buf"], w=m["w"], h=m["h"], ncomp=m["ncomp"],
+                                        style=style)
+                         for m in images]
     run_plain()
     ctx.sync()
     t = time.perf_counter()
@@ -72,7 +81,8 @@ def bench_batched(ctx, args):
     for e in encs:
         e.close()
     batch.close()
-    return {"images": len(images), "mpix": round(mpix, 2), "codeblocks": ncb, "autotune": tuned,
+    return {"images": len(images), "mpix": round(mpix, 2), "codeblocks": ncb, "codeblock_style": style,
+            "autotune": tuned,
             "batched_mpix_s": round(mpix / t_batch, 1), "plain_mpix_s": round(mpix / t_plain, 1),
             "batched_ms": round(t_batch * 1e3, 3), "plain_ms": round(t_plain * 1e3, 3),
             "speedup": round(t_plain / t_batch, 3)}
@@ -116,6 +126,8 @@ def main():
     ap.add_argument("--batch-images", type=int, default=256)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--style", choices=["parallel", "standard"], default="parallel",
+                    help="code-block style of the headline encode")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (single-GPU hot path; see DESIGN.md "
@@ -124,30 +136,38 @@ def main():
 
     from roger_amd import runtime as R
     from roger_amd import synth
-    from roger_amd._lib import RG_COLOUR_ICT, RG_FILTER_97, lib
+    from roger_amd._lib import RG_COLOUR_ICT, RG_FILTER_97, RG_STYLE_PARALLEL, lib
 
     W = H = args.size
     NC, LEVELS, DELTA, CB = 3, 6, 1.0 / 64, 64
+    STYLE = RG_STYLE_PARALLEL if args.style == "parallel" else 0
     print("generating %dx%d RGB input" % (W, H), file=sys.stderr, flush=True)
     img = synth.rgb(W, H, 1)
     ctx = R.Context(0)
     d_src = ctx.upload(img)
     enc = R.Encoder(ctx)
-    run = lambda: enc.encode_u8(None, levels=LEVELS, reversible=False, cbw=CB, cbh=CB, delta=DELTA,
-                                d_src=d_src, w=W, h=H, ncomp=NC)
-    for _ in range(args.warmup):
-        run()
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        run()
-    ctx.sync()
-    dt = time.perf_counter() - t0
+
+    def encode_rate(style, steps, warmup):
+        run = lambda: enc.encode_u8(None, levels=LEVELS, reversible=False, cbw=CB, cbh=CB, delta=DELTA,
+                                    d_src=d_src, w=W, h=H, ncomp=NC, style=style)
+        for _ in range(warmup):
+            run()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            run()
+        ctx.sync()
+        return time.perf_counter() - t0
+
+    other = 0 if STYLE else RG_STYLE_PARALLEL
+    dt_other = encode_rate(other, max(3, args.steps // 4), 1)
+    _, res_other, _ = enc.codeblocks()
+    dt = encode_rate(STYLE, args.steps, args.warmup)
     mpix = W * H / 1e6
     ms_step = dt * 1e3 / args.steps
     value = mpix * args.steps / dt
     ncb = lib().rg_encoder_num_codeblocks(enc.handle)
-    _, res, blobs = enc.codeblocks()
+    desc_enc, res, blobs = enc.codeblocks()
     total_bytes = int(res["len"].sum())
 
     # ---- per-stage timing with HIP events on the context stream ----
@@ -177,6 +197,35 @@ def main():
     t_t1 = timed(lambda: lib().rg_t1_encode(ctx.handle, ctypes.c_void_p(d_q.ptr), W, W * H,
                                               ctypes.c_void_p(d_desc.ptr), len(desc), ctypes.c_void_p(d_out.ptr),
                                               slot, ctypes.c_void_p(d_res.ptr)))
+    # the encoder's tier-1 in the parallel style (context modelling, pass-parallel MQ, segment packing)
+    enc.encode_u8(None, levels=LEVELS, reversible=False, cbw=CB, cbh=CB, delta=DELTA, d_src=d_src, w=W, h=H,
+                  ncomp=NC, style=RG_STYLE_PARALLEL)
+    t_enc_par = timed(lambda: enc.encode_u8(None, levels=LEVELS, reversible=False, cbw=CB, cbh=CB, delta=DELTA,
+                                            d_src=d_src, w=W, h=H, ncomp=NC, style=RG_STYLE_PARALLEL))
+    # reverse path on device-resident standard-style codewords of the same image
+    if STYLE:
+        enc.encode_u8(None, levels=LEVELS, reversible=False, cbw=CB, cbh=CB, delta=DELTA, d_src=d_src, w=W, h=H,
+                      ncomp=NC, style=0)
+    _, res0, blobs0 = enc.codeblocks()
+    data0, offs0 = R.pack_codewords(blobs0)
+    d_data0, d_offs0 = ctx.upload(data0), ctx.upload(offs0)
+    d_res0 = ctx.upload(np.ascontiguousarray(res0).view(np.int32))
+    d_qd = ctx.alloc(NC * W * H * 4)
+    d_rec = ctx.alloc(NC * W * H * 4)
+    d_u8 = ctx.alloc(NC * W * H)
+    step_f = R.band_steps(LEVELS, DELTA)[0]
+
+    def decode_once():
+        check_rc(lib().rg_t1_decode(ctx.handle, ctypes.c_void_p(d_data0.ptr), ctypes.c_void_p(d_offs0.ptr),
+                                    ctypes.c_void_p(d_res0.ptr), ctypes.c_void_p(d_desc.ptr), len(desc),
+                                    ctypes.c_void_p(d_qd.ptr), W, W * H))
+        R.dwt_inv(ctx, RG_FILTER_97, d_qd, W, H, NC, LEVELS, step_f, d_rec, d_s)
+        R.colour_inv(ctx, d_rec, W, H, NC, RG_COLOUR_ICT, 1, d_u8)
+
+    from roger_amd._lib import check as check_rc
+    t_dec = timed(decode_once)
+    rec = d_u8.read(np.uint8, img.shape)
+    dec_maxerr = int(np.abs(rec.astype(int) - img.astype(int)).max())
     # measured device-to-device copy ceiling (read + write of the same bytes as the level-1 DWT)
     cp_n = 4 * W * H * NC
     d_cp = ctx.alloc(cp_n)
@@ -192,8 +241,10 @@ def main():
         "metric": METRIC, "value": round(value, 2), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "%dx%d RGB uint8 -> ICT -> 9/7 FP32 %d levels, step 1/64 -> EBCOT %dx%d "
-                               "(plain per-image path, input resident in HBM)" % (W, H, LEVELS, CB, CB),
+        "config": {"workload": "%dx%d RGB uint8 -> ICT -> 9/7 FP32 %d levels, step 1/64 -> EBCOT %dx%d, %s "
+                               "code-block style (plain per-image path, input resident in HBM)"
+                               % (W, H, LEVELS, CB, CB, "RESET|TERMALL" if STYLE else "standard"),
+                   "codeblock_style": STYLE,
                    "image": [W, H, NC], "levels": LEVELS, "delta": DELTA, "codeblock": [CB, CB],
                    "codeblocks": int(ncb), "codestream_bytes": total_bytes},
         "roofline": {"bound": "hbm", "kernel": "fwd_level_kernel<9/7> (level 1, 3 planes)",
@@ -203,12 +254,19 @@ def main():
         "hbm_copy_ceiling_GBps": round(copy_gbs, 1),
         "stages_ms": {"colour_ict": round(t_colour, 4), "colour_GBps": round(colour_gbs, 1),
                       "dwt97_quant_6lv": round(t_dwt, 4), "dwt97_level1": round(t_dwt_l1, 4),
-                      "t1_encode": round(t_t1, 4),
-                      "t1_codeblocks_per_s": round(len(desc) / (t_t1 * 1e-3), 1)},
+                      "t1_encode_standard_style": round(t_t1, 4),
+                      "t1_standard_codeblocks_per_s": round(len(desc) / (t_t1 * 1e-3), 1),
+                      "encode_parallel_style": round(t_enc_par, 4)},
+        "standard_style": {"mpix_s": round(mpix * max(3, args.steps // 4) / dt_other, 2) if STYLE else None,
+                           "codestream_bytes": int((res_other if STYLE else res)["len"].sum())},
+        "decode": {"what": "device-resident standard-style codewords -> tier-1 decode -> dequantisation + "
+                           "inverse 9/7 -> inverse ICT -> uint8 RGB",
+                   "ms": round(t_dec, 4), "mpix_s": round(mpix / (t_dec * 1e-3), 1),
+                   "max_abs_err_vs_input": dec_maxerr},
     }
 
     if not args.no_batch:
-        out["batched"] = bench_batched(ctx, args)
+        out["batched"] = bench_batched(ctx, args, STYLE)
     if not args.no_serve:
         out["serving"] = bench_serving(ctx, args)
 

@@ -70,7 +70,18 @@ typedef struct {
     double delta;          /* 9/7 base quantisation step (e.g. 1/64) */
     int32_t ll_fp16;       /* 9/7: store the inter-level LL band as FP16 */
     int32_t bitdepth;      /* sample precision (8) */
+    int32_t style;         /* code-block style: 0 or RG_STYLE_PARALLEL */
 } rg_encode_params;
+
+/* Code-block styles (T.800 Table A.19).  RG_STYLE_PARALLEL = RESET | TERMALL:
+ * every coding pass is its own MQ codeword segment starting from the initial
+ * contexts, so the passes of a block code in parallel (one GPU lane each);
+ * the codestream signals the style in COD and each segment's length in the
+ * packet headers.  About 1% larger codestreams than style 0. */
+#define RG_STYLE_RESET 0x02
+#define RG_STYLE_TERMALL 0x04
+#define RG_STYLE_PARALLEL 0x06
+#define RG_MAX_PASSES 96   /* stride of per-block pass arrays (>= 3*32-2) */
 
 /* ---- errors ---- */
 const char *rg_error_string(int code);
@@ -202,6 +213,10 @@ int rg_encoder_num_codeblocks(rg_encoder *enc);
  * concatenated codewords (cap bytes), offsets[i] their start.  Waits. */
 int rg_encoder_get_codeblocks(rg_encoder *enc, rg_cb_desc *desc, rg_cb_result *res, long *offsets,
                               uint8_t *data, long cap, long *total);
+/* TERMALL styles: the codeword segment length of every coding pass,
+ * RG_MAX_PASSES entries per code-block (ncb * RG_MAX_PASSES, host).  Waits.
+ * No counterpart in the reference (its bit-plane coder has no output). */
+int rg_encoder_get_pass_lengths(rg_encoder *enc, int32_t *seglen);
 /* Tier-2: the JPEG 2000 codestream (.j2k: one tile, one layer, LRCP) of the
  * last encode.  out == NULL only reports the size.  The reference has no
  * tier-2 (it stops after its bit-plane coder, src/OCLEncoder.cpp:43-52). */
@@ -239,10 +254,12 @@ int rg_t1_decode(rg_context *ctx, const uint8_t *d_data, const long *d_offsets, 
                  const rg_cb_desc *d_desc, int ncb, int32_t *d_q, int pitch, size_t plane_stride);
 /* Full native decode of one image from its code-blocks (host arrays, the
  * layout rg_encoder_get_codeblocks returns): tier-1 decode -> dequantisation
- * + inverse DWT -> inverse RCT/ICT -> interleaved uint8 on the device. */
+ * + inverse DWT -> inverse RCT/ICT -> interleaved uint8 on the device.
+ * p->style selects the code-block style; with TERMALL, seglen gives each
+ * block's pass segment lengths (RG_MAX_PASSES per block), else NULL. */
 int rg_decoder_decode_codeblocks(rg_decoder *dec, const rg_cb_desc *desc, const rg_cb_result *res,
                                  const long *offsets, const uint8_t *data, long total, int ncb, int w, int h,
-                                 int ncomp, const rg_encode_params *p);
+                                 int ncomp, const rg_encode_params *p, const int32_t *seglen);
 int rg_decoder_image_u8(rg_decoder *dec, uint8_t **d_img);
 int rg_decoder_coefficients(rg_decoder *dec, int32_t **d_q);
 
@@ -259,8 +276,9 @@ int rg_batch_create(rg_context *ctx, rg_batch **out);
 int rg_batch_destroy(rg_batch *b);
 /* Lays out all buffers and job tables (tier-1 jobs sorted largest block
  * first); with use_graph the pipeline (colour, every DWT level, tier-1) is
- * captured once into a HIP graph. */
-int rg_batch_plan(rg_batch *b, const rg_batch_image *imgs, int n, int cbw, int cbh, int use_graph);
+ * captured once into a HIP graph.  style: code-block style of every image
+ * (0 or RG_STYLE_PARALLEL). */
+int rg_batch_plan(rg_batch *b, const rg_batch_image *imgs, int n, int cbw, int cbh, int use_graph, int style);
 /* Runs (replays) the plan asynchronously on the context stream. */
 int rg_batch_run(rg_batch *b);
 /* Code-blocks of one image (image = -1: the whole batch). */
@@ -268,6 +286,9 @@ int rg_batch_num_codeblocks(rg_batch *b, int image);
 int rg_batch_coefficients(rg_batch *b, int image, int32_t **d_q);
 int rg_batch_get_codeblocks(rg_batch *b, int image, rg_cb_desc *desc, rg_cb_result *res, long *offsets,
                             uint8_t *data, long cap, long *total);
+/* TERMALL style: segment lengths of image `image`'s blocks (RG_MAX_PASSES
+ * each, host).  Waits. */
+int rg_batch_get_pass_lengths(rg_batch *b, int image, int32_t *seglen);
 int rg_batch_write_j2k(rg_batch *b, int image, uint8_t *out, long cap, long *size);
 
 /* ---- synthetic test images generated on the device (the seeded recipe of

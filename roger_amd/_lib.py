@@ -18,6 +18,8 @@ RG_ERR_OVERFLOW = -4
 RG_ERR_NOT_READY = -5
 
 RG_F32, RG_I16, RG_I32 = 0, 1, 2
+RG_STYLE_RESET, RG_STYLE_TERMALL, RG_STYLE_PARALLEL = 0x02, 0x04, 0x06
+RG_MAX_PASSES = 96
 RG_COLOUR_NONE, RG_COLOUR_RCT, RG_COLOUR_ICT = 0, 1, 2
 RG_FILTER_53, RG_FILTER_97 = 0, 1
 
@@ -52,7 +54,7 @@ class BatchImage(ctypes.Structure):
 class EncodeParams(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_int32), ("filter", ctypes.c_int32), ("colour", ctypes.c_int32),
                 ("cbw", ctypes.c_int32), ("cbh", ctypes.c_int32), ("delta", ctypes.c_double),
-                ("ll_fp16", ctypes.c_int32), ("bitdepth", ctypes.c_int32)]
+                ("ll_fp16", ctypes.c_int32), ("bitdepth", ctypes.c_int32), ("style", ctypes.c_int32)]
 
 
 P = ctypes.c_void_p
@@ -100,6 +102,7 @@ SIGNATURES = {
     "rg_encoder_float_coefficients": (I, [P, P]),
     "rg_encoder_num_codeblocks": (I, [P]),
     "rg_encoder_get_codeblocks": (I, [P, P, P, P, P, L, P]),
+    "rg_encoder_get_pass_lengths": (I, [P, P]),
     "rg_encoder_write_j2k": (I, [P, P, L, P]),
     "rg_encoder_map_dwt_out": (I, [P, P]),
     "rg_encoder_unmap_dwt_out": (I, [P, P]),
@@ -110,12 +113,13 @@ SIGNATURES = {
     "rg_decoder_idwt": (I, [P, I]),
     "rg_decoder_map_out": (I, [P, P]),
     "rg_t1_decode": (I, [P, P, P, P, P, I, P, I, SZ]),
-    "rg_decoder_decode_codeblocks": (I, [P, P, P, P, P, L, I, I, I, I, P]),
+    "rg_decoder_decode_codeblocks": (I, [P, P, P, P, P, L, I, I, I, I, P, P]),
     "rg_decoder_image_u8": (I, [P, P]),
     "rg_decoder_coefficients": (I, [P, P]),
     "rg_batch_create": (I, [P, P]),
     "rg_batch_destroy": (I, [P]),
-    "rg_batch_plan": (I, [P, P, I, I, I, I]),
+    "rg_batch_plan": (I, [P, P, I, I, I, I, I]),
+    "rg_batch_get_pass_lengths": (I, [P, I, P]),
     "rg_batch_run": (I, [P]),
     "rg_batch_num_codeblocks": (I, [P, I]),
     "rg_batch_coefficients": (I, [P, I, P]),

@@ -26,18 +26,19 @@ def _load():
         return {}
 
 
-def tune(ctx, batch, images, candidates=CANDIDATES, reps=2, log=None):
-    """Plans `batch` with each candidate code-block size, times `reps` runs,
-    leaves the batch planned with the best one and returns {cb, ms, table}."""
-    key = signature(images)
+def tune(ctx, batch, images, candidates=CANDIDATES, reps=2, log=None, style=0):
+    """Plans `batch` (code-block style `style`) with each candidate code-block
+    size, times `reps` runs, leaves the batch planned with the best one and
+    returns {cb, ms, table}."""
+    key = signature(images) + "_style%d" % style
     cache = _load()
     if key in cache and "cb" in cache[key]:
         best = cache[key]
-        batch.plan(images, cbw=best["cb"], cbh=best["cb"], use_graph=True)
+        batch.plan(images, cbw=best["cb"], cbh=best["cb"], use_graph=True, style=style)
         return dict(best, cached=True)
     table = []
     for cb in candidates:
-        batch.plan(images, cbw=cb, cbh=cb, use_graph=True)
+        batch.plan(images, cbw=cb, cbh=cb, use_graph=True, style=style)
         batch.run()
         ctx.sync()
         t = time.perf_counter()
@@ -55,5 +56,5 @@ def tune(ctx, batch, images, candidates=CANDIDATES, reps=2, log=None):
             json.dump(cache, f, indent=1)
     except OSError:
         pass
-    batch.plan(images, cbw=best["cb"], cbh=best["cb"], use_graph=True)
+    batch.plan(images, cbw=best["cb"], cbh=best["cb"], use_graph=True, style=style)
     return dict(best, cached=False, table=table)

@@ -58,12 +58,12 @@ void steps_for(int levels, int bitdepth, double delta, float *step, float *inv)
 struct rg_batch {
     rg_context *ctx = nullptr;
     std::vector<rg_batch_image> img;
-    int cbw = 64, cbh = 64, maxlev = 0, slot = 0, planned = 0, use_graph = 0, ran = 0;
+    int cbw = 64, cbh = 64, maxlev = 0, slot = 0, planned = 0, use_graph = 0, ran = 0, style = 0;
     std::vector<size_t> plane_off, scratch_off;     // element / byte offsets per image
     std::vector<int> cb_first, cb_count;            // per image, into the block list
     std::vector<rg_cb_desc> desc;                    // all blocks, image-major
     std::vector<int> desc_img;
-    Buf planes, q, scratch, slots, res, t1jobs, cparams, cjobs, syms, cm;
+    Buf planes, q, scratch, slots, res, t1jobs, cparams, cjobs, syms, cm, poffs, arena, seglens;
     Buf fargs[16], fjobs[16];
     int nfjobs[16] = {0}, ncjobs = 0, ncb = 0;
     hipGraph_t graph = nullptr;
@@ -93,7 +93,7 @@ extern "C" int rg_batch_destroy(rg_batch *b)
     hipStreamSynchronize(b->ctx->stream);
     drop_graph(b);
     for (Buf *x : {&b->planes, &b->q, &b->scratch, &b->slots, &b->res, &b->t1jobs, &b->cparams, &b->cjobs,
-                   &b->syms, &b->cm})
+                   &b->syms, &b->cm, &b->poffs, &b->arena, &b->seglens})
         x->release();
     for (int l = 0; l < 16; ++l) { b->fargs[l].release(); b->fjobs[l].release(); }
     delete b;
@@ -111,13 +111,16 @@ static int issue(rg_batch *b)
         if ((rc = rg_launch_fwd_level_batch(s, (const rg_fwd_level_args *)b->fargs[l].p, (const int4 *)b->fjobs[l].p,
                                             b->nfjobs[l])))
             return rc;
+    if (b->style == RG_STYLE_PARALLEL) return rg_launch_t1_jobs_parallel(s, b->t1jobs.p, b->ncb);
     if (rg_t1_inline()) return rg_launch_t1_jobs(s, b->t1jobs.p, b->ncb);
     return rg_launch_t1_jobs_split(s, b->t1jobs.p, b->ncb);
 }
 
-extern "C" int rg_batch_plan(rg_batch *b, const rg_batch_image *imgs, int n, int cbw, int cbh, int use_graph)
+extern "C" int rg_batch_plan(rg_batch *b, const rg_batch_image *imgs, int n, int cbw, int cbh, int use_graph,
+                             int style)
 {
     RG_CHECK(b && imgs && n > 0, "rg_batch_plan: bad arguments");
+    RG_CHECK(style == 0 || style == RG_STYLE_PARALLEL, "code-block style %d: only 0 and RG_STYLE_PARALLEL", style);
     RG_HIP(hipSetDevice(b->ctx->device));
     RG_HIP(hipStreamSynchronize(b->ctx->stream));
     drop_graph(b);
@@ -228,7 +231,14 @@ extern "C" int rg_batch_plan(rg_batch *b, const rg_batch_image *imgs, int n, int
     const int symcap = rg_t1_symcap(cbw, cbh);
     if ((rc = b->slots.ensure((size_t)b->slot * b->ncb)) || (rc = b->res.ensure(sizeof(rg_cb_result) * b->ncb)) ||
         (rc = b->cm.ensure(sizeof(int4) * b->ncb)) ||
-        (!rg_t1_inline() && (rc = b->syms.ensure((size_t)symcap * b->ncb))))
+        ((style || !rg_t1_inline()) && (rc = b->syms.ensure((size_t)symcap * b->ncb))))
+        return rc;
+    b->style = style;
+    const int maxp = rg_t1_maxp();
+    const size_t astride = rg_t1_seg_bytes(symcap);
+    if (style && ((rc = b->poffs.ensure(sizeof(int32_t) * maxp * b->ncb)) ||
+                  (rc = b->seglens.ensure(sizeof(int32_t) * maxp * b->ncb)) ||
+                  (rc = b->arena.ensure(astride * b->ncb))))
         return rc;
     std::vector<int> order(b->ncb);
     std::iota(order.begin(), order.end(), 0);
@@ -244,7 +254,10 @@ extern "C" int rg_batch_plan(rg_batch *b, const rg_batch_image *imgs, int n, int
                               (size_t)d.comp * m.w * m.h + (size_t)d.y0 * m.w + d.x0;
         rg_t1_job_fill(jobs.data() + (size_t)k * jb, base, m.w, d.w, d.h, d.orient,
                        (uint8_t *)b->slots.p + (size_t)idx * b->slot, b->slot, (rg_cb_result *)b->res.p + idx,
-                       b->syms.p ? (uint8_t *)b->syms.p + (size_t)k * symcap : nullptr, symcap, (int4 *)b->cm.p + k);
+                       b->syms.p ? (uint8_t *)b->syms.p + (size_t)k * symcap : nullptr, symcap, (int4 *)b->cm.p + k,
+                       style ? (int32_t *)b->poffs.p + (size_t)k * maxp : nullptr,
+                       style ? (uint8_t *)b->arena.p + (size_t)k * astride : nullptr,
+                       style ? (int32_t *)b->seglens.p + (size_t)idx * maxp : nullptr);
     }
     if ((rc = b->t1jobs.ensure(jobs.size()))) return rc;
     RG_HIP(hipMemcpy(b->t1jobs.p, jobs.data(), jobs.size(), hipMemcpyHostToDevice));
@@ -329,6 +342,20 @@ extern "C" int rg_batch_get_codeblocks(rg_batch *b, int image, rg_cb_desc *desc,
     return RG_SUCCESS;
 }
 
+extern "C" int rg_batch_get_pass_lengths(rg_batch *b, int image, int32_t *seglen)
+{
+    RG_CHECK(b && b->planned && seglen, "bad arguments");
+    RG_CHECK(image >= 0 && image < (int)b->img.size(), "image index out of range");
+    if (!b->ran) return rg_fail(RG_ERR_NOT_READY, "batch has not run");
+    RG_CHECK(b->style & RG_STYLE_TERMALL, "per-pass segments exist only with the TERMALL style");
+    RG_CHECK(rg_t1_maxp() == RG_MAX_PASSES, "pass-array stride mismatch");
+    const size_t first = (size_t)b->cb_first[image] * RG_MAX_PASSES;
+    RG_HIP(hipMemcpyAsync(seglen, (int32_t *)b->seglens.p + first, sizeof(int32_t) * RG_MAX_PASSES * b->cb_count[image],
+                          hipMemcpyDeviceToHost, b->ctx->stream));
+    RG_HIP(hipStreamSynchronize(b->ctx->stream));
+    return RG_SUCCESS;
+}
+
 extern "C" int rg_batch_write_j2k(rg_batch *b, int image, uint8_t *out, long cap, long *size)
 {
     RG_CHECK(b && b->planned, "batch not planned");
@@ -345,11 +372,17 @@ extern "C" int rg_batch_write_j2k(rg_batch *b, int image, uint8_t *out, long cap
         return rc;
     std::vector<const uint8_t *> ptr(cnt);
     for (int i = 0; i < cnt; ++i) ptr[i] = data.data() + offs[i];
+    std::vector<int32_t> seglen;
+    if (b->style & RG_STYLE_TERMALL) {
+        seglen.resize((size_t)RG_MAX_PASSES * cnt);
+        if ((rc = rg_batch_get_pass_lengths(b, image, seglen.data()))) return rc;
+    }
     const rg_batch_image &m = b->img[image];
     rg_t2_image im{m.w, m.h, m.ncomp, m.levels, b->cbw, b->cbh, m.filter == RG_FILTER_53,
-                   m.ncomp == 3 && m.colour != RG_COLOUR_NONE, 2, 8, m.delta};
+                   m.ncomp == 3 && m.colour != RG_COLOUR_NONE, 2, 8, m.delta, b->style};
     std::vector<uint8_t> cs;
-    if ((rc = rg_j2k_assemble(im, b->desc.data() + b->cb_first[image], res.data(), ptr.data(), cnt / m.ncomp, cs)))
+    if ((rc = rg_j2k_assemble(im, b->desc.data() + b->cb_first[image], res.data(), ptr.data(),
+                              seglen.empty() ? nullptr : seglen.data(), RG_MAX_PASSES, cnt / m.ncomp, cs)))
         return rc;
     if (size) *size = (long)cs.size();
     if (out) {

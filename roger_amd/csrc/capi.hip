@@ -432,10 +432,10 @@ struct rg_encoder {
     rg_context *ctx = nullptr;
     int w = 0, h = 0, ncomp = 0, levels = 0, cbw = 0, cbh = 0, filter = 0;
     int lossy = 0, output_dwt = 0, ran = 0, t1_ran = 0, loaded = 0, i16_ran = 0, bitdepth = 8;
-    int ncb_comp = 0, ncb = 0, slot_bytes = 0, colour = 0;
+    int ncb_comp = 0, ncb = 0, slot_bytes = 0, colour = 0, style = 0;
     double delta = 1.0 / 64;
     float step[97], inv[97];
-    DevBuf input, planes, q, fcoeff, scratch, desc, res, slots, chan16, syms, cm;
+    DevBuf input, planes, q, fcoeff, scratch, desc, res, slots, chan16, syms, cm, poffs, arena, seglens;
     HostBuf staging, map;
     std::vector<rg_cb_desc> h_desc;
     int desc_key[6] = {0, 0, 0, 0, 0, 0};
@@ -456,7 +456,7 @@ extern "C" int rg_encoder_destroy(rg_encoder *e)
     hipSetDevice(e->ctx->device);
     hipStreamSynchronize(e->ctx->stream);
     for (DevBuf *b : {&e->input, &e->planes, &e->q, &e->fcoeff, &e->scratch, &e->desc, &e->res, &e->slots, &e->chan16,
-                      &e->syms, &e->cm})
+                      &e->syms, &e->cm, &e->poffs, &e->arena, &e->seglens})
         b->release();
     e->staging.release();
     e->map.release();
@@ -492,6 +492,21 @@ static int run_t1(rg_encoder *e)
     e->slot_bytes = std::max(256, e->cbw * e->cbh * 3);
     if ((rc = e->res.ensure(sizeof(rg_cb_result) * e->ncb))) return rc;
     if ((rc = e->slots.ensure((size_t)e->slot_bytes * e->ncb))) return rc;
+    if (e->style == RG_STYLE_PARALLEL) {
+        const int symcap = rg_t1_symcap(e->cbw, e->cbh), maxp = rg_t1_maxp();
+        if ((rc = e->syms.ensure((size_t)symcap * e->ncb)) || (rc = e->cm.ensure(sizeof(int4) * e->ncb)) ||
+            (rc = e->poffs.ensure(sizeof(int32_t) * maxp * e->ncb)) ||
+            (rc = e->seglens.ensure(sizeof(int32_t) * maxp * e->ncb)) ||
+            (rc = e->arena.ensure(rg_t1_seg_bytes(symcap) * e->ncb)))
+            return rc;
+        rc = rg_launch_t1_parallel(e->ctx->stream, (const int32_t *)e->q.p, e->w, (size_t)e->w * e->h,
+                                   (const rg_cb_desc *)e->desc.p, e->ncb, (uint8_t *)e->syms.p, symcap,
+                                   (int4 *)e->cm.p, (int32_t *)e->poffs.p, (uint8_t *)e->arena.p,
+                                   (int32_t *)e->seglens.p, (uint8_t *)e->slots.p, e->slot_bytes,
+                                   (rg_cb_result *)e->res.p);
+        if (rc == RG_SUCCESS) e->t1_ran = 1;
+        return rc;
+    }
     if (!rg_t1_inline()) {
         const int symcap = rg_t1_symcap(e->cbw, e->cbh);
         if ((rc = e->syms.ensure((size_t)symcap * e->ncb)) || (rc = e->cm.ensure(sizeof(int4) * e->ncb))) return rc;
@@ -519,6 +534,8 @@ static int check_params(const rg_encode_params *p, int w, int h, int ncomp)
     RG_CHECK(p->colour != RG_COLOUR_ICT || p->filter == RG_FILTER_97, "ICT pairs with the 9/7 filter");
     RG_CHECK(p->filter == RG_FILTER_53 || (p->delta > 0 && std::isfinite(p->delta)), "bad quantisation step");
     RG_CHECK(p->bitdepth >= 1 && p->bitdepth <= 16, "bad bitdepth %d", p->bitdepth);
+    RG_CHECK(p->style == 0 || p->style == RG_STYLE_PARALLEL, "code-block style %d: only 0 and RG_STYLE_PARALLEL",
+             p->style);
     if (rg_codeblocks(w, h, p->levels, p->cbw, p->cbh, nullptr, 0) < 0) return RG_ERR_INVALID;
     return RG_SUCCESS;
 }
@@ -533,7 +550,7 @@ extern "C" int rg_encoder_encode_u8(rg_encoder *e, const uint8_t *src, int on_de
     const size_t npix = (size_t)w * h;
     e->w = w; e->h = h; e->ncomp = ncomp; e->levels = p->levels; e->cbw = p->cbw; e->cbh = p->cbh;
     e->filter = p->filter; e->lossy = p->filter == RG_FILTER_97; e->output_dwt = 0; e->bitdepth = p->bitdepth;
-    e->colour = p->colour; e->delta = p->delta;
+    e->colour = p->colour; e->delta = p->delta; e->style = p->style;
     e->ran = 0; e->t1_ran = 0; e->loaded = 1; e->i16_ran = 0;
     band_steps(p->levels, p->bitdepth, p->delta, e->step, e->inv);
     const uint8_t *d_src = src;
@@ -732,6 +749,19 @@ extern "C" int rg_encoder_get_codeblocks(rg_encoder *e, rg_cb_desc *desc, rg_cb_
     return RG_SUCCESS;
 }
 
+extern "C" int rg_encoder_get_pass_lengths(rg_encoder *e, int32_t *seglen)
+{
+    RG_CHECK(e && seglen, "null argument");
+    if (!e->t1_ran) return rg_fail(RG_ERR_NOT_READY, "tier-1 has not run");
+    RG_CHECK(e->style & RG_STYLE_TERMALL, "per-pass segments exist only with the TERMALL style");
+    static_assert(RG_MAX_PASSES == 96, "pass-array stride");
+    RG_CHECK(rg_t1_maxp() == RG_MAX_PASSES, "pass-array stride mismatch");
+    RG_HIP(hipMemcpyAsync(seglen, e->seglens.p, sizeof(int32_t) * RG_MAX_PASSES * e->ncb, hipMemcpyDeviceToHost,
+                          e->ctx->stream));
+    RG_HIP(hipStreamSynchronize(e->ctx->stream));
+    return RG_SUCCESS;
+}
+
 extern "C" int rg_encoder_write_j2k(rg_encoder *e, uint8_t *out, long cap, long *size)
 {
     RG_CHECK(e, "null encoder");
@@ -746,10 +776,17 @@ extern "C" int rg_encoder_write_j2k(rg_encoder *e, uint8_t *out, long cap, long 
         return rc;
     std::vector<const uint8_t *> ptr(e->ncb);
     for (int i = 0; i < e->ncb; ++i) ptr[i] = data.data() + offs[i];
+    std::vector<int32_t> seglen;
+    if (e->style & RG_STYLE_TERMALL) {
+        seglen.resize((size_t)RG_MAX_PASSES * e->ncb);
+        if ((rc = rg_encoder_get_pass_lengths(e, seglen.data()))) return rc;
+    }
     rg_t2_image im{e->w, e->h, e->ncomp, e->levels, e->cbw, e->cbh, e->filter == RG_FILTER_53,
-                   e->ncomp == 3 && e->colour != RG_COLOUR_NONE, 2, e->bitdepth, e->delta};
+                   e->ncomp == 3 && e->colour != RG_COLOUR_NONE, 2, e->bitdepth, e->delta, e->style};
     std::vector<uint8_t> cs;
-    if ((rc = rg_j2k_assemble(im, e->h_desc.data(), res.data(), ptr.data(), e->ncb_comp, cs))) return rc;
+    if ((rc = rg_j2k_assemble(im, e->h_desc.data(), res.data(), ptr.data(), seglen.empty() ? nullptr : seglen.data(),
+                              RG_MAX_PASSES, e->ncb_comp, cs)))
+        return rc;
     if (size) *size = (long)cs.size();
     if (out) {
         RG_CHECK(cap >= (long)cs.size(), "output buffer too small (%ld < %zu)", cap, cs.size());
@@ -799,28 +836,31 @@ extern "C" int rg_encoder_unmap_dwt_out(rg_encoder *e, void *host)
 struct rg_decoder {
     rg_context *ctx = nullptr;
     int w = 0, h = 0, ncomp = 0, lossy = 0, ran = 0, loaded = 0, u8_ran = 0;
-    DevBuf coeff, out, scratch, cbdata, cboffs, cbres, cbdesc, u8;
+    DevBuf coeff, out, scratch, cbdata, cboffs, cbres, cbdesc, u8, cbseg;
     HostBuf staging, map;
 };
 
 int rg_launch_t1_decode(hipStream_t s, const uint8_t *data, const long *offs, const rg_cb_result *res,
-                        const rg_cb_desc *desc, int ncb, int32_t *q, int pitch, size_t plane_stride);
+                        const rg_cb_desc *desc, int ncb, int32_t *q, int pitch, size_t plane_stride, int style,
+                        const int32_t *seglens);
 
 extern "C" int rg_t1_decode(rg_context *c, const uint8_t *d_data, const long *d_offsets, const rg_cb_result *d_res,
                             const rg_cb_desc *d_desc, int ncb, int32_t *d_q, int pitch, size_t plane_stride)
 {
     RG_CHECK(c && d_data && d_offsets && d_res && d_desc && d_q, "rg_t1_decode: null argument");
     RG_CHECK(ncb >= 0 && pitch > 0, "rg_t1_decode: bad sizes");
-    return rg_launch_t1_decode(c->stream, d_data, d_offsets, d_res, d_desc, ncb, d_q, pitch, plane_stride);
+    return rg_launch_t1_decode(c->stream, d_data, d_offsets, d_res, d_desc, ncb, d_q, pitch, plane_stride, 0, nullptr);
 }
 
 extern "C" int rg_decoder_decode_codeblocks(rg_decoder *d, const rg_cb_desc *desc, const rg_cb_result *res,
                                             const long *offsets, const uint8_t *data, long total, int ncb, int w,
-                                            int h, int ncomp, const rg_encode_params *p)
+                                            int h, int ncomp, const rg_encode_params *p, const int32_t *seglen)
 {
     RG_CHECK(d && desc && res && offsets && (data || total == 0), "rg_decoder_decode_codeblocks: null argument");
     int rc = check_params(p, w, h, ncomp);
     if (rc) return rc;
+    const bool termall = (p->style & RG_STYLE_TERMALL) != 0;
+    RG_CHECK(!termall || seglen, "TERMALL style needs the per-pass segment lengths");
     const int per = rg_codeblocks(w, h, p->levels, p->cbw, p->cbh, nullptr, 0);
     RG_CHECK(per * ncomp == ncb, "expected %d code-blocks, got %d", per * ncomp, ncb);
     for (int i = 0; i < ncb; ++i) {
@@ -831,32 +871,46 @@ extern "C" int rg_decoder_decode_codeblocks(rg_decoder *d, const rg_cb_desc *des
                  "code-block %d: bad codeword extent", i);
         RG_CHECK(res[i].nbp >= 0 && res[i].nbp <= 31 && res[i].npasses >= 0 && res[i].npasses <= 3 * res[i].nbp,
                  "code-block %d: bad pass counts", i);
+        if (termall) {
+            long sum = 0;
+            for (int k = 0; k < res[i].npasses; ++k) {
+                RG_CHECK(seglen[(size_t)i * RG_MAX_PASSES + k] >= 0, "code-block %d: negative segment length", i);
+                sum += seglen[(size_t)i * RG_MAX_PASSES + k];
+            }
+            RG_CHECK(sum == res[i].len, "code-block %d: segment lengths do not add up to the codeword", i);
+        }
     }
     RG_HIP(hipSetDevice(d->ctx->device));
     const size_t npix = (size_t)w * h;
     if ((rc = d->cbdata.ensure(total + 16)) || (rc = d->cboffs.ensure(sizeof(long) * ncb)) ||
         (rc = d->cbres.ensure(sizeof(rg_cb_result) * ncb)) || (rc = d->cbdesc.ensure(sizeof(rg_cb_desc) * ncb)) ||
         (rc = d->coeff.ensure(npix * ncomp * 4)) || (rc = d->out.ensure(npix * ncomp * 4)) ||
-        (rc = d->scratch.ensure(rg_dwt_scratch_bytes(w, h, ncomp, p->filter))) || (rc = d->u8.ensure(npix * ncomp)))
+        (rc = d->scratch.ensure(rg_dwt_scratch_bytes(w, h, ncomp, p->filter))) || (rc = d->u8.ensure(npix * ncomp)) ||
+        (termall && (rc = d->cbseg.ensure(sizeof(int32_t) * RG_MAX_PASSES * ncb))))
         return rc;
     hipStream_t s = d->ctx->stream;
     // stage the host arrays in pinned memory so the uploads are truly
     // asynchronous; the previous call's copies must have drained first
     RG_HIP(hipStreamSynchronize(s));
     const size_t o1 = (total + 15) / 16 * 16, o2 = o1 + sizeof(long) * ncb, o3 = o2 + sizeof(rg_cb_result) * ncb;
-    if ((rc = d->staging.ensure(o3 + sizeof(rg_cb_desc) * ncb))) return rc;
+    const size_t o4 = o3 + sizeof(rg_cb_desc) * ncb, segb = termall ? sizeof(int32_t) * RG_MAX_PASSES * ncb : 0;
+    if ((rc = d->staging.ensure(o4 + segb))) return rc;
     char *st = (char *)d->staging.p;
     if (total) memcpy(st, data, total);
     memcpy(st + o1, offsets, sizeof(long) * ncb);
     memcpy(st + o2, res, sizeof(rg_cb_result) * ncb);
     memcpy(st + o3, desc, sizeof(rg_cb_desc) * ncb);
+    if (termall) {
+        memcpy(st + o4, seglen, segb);
+        RG_HIP(hipMemcpyAsync(d->cbseg.p, st + o4, segb, hipMemcpyHostToDevice, s));
+    }
     RG_HIP(hipMemcpyAsync(d->cbdata.p, st, total, hipMemcpyHostToDevice, s));
     RG_HIP(hipMemcpyAsync(d->cboffs.p, st + o1, sizeof(long) * ncb, hipMemcpyHostToDevice, s));
     RG_HIP(hipMemcpyAsync(d->cbres.p, st + o2, sizeof(rg_cb_result) * ncb, hipMemcpyHostToDevice, s));
     RG_HIP(hipMemcpyAsync(d->cbdesc.p, st + o3, sizeof(rg_cb_desc) * ncb, hipMemcpyHostToDevice, s));
     if ((rc = rg_launch_t1_decode(s, (const uint8_t *)d->cbdata.p, (const long *)d->cboffs.p,
                                   (const rg_cb_result *)d->cbres.p, (const rg_cb_desc *)d->cbdesc.p, ncb,
-                                  (int32_t *)d->coeff.p, w, npix)))
+                                  (int32_t *)d->coeff.p, w, npix, p->style, termall ? (const int32_t *)d->cbseg.p : nullptr)))
         return rc;
     float step[97], inv[97];
     band_steps(p->levels, p->bitdepth, p->delta, step, inv);
@@ -899,7 +953,7 @@ extern "C" int rg_decoder_destroy(rg_decoder *d)
     if (!d) return RG_SUCCESS;
     hipSetDevice(d->ctx->device);
     hipStreamSynchronize(d->ctx->stream);
-    for (DevBuf *b : {&d->coeff, &d->out, &d->scratch, &d->cbdata, &d->cboffs, &d->cbres, &d->cbdesc, &d->u8})
+    for (DevBuf *b : {&d->coeff, &d->out, &d->scratch, &d->cbdata, &d->cboffs, &d->cbres, &d->cbdesc, &d->u8, &d->cbseg})
         b->release();
     d->staging.release(); d->map.release();
     delete d;

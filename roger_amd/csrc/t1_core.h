@@ -216,6 +216,11 @@ RG_DEV void mq_stripe(MQEnc &m, int &ctxv, int mqtv, const SymList &sl, int lane
     }
 }
 
+// Code-block styles (T.800 Table A.19) the coders implement.
+constexpr int STYLE_RESET = 0x02;     // contexts reset after every coding pass
+constexpr int STYLE_TERMALL = 0x04;   // codeword terminated after every pass
+constexpr int MAXP = 96;              // per-block pass-array stride (>= 3*32-2)
+
 // One code-block job: coefficients (row pitch), size, orientation, output slot.
 struct T1Job {
     const int32_t *src;
@@ -225,7 +230,17 @@ struct T1Job {
     uint8_t *sym;   // split path: this block's symbol stream (symcap bytes, 16-byte aligned)
     int symcap;
     i32x4 *cm;      // split path: {symbols, passes, bit-planes, overflow}
+    int32_t *poff;  // split path (may be null): stream offset where each pass starts (MAXP)
+    uint8_t *seg;   // parallel style: this block's segment arena (seg_bytes(symcap) bytes)
+    int32_t *seglen;// parallel style: codeword segment length of each pass (MAXP)
 };
+
+// Segment arena of the parallel style: pass p, whose symbols start at stream
+// offset s0, codes into [seg_at(s0, p), +2*(symbols)+16): an MQ step emits at
+// most 15 bits and a flush at most 5 bytes, so neighbouring passes never
+// overlap and every segment starts 16-byte aligned.
+RG_DEV int seg_at(int s0, int p) { return ((2 * s0 + 15) & ~15) + 32 * p; }
+inline int seg_bytes(int symcap) { return 2 * symcap + 32 * MAXP + 32; }
 
 RG_DEV int wave_incl_scan(int v, int lane)
 {
@@ -289,6 +304,7 @@ RG_DEV void t1_block(const T1Job &job)
         return;
     }
     int spos = 0, sovf = 0;   // split path: stream position / overflow (uniform)
+    int pidx = 0;             // split path: coding pass index
 
     // initial context states (Table D.7): ZC ctx 0 -> state 4, RL -> 3, UNIFORM -> 46, others 0
     int ctxv = lane == 0 ? (4 << 1) : lane == CTX_RL ? (3 << 1) : lane == CTX_UNI ? (46 << 1) : 0;
@@ -317,6 +333,8 @@ RG_DEV void t1_block(const T1Job &job)
         const uint64_t B = ((uint64_t)hi << 32) | lo;
 
         for (int pass = (p == nbp - 1) ? 2 : 0; pass < 3; ++pass) {
+            if (SPLIT && job.poff && lane == 0) job.poff[pidx] = spos;
+            ++pidx;
             for (int s = 0; s < nstripes; ++s) {
                 const int r0 = 4 * s;
                 const uint32_t vrow = colok ? (uint32_t)(rowmask >> r0) & 0xFu : 0u;
@@ -613,6 +631,92 @@ RG_DEV void t1_mq_lanes(uint32_t *tab, const T1Job *jobs, int i, int njobs, cons
     if ((len & 15) != 0 && (len & ~15) < cap) wv_store16(out + (len & ~15), m.o0, m.o1, m.o2, m.o3);
     if (len > cap) m.ovf = 1;
     *dst = rg_cb_result{len, c.y, c.z, (m.ovf || c.w) ? 1 : 0};
+}
+
+// ---- parallel style (RESET | TERMALL): one MQ chain per (code-block, pass) ----
+//
+// With both flags every coding pass starts from the initial context states
+// and a freshly initialised coder and ends in a flush, so the passes of a
+// block are independent.  A lane codes pass p of one block: symbols
+// [poff[p], poff[p+1]) of its stream into the block's segment arena, and
+// records the segment length; t1_pack_block then concatenates the segments.
+// `len_out` < 0 marks a segment that did not fit its arena region.
+RG_DEV void t1_mq_pass_lane(uint32_t *tab, bool active, const uint8_t *sym, int s0, int s1, uint8_t *seg, int cap,
+                            int32_t *len_out)
+{
+    const int lane = wv_lane();
+    if (lane < 47) tab[lane] = MQT[lane];
+    wv_sync();
+    const int c0 = active ? s0 >> 4 : 0, c1 = active ? (s1 + 15) >> 4 : 0;
+    int nch = c1 - c0, nmax = nch;
+    for (int o = 32; o >= 1; o >>= 1) {
+        const int t = wv_shfl_xor(nmax, o);
+        nmax = nmax > t ? nmax : t;
+    }
+    LaneMQ m;
+    m.A = 0x8000; m.C = 0; m.B = 0; m.CT = 12; m.bp = -1; m.ovf = 0;
+    m.o0 = m.o1 = m.o2 = m.o3 = 0;
+    uint32_t cs[5] = {4u << 1, 0, 0, 0, (3u << 1) << 8 | (46u << 1) << 16};
+    const int last = nch > 0 ? c1 - 1 : c0;
+    u32x4 nxt = wv_load16(sym + 16 * c0);
+    for (int i = 0; i < nmax; ++i) {
+        const u32x4 cur = nxt;
+        const int cn = c0 + i + 1;
+        nxt = wv_load16(sym + 16 * (cn < last ? cn : last));
+        const uint32_t w4[4] = {cur.x, cur.y, cur.z, cur.w};
+        const int base = 16 * (c0 + i);
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if (i < nch && base + k >= s0 && base + k < s1)
+                lane_encode(m, cs, tab, (w4[k >> 2] >> (8 * (k & 3))) & 0xFFu, seg, cap);
+    }
+    if (!active) return;
+    // FLUSH (C.2.9)
+    const uint32_t tmp = m.C + m.A;
+    m.C |= 0xFFFF;
+    if (m.C >= tmp) m.C -= 0x8000;
+    m.C <<= m.CT; lane_byteout(m, seg, cap);
+    m.C <<= m.CT; lane_byteout(m, seg, cap);
+    int len;
+    if (m.B != 0xFF) {
+        lane_put(m, m.B, m.bp, seg, cap);
+        len = m.bp + 1;
+    } else {
+        len = m.bp;
+    }
+    if ((len & 15) != 0 && (len & ~15) < cap) wv_store16(seg + (len & ~15), m.o0, m.o1, m.o2, m.o3);
+    *len_out = (m.ovf || len > cap) ? -1 : len;
+}
+
+// Coding pass p of a block, as the lane function needs it: its symbol range
+// and its arena region.  Returns false for a pass the block does not have.
+RG_DEV bool pass_range(const i32x4 &c, const int32_t *poff, int p, int &s0, int &s1, int &at, int &cap)
+{
+    if (p >= c.y) return false;
+    s0 = poff[p];
+    s1 = p + 1 < c.y ? poff[p + 1] : c.x;
+    at = seg_at(s0, p);
+    cap = 2 * (s1 - s0) + 16;
+    return true;
+}
+
+// One wavefront per block: concatenates the pass segments into the block's
+// output slot (lanes copy consecutive bytes) and writes the block result.
+RG_DEV void t1_pack_block(const i32x4 &c, const int32_t *poff, const uint8_t *seg, const int32_t *seglen,
+                          uint8_t *out, int cap, rg_cb_result *res)
+{
+    const int lane = wv_lane();
+    int ovf = c.w, pos = 0;
+    for (int p = 0; p < c.y; ++p) {
+        const int len = seglen[p];
+        if (len < 0) { ovf = 1; break; }
+        const uint8_t *src = seg + seg_at(poff[p], p);
+        for (int k = lane; k < len; k += 64)
+            if (pos + k < cap) out[pos + k] = src[k];
+        pos += len;
+    }
+    if (pos > cap) ovf = 1;
+    if (lane == 0) *res = rg_cb_result{c.z == 0 ? 0 : pos, c.y, c.z, ovf ? 1 : 0};
 }
 
 }  // namespace rg_t1

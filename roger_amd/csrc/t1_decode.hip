@@ -145,7 +145,18 @@ struct DecJob {
     const uint8_t *data;
     int len, npasses, nbp, w, h, orient, pitch;
     int32_t *dst;
+    int style;              // code-block style: RESET (0x02) / TERMALL (0x04) bits
+    const int32_t *seglen;  // TERMALL: codeword segment length of each pass
 };
+
+// INITDEC (C.3.5) on the codeword segment [off, off + len) of the LDS copy.
+__device__ __forceinline__ void mqd_start(MQDec &m, const uint8_t *lds, int off, int len)
+{
+    m.buf = lds + off; m.len = len; m.bp = 0;
+    m.C = mqd_byte(m, 0) << 16;
+    mqd_bytein(m);
+    m.C <<= 7; m.CT -= 7; m.A = 0x8000;
+}
 
 __device__ void t1_decode_block(const DecJob &job, uint8_t *lds)
 {
@@ -161,19 +172,28 @@ __device__ void t1_decode_block(const DecJob &job, uint8_t *lds)
     uint64_t S = 0, NEG = 0, V = 0, R = 0;
     const uint64_t rowmask = h >= 64 ? ~0ull : ((1ull << h) - 1);
     const int nstripes = (h + 3) >> 2;
-    int ctxv = lane == 0 ? (4 << 1) : lane == CTX_RL ? (3 << 1) : lane == CTX_UNI ? (46 << 1) : 0;
+    const int ctx0 = lane == 0 ? (4 << 1) : lane == CTX_RL ? (3 << 1) : lane == CTX_UNI ? (46 << 1) : 0;
+    int ctxv = ctx0;
     const int mqtv = lane < 47 ? (int)MQT_D[lane] : 0;
+    const bool termall = (job.style & 0x04) && job.seglen;
 
     MQDec m;
-    m.len = len; m.buf = lds; m.bp = 0;
-    m.C = mqd_byte(m, 0) << 16;
-    mqd_bytein(m);
-    m.C <<= 7; m.CT -= 7; m.A = 0x8000;
+    int seg_off = 0;
+    mqd_start(m, lds, 0, termall ? min(job.seglen[0], len) : len);
 
     int passes = 0;
     for (int p = nbp - 1; p >= 0; --p) {
         uint64_t B = 0;  // bit p of this lane's column
         for (int pass = (p == nbp - 1) ? 2 : 0; pass < 3 && passes < job.npasses; ++pass, ++passes) {
+            if (passes > 0) {
+                if (job.style & 0x02) ctxv = ctx0;      // RESET: initial context states
+                if (termall) {                          // TERMALL: next codeword segment
+                    seg_off += job.seglen[passes - 1];
+                    const int rest = len - seg_off;
+                    const int sl = job.seglen[passes];
+                    mqd_start(m, lds, seg_off < len ? seg_off : len, rest <= 0 ? 0 : (sl < rest ? sl : rest));
+                }
+            }
             for (int s = 0; s < nstripes; ++s) {
                 const int r0 = 4 * s;
                 const uint32_t vrow = (uint32_t)(rowmask >> r0) & 0xFu;
@@ -306,7 +326,8 @@ __device__ void t1_decode_block(const DecJob &job, uint8_t *lds)
 __global__ void __launch_bounds__(64) t1_decode_kernel(const uint8_t *__restrict__ data, const long *__restrict__ offs,
                                                        const rg_cb_result *__restrict__ res,
                                                        const rg_cb_desc *__restrict__ desc, int32_t *__restrict__ q,
-                                                       int pitch, size_t plane_stride)
+                                                       int pitch, size_t plane_stride, int style,
+                                                       const int32_t *__restrict__ seglens, int maxp)
 {
     __shared__ uint8_t lds[DEC_LDS];
     const rg_cb_desc d = desc[blockIdx.x];
@@ -316,16 +337,20 @@ __global__ void __launch_bounds__(64) t1_decode_kernel(const uint8_t *__restrict
     job.len = r.len; job.npasses = r.npasses; job.nbp = r.nbp;
     job.w = d.w; job.h = d.h; job.orient = d.orient; job.pitch = pitch;
     job.dst = q + (size_t)d.comp * plane_stride + (size_t)d.y0 * pitch + d.x0;
+    job.style = style;
+    job.seglen = seglens ? seglens + (size_t)blockIdx.x * maxp : nullptr;
     t1_decode_block(job, lds);
 }
 
 }  // namespace
 
 int rg_launch_t1_decode(hipStream_t s, const uint8_t *data, const long *offs, const rg_cb_result *res,
-                        const rg_cb_desc *desc, int ncb, int32_t *q, int pitch, size_t plane_stride)
+                        const rg_cb_desc *desc, int ncb, int32_t *q, int pitch, size_t plane_stride, int style,
+                        const int32_t *seglens)
 {
     if (ncb <= 0) return RG_SUCCESS;
-    t1_decode_kernel<<<dim3((unsigned)ncb), dim3(64), 0, s>>>(data, offs, res, desc, q, pitch, plane_stride);
+    t1_decode_kernel<<<dim3((unsigned)ncb), dim3(64), 0, s>>>(data, offs, res, desc, q, pitch, plane_stride, style,
+                                                              seglens, RG_MAX_PASSES);
     RG_LAUNCH_CHECK();
     return RG_SUCCESS;
 }

@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(64) t1_encode_jobs_kernel(const T1Job *__restr
 // ------------------------------------------------------------------------
 __global__ void __launch_bounds__(64) t1_cm_kernel(const int32_t *__restrict__ q, int pitch, size_t plane_stride,
                                                    const rg_cb_desc *__restrict__ desc, uint8_t *__restrict__ syms,
-                                                   int symcap, int4 *__restrict__ cm)
+                                                   int symcap, int4 *__restrict__ cm, int32_t *__restrict__ poffs)
 {
     const rg_cb_desc d = desc[blockIdx.x];
     T1Job job;
@@ -64,6 +64,7 @@ __global__ void __launch_bounds__(64) t1_cm_kernel(const int32_t *__restrict__ q
     job.sym = syms + (size_t)blockIdx.x * symcap;
     job.symcap = symcap;
     job.cm = cm + blockIdx.x;
+    job.poff = poffs ? poffs + (size_t)blockIdx.x * MAXP : nullptr;
     t1_block<true>(job);
 }
 
@@ -86,7 +87,82 @@ __global__ void __launch_bounds__(64) t1_mq_jobs_kernel(const T1Job *__restrict_
     t1_mq_lanes<true>(tab, jobs, blockIdx.x * 64 + threadIdx.x, njobs, nullptr, 0, nullptr, nullptr, 0, nullptr);
 }
 
+// ---- parallel style: grid (block groups of 64) x (pass index) ----
+__global__ void __launch_bounds__(64) t1_mqp_kernel(const uint8_t *__restrict__ syms, int symcap,
+                                                    const int4 *__restrict__ cm, const int32_t *__restrict__ poffs,
+                                                    int ncb, uint8_t *__restrict__ arena, size_t arena_stride,
+                                                    int32_t *__restrict__ seglens)
+{
+    __shared__ uint32_t tab[48];
+    const int b = blockIdx.x * 64 + threadIdx.x, p = blockIdx.y;
+    bool active = b < ncb;
+    int s0 = 0, s1 = 0, at = 0, cap = 0;
+    if (active) active = pass_range(cm[b], poffs + (size_t)b * MAXP, p, s0, s1, at, cap);
+    const size_t bb = active ? (size_t)b : 0;
+    t1_mq_pass_lane(tab, active, syms + bb * symcap, s0, s1, arena + bb * arena_stride + at, cap,
+                    seglens + bb * MAXP + p);
+}
+
+__global__ void __launch_bounds__(64) t1_mqp_jobs_kernel(const T1Job *__restrict__ jobs, int njobs)
+{
+    __shared__ uint32_t tab[48];
+    const int b = blockIdx.x * 64 + threadIdx.x, p = blockIdx.y;
+    bool active = b < njobs;
+    int s0 = 0, s1 = 0, at = 0, cap = 0;
+    const T1Job &j = jobs[active ? b : 0];
+    if (active) active = pass_range(*j.cm, j.poff, p, s0, s1, at, cap);
+    t1_mq_pass_lane(tab, active, j.sym, s0, s1, j.seg + at, cap, j.seglen + p);
+}
+
+__global__ void __launch_bounds__(64) t1_pack_kernel(const int4 *__restrict__ cm, const int32_t *__restrict__ poffs,
+                                                     const uint8_t *__restrict__ arena, size_t arena_stride,
+                                                     const int32_t *__restrict__ seglens, uint8_t *__restrict__ slots,
+                                                     int slot, rg_cb_result *__restrict__ res)
+{
+    const size_t b = blockIdx.x;
+    t1_pack_block(cm[b], poffs + b * MAXP, arena + b * arena_stride, seglens + b * MAXP, slots + b * slot, slot,
+                  res + b);
+}
+
+__global__ void __launch_bounds__(64) t1_pack_jobs_kernel(const T1Job *__restrict__ jobs)
+{
+    const T1Job &j = jobs[blockIdx.x];
+    t1_pack_block(*j.cm, j.poff, j.seg, j.seglen, j.out, j.cap, j.res);
+}
+
 }  // namespace
+
+int rg_t1_maxp() { return MAXP; }
+size_t rg_t1_seg_bytes(int symcap) { return (size_t)seg_bytes(symcap); }
+
+int rg_launch_t1_parallel(hipStream_t s, const int32_t *q, int pitch, size_t plane_stride, const rg_cb_desc *desc,
+                          int ncb, uint8_t *syms, int symcap, int4 *cm, int32_t *poffs, uint8_t *arena,
+                          int32_t *seglens, uint8_t *out, int slot_bytes, rg_cb_result *res)
+{
+    if (ncb <= 0) return RG_SUCCESS;
+    const size_t astride = rg_t1_seg_bytes(symcap);
+    t1_cm_kernel<<<dim3((unsigned)ncb), dim3(64), 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm, poffs);
+    RG_LAUNCH_CHECK();
+    t1_mqp_kernel<<<dim3((unsigned)((ncb + 63) / 64), (unsigned)(MAXP - 2)), dim3(64), 0, s>>>(
+        syms, symcap, cm, poffs, ncb, arena, astride, seglens);
+    RG_LAUNCH_CHECK();
+    t1_pack_kernel<<<dim3((unsigned)ncb), dim3(64), 0, s>>>(cm, poffs, arena, astride, seglens, out, slot_bytes, res);
+    RG_LAUNCH_CHECK();
+    return RG_SUCCESS;
+}
+
+int rg_launch_t1_jobs_parallel(hipStream_t s, const void *jobs, int njobs)
+{
+    if (njobs <= 0) return RG_SUCCESS;
+    t1_cm_jobs_kernel<<<dim3((unsigned)njobs), dim3(64), 0, s>>>((const T1Job *)jobs);
+    RG_LAUNCH_CHECK();
+    t1_mqp_jobs_kernel<<<dim3((unsigned)((njobs + 63) / 64), (unsigned)(MAXP - 2)), dim3(64), 0, s>>>(
+        (const T1Job *)jobs, njobs);
+    RG_LAUNCH_CHECK();
+    t1_pack_jobs_kernel<<<dim3((unsigned)njobs), dim3(64), 0, s>>>((const T1Job *)jobs);
+    RG_LAUNCH_CHECK();
+    return RG_SUCCESS;
+}
 
 // Symbol-stream capacity per code-block sample for the split path.
 int rg_t1_symcap(int cbw, int cbh) { return ((cbw * cbh * 24) + 15) / 16 * 16; }
@@ -95,7 +171,7 @@ int rg_launch_t1_split(hipStream_t s, const int32_t *q, int pitch, size_t plane_
                        uint8_t *syms, int symcap, int4 *cm, uint8_t *out, int slot_bytes, rg_cb_result *res)
 {
     if (ncb <= 0) return RG_SUCCESS;
-    t1_cm_kernel<<<dim3((unsigned)ncb), dim3(64), 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm);
+    t1_cm_kernel<<<dim3((unsigned)ncb), dim3(64), 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm, nullptr);
     RG_LAUNCH_CHECK();
     t1_mq_kernel<<<dim3((unsigned)((ncb + 63) / 64)), dim3(64), 0, s>>>(syms, symcap, cm, ncb, out, slot_bytes, res);
     RG_LAUNCH_CHECK();
@@ -124,11 +200,12 @@ int rg_launch_t1_jobs(hipStream_t s, const void *jobs, int njobs)
 
 // Host-side fill of one job record (the struct layout stays private to the tier-1 sources).
 void rg_t1_job_fill(void *dst, const int32_t *src, int pitch, int w, int h, int orient, uint8_t *out, int cap,
-                    rg_cb_result *res, uint8_t *sym, int symcap, int4 *cm)
+                    rg_cb_result *res, uint8_t *sym, int symcap, int4 *cm, int32_t *poff, uint8_t *seg,
+                    int32_t *seglen)
 {
     T1Job j;
     j.src = src; j.pitch = pitch; j.w = w; j.h = h; j.orient = orient; j.cap = cap; j.out = out; j.res = res;
-    j.sym = sym; j.symcap = symcap; j.cm = cm;
+    j.sym = sym; j.symcap = symcap; j.cm = cm; j.poff = poff; j.seg = seg; j.seglen = seglen;
     *reinterpret_cast<T1Job *>(dst) = j;
 }
 

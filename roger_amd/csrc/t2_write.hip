@@ -137,8 +137,10 @@ void rg_band_expmant(int Rb, double delta, int *eps, int *mu)
 }
 
 int rg_j2k_assemble(const rg_t2_image &im, const rg_cb_desc *desc, const rg_cb_result *res, const uint8_t *const *data,
-                    int ncb_per_comp, std::vector<uint8_t> &out)
+                    const int32_t *seglen, int maxp, int ncb_per_comp, std::vector<uint8_t> &out)
 {
+    const bool termall = (im.style & RG_STYLE_TERMALL) != 0;
+    if (termall && !seglen) return rg_fail(RG_ERR_INVALID, "TERMALL style needs the per-pass segment lengths");
     const int nb = 3 * im.levels + 1;
     int eps[97], mu[97];
     for (int b = 0; b < nb; ++b) {
@@ -157,7 +159,7 @@ int rg_j2k_assemble(const rg_t2_image &im, const rg_cb_desc *desc, const rg_cb_r
     for (int c = 0; c < im.ncomp; ++c) { s.u8(im.bitdepth - 1); s.u8(1); s.u8(1); }
     s.u16(0xFF52); s.u16(12); s.u8(0);                        // COD
     s.u8(0); s.u16(1); s.u8(im.mct);
-    s.u8(im.levels); s.u8(ilog2(im.cbw) - 2); s.u8(ilog2(im.cbh) - 2); s.u8(0); s.u8(im.reversible ? 1 : 0);
+    s.u8(im.levels); s.u8(ilog2(im.cbw) - 2); s.u8(ilog2(im.cbh) - 2); s.u8(im.style); s.u8(im.reversible ? 1 : 0);
     s.u16(0xFF5C);                                            // QCD
     if (im.reversible) {
         s.u16(3 + nb); s.u8((im.guard_bits << 5) | 0);
@@ -216,11 +218,19 @@ int rg_j2k_assemble(const rg_t2_image &im, const rg_cb_desc *desc, const rg_cb_r
                     if (q.npasses == 0) continue;
                     msb.code(hb, x, y, 1 << 20);
                     pass_count(hb, q.npasses);
-                    const int need = ilog2(q.len) + 1, have = 3 + ilog2(q.npasses);
-                    const int inc = need > have ? need - have : 0;
+                    // B.10.7: one Lblock increment, then each codeword segment's
+                    // length in Lblock + log2(passes in the segment) bits (one
+                    // segment per pass with TERMALL, else one for all passes)
+                    const int32_t *sl = termall ? seglen + (size_t)(cbase + bfirst[b] + i) * maxp : nullptr;
+                    const int nseg = termall ? q.npasses : 1, lp = termall ? 0 : ilog2(q.npasses);
+                    int inc = 0;
+                    for (int g = 0; g < nseg; ++g) {
+                        const int need = ilog2(termall ? sl[g] : q.len) + 1 - (3 + lp);
+                        inc = need > inc ? need : inc;
+                    }
                     for (int t = 0; t < inc; ++t) hb.bit(1);
                     hb.bit(0);
-                    hb.bits((uint32_t)q.len, 3 + inc + ilog2(q.npasses));
+                    for (int g = 0; g < nseg; ++g) hb.bits((uint32_t)(termall ? sl[g] : q.len), 3 + inc + lp);
                 }
             }
             hb.finish();

@@ -5,7 +5,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import (CbDesc, CbResult, EncodeParams, RG_COLOUR_ICT, RG_COLOUR_NONE, RG_COLOUR_RCT, RG_FILTER_53,
+from ._lib import (CbDesc, CbResult, EncodeParams, RG_MAX_PASSES, RG_COLOUR_ICT, RG_COLOUR_NONE, RG_COLOUR_RCT, RG_FILTER_53,
                    RG_FILTER_97, check, lib)
 
 
@@ -203,7 +203,7 @@ class Batch:
         check(lib().rg_batch_create(ctx.handle, ctypes.byref(h)))
         self.handle = h
 
-    def plan(self, images, cbw=64, cbh=64, use_graph=True):
+    def plan(self, images, cbw=64, cbh=64, use_graph=True, style=0):
         """images: list of dicts {buf: DeviceBuffer, w, h, ncomp, reversible, levels, delta}."""
         arr = (self._BatchImage * len(images))()
         for a, m in zip(arr, images):
@@ -215,7 +215,8 @@ class Batch:
             a.levels, a.delta = m["levels"], float(m.get("delta", 1.0 / 64))
         self.images = images
         check(lib().rg_batch_plan(self.handle, ctypes.cast(arr, ctypes.c_void_p), len(images), cbw, cbh,
-                                  int(use_graph)))
+                                  int(use_graph), int(style)))
+        self.style = int(style)
 
     def run(self):
         check(lib().rg_batch_run(self.handle))
@@ -245,6 +246,12 @@ class Batch:
         check(lib().rg_batch_get_codeblocks(self.handle, image, None, None, offs.ctypes.data_as(ctypes.c_void_p),
                                             data.ctypes.data_as(ctypes.c_void_p), data.size, None))
         return desc, res, [bytes(data[offs[i]:offs[i] + res["len"][i]]) for i in range(n)]
+
+    def pass_lengths(self, image):
+        """TERMALL style: [ncb(image), RG_MAX_PASSES] segment lengths."""
+        out = np.zeros((self.num_codeblocks(image), RG_MAX_PASSES), np.int32)
+        check(lib().rg_batch_get_pass_lengths(self.handle, image, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
 
     def j2k(self, image):
         size = ctypes.c_long()
@@ -307,7 +314,8 @@ class Decoder:
         self.handle = h
 
     def decode(self, descs, results, blobs, w, h, ncomp, levels=5, reversible=True, cbw=64, cbh=64,
-               delta=1.0 / 64):
+               delta=1.0 / 64, style=0, seglen=None):
+        """seglen: [ncb, RG_MAX_PASSES] pass segment lengths (TERMALL style)."""
         descs = np.ascontiguousarray(descs).view(CB_DESC_DTYPE)
         res = np.ascontiguousarray(results).view(CB_RES_DTYPE)
         data, offs = pack_codewords(blobs) if isinstance(blobs, list) else blobs
@@ -316,11 +324,15 @@ class Decoder:
         p.levels = levels
         p.filter = RG_FILTER_53 if reversible else RG_FILTER_97
         p.colour = RG_COLOUR_NONE if ncomp != 3 else (RG_COLOUR_RCT if reversible else RG_COLOUR_ICT)
-        p.cbw, p.cbh, p.delta, p.ll_fp16, p.bitdepth = cbw, cbh, float(delta), 0, 8
+        p.cbw, p.cbh, p.delta, p.ll_fp16, p.bitdepth, p.style = cbw, cbh, float(delta), 0, 8, int(style)
+        sl = None
+        if seglen is not None:
+            sl = np.ascontiguousarray(seglen, np.int32)
+            assert sl.shape == (len(res), RG_MAX_PASSES)
         check(lib().rg_decoder_decode_codeblocks(
             self.handle, descs.ctypes.data_as(ctypes.c_void_p), res.ctypes.data_as(ctypes.c_void_p),
             offs.ctypes.data_as(ctypes.c_void_p), data.ctypes.data_as(ctypes.c_void_p), pos, len(res), w, h,
-            ncomp, ctypes.byref(p)))
+            ncomp, ctypes.byref(p), None if sl is None else sl.ctypes.data_as(ctypes.c_void_p)))
         self.shape = (h, w) if ncomp == 1 else (h, w, ncomp)
 
     def image(self):
@@ -353,7 +365,7 @@ class Encoder:
         self.handle = h
 
     def encode_u8(self, img, levels=5, reversible=True, cbw=64, cbh=64, delta=1.0 / 64, ll_fp16=False,
-                  d_src=None, w=None, h=None, ncomp=None):
+                  d_src=None, w=None, h=None, ncomp=None, style=0):
         if d_src is None:
             img = np.ascontiguousarray(img, np.uint8)
             h, w = img.shape[:2]
@@ -366,7 +378,9 @@ class Encoder:
         p.filter = RG_FILTER_53 if reversible else RG_FILTER_97
         p.colour = RG_COLOUR_NONE if ncomp != 3 else (RG_COLOUR_RCT if reversible else RG_COLOUR_ICT)
         p.cbw, p.cbh, p.delta, p.ll_fp16, p.bitdepth = cbw, cbh, float(delta), int(ll_fp16), 8
+        p.style = int(style)
         self.shape = (w, h, ncomp)
+        self.style = int(style)
         check(lib().rg_encoder_encode_u8(self.handle, src, on_dev, w, h, ncomp, ctypes.byref(p)))
 
     def coefficients(self):
@@ -393,6 +407,13 @@ class Encoder:
                                               data.ctypes.data_as(ctypes.c_void_p), data.size, None))
         blobs = [bytes(data[offs[i]:offs[i] + res["len"][i]]) for i in range(n)]
         return desc, res, blobs
+
+    def pass_lengths(self):
+        """TERMALL styles: [ncb, RG_MAX_PASSES] codeword segment lengths per pass."""
+        n = check(lib().rg_encoder_num_codeblocks(self.handle))
+        out = np.zeros((n, RG_MAX_PASSES), np.int32)
+        check(lib().rg_encoder_get_pass_lengths(self.handle, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
 
     def j2k(self):
         """The JPEG 2000 codestream of the last encode (bytes)."""

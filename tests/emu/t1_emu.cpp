@@ -3,7 +3,7 @@
 // (not a restatement) can be checked against the oracle without a GPU.
 //
 //   t1_emu MODE DIR PITCH PLANE_STRIDE SLOT
-//     MODE  inline | split | queue_split
+//     MODE  inline | split | queue_split | parallel | queue_parallel
 //     DIR   holds q.bin (int32 coefficient planes) and desc.bin (rg_cb_desc
 //           records); res.bin (rg_cb_result per block) and out.bin (SLOT
 //           bytes per block) are written there.
@@ -11,6 +11,10 @@
 //   inline       t1_block<false>, one wave per block
 //   split        t1_block<true> per block, then t1_mq_lanes<false> per 64 blocks
 //   queue_split  the same over a job list (t1_mq_lanes<true>)
+//   parallel     RESET|TERMALL style: t1_block<true> recording pass offsets,
+//                t1_mq_pass_lane per (64 blocks, pass), t1_pack_block per block
+//   queue_parallel  the same reading every pointer from the job list
+// The parallel modes also write seglen.bin (MAXP int32 per block).
 // Exit status 3: an emulator check failed (message on stderr).
 #include <stdint.h>
 #include <stdio.h>
@@ -152,6 +156,27 @@ struct MQQueue {
     const T1Job *jobs;
     int i0, njobs;
 };
+
+// parallel style, driven the way t1_mqp_kernel / t1_mqp_jobs_kernel drive it
+struct MQPass {
+    const T1Job *jobs;
+    int i0, njobs, p;
+};
+void mq_pass_body(void *a)
+{
+    const MQPass &m = *static_cast<MQPass *>(a);
+    const int b = m.i0 + wv_lane();
+    bool active = b < m.njobs;
+    int s0 = 0, s1 = 0, at = 0, cap = 0;
+    const T1Job &j = m.jobs[active ? b : 0];
+    if (active) active = pass_range(*j.cm, j.poff, m.p, s0, s1, at, cap);
+    t1_mq_pass_lane(g_tab, active, j.sym, s0, s1, j.seg + at, cap, j.seglen + m.p);
+}
+void pack_body(void *a)
+{
+    const T1Job &j = *static_cast<T1Job *>(a);
+    t1_pack_block(*j.cm, j.poff, j.seg, j.seglen, j.out, j.cap, j.res);
+}
 void mq_queue_body(void *a)
 {
     const MQQueue &m = *static_cast<MQQueue *>(a);
@@ -182,7 +207,7 @@ void spill(const std::string &path, const void *p, size_t n)
 int main(int argc, char **argv)
 {
     if (argc != 6) {
-        fprintf(stderr, "usage: t1_emu inline|split|queue_split DIR PITCH PLANE_STRIDE SLOT\n");
+        fprintf(stderr, "usage: t1_emu MODE DIR PITCH PLANE_STRIDE SLOT\n");
         return 2;
     }
     const std::string mode = argv[1], dir = argv[2];
@@ -200,6 +225,9 @@ int main(int argc, char **argv)
     uint8_t *syms = static_cast<uint8_t *>(aligned_alloc(16, (size_t)symcap * (ncb ? ncb : 1)));
     std::vector<i32x4> cm(ncb);
     std::vector<T1Job> jobs(ncb);
+    const size_t astride = (size_t)seg_bytes(symcap);
+    uint8_t *arena = static_cast<uint8_t *>(aligned_alloc(16, astride * (ncb ? ncb : 1)));
+    std::vector<int32_t> poffs((size_t)MAXP * (ncb ? ncb : 1), 0), seglens((size_t)MAXP * (ncb ? ncb : 1), 0);
     for (int k = 0; k < ncb; ++k) {
         const rg_cb_desc &d = desc[k];
         T1Job &j = jobs[k];
@@ -210,6 +238,10 @@ int main(int argc, char **argv)
         j.sym = syms + (size_t)k * symcap;
         j.symcap = symcap;
         j.cm = &cm[k];
+        const bool par = mode == "parallel" || mode == "queue_parallel";
+        j.poff = par ? &poffs[(size_t)k * MAXP] : nullptr;
+        j.seg = arena + (size_t)k * astride;
+        j.seglen = &seglens[(size_t)k * MAXP];
     }
     if (mode == "inline") {
         for (int k = 0; k < ncb; ++k) emu::run(inline_body, &jobs[k]);
@@ -224,6 +256,15 @@ int main(int argc, char **argv)
                 emu::run(mq_queue_body, &m);
             }
         }
+    } else if (mode == "parallel" || mode == "queue_parallel") {
+        for (int k = 0; k < ncb; ++k) emu::run(cm_body, &jobs[k]);
+        for (int p = 0; p < MAXP - 2; ++p)
+            for (int i0 = 0; i0 < ncb; i0 += 64) {
+                MQPass m{jobs.data(), i0, ncb, p};
+                emu::run(mq_pass_body, &m);
+            }
+        for (int k = 0; k < ncb; ++k) emu::run(pack_body, &jobs[k]);
+        spill(dir + "/seglen.bin", seglens.data(), sizeof(int32_t) * seglens.size());
     } else {
         fprintf(stderr, "unknown mode %s\n", mode.c_str());
         return 2;
@@ -231,5 +272,6 @@ int main(int argc, char **argv)
     spill(dir + "/res.bin", res.data(), sizeof(rg_cb_result) * ncb);
     spill(dir + "/out.bin", slots.data(), slots.size());
     free(syms);
+    free(arena);
     return 0;
 }

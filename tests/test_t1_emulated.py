@@ -121,3 +121,59 @@ def test_small_slots_flag_overflow(emu, tmp_path, mode):
             assert blobs[i] == data
         else:
             assert bytes(raw[i]) == data[:slot]   # the slot holds the codeword's first bytes
+
+
+def _run_parallel(emu, tmp_path, mode, q, desc, pitch, plane_stride, slot):
+    res, blobs = _run(emu, tmp_path, mode, q, desc, pitch, plane_stride, slot)
+    seglen = np.fromfile(os.path.join(str(tmp_path), "seglen.bin"), np.int32).reshape(len(desc), O.MAX_PASSES)
+    return res, blobs, seglen
+
+
+@pytest.mark.parametrize("mode", ["parallel", "queue_parallel"])
+@pytest.mark.parametrize("w,h,nc,rev,levels,seed,cb", [
+    (97, 61, 1, False, 3, 52, 64),
+    (33, 1, 1, False, 1, 56, 64),
+    (64, 64, 3, False, 2, 54, 32),
+    (129, 57, 1, True, 4, 55, 64),
+])
+def test_parallel_style_matches_oracle(emu, tmp_path, mode, w, h, nc, rev, levels, seed, cb):
+    """RESET|TERMALL: pass-parallel MQ lanes + segment packing give the
+    oracle's codeword and per-pass segment lengths."""
+    img = synth.image(w, h, nc, seed)
+    rs, data, q, passlen = O.encode_image(img, levels=levels, reversible=rev, cbw=cb, cbh=cb, delta=1.0 / 64,
+                                          style=O.STYLE_PARALLEL, return_passlen=True)
+    one = O.codeblocks(w, h, levels, cb, cb)
+    desc = np.zeros(len(one) * nc, CB_DESC_DTYPE)
+    k = 0
+    for c in range(nc):
+        for d in one:
+            desc[k] = (d["x0"], d["y0"], d["w"], d["h"], c, d["orient"], d["band"], d["res"])
+            k += 1
+    res, blobs, seglen = _run_parallel(emu, tmp_path, mode, q, desc, w, w * h, max(256, cb * cb * 3))
+    assert (res["flags"] == 0).all()
+    np.testing.assert_array_equal(res["nbp"], rs[:, 2])
+    np.testing.assert_array_equal(res["npasses"], rs[:, 1])
+    for i in range(len(desc)):
+        n = rs[i, 1]
+        np.testing.assert_array_equal(seglen[i, :n], passlen[i, :n])
+    assert blobs == data
+
+
+def test_parallel_style_dense_blocks(emu, tmp_path):
+    rng = np.random.default_rng(23)
+    blocks = [np.round(rng.laplace(0, s, (64, 64))).astype(np.int32) for s in (2, 60, 3000, 40000)]
+    blocks.append(np.zeros((64, 64), np.int32))
+    plane = np.concatenate(blocks, axis=1)
+    desc = np.zeros(len(blocks), CB_DESC_DTYPE)
+    for i in range(len(blocks)):
+        desc[i] = (64 * i, 0, 64, 64, 0, (i + 1) % 4, 0, 0)
+    res, blobs, seglen = _run_parallel(emu, tmp_path, "parallel", plane, desc, plane.shape[1], plane.size,
+                                       64 * 64 * 4)
+    for i, b in enumerate(blocks):
+        if not b.any():
+            assert (res["len"][i], res["npasses"][i], res["nbp"][i]) == (0, 0, 0)
+            continue
+        data, npasses, nbp, passlen = O.t1_encode(b, (i + 1) % 4, style=O.STYLE_PARALLEL)
+        assert (res["nbp"][i], res["npasses"][i], res["flags"][i]) == (nbp, npasses, 0)
+        np.testing.assert_array_equal(seglen[i, :npasses], passlen)
+        assert blobs[i] == data
