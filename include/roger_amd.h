@@ -277,8 +277,11 @@ int rg_batch_destroy(rg_batch *b);
 /* Lays out all buffers and job tables (tier-1 jobs sorted largest block
  * first); with use_graph the pipeline (colour, every DWT level, tier-1) is
  * captured once into a HIP graph.  style: code-block style of every image
- * (0 or RG_STYLE_PARALLEL). */
-int rg_batch_plan(rg_batch *b, const rg_batch_image *imgs, int n, int cbw, int cbh, int use_graph, int style);
+ * (0 or RG_STYLE_PARALLEL).  waves_per_cu > 0: tier-1 context modelling runs
+ * as a persistent grid of that many wavefronts per CU pulling code-blocks
+ * from a device work queue; 0: one workgroup per code-block. */
+int rg_batch_plan(rg_batch *b, const rg_batch_image *imgs, int n, int cbw, int cbh, int use_graph, int style,
+                  int waves_per_cu);
 /* Runs (replays) the plan asynchronously on the context stream. */
 int rg_batch_run(rg_batch *b);
 /* Code-blocks of one image (image = -1: the whole batch). */

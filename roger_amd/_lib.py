@@ -118,7 +118,7 @@ SIGNATURES = {
     "rg_decoder_coefficients": (I, [P, P]),
     "rg_batch_create": (I, [P, P]),
     "rg_batch_destroy": (I, [P]),
-    "rg_batch_plan": (I, [P, P, I, I, I, I, I]),
+    "rg_batch_plan": (I, [P, P, I, I, I, I, I, I]),
     "rg_batch_get_pass_lengths": (I, [P, I, P]),
     "rg_batch_run": (I, [P]),
     "rg_batch_num_codeblocks": (I, [P, I]),

@@ -7,10 +7,10 @@
 //   colour : one workgroup per {image, 4096-pixel chunk};
 //   DWT    : per level, one workgroup per {image, plane, 64x64 tile}, both
 //            filters in the same launch;
-//   tier-1 : one workgroup per code-block, largest blocks first (the split
-//            path adds one MQ launch with one lane per code-block).
-// Running a plan issues (or replays) 1 + levels + 1 (inline tier-1) or
-// 1 + levels + 2 (split tier-1) launches.
+//   tier-1 : code-blocks sorted largest first, either one workgroup per
+//            block or a persistent grid (waves_per_cu per CU) pulling block
+//            indices from a device counter; the split styles add the
+//            lane-parallel MQ launch (and, for the parallel style, packing).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -58,12 +58,12 @@ void steps_for(int levels, int bitdepth, double delta, float *step, float *inv)
 struct rg_batch {
     rg_context *ctx = nullptr;
     std::vector<rg_batch_image> img;
-    int cbw = 64, cbh = 64, maxlev = 0, slot = 0, planned = 0, use_graph = 0, ran = 0, style = 0;
+    int cbw = 64, cbh = 64, maxlev = 0, slot = 0, planned = 0, use_graph = 0, ran = 0, style = 0, nwaves = 0;
     std::vector<size_t> plane_off, scratch_off;     // element / byte offsets per image
     std::vector<int> cb_first, cb_count;            // per image, into the block list
     std::vector<rg_cb_desc> desc;                    // all blocks, image-major
     std::vector<int> desc_img;
-    Buf planes, q, scratch, slots, res, t1jobs, cparams, cjobs, syms, cm, poffs, arena, seglens;
+    Buf planes, q, scratch, slots, res, t1jobs, cparams, cjobs, syms, cm, poffs, arena, seglens, counter;
     Buf fargs[16], fjobs[16];
     int nfjobs[16] = {0}, ncjobs = 0, ncb = 0;
     hipGraph_t graph = nullptr;
@@ -93,7 +93,7 @@ extern "C" int rg_batch_destroy(rg_batch *b)
     hipStreamSynchronize(b->ctx->stream);
     drop_graph(b);
     for (Buf *x : {&b->planes, &b->q, &b->scratch, &b->slots, &b->res, &b->t1jobs, &b->cparams, &b->cjobs,
-                   &b->syms, &b->cm, &b->poffs, &b->arena, &b->seglens})
+                   &b->syms, &b->cm, &b->poffs, &b->arena, &b->seglens, &b->counter})
         x->release();
     for (int l = 0; l < 16; ++l) { b->fargs[l].release(); b->fjobs[l].release(); }
     delete b;
@@ -111,15 +111,17 @@ static int issue(rg_batch *b)
         if ((rc = rg_launch_fwd_level_batch(s, (const rg_fwd_level_args *)b->fargs[l].p, (const int4 *)b->fjobs[l].p,
                                             b->nfjobs[l])))
             return rc;
+    if (b->nwaves > 0) return rg_launch_t1_queue(s, b->t1jobs.p, b->ncb, b->style, (int *)b->counter.p, b->nwaves);
     if (b->style == RG_STYLE_PARALLEL) return rg_launch_t1_jobs_parallel(s, b->t1jobs.p, b->ncb);
     if (rg_t1_inline()) return rg_launch_t1_jobs(s, b->t1jobs.p, b->ncb);
     return rg_launch_t1_jobs_split(s, b->t1jobs.p, b->ncb);
 }
 
 extern "C" int rg_batch_plan(rg_batch *b, const rg_batch_image *imgs, int n, int cbw, int cbh, int use_graph,
-                             int style)
+                             int style, int waves_per_cu)
 {
     RG_CHECK(b && imgs && n > 0, "rg_batch_plan: bad arguments");
+    RG_CHECK(waves_per_cu >= 0 && waves_per_cu <= 32, "waves_per_cu %d out of range 0..32", waves_per_cu);
     RG_CHECK(style == 0 || style == RG_STYLE_PARALLEL, "code-block style %d: only 0 and RG_STYLE_PARALLEL", style);
     RG_HIP(hipSetDevice(b->ctx->device));
     RG_HIP(hipStreamSynchronize(b->ctx->stream));
@@ -230,7 +232,7 @@ extern "C" int rg_batch_plan(rg_batch *b, const rg_batch_image *imgs, int n, int
     b->slot = std::max(256, cbw * cbh * 3);
     const int symcap = rg_t1_symcap(cbw, cbh);
     if ((rc = b->slots.ensure((size_t)b->slot * b->ncb)) || (rc = b->res.ensure(sizeof(rg_cb_result) * b->ncb)) ||
-        (rc = b->cm.ensure(sizeof(int4) * b->ncb)) ||
+        (rc = b->cm.ensure(sizeof(int4) * b->ncb)) || (rc = b->counter.ensure(256)) ||
         ((style || !rg_t1_inline()) && (rc = b->syms.ensure((size_t)symcap * b->ncb))))
         return rc;
     b->style = style;
@@ -261,6 +263,12 @@ extern "C" int rg_batch_plan(rg_batch *b, const rg_batch_image *imgs, int n, int
     }
     if ((rc = b->t1jobs.ensure(jobs.size()))) return rc;
     RG_HIP(hipMemcpy(b->t1jobs.p, jobs.data(), jobs.size(), hipMemcpyHostToDevice));
+    b->nwaves = 0;
+    if (waves_per_cu > 0) {
+        hipDeviceProp_t prop;
+        RG_HIP(hipGetDeviceProperties(&prop, b->ctx->device));
+        b->nwaves = std::max(1, std::min(b->ncb, prop.multiProcessorCount * waves_per_cu));
+    }
     b->use_graph = use_graph;
     if (use_graph) {
         RG_HIP(hipStreamBeginCapture(b->ctx->stream, hipStreamCaptureModeThreadLocal));

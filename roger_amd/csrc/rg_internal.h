@@ -128,6 +128,9 @@ int rg_launch_t1_parallel(hipStream_t s, const int32_t *q, int pitch, size_t pla
                           int ncb, uint8_t *syms, int symcap, int4 *cm, int32_t *poffs, uint8_t *arena,
                           int32_t *seglens, uint8_t *out, int slot_bytes, rg_cb_result *res);
 int rg_launch_t1_jobs_parallel(hipStream_t s, const void *jobs, int njobs);
+// Persistent batched tier-1 (any style): nwaves workgroups pull job indices
+// from *counter (reset by a kernel inside the launch sequence).
+int rg_launch_t1_queue(hipStream_t s, const void *jobs, int njobs, int style, int *counter, int nwaves);
 
 // Batched colour: per-image parameters; jobs {image, first chunk} cover 256
 // chunks of 16 pixels each.

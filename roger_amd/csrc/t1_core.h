@@ -131,7 +131,7 @@ struct MQEnc {
 RG_DEV void mq_store(MQEnc &m, int lane)
 {
     if (m.bp >= 0) {
-        if (m.bp < m.cap) { if (lane == 0) m.out[m.bp] = (uint8_t)m.B; }
+        if (m.bp < m.cap) m.out[m.bp] = (uint8_t)m.B;   // B is uniform: every lane stores the same byte
         else m.overflow = 1;
     }
 }
@@ -296,11 +296,12 @@ RG_DEV void t1_block(const T1Job &job)
     // the butterfly leaves the same value in every lane; saying so keeps the
     // whole bit-plane loop (and the MQ coder) in scalar registers
     const int nbp = wv_uniform(orm ? 32 - wv_clz(orm) : 0);
+    // (block results are stored by every lane -- the same value to the same
+    // address -- so no statement of the block body depends on the lane index
+    // and a caller's loop around it stays wave-uniform)
     if (nbp == 0) {
-        if (lane == 0) {
-            if (SPLIT) *job.cm = mk_i32x4(0, 0, 0, 0);
-            else *job.res = rg_cb_result{0, 0, 0, 0};
-        }
+        if (SPLIT) *job.cm = mk_i32x4(0, 0, 0, 0);
+        else *job.res = rg_cb_result{0, 0, 0, 0};
         return;
     }
     int spos = 0, sovf = 0;   // split path: stream position / overflow (uniform)
@@ -333,7 +334,7 @@ RG_DEV void t1_block(const T1Job &job)
         const uint64_t B = ((uint64_t)hi << 32) | lo;
 
         for (int pass = (p == nbp - 1) ? 2 : 0; pass < 3; ++pass) {
-            if (SPLIT && job.poff && lane == 0) job.poff[pidx] = spos;
+            if (SPLIT && job.poff) job.poff[pidx] = spos;   // uniform value, stored by every lane
             ++pidx;
             for (int s = 0; s < nstripes; ++s) {
                 const int r0 = 4 * s;
@@ -457,7 +458,7 @@ RG_DEV void t1_block(const T1Job &job)
     }
 
     if constexpr (SPLIT) {
-        if (lane == 0) *job.cm = mk_i32x4(spos, 3 * nbp - 2, nbp, sovf);
+        *job.cm = mk_i32x4(spos, 3 * nbp - 2, nbp, sovf);
         return;
     }
     // ---- FLUSH (C.2.9) ----
@@ -471,7 +472,7 @@ RG_DEV void t1_block(const T1Job &job)
         if (m.B != 0xFF) { mq_store(m, lane); len = m.bp + 1; }
         else len = m.bp;
         if (len > job.cap) m.overflow = 1;
-        if (lane == 0) *job.res = rg_cb_result{len, 3 * nbp - 2, nbp, m.overflow ? 1 : 0};
+        *job.res = rg_cb_result{len, 3 * nbp - 2, nbp, m.overflow ? 1 : 0};
     }
 }
 

@@ -47,6 +47,39 @@ __global__ void __launch_bounds__(64) t1_encode_jobs_kernel(const T1Job *__restr
     t1_block<false>(jobs[blockIdx.x]);
 }
 
+// Persistent form of the batched tier-1: a grid sized to the machine pulls job
+// indices from a device counter (jobs sorted largest block first) until the
+// list is exhausted.  Every lane executes the atomic (lane 0 adds 1, the
+// others 0) and takes lane 0's result with readlane, and the block body has no
+// lane-dependent statement, so the loop's exit is wave-uniform by
+// construction: the index never comes from a broadcast that assumes the lanes
+// are converged.
+template <bool SPLIT>
+__device__ __forceinline__ void t1_queue_loop(const T1Job *__restrict__ jobs, int njobs, int *__restrict__ counter)
+{
+    for (;;) {
+        const int got = atomicAdd(counter, threadIdx.x == 0 ? 1 : 0);
+        const int i = __builtin_amdgcn_readlane(got, 0);
+        if (i >= njobs) break;
+        t1_block<SPLIT>(jobs[i]);
+    }
+}
+
+__global__ void __launch_bounds__(64) t1_encode_queue_kernel(const T1Job *__restrict__ jobs, int njobs,
+                                                             int *__restrict__ counter)
+{
+    t1_queue_loop<false>(jobs, njobs, counter);
+}
+
+__global__ void __launch_bounds__(64) t1_cm_queue_kernel(const T1Job *__restrict__ jobs, int njobs,
+                                                         int *__restrict__ counter)
+{
+    t1_queue_loop<true>(jobs, njobs, counter);
+}
+
+// Queue counter reset by a kernel (not a memset node: graph replays stay exact).
+__global__ void reset_counter_kernel(int *counter) { *counter = 0; }
+
 // ------------------------------------------------------------------------
 // Split path.  Context modelling runs one wavefront per code-block and writes
 // the block's (context, bit) stream; the MQ coder then runs one LANE per
@@ -189,6 +222,32 @@ int rg_launch_t1_jobs_split(hipStream_t s, const void *jobs, int njobs)
 }
 
 int rg_t1_job_bytes() { return (int)sizeof(T1Job); }
+
+// Persistent forms: reset the queue counter, then `nwaves` workgroups pull
+// code-blocks until the job list is exhausted.
+int rg_launch_t1_queue(hipStream_t s, const void *jobs, int njobs, int style, int *counter, int nwaves)
+{
+    if (njobs <= 0) return RG_SUCCESS;
+    reset_counter_kernel<<<1, 1, 0, s>>>(counter);
+    RG_LAUNCH_CHECK();
+    const dim3 g((unsigned)nwaves), b(64);
+    const T1Job *J = (const T1Job *)jobs;
+    if (style == RG_STYLE_PARALLEL) {
+        t1_cm_queue_kernel<<<g, b, 0, s>>>(J, njobs, counter);
+        RG_LAUNCH_CHECK();
+        t1_mqp_jobs_kernel<<<dim3((unsigned)((njobs + 63) / 64), (unsigned)(MAXP - 2)), b, 0, s>>>(J, njobs);
+        RG_LAUNCH_CHECK();
+        t1_pack_jobs_kernel<<<dim3((unsigned)njobs), b, 0, s>>>(J);
+    } else if (rg_t1_inline()) {
+        t1_encode_queue_kernel<<<g, b, 0, s>>>(J, njobs, counter);
+    } else {
+        t1_cm_queue_kernel<<<g, b, 0, s>>>(J, njobs, counter);
+        RG_LAUNCH_CHECK();
+        t1_mq_jobs_kernel<<<dim3((unsigned)((njobs + 63) / 64)), b, 0, s>>>(J, njobs);
+    }
+    RG_LAUNCH_CHECK();
+    return RG_SUCCESS;
+}
 
 int rg_launch_t1_jobs(hipStream_t s, const void *jobs, int njobs)
 {

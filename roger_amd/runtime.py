@@ -203,7 +203,7 @@ class Batch:
         check(lib().rg_batch_create(ctx.handle, ctypes.byref(h)))
         self.handle = h
 
-    def plan(self, images, cbw=64, cbh=64, use_graph=True, style=0):
+    def plan(self, images, cbw=64, cbh=64, use_graph=True, style=0, waves_per_cu=0):
         """images: list of dicts {buf: DeviceBuffer, w, h, ncomp, reversible, levels, delta}."""
         arr = (self._BatchImage * len(images))()
         for a, m in zip(arr, images):
@@ -215,7 +215,7 @@ class Batch:
             a.levels, a.delta = m["levels"], float(m.get("delta", 1.0 / 64))
         self.images = images
         check(lib().rg_batch_plan(self.handle, ctypes.cast(arr, ctypes.c_void_p), len(images), cbw, cbh,
-                                  int(use_graph), int(style)))
+                                  int(use_graph), int(style), int(waves_per_cu)))
         self.style = int(style)
 
     def run(self):

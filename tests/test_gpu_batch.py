@@ -1,5 +1,6 @@
 """GPU: the batched encode path (one launch per stage over many images of
-uneven sizes, HIP-graph replay, tier-1 job list sorted largest first) produces, for
+uneven sizes, HIP-graph replay, tier-1 job list sorted largest first, per-block
+grid or persistent work-queue grid) produces, for
 every image, exactly the code-blocks the CPU oracle produces; and the device
 image generator reproduces the numpy generator bit-exactly."""
 import numpy as np
@@ -25,16 +26,17 @@ SPECS = [  # w, h, ncomp, reversible, levels
 ]
 
 
-@pytest.mark.parametrize("use_graph", [True, False])
+@pytest.mark.parametrize("use_graph,wpc", [(True, 0), (False, 0), (True, 4), (False, 2)])
 @pytest.mark.parametrize("cb", [64, 32])
-def test_batch_matches_oracle(ctx, use_graph, cb):
+def test_batch_matches_oracle(ctx, use_graph, wpc, cb):
+    """wpc > 0: tier-1 as a persistent grid pulling from the device queue."""
     imgs = []
     for i, (w, h, nc, rev, lv) in enumerate(SPECS):
         buf = synth.device_image(ctx, w, h, nc, 50 + i)
         imgs.append(dict(buf=buf, w=w, h=h, ncomp=nc, reversible=rev, levels=lv, delta=1.0 / 64,
                          host=synth.image(w, h, nc, 50 + i)))
     b = R.Batch(ctx)
-    b.plan(imgs, cbw=cb, cbh=cb, use_graph=use_graph)
+    b.plan(imgs, cbw=cb, cbh=cb, use_graph=use_graph, waves_per_cu=wpc)
     for rep in range(2):  # the second run replays the plan (counter reset inside)
         b.run()
         ctx.sync()

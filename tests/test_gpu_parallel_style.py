@@ -71,7 +71,8 @@ def test_encoder_parallel_style_vs_oracle(ctx, w, h, nc, rev, levels, cb):
     enc.close()
 
 
-def test_batch_parallel_style_vs_oracle(ctx):
+@pytest.mark.parametrize("wpc", [0, 4])
+def test_batch_parallel_style_vs_oracle(ctx, wpc):
     specs = [(256, 256, 1, True, 5), (300, 190, 3, False, 5), (97, 61, 1, False, 3), (64, 64, 3, False, 2),
              (33, 1, 1, False, 1), (129, 257, 1, True, 4)]
     imgs = []
@@ -80,7 +81,7 @@ def test_batch_parallel_style_vs_oracle(ctx):
         imgs.append(dict(buf=buf, w=w, h=h, ncomp=nc, reversible=rev, levels=lv, delta=1.0 / 64,
                          host=synth.image(w, h, nc, 70 + i)))
     b = R.Batch(ctx)
-    b.plan(imgs, cbw=64, cbh=64, use_graph=True, style=PAR)
+    b.plan(imgs, cbw=64, cbh=64, use_graph=True, style=PAR, waves_per_cu=wpc)
     for rep in range(2):
         b.run()
         ctx.sync()
