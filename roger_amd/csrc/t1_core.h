@@ -242,23 +242,29 @@ struct T1Job {
 RG_DEV int seg_at(int s0, int p) { return ((2 * s0 + 15) & ~15) + 32 * p; }
 inline int seg_bytes(int symcap) { return 2 * symcap + 32 * MAXP + 32; }
 
-RG_DEV int wave_incl_scan(int v, int lane)
+// Exclusive prefix sum over the wavefront of a per-lane count n (0..31) and
+// the wave total, from five ballots of n's bits (mbcnt counts the set bits
+// of a ballot below the lane): no cross-lane data movement through LDS.
+RG_DEV int wave_excl_count(int n, int &total)
 {
-    for (int o = 1; o < 64; o <<= 1) {
-        const int t = wv_shfl_up(v, o);
-        if (lane >= o) v += t;
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) {
+        const uint64_t m = wv_ballot((n >> b) & 1);
+        pre += wv_mbcnt(m) << b;
+        tot += __builtin_popcountll(m) << b;
     }
-    return v;
+    total = tot;
+    return pre;
 }
 
 // Split path: appends one stripe's symbols (scan order) to the block's stream.
 RG_DEV void emit_stripe(const SymList &sl, const T1Job &job, int &pos, int &ovf, int lane)
 {
-    const int incl = wave_incl_scan(sl.n, lane);
-    const int total = wv_readlane(incl, 63);
+    int total;
+    const int off = pos + wave_excl_count(sl.n, total);
     if (total == 0) return;
     if (pos + total > job.symcap) { ovf = 1; return; }
-    const int off = pos + incl - sl.n;
     uint64_t a = sl.lo, b = sl.hi;
     for (int t = 0; t < 16; ++t) {
         if (!wv_any(t < sl.n)) break;
@@ -340,9 +346,7 @@ RG_DEV void t1_block(const T1Job &job)
                 const int r0 = 4 * s;
                 const uint32_t vrow = colok ? (uint32_t)(rowmask >> r0) & 0xFu : 0u;
                 const uint32_t sw = win6(S, r0);
-                uint32_t lw = wv_shfl_up(sw, 1), rw = wv_shfl_down(sw, 1);
-                if (lane == 0) lw = 0;
-                if (lane == 63) rw = 0;
+                const uint32_t lw = wv_from_left(sw), rw = wv_from_right(sw);
                 const uint32_t bw = win6(B, r0), negw = win6(NEG, r0);
                 const uint32_t neglw = win6(negl, r0), negrw = win6(negr, r0);
                 SymList sl;
@@ -355,8 +359,7 @@ RG_DEV void t1_block(const T1Job &job)
                     bool again = true;
                     // monotone fix-point: converges within 65 rounds (chain across <= 64 lanes)
                     for (int round = 0; again && round < 80; ++round) {
-                        uint32_t NL = wv_shfl_up(N, 1);
-                        if (lane == 0) NL = 0;
+                        const uint32_t NL = wv_from_left(N);
                         Lw = lw | (NL << 1);
                         uint32_t cur = sw, Nn = 0;
                         P = 0;
@@ -411,8 +414,7 @@ RG_DEV void t1_block(const T1Job &job)
                     const uint32_t vw = (uint32_t)(V >> r0) & 0xFu;
                     const uint32_t Cm = ~(sw >> 1) & ~vw & vrow & 0xFu;
                     const uint32_t N = Cm & (bw >> 1);
-                    uint32_t NL = wv_shfl_up(N, 1);
-                    if (lane == 0) NL = 0;
+                    const uint32_t NL = wv_from_left(N);
                     const uint32_t Lw = lw | (NL << 1);
                     const bool full = r0 + 4 <= h;
                     const bool rl = full && Cm == 0xFu && (Lw & 0x3Fu) == 0 && (rw & 0x3Fu) == 0 && (sw & 0x21u) == 0;

@@ -29,6 +29,22 @@ RG_DEV int wv_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 template <class T> RG_DEV T wv_shfl_up(T v, int d) { return __shfl_up(v, d, 64); }
 template <class T> RG_DEV T wv_shfl_down(T v, int d) { return __shfl_down(v, d, 64); }
 template <class T> RG_DEV T wv_shfl_xor(T v, int m) { return __shfl_xor(v, m, 64); }
+// Neighbour exchange across the whole wavefront as a DPP row operation (no
+// LDS round trip): lane i receives lane i-1's value (lane 0 receives 0) /
+// lane i+1's value (lane 63 receives 0).
+RG_DEV uint32_t wv_from_left(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138 /* wave_shr:1 */, 0xF, 0xF, true);
+}
+RG_DEV uint32_t wv_from_right(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130 /* wave_shl:1 */, 0xF, 0xF, true);
+}
+// Number of set bits of a 64-bit lane mask below this lane.
+RG_DEV int wv_mbcnt(uint64_t m)
+{
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
 
 // 16-byte global-memory load / store (p 16-byte aligned).  The explicit global
 // address space gives global_load/store (vector-memory counter only) where a

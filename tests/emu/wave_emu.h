@@ -84,3 +84,18 @@ template <class T> static inline T wv_shfl_xor(T x, int m)
     const uint64_t *v = emu::exchange((uint64_t)x, 0);
     return (T)v[emu::lane() ^ m];
 }
+static inline uint32_t wv_from_left(uint32_t x)
+{
+    const uint64_t *v = emu::exchange(x, 0);
+    return emu::lane() == 0 ? 0u : (uint32_t)v[emu::lane() - 1];
+}
+static inline uint32_t wv_from_right(uint32_t x)
+{
+    const uint64_t *v = emu::exchange(x, 0);
+    return emu::lane() == 63 ? 0u : (uint32_t)v[emu::lane() + 1];
+}
+static inline int wv_mbcnt(uint64_t m)
+{
+    const int l = emu::lane();
+    return __builtin_popcountll(l == 0 ? 0ull : (m & (~0ull >> (64 - l))));
+}
