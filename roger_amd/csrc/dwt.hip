@@ -301,18 +301,42 @@ __device__ __forceinline__ void inv_tile(const rg_inv_level_args &a, int plane, 
     }
 }
 
+// XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
+// XCDs (each with its own L2), so in launch order neighbouring tiles land on
+// different XCDs and every halo column is fetched from HBM again.  The
+// bijection below gives XCD k (workgroups with linear id = k mod 8) the k-th
+// contiguous run of tiles in raster order (plane, row, column): horizontal
+// and vertical neighbours then share an L2.  Ids past the largest multiple
+// of 8 keep their own tile.
+constexpr int NXCD = 8;
+__device__ __forceinline__ void xcd_tile(int &plane, int &bx, int &by)
+{
+    const int tx = gridDim.x, ty = gridDim.y;
+    const int n = tx * ty * gridDim.z;
+    const int lin = (blockIdx.z * ty + blockIdx.y) * tx + blockIdx.x;
+    const int q = n / NXCD;
+    const int t = lin < q * NXCD ? (lin % NXCD) * q + lin / NXCD : lin;
+    plane = t / (tx * ty);
+    by = (t / tx) % ty;
+    bx = t % tx;
+}
+
 template <int F>
 __global__ void __launch_bounds__(256) fwd_level_kernel(rg_fwd_level_args a)
 {
     __shared__ __attribute__((aligned(16))) char lds[FWD_LDS_BYTES];
-    fwd_tile<F>(a, blockIdx.z, blockIdx.x, blockIdx.y, lds);
+    int plane, bx, by;
+    xcd_tile(plane, bx, by);
+    fwd_tile<F>(a, plane, bx, by, lds);
 }
 
 template <int F>
 __global__ void __launch_bounds__(256) inv_level_kernel(rg_inv_level_args a)
 {
     __shared__ __attribute__((aligned(16))) char lds[INV_LDS_BYTES];
-    inv_tile<F>(a, blockIdx.z, blockIdx.x, blockIdx.y, lds);
+    int plane, bx, by;
+    xcd_tile(plane, bx, by);
+    inv_tile<F>(a, plane, bx, by, lds);
 }
 
 // Batched: one workgroup per job {image args index, plane, tile x, tile y};

@@ -691,11 +691,16 @@ RG_DEV void t1_mq_pass_lane(uint32_t *tab, bool active, const uint8_t *sym, int 
     *len_out = (m.ovf || len > cap) ? -1 : len;
 }
 
-// Coding pass p of a block, as the lane function needs it: its symbol range
-// and its arena region.  Returns false for a pass the block does not have.
-RG_DEV bool pass_range(const i32x4 &c, const int32_t *poff, int p, int &s0, int &s1, int &at, int &cap)
+// Coding pass of a block for grid row q, counted from the block's LAST pass
+// (p = passes-1-q): row q then holds the same bit-plane, counted from the
+// least significant one, in every block, so the 64 chains of a wavefront have
+// similar lengths whatever each block's number of bit-planes.  Gives the pass
+// index, its symbol range and its arena region; false if the block has fewer
+// than q+1 passes.
+RG_DEV bool pass_range(const i32x4 &c, const int32_t *poff, int q, int &p, int &s0, int &s1, int &at, int &cap)
 {
-    if (p >= c.y) return false;
+    if (q >= c.y) return false;
+    p = c.y - 1 - q;
     s0 = poff[p];
     s1 = p + 1 < c.y ? poff[p + 1] : c.x;
     at = seg_at(s0, p);

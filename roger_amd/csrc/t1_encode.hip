@@ -127,10 +127,10 @@ __global__ void __launch_bounds__(64) t1_mqp_kernel(const uint8_t *__restrict__ 
                                                     int32_t *__restrict__ seglens)
 {
     __shared__ uint32_t tab[48];
-    const int b = blockIdx.x * 64 + threadIdx.x, p = blockIdx.y;
+    const int b = blockIdx.x * 64 + threadIdx.x;
     bool active = b < ncb;
-    int s0 = 0, s1 = 0, at = 0, cap = 0;
-    if (active) active = pass_range(cm[b], poffs + (size_t)b * MAXP, p, s0, s1, at, cap);
+    int p = 0, s0 = 0, s1 = 0, at = 0, cap = 0;
+    if (active) active = pass_range(cm[b], poffs + (size_t)b * MAXP, blockIdx.y, p, s0, s1, at, cap);
     const size_t bb = active ? (size_t)b : 0;
     t1_mq_pass_lane(tab, active, syms + bb * symcap, s0, s1, arena + bb * arena_stride + at, cap,
                     seglens + bb * MAXP + p);
@@ -139,11 +139,11 @@ __global__ void __launch_bounds__(64) t1_mqp_kernel(const uint8_t *__restrict__ 
 __global__ void __launch_bounds__(64) t1_mqp_jobs_kernel(const T1Job *__restrict__ jobs, int njobs)
 {
     __shared__ uint32_t tab[48];
-    const int b = blockIdx.x * 64 + threadIdx.x, p = blockIdx.y;
+    const int b = blockIdx.x * 64 + threadIdx.x;
     bool active = b < njobs;
-    int s0 = 0, s1 = 0, at = 0, cap = 0;
+    int p = 0, s0 = 0, s1 = 0, at = 0, cap = 0;
     const T1Job &j = jobs[active ? b : 0];
-    if (active) active = pass_range(*j.cm, j.poff, p, s0, s1, at, cap);
+    if (active) active = pass_range(*j.cm, j.poff, blockIdx.y, p, s0, s1, at, cap);
     t1_mq_pass_lane(tab, active, j.sym, s0, s1, j.seg + at, cap, j.seglen + p);
 }
 

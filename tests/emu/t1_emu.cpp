@@ -167,10 +167,10 @@ void mq_pass_body(void *a)
     const MQPass &m = *static_cast<MQPass *>(a);
     const int b = m.i0 + wv_lane();
     bool active = b < m.njobs;
-    int s0 = 0, s1 = 0, at = 0, cap = 0;
+    int p = 0, s0 = 0, s1 = 0, at = 0, cap = 0;
     const T1Job &j = m.jobs[active ? b : 0];
-    if (active) active = pass_range(*j.cm, j.poff, m.p, s0, s1, at, cap);
-    t1_mq_pass_lane(g_tab, active, j.sym, s0, s1, j.seg + at, cap, j.seglen + m.p);
+    if (active) active = pass_range(*j.cm, j.poff, m.p, p, s0, s1, at, cap);
+    t1_mq_pass_lane(g_tab, active, j.sym, s0, s1, j.seg + at, cap, j.seglen + p);
 }
 void pack_body(void *a)
 {
