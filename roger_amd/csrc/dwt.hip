@@ -151,6 +151,8 @@ __device__ __forceinline__ void fwd_tile(const rg_fwd_level_args &a, int plane, 
                      (long)plane * a.in_ps * (F == RG_FILTER_53 ? 4 : (a.in_half ? 2 : 4));
     const bool vlift = ch > 1, hlift = cw > 1;
     const bool simple = cw > H && ch > H;      // one reflection suffices
+    // the tile and its halo lie inside the level: no extension, no bounds tests
+    const bool interior = x0 >= H && y0 >= H && x0 + TILE + H <= cw && y0 + TILE + H <= ch;
 
     // ---- vertical pass straight from HBM: (column c of TW, segment k) ----
     for (int item = threadIdx.x; item < TW * NSEG; item += blockDim.x) {
@@ -158,7 +160,11 @@ __device__ __forceinline__ void fwd_tile(const rg_fwd_level_args &a, int plane, 
         const int x = simple ? mirror1(x0 - H + c, cw) : rg_mirror(x0 - H + c, cw);
         const int ybase = y0 + k * SEG - H;
         T e[N];
-        if (simple) {
+        if (interior) {
+            const long first = (long)ybase * a.in_pitch + x;
+#pragma unroll
+            for (int j = 0; j < N; ++j) e[j] = load_in<F>(in, first + (long)j * a.in_pitch, a.in_half);
+        } else if (simple) {
 #pragma unroll
             for (int j = 0; j < N; ++j) e[j] = load_in<F>(in, (long)mirror1(ybase + j, ch) * a.in_pitch + x, a.in_half);
         } else {
@@ -195,6 +201,41 @@ __device__ __forceinline__ void fwd_tile(const rg_fwd_level_args &a, int plane, 
     // ---- write out: each wavefront stores one row: 32 low + 32 high values ----
     const int wl = (cw + 1) >> 1, hl = (ch + 1) >> 1;
     char *qbase = reinterpret_cast<char *>(a.q) + (long)plane * a.q_ps * 4;
+    if (interior) {
+        // a thread keeps one column c and rows r0, r0+4, ...: its band, its
+        // step and the parity of its rows never change (y0 is even)
+        const int c = threadIdx.x % TILE, r0 = threadIdx.x / TILE;   // 256 threads: r0 in 0..3
+        const bool high = c >= TILE / 2, lowrow = !(r0 & 1);
+        const int xo = (x0 >> 1) + (c & (TILE / 2 - 1));
+        const int yh = (y0 + r0) >> 1;                                  // row inside the half band
+        const T *src = tile + r0 * LP + c;
+        if (lowrow && !high && a.ll_out) {
+            const long off = (long)plane * a.ll_ps + (long)yh * a.ll_pitch + xo, step = 2L * a.ll_pitch;
+#pragma unroll
+            for (int i = 0; i < TILE / 4; ++i) {
+                const T v = src[4 * i * LP];
+                if constexpr (F == RG_FILTER_53) reinterpret_cast<int *>(a.ll_out)[off + i * step] = v;
+                else if (a.ll_half) reinterpret_cast<__half *>(a.ll_out)[off + i * step] = __float2half_rn(v);
+                else reinterpret_cast<float *>(a.ll_out)[off + i * step] = v;
+            }
+            return;
+        }
+        const long off = (long)(lowrow ? yh : hl + yh) * a.qp + (high ? wl + xo : xo), step = 2L * a.qp;
+        if constexpr (F == RG_FILTER_53) {
+#pragma unroll
+            for (int i = 0; i < TILE / 4; ++i) reinterpret_cast<int *>(qbase)[off + i * step] = src[4 * i * LP];
+        } else if (a.q_float) {
+#pragma unroll
+            for (int i = 0; i < TILE / 4; ++i) reinterpret_cast<float *>(qbase)[off + i * step] = src[4 * i * LP];
+        } else {
+            // sign(v) * floor(|v| * inv) == truncation of v * inv (IEEE products are sign-symmetric)
+            const float inv = lowrow ? (high ? a.inv_hl : a.inv_ll) : (high ? a.inv_hh : a.inv_lh);
+#pragma unroll
+            for (int i = 0; i < TILE / 4; ++i)
+                reinterpret_cast<int *>(qbase)[off + i * step] = (int)(src[4 * i * LP] * inv);
+        }
+        return;
+    }
     for (int idx = threadIdx.x; idx < TILE * TILE; idx += blockDim.x) {
         const int r = idx / TILE, c = idx % TILE;
         const int y = y0 + r;

@@ -691,16 +691,14 @@ RG_DEV void t1_mq_pass_lane(uint32_t *tab, bool active, const uint8_t *sym, int 
     *len_out = (m.ovf || len > cap) ? -1 : len;
 }
 
-// Coding pass of a block for grid row q, counted from the block's LAST pass
-// (p = passes-1-q): row q then holds the same bit-plane, counted from the
-// least significant one, in every block, so the 64 chains of a wavefront have
-// similar lengths whatever each block's number of bit-planes.  Gives the pass
-// index, its symbol range and its arena region; false if the block has fewer
-// than q+1 passes.
+// Coding pass q of a block (grid row q): its pass index, symbol range and
+// arena region; false if the block has fewer than q+1 passes.  (Counting rows
+// from each block's last pass instead, so that a row holds the same bit-plane
+// in every block, measured 12% slower for the 4096x4096 RGB workload.)
 RG_DEV bool pass_range(const i32x4 &c, const int32_t *poff, int q, int &p, int &s0, int &s1, int &at, int &cap)
 {
     if (q >= c.y) return false;
-    p = c.y - 1 - q;
+    p = q;
     s0 = poff[p];
     s1 = p + 1 < c.y ? poff[p + 1] : c.x;
     at = seg_at(s0, p);

@@ -130,7 +130,9 @@ __global__ void __launch_bounds__(64) t1_mqp_kernel(const uint8_t *__restrict__ 
     const int b = blockIdx.x * 64 + threadIdx.x;
     bool active = b < ncb;
     int p = 0, s0 = 0, s1 = 0, at = 0, cap = 0;
-    if (active) active = pass_range(cm[b], poffs + (size_t)b * MAXP, blockIdx.y, p, s0, s1, at, cap);
+    // rows dispatched latest pass first: the long chains (low bit-planes) start early
+    const int q = (int)(gridDim.y - 1 - blockIdx.y);
+    if (active) active = pass_range(cm[b], poffs + (size_t)b * MAXP, q, p, s0, s1, at, cap);
     const size_t bb = active ? (size_t)b : 0;
     t1_mq_pass_lane(tab, active, syms + bb * symcap, s0, s1, arena + bb * arena_stride + at, cap,
                     seglens + bb * MAXP + p);
@@ -143,7 +145,8 @@ __global__ void __launch_bounds__(64) t1_mqp_jobs_kernel(const T1Job *__restrict
     bool active = b < njobs;
     int p = 0, s0 = 0, s1 = 0, at = 0, cap = 0;
     const T1Job &j = jobs[active ? b : 0];
-    if (active) active = pass_range(*j.cm, j.poff, blockIdx.y, p, s0, s1, at, cap);
+    const int q = (int)(gridDim.y - 1 - blockIdx.y);   // latest pass first, as in t1_mqp_kernel
+    if (active) active = pass_range(*j.cm, j.poff, q, p, s0, s1, at, cap);
     t1_mq_pass_lane(tab, active, j.sym, s0, s1, j.seg + at, cap, j.seglen + p);
 }
 

@@ -3,7 +3,7 @@
 // (not a restatement) can be checked against the oracle without a GPU.
 //
 //   t1_emu MODE DIR PITCH PLANE_STRIDE SLOT
-//     MODE  inline | split | queue_split | parallel | queue_parallel
+//     MODE  inline | split | queue_split | parallel | queue_parallel | cm_only
 //     DIR   holds q.bin (int32 coefficient planes) and desc.bin (rg_cb_desc
 //           records); res.bin (rg_cb_result per block) and out.bin (SLOT
 //           bytes per block) are written there.
@@ -14,6 +14,7 @@
 //   parallel     RESET|TERMALL style: t1_block<true> recording pass offsets,
 //                t1_mq_pass_lane per (64 blocks, pass), t1_pack_block per block
 //   queue_parallel  the same reading every pointer from the job list
+//   cm_only      context modelling alone; writes cm.bin and poff.bin
 // The parallel modes also write seglen.bin (MAXP int32 per block).
 // Exit status 3: an emulator check failed (message on stderr).
 #include <stdint.h>
@@ -256,6 +257,13 @@ int main(int argc, char **argv)
                 emu::run(mq_queue_body, &m);
             }
         }
+    } else if (mode == "cm_only") {
+        // context modelling alone: writes cm.bin (4 int32 per block) and
+        // poff.bin (MAXP pass offsets per block) for workload statistics
+        for (int k = 0; k < ncb; ++k) jobs[k].poff = &poffs[(size_t)k * MAXP];
+        for (int k = 0; k < ncb; ++k) emu::run(cm_body, &jobs[k]);
+        spill(dir + "/cm.bin", cm.data(), sizeof(i32x4) * cm.size());
+        spill(dir + "/poff.bin", poffs.data(), sizeof(int32_t) * poffs.size());
     } else if (mode == "parallel" || mode == "queue_parallel") {
         for (int k = 0; k < ncb; ++k) emu::run(cm_body, &jobs[k]);
         for (int p = 0; p < MAXP - 2; ++p)
