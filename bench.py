@@ -4,8 +4,12 @@ Workload (the headline configuration): one 4096x4096 interleaved RGB uint8
 image (synthetic, seeded), resident in HBM; one step = the whole encode on
 the device: RGB -> planar + ICT, irreversible 9/7 FP32 DWT with 6 levels and
 scalar quantisation at step 1/64, EBCOT tier-1 on 64x64 code-blocks in the
-parallel code-block style (RESET|TERMALL: every coding pass its own MQ
-codeword segment; --style standard for one segment per block).
+code-block style chosen by --style:
+  bypass    RESET|TERMALL|BYPASS (default): every coding pass its own
+            codeword segment, significance/refinement passes from the fifth
+            bit-plane sent raw;
+  parallel  RESET|TERMALL: every coding pass its own MQ codeword segment;
+  standard  one MQ codeword per code-block.
 value = image Mpix/s over the timed steps.
 
 Also reported:
@@ -14,7 +18,8 @@ Also reported:
                   int32 indices out) over its HIP-event-timed launch, vs 8 TB/s.
   stages_ms     : per-stage times (colour, DWT all levels, tier-1 in both
                   styles).
-  standard_style: the same encode with single-segment code-blocks.
+  styles        : the same encode in each code-block style (rate and
+                  codestream bytes).
   decode        : the reverse path on the device-resident codewords (tier-1
                   decode, dequantisation + inverse 9/7, inverse ICT).
   cpu_baseline  : the CPU restatement (oracle) of the same encode on the host
@@ -126,7 +131,7 @@ def main():
     ap.add_argument("--batch-images", type=int, default=256)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--serve-requests", type=int, default=64)
-    ap.add_argument("--style", choices=["parallel", "standard"], default="parallel",
+    ap.add_argument("--style", choices=["bypass", "parallel", "standard"], default="bypass",
                     help="code-block style of the headline encode")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -136,11 +141,13 @@ def main():
 
     from roger_amd import runtime as R
     from roger_amd import synth
-    from roger_amd._lib import RG_COLOUR_ICT, RG_FILTER_97, RG_STYLE_PARALLEL, lib
+    from roger_amd._lib import RG_COLOUR_ICT, RG_FILTER_97, RG_STYLE_PARALLEL, RG_STYLE_PARALLEL_BYPASS, lib
 
     W = H = args.size
     NC, LEVELS, DELTA, CB = 3, 6, 1.0 / 64, 64
-    STYLE = RG_STYLE_PARALLEL if args.style == "parallel" else 0
+    STYLES = {"bypass": RG_STYLE_PARALLEL_BYPASS, "parallel": RG_STYLE_PARALLEL, "standard": 0}
+    STYLE_NAMES = {"bypass": "RESET|TERMALL|BYPASS", "parallel": "RESET|TERMALL", "standard": "standard"}
+    STYLE = STYLES[args.style]
     print("generating %dx%d RGB input" % (W, H), file=sys.stderr, flush=True)
     img = synth.rgb(W, H, 1)
     ctx = R.Context(0)
@@ -159,9 +166,14 @@ def main():
         ctx.sync()
         return time.perf_counter() - t0
 
-    other = 0 if STYLE else RG_STYLE_PARALLEL
-    dt_other = encode_rate(other, max(3, args.steps // 4), 1)
-    _, res_other, _ = enc.codeblocks()
+    styles = {}
+    for name, st in STYLES.items():
+        if st == STYLE:
+            continue
+        n_other = max(3, args.steps // 4)
+        dt_other = encode_rate(st, n_other, 1)
+        styles[name] = {"style_bits": st, "mpix_s": round(W * H / 1e6 * n_other / dt_other, 2),
+                        "codestream_bytes": int(enc.codeblocks()[1]["len"].sum())}
     dt = encode_rate(STYLE, args.steps, args.warmup)
     mpix = W * H / 1e6
     ms_step = dt * 1e3 / args.steps
@@ -197,11 +209,12 @@ def main():
     t_t1 = timed(lambda: lib().rg_t1_encode(ctx.handle, ctypes.c_void_p(d_q.ptr), W, W * H,
                                               ctypes.c_void_p(d_desc.ptr), len(desc), ctypes.c_void_p(d_out.ptr),
                                               slot, ctypes.c_void_p(d_res.ptr)))
-    # the encoder's tier-1 in the parallel style (context modelling, pass-parallel MQ, segment packing)
-    enc.encode_u8(None, levels=LEVELS, reversible=False, cbw=CB, cbh=CB, delta=DELTA, d_src=d_src, w=W, h=H,
-                  ncomp=NC, style=RG_STYLE_PARALLEL)
-    t_enc_par = timed(lambda: enc.encode_u8(None, levels=LEVELS, reversible=False, cbw=CB, cbh=CB, delta=DELTA,
-                                            d_src=d_src, w=W, h=H, ncomp=NC, style=RG_STYLE_PARALLEL))
+    # whole encodes in the two pass-parallel styles (context modelling, pass-parallel MQ / raw, packing)
+    def enc_style(st):
+        return lambda: enc.encode_u8(None, levels=LEVELS, reversible=False, cbw=CB, cbh=CB, delta=DELTA,
+                                     d_src=d_src, w=W, h=H, ncomp=NC, style=st)
+    t_enc_par = timed(enc_style(RG_STYLE_PARALLEL))
+    t_enc_byp = timed(enc_style(RG_STYLE_PARALLEL_BYPASS))
     # reverse path on device-resident standard-style codewords of the same image
     if STYLE:
         enc.encode_u8(None, levels=LEVELS, reversible=False, cbw=CB, cbh=CB, delta=DELTA, d_src=d_src, w=W, h=H,
@@ -243,7 +256,7 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "%dx%d RGB uint8 -> ICT -> 9/7 FP32 %d levels, step 1/64 -> EBCOT %dx%d, %s "
                                "code-block style (plain per-image path, input resident in HBM)"
-                               % (W, H, LEVELS, CB, CB, "RESET|TERMALL" if STYLE else "standard"),
+                               % (W, H, LEVELS, CB, CB, STYLE_NAMES[args.style]),
                    "codeblock_style": STYLE,
                    "image": [W, H, NC], "levels": LEVELS, "delta": DELTA, "codeblock": [CB, CB],
                    "codeblocks": int(ncb), "codestream_bytes": total_bytes},
@@ -256,9 +269,9 @@ def main():
                       "dwt97_quant_6lv": round(t_dwt, 4), "dwt97_level1": round(t_dwt_l1, 4),
                       "t1_encode_standard_style": round(t_t1, 4),
                       "t1_standard_codeblocks_per_s": round(len(desc) / (t_t1 * 1e-3), 1),
-                      "encode_parallel_style": round(t_enc_par, 4)},
-        "standard_style": {"mpix_s": round(mpix * max(3, args.steps // 4) / dt_other, 2) if STYLE else None,
-                           "codestream_bytes": int((res_other if STYLE else res)["len"].sum())},
+                      "encode_parallel_style": round(t_enc_par, 4),
+                      "encode_bypass_style": round(t_enc_byp, 4)},
+        "styles": styles,
         "decode": {"what": "device-resident standard-style codewords -> tier-1 decode -> dequantisation + "
                            "inverse 9/7 -> inverse ICT -> uint8 RGB",
                    "ms": round(t_dec, 4), "mpix_s": round(mpix / (t_dec * 1e-3), 1),

@@ -70,17 +70,24 @@ typedef struct {
     double delta;          /* 9/7 base quantisation step (e.g. 1/64) */
     int32_t ll_fp16;       /* 9/7: store the inter-level LL band as FP16 */
     int32_t bitdepth;      /* sample precision (8) */
-    int32_t style;         /* code-block style: 0 or RG_STYLE_PARALLEL */
+    int32_t style;         /* code-block style: 0, RG_STYLE_PARALLEL or RG_STYLE_PARALLEL_BYPASS */
 } rg_encode_params;
 
 /* Code-block styles (T.800 Table A.19).  RG_STYLE_PARALLEL = RESET | TERMALL:
  * every coding pass is its own MQ codeword segment starting from the initial
  * contexts, so the passes of a block code in parallel (one GPU lane each);
  * the codestream signals the style in COD and each segment's length in the
- * packet headers.  About 1% larger codestreams than style 0. */
+ * packet headers.  About 1% larger codestreams than style 0.
+ * RG_STYLE_PARALLEL_BYPASS adds BYPASS: the significance-propagation and
+ * refinement passes from the fifth bit-plane on are sent as raw bits (T.800
+ * D.6) instead of MQ-coded -- less coder work, codestreams within about 1% of
+ * RG_STYLE_PARALLEL (smaller on most images, since reset contexts code
+ * near-random low bit-planes poorly). */
+#define RG_STYLE_BYPASS 0x01
 #define RG_STYLE_RESET 0x02
 #define RG_STYLE_TERMALL 0x04
 #define RG_STYLE_PARALLEL 0x06
+#define RG_STYLE_PARALLEL_BYPASS 0x07
 #define RG_MAX_PASSES 96   /* stride of per-block pass arrays (>= 3*32-2) */
 
 /* ---- errors ---- */
@@ -277,7 +284,7 @@ int rg_batch_destroy(rg_batch *b);
 /* Lays out all buffers and job tables (tier-1 jobs sorted largest block
  * first); with use_graph the pipeline (colour, every DWT level, tier-1) is
  * captured once into a HIP graph.  style: code-block style of every image
- * (0 or RG_STYLE_PARALLEL).  waves_per_cu > 0: tier-1 context modelling runs
+ * (0, RG_STYLE_PARALLEL or RG_STYLE_PARALLEL_BYPASS).  waves_per_cu > 0: tier-1 context modelling runs
  * as a persistent grid of that many wavefronts per CU pulling code-blocks
  * from a device work queue; 0: one workgroup per code-block. */
 int rg_batch_plan(rg_batch *b, const rg_batch_image *imgs, int n, int cbw, int cbh, int use_graph, int style,

@@ -94,6 +94,7 @@ typedef struct {
 } orc_cb_result;
 
 /* Code-block style bits (T.800 Table A.19) the oracle implements. */
+#define ORC_STYLE_BYPASS  0x01  /* raw SPP/MRP passes from the fifth bit-plane (with TERMALL) */
 #define ORC_STYLE_RESET   0x02  /* contexts reset after every coding pass */
 #define ORC_STYLE_TERMALL 0x04  /* codeword terminated after every coding pass */
 #define ORC_MAX_PASSES    96    /* >= 3*32-2: stride of per-pass length arrays */

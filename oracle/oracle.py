@@ -274,9 +274,11 @@ def sc_context(H, V):
     return c, x.value
 
 
-STYLE_RESET = 0x02     # T.800 Table A.19: contexts reset after every pass
+STYLE_BYPASS = 0x01    # T.800 Table A.19: raw SPP/MRP passes from the 5th bit-plane
+STYLE_RESET = 0x02     # contexts reset after every pass
 STYLE_TERMALL = 0x04   # codeword terminated after every pass
 STYLE_PARALLEL = STYLE_RESET | STYLE_TERMALL
+STYLE_PARALLEL_BYPASS = STYLE_PARALLEL | STYLE_BYPASS
 MAX_PASSES = 96        # stride of the oracle's per-pass length arrays
 
 

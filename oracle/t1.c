@@ -209,6 +209,85 @@ static int mqd_decode(mq_dec *m, int cx)
     return d;
 }
 
+/* ---------------- raw (bypass) coding (T.800 D.6) ----------------
+ * With the BYPASS code-block style the significance-propagation and
+ * magnitude-refinement passes from the fifth bit-plane on (pass index >= 10,
+ * counting the first cleanup pass as 0) carry their decisions as plain bits:
+ * MSB first, and after an 0xFF byte the next byte holds only 7 (its MSB is a
+ * stuffed 0).  A sign is sent as the sign itself (1 = negative), with no
+ * context and no XOR prediction.  Termination pads an unfinished byte -- also
+ * the 7-bit byte owed after a final 0xFF -- with the alternating pattern
+ * 0,1,0,... so a segment never ends in 0xFF. */
+typedef struct {
+    uint8_t *buf;
+    long bp, cap;     /* next byte index */
+    uint32_t c;       /* bits of the byte being filled */
+    int ct;           /* free bits left in it (8, or 7 after 0xFF) */
+    int overflow;
+} raw_enc;
+
+static void rawe_init(raw_enc *r, uint8_t *buf, long cap)
+{
+    r->buf = buf; r->bp = 0; r->cap = cap; r->c = 0; r->ct = 8; r->overflow = 0;
+}
+
+static void rawe_emit(raw_enc *r)
+{
+    if (r->bp < r->cap) r->buf[r->bp] = (uint8_t)r->c;
+    else r->overflow = 1;
+    r->bp++;
+    r->ct = r->c == 0xFF ? 7 : 8;
+    r->c = 0;
+}
+
+static inline void rawe_put(raw_enc *r, int bit)
+{
+    r->c = (r->c << 1) | (uint32_t)(bit & 1);
+    if (--r->ct == 0) rawe_emit(r);
+}
+
+/* Returns the segment length in bytes. */
+static long rawe_flush(raw_enc *r)
+{
+    if (r->ct < 8) {
+        const int n = r->ct;   /* the last put of the pattern completes the byte */
+        for (int k = 0; k < n; ++k) rawe_put(r, k & 1);
+    }
+    return r->bp;
+}
+
+typedef struct {
+    const uint8_t *d;
+    long len, bp;
+    uint32_t c;       /* current byte */
+    int ct;           /* bits of it not yet read */
+} raw_dec;
+
+static void rawd_start(raw_dec *r, const uint8_t *d, long len)
+{
+    r->d = d; r->len = len; r->bp = 0; r->c = 0; r->ct = 0;
+}
+
+static inline int rawd_get(raw_dec *r)
+{
+    if (r->ct == 0) {
+        const int after_ff = r->c == 0xFF;
+        r->c = r->bp < r->len ? r->d[r->bp] : 0xFF;
+        r->bp++;
+        r->ct = after_ff ? 7 : 8;
+    }
+    r->ct--;
+    return (int)((r->c >> r->ct) & 1u);
+}
+
+/* Pass index k (0 = the first cleanup pass) is raw-coded under BYPASS.  Only
+ * BYPASS together with TERMALL is implemented (every pass its own segment);
+ * BYPASS alone groups passes into segments differently and is not offered. */
+static inline int pass_is_raw(int style, int k)
+{
+    return (style & ORC_STYLE_BYPASS) && (style & ORC_STYLE_TERMALL) && k >= 10 && k % 3 != 0;
+}
+
 /* ---------------- contexts (Tables D.1, D.3, D.4) ---------------- */
 int orc_zc_context(int orient, int h, int v, int d)
 {
@@ -281,7 +360,8 @@ static inline int any_sig_nbr(const cbstate *s, int x, int y)
 }
 
 /* ---------------- encoder passes ---------------- */
-static void enc_spp(cbstate *s, mq_enc *e, int p)
+/* raw != NULL: the pass is raw-coded (bypass), contexts are not used */
+static void enc_spp(cbstate *s, mq_enc *e, raw_enc *raw, int p)
 {
     for (int y0 = 0; y0 < s->h; y0 += 4)
         for (int x = 0; x < s->w; ++x)
@@ -291,25 +371,36 @@ static void enc_spp(cbstate *s, mq_enc *e, int p)
                 int cx = zc_of(s, x, y);
                 if (cx == 0) continue;
                 int bit = (s->m[y * s->w + x] >> p) & 1;
-                mqe_encode(e, cx, bit);
+                if (raw) rawe_put(raw, bit);
+                else mqe_encode(e, cx, bit);
                 *f |= F_VIS;
                 if (bit) {
-                    int xr, sc = sc_of(s, x, y, &xr);
-                    mqe_encode(e, sc, ((*f & F_NEG) ? 1 : 0) ^ xr);
+                    const int neg = (*f & F_NEG) ? 1 : 0;
+                    if (raw) {
+                        rawe_put(raw, neg);
+                    } else {
+                        int xr, sc = sc_of(s, x, y, &xr);
+                        mqe_encode(e, sc, neg ^ xr);
+                    }
                     *f |= F_SIG;
                 }
             }
 }
 
-static void enc_mrp(cbstate *s, mq_enc *e, int p)
+static void enc_mrp(cbstate *s, mq_enc *e, raw_enc *raw, int p)
 {
     for (int y0 = 0; y0 < s->h; y0 += 4)
         for (int x = 0; x < s->w; ++x)
             for (int y = y0; y < y0 + 4 && y < s->h; ++y) {
                 uint8_t *f = &FL(s, x, y);
                 if ((*f & (F_SIG | F_VIS)) != F_SIG) continue;
-                int cx = (*f & F_REF) ? 16 : (any_sig_nbr(s, x, y) ? 15 : 14);
-                mqe_encode(e, cx, (s->m[y * s->w + x] >> p) & 1);
+                const int bit = (s->m[y * s->w + x] >> p) & 1;
+                if (raw) {
+                    rawe_put(raw, bit);
+                } else {
+                    int cx = (*f & F_REF) ? 16 : (any_sig_nbr(s, x, y) ? 15 : 14);
+                    mqe_encode(e, cx, bit);
+                }
                 *f |= F_REF;
             }
 }
@@ -356,7 +447,8 @@ static void enc_cup(cbstate *s, mq_enc *e, int p)
  * state after each coding pass; TERMALL terminates the codeword (C.2.9) after
  * each pass and restarts the coder (C.2.8) for the next one, so every pass is
  * its own codeword segment.  Together they make the passes of a block
- * independent MQ chains. */
+ * independent MQ chains.  BYPASS (with TERMALL) sends the significance and
+ * refinement passes from the fifth bit-plane on as raw bits (D.6). */
 void orc_t1_encode_style(const int32_t *q, int w, int h, int stride, int orient, int style,
                          uint8_t *out, int cap, orc_cb_result *res, int32_t *passlen)
 {
@@ -391,17 +483,29 @@ void orc_t1_encode_style(const int32_t *q, int w, int h, int stride, int orient,
     mqe_init(&e, buf, icap);
     for (int p = nbp - 1; p >= 0; --p)
         for (int t = (p == nbp - 1) ? 2 : 0; t < 3; ++t) {
-            if (t == 0) enc_spp(&s, &e, p);
-            else if (t == 1) enc_mrp(&s, &e, p);
-            else enc_cup(&s, &e, p);
             const int last = (p == 0 && t == 2);
-            if ((style & ORC_STYLE_TERMALL) || last) {
-                long len = mqe_flush(&e);
-                overflow |= e.overflow;
-                memcpy(code + total, buf + 1, (size_t)len);
+            if (pass_is_raw(style, pass)) {
+                /* a raw pass is its own segment (BYPASS is used with TERMALL) */
+                raw_enc r;
+                rawe_init(&r, code + total, icap - total);
+                if (t == 0) enc_spp(&s, &e, &r, p);
+                else enc_mrp(&s, &e, &r, p);
+                const long len = rawe_flush(&r);
+                overflow |= r.overflow;
                 total += len;
                 if (passlen) passlen[pass] = (int32_t)len;
-                if (!last) mqe_restart(&e, buf, icap);
+            } else {
+                if (t == 0) enc_spp(&s, &e, NULL, p);
+                else if (t == 1) enc_mrp(&s, &e, NULL, p);
+                else enc_cup(&s, &e, p);
+                if ((style & ORC_STYLE_TERMALL) || last) {
+                    long len = mqe_flush(&e);
+                    overflow |= e.overflow;
+                    memcpy(code + total, buf + 1, (size_t)len);
+                    total += len;
+                    if (passlen) passlen[pass] = (int32_t)len;
+                    if (!last) mqe_restart(&e, buf, icap);
+                }
             }
             if (style & ORC_STYLE_RESET) mq_reset_ctx(e.I, e.MPS);
             ++pass;
@@ -420,7 +524,8 @@ void orc_t1_encode(const int32_t *q, int w, int h, int stride, int orient,
 }
 
 /* ---------------- decoder passes ---------------- */
-static void dec_spp(cbstate *s, mq_dec *m, int p)
+/* raw != NULL: the pass is raw-coded (bypass) */
+static void dec_spp(cbstate *s, mq_dec *m, raw_dec *raw, int p)
 {
     for (int y0 = 0; y0 < s->h; y0 += 4)
         for (int x = 0; x < s->w; ++x)
@@ -429,18 +534,23 @@ static void dec_spp(cbstate *s, mq_dec *m, int p)
                 if (*f & F_SIG) continue;
                 int cx = zc_of(s, x, y);
                 if (cx == 0) continue;
-                int bit = mqd_decode(m, cx);
+                int bit = raw ? rawd_get(raw) : mqd_decode(m, cx);
                 *f |= F_VIS;
                 if (bit) {
-                    int xr, sc = sc_of(s, x, y, &xr);
-                    int neg = mqd_decode(m, sc) ^ xr;
+                    int neg;
+                    if (raw) {
+                        neg = rawd_get(raw);
+                    } else {
+                        int xr, sc = sc_of(s, x, y, &xr);
+                        neg = mqd_decode(m, sc) ^ xr;
+                    }
                     *f |= F_SIG | (neg ? F_NEG : 0);
                     s->m[y * s->w + x] |= 1u << p;
                 }
             }
 }
 
-static void dec_mrp(cbstate *s, mq_dec *m, int p)
+static void dec_mrp(cbstate *s, mq_dec *m, raw_dec *raw, int p)
 {
     for (int y0 = 0; y0 < s->h; y0 += 4)
         for (int x = 0; x < s->w; ++x)
@@ -448,7 +558,7 @@ static void dec_mrp(cbstate *s, mq_dec *m, int p)
                 uint8_t *f = &FL(s, x, y);
                 if ((*f & (F_SIG | F_VIS)) != F_SIG) continue;
                 int cx = (*f & F_REF) ? 16 : (any_sig_nbr(s, x, y) ? 15 : 14);
-                if (mqd_decode(m, cx)) s->m[y * s->w + x] |= 1u << p;
+                if (raw ? rawd_get(raw) : mqd_decode(m, cx)) s->m[y * s->w + x] |= 1u << p;
                 *f |= F_REF;
             }
 }
@@ -498,20 +608,23 @@ int orc_t1_decode_style(const uint8_t *data, int len, int w, int h, int orient, 
     if (nbp > 0 && npasses > 0) {
         const int termall = (style & ORC_STYLE_TERMALL) && passlen;
         mq_dec m;
+        raw_dec r;
         long off = 0;
         mqd_init(&m, data, termall ? passlen[0] : len);
         int pass = 0;
         for (int p = nbp - 1; p >= 0 && pass < npasses; --p)
             for (int t = (p == nbp - 1) ? 2 : 0; t < 3 && pass < npasses; ++t) {
+                const int raw = termall && pass_is_raw(style, pass);
                 if (pass > 0) {
                     if (style & ORC_STYLE_RESET) mq_reset_ctx(m.I, m.MPS);
                     if (termall) {
                         off += passlen[pass - 1];
-                        mqd_start(&m, data + off, passlen[pass]);
+                        if (raw) rawd_start(&r, data + off, passlen[pass]);
+                        else mqd_start(&m, data + off, passlen[pass]);
                     }
                 }
-                if (t == 0) dec_spp(&s, &m, p);
-                else if (t == 1) dec_mrp(&s, &m, p);
+                if (t == 0) dec_spp(&s, &m, raw ? &r : NULL, p);
+                else if (t == 1) dec_mrp(&s, &m, raw ? &r : NULL, p);
                 else dec_cup(&s, &m, p);
                 ++pass;
             }

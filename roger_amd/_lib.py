@@ -18,7 +18,8 @@ RG_ERR_OVERFLOW = -4
 RG_ERR_NOT_READY = -5
 
 RG_F32, RG_I16, RG_I32 = 0, 1, 2
-RG_STYLE_RESET, RG_STYLE_TERMALL, RG_STYLE_PARALLEL = 0x02, 0x04, 0x06
+RG_STYLE_BYPASS, RG_STYLE_RESET, RG_STYLE_TERMALL = 0x01, 0x02, 0x04
+RG_STYLE_PARALLEL, RG_STYLE_PARALLEL_BYPASS = 0x06, 0x07
 RG_MAX_PASSES = 96
 RG_COLOUR_NONE, RG_COLOUR_RCT, RG_COLOUR_ICT = 0, 1, 2
 RG_FILTER_53, RG_FILTER_97 = 0, 1

@@ -112,7 +112,7 @@ static int issue(rg_batch *b)
                                             b->nfjobs[l])))
             return rc;
     if (b->nwaves > 0) return rg_launch_t1_queue(s, b->t1jobs.p, b->ncb, b->style, (int *)b->counter.p, b->nwaves);
-    if (b->style == RG_STYLE_PARALLEL) return rg_launch_t1_jobs_parallel(s, b->t1jobs.p, b->ncb);
+    if (b->style & RG_STYLE_TERMALL) return rg_launch_t1_jobs_parallel(s, b->t1jobs.p, b->ncb, b->style);
     if (rg_t1_inline()) return rg_launch_t1_jobs(s, b->t1jobs.p, b->ncb);
     return rg_launch_t1_jobs_split(s, b->t1jobs.p, b->ncb);
 }
@@ -122,7 +122,8 @@ extern "C" int rg_batch_plan(rg_batch *b, const rg_batch_image *imgs, int n, int
 {
     RG_CHECK(b && imgs && n > 0, "rg_batch_plan: bad arguments");
     RG_CHECK(waves_per_cu >= 0 && waves_per_cu <= 32, "waves_per_cu %d out of range 0..32", waves_per_cu);
-    RG_CHECK(style == 0 || style == RG_STYLE_PARALLEL, "code-block style %d: only 0 and RG_STYLE_PARALLEL", style);
+    RG_CHECK(style == 0 || style == RG_STYLE_PARALLEL || style == RG_STYLE_PARALLEL_BYPASS,
+             "code-block style %d: only 0, RG_STYLE_PARALLEL and RG_STYLE_PARALLEL_BYPASS", style);
     RG_HIP(hipSetDevice(b->ctx->device));
     RG_HIP(hipStreamSynchronize(b->ctx->stream));
     drop_graph(b);
@@ -259,7 +260,7 @@ extern "C" int rg_batch_plan(rg_batch *b, const rg_batch_image *imgs, int n, int
                        b->syms.p ? (uint8_t *)b->syms.p + (size_t)k * symcap : nullptr, symcap, (int4 *)b->cm.p + k,
                        style ? (int32_t *)b->poffs.p + (size_t)k * maxp : nullptr,
                        style ? (uint8_t *)b->arena.p + (size_t)k * astride : nullptr,
-                       style ? (int32_t *)b->seglens.p + (size_t)idx * maxp : nullptr);
+                       style ? (int32_t *)b->seglens.p + (size_t)idx * maxp : nullptr, style);
     }
     if ((rc = b->t1jobs.ensure(jobs.size()))) return rc;
     RG_HIP(hipMemcpy(b->t1jobs.p, jobs.data(), jobs.size(), hipMemcpyHostToDevice));

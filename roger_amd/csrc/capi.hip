@@ -492,7 +492,7 @@ static int run_t1(rg_encoder *e)
     e->slot_bytes = std::max(256, e->cbw * e->cbh * 3);
     if ((rc = e->res.ensure(sizeof(rg_cb_result) * e->ncb))) return rc;
     if ((rc = e->slots.ensure((size_t)e->slot_bytes * e->ncb))) return rc;
-    if (e->style == RG_STYLE_PARALLEL) {
+    if (e->style & RG_STYLE_TERMALL) {
         const int symcap = rg_t1_symcap(e->cbw, e->cbh), maxp = rg_t1_maxp();
         if ((rc = e->syms.ensure((size_t)symcap * e->ncb)) || (rc = e->cm.ensure(sizeof(int4) * e->ncb)) ||
             (rc = e->poffs.ensure(sizeof(int32_t) * maxp * e->ncb)) ||
@@ -503,7 +503,7 @@ static int run_t1(rg_encoder *e)
                                    (const rg_cb_desc *)e->desc.p, e->ncb, (uint8_t *)e->syms.p, symcap,
                                    (int4 *)e->cm.p, (int32_t *)e->poffs.p, (uint8_t *)e->arena.p,
                                    (int32_t *)e->seglens.p, (uint8_t *)e->slots.p, e->slot_bytes,
-                                   (rg_cb_result *)e->res.p);
+                                   (rg_cb_result *)e->res.p, e->style);
         if (rc == RG_SUCCESS) e->t1_ran = 1;
         return rc;
     }
@@ -534,8 +534,8 @@ static int check_params(const rg_encode_params *p, int w, int h, int ncomp)
     RG_CHECK(p->colour != RG_COLOUR_ICT || p->filter == RG_FILTER_97, "ICT pairs with the 9/7 filter");
     RG_CHECK(p->filter == RG_FILTER_53 || (p->delta > 0 && std::isfinite(p->delta)), "bad quantisation step");
     RG_CHECK(p->bitdepth >= 1 && p->bitdepth <= 16, "bad bitdepth %d", p->bitdepth);
-    RG_CHECK(p->style == 0 || p->style == RG_STYLE_PARALLEL, "code-block style %d: only 0 and RG_STYLE_PARALLEL",
-             p->style);
+    RG_CHECK(p->style == 0 || p->style == RG_STYLE_PARALLEL || p->style == RG_STYLE_PARALLEL_BYPASS,
+             "code-block style %d: only 0, RG_STYLE_PARALLEL and RG_STYLE_PARALLEL_BYPASS", p->style);
     if (rg_codeblocks(w, h, p->levels, p->cbw, p->cbh, nullptr, 0) < 0) return RG_ERR_INVALID;
     return RG_SUCCESS;
 }

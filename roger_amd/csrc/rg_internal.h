@@ -109,7 +109,7 @@ int rg_launch_t1_encode(hipStream_t s, const int32_t *q, int pitch, size_t plane
 int rg_t1_job_bytes();
 void rg_t1_job_fill(void *dst, const int32_t *src, int pitch, int w, int h, int orient, uint8_t *out, int cap,
                     rg_cb_result *res, uint8_t *sym, int symcap, int4 *cm, int32_t *poff, uint8_t *seg,
-                    int32_t *seglen);
+                    int32_t *seglen, int style);
 int rg_launch_t1_jobs(hipStream_t s, const void *jobs, int njobs);
 // Split tier-1: context modelling writes per-block symbol streams (symcap
 // bytes each), then one lane per block runs the MQ coder.
@@ -118,16 +118,17 @@ int rg_t1_symcap(int cbw, int cbh);
 int rg_launch_t1_split(hipStream_t s, const int32_t *q, int pitch, size_t plane_stride, const rg_cb_desc *desc, int ncb,
                        uint8_t *syms, int symcap, int4 *cm, uint8_t *out, int slot_bytes, rg_cb_result *res);
 int rg_launch_t1_jobs_split(hipStream_t s, const void *jobs, int njobs);
-// Parallel code-block style (RESET | TERMALL): context modelling, one MQ lane
-// per (block, pass), then concatenation of the pass segments into the slots.
+// Parallel code-block styles (RESET | TERMALL, optionally | BYPASS): context
+// modelling, one MQ (or raw) lane per (block, pass), then concatenation of
+// the pass segments into the slots.
 // Per block: MAXP pass offsets / segment lengths, rg_t1_seg_bytes(symcap)
 // bytes of segment arena.
 int rg_t1_maxp();
 size_t rg_t1_seg_bytes(int symcap);
 int rg_launch_t1_parallel(hipStream_t s, const int32_t *q, int pitch, size_t plane_stride, const rg_cb_desc *desc,
                           int ncb, uint8_t *syms, int symcap, int4 *cm, int32_t *poffs, uint8_t *arena,
-                          int32_t *seglens, uint8_t *out, int slot_bytes, rg_cb_result *res);
-int rg_launch_t1_jobs_parallel(hipStream_t s, const void *jobs, int njobs);
+                          int32_t *seglens, uint8_t *out, int slot_bytes, rg_cb_result *res, int style);
+int rg_launch_t1_jobs_parallel(hipStream_t s, const void *jobs, int njobs, int style);
 // Persistent batched tier-1 (any style): nwaves workgroups pull job indices
 // from *counter (reset by a kernel inside the launch sequence).
 int rg_launch_t1_queue(hipStream_t s, const void *jobs, int njobs, int style, int *counter, int nwaves);

@@ -102,15 +102,16 @@ struct SymList {
 
 // Sign symbol for window row i with neighbour windows: own column (cur state
 // above, pre state below), left (Lw), right (rw); sign windows negw/neglw/negrw.
+// raw (bypass pass): the bit is the sign itself, without the XOR prediction.
 RG_DEV void push_sign(SymList &sl, int i, uint32_t cur, uint32_t sw, uint32_t Lw, uint32_t rw, uint32_t negw,
-                      uint32_t neglw, uint32_t negrw)
+                      uint32_t neglw, uint32_t negrw, bool raw = false)
 {
     int H = (b1(Lw, i) ? (b1(neglw, i) ? -1 : 1) : 0) + (b1(rw, i) ? (b1(negrw, i) ? -1 : 1) : 0);
     int V = (b1(cur, i - 1) ? (b1(negw, i - 1) ? -1 : 1) : 0) + (b1(sw, i + 1) ? (b1(negw, i + 1) ? -1 : 1) : 0);
     H = H > 1 ? 1 : (H < -1 ? -1 : H);
     V = V > 1 ? 1 : (V < -1 ? -1 : V);
     int sc = sc_ctx(H, V);
-    sl.push(sc >> 1, (int)b1(negw, i) ^ (sc & 1));
+    sl.push(sc >> 1, (int)b1(negw, i) ^ (raw ? 0 : (sc & 1)));
 }
 
 RG_DEV int zc_at(int orient, int i, uint32_t cur, uint32_t sw, uint32_t Lw, uint32_t rw)
@@ -217,9 +218,15 @@ RG_DEV void mq_stripe(MQEnc &m, int &ctxv, int mqtv, const SymList &sl, int lane
 }
 
 // Code-block styles (T.800 Table A.19) the coders implement.
+constexpr int STYLE_BYPASS = 0x01;    // raw SPP/MRP passes from the fifth bit-plane (with TERMALL)
 constexpr int STYLE_RESET = 0x02;     // contexts reset after every coding pass
 constexpr int STYLE_TERMALL = 0x04;   // codeword terminated after every pass
 constexpr int MAXP = 96;              // per-block pass-array stride (>= 3*32-2)
+
+// Under BYPASS, coding pass k (0 = the first cleanup pass) is raw-coded when
+// it is a significance-propagation or refinement pass of the fifth or a later
+// bit-plane: k >= 10 and k mod 3 != 0.
+RG_DEV bool pass_is_raw(int style, int k) { return (style & STYLE_BYPASS) && k >= 10 && k % 3 != 0; }
 
 // One code-block job: coefficients (row pitch), size, orientation, output slot.
 struct T1Job {
@@ -229,6 +236,7 @@ struct T1Job {
     rg_cb_result *res;
     uint8_t *sym;   // split path: this block's symbol stream (symcap bytes, 16-byte aligned)
     int symcap;
+    int style;      // split path: code-block style bits (STYLE_*); BYPASS changes raw signs
     i32x4 *cm;      // split path: {symbols, passes, bit-planes, overflow}
     int32_t *poff;  // split path (may be null): stream offset where each pass starts (MAXP)
     uint8_t *seg;   // parallel style: this block's segment arena (seg_bytes(symcap) bytes)
@@ -353,6 +361,8 @@ RG_DEV void t1_block(const T1Job &job)
 
                 if (pass == 0) {
                     // ---- significance propagation ----
+                    // (a raw pass under BYPASS sends signs unpredicted; this pass is pidx - 1)
+                    const bool rawp = SPLIT && pass_is_raw(job.style, pidx - 1);
                     // no significant sample in any column's 6-row window: nobody joins the pass
                     if (!wv_any((sw & 0x3Fu) != 0)) continue;
                     uint32_t N = 0, P = 0, Lw = lw;
@@ -384,7 +394,7 @@ RG_DEV void t1_block(const T1Job &job)
                         const int bit = (int)b1(bw, i);
                         sl.push(zc_at(orient, i, cur, sw, Lw, rw), bit);
                         if (bit) {
-                            push_sign(sl, i, cur, sw, Lw, rw, negw, neglw, negrw);
+                            push_sign(sl, i, cur, sw, Lw, rw, negw, neglw, negrw, rawp);
                             cur |= 1u << i;
                         }
                     }
@@ -687,6 +697,75 @@ RG_DEV void t1_mq_pass_lane(uint32_t *tab, bool active, const uint8_t *sym, int 
     } else {
         len = m.bp;
     }
+    if ((len & 15) != 0 && (len & ~15) < cap) wv_store16(seg + (len & ~15), m.o0, m.o1, m.o2, m.o3);
+    *len_out = (m.ovf || len > cap) ? -1 : len;
+}
+
+// Raw (bypass) pass of the parallel style with BYPASS (T.800 D.6): the
+// decision bits of symbols [s0, s1) go out as they are, MSB first, with 7
+// bits in the byte after an 0xFF (its MSB a stuffed 0); an unfinished byte,
+// including the 7-bit byte owed after a final 0xFF, is padded with 0,1,0,...
+// so the segment never ends in 0xFF.  Bits are taken 16 symbols at a time:
+// the low bit of each symbol byte is gathered with one multiply per word.
+// The coder reuses LaneMQ: B holds the bits of the byte being filled, CT the
+// free bits left in it, bp the next byte index.
+RG_DEV void raw_put_bits(LaneMQ &m, uint32_t bits, int nb, uint8_t *out, int cap)
+{
+    // at most three bytes complete from 16 bits (bytes hold >= 7)
+#pragma unroll
+    for (int it = 0; it < 3; ++it) {
+        if (nb >= m.CT) {
+            nb -= m.CT;
+            const uint32_t byte = ((m.B << m.CT) | (bits >> nb)) & 0xFFu;
+            bits &= (1u << nb) - 1u;
+            lane_put(m, byte, m.bp, out, cap);
+            m.bp++;
+            m.CT = byte == 0xFFu ? 7 : 8;
+            m.B = 0;
+        }
+    }
+    m.B = (m.B << nb) | bits;
+    m.CT -= nb;
+}
+
+RG_DEV void t1_raw_pass_lane(bool active, const uint8_t *sym, int s0, int s1, uint8_t *seg, int cap,
+                             int32_t *len_out)
+{
+    const int c0 = active ? s0 >> 4 : 0, c1 = active ? (s1 + 15) >> 4 : 0;
+    int nch = c1 - c0, nmax = nch;
+    for (int o = 32; o >= 1; o >>= 1) {
+        const int t = wv_shfl_xor(nmax, o);
+        nmax = nmax > t ? nmax : t;
+    }
+    LaneMQ m;
+    m.A = 0; m.C = 0; m.B = 0; m.CT = 8; m.bp = 0; m.ovf = 0;
+    m.o0 = m.o1 = m.o2 = m.o3 = 0;
+    const int last = nch > 0 ? c1 - 1 : c0;
+    u32x4 nxt = wv_load16(sym + 16 * c0);
+    for (int i = 0; i < nmax; ++i) {
+        const u32x4 cur = nxt;
+        const int cn = c0 + i + 1;
+        nxt = wv_load16(sym + 16 * (cn < last ? cn : last));
+        // symbol k of the chunk -> bit 15 - k (first symbol most significant)
+        const uint32_t g0 = ((cur.x & 0x01010101u) * 0x08040201u) >> 24;
+        const uint32_t g1 = ((cur.y & 0x01010101u) * 0x08040201u) >> 24;
+        const uint32_t g2 = ((cur.z & 0x01010101u) * 0x08040201u) >> 24;
+        const uint32_t g3 = ((cur.w & 0x01010101u) * 0x08040201u) >> 24;
+        const uint32_t b16 = (g0 << 12) | (g1 << 8) | (g2 << 4) | g3;
+        const int base = 16 * (c0 + i);
+        const int lo = s0 > base ? s0 - base : 0;
+        const int hi = s1 - base < 16 ? s1 - base : 16;
+        const int nb = i < nch && hi > lo ? hi - lo : 0;
+        raw_put_bits(m, (b16 >> (16 - hi < 16 ? 16 - hi : 0)) & ((1u << nb) - 1u), nb, seg, cap);
+    }
+    if (!active) return;
+    if (m.CT < 8) {
+        const int n = m.CT;
+        const uint32_t byte = ((m.B << n) | (0x55u >> (8 - n))) & 0xFFu;
+        lane_put(m, byte, m.bp, seg, cap);
+        m.bp++;
+    }
+    const int len = m.bp;
     if ((len & 15) != 0 && (len & ~15) < cap) wv_store16(seg + (len & ~15), m.o0, m.o1, m.o2, m.o3);
     *len_out = (m.ovf || len > cap) ? -1 : len;
 }

@@ -145,9 +145,35 @@ struct DecJob {
     const uint8_t *data;
     int len, npasses, nbp, w, h, orient, pitch;
     int32_t *dst;
-    int style;              // code-block style: RESET (0x02) / TERMALL (0x04) bits
+    int style;              // code-block style: BYPASS (0x01) / RESET (0x02) / TERMALL (0x04) bits
     const int32_t *seglen;  // TERMALL: codeword segment length of each pass
 };
+
+// Raw (bypass) segment reader (T.800 D.6): bits MSB first, 7 in the byte
+// after an 0xFF; past the segment end it reads 0xFF bytes, as the MQ decoder
+// does.  Reuses MQDec: C holds the current byte, CT its unread bits.
+__device__ __forceinline__ void raw_start(MQDec &m, const uint8_t *lds, int off, int len)
+{
+    m.buf = lds + off; m.len = len; m.bp = 0; m.C = 0; m.CT = 0; m.A = 0;
+}
+
+__device__ __forceinline__ int raw_bit(MQDec &m)
+{
+    if (m.CT == 0) {
+        const bool after_ff = m.C == 0xFFu;
+        m.C = mqd_byte(m, m.bp);
+        m.bp++;
+        m.CT = after_ff ? 7 : 8;
+    }
+    m.CT--;
+    return (int)((m.C >> m.CT) & 1u);
+}
+
+// One decision of the current pass: raw bit (bypass pass) or MQ symbol.
+__device__ __forceinline__ int dec_bit(MQDec &m, bool raw, int &ctxv, int mqtv, int cx, int lane)
+{
+    return raw ? raw_bit(m) : mqd_decode(m, ctxv, mqtv, cx, lane);
+}
 
 // INITDEC (C.3.5) on the codeword segment [off, off + len) of the LDS copy.
 __device__ __forceinline__ void mqd_start(MQDec &m, const uint8_t *lds, int off, int len)
@@ -185,13 +211,17 @@ __device__ void t1_decode_block(const DecJob &job, uint8_t *lds)
     for (int p = nbp - 1; p >= 0; --p) {
         uint64_t B = 0;  // bit p of this lane's column
         for (int pass = (p == nbp - 1) ? 2 : 0; pass < 3 && passes < job.npasses; ++pass, ++passes) {
+            // BYPASS (with TERMALL): SPP/MRP passes from index 10 on are raw segments
+            const bool raw = termall && (job.style & 0x01) && passes >= 10 && pass < 2;
             if (passes > 0) {
                 if (job.style & 0x02) ctxv = ctx0;      // RESET: initial context states
                 if (termall) {                          // TERMALL: next codeword segment
                     seg_off += job.seglen[passes - 1];
                     const int rest = len - seg_off;
                     const int sl = job.seglen[passes];
-                    mqd_start(m, lds, seg_off < len ? seg_off : len, rest <= 0 ? 0 : (sl < rest ? sl : rest));
+                    const int o = seg_off < len ? seg_off : len, n = rest <= 0 ? 0 : (sl < rest ? sl : rest);
+                    if (raw) raw_start(m, lds, o, n);
+                    else mqd_start(m, lds, o, n);
                 }
             }
             for (int s = 0; s < nstripes; ++s) {
@@ -222,7 +252,7 @@ __device__ void t1_decode_block(const DecJob &job, uint8_t *lds)
                         const uint32_t cj = (uint32_t)__builtin_amdgcn_readlane((int)cxs, j);
                         uint32_t bits = 0;
                         for (int k = 0; k < 4; ++k)
-                            if (bit_(Mj, k) && mqd_decode(m, ctxv, mqtv, (int)((cj >> (5 * k)) & 31u), lane))
+                            if (bit_(Mj, k) && dec_bit(m, raw, ctxv, mqtv, (int)((cj >> (5 * k)) & 31u), lane))
                                 bits |= 1u << k;
                         if (lane == j) B |= (uint64_t)bits << r0;
                     }
@@ -251,8 +281,10 @@ __device__ void t1_decode_block(const DecJob &job, uint8_t *lds)
                                                     ((Rw >> (i - 1)) & 7u);
                                 if (!nb) continue;
                                 vis |= 1u << k;
-                                if (mqd_decode(m, ctxv, mqtv, zc_at_d(orient, i, Lw, Cw, Rw), lane)) {
-                                    const int neg = decode_sign(m, ctxv, mqtv, lane, i, Lw, Lneg, Cw, Cneg, Rw, Rneg);
+                                if (dec_bit(m, raw, ctxv, mqtv, zc_at_d(orient, i, Lw, Cw, Rw), lane)) {
+                                    const int neg = raw ? raw_bit(m)
+                                                        : decode_sign(m, ctxv, mqtv, lane, i, Lw, Lneg, Cw, Cneg, Rw,
+                                                                      Rneg);
                                     Cw |= 1u << i;
                                     Cneg |= (uint32_t)neg << i;
                                     nsig |= 1u << k;

@@ -30,6 +30,7 @@ __global__ void __launch_bounds__(64) t1_encode_kernel(const int32_t *__restrict
     job.pitch = pitch; job.w = d.w; job.h = d.h; job.orient = d.orient; job.cap = slot_bytes;
     job.out = out + (size_t)blockIdx.x * slot_bytes;
     job.res = res + blockIdx.x;
+    job.style = 0;
     t1_block<false>(job);
 }
 
@@ -88,7 +89,8 @@ __global__ void reset_counter_kernel(int *counter) { *counter = 0; }
 // ------------------------------------------------------------------------
 __global__ void __launch_bounds__(64) t1_cm_kernel(const int32_t *__restrict__ q, int pitch, size_t plane_stride,
                                                    const rg_cb_desc *__restrict__ desc, uint8_t *__restrict__ syms,
-                                                   int symcap, int4 *__restrict__ cm, int32_t *__restrict__ poffs)
+                                                   int symcap, int4 *__restrict__ cm, int32_t *__restrict__ poffs,
+                                                   int style)
 {
     const rg_cb_desc d = desc[blockIdx.x];
     T1Job job;
@@ -98,6 +100,7 @@ __global__ void __launch_bounds__(64) t1_cm_kernel(const int32_t *__restrict__ q
     job.symcap = symcap;
     job.cm = cm + blockIdx.x;
     job.poff = poffs ? poffs + (size_t)blockIdx.x * MAXP : nullptr;
+    job.style = style;
     t1_block<true>(job);
 }
 
@@ -121,33 +124,41 @@ __global__ void __launch_bounds__(64) t1_mq_jobs_kernel(const T1Job *__restrict_
 }
 
 // ---- parallel style: grid (block groups of 64) x (pass index) ----
+// Row q codes pass q of 64 blocks; under BYPASS a raw row (pass_is_raw, the
+// same for the whole workgroup) packs bits instead of running MQ coders.
 __global__ void __launch_bounds__(64) t1_mqp_kernel(const uint8_t *__restrict__ syms, int symcap,
                                                     const int4 *__restrict__ cm, const int32_t *__restrict__ poffs,
                                                     int ncb, uint8_t *__restrict__ arena, size_t arena_stride,
-                                                    int32_t *__restrict__ seglens)
+                                                    int32_t *__restrict__ seglens, int style)
 {
     __shared__ uint32_t tab[48];
     const int b = blockIdx.x * 64 + threadIdx.x;
     bool active = b < ncb;
     int p = 0, s0 = 0, s1 = 0, at = 0, cap = 0;
-    // rows dispatched latest pass first: the long chains (low bit-planes) start early
-    const int q = (int)(gridDim.y - 1 - blockIdx.y);
+    const int q = (int)blockIdx.y;
     if (active) active = pass_range(cm[b], poffs + (size_t)b * MAXP, q, p, s0, s1, at, cap);
     const size_t bb = active ? (size_t)b : 0;
-    t1_mq_pass_lane(tab, active, syms + bb * symcap, s0, s1, arena + bb * arena_stride + at, cap,
-                    seglens + bb * MAXP + p);
+    if (pass_is_raw(style, q))
+        t1_raw_pass_lane(active, syms + bb * symcap, s0, s1, arena + bb * arena_stride + at, cap,
+                         seglens + bb * MAXP + p);
+    else
+        t1_mq_pass_lane(tab, active, syms + bb * symcap, s0, s1, arena + bb * arena_stride + at, cap,
+                        seglens + bb * MAXP + p);
 }
 
-__global__ void __launch_bounds__(64) t1_mqp_jobs_kernel(const T1Job *__restrict__ jobs, int njobs)
+__global__ void __launch_bounds__(64) t1_mqp_jobs_kernel(const T1Job *__restrict__ jobs, int njobs, int style)
 {
     __shared__ uint32_t tab[48];
     const int b = blockIdx.x * 64 + threadIdx.x;
     bool active = b < njobs;
     int p = 0, s0 = 0, s1 = 0, at = 0, cap = 0;
     const T1Job &j = jobs[active ? b : 0];
-    const int q = (int)(gridDim.y - 1 - blockIdx.y);   // latest pass first, as in t1_mqp_kernel
+    const int q = (int)blockIdx.y;
     if (active) active = pass_range(*j.cm, j.poff, q, p, s0, s1, at, cap);
-    t1_mq_pass_lane(tab, active, j.sym, s0, s1, j.seg + at, cap, j.seglen + p);
+    if (pass_is_raw(style, q))
+        t1_raw_pass_lane(active, j.sym, s0, s1, j.seg + at, cap, j.seglen + p);
+    else
+        t1_mq_pass_lane(tab, active, j.sym, s0, s1, j.seg + at, cap, j.seglen + p);
 }
 
 __global__ void __launch_bounds__(64) t1_pack_kernel(const int4 *__restrict__ cm, const int32_t *__restrict__ poffs,
@@ -173,27 +184,28 @@ size_t rg_t1_seg_bytes(int symcap) { return (size_t)seg_bytes(symcap); }
 
 int rg_launch_t1_parallel(hipStream_t s, const int32_t *q, int pitch, size_t plane_stride, const rg_cb_desc *desc,
                           int ncb, uint8_t *syms, int symcap, int4 *cm, int32_t *poffs, uint8_t *arena,
-                          int32_t *seglens, uint8_t *out, int slot_bytes, rg_cb_result *res)
+                          int32_t *seglens, uint8_t *out, int slot_bytes, rg_cb_result *res, int style)
 {
     if (ncb <= 0) return RG_SUCCESS;
     const size_t astride = rg_t1_seg_bytes(symcap);
-    t1_cm_kernel<<<dim3((unsigned)ncb), dim3(64), 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm, poffs);
+    t1_cm_kernel<<<dim3((unsigned)ncb), dim3(64), 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm, poffs,
+                                                          style);
     RG_LAUNCH_CHECK();
     t1_mqp_kernel<<<dim3((unsigned)((ncb + 63) / 64), (unsigned)(MAXP - 2)), dim3(64), 0, s>>>(
-        syms, symcap, cm, poffs, ncb, arena, astride, seglens);
+        syms, symcap, cm, poffs, ncb, arena, astride, seglens, style);
     RG_LAUNCH_CHECK();
     t1_pack_kernel<<<dim3((unsigned)ncb), dim3(64), 0, s>>>(cm, poffs, arena, astride, seglens, out, slot_bytes, res);
     RG_LAUNCH_CHECK();
     return RG_SUCCESS;
 }
 
-int rg_launch_t1_jobs_parallel(hipStream_t s, const void *jobs, int njobs)
+int rg_launch_t1_jobs_parallel(hipStream_t s, const void *jobs, int njobs, int style)
 {
     if (njobs <= 0) return RG_SUCCESS;
     t1_cm_jobs_kernel<<<dim3((unsigned)njobs), dim3(64), 0, s>>>((const T1Job *)jobs);
     RG_LAUNCH_CHECK();
     t1_mqp_jobs_kernel<<<dim3((unsigned)((njobs + 63) / 64), (unsigned)(MAXP - 2)), dim3(64), 0, s>>>(
-        (const T1Job *)jobs, njobs);
+        (const T1Job *)jobs, njobs, style);
     RG_LAUNCH_CHECK();
     t1_pack_jobs_kernel<<<dim3((unsigned)njobs), dim3(64), 0, s>>>((const T1Job *)jobs);
     RG_LAUNCH_CHECK();
@@ -207,7 +219,8 @@ int rg_launch_t1_split(hipStream_t s, const int32_t *q, int pitch, size_t plane_
                        uint8_t *syms, int symcap, int4 *cm, uint8_t *out, int slot_bytes, rg_cb_result *res)
 {
     if (ncb <= 0) return RG_SUCCESS;
-    t1_cm_kernel<<<dim3((unsigned)ncb), dim3(64), 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm, nullptr);
+    t1_cm_kernel<<<dim3((unsigned)ncb), dim3(64), 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm, nullptr,
+                                                          0);
     RG_LAUNCH_CHECK();
     t1_mq_kernel<<<dim3((unsigned)((ncb + 63) / 64)), dim3(64), 0, s>>>(syms, symcap, cm, ncb, out, slot_bytes, res);
     RG_LAUNCH_CHECK();
@@ -235,10 +248,11 @@ int rg_launch_t1_queue(hipStream_t s, const void *jobs, int njobs, int style, in
     RG_LAUNCH_CHECK();
     const dim3 g((unsigned)nwaves), b(64);
     const T1Job *J = (const T1Job *)jobs;
-    if (style == RG_STYLE_PARALLEL) {
+    if (style & RG_STYLE_TERMALL) {
         t1_cm_queue_kernel<<<g, b, 0, s>>>(J, njobs, counter);
         RG_LAUNCH_CHECK();
-        t1_mqp_jobs_kernel<<<dim3((unsigned)((njobs + 63) / 64), (unsigned)(MAXP - 2)), b, 0, s>>>(J, njobs);
+        t1_mqp_jobs_kernel<<<dim3((unsigned)((njobs + 63) / 64), (unsigned)(MAXP - 2)), b, 0, s>>>(J, njobs,
+                                                                                                  style);
         RG_LAUNCH_CHECK();
         t1_pack_jobs_kernel<<<dim3((unsigned)njobs), b, 0, s>>>(J);
     } else if (rg_t1_inline()) {
@@ -263,11 +277,11 @@ int rg_launch_t1_jobs(hipStream_t s, const void *jobs, int njobs)
 // Host-side fill of one job record (the struct layout stays private to the tier-1 sources).
 void rg_t1_job_fill(void *dst, const int32_t *src, int pitch, int w, int h, int orient, uint8_t *out, int cap,
                     rg_cb_result *res, uint8_t *sym, int symcap, int4 *cm, int32_t *poff, uint8_t *seg,
-                    int32_t *seglen)
+                    int32_t *seglen, int style)
 {
     T1Job j;
     j.src = src; j.pitch = pitch; j.w = w; j.h = h; j.orient = orient; j.cap = cap; j.out = out; j.res = res;
-    j.sym = sym; j.symcap = symcap; j.cm = cm; j.poff = poff; j.seg = seg; j.seglen = seglen;
+    j.sym = sym; j.symcap = symcap; j.style = style; j.cm = cm; j.poff = poff; j.seg = seg; j.seglen = seglen;
     *reinterpret_cast<T1Job *>(dst) = j;
 }
 

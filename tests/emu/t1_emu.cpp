@@ -2,8 +2,10 @@
 // the lock-step wavefront emulator of wave_emu.h, so the kernels' own code
 // (not a restatement) can be checked against the oracle without a GPU.
 //
-//   t1_emu MODE DIR PITCH PLANE_STRIDE SLOT
+//   t1_emu MODE DIR PITCH PLANE_STRIDE SLOT [STYLE]
 //     MODE  inline | split | queue_split | parallel | queue_parallel | cm_only
+//     STYLE code-block style bits of the parallel modes (default 6 =
+//           RESET|TERMALL; 7 adds BYPASS)
 //     DIR   holds q.bin (int32 coefficient planes) and desc.bin (rg_cb_desc
 //           records); res.bin (rg_cb_result per block) and out.bin (SLOT
 //           bytes per block) are written there.
@@ -161,7 +163,7 @@ struct MQQueue {
 // parallel style, driven the way t1_mqp_kernel / t1_mqp_jobs_kernel drive it
 struct MQPass {
     const T1Job *jobs;
-    int i0, njobs, p;
+    int i0, njobs, p, style;
 };
 void mq_pass_body(void *a)
 {
@@ -171,7 +173,10 @@ void mq_pass_body(void *a)
     int p = 0, s0 = 0, s1 = 0, at = 0, cap = 0;
     const T1Job &j = m.jobs[active ? b : 0];
     if (active) active = pass_range(*j.cm, j.poff, m.p, p, s0, s1, at, cap);
-    t1_mq_pass_lane(g_tab, active, j.sym, s0, s1, j.seg + at, cap, j.seglen + p);
+    if (pass_is_raw(m.style, m.p))
+        t1_raw_pass_lane(active, j.sym, s0, s1, j.seg + at, cap, j.seglen + p);
+    else
+        t1_mq_pass_lane(g_tab, active, j.sym, s0, s1, j.seg + at, cap, j.seglen + p);
 }
 void pack_body(void *a)
 {
@@ -207,10 +212,11 @@ void spill(const std::string &path, const void *p, size_t n)
 
 int main(int argc, char **argv)
 {
-    if (argc != 6) {
-        fprintf(stderr, "usage: t1_emu MODE DIR PITCH PLANE_STRIDE SLOT\n");
+    if (argc != 6 && argc != 7) {
+        fprintf(stderr, "usage: t1_emu MODE DIR PITCH PLANE_STRIDE SLOT [STYLE]\n");
         return 2;
     }
+    const int style = argc == 7 ? atoi(argv[6]) : STYLE_RESET | STYLE_TERMALL;
     const std::string mode = argv[1], dir = argv[2];
     const int pitch = atoi(argv[3]);
     const long plane_stride = atol(argv[4]);
@@ -240,6 +246,7 @@ int main(int argc, char **argv)
         j.symcap = symcap;
         j.cm = &cm[k];
         const bool par = mode == "parallel" || mode == "queue_parallel";
+        j.style = par ? style : 0;
         j.poff = par ? &poffs[(size_t)k * MAXP] : nullptr;
         j.seg = arena + (size_t)k * astride;
         j.seglen = &seglens[(size_t)k * MAXP];
@@ -268,7 +275,7 @@ int main(int argc, char **argv)
         for (int k = 0; k < ncb; ++k) emu::run(cm_body, &jobs[k]);
         for (int p = 0; p < MAXP - 2; ++p)
             for (int i0 = 0; i0 < ncb; i0 += 64) {
-                MQPass m{jobs.data(), i0, ncb, p};
+                MQPass m{jobs.data(), i0, ncb, p, style};
                 emu::run(mq_pass_body, &m);
             }
         for (int k = 0; k < ncb; ++k) emu::run(pack_body, &jobs[k]);

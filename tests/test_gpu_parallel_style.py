@@ -1,8 +1,8 @@
-"""GPU: the parallel code-block style (RESET | TERMALL: one MQ chain per
-coding pass) against the CPU oracle in the same style -- bit-exact codewords
-and per-pass segment lengths, the same codestream bytes from the library's
-tier-2 writer, OpenJPEG decoding it, the native decoder inverting it, and the
-batched path."""
+"""GPU: the parallel code-block styles (RESET | TERMALL: one MQ chain per
+coding pass; with BYPASS, raw low bit-plane passes) against the CPU oracle in
+the same style -- bit-exact codewords and per-pass segment lengths, the same
+codestream bytes from the library's tier-2 writer, OpenJPEG decoding it, the
+native decoder inverting it, and the batched path."""
 import io
 
 import numpy as np
@@ -11,14 +11,15 @@ import pytest
 from oracle import oracle as O
 from roger_amd import runtime as R
 from roger_amd import synth
-from roger_amd._lib import RG_STYLE_PARALLEL
+from roger_amd._lib import RG_STYLE_PARALLEL, RG_STYLE_PARALLEL_BYPASS
 
 pytestmark = pytest.mark.gpu
 
-PAR = RG_STYLE_PARALLEL
-assert PAR == O.STYLE_PARALLEL
+assert RG_STYLE_PARALLEL == O.STYLE_PARALLEL and RG_STYLE_PARALLEL_BYPASS == O.STYLE_PARALLEL_BYPASS
+STYLES = [RG_STYLE_PARALLEL, RG_STYLE_PARALLEL_BYPASS]
 
 
+@pytest.mark.parametrize("PAR", STYLES)
 @pytest.mark.parametrize("w,h,nc,rev,levels,cb", [
     (512, 512, 1, True, 5, 64),
     (300, 257, 3, True, 4, 64),
@@ -26,7 +27,7 @@ assert PAR == O.STYLE_PARALLEL
     (128, 96, 1, False, 3, 32),
     (33, 1, 1, False, 1, 64),
 ])
-def test_encoder_parallel_style_vs_oracle(ctx, w, h, nc, rev, levels, cb):
+def test_encoder_parallel_style_vs_oracle(ctx, w, h, nc, rev, levels, cb, PAR):
     img = synth.image(w, h, nc, 21)
     enc = R.Encoder(ctx)
     enc.encode_u8(img, levels=levels, reversible=rev, cbw=cb, cbh=cb, delta=1.0 / 64, style=PAR)
@@ -71,8 +72,9 @@ def test_encoder_parallel_style_vs_oracle(ctx, w, h, nc, rev, levels, cb):
     enc.close()
 
 
+@pytest.mark.parametrize("PAR", STYLES)
 @pytest.mark.parametrize("wpc", [0, 4])
-def test_batch_parallel_style_vs_oracle(ctx, wpc):
+def test_batch_parallel_style_vs_oracle(ctx, wpc, PAR):
     specs = [(256, 256, 1, True, 5), (300, 190, 3, False, 5), (97, 61, 1, False, 3), (64, 64, 3, False, 2),
              (33, 1, 1, False, 1), (129, 257, 1, True, 4)]
     imgs = []
@@ -98,3 +100,20 @@ def test_batch_parallel_style_vs_oracle(ctx, wpc):
                                   style=PAR)
             assert b.j2k(i) == O.t2_write(params, rs, data, passlen)
     b.close()
+
+
+def test_bypass_dense_block_vs_oracle(ctx):
+    """Uniform noise with no transform: one dense 64x64 block whose eight
+    bit-planes are near-random, so the raw passes (from pass 10 on) carry
+    most of the codeword."""
+    rng = np.random.default_rng(5)
+    img = rng.integers(0, 256, (64, 64), dtype=np.uint8)
+    enc = R.Encoder(ctx)
+    enc.encode_u8(img, levels=0, reversible=True, cbw=64, cbh=64, style=RG_STYLE_PARALLEL_BYPASS)
+    desc, res, blobs = enc.codeblocks()
+    seg = enc.pass_lengths()
+    rs, data, q, passlen = O.encode_image(img, levels=0, reversible=True, cbw=64, cbh=64,
+                                          style=RG_STYLE_PARALLEL_BYPASS, return_passlen=True)
+    assert blobs == data
+    np.testing.assert_array_equal(seg[0, :rs[0, 1]], passlen[0, :rs[0, 1]])
+    enc.close()

@@ -219,7 +219,12 @@ def test_openjpeg_decodes_lossy_like_oracle(w, h, nc, levels, delta):
 
 
 # ---------------------------------------------------------------- code-block styles (RESET / TERMALL)
-@pytest.mark.parametrize("style", [O.STYLE_RESET, O.STYLE_TERMALL, O.STYLE_PARALLEL])
+def _raw_pass(style, k):
+    """Pass k is raw-coded under BYPASS: SPP/MRP from the fifth bit-plane."""
+    return bool(style & O.STYLE_BYPASS) and k >= 10 and k % 3 != 0
+
+
+@pytest.mark.parametrize("style", [O.STYLE_RESET, O.STYLE_TERMALL, O.STYLE_PARALLEL, O.STYLE_PARALLEL_BYPASS])
 def test_t1_style_roundtrip(style):
     rng = np.random.default_rng(style)
     for (w, h) in [(64, 64), (37, 13), (1, 1), (5, 64)]:
@@ -230,8 +235,35 @@ def test_t1_style_roundtrip(style):
             data, npass, nbp, passlen = O.t1_encode(b, orient, style=style)
             if style & O.STYLE_TERMALL:
                 assert len(passlen) == npass and int(passlen.sum()) == len(data)
-                assert (passlen > 0).all()
+                # an MQ segment holds at least its flush byte; a raw one may be empty
+                mq = np.array([not _raw_pass(style, k) for k in range(npass)])
+                assert (passlen[mq] > 0).all()
             np.testing.assert_array_equal(O.t1_decode(data, w, h, orient, nbp, npass, style, passlen), b)
+
+
+def test_bypass_raw_segments_closed_form():
+    """All 64x64 magnitudes 2^14-1: after the first cleanup every sample is
+    significant, so each raw refinement pass carries 4096 one-bits and each
+    raw significance pass nothing.  With bit stuffing that is 273 x (FF 7F)
+    -- 15 bits per pair -- then one bit padded 0,1,0,... = 0xAA."""
+    b = np.full((64, 64), (1 << 14) - 1, np.int32)
+    b[::3, ::5] *= -1
+    data, npass, nbp, passlen = O.t1_encode(b, 0, style=O.STYLE_PARALLEL_BYPASS)
+    assert nbp == 14 and npass == 3 * 14 - 2
+    offs = np.concatenate([[0], np.cumsum(passlen)])
+    mrp = bytes([0xFF, 0x7F]) * 273 + bytes([0xAA])
+    for k in range(10, npass):
+        seg = data[offs[k]:offs[k + 1]]
+        if k % 3 == 1:
+            assert seg == b""
+        elif k % 3 == 2:
+            assert seg == mrp
+    # the MQ-coded passes are those of the parallel style without BYPASS
+    data0, _, _, passlen0 = O.t1_encode(b, 0, style=O.STYLE_PARALLEL)
+    offs0 = np.concatenate([[0], np.cumsum(passlen0)])
+    for k in range(npass):
+        if not _raw_pass(O.STYLE_PARALLEL_BYPASS, k):
+            assert data[offs[k]:offs[k + 1]] == data0[offs0[k]:offs0[k + 1]]
 
 
 def test_t1_style_changes_the_codeword():
@@ -240,15 +272,18 @@ def test_t1_style_changes_the_codeword():
     b = np.random.default_rng(1).laplace(0, 50, size=(64, 64)).astype(np.int32)
     plain = O.t1_encode(b, 0)[0]
     assert O.t1_encode(b, 0, style=0)[0] == plain
-    outs = {s: O.t1_encode(b, 0, style=s)[0] for s in (O.STYLE_RESET, O.STYLE_TERMALL, O.STYLE_PARALLEL)}
-    assert len({plain, *outs.values()}) == 4
-    # a terminated segment never ends in 0xFF (so a restarted coder starts with CT = 12)
-    data, npass, nbp, passlen = O.t1_encode(b, 0, style=O.STYLE_PARALLEL)
-    ends = np.cumsum(passlen) - 1
-    assert all(data[e] != 0xFF for e in ends)
+    styles = (O.STYLE_RESET, O.STYLE_TERMALL, O.STYLE_PARALLEL, O.STYLE_PARALLEL_BYPASS)
+    outs = {s: O.t1_encode(b, 0, style=s)[0] for s in styles}
+    assert len({plain, *outs.values()}) == 5
+    # a terminated segment never ends in 0xFF (so a restarted coder starts with
+    # CT = 12, and a raw segment leaves no stuffed bit owed)
+    for st in (O.STYLE_PARALLEL, O.STYLE_PARALLEL_BYPASS):
+        data, npass, nbp, passlen = O.t1_encode(b, 0, style=st)
+        ends = np.cumsum(passlen)[passlen > 0] - 1
+        assert all(data[e] != 0xFF for e in ends)
 
 
-@pytest.mark.parametrize("style", [O.STYLE_RESET, O.STYLE_TERMALL, O.STYLE_PARALLEL])
+@pytest.mark.parametrize("style", [O.STYLE_RESET, O.STYLE_TERMALL, O.STYLE_PARALLEL, O.STYLE_PARALLEL_BYPASS])
 @pytest.mark.parametrize("w,h,nc,levels,cb", [(256, 200, 3, 5, 64), (97, 61, 1, 3, 32), (8, 3, 3, 4, 64)])
 def test_openjpeg_decodes_styles_lossless(style, w, h, nc, levels, cb):
     """OpenJPEG (an independent decoder) reads the COD style byte and the
@@ -264,3 +299,7 @@ def test_openjpeg_decodes_parallel_style_lossy():
     cs_par = O.encode_j2k(img, levels=6, reversible=False, delta=1.0 / 64, style=O.STYLE_PARALLEL)[0]
     np.testing.assert_array_equal(_decode(cs_par), _decode(cs_std))   # same coefficients, same image
     assert len(cs_par) > len(cs_std)                                     # termination costs bytes
+    # BYPASS: raw low bit-planes, decoded by OpenJPEG to the same image
+    cs_byp = O.encode_j2k(img, levels=6, reversible=False, delta=1.0 / 64, style=O.STYLE_PARALLEL_BYPASS)[0]
+    np.testing.assert_array_equal(_decode(cs_byp), _decode(cs_std))
+    assert abs(len(cs_byp) / len(cs_par) - 1) < 0.05

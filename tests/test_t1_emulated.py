@@ -31,12 +31,12 @@ def emu(tmp_path_factory):
     return exe
 
 
-def _run(emu, tmp_path, mode, q, desc, pitch, plane_stride, slot):
+def _run(emu, tmp_path, mode, q, desc, pitch, plane_stride, slot, style=None):
     d = str(tmp_path)
     np.ascontiguousarray(q, np.int32).tofile(os.path.join(d, "q.bin"))
     np.ascontiguousarray(desc, CB_DESC_DTYPE).tofile(os.path.join(d, "desc.bin"))
-    r = subprocess.run([emu, mode, d, str(pitch), str(plane_stride), str(slot)], capture_output=True, text=True,
-                       timeout=300)
+    args = [emu, mode, d, str(pitch), str(plane_stride), str(slot)] + ([str(style)] if style is not None else [])
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     res = np.fromfile(os.path.join(d, "res.bin"), CB_RES_DTYPE)
     out = np.fromfile(os.path.join(d, "out.bin"), np.uint8).reshape(len(desc), slot)
@@ -123,12 +123,13 @@ def test_small_slots_flag_overflow(emu, tmp_path, mode):
             assert bytes(raw[i]) == data[:slot]   # the slot holds the codeword's first bytes
 
 
-def _run_parallel(emu, tmp_path, mode, q, desc, pitch, plane_stride, slot):
-    res, blobs = _run(emu, tmp_path, mode, q, desc, pitch, plane_stride, slot)
+def _run_parallel(emu, tmp_path, mode, q, desc, pitch, plane_stride, slot, style=O.STYLE_PARALLEL):
+    res, blobs = _run(emu, tmp_path, mode, q, desc, pitch, plane_stride, slot, style)
     seglen = np.fromfile(os.path.join(str(tmp_path), "seglen.bin"), np.int32).reshape(len(desc), O.MAX_PASSES)
     return res, blobs, seglen
 
 
+@pytest.mark.parametrize("style", [O.STYLE_PARALLEL, O.STYLE_PARALLEL_BYPASS])
 @pytest.mark.parametrize("mode", ["parallel", "queue_parallel"])
 @pytest.mark.parametrize("w,h,nc,rev,levels,seed,cb", [
     (97, 61, 1, False, 3, 52, 64),
@@ -136,12 +137,12 @@ def _run_parallel(emu, tmp_path, mode, q, desc, pitch, plane_stride, slot):
     (64, 64, 3, False, 2, 54, 32),
     (129, 57, 1, True, 4, 55, 64),
 ])
-def test_parallel_style_matches_oracle(emu, tmp_path, mode, w, h, nc, rev, levels, seed, cb):
-    """RESET|TERMALL: pass-parallel MQ lanes + segment packing give the
-    oracle's codeword and per-pass segment lengths."""
+def test_parallel_style_matches_oracle(emu, tmp_path, mode, w, h, nc, rev, levels, seed, cb, style):
+    """RESET|TERMALL (and with BYPASS): pass-parallel MQ / raw lanes + segment
+    packing give the oracle's codeword and per-pass segment lengths."""
     img = synth.image(w, h, nc, seed)
     rs, data, q, passlen = O.encode_image(img, levels=levels, reversible=rev, cbw=cb, cbh=cb, delta=1.0 / 64,
-                                          style=O.STYLE_PARALLEL, return_passlen=True)
+                                          style=style, return_passlen=True)
     one = O.codeblocks(w, h, levels, cb, cb)
     desc = np.zeros(len(one) * nc, CB_DESC_DTYPE)
     k = 0
@@ -149,7 +150,7 @@ def test_parallel_style_matches_oracle(emu, tmp_path, mode, w, h, nc, rev, level
         for d in one:
             desc[k] = (d["x0"], d["y0"], d["w"], d["h"], c, d["orient"], d["band"], d["res"])
             k += 1
-    res, blobs, seglen = _run_parallel(emu, tmp_path, mode, q, desc, w, w * h, max(256, cb * cb * 3))
+    res, blobs, seglen = _run_parallel(emu, tmp_path, mode, q, desc, w, w * h, max(256, cb * cb * 3), style)
     assert (res["flags"] == 0).all()
     np.testing.assert_array_equal(res["nbp"], rs[:, 2])
     np.testing.assert_array_equal(res["npasses"], rs[:, 1])
@@ -159,21 +160,24 @@ def test_parallel_style_matches_oracle(emu, tmp_path, mode, w, h, nc, rev, level
     assert blobs == data
 
 
-def test_parallel_style_dense_blocks(emu, tmp_path):
+@pytest.mark.parametrize("style", [O.STYLE_PARALLEL, O.STYLE_PARALLEL_BYPASS])
+def test_parallel_style_dense_blocks(emu, tmp_path, style):
     rng = np.random.default_rng(23)
     blocks = [np.round(rng.laplace(0, s, (64, 64))).astype(np.int32) for s in (2, 60, 3000, 40000)]
     blocks.append(np.zeros((64, 64), np.int32))
+    # raw segments with many 0xFF bytes (bit stuffing): all-ones refinement bits
+    blocks.append(np.full((64, 64), (1 << 14) - 1, np.int32) * np.where(rng.random((64, 64)) < 0.5, 1, -1))
     plane = np.concatenate(blocks, axis=1)
     desc = np.zeros(len(blocks), CB_DESC_DTYPE)
     for i in range(len(blocks)):
         desc[i] = (64 * i, 0, 64, 64, 0, (i + 1) % 4, 0, 0)
     res, blobs, seglen = _run_parallel(emu, tmp_path, "parallel", plane, desc, plane.shape[1], plane.size,
-                                       64 * 64 * 4)
+                                       64 * 64 * 4, style)
     for i, b in enumerate(blocks):
         if not b.any():
             assert (res["len"][i], res["npasses"][i], res["nbp"][i]) == (0, 0, 0)
             continue
-        data, npasses, nbp, passlen = O.t1_encode(b, (i + 1) % 4, style=O.STYLE_PARALLEL)
+        data, npasses, nbp, passlen = O.t1_encode(b, (i + 1) % 4, style=style)
         assert (res["nbp"][i], res["npasses"][i], res["flags"][i]) == (nbp, npasses, 0)
         np.testing.assert_array_equal(seglen[i, :npasses], passlen)
         assert blobs[i] == data
