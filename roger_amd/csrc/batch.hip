@@ -63,7 +63,7 @@ struct rg_batch {
     std::vector<int> cb_first, cb_count;            // per image, into the block list
     std::vector<rg_cb_desc> desc;                    // all blocks, image-major
     std::vector<int> desc_img;
-    Buf planes, q, scratch, slots, res, t1jobs, cparams, cjobs, syms, cm, poffs, arena, seglens, counter;
+    Buf planes, q, scratch, slots, res, t1jobs, cparams, cjobs, syms, cm, poffs, arena, seglens, counter, chains;
     Buf fargs[16], fjobs[16];
     int nfjobs[16] = {0}, ncjobs = 0, ncb = 0;
     hipGraph_t graph = nullptr;
@@ -93,7 +93,7 @@ extern "C" int rg_batch_destroy(rg_batch *b)
     hipStreamSynchronize(b->ctx->stream);
     drop_graph(b);
     for (Buf *x : {&b->planes, &b->q, &b->scratch, &b->slots, &b->res, &b->t1jobs, &b->cparams, &b->cjobs,
-                   &b->syms, &b->cm, &b->poffs, &b->arena, &b->seglens, &b->counter})
+                   &b->syms, &b->cm, &b->poffs, &b->arena, &b->seglens, &b->counter, &b->chains})
         x->release();
     for (int l = 0; l < 16; ++l) { b->fargs[l].release(); b->fjobs[l].release(); }
     delete b;
@@ -111,8 +111,11 @@ static int issue(rg_batch *b)
         if ((rc = rg_launch_fwd_level_batch(s, (const rg_fwd_level_args *)b->fargs[l].p, (const int4 *)b->fjobs[l].p,
                                             b->nfjobs[l])))
             return rc;
-    if (b->nwaves > 0) return rg_launch_t1_queue(s, b->t1jobs.p, b->ncb, b->style, (int *)b->counter.p, b->nwaves);
-    if (b->style & RG_STYLE_TERMALL) return rg_launch_t1_jobs_parallel(s, b->t1jobs.p, b->ncb, b->style);
+    if (b->nwaves > 0)
+        return rg_launch_t1_queue(s, b->t1jobs.p, b->ncb, b->style, (int *)b->counter.p, b->nwaves,
+                                  (int32_t *)b->chains.p);
+    if (b->style & RG_STYLE_TERMALL)
+        return rg_launch_t1_jobs_parallel(s, b->t1jobs.p, b->ncb, b->style, (int32_t *)b->chains.p);
     if (rg_t1_inline()) return rg_launch_t1_jobs(s, b->t1jobs.p, b->ncb);
     return rg_launch_t1_jobs_split(s, b->t1jobs.p, b->ncb);
 }
@@ -241,7 +244,8 @@ extern "C" int rg_batch_plan(rg_batch *b, const rg_batch_image *imgs, int n, int
     const size_t astride = rg_t1_seg_bytes(symcap);
     if (style && ((rc = b->poffs.ensure(sizeof(int32_t) * maxp * b->ncb)) ||
                   (rc = b->seglens.ensure(sizeof(int32_t) * maxp * b->ncb)) ||
-                  (rc = b->arena.ensure(astride * b->ncb))))
+                  (rc = b->arena.ensure(astride * b->ncb)) ||
+                  (rc = b->chains.ensure(rg_t1_chain_ws_bytes(b->ncb)))))
         return rc;
     std::vector<int> order(b->ncb);
     std::iota(order.begin(), order.end(), 0);

@@ -435,7 +435,7 @@ struct rg_encoder {
     int ncb_comp = 0, ncb = 0, slot_bytes = 0, colour = 0, style = 0;
     double delta = 1.0 / 64;
     float step[97], inv[97];
-    DevBuf input, planes, q, fcoeff, scratch, desc, res, slots, chan16, syms, cm, poffs, arena, seglens;
+    DevBuf input, planes, q, fcoeff, scratch, desc, res, slots, chan16, syms, cm, poffs, arena, seglens, chains;
     HostBuf staging, map;
     std::vector<rg_cb_desc> h_desc;
     int desc_key[6] = {0, 0, 0, 0, 0, 0};
@@ -456,7 +456,7 @@ extern "C" int rg_encoder_destroy(rg_encoder *e)
     hipSetDevice(e->ctx->device);
     hipStreamSynchronize(e->ctx->stream);
     for (DevBuf *b : {&e->input, &e->planes, &e->q, &e->fcoeff, &e->scratch, &e->desc, &e->res, &e->slots, &e->chan16,
-                      &e->syms, &e->cm, &e->poffs, &e->arena, &e->seglens})
+                      &e->syms, &e->cm, &e->poffs, &e->arena, &e->seglens, &e->chains})
         b->release();
     e->staging.release();
     e->map.release();
@@ -497,13 +497,14 @@ static int run_t1(rg_encoder *e)
         if ((rc = e->syms.ensure((size_t)symcap * e->ncb)) || (rc = e->cm.ensure(sizeof(int4) * e->ncb)) ||
             (rc = e->poffs.ensure(sizeof(int32_t) * maxp * e->ncb)) ||
             (rc = e->seglens.ensure(sizeof(int32_t) * maxp * e->ncb)) ||
-            (rc = e->arena.ensure(rg_t1_seg_bytes(symcap) * e->ncb)))
+            (rc = e->arena.ensure(rg_t1_seg_bytes(symcap) * e->ncb)) ||
+            (rc = e->chains.ensure(rg_t1_chain_ws_bytes(e->ncb))))
             return rc;
         rc = rg_launch_t1_parallel(e->ctx->stream, (const int32_t *)e->q.p, e->w, (size_t)e->w * e->h,
                                    (const rg_cb_desc *)e->desc.p, e->ncb, (uint8_t *)e->syms.p, symcap,
                                    (int4 *)e->cm.p, (int32_t *)e->poffs.p, (uint8_t *)e->arena.p,
                                    (int32_t *)e->seglens.p, (uint8_t *)e->slots.p, e->slot_bytes,
-                                   (rg_cb_result *)e->res.p, e->style);
+                                   (rg_cb_result *)e->res.p, e->style, (int32_t *)e->chains.p);
         if (rc == RG_SUCCESS) e->t1_ran = 1;
         return rc;
     }

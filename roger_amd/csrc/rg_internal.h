@@ -122,16 +122,20 @@ int rg_launch_t1_jobs_split(hipStream_t s, const void *jobs, int njobs);
 // modelling, one MQ (or raw) lane per (block, pass), then concatenation of
 // the pass segments into the slots.
 // Per block: MAXP pass offsets / segment lengths, rg_t1_seg_bytes(symcap)
-// bytes of segment arena.
+// bytes of segment arena.  chain_ws (rg_t1_chain_ws_bytes(blocks) bytes, or
+// null for the pass-row grid): workspace of the sorted chain lists.
 int rg_t1_maxp();
 size_t rg_t1_seg_bytes(int symcap);
+size_t rg_t1_chain_ws_bytes(int nblocks);
 int rg_launch_t1_parallel(hipStream_t s, const int32_t *q, int pitch, size_t plane_stride, const rg_cb_desc *desc,
                           int ncb, uint8_t *syms, int symcap, int4 *cm, int32_t *poffs, uint8_t *arena,
-                          int32_t *seglens, uint8_t *out, int slot_bytes, rg_cb_result *res, int style);
-int rg_launch_t1_jobs_parallel(hipStream_t s, const void *jobs, int njobs, int style);
+                          int32_t *seglens, uint8_t *out, int slot_bytes, rg_cb_result *res, int style,
+                          int32_t *chain_ws);
+int rg_launch_t1_jobs_parallel(hipStream_t s, const void *jobs, int njobs, int style, int32_t *chain_ws);
 // Persistent batched tier-1 (any style): nwaves workgroups pull job indices
 // from *counter (reset by a kernel inside the launch sequence).
-int rg_launch_t1_queue(hipStream_t s, const void *jobs, int njobs, int style, int *counter, int nwaves);
+int rg_launch_t1_queue(hipStream_t s, const void *jobs, int njobs, int style, int *counter, int nwaves,
+                       int32_t *chain_ws);
 
 // Batched colour: per-image parameters; jobs {image, first chunk} cover 256
 // chunks of 16 pixels each.

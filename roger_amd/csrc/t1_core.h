@@ -89,37 +89,69 @@ RG_DEV uint32_t win6(uint64_t m, int r0)
 
 RG_DEV uint32_t b1(uint32_t w, int i) { return (w >> i) & 1u; }
 
+// A lane's (context << 1 | bit) symbols of one stripe -- at most 10 (cleanup:
+// run-length, 2 uniform, sign, then 3 x (ZC, sign)) -- newest in the low
+// byte of a 96-bit shift register, so a push is a constant 8-bit shift.
 struct SymList {
-    uint64_t lo = 0, hi = 0;
+    uint32_t a0 = 0, a1 = 0, a2 = 0;
     int n = 0;
-    RG_DEV void push(int cx, int bit)
+    RG_DEV void push_sym(uint32_t s)
     {
-        uint64_t s = (uint64_t)((cx << 1) | bit);
-        if (n < 8) lo |= s << (8 * n); else hi |= s << (8 * (n - 8));
+        a2 = (a2 << 8) | (a1 >> 24);
+        a1 = (a1 << 8) | (a0 >> 24);
+        a0 = (a0 << 8) | s;
         ++n;
     }
+    RG_DEV void push(int cx, int bit) { push_sym((uint32_t)((cx << 1) | bit)); }
 };
+
+// Context tables of one code-block in workgroup memory (filled by
+// t1_lut_fill for the block's orientation):
+//   zc[256]: ZC context (Table D.1) of a neighbourhood pattern -- bits 0-2
+//            left column rows i-1..i+1, 3-5 right column rows i-1..i+1,
+//            6 the row above, 7 the row below (1 = significant);
+//   sc[256]: SC context << 1 | XOR bit (Table D.3) of the (significant,
+//            negative) pairs of the left (bits 0-1), right (2-3), above (4-5)
+//            and below (6-7) neighbours.
+constexpr int LUT_BYTES = 512;
+
+RG_DEV void t1_lut_fill(uint8_t *lut, int orient)
+{
+    const int lane = wv_lane();
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int x = lane * 4 + e;
+        const int h = ((x >> 1) & 1) + ((x >> 4) & 1);
+        const int v = ((x >> 6) & 1) + ((x >> 7) & 1);
+        const int d = (x & 1) + ((x >> 2) & 1) + ((x >> 3) & 1) + ((x >> 5) & 1);
+        lut[x] = (uint8_t)zc_ctx(orient, h, v, d);
+        int c[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) c[k] = ((x >> (2 * k)) & 1) ? (((x >> (2 * k + 1)) & 1) ? -1 : 1) : 0;
+        int H = c[0] + c[1], V = c[2] + c[3];
+        H = H > 1 ? 1 : (H < -1 ? -1 : H);
+        V = V > 1 ? 1 : (V < -1 ? -1 : V);
+        lut[256 + x] = (uint8_t)sc_ctx(H, V);
+    }
+    wv_sync();
+}
 
 // Sign symbol for window row i with neighbour windows: own column (cur state
 // above, pre state below), left (Lw), right (rw); sign windows negw/neglw/negrw.
 // raw (bypass pass): the bit is the sign itself, without the XOR prediction.
-RG_DEV void push_sign(SymList &sl, int i, uint32_t cur, uint32_t sw, uint32_t Lw, uint32_t rw, uint32_t negw,
-                      uint32_t neglw, uint32_t negrw, bool raw = false)
+RG_DEV void push_sign(SymList &sl, const uint8_t *lut, int i, uint32_t cur, uint32_t sw, uint32_t Lw, uint32_t rw,
+                      uint32_t negw, uint32_t neglw, uint32_t negrw, bool raw = false)
 {
-    int H = (b1(Lw, i) ? (b1(neglw, i) ? -1 : 1) : 0) + (b1(rw, i) ? (b1(negrw, i) ? -1 : 1) : 0);
-    int V = (b1(cur, i - 1) ? (b1(negw, i - 1) ? -1 : 1) : 0) + (b1(sw, i + 1) ? (b1(negw, i + 1) ? -1 : 1) : 0);
-    H = H > 1 ? 1 : (H < -1 ? -1 : H);
-    V = V > 1 ? 1 : (V < -1 ? -1 : V);
-    int sc = sc_ctx(H, V);
-    sl.push(sc >> 1, (int)b1(negw, i) ^ (raw ? 0 : (sc & 1)));
+    const uint32_t x = b1(Lw, i) | (b1(neglw, i) << 1) | (b1(rw, i) << 2) | (b1(negrw, i) << 3) |
+                       (b1(cur, i - 1) << 4) | (b1(negw, i - 1) << 5) | (b1(sw, i + 1) << 6) | (b1(negw, i + 1) << 7);
+    const uint32_t sc = lut[256 + x];
+    sl.push_sym((sc & ~1u) | (b1(negw, i) ^ (raw ? 0u : (sc & 1u))));
 }
 
-RG_DEV int zc_at(int orient, int i, uint32_t cur, uint32_t sw, uint32_t Lw, uint32_t rw)
+// ZC context of window row i (own column: cur above, sw below).
+RG_DEV int zc_at(const uint8_t *lut, int i, uint32_t cur, uint32_t sw, uint32_t Lw, uint32_t rw)
 {
-    int h = (int)(b1(Lw, i) + b1(rw, i));
-    int v = (int)(b1(cur, i - 1) + b1(sw, i + 1));
-    int d = (int)(b1(Lw, i - 1) + b1(Lw, i + 1) + b1(rw, i - 1) + b1(rw, i + 1));
-    return zc_ctx(orient, h, v, d);
+    return lut[((Lw >> (i - 1)) & 7u) | (((rw >> (i - 1)) & 7u) << 3) | (b1(cur, i - 1) << 6) | (b1(sw, i + 1) << 7)];
 }
 
 struct MQEnc {
@@ -194,24 +226,20 @@ RG_DEV void mq_encode(MQEnc &m, int &ctxv, int mqtv, int cx, int d, int lane)
 }
 
 // Codes one stripe's symbols in scan order: columns (lanes) left to right,
-// each lane's own list in order.  Lanes without symbols are skipped via the
-// ballot mask; a lane's list (<= 16 symbols) is read with four readlanes.
+// each lane's own list oldest first.  Lanes without symbols are skipped via
+// the ballot mask; a lane's list is read with three readlanes.
 RG_DEV void mq_stripe(MQEnc &m, int &ctxv, int mqtv, const SymList &sl, int lane)
 {
     uint64_t mask = wv_ballot(sl.n > 0);
-    const int lo0 = (int)(uint32_t)sl.lo, lo1 = (int)(uint32_t)(sl.lo >> 32);
-    const int hi0 = (int)(uint32_t)sl.hi, hi1 = (int)(uint32_t)(sl.hi >> 32);
     while (mask) {
         const int j = __builtin_ctzll(mask);
         mask &= mask - 1;
         const int n = wv_readlane(sl.n, j);
-        uint64_t a = ((uint64_t)(uint32_t)wv_readlane(lo1, j) << 32) | (uint32_t)wv_readlane(lo0, j);
-        uint64_t b = 0;
-        if (n > 8) b = ((uint64_t)(uint32_t)wv_readlane(hi1, j) << 32) | (uint32_t)wv_readlane(hi0, j);
-        for (int t = 0; t < n; ++t) {
-            const int sym = (int)(a & 0xFF);
-            a = (a >> 8) | (b << 56);
-            b >>= 8;
+        const uint32_t x0 = (uint32_t)wv_readlane((int)sl.a0, j), x1 = (uint32_t)wv_readlane((int)sl.a1, j),
+                       x2 = (uint32_t)wv_readlane((int)sl.a2, j);
+        for (int t = n - 1; t >= 0; --t) {     // byte t holds the symbol pushed (n - 1 - t)-th
+            const uint32_t w = t < 4 ? x0 : (t < 8 ? x1 : x2);
+            const int sym = (int)((w >> ((t & 3) * 8)) & 0xFFu);
             mq_encode(m, ctxv, mqtv, sym >> 1, sym & 1, lane);
         }
     }
@@ -250,43 +278,45 @@ struct T1Job {
 RG_DEV int seg_at(int s0, int p) { return ((2 * s0 + 15) & ~15) + 32 * p; }
 inline int seg_bytes(int symcap) { return 2 * symcap + 32 * MAXP + 32; }
 
-// Exclusive prefix sum over the wavefront of a per-lane count n (0..31) and
-// the wave total, from five ballots of n's bits (mbcnt counts the set bits
-// of a ballot below the lane): no cross-lane data movement through LDS.
+// Exclusive prefix sum over the wavefront of a per-lane count n and the wave
+// total (a DPP scan; lane 63's inclusive sum is the total).
 RG_DEV int wave_excl_count(int n, int &total)
 {
-    int pre = 0, tot = 0;
-#pragma unroll
-    for (int b = 0; b < 5; ++b) {
-        const uint64_t m = wv_ballot((n >> b) & 1);
-        pre += wv_mbcnt(m) << b;
-        tot += __builtin_popcountll(m) << b;
-    }
-    total = tot;
-    return pre;
+    const int incl = wv_incl_scan(n);
+    total = wv_readlane(incl, 63);
+    return incl - n;
 }
 
-// Split path: appends one stripe's symbols (scan order) to the block's stream.
+// Split path: appends one stripe's symbols (scan order) to the block's
+// stream.  The register holds them newest first, so they are stored from the
+// end of the lane's range backwards.
 RG_DEV void emit_stripe(const SymList &sl, const T1Job &job, int &pos, int &ovf, int lane)
 {
     int total;
     const int off = pos + wave_excl_count(sl.n, total);
     if (total == 0) return;
     if (pos + total > job.symcap) { ovf = 1; return; }
-    uint64_t a = sl.lo, b = sl.hi;
-    for (int t = 0; t < 16; ++t) {
+    // every lane's range must lie inside the stripe's [pos, pos + total)
+    const bool inside = off >= pos && off + sl.n <= pos + total;
+    if (wv_any(!inside)) { ovf = 1; return; }
+    uint8_t *last = job.sym + off + sl.n - 1;
+    uint32_t a0 = sl.a0, a1 = sl.a1, a2 = sl.a2;
+#pragma unroll
+    for (int t = 0; t < 10; ++t) {
         if (!wv_any(t < sl.n)) break;
-        if (t < sl.n) job.sym[off + t] = (uint8_t)(a & 0xFF);
-        a = (a >> 8) | (b << 56);
-        b >>= 8;
+        if (t < sl.n) last[-t] = (uint8_t)a0;
+        a0 = (a0 >> 8) | (a1 << 24);
+        a1 = (a1 >> 8) | (a2 << 24);
+        a2 >>= 8;
     }
     pos += total;
 }
 
 // Encodes one code-block with the calling wavefront (64 lanes).  SPLIT:
 // context modelling only, the symbols go to job.sym for the lane MQ coder.
+// lut: LUT_BYTES of workgroup memory for the block's context tables.
 template <bool SPLIT>
-RG_DEV void t1_block(const T1Job &job)
+RG_DEV void t1_block(const T1Job &job, uint8_t *lut)
 {
     const int lane = wv_lane();
     const int w = job.w, h = job.h, orient = job.orient, pitch = job.pitch;
@@ -318,6 +348,7 @@ RG_DEV void t1_block(const T1Job &job)
         else *job.res = rg_cb_result{0, 0, 0, 0};
         return;
     }
+    t1_lut_fill(lut, orient);
     int spos = 0, sovf = 0;   // split path: stream position / overflow (uniform)
     int pidx = 0;             // split path: coding pass index
 
@@ -363,13 +394,21 @@ RG_DEV void t1_block(const T1Job &job)
                     // ---- significance propagation ----
                     // (a raw pass under BYPASS sends signs unpredicted; this pass is pidx - 1)
                     const bool rawp = SPLIT && pass_is_raw(job.style, pidx - 1);
-                    // no significant sample in any column's 6-row window: nobody joins the pass
-                    if (!wv_any((sw & 0x3Fu) != 0)) continue;
-                    uint32_t N = 0, P = 0, Lw = lw;
-                    bool again = true;
-                    // monotone fix-point: converges within 65 rounds (chain across <= 64 lanes)
-                    for (int round = 0; again && round < 80; ++round) {
-                        const uint32_t NL = wv_from_left(N);
+                    // Candidates: insignificant samples.  hn: rows with a neighbour that
+                    // is significant before the pass (own column above and below, rows
+                    // i-1..i+1 of the left and right columns).  Joining the pass needs a
+                    // neighbour significant before it or newly significant in it, and the
+                    // first newly significant sample needs the former, so if no candidate
+                    // of any column has one the stripe has no pass symbols.
+                    const uint32_t cand0 = vrow & ~(sw >> 1) & 0xFu;
+                    const uint32_t hn = (sw | (sw >> 2) | lw | (lw >> 1) | (lw >> 2) | rw | (rw >> 1) | (rw >> 2)) & 0xFu;
+                    if (!wv_any((hn & cand0) != 0)) continue;
+                    uint32_t N = 0, P = 0, NL = 0, Lw = lw;
+                    // monotone fix-point over the left-column dependency: converges within
+                    // 65 rounds (a chain across <= 64 lanes).  Another round runs only when
+                    // the left column's newly significant samples border a row of this
+                    // column that is not yet in the pass (the only way membership grows).
+                    for (int round = 0; round < 80; ++round) {
                         Lw = lw | (NL << 1);
                         uint32_t cur = sw, Nn = 0;
                         P = 0;
@@ -383,18 +422,22 @@ RG_DEV void t1_block(const T1Job &job)
                                 if (b1(bw, i)) { Nn |= 1u << k; cur |= 1u << i; }
                             }
                         }
-                        again = wv_any(Nn != N);
                         N = Nn;
+                        const uint32_t NLn = wv_from_left(N);
+                        const uint32_t grown = NLn & ~NL;
+                        NL = NLn;
+                        if (!wv_any(((grown | (grown << 1) | (grown >> 1)) & cand0 & ~P) != 0)) break;
                     }
+                    Lw = lw | (NL << 1);   // the left column's final state for the contexts
                     uint32_t cur = sw;
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
                         const int i = k + 1;
                         if (!((P >> k) & 1u)) continue;
                         const int bit = (int)b1(bw, i);
-                        sl.push(zc_at(orient, i, cur, sw, Lw, rw), bit);
+                        sl.push(zc_at(lut, i, cur, sw, Lw, rw), bit);
                         if (bit) {
-                            push_sign(sl, i, cur, sw, Lw, rw, negw, neglw, negrw, rawp);
+                            push_sign(sl, lut, i, cur, sw, Lw, rw, negw, neglw, negrw, rawp);
                             cur |= 1u << i;
                         }
                     }
@@ -423,6 +466,9 @@ RG_DEV void t1_block(const T1Job &job)
                     // ---- cleanup ----
                     const uint32_t vw = (uint32_t)(V >> r0) & 0xFu;
                     const uint32_t Cm = ~(sw >> 1) & ~vw & vrow & 0xFu;
+                    // every sample of the stripe significant or visited: no cleanup
+                    // symbols (most stripes of the low bit-planes)
+                    if (!wv_any(Cm != 0)) continue;
                     const uint32_t N = Cm & (bw >> 1);
                     const uint32_t NL = wv_from_left(N);
                     const uint32_t Lw = lw | (NL << 1);
@@ -440,7 +486,7 @@ RG_DEV void t1_block(const T1Job &job)
                             sl.push(CTX_RL, 1);
                             sl.push(CTX_UNI, k >> 1);
                             sl.push(CTX_UNI, k & 1);
-                            push_sign(sl, k + 1, cur, sw, Lw, rw, negw, neglw, negrw);
+                            push_sign(sl, lut, k + 1, cur, sw, Lw, rw, negw, neglw, negrw);
                             cur |= 1u << (k + 1);
                             kstart = k + 1;
                         }
@@ -451,9 +497,9 @@ RG_DEV void t1_block(const T1Job &job)
                             const int i = k + 1;
                             if (k < kstart || !((Cm >> k) & 1u)) continue;
                             const int bit = (int)((N >> k) & 1u);
-                            sl.push(zc_at(orient, i, cur, sw, Lw, rw), bit);
+                            sl.push(zc_at(lut, i, cur, sw, Lw, rw), bit);
                             if (bit) {
-                                push_sign(sl, i, cur, sw, Lw, rw, negw, neglw, negrw);
+                                push_sign(sl, lut, i, cur, sw, Lw, rw, negw, neglw, negrw);
                                 cur |= 1u << i;
                             }
                         }
@@ -488,14 +534,14 @@ RG_DEV void t1_block(const T1Job &job)
     }
 }
 
-// ---- split path, second half: one MQ coder per lane ----
+// ---- split path, second half: one coder per lane ----
 //
-// Memory traffic is kept off the coder's dependency chain: symbols arrive 16
-// at a time from an unconditional (index-clamped) 16-byte load issued one
-// chunk ahead, and codeword bytes collect in four registers that leave as one
-// 16-byte store per 16 bytes.  (A byte store per output byte would make every
-// later wait on the symbol load also wait for that store.)  Output slots must
-// be 16-byte aligned with a capacity that is a multiple of 16.
+// Symbols arrive 16 at a time from an unconditional (index-clamped) 16-byte
+// load issued one chunk ahead, so the memory latency stays off a chain's
+// dependency path.  The raw (bypass) coder collects its output bytes in four
+// registers that leave as one 16-byte store per 16 bytes (a byte store per
+// output byte would make every later wait on the symbol load also wait for
+// that store); output regions are 16-byte aligned.
 struct LaneMQ {
     uint32_t A, C, B;
     int CT, bp, ovf;
@@ -522,35 +568,81 @@ RG_DEV void lane_put(LaneMQ &m, uint32_t v, int pos, uint8_t *out, int cap)
     }
 }
 
-RG_DEV void lane_byteout(LaneMQ &m, uint8_t *out, int cap)
+// ---- lane MQ coder, pipelined form ----
+//
+// One MQ chain per lane.  A chain's serial dependency runs through each
+// symbol's probability-table read (workgroup memory) and the interval
+// arithmetic; the renormalisation shifts, BYTEOUT and output can overlap the
+// next symbol's table read.  So a step updates the interval and the context
+// state (ring_code), then reads the NEXT symbol's table entry -- its context
+// state is final at that point -- and only then does the current symbol's
+// renormalisation and BYTEOUT (ring_renorm).  Output bytes go to a per-lane
+// 64-byte ring in workgroup memory with one unconditional store per BYTEOUT
+// (a step without BYTEOUT rewrites the pending byte's own slot with its
+// current value), so a step has no store branch; completed 16-byte groups
+// leave for global memory once per 16-symbol chunk (ring_flush).  A chunk
+// adds at most 48 bytes (three BYTEOUTs per step at most) to fewer than 16
+// unflushed ones, so the ring never overwrites an unflushed byte.  Chunks
+// wholly inside a lane's symbol range run without per-symbol range tests.
+constexpr int RING_STRIDE = 68;          // bytes per lane: 17 dwords, so lanes start in distinct LDS banks
+constexpr int RING_BYTES = 64 * RING_STRIDE;
+
+struct RingMQ {
+    uint32_t A, C, B;
+    int CT, bp;    // bp: position of the pending byte B (-1: the byte before the codeword)
+    int fl;        // bytes [0, fl) are in global memory
+    int ovf;
+};
+
+// The 19 context states (8 bits each: I << 1 | MPS), byte cx of the 20-byte
+// value c4:c3:c2:c1:c0.  A read picks the byte out of each register pair with
+// one byte permute and chooses among the three; a write inserts the byte
+// under a mask that is zero for the four other registers.  (A compare/select
+// chain over the registers is turned by the compiler into an indexed access
+// to scratch memory.)
+struct CtxStates {
+    uint32_t c0, c1, c2, c3, c4;
+    // initial states (Table D.7): ctx 0 -> 4, RL (17) -> 3, UNIFORM (18) -> 46
+    RG_DEV void init() { c0 = 4u << 1; c1 = c2 = c3 = 0; c4 = (3u << 1) << 8 | (46u << 1) << 16; }
+    RG_DEV uint32_t get(uint32_t cx) const
+    {
+        const uint32_t sel = (cx & 7u) | 0x0C0C0C00u;   // byte cx & 7, upper bytes zero
+        const uint32_t a = wv_perm(c1, c0, sel), b = wv_perm(c3, c2, sel), c = wv_perm(c4, c4, sel);
+        return cx < 8 ? a : (cx < 16 ? b : c);
+    }
+    RG_DEV void set(uint32_t cx, uint32_t st)
+    {
+        const uint32_t sh = (cx & 3u) * 8u, m = 0xFFu << sh, v = st << sh, w = cx >> 2;
+        c0 = (c0 & ~(w == 0 ? m : 0u)) | (w == 0 ? v : 0u);
+        c1 = (c1 & ~(w == 1 ? m : 0u)) | (w == 1 ? v : 0u);
+        c2 = (c2 & ~(w == 2 ? m : 0u)) | (w == 2 ? v : 0u);
+        c3 = (c3 & ~(w == 3 ? m : 0u)) | (w == 3 ? v : 0u);
+        c4 = (c4 & ~(w == 4 ? m : 0u)) | (w == 4 ? v : 0u);
+    }
+};
+
+// BYTEOUT (C.2.7) when `need`, as predicated code: a carry into B, then 8 or
+// (after 0xFF, bit stuffing) 7 bits move out of C.  B's slot is written in
+// either case.
+RG_DEV void ring_byteout(RingMQ &m, bool need, uint8_t *rb)
 {
-    // BYTEOUT (C.2.7) without branches on the byte value: a carry into B,
-    // then either 8 or 7 (after 0xFF, bit stuffing) bits move out of C.
-    const bool carry = m.B != 0xFF && m.C >= 0x8000000u;
+    const bool carry = need && m.B != 0xFF && m.C >= 0x8000000u;
     const uint32_t b2 = m.B + (carry ? 1u : 0u);
-    if (carry) m.C &= 0x7FFFFFF;
+    const uint32_t c = carry ? (m.C & 0x7FFFFFFu) : m.C;
     const bool big = b2 == 0xFF;
-    if (m.bp >= 0) lane_put(m, b2, m.bp, out, cap);
-    m.bp++;
-    m.B = big ? (m.C >> 20) : (m.C >> 19);
-    m.C &= big ? 0xFFFFFu : 0x7FFFFu;
-    m.CT = big ? 7 : 8;
+    rb[m.bp & 63] = (uint8_t)b2;   // bp = -1 (no byte yet) lands in slot 63, rewritten before use
+    m.bp += need ? 1 : 0;
+    m.B = need ? (big ? (c >> 20) : (c >> 19)) : m.B;
+    m.C = need ? (c & (big ? 0xFFFFFu : 0x7FFFFu)) : m.C;
+    m.CT = need ? (big ? 7 : 8) : m.CT;
 }
 
-// One ENCODE step (C.2.2-C.2.6) for this lane's block, branch-free except for
-// the renormalisation loop.  The 19 context states are packed 4 per register
-// (8 bits: I << 1 | MPS); the probability table is read from `tab` (LDS).
-RG_DEV void lane_encode(LaneMQ &m, uint32_t (&cs)[5], const uint32_t *tab, uint32_t sym, uint8_t *out, int cap)
+// ENCODE's interval and state update (C.2.2-C.2.5) of one (context << 1 | bit)
+// symbol whose context is in state st with table entry t; A is renormalised
+// (shifted) here, and the returned shift count is C's share for ring_renorm.
+RG_DEV int ring_code(RingMQ &m, CtxStates &cs, uint32_t sym, uint32_t st, uint32_t t)
 {
-    const uint32_t cx = sym >> 1, d = sym & 1u;
-    const uint32_t wsel = cx >> 2, sh = (cx & 3u) * 8u;
-    uint32_t cw = cs[0];
-    cw = wsel == 1 ? cs[1] : cw;
-    cw = wsel == 2 ? cs[2] : cw;
-    cw = wsel == 3 ? cs[3] : cw;
-    cw = wsel == 4 ? cs[4] : cw;
-    const uint32_t st = (cw >> sh) & 0xFFu, mps = st & 1u;
-    const uint32_t t = tab[st >> 1];
+    const uint32_t cx = sym >> 1, d = sym & 1u, mps = st & 1u;
     const uint32_t qe = t & 0xFFFFu;
     const uint32_t a = m.A - qe;
     const bool lps = d != mps;
@@ -561,27 +653,130 @@ RG_DEV void lane_encode(LaneMQ &m, uint32_t (&cs)[5], const uint32_t *tab, uint3
     const bool upper = !renorm || (lps ? small : !small);
     m.C += upper ? qe : 0u;
     m.A = upper ? a : qe;
-    const uint32_t nst = lps ? ((((t >> 22) & 63u) << 1) | (mps ^ (t >> 28))) : ((((t >> 16) & 63u) << 1) | mps);
-    cw = (cw & ~(0xFFu << sh)) | ((renorm ? nst : st) << sh);
-    cs[0] = wsel == 0 ? cw : cs[0];
-    cs[1] = wsel == 1 ? cw : cs[1];
-    cs[2] = wsel == 2 ? cw : cs[2];
-    cs[3] = wsel == 3 ? cw : cs[3];
-    cs[4] = wsel == 4 ? cw : cs[4];
-    int n = wv_clz(m.A) - 16;
-    while (n > 0) {
-        const int sft = n < m.CT ? n : m.CT;
-        m.A <<= sft; m.C <<= sft; m.CT -= sft; n -= sft;
-        if (m.CT == 0) lane_byteout(m, out, cap);
+    // next state: NLPS (bits 22..27) with MPS ^ SWITCH after an LPS, NMPS
+    // (bits 16..21) after an MPS
+    const uint32_t nst = (((t >> (lps ? 22u : 16u)) & 63u) << 1) | (mps ^ (lps ? (t >> 28) : 0u));
+    cs.set(cx, renorm ? nst : st);
+    const int n = wv_clz(m.A) - 16;
+    m.A <<= n;
+    return n;
+}
+
+// RENORME's C side (C.2.6): n shifts with BYTEOUT whenever CT reaches 0.  C
+// takes min(n, CT), the predicated BYTEOUT, then the rest -- fewer than the 7
+// or 8 bits of the new CT, except after an LPS in a very skewed state (n up
+// to 15), which the loop finishes (entered only when a lane needs it).
+RG_DEV void ring_renorm(RingMQ &m, int n, uint8_t *rb)
+{
+    const int sft = n < m.CT ? n : m.CT;
+    m.C <<= sft; m.CT -= sft; n -= sft;
+    ring_byteout(m, m.CT == 0, rb);
+    while (n > 0 && n >= m.CT) {
+        m.C <<= m.CT; n -= m.CT; m.CT = 0;
+        ring_byteout(m, true, rb);
+    }
+    m.C <<= n; m.CT -= n;
+}
+
+// Moves the ring's completed 16-byte groups [fl, bp) to out (bytes at or
+// beyond cap only set the overflow flag).
+RG_DEV void ring_flush(RingMQ &m, const uint8_t *rb, uint8_t *out, int cap)
+{
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const bool g = m.bp - m.fl >= 16;
+        if (!wv_any(g)) break;
+        if (g) {
+            const uint32_t *w = reinterpret_cast<const uint32_t *>(rb + (m.fl & 63));
+            if (m.fl + 16 <= cap) wv_store16(out + m.fl, w[0], w[1], w[2], w[3]);
+            else m.ovf = 1;
+            m.fl += 16;
+        }
     }
 }
 
+// Codes symbols [s0, s1) of the 16-byte-aligned stream `sym` as one MQ
+// codeword (INITENC, ENCODE per symbol, FLUSH C.2.9) into `out` (16-byte
+// aligned, capacity cap), from the context states cs.  Every lane of the
+// wavefront calls it; idle lanes pass active = false.  Returns the codeword
+// length; ovf is set if it did not fit (out then holds its first complete
+// 16-byte groups).  `tab`: the probability table in workgroup memory; `ring`:
+// RING_BYTES of workgroup memory.
+RG_DEV int ring_chain(const uint32_t *tab, uint8_t *ring, bool active, const uint8_t *sym, int s0, int s1,
+                      uint8_t *out, int cap, CtxStates &cs, bool &ovf)
+{
+    uint8_t *rb = ring + wv_lane() * RING_STRIDE;
+    const int c0 = active ? s0 >> 4 : 0, c1 = active ? (s1 + 15) >> 4 : 0;
+    const int nch = c1 - c0;
+    int nmax = nch;
+    for (int o = 32; o >= 1; o >>= 1) {
+        const int t = wv_shfl_xor(nmax, o);
+        nmax = nmax > t ? nmax : t;
+    }
+    RingMQ m;
+    m.A = 0x8000; m.C = 0; m.B = 0; m.CT = 12; m.bp = -1; m.fl = 0; m.ovf = 0;
+    const int last = nch > 0 ? c1 - 1 : c0;
+    u32x4 nxt = wv_load16(sym + 16 * c0);
+    for (int i = 0; i < nmax; ++i) {
+        const u32x4 cur = nxt;
+        const int cn = c0 + i + 1;
+        nxt = wv_load16(sym + 16 * (cn < last ? cn : last));
+        const uint32_t w4[4] = {cur.x, cur.y, cur.z, cur.w};
+        const int base = 16 * (c0 + i);
+        const bool full = i < nch && base >= s0 && base + 16 <= s1;
+        if (full) {
+            uint32_t sy = w4[0] & 0xFFu, st = cs.get(sy >> 1), t = tab[st >> 1];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int n = ring_code(m, cs, sy, st, t);
+                if (k < 15) {   // the next symbol's table read before this one's byte work
+                    sy = (w4[(k + 1) >> 2] >> (8 * ((k + 1) & 3))) & 0xFFu;
+                    st = cs.get(sy >> 1);
+                    t = tab[st >> 1];
+                    wv_sched_fence();
+                }
+                ring_renorm(m, n, rb);
+            }
+        } else if (i < nch) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k)
+                if (base + k >= s0 && base + k < s1) {
+                    const uint32_t sy = (w4[k >> 2] >> (8 * (k & 3))) & 0xFFu;
+                    const uint32_t st = cs.get(sy >> 1);
+                    ring_renorm(m, ring_code(m, cs, sy, st, tab[st >> 1]), rb);
+                }
+        }
+        ring_flush(m, rb, out, cap);
+    }
+    // FLUSH (C.2.9); idle lanes run it on their untouched coder, unused
+    const uint32_t tmp = m.C + m.A;
+    m.C |= 0xFFFF;
+    if (m.C >= tmp) m.C -= 0x8000;
+    m.C <<= m.CT; ring_byteout(m, true, rb);
+    m.C <<= m.CT; ring_byteout(m, true, rb);
+    int len = m.bp;
+    if (m.B != 0xFF) {
+        rb[m.bp & 63] = (uint8_t)m.B;
+        len = m.bp + 1;
+    }
+    m.bp = len;
+    ring_flush(m, rb, out, cap);
+    if (active && (len & 15) != 0 && (len & ~15) < cap) {
+        const uint32_t *w = reinterpret_cast<const uint32_t *>(rb + ((len & ~15) & 63));
+        wv_store16(out + (len & ~15), w[0], w[1], w[2], w[3]);
+    }
+    ovf = m.ovf || len > cap;
+    return len;
+}
+
+
 // MQ-codes one block per lane from the streams the context-modelling pass
 // wrote.  Block `i` (i >= njobs: idle lane) is either jobs[i] (QUEUE) or
-// entry i of the flat arrays.  `tab` is 48 words of workgroup-shared memory.
+// entry i of the flat arrays.  `tab` is 48 words of workgroup-shared memory,
+// `ring` RING_BYTES of it.
 template <bool QUEUE>
-RG_DEV void t1_mq_lanes(uint32_t *tab, const T1Job *jobs, int i, int njobs, const uint8_t *syms, int symcap,
-                        const i32x4 *cm, uint8_t *slots, int slot, rg_cb_result *res)
+RG_DEV void t1_mq_lanes(uint32_t *tab, uint8_t *ring, const T1Job *jobs, int i, int njobs, const uint8_t *syms,
+                        int symcap, const i32x4 *cm, uint8_t *slots, int slot, rg_cb_result *res)
 {
     const int lane = wv_lane();
     if (lane < 47) tab[lane] = MQT[lane];
@@ -601,49 +796,17 @@ RG_DEV void t1_mq_lanes(uint32_t *tab, const T1Job *jobs, int i, int njobs, cons
             c = cm[i]; src = syms + (size_t)i * symcap; out = slots + (size_t)i * slot; cap = slot; dst = res + i;
         }
     }
-    const int n = c.x;
-    int nmax = n;
-    for (int o = 32; o >= 1; o >>= 1) {
-        const int t = wv_shfl_xor(nmax, o);
-        nmax = nmax > t ? nmax : t;
-    }
-    LaneMQ m;
-    m.A = 0x8000; m.C = 0; m.B = 0; m.CT = 12; m.bp = -1; m.ovf = 0;
-    m.o0 = m.o1 = m.o2 = m.o3 = 0;
-    // initial states (Table D.7): ctx 0 -> 4, RL (17) -> 3, UNIFORM (18) -> 46
-    uint32_t cs[5] = {4u << 1, 0, 0, 0, (3u << 1) << 8 | (46u << 1) << 16};
-    const int last = n > 0 ? (n - 1) >> 4 : 0;   // last chunk index with symbols
-    u32x4 nxt = wv_load16(src);
-    for (int base = 0; base < nmax; base += 16) {
-        const u32x4 cur = nxt;
-        const int k1 = (base >> 4) + 1;
-        nxt = wv_load16(src + 16 * (k1 < last ? k1 : last));
-        const uint32_t w4[4] = {cur.x, cur.y, cur.z, cur.w};
-#pragma unroll
-        for (int k = 0; k < 16; ++k)
-            if (base + k < n) lane_encode(m, cs, tab, (w4[k >> 2] >> (8 * (k & 3))) & 0xFFu, out, cap);
-    }
+    CtxStates cs;
+    cs.init();
+    const bool coded = active && c.z > 0;   // a block without bit-planes has no codeword
+    bool ovf;
+    const int len = ring_chain(tab, ring, coded, src, 0, c.x, out, cap, cs, ovf);
     if (!active) return;
-    if (c.z == 0) {
+    if (!coded) {
         *dst = rg_cb_result{0, 0, 0, 0};
         return;
     }
-    // FLUSH (C.2.9)
-    const uint32_t tmp = m.C + m.A;
-    m.C |= 0xFFFF;
-    if (m.C >= tmp) m.C -= 0x8000;
-    m.C <<= m.CT; lane_byteout(m, out, cap);
-    m.C <<= m.CT; lane_byteout(m, out, cap);
-    int len;
-    if (m.B != 0xFF) {
-        lane_put(m, m.B, m.bp, out, cap);
-        len = m.bp + 1;
-    } else {
-        len = m.bp;
-    }
-    if ((len & 15) != 0 && (len & ~15) < cap) wv_store16(out + (len & ~15), m.o0, m.o1, m.o2, m.o3);
-    if (len > cap) m.ovf = 1;
-    *dst = rg_cb_result{len, c.y, c.z, (m.ovf || c.w) ? 1 : 0};
+    *dst = rg_cb_result{len, c.y, c.z, (ovf || c.w) ? 1 : 0};
 }
 
 // ---- parallel style (RESET | TERMALL): one MQ chain per (code-block, pass) ----
@@ -654,51 +817,17 @@ RG_DEV void t1_mq_lanes(uint32_t *tab, const T1Job *jobs, int i, int njobs, cons
 // [poff[p], poff[p+1]) of its stream into the block's segment arena, and
 // records the segment length; t1_pack_block then concatenates the segments.
 // `len_out` < 0 marks a segment that did not fit its arena region.
-RG_DEV void t1_mq_pass_lane(uint32_t *tab, bool active, const uint8_t *sym, int s0, int s1, uint8_t *seg, int cap,
-                            int32_t *len_out)
+RG_DEV void t1_mq_pass_lane(uint32_t *tab, uint8_t *ring, bool active, const uint8_t *sym, int s0, int s1,
+                            uint8_t *seg, int cap, int32_t *len_out)
 {
     const int lane = wv_lane();
     if (lane < 47) tab[lane] = MQT[lane];
     wv_sync();
-    const int c0 = active ? s0 >> 4 : 0, c1 = active ? (s1 + 15) >> 4 : 0;
-    int nch = c1 - c0, nmax = nch;
-    for (int o = 32; o >= 1; o >>= 1) {
-        const int t = wv_shfl_xor(nmax, o);
-        nmax = nmax > t ? nmax : t;
-    }
-    LaneMQ m;
-    m.A = 0x8000; m.C = 0; m.B = 0; m.CT = 12; m.bp = -1; m.ovf = 0;
-    m.o0 = m.o1 = m.o2 = m.o3 = 0;
-    uint32_t cs[5] = {4u << 1, 0, 0, 0, (3u << 1) << 8 | (46u << 1) << 16};
-    const int last = nch > 0 ? c1 - 1 : c0;
-    u32x4 nxt = wv_load16(sym + 16 * c0);
-    for (int i = 0; i < nmax; ++i) {
-        const u32x4 cur = nxt;
-        const int cn = c0 + i + 1;
-        nxt = wv_load16(sym + 16 * (cn < last ? cn : last));
-        const uint32_t w4[4] = {cur.x, cur.y, cur.z, cur.w};
-        const int base = 16 * (c0 + i);
-#pragma unroll
-        for (int k = 0; k < 16; ++k)
-            if (i < nch && base + k >= s0 && base + k < s1)
-                lane_encode(m, cs, tab, (w4[k >> 2] >> (8 * (k & 3))) & 0xFFu, seg, cap);
-    }
-    if (!active) return;
-    // FLUSH (C.2.9)
-    const uint32_t tmp = m.C + m.A;
-    m.C |= 0xFFFF;
-    if (m.C >= tmp) m.C -= 0x8000;
-    m.C <<= m.CT; lane_byteout(m, seg, cap);
-    m.C <<= m.CT; lane_byteout(m, seg, cap);
-    int len;
-    if (m.B != 0xFF) {
-        lane_put(m, m.B, m.bp, seg, cap);
-        len = m.bp + 1;
-    } else {
-        len = m.bp;
-    }
-    if ((len & 15) != 0 && (len & ~15) < cap) wv_store16(seg + (len & ~15), m.o0, m.o1, m.o2, m.o3);
-    *len_out = (m.ovf || len > cap) ? -1 : len;
+    CtxStates cs;
+    cs.init();
+    bool ovf;
+    const int len = ring_chain(tab, ring, active, sym, s0, s1, seg, cap, cs, ovf);
+    if (active) *len_out = ovf ? -1 : len;
 }
 
 // Raw (bypass) pass of the parallel style with BYPASS (T.800 D.6): the
@@ -802,6 +931,55 @@ RG_DEV void t1_pack_block(const i32x4 &c, const int32_t *poff, const uint8_t *se
     }
     if (pos > cap) ovf = 1;
     if (lane == 0) *res = rg_cb_result{c.z == 0 ? 0 : pos, c.y, c.z, ovf ? 1 : 0};
+}
+
+// ---- parallel styles, sorted form: chains ordered longest first ----
+//
+// A (block, pass) chain's coding time is its symbol count, and a wavefront
+// takes as long as its longest lane, so lanes are grouped by length: chains
+// are bucketed by length (32-symbol buckets, longest first) into one list of
+// MQ chains followed by one list of raw chains (a device counting sort), and
+// wavefront w codes list entries [64w, 64w + 64).  The first wavefronts then
+// carry the longest chains and start first.  Which lane codes a chain does
+// not change its bytes.  Workspace (int32): CH_HIST histogram words (MQ
+// buckets, then raw buckets), as many cursors, the two list lengths, then
+// the list of chain ids (block * MAXP + pass).
+constexpr int CH_BUCKETS = 256, CH_HIST = 2 * CH_BUCKETS, CH_COUNTS = 2 * CH_HIST, CH_LIST = CH_COUNTS + 16;
+
+// Bucket of a chain: 0 = longest.
+RG_DEV int chain_bucket(int len)
+{
+    const int k = len >> 5;
+    return CH_BUCKETS - 1 - (k < CH_BUCKETS - 1 ? k : CH_BUCKETS - 1);
+}
+
+// Histogram key of chain (block c/poff, pass q), or -1 if the block has no
+// pass q.  Raw chains (BYPASS) take the second half of the keys.
+RG_DEV int chain_key(const i32x4 &c, const int32_t *poff, int q, int style)
+{
+    int p, s0, s1, at, cap;
+    if (!pass_range(c, poff, q, p, s0, s1, at, cap)) return -1;
+    return (pass_is_raw(style, q) ? CH_BUCKETS : 0) + chain_bucket(s1 - s0);
+}
+
+// Chain coder of the sorted form for list entry j of a wavefront whose
+// entries are [base, end) (end <= base: idle lane); `raw` is uniform.
+// J(b) returns block b's job record.
+template <class J>
+RG_DEV void t1_chain_lane(uint32_t *tab, uint8_t *ring, bool raw, const int32_t *list, int base, int end,
+                          const J &job)
+{
+    const int j = base + wv_lane();
+    bool active = j < end;
+    const int id = list[active ? j : base];
+    const T1Job &jb = job(id / MAXP);
+    const int q = id % MAXP;
+    int p = 0, s0 = 0, s1 = 0, at = 0, cap = 0;
+    if (active) active = pass_range(*jb.cm, jb.poff, q, p, s0, s1, at, cap);
+    if (raw)
+        t1_raw_pass_lane(active, jb.sym, s0, s1, jb.seg + at, cap, jb.seglen + p);
+    else
+        t1_mq_pass_lane(tab, ring, active, jb.sym, s0, s1, jb.seg + at, cap, jb.seglen + p);
 }
 
 }  // namespace rg_t1

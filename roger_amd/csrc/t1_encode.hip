@@ -31,7 +31,8 @@ __global__ void __launch_bounds__(64) t1_encode_kernel(const int32_t *__restrict
     job.out = out + (size_t)blockIdx.x * slot_bytes;
     job.res = res + blockIdx.x;
     job.style = 0;
-    t1_block<false>(job);
+    __shared__ uint8_t lut[LUT_BYTES];
+    t1_block<false>(job, lut);
 }
 
 // Batched path: one workgroup per entry of a job list sorted largest block
@@ -45,7 +46,8 @@ __global__ void __launch_bounds__(64) t1_encode_kernel(const int32_t *__restrict
 // never leave the loop.)
 __global__ void __launch_bounds__(64) t1_encode_jobs_kernel(const T1Job *__restrict__ jobs)
 {
-    t1_block<false>(jobs[blockIdx.x]);
+    __shared__ uint8_t lut[LUT_BYTES];
+    t1_block<false>(jobs[blockIdx.x], lut);
 }
 
 // Persistent form of the batched tier-1: a grid sized to the machine pulls job
@@ -58,11 +60,12 @@ __global__ void __launch_bounds__(64) t1_encode_jobs_kernel(const T1Job *__restr
 template <bool SPLIT>
 __device__ __forceinline__ void t1_queue_loop(const T1Job *__restrict__ jobs, int njobs, int *__restrict__ counter)
 {
+    __shared__ uint8_t lut[LUT_BYTES];
     for (;;) {
         const int got = atomicAdd(counter, threadIdx.x == 0 ? 1 : 0);
         const int i = __builtin_amdgcn_readlane(got, 0);
         if (i >= njobs) break;
-        t1_block<SPLIT>(jobs[i]);
+        t1_block<SPLIT>(jobs[i], lut);
     }
 }
 
@@ -101,12 +104,14 @@ __global__ void __launch_bounds__(64) t1_cm_kernel(const int32_t *__restrict__ q
     job.cm = cm + blockIdx.x;
     job.poff = poffs ? poffs + (size_t)blockIdx.x * MAXP : nullptr;
     job.style = style;
-    t1_block<true>(job);
+    __shared__ uint8_t lut[LUT_BYTES];
+    t1_block<true>(job, lut);
 }
 
 __global__ void __launch_bounds__(64) t1_cm_jobs_kernel(const T1Job *__restrict__ jobs)
 {
-    t1_block<true>(jobs[blockIdx.x]);
+    __shared__ uint8_t lut[LUT_BYTES];
+    t1_block<true>(jobs[blockIdx.x], lut);
 }
 
 __global__ void __launch_bounds__(64) t1_mq_kernel(const uint8_t *__restrict__ syms, int symcap,
@@ -114,13 +119,16 @@ __global__ void __launch_bounds__(64) t1_mq_kernel(const uint8_t *__restrict__ s
                                                    int slot, rg_cb_result *__restrict__ res)
 {
     __shared__ uint32_t tab[48];
-    t1_mq_lanes<false>(tab, nullptr, blockIdx.x * 64 + threadIdx.x, ncb, syms, symcap, cm, slots, slot, res);
+    __shared__ uint8_t ring[RING_BYTES];
+    t1_mq_lanes<false>(tab, ring, nullptr, blockIdx.x * 64 + threadIdx.x, ncb, syms, symcap, cm, slots, slot, res);
 }
 
 __global__ void __launch_bounds__(64) t1_mq_jobs_kernel(const T1Job *__restrict__ jobs, int njobs)
 {
     __shared__ uint32_t tab[48];
-    t1_mq_lanes<true>(tab, jobs, blockIdx.x * 64 + threadIdx.x, njobs, nullptr, 0, nullptr, nullptr, 0, nullptr);
+    __shared__ uint8_t ring[RING_BYTES];
+    t1_mq_lanes<true>(tab, ring, jobs, blockIdx.x * 64 + threadIdx.x, njobs, nullptr, 0, nullptr, nullptr, 0,
+                      nullptr);
 }
 
 // ---- parallel style: grid (block groups of 64) x (pass index) ----
@@ -132,6 +140,7 @@ __global__ void __launch_bounds__(64) t1_mqp_kernel(const uint8_t *__restrict__ 
                                                     int32_t *__restrict__ seglens, int style)
 {
     __shared__ uint32_t tab[48];
+    __shared__ uint8_t ring[RING_BYTES];
     const int b = blockIdx.x * 64 + threadIdx.x;
     bool active = b < ncb;
     int p = 0, s0 = 0, s1 = 0, at = 0, cap = 0;
@@ -142,13 +151,14 @@ __global__ void __launch_bounds__(64) t1_mqp_kernel(const uint8_t *__restrict__ 
         t1_raw_pass_lane(active, syms + bb * symcap, s0, s1, arena + bb * arena_stride + at, cap,
                          seglens + bb * MAXP + p);
     else
-        t1_mq_pass_lane(tab, active, syms + bb * symcap, s0, s1, arena + bb * arena_stride + at, cap,
+        t1_mq_pass_lane(tab, ring, active, syms + bb * symcap, s0, s1, arena + bb * arena_stride + at, cap,
                         seglens + bb * MAXP + p);
 }
 
 __global__ void __launch_bounds__(64) t1_mqp_jobs_kernel(const T1Job *__restrict__ jobs, int njobs, int style)
 {
     __shared__ uint32_t tab[48];
+    __shared__ uint8_t ring[RING_BYTES];
     const int b = blockIdx.x * 64 + threadIdx.x;
     bool active = b < njobs;
     int p = 0, s0 = 0, s1 = 0, at = 0, cap = 0;
@@ -158,7 +168,149 @@ __global__ void __launch_bounds__(64) t1_mqp_jobs_kernel(const T1Job *__restrict
     if (pass_is_raw(style, q))
         t1_raw_pass_lane(active, j.sym, s0, s1, j.seg + at, cap, j.seglen + p);
     else
-        t1_mq_pass_lane(tab, active, j.sym, s0, s1, j.seg + at, cap, j.seglen + p);
+        t1_mq_pass_lane(tab, ring, active, j.sym, s0, s1, j.seg + at, cap, j.seglen + p);
+}
+
+// ---- parallel styles, sorted form (see t1_chain_lane) ----
+// Block b's record from the encoder's flat per-block arrays.
+struct FlatJobs {
+    const uint8_t *syms;
+    int symcap;
+    const int4 *cm;
+    const int32_t *poffs;
+    uint8_t *arena;
+    size_t astride;
+    int32_t *seglens;
+    __device__ T1Job operator()(int b) const
+    {
+        T1Job j{};
+        j.sym = const_cast<uint8_t *>(syms) + (size_t)b * symcap;
+        j.symcap = symcap;
+        j.cm = const_cast<i32x4 *>(cm + b);
+        j.poff = const_cast<int32_t *>(poffs) + (size_t)b * MAXP;
+        j.seg = arena + (size_t)b * astride;
+        j.seglen = seglens + (size_t)b * MAXP;
+        return j;
+    }
+};
+// Job b of a job list (batched path).
+struct ListJobs {
+    const T1Job *jobs;
+    __device__ const T1Job &operator()(int b) const { return jobs[b]; }
+};
+
+__global__ void __launch_bounds__(256) chain_zero_kernel(int *ws)
+{
+    for (int i = threadIdx.x; i < CH_LIST; i += 256) ws[i] = 0;
+}
+
+// Histogram of chain keys, privatised per workgroup in LDS.
+template <class J>
+__global__ void __launch_bounds__(256) chain_hist_kernel(J job, int nblocks, int style, int *ws)
+{
+    __shared__ int h[CH_HIST];
+    for (int k = threadIdx.x; k < CH_HIST; k += 256) h[k] = 0;
+    __syncthreads();
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < (long)nblocks * MAXP) {
+        const T1Job &jb = job((int)(i / MAXP));
+        const int key = chain_key(*jb.cm, jb.poff, (int)(i % MAXP), style);
+        if (key >= 0) atomicAdd(&h[key], 1);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < CH_HIST; k += 256)
+        if (h[k]) atomicAdd(&ws[k], h[k]);
+}
+
+// Exclusive prefix sum of the histogram into the cursors; list lengths.
+__global__ void __launch_bounds__(CH_HIST) chain_scan_kernel(int *ws)
+{
+    __shared__ int a[CH_HIST];
+    const int t = threadIdx.x;
+    const int v = ws[t];
+    a[t] = v;
+    __syncthreads();
+    for (int o = 1; o < CH_HIST; o <<= 1) {
+        const int x = t >= o ? a[t - o] : 0;
+        __syncthreads();
+        a[t] += x;
+        __syncthreads();
+    }
+    ws[CH_HIST + t] = a[t] - v;
+    if (t == CH_BUCKETS - 1) ws[CH_COUNTS] = a[t];
+    if (t == CH_HIST - 1) ws[CH_COUNTS + 1] = a[t] - a[CH_BUCKETS - 1];
+}
+
+// Places every chain id at its bucket's cursor: one global atomic per
+// (workgroup, bucket) reserves a range, LDS atomics number the entries in it.
+template <class J>
+__global__ void __launch_bounds__(256) chain_scatter_kernel(J job, int nblocks, int style, int *ws)
+{
+    __shared__ int h[CH_HIST];
+    for (int k = threadIdx.x; k < CH_HIST; k += 256) h[k] = 0;
+    __syncthreads();
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    int key = -1, local = 0;
+    if (i < (long)nblocks * MAXP) {
+        const T1Job &jb = job((int)(i / MAXP));
+        key = chain_key(*jb.cm, jb.poff, (int)(i % MAXP), style);
+        if (key >= 0) local = atomicAdd(&h[key], 1);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < CH_HIST; k += 256)
+        if (h[k]) h[k] = atomicAdd(&ws[CH_HIST + k], h[k]);
+    __syncthreads();
+    if (key >= 0) ws[CH_LIST + h[key] + local] = (int)i;
+}
+
+// Codes the sorted lists: MQ wavefronts first (longest chains first), then
+// raw ones; workgroups past the end of the lists exit.
+template <class J>
+__global__ void __launch_bounds__(64) chain_code_kernel(J job, const int *ws)
+{
+    __shared__ uint32_t tab[48];
+    __shared__ uint8_t ring[RING_BYTES];
+    const int nmq = ws[CH_COUNTS], ntot = nmq + ws[CH_COUNTS + 1];
+    const int mqw = (nmq + 63) >> 6;
+    const int w = (int)blockIdx.x;
+    const bool raw = w >= mqw;
+    const int base = raw ? nmq + (w - mqw) * 64 : w * 64;
+    const int end = raw ? ntot : nmq;
+    if (base >= end) return;
+    // The kernel lasts as long as its longest chain, whose wavefront shares a
+    // SIMD with shorter ones: issue priority by the length of the wavefront's
+    // first (longest) chain lets the long chains run at their own latency
+    // while short wavefronts fill their stalls.
+    {
+        const int id = ws[CH_LIST + base];
+        const T1Job &jb = job(id / MAXP);
+        int p, s0, s1, at, cap;
+        const int len = pass_range(*jb.cm, jb.poff, id % MAXP, p, s0, s1, at, cap) ? s1 - s0 : 0;
+        if (len >= 2048) __builtin_amdgcn_s_setprio(3);
+        else if (len >= 1024) __builtin_amdgcn_s_setprio(2);
+        else if (len >= 256) __builtin_amdgcn_s_setprio(1);
+    }
+    t1_chain_lane(tab, ring, raw, ws + CH_LIST, base, end, job);
+}
+
+template <class J>
+int launch_chains(hipStream_t s, const J &job, int nblocks, int style, int *ws)
+{
+    const long n = (long)nblocks * MAXP;
+    const unsigned g = (unsigned)((n + 255) / 256);
+    chain_zero_kernel<<<1, 256, 0, s>>>(ws);
+    RG_LAUNCH_CHECK();
+    chain_hist_kernel<<<g, 256, 0, s>>>(job, nblocks, style, ws);
+    RG_LAUNCH_CHECK();
+    chain_scan_kernel<<<1, CH_HIST, 0, s>>>(ws);
+    RG_LAUNCH_CHECK();
+    chain_scatter_kernel<<<g, 256, 0, s>>>(job, nblocks, style, ws);
+    RG_LAUNCH_CHECK();
+    // at most nblocks * (MAXP - 2) chains; +1 wavefront for the raw list's
+    // unaligned start
+    chain_code_kernel<<<(unsigned)(((long)nblocks * (MAXP - 2) + 63) / 64 + 2), 64, 0, s>>>(job, ws);
+    RG_LAUNCH_CHECK();
+    return RG_SUCCESS;
 }
 
 __global__ void __launch_bounds__(64) t1_pack_kernel(const int4 *__restrict__ cm, const int32_t *__restrict__ poffs,
@@ -181,32 +333,43 @@ __global__ void __launch_bounds__(64) t1_pack_jobs_kernel(const T1Job *__restric
 
 int rg_t1_maxp() { return MAXP; }
 size_t rg_t1_seg_bytes(int symcap) { return (size_t)seg_bytes(symcap); }
+size_t rg_t1_chain_ws_bytes(int nblocks) { return sizeof(int) * ((size_t)CH_LIST + (size_t)nblocks * MAXP); }
 
 int rg_launch_t1_parallel(hipStream_t s, const int32_t *q, int pitch, size_t plane_stride, const rg_cb_desc *desc,
                           int ncb, uint8_t *syms, int symcap, int4 *cm, int32_t *poffs, uint8_t *arena,
-                          int32_t *seglens, uint8_t *out, int slot_bytes, rg_cb_result *res, int style)
+                          int32_t *seglens, uint8_t *out, int slot_bytes, rg_cb_result *res, int style,
+                          int32_t *chain_ws)
 {
     if (ncb <= 0) return RG_SUCCESS;
     const size_t astride = rg_t1_seg_bytes(symcap);
     t1_cm_kernel<<<dim3((unsigned)ncb), dim3(64), 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm, poffs,
                                                           style);
     RG_LAUNCH_CHECK();
-    t1_mqp_kernel<<<dim3((unsigned)((ncb + 63) / 64), (unsigned)(MAXP - 2)), dim3(64), 0, s>>>(
-        syms, symcap, cm, poffs, ncb, arena, astride, seglens, style);
-    RG_LAUNCH_CHECK();
+    if (chain_ws) {
+        const FlatJobs fj{syms, symcap, cm, poffs, arena, astride, seglens};
+        if (int rc = launch_chains(s, fj, ncb, style, chain_ws)) return rc;
+    } else {
+        t1_mqp_kernel<<<dim3((unsigned)((ncb + 63) / 64), (unsigned)(MAXP - 2)), dim3(64), 0, s>>>(
+            syms, symcap, cm, poffs, ncb, arena, astride, seglens, style);
+        RG_LAUNCH_CHECK();
+    }
     t1_pack_kernel<<<dim3((unsigned)ncb), dim3(64), 0, s>>>(cm, poffs, arena, astride, seglens, out, slot_bytes, res);
     RG_LAUNCH_CHECK();
     return RG_SUCCESS;
 }
 
-int rg_launch_t1_jobs_parallel(hipStream_t s, const void *jobs, int njobs, int style)
+int rg_launch_t1_jobs_parallel(hipStream_t s, const void *jobs, int njobs, int style, int32_t *chain_ws)
 {
     if (njobs <= 0) return RG_SUCCESS;
     t1_cm_jobs_kernel<<<dim3((unsigned)njobs), dim3(64), 0, s>>>((const T1Job *)jobs);
     RG_LAUNCH_CHECK();
-    t1_mqp_jobs_kernel<<<dim3((unsigned)((njobs + 63) / 64), (unsigned)(MAXP - 2)), dim3(64), 0, s>>>(
-        (const T1Job *)jobs, njobs, style);
-    RG_LAUNCH_CHECK();
+    if (chain_ws) {
+        if (int rc = launch_chains(s, ListJobs{(const T1Job *)jobs}, njobs, style, chain_ws)) return rc;
+    } else {
+        t1_mqp_jobs_kernel<<<dim3((unsigned)((njobs + 63) / 64), (unsigned)(MAXP - 2)), dim3(64), 0, s>>>(
+            (const T1Job *)jobs, njobs, style);
+        RG_LAUNCH_CHECK();
+    }
     t1_pack_jobs_kernel<<<dim3((unsigned)njobs), dim3(64), 0, s>>>((const T1Job *)jobs);
     RG_LAUNCH_CHECK();
     return RG_SUCCESS;
@@ -241,7 +404,8 @@ int rg_t1_job_bytes() { return (int)sizeof(T1Job); }
 
 // Persistent forms: reset the queue counter, then `nwaves` workgroups pull
 // code-blocks until the job list is exhausted.
-int rg_launch_t1_queue(hipStream_t s, const void *jobs, int njobs, int style, int *counter, int nwaves)
+int rg_launch_t1_queue(hipStream_t s, const void *jobs, int njobs, int style, int *counter, int nwaves,
+                       int32_t *chain_ws)
 {
     if (njobs <= 0) return RG_SUCCESS;
     reset_counter_kernel<<<1, 1, 0, s>>>(counter);
@@ -251,8 +415,13 @@ int rg_launch_t1_queue(hipStream_t s, const void *jobs, int njobs, int style, in
     if (style & RG_STYLE_TERMALL) {
         t1_cm_queue_kernel<<<g, b, 0, s>>>(J, njobs, counter);
         RG_LAUNCH_CHECK();
-        t1_mqp_jobs_kernel<<<dim3((unsigned)((njobs + 63) / 64), (unsigned)(MAXP - 2)), b, 0, s>>>(J, njobs,
-                                                                                                  style);
+        if (chain_ws) {
+            if (int rc = launch_chains(s, ListJobs{J}, njobs, style, chain_ws)) return rc;
+        } else {
+            t1_mqp_jobs_kernel<<<dim3((unsigned)((njobs + 63) / 64), (unsigned)(MAXP - 2)), b, 0, s>>>(J, njobs,
+                                                                                                      style);
+            RG_LAUNCH_CHECK();
+        }
         RG_LAUNCH_CHECK();
         t1_pack_jobs_kernel<<<dim3((unsigned)njobs), b, 0, s>>>(J);
     } else if (rg_t1_inline()) {

@@ -40,6 +40,27 @@ RG_DEV uint32_t wv_from_right(uint32_t v)
 {
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130 /* wave_shl:1 */, 0xF, 0xF, true);
 }
+// Inclusive prefix sum over the wavefront's lanes (lane i: v[0] + ... + v[i]),
+// by DPP alone: Hillis-Steele inside each 16-lane row (row_shr 1, 2, 4, 8
+// with zero fill), then row 0's total into row 1 and row 2's into row 3
+// (row_bcast:15 to rows 1 and 3), then the first half's total into rows 2
+// and 3 (row_bcast:31).
+RG_DEV int wv_incl_scan(int v)
+{
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111 /* row_shr:1 */, 0xF, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112 /* row_shr:2 */, 0xF, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114 /* row_shr:4 */, 0xF, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118 /* row_shr:8 */, 0xF, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142 /* row_bcast:15 */, 0xA, 0xF, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143 /* row_bcast:31 */, 0xC, 0xF, false);
+    return v;
+}
+// Byte permute (v_perm_b32): byte i of the result is byte s.byte[i] of the
+// 8-byte value hi:lo (selector 12 gives 0x00).
+RG_DEV uint32_t wv_perm(uint32_t hi, uint32_t lo, uint32_t s) { return __builtin_amdgcn_perm(hi, lo, s); }
+// Compiler scheduling fence: no instruction moves across it (used to issue
+// a memory read before independent work that should hide its latency).
+RG_DEV void wv_sched_fence() { __builtin_amdgcn_sched_barrier(0); }
 // Number of set bits of a 64-bit lane mask below this lane.
 RG_DEV int wv_mbcnt(uint64_t m)
 {

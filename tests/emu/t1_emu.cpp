@@ -3,7 +3,7 @@
 // (not a restatement) can be checked against the oracle without a GPU.
 //
 //   t1_emu MODE DIR PITCH PLANE_STRIDE SLOT [STYLE]
-//     MODE  inline | split | queue_split | parallel | queue_parallel | cm_only
+//     MODE  inline | split | queue_split | parallel | queue_parallel | sorted | cm_only
 //     STYLE code-block style bits of the parallel modes (default 6 =
 //           RESET|TERMALL; 7 adds BYPASS)
 //     DIR   holds q.bin (int32 coefficient planes) and desc.bin (rg_cb_desc
@@ -16,6 +16,9 @@
 //   parallel     RESET|TERMALL style: t1_block<true> recording pass offsets,
 //                t1_mq_pass_lane per (64 blocks, pass), t1_pack_block per block
 //   queue_parallel  the same reading every pointer from the job list
+//   sorted       parallel styles with the chains in the longest-first lists
+//                (built here with the kernels' chain_key) coded 64 per
+//                wavefront by t1_chain_lane, as chain_code_kernel does
 //   cm_only      context modelling alone; writes cm.bin and poff.bin
 // The parallel modes also write seglen.bin (MAXP int32 per block).
 // Exit status 3: an emulator check failed (message on stderr).
@@ -138,9 +141,11 @@ void run(void (*fn)(void *), void *arg)
 
 namespace {
 uint32_t g_tab[48];
+uint8_t g_lut[LUT_BYTES];   // the kernels' workgroup-memory context tables
+uint8_t g_ring[RING_BYTES]; // and the lane MQ coders' output rings
 
-void inline_body(void *a) { t1_block<false>(*static_cast<T1Job *>(a)); }
-void cm_body(void *a) { t1_block<true>(*static_cast<T1Job *>(a)); }
+void inline_body(void *a) { t1_block<false>(*static_cast<T1Job *>(a), g_lut); }
+void cm_body(void *a) { t1_block<true>(*static_cast<T1Job *>(a), g_lut); }
 
 struct MQFlat {
     int i0, ncb, symcap, slot;
@@ -152,7 +157,8 @@ struct MQFlat {
 void mq_flat_body(void *a)
 {
     const MQFlat &m = *static_cast<MQFlat *>(a);
-    t1_mq_lanes<false>(g_tab, nullptr, m.i0 + wv_lane(), m.ncb, m.syms, m.symcap, m.cm, m.slots, m.slot, m.res);
+    t1_mq_lanes<false>(g_tab, g_ring, nullptr, m.i0 + wv_lane(), m.ncb, m.syms, m.symcap, m.cm, m.slots, m.slot,
+                       m.res);
 }
 
 struct MQQueue {
@@ -176,7 +182,22 @@ void mq_pass_body(void *a)
     if (pass_is_raw(m.style, m.p))
         t1_raw_pass_lane(active, j.sym, s0, s1, j.seg + at, cap, j.seglen + p);
     else
-        t1_mq_pass_lane(g_tab, active, j.sym, s0, s1, j.seg + at, cap, j.seglen + p);
+        t1_mq_pass_lane(g_tab, g_ring, active, j.sym, s0, s1, j.seg + at, cap, j.seglen + p);
+}
+struct JobAt {
+    const T1Job *jobs;
+    const T1Job &operator()(int b) const { return jobs[b]; }
+};
+struct ChainRun {
+    const T1Job *jobs;
+    const int32_t *list;
+    int base, end;
+    bool raw;
+};
+void chain_body(void *a)
+{
+    const ChainRun &c = *static_cast<ChainRun *>(a);
+    t1_chain_lane(g_tab, g_ring, c.raw, c.list, c.base, c.end, JobAt{c.jobs});
 }
 void pack_body(void *a)
 {
@@ -186,7 +207,7 @@ void pack_body(void *a)
 void mq_queue_body(void *a)
 {
     const MQQueue &m = *static_cast<MQQueue *>(a);
-    t1_mq_lanes<true>(g_tab, m.jobs, m.i0 + wv_lane(), m.njobs, nullptr, 0, nullptr, nullptr, 0, nullptr);
+    t1_mq_lanes<true>(g_tab, g_ring, m.jobs, m.i0 + wv_lane(), m.njobs, nullptr, 0, nullptr, nullptr, 0, nullptr);
 }
 
 std::vector<char> slurp(const std::string &path)
@@ -245,7 +266,7 @@ int main(int argc, char **argv)
         j.sym = syms + (size_t)k * symcap;
         j.symcap = symcap;
         j.cm = &cm[k];
-        const bool par = mode == "parallel" || mode == "queue_parallel";
+        const bool par = mode == "parallel" || mode == "queue_parallel" || mode == "sorted";
         j.style = par ? style : 0;
         j.poff = par ? &poffs[(size_t)k * MAXP] : nullptr;
         j.seg = arena + (size_t)k * astride;
@@ -271,6 +292,32 @@ int main(int argc, char **argv)
         for (int k = 0; k < ncb; ++k) emu::run(cm_body, &jobs[k]);
         spill(dir + "/cm.bin", cm.data(), sizeof(i32x4) * cm.size());
         spill(dir + "/poff.bin", poffs.data(), sizeof(int32_t) * poffs.size());
+    } else if (mode == "sorted") {
+        for (int k = 0; k < ncb; ++k) emu::run(cm_body, &jobs[k]);
+        // counting sort by chain_key: MQ buckets (longest first), then raw ones
+        std::vector<std::vector<int32_t>> bucket(CH_HIST);
+        for (int k = 0; k < ncb; ++k)
+            for (int q = 0; q < MAXP; ++q) {
+                const int key = chain_key(cm[k], jobs[k].poff, q, style);
+                if (key >= 0) bucket[key].push_back(k * MAXP + q);
+            }
+        std::vector<int32_t> list;
+        int nmq = 0;
+        for (int key = 0; key < CH_HIST; ++key) {
+            list.insert(list.end(), bucket[key].begin(), bucket[key].end());
+            if (key == CH_BUCKETS - 1) nmq = (int)list.size();
+        }
+        const int ntot = (int)list.size();
+        for (int base = 0; base < nmq; base += 64) {
+            ChainRun c{jobs.data(), list.data(), base, nmq, false};
+            emu::run(chain_body, &c);
+        }
+        for (int base = nmq; base < ntot; base += 64) {
+            ChainRun c{jobs.data(), list.data(), base, ntot, true};
+            emu::run(chain_body, &c);
+        }
+        for (int k = 0; k < ncb; ++k) emu::run(pack_body, &jobs[k]);
+        spill(dir + "/seglen.bin", seglens.data(), sizeof(int32_t) * seglens.size());
     } else if (mode == "parallel" || mode == "queue_parallel") {
         for (int k = 0; k < ncb; ++k) emu::run(cm_body, &jobs[k]);
         for (int p = 0; p < MAXP - 2; ++p)

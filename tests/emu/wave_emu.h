@@ -94,6 +94,28 @@ static inline uint32_t wv_from_right(uint32_t x)
     const uint64_t *v = emu::exchange(x, 0);
     return emu::lane() == 63 ? 0u : (uint32_t)v[emu::lane() + 1];
 }
+static inline void wv_sched_fence() {}
+static inline uint32_t wv_perm(uint32_t hi, uint32_t lo, uint32_t s)
+{
+    const uint64_t v = ((uint64_t)hi << 32) | lo;
+    uint32_t r = 0;
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t k = (s >> (8 * i)) & 0xFFu;
+        uint32_t b;
+        if (k < 8) b = (uint32_t)(v >> (8 * k)) & 0xFFu;
+        else if (k == 12) b = 0;
+        else { fprintf(stderr, "wave emulator: unsupported perm selector\n"); exit(3); }
+        r |= b << (8 * i);
+    }
+    return r;
+}
+static inline int wv_incl_scan(int x)
+{
+    const uint64_t *v = emu::exchange((uint32_t)x, 0);
+    int s = 0;
+    for (int l = 0; l <= emu::lane(); ++l) s += (int)(uint32_t)v[l];
+    return s;
+}
 static inline int wv_mbcnt(uint64_t m)
 {
     const int l = emu::lane();

@@ -72,6 +72,25 @@ def test_image_blocks_match_oracle(emu, tmp_path, mode, w, h, nc, rev, levels, s
     assert blobs == data
 
 
+def _skewed_lps_blocks():
+    """Mostly-zero blocks whose rare 1s are LPS decisions in very skewed
+    states (hundreds of run-length / ZC zeros first), so that an LPS
+    renormalisation shifts 9-15 bits and, for these patterns and
+    orientations, needs two BYTEOUTs in one coding step.  Returns (block,
+    orientation) pairs."""
+    cases = [(3, [(41, 31, 1), (41, 46, 3), (51, 40, 1)]), (0, [(44, 53, 1), (53, 0, 3)]),
+             (3, [(38, 44, 1), (47, 24, 3)]), (1, [(33, 5, -1), (33, 34, 3), (38, 45, 1)]),
+             (3, [(47, 55, -1), (54, 16, 3), (58, 61, 1)]), (2, [(36, 16, -1), (52, 15, 3), (53, 47, 1)]),
+             (0, [(63, 63, 1)])]
+    out = []
+    for orient, pts in cases:
+        b = np.zeros((64, 64), np.int32)
+        for y, x, v in pts:
+            b[y, x] = v
+        out.append((b, orient))
+    return out
+
+
 @pytest.mark.parametrize("mode", MODES)
 def test_dense_blocks_match_oracle(emu, tmp_path, mode):
     """Full 64x64 blocks of wide-range Laplacian-like values (many bit-planes,
@@ -88,14 +107,18 @@ def test_dense_blocks_match_oracle(emu, tmp_path, mode):
     one = np.zeros((64, 64), np.int32)
     one[37, 21] = -5
     blocks.append(one)
+    orients = [i % 4 for i in range(len(blocks))]
+    for b, o in _skewed_lps_blocks():
+        blocks.append(b)
+        orients.append(o)
     plane = np.concatenate(blocks, axis=1)               # 64 x (64 * nblocks)
     n = len(blocks)
     desc = np.zeros(n, CB_DESC_DTYPE)
     for i in range(n):
-        desc[i] = (64 * i, 0, 64, 64, 0, i % 4, 0, 0)
+        desc[i] = (64 * i, 0, 64, 64, 0, orients[i], 0, 0)
     res, blobs = _run(emu, tmp_path, mode, plane, desc, plane.shape[1], plane.size, 64 * 64 * 4)
     for i, b in enumerate(blocks):
-        data, npasses, nbp = O.t1_encode(b, i % 4)
+        data, npasses, nbp = O.t1_encode(b, orients[i])
         assert res["flags"][i] == 0
         assert (res["nbp"][i], res["npasses"][i]) == (nbp, npasses), i
         assert blobs[i] == data, i
@@ -130,7 +153,7 @@ def _run_parallel(emu, tmp_path, mode, q, desc, pitch, plane_stride, slot, style
 
 
 @pytest.mark.parametrize("style", [O.STYLE_PARALLEL, O.STYLE_PARALLEL_BYPASS])
-@pytest.mark.parametrize("mode", ["parallel", "queue_parallel"])
+@pytest.mark.parametrize("mode", ["parallel", "queue_parallel", "sorted"])
 @pytest.mark.parametrize("w,h,nc,rev,levels,seed,cb", [
     (97, 61, 1, False, 3, 52, 64),
     (33, 1, 1, False, 1, 56, 64),
@@ -160,24 +183,29 @@ def test_parallel_style_matches_oracle(emu, tmp_path, mode, w, h, nc, rev, level
     assert blobs == data
 
 
+@pytest.mark.parametrize("mode", ["parallel", "sorted"])
 @pytest.mark.parametrize("style", [O.STYLE_PARALLEL, O.STYLE_PARALLEL_BYPASS])
-def test_parallel_style_dense_blocks(emu, tmp_path, style):
+def test_parallel_style_dense_blocks(emu, tmp_path, style, mode):
     rng = np.random.default_rng(23)
     blocks = [np.round(rng.laplace(0, s, (64, 64))).astype(np.int32) for s in (2, 60, 3000, 40000)]
     blocks.append(np.zeros((64, 64), np.int32))
     # raw segments with many 0xFF bytes (bit stuffing): all-ones refinement bits
     blocks.append(np.full((64, 64), (1 << 14) - 1, np.int32) * np.where(rng.random((64, 64)) < 0.5, 1, -1))
+    orients = [(i + 1) % 4 for i in range(len(blocks))]
+    for b, o in _skewed_lps_blocks():
+        blocks.append(b)
+        orients.append(o)
     plane = np.concatenate(blocks, axis=1)
     desc = np.zeros(len(blocks), CB_DESC_DTYPE)
     for i in range(len(blocks)):
-        desc[i] = (64 * i, 0, 64, 64, 0, (i + 1) % 4, 0, 0)
-    res, blobs, seglen = _run_parallel(emu, tmp_path, "parallel", plane, desc, plane.shape[1], plane.size,
+        desc[i] = (64 * i, 0, 64, 64, 0, orients[i], 0, 0)
+    res, blobs, seglen = _run_parallel(emu, tmp_path, mode, plane, desc, plane.shape[1], plane.size,
                                        64 * 64 * 4, style)
     for i, b in enumerate(blocks):
         if not b.any():
             assert (res["len"][i], res["npasses"][i], res["nbp"][i]) == (0, 0, 0)
             continue
-        data, npasses, nbp, passlen = O.t1_encode(b, (i + 1) % 4, style=style)
+        data, npasses, nbp, passlen = O.t1_encode(b, orients[i], style=style)
         assert (res["nbp"][i], res["npasses"][i], res["flags"][i]) == (nbp, npasses, 0)
         np.testing.assert_array_equal(seglen[i, :npasses], passlen)
         assert blobs[i] == data
