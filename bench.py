@@ -250,6 +250,15 @@ def main():
     d_cp.free()
     l1_bytes = 8.0 * W * H * NC
     achieved = l1_bytes / (t_dwt_l1 * 1e-3) / 1e9
+    # HBM bytes per level-1 launch measured with rocprofv3 counters (FETCH_SIZE
+    # and WRITE_SIZE scaled by a copy kernel of known size), as recorded in the
+    # committed counter summary for this workload
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "encode_4096rgb_pmc_counters.md")
+    if W == 4096 and H == 4096 and os.path.exists(prof):
+        import re
+        mt = re.search(r"^traffic_bytes_per_launch: (\d+)$", open(prof).read(), re.M)
+        traffic = int(mt.group(1)) if mt else None
     colour_gbs = 15.0 * W * H / (t_colour * 1e-3) / 1e9
 
     out = {
@@ -264,7 +273,7 @@ def main():
                    "codeblocks": int(ncb), "codestream_bytes": total_bytes, "input_sha256": img_sha},
         "roofline": {"bound": "hbm", "kernel": "fwd_level_kernel<9/7> (level 1, 3 planes)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes": l1_bytes, "avg_launch_ms": round(t_dwt_l1, 4)},
         "hbm_copy_ceiling_GBps": round(copy_gbs, 1),
         "stages_ms": {"colour_ict": round(t_colour, 4), "colour_GBps": round(colour_gbs, 1),

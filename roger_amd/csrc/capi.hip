@@ -435,8 +435,12 @@ struct rg_encoder {
     int ncb_comp = 0, ncb = 0, slot_bytes = 0, colour = 0, style = 0;
     double delta = 1.0 / 64;
     float step[97], inv[97];
-    DevBuf input, planes, q, fcoeff, scratch, desc, res, slots, chan16, syms, cm, poffs, arena, seglens, chains;
+    DevBuf input, planes, q, fcoeff, scratch, desc, res, slots, chan16, syms, cm, poffs, arena, seglens, chains,
+        chains2, cms;
     HostBuf staging, map;
+    // second stream and events of the overlapped two-phase tier-1 (created on first use)
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
     std::vector<rg_cb_desc> h_desc;
     int desc_key[6] = {0, 0, 0, 0, 0, 0};
 };
@@ -455,9 +459,13 @@ extern "C" int rg_encoder_destroy(rg_encoder *e)
     if (!e) return RG_SUCCESS;
     hipSetDevice(e->ctx->device);
     hipStreamSynchronize(e->ctx->stream);
+    if (e->s2) hipStreamSynchronize(e->s2);
     for (DevBuf *b : {&e->input, &e->planes, &e->q, &e->fcoeff, &e->scratch, &e->desc, &e->res, &e->slots, &e->chan16,
-                      &e->syms, &e->cm, &e->poffs, &e->arena, &e->seglens, &e->chains})
+                      &e->syms, &e->cm, &e->poffs, &e->arena, &e->seglens, &e->chains, &e->chains2, &e->cms})
         b->release();
+    if (e->ev_a) hipEventDestroy(e->ev_a);
+    if (e->ev_b) hipEventDestroy(e->ev_b);
+    if (e->s2) hipStreamDestroy(e->s2);
     e->staging.release();
     e->map.release();
     delete e;
@@ -500,11 +508,22 @@ static int run_t1(rg_encoder *e)
             (rc = e->arena.ensure(rg_t1_seg_bytes(symcap) * e->ncb)) ||
             (rc = e->chains.ensure(rg_t1_chain_ws_bytes(e->ncb))))
             return rc;
+        rg_t1_overlap ov{};
+        const bool two = rg_t1_two_phase();
+        if (two) {
+            if ((rc = e->chains2.ensure(rg_t1_chain_ws_bytes(e->ncb))) || (rc = e->cms.ensure((size_t)e->ncb * 64 * 16)))
+                return rc;
+            if (!e->s2) RG_HIP(hipStreamCreateWithFlags(&e->s2, hipStreamNonBlocking));
+            if (!e->ev_a) RG_HIP(hipEventCreateWithFlags(&e->ev_a, hipEventDisableTiming));
+            if (!e->ev_b) RG_HIP(hipEventCreateWithFlags(&e->ev_b, hipEventDisableTiming));
+            ov.s2 = e->s2; ov.ev_a = e->ev_a; ov.ev_b = e->ev_b;
+            ov.chain_ws2 = (int32_t *)e->chains2.p; ov.cms = e->cms.p;
+        }
         rc = rg_launch_t1_parallel(e->ctx->stream, (const int32_t *)e->q.p, e->w, (size_t)e->w * e->h,
                                    (const rg_cb_desc *)e->desc.p, e->ncb, (uint8_t *)e->syms.p, symcap,
                                    (int4 *)e->cm.p, (int32_t *)e->poffs.p, (uint8_t *)e->arena.p,
                                    (int32_t *)e->seglens.p, (uint8_t *)e->slots.p, e->slot_bytes,
-                                   (rg_cb_result *)e->res.p, e->style, (int32_t *)e->chains.p);
+                                   (rg_cb_result *)e->res.p, e->style, (int32_t *)e->chains.p, two ? &ov : nullptr);
         if (rc == RG_SUCCESS) e->t1_ran = 1;
         return rc;
     }

@@ -127,10 +127,40 @@ int rg_launch_t1_jobs_split(hipStream_t s, const void *jobs, int njobs);
 int rg_t1_maxp();
 size_t rg_t1_seg_bytes(int symcap);
 size_t rg_t1_chain_ws_bytes(int nblocks);
+// Two-phase overlapped tier-1 (parallel styles, plain path): context
+// modelling split before pass RG_T1_PHASE_SPLIT; the first phase's chains are
+// coded on s2 while s continues.  cms: 64 x 16 bytes per block of modelling
+// state kept between the phases; chain_ws2: a second chain workspace.
+#define RG_T1_PHASE_SPLIT 10
+bool rg_t1_two_phase();
+struct rg_t1_overlap {
+    hipStream_t s2;
+    hipEvent_t ev_a, ev_b;
+    int32_t *chain_ws2;
+    void *cms;
+};
+// The encoder's flat per-block tier-1 arrays, for the chain kernels
+// (t1_chain.hip); astride = rg_t1_seg_bytes(symcap).
+struct rg_t1_flat {
+    const uint8_t *syms;
+    int symcap;
+    const int4 *cm;
+    const int32_t *poffs;
+    uint8_t *arena;
+    size_t astride;
+    int32_t *seglens;
+};
+// Chain lists and coding of passes [qlo, qhi) of every block (t1_chain.hip).
+int rg_launch_chains_flat(hipStream_t s, const rg_t1_flat &f, int nblocks, int style, int32_t *ws, int qlo, int qhi);
+int rg_launch_chains_jobs(hipStream_t s, const void *jobs, int njobs, int style, int32_t *ws, int qlo, int qhi);
+// Standard-style lane MQ coders (t1_mq.hip).
+int rg_launch_t1_mq(hipStream_t s, const uint8_t *syms, int symcap, const int4 *cm, int ncb, uint8_t *out,
+                    int slot_bytes, rg_cb_result *res);
+int rg_launch_t1_mq_jobs(hipStream_t s, const void *jobs, int njobs);
 int rg_launch_t1_parallel(hipStream_t s, const int32_t *q, int pitch, size_t plane_stride, const rg_cb_desc *desc,
                           int ncb, uint8_t *syms, int symcap, int4 *cm, int32_t *poffs, uint8_t *arena,
                           int32_t *seglens, uint8_t *out, int slot_bytes, rg_cb_result *res, int style,
-                          int32_t *chain_ws);
+                          int32_t *chain_ws, const rg_t1_overlap *ov);
 int rg_launch_t1_jobs_parallel(hipStream_t s, const void *jobs, int njobs, int style, int32_t *chain_ws);
 // Persistent batched tier-1 (any style): nwaves workgroups pull job indices
 // from *counter (reset by a kernel inside the launch sequence).

@@ -269,6 +269,15 @@ struct T1Job {
     int32_t *poff;  // split path (may be null): stream offset where each pass starts (MAXP)
     uint8_t *seg;   // parallel style: this block's segment arena (seg_bytes(symcap) bytes)
     int32_t *seglen;// parallel style: codeword segment length of each pass (MAXP)
+    u32x4 *cms;     // two-phase context modelling: 64 per-lane state words kept between the phases
+};
+
+// Context modelling of coding passes [qbeg, qend) only (split path).  A phase
+// ends before a significance-propagation pass (qend = 1 mod 3), where the
+// per-lane state is just the significance and refinement masks: they, the
+// stream position and the overflow flag carry over (job.cms, job.cm).
+struct CMPhase {
+    int qbeg = 0, qend = 1 << 30;
 };
 
 // Segment arena of the parallel style: pass p, whose symbols start at stream
@@ -316,7 +325,7 @@ RG_DEV void emit_stripe(const SymList &sl, const T1Job &job, int &pos, int &ovf,
 // context modelling only, the symbols go to job.sym for the lane MQ coder.
 // lut: LUT_BYTES of workgroup memory for the block's context tables.
 template <bool SPLIT>
-RG_DEV void t1_block(const T1Job &job, uint8_t *lut)
+RG_DEV void t1_block(const T1Job &job, uint8_t *lut, CMPhase ph = CMPhase())
 {
     const int lane = wv_lane();
     const int w = job.w, h = job.h, orient = job.orient, pitch = job.pitch;
@@ -348,6 +357,8 @@ RG_DEV void t1_block(const T1Job &job, uint8_t *lut)
         else *job.res = rg_cb_result{0, 0, 0, 0};
         return;
     }
+    const int npass = 3 * nbp - 2;
+    if (SPLIT && ph.qbeg >= npass) return;   // a later phase with nothing left: the record is final
     t1_lut_fill(lut, orient);
     int spos = 0, sovf = 0;   // split path: stream position / overflow (uniform)
     int pidx = 0;             // split path: coding pass index
@@ -370,7 +381,22 @@ RG_DEV void t1_block(const T1Job &job, uint8_t *lut)
     uint64_t S = 0, V = 0, R = 0;
     const int nstripes = (h + 3) >> 2;
 
-    for (int p = nbp - 1; p >= 0; --p) {
+    // a later phase resumes at pass qbeg (bit-plane (qbeg + 2) / 3 from the top,
+    // pass type (qbeg - 1) mod 3) from the state the previous phase left
+    int pstart = nbp - 1, t0 = 2;
+    if (SPLIT && ph.qbeg > 0) {
+        const i32x4 c = *job.cm;
+        spos = wv_uniform(c.x);
+        sovf = wv_uniform(c.w);
+        const u32x4 st = wv_load16(reinterpret_cast<const uint8_t *>(job.cms + lane));
+        S = ((uint64_t)st.y << 32) | st.x;
+        R = ((uint64_t)st.w << 32) | st.z;
+        pidx = ph.qbeg;
+        pstart = nbp - 1 - (ph.qbeg + 2) / 3;
+        t0 = (ph.qbeg - 1) % 3;
+    }
+
+    for (int p = pstart; p >= 0; --p) {
         uint32_t lo = 0, hi = 0;
 #pragma unroll
         for (int r = 0; r < 32; ++r) lo |= ((mag[r] >> p) & 1u) << r;
@@ -378,8 +404,16 @@ RG_DEV void t1_block(const T1Job &job, uint8_t *lut)
         for (int r = 0; r < 32; ++r) hi |= ((mag[r + 32] >> p) & 1u) << r;
         const uint64_t B = ((uint64_t)hi << 32) | lo;
 
-        for (int pass = (p == nbp - 1) ? 2 : 0; pass < 3; ++pass) {
+        for (int pass = p == pstart ? t0 : 0; pass < 3; ++pass) {
             if (SPLIT && job.poff) job.poff[pidx] = spos;   // uniform value, stored by every lane
+            if (SPLIT && pidx >= ph.qend) {
+                // end of this phase: keep the state for the next one (V is clear
+                // after a cleanup pass); the record holds the stream position
+                wv_store16(reinterpret_cast<uint8_t *>(job.cms + lane), (uint32_t)S, (uint32_t)(S >> 32), (uint32_t)R,
+                           (uint32_t)(R >> 32));
+                *job.cm = mk_i32x4(spos, npass, nbp, sovf);
+                return;
+            }
             ++pidx;
             for (int s = 0; s < nstripes; ++s) {
                 const int r0 = 4 * s;
@@ -516,7 +550,7 @@ RG_DEV void t1_block(const T1Job &job, uint8_t *lut)
     }
 
     if constexpr (SPLIT) {
-        *job.cm = mk_i32x4(spos, 3 * nbp - 2, nbp, sovf);
+        *job.cm = mk_i32x4(spos, npass, nbp, sovf);
         return;
     }
     // ---- FLUSH (C.2.9) ----
@@ -665,7 +699,8 @@ RG_DEV int ring_code(RingMQ &m, CtxStates &cs, uint32_t sym, uint32_t st, uint32
 // RENORME's C side (C.2.6): n shifts with BYTEOUT whenever CT reaches 0.  C
 // takes min(n, CT), the predicated BYTEOUT, then the rest -- fewer than the 7
 // or 8 bits of the new CT, except after an LPS in a very skewed state (n up
-// to 15), which the loop finishes (entered only when a lane needs it).
+// to 15), which the loop finishes (entered only when a lane needs it; a
+// second predicated BYTEOUT on every step instead measured slower).
 RG_DEV void ring_renorm(RingMQ &m, int n, uint8_t *rb)
 {
     const int sft = n < m.CT ? n : m.CT;
@@ -954,11 +989,12 @@ RG_DEV int chain_bucket(int len)
 }
 
 // Histogram key of chain (block c/poff, pass q), or -1 if the block has no
-// pass q.  Raw chains (BYPASS) take the second half of the keys.
-RG_DEV int chain_key(const i32x4 &c, const int32_t *poff, int q, int style)
+// pass q or q is outside [qlo, qhi).  Raw chains (BYPASS) take the second
+// half of the keys.
+RG_DEV int chain_key(const i32x4 &c, const int32_t *poff, int q, int style, int qlo = 0, int qhi = MAXP)
 {
     int p, s0, s1, at, cap;
-    if (!pass_range(c, poff, q, p, s0, s1, at, cap)) return -1;
+    if (q < qlo || q >= qhi || !pass_range(c, poff, q, p, s0, s1, at, cap)) return -1;
     return (pass_is_raw(style, q) ? CH_BUCKETS : 0) + chain_bucket(s1 - s0);
 }
 

@@ -8,8 +8,10 @@
 //   t1_encode_jobs_kernel  inline path, one workgroup per job-list entry
 //   t1_cm_kernel           split path, context modelling per descriptor
 //   t1_cm_jobs_kernel      split path, context modelling per job-list entry
-//   t1_mq_kernel           split path, MQ coder, one lane per code-block
-//   t1_mq_jobs_kernel      same over a job list
+//   t1_pack_kernel         parallel styles, pass segments into the slots
+// The lane coders live in t1_mq.hip (one MQ chain per code-block) and
+// t1_chain.hip (the sorted pass chains of the parallel styles); the three
+// files compile in parallel.
 #include <cstdlib>
 #include "rg_internal.h"
 #include "wave_hip.h"
@@ -93,7 +95,7 @@ __global__ void reset_counter_kernel(int *counter) { *counter = 0; }
 __global__ void __launch_bounds__(64) t1_cm_kernel(const int32_t *__restrict__ q, int pitch, size_t plane_stride,
                                                    const rg_cb_desc *__restrict__ desc, uint8_t *__restrict__ syms,
                                                    int symcap, int4 *__restrict__ cm, int32_t *__restrict__ poffs,
-                                                   int style)
+                                                   int style, int qbeg, int qend, uint4 *__restrict__ cms)
 {
     const rg_cb_desc d = desc[blockIdx.x];
     T1Job job;
@@ -104,213 +106,18 @@ __global__ void __launch_bounds__(64) t1_cm_kernel(const int32_t *__restrict__ q
     job.cm = cm + blockIdx.x;
     job.poff = poffs ? poffs + (size_t)blockIdx.x * MAXP : nullptr;
     job.style = style;
+    job.cms = cms ? cms + (size_t)blockIdx.x * 64 : nullptr;
     __shared__ uint8_t lut[LUT_BYTES];
-    t1_block<true>(job, lut);
+    CMPhase ph;
+    ph.qbeg = qbeg;
+    ph.qend = qend;
+    t1_block<true>(job, lut, ph);
 }
 
 __global__ void __launch_bounds__(64) t1_cm_jobs_kernel(const T1Job *__restrict__ jobs)
 {
     __shared__ uint8_t lut[LUT_BYTES];
     t1_block<true>(jobs[blockIdx.x], lut);
-}
-
-__global__ void __launch_bounds__(64) t1_mq_kernel(const uint8_t *__restrict__ syms, int symcap,
-                                                   const int4 *__restrict__ cm, int ncb, uint8_t *__restrict__ slots,
-                                                   int slot, rg_cb_result *__restrict__ res)
-{
-    __shared__ uint32_t tab[48];
-    __shared__ uint8_t ring[RING_BYTES];
-    t1_mq_lanes<false>(tab, ring, nullptr, blockIdx.x * 64 + threadIdx.x, ncb, syms, symcap, cm, slots, slot, res);
-}
-
-__global__ void __launch_bounds__(64) t1_mq_jobs_kernel(const T1Job *__restrict__ jobs, int njobs)
-{
-    __shared__ uint32_t tab[48];
-    __shared__ uint8_t ring[RING_BYTES];
-    t1_mq_lanes<true>(tab, ring, jobs, blockIdx.x * 64 + threadIdx.x, njobs, nullptr, 0, nullptr, nullptr, 0,
-                      nullptr);
-}
-
-// ---- parallel style: grid (block groups of 64) x (pass index) ----
-// Row q codes pass q of 64 blocks; under BYPASS a raw row (pass_is_raw, the
-// same for the whole workgroup) packs bits instead of running MQ coders.
-__global__ void __launch_bounds__(64) t1_mqp_kernel(const uint8_t *__restrict__ syms, int symcap,
-                                                    const int4 *__restrict__ cm, const int32_t *__restrict__ poffs,
-                                                    int ncb, uint8_t *__restrict__ arena, size_t arena_stride,
-                                                    int32_t *__restrict__ seglens, int style)
-{
-    __shared__ uint32_t tab[48];
-    __shared__ uint8_t ring[RING_BYTES];
-    const int b = blockIdx.x * 64 + threadIdx.x;
-    bool active = b < ncb;
-    int p = 0, s0 = 0, s1 = 0, at = 0, cap = 0;
-    const int q = (int)blockIdx.y;
-    if (active) active = pass_range(cm[b], poffs + (size_t)b * MAXP, q, p, s0, s1, at, cap);
-    const size_t bb = active ? (size_t)b : 0;
-    if (pass_is_raw(style, q))
-        t1_raw_pass_lane(active, syms + bb * symcap, s0, s1, arena + bb * arena_stride + at, cap,
-                         seglens + bb * MAXP + p);
-    else
-        t1_mq_pass_lane(tab, ring, active, syms + bb * symcap, s0, s1, arena + bb * arena_stride + at, cap,
-                        seglens + bb * MAXP + p);
-}
-
-__global__ void __launch_bounds__(64) t1_mqp_jobs_kernel(const T1Job *__restrict__ jobs, int njobs, int style)
-{
-    __shared__ uint32_t tab[48];
-    __shared__ uint8_t ring[RING_BYTES];
-    const int b = blockIdx.x * 64 + threadIdx.x;
-    bool active = b < njobs;
-    int p = 0, s0 = 0, s1 = 0, at = 0, cap = 0;
-    const T1Job &j = jobs[active ? b : 0];
-    const int q = (int)blockIdx.y;
-    if (active) active = pass_range(*j.cm, j.poff, q, p, s0, s1, at, cap);
-    if (pass_is_raw(style, q))
-        t1_raw_pass_lane(active, j.sym, s0, s1, j.seg + at, cap, j.seglen + p);
-    else
-        t1_mq_pass_lane(tab, ring, active, j.sym, s0, s1, j.seg + at, cap, j.seglen + p);
-}
-
-// ---- parallel styles, sorted form (see t1_chain_lane) ----
-// Block b's record from the encoder's flat per-block arrays.
-struct FlatJobs {
-    const uint8_t *syms;
-    int symcap;
-    const int4 *cm;
-    const int32_t *poffs;
-    uint8_t *arena;
-    size_t astride;
-    int32_t *seglens;
-    __device__ T1Job operator()(int b) const
-    {
-        T1Job j{};
-        j.sym = const_cast<uint8_t *>(syms) + (size_t)b * symcap;
-        j.symcap = symcap;
-        j.cm = const_cast<i32x4 *>(cm + b);
-        j.poff = const_cast<int32_t *>(poffs) + (size_t)b * MAXP;
-        j.seg = arena + (size_t)b * astride;
-        j.seglen = seglens + (size_t)b * MAXP;
-        return j;
-    }
-};
-// Job b of a job list (batched path).
-struct ListJobs {
-    const T1Job *jobs;
-    __device__ const T1Job &operator()(int b) const { return jobs[b]; }
-};
-
-__global__ void __launch_bounds__(256) chain_zero_kernel(int *ws)
-{
-    for (int i = threadIdx.x; i < CH_LIST; i += 256) ws[i] = 0;
-}
-
-// Histogram of chain keys, privatised per workgroup in LDS.
-template <class J>
-__global__ void __launch_bounds__(256) chain_hist_kernel(J job, int nblocks, int style, int *ws)
-{
-    __shared__ int h[CH_HIST];
-    for (int k = threadIdx.x; k < CH_HIST; k += 256) h[k] = 0;
-    __syncthreads();
-    const long i = (long)blockIdx.x * 256 + threadIdx.x;
-    if (i < (long)nblocks * MAXP) {
-        const T1Job &jb = job((int)(i / MAXP));
-        const int key = chain_key(*jb.cm, jb.poff, (int)(i % MAXP), style);
-        if (key >= 0) atomicAdd(&h[key], 1);
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < CH_HIST; k += 256)
-        if (h[k]) atomicAdd(&ws[k], h[k]);
-}
-
-// Exclusive prefix sum of the histogram into the cursors; list lengths.
-__global__ void __launch_bounds__(CH_HIST) chain_scan_kernel(int *ws)
-{
-    __shared__ int a[CH_HIST];
-    const int t = threadIdx.x;
-    const int v = ws[t];
-    a[t] = v;
-    __syncthreads();
-    for (int o = 1; o < CH_HIST; o <<= 1) {
-        const int x = t >= o ? a[t - o] : 0;
-        __syncthreads();
-        a[t] += x;
-        __syncthreads();
-    }
-    ws[CH_HIST + t] = a[t] - v;
-    if (t == CH_BUCKETS - 1) ws[CH_COUNTS] = a[t];
-    if (t == CH_HIST - 1) ws[CH_COUNTS + 1] = a[t] - a[CH_BUCKETS - 1];
-}
-
-// Places every chain id at its bucket's cursor: one global atomic per
-// (workgroup, bucket) reserves a range, LDS atomics number the entries in it.
-template <class J>
-__global__ void __launch_bounds__(256) chain_scatter_kernel(J job, int nblocks, int style, int *ws)
-{
-    __shared__ int h[CH_HIST];
-    for (int k = threadIdx.x; k < CH_HIST; k += 256) h[k] = 0;
-    __syncthreads();
-    const long i = (long)blockIdx.x * 256 + threadIdx.x;
-    int key = -1, local = 0;
-    if (i < (long)nblocks * MAXP) {
-        const T1Job &jb = job((int)(i / MAXP));
-        key = chain_key(*jb.cm, jb.poff, (int)(i % MAXP), style);
-        if (key >= 0) local = atomicAdd(&h[key], 1);
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < CH_HIST; k += 256)
-        if (h[k]) h[k] = atomicAdd(&ws[CH_HIST + k], h[k]);
-    __syncthreads();
-    if (key >= 0) ws[CH_LIST + h[key] + local] = (int)i;
-}
-
-// Codes the sorted lists: MQ wavefronts first (longest chains first), then
-// raw ones; workgroups past the end of the lists exit.
-template <class J>
-__global__ void __launch_bounds__(64) chain_code_kernel(J job, const int *ws)
-{
-    __shared__ uint32_t tab[48];
-    __shared__ uint8_t ring[RING_BYTES];
-    const int nmq = ws[CH_COUNTS], ntot = nmq + ws[CH_COUNTS + 1];
-    const int mqw = (nmq + 63) >> 6;
-    const int w = (int)blockIdx.x;
-    const bool raw = w >= mqw;
-    const int base = raw ? nmq + (w - mqw) * 64 : w * 64;
-    const int end = raw ? ntot : nmq;
-    if (base >= end) return;
-    // The kernel lasts as long as its longest chain, whose wavefront shares a
-    // SIMD with shorter ones: issue priority by the length of the wavefront's
-    // first (longest) chain lets the long chains run at their own latency
-    // while short wavefronts fill their stalls.
-    {
-        const int id = ws[CH_LIST + base];
-        const T1Job &jb = job(id / MAXP);
-        int p, s0, s1, at, cap;
-        const int len = pass_range(*jb.cm, jb.poff, id % MAXP, p, s0, s1, at, cap) ? s1 - s0 : 0;
-        if (len >= 2048) __builtin_amdgcn_s_setprio(3);
-        else if (len >= 1024) __builtin_amdgcn_s_setprio(2);
-        else if (len >= 256) __builtin_amdgcn_s_setprio(1);
-    }
-    t1_chain_lane(tab, ring, raw, ws + CH_LIST, base, end, job);
-}
-
-template <class J>
-int launch_chains(hipStream_t s, const J &job, int nblocks, int style, int *ws)
-{
-    const long n = (long)nblocks * MAXP;
-    const unsigned g = (unsigned)((n + 255) / 256);
-    chain_zero_kernel<<<1, 256, 0, s>>>(ws);
-    RG_LAUNCH_CHECK();
-    chain_hist_kernel<<<g, 256, 0, s>>>(job, nblocks, style, ws);
-    RG_LAUNCH_CHECK();
-    chain_scan_kernel<<<1, CH_HIST, 0, s>>>(ws);
-    RG_LAUNCH_CHECK();
-    chain_scatter_kernel<<<g, 256, 0, s>>>(job, nblocks, style, ws);
-    RG_LAUNCH_CHECK();
-    // at most nblocks * (MAXP - 2) chains; +1 wavefront for the raw list's
-    // unaligned start
-    chain_code_kernel<<<(unsigned)(((long)nblocks * (MAXP - 2) + 63) / 64 + 2), 64, 0, s>>>(job, ws);
-    RG_LAUNCH_CHECK();
-    return RG_SUCCESS;
 }
 
 __global__ void __launch_bounds__(64) t1_pack_kernel(const int4 *__restrict__ cm, const int32_t *__restrict__ poffs,
@@ -333,26 +140,42 @@ __global__ void __launch_bounds__(64) t1_pack_jobs_kernel(const T1Job *__restric
 
 int rg_t1_maxp() { return MAXP; }
 size_t rg_t1_seg_bytes(int symcap) { return (size_t)seg_bytes(symcap); }
-size_t rg_t1_chain_ws_bytes(int nblocks) { return sizeof(int) * ((size_t)CH_LIST + (size_t)nblocks * MAXP); }
 
 int rg_launch_t1_parallel(hipStream_t s, const int32_t *q, int pitch, size_t plane_stride, const rg_cb_desc *desc,
                           int ncb, uint8_t *syms, int symcap, int4 *cm, int32_t *poffs, uint8_t *arena,
                           int32_t *seglens, uint8_t *out, int slot_bytes, rg_cb_result *res, int style,
-                          int32_t *chain_ws)
+                          int32_t *chain_ws, const rg_t1_overlap *ov)
 {
     if (ncb <= 0) return RG_SUCCESS;
+    if (!chain_ws) return rg_fail(RG_ERR_INVALID, "parallel code-block styles need the chain workspace");
     const size_t astride = rg_t1_seg_bytes(symcap);
-    t1_cm_kernel<<<dim3((unsigned)ncb), dim3(64), 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm, poffs,
-                                                          style);
-    RG_LAUNCH_CHECK();
-    if (chain_ws) {
-        const FlatJobs fj{syms, symcap, cm, poffs, arena, astride, seglens};
-        if (int rc = launch_chains(s, fj, ncb, style, chain_ws)) return rc;
-    } else {
-        t1_mqp_kernel<<<dim3((unsigned)((ncb + 63) / 64), (unsigned)(MAXP - 2)), dim3(64), 0, s>>>(
-            syms, symcap, cm, poffs, ncb, arena, astride, seglens, style);
+    const dim3 g((unsigned)ncb), b(64);
+    const rg_t1_flat fj{syms, symcap, cm, poffs, arena, astride, seglens};
+    if (ov) {
+        // Two phases overlapped: passes 0..9 (the first four bit-planes, where
+        // every code-block's longest MQ chains are) are modelled first and
+        // coded on the second stream while the first models the remaining
+        // passes and codes their (raw or short) chains; packing joins both.
+        t1_cm_kernel<<<g, b, 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm, poffs, style, 0,
+                                     RG_T1_PHASE_SPLIT, (uint4 *)ov->cms);
         RG_LAUNCH_CHECK();
+        RG_HIP(hipEventRecord(ov->ev_a, s));
+        RG_HIP(hipStreamWaitEvent(ov->s2, ov->ev_a, 0));
+        if (int rc = rg_launch_chains_flat(ov->s2, fj, ncb, style, ov->chain_ws2, 0, RG_T1_PHASE_SPLIT)) return rc;
+        RG_HIP(hipEventRecord(ov->ev_b, ov->s2));
+        t1_cm_kernel<<<g, b, 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm, poffs, style,
+                                     RG_T1_PHASE_SPLIT, 1 << 30, (uint4 *)ov->cms);
+        RG_LAUNCH_CHECK();
+        if (int rc = rg_launch_chains_flat(s, fj, ncb, style, chain_ws, RG_T1_PHASE_SPLIT, MAXP)) return rc;
+        RG_HIP(hipStreamWaitEvent(s, ov->ev_b, 0));
+        t1_pack_kernel<<<g, b, 0, s>>>(cm, poffs, arena, astride, seglens, out, slot_bytes, res);
+        RG_LAUNCH_CHECK();
+        return RG_SUCCESS;
     }
+    t1_cm_kernel<<<g, b, 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm, poffs, style, 0, 1 << 30,
+                                 nullptr);
+    RG_LAUNCH_CHECK();
+    if (int rc = rg_launch_chains_flat(s, fj, ncb, style, chain_ws, 0, MAXP)) return rc;
     t1_pack_kernel<<<dim3((unsigned)ncb), dim3(64), 0, s>>>(cm, poffs, arena, astride, seglens, out, slot_bytes, res);
     RG_LAUNCH_CHECK();
     return RG_SUCCESS;
@@ -361,15 +184,10 @@ int rg_launch_t1_parallel(hipStream_t s, const int32_t *q, int pitch, size_t pla
 int rg_launch_t1_jobs_parallel(hipStream_t s, const void *jobs, int njobs, int style, int32_t *chain_ws)
 {
     if (njobs <= 0) return RG_SUCCESS;
+    if (!chain_ws) return rg_fail(RG_ERR_INVALID, "parallel code-block styles need the chain workspace");
     t1_cm_jobs_kernel<<<dim3((unsigned)njobs), dim3(64), 0, s>>>((const T1Job *)jobs);
     RG_LAUNCH_CHECK();
-    if (chain_ws) {
-        if (int rc = launch_chains(s, ListJobs{(const T1Job *)jobs}, njobs, style, chain_ws)) return rc;
-    } else {
-        t1_mqp_jobs_kernel<<<dim3((unsigned)((njobs + 63) / 64), (unsigned)(MAXP - 2)), dim3(64), 0, s>>>(
-            (const T1Job *)jobs, njobs, style);
-        RG_LAUNCH_CHECK();
-    }
+    if (int rc = rg_launch_chains_jobs(s, jobs, njobs, style, chain_ws, 0, MAXP)) return rc;
     t1_pack_jobs_kernel<<<dim3((unsigned)njobs), dim3(64), 0, s>>>((const T1Job *)jobs);
     RG_LAUNCH_CHECK();
     return RG_SUCCESS;
@@ -383,11 +201,9 @@ int rg_launch_t1_split(hipStream_t s, const int32_t *q, int pitch, size_t plane_
 {
     if (ncb <= 0) return RG_SUCCESS;
     t1_cm_kernel<<<dim3((unsigned)ncb), dim3(64), 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm, nullptr,
-                                                          0);
+                                                          0, 0, 1 << 30, nullptr);
     RG_LAUNCH_CHECK();
-    t1_mq_kernel<<<dim3((unsigned)((ncb + 63) / 64)), dim3(64), 0, s>>>(syms, symcap, cm, ncb, out, slot_bytes, res);
-    RG_LAUNCH_CHECK();
-    return RG_SUCCESS;
+    return rg_launch_t1_mq(s, syms, symcap, cm, ncb, out, slot_bytes, res);
 }
 
 int rg_launch_t1_jobs_split(hipStream_t s, const void *jobs, int njobs)
@@ -395,9 +211,7 @@ int rg_launch_t1_jobs_split(hipStream_t s, const void *jobs, int njobs)
     if (njobs <= 0) return RG_SUCCESS;
     t1_cm_jobs_kernel<<<dim3((unsigned)njobs), dim3(64), 0, s>>>((const T1Job *)jobs);
     RG_LAUNCH_CHECK();
-    t1_mq_jobs_kernel<<<dim3((unsigned)((njobs + 63) / 64)), dim3(64), 0, s>>>((const T1Job *)jobs, njobs);
-    RG_LAUNCH_CHECK();
-    return RG_SUCCESS;
+    return rg_launch_t1_mq_jobs(s, jobs, njobs);
 }
 
 int rg_t1_job_bytes() { return (int)sizeof(T1Job); }
@@ -408,6 +222,8 @@ int rg_launch_t1_queue(hipStream_t s, const void *jobs, int njobs, int style, in
                        int32_t *chain_ws)
 {
     if (njobs <= 0) return RG_SUCCESS;
+    if ((style & RG_STYLE_TERMALL) && !chain_ws)
+        return rg_fail(RG_ERR_INVALID, "parallel code-block styles need the chain workspace");
     reset_counter_kernel<<<1, 1, 0, s>>>(counter);
     RG_LAUNCH_CHECK();
     const dim3 g((unsigned)nwaves), b(64);
@@ -415,21 +231,14 @@ int rg_launch_t1_queue(hipStream_t s, const void *jobs, int njobs, int style, in
     if (style & RG_STYLE_TERMALL) {
         t1_cm_queue_kernel<<<g, b, 0, s>>>(J, njobs, counter);
         RG_LAUNCH_CHECK();
-        if (chain_ws) {
-            if (int rc = launch_chains(s, ListJobs{J}, njobs, style, chain_ws)) return rc;
-        } else {
-            t1_mqp_jobs_kernel<<<dim3((unsigned)((njobs + 63) / 64), (unsigned)(MAXP - 2)), b, 0, s>>>(J, njobs,
-                                                                                                      style);
-            RG_LAUNCH_CHECK();
-        }
-        RG_LAUNCH_CHECK();
+        if (int rc = rg_launch_chains_jobs(s, jobs, njobs, style, chain_ws, 0, MAXP)) return rc;
         t1_pack_jobs_kernel<<<dim3((unsigned)njobs), b, 0, s>>>(J);
     } else if (rg_t1_inline()) {
         t1_encode_queue_kernel<<<g, b, 0, s>>>(J, njobs, counter);
     } else {
         t1_cm_queue_kernel<<<g, b, 0, s>>>(J, njobs, counter);
         RG_LAUNCH_CHECK();
-        t1_mq_jobs_kernel<<<dim3((unsigned)((njobs + 63) / 64)), b, 0, s>>>(J, njobs);
+        return rg_launch_t1_mq_jobs(s, jobs, njobs);
     }
     RG_LAUNCH_CHECK();
     return RG_SUCCESS;
@@ -452,6 +261,17 @@ void rg_t1_job_fill(void *dst, const int32_t *src, int pitch, int w, int h, int 
     j.src = src; j.pitch = pitch; j.w = w; j.h = h; j.orient = orient; j.cap = cap; j.out = out; j.res = res;
     j.sym = sym; j.symcap = symcap; j.style = style; j.cm = cm; j.poff = poff; j.seg = seg; j.seglen = seglen;
     *reinterpret_cast<T1Job *>(dst) = j;
+}
+
+// Overlapped two-phase tier-1 for the parallel styles unless RG_T1_TWO_PHASE=0.
+bool rg_t1_two_phase()
+{
+    static int v = -1;
+    if (v < 0) {
+        const char *e = getenv("RG_T1_TWO_PHASE");
+        v = (e && e[0] == '0') ? 0 : 1;
+    }
+    return v == 1;
 }
 
 // Tier-1 variant: the split context-model + lane-MQ path unless RG_T1_INLINE=1

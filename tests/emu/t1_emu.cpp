@@ -3,7 +3,7 @@
 // (not a restatement) can be checked against the oracle without a GPU.
 //
 //   t1_emu MODE DIR PITCH PLANE_STRIDE SLOT [STYLE]
-//     MODE  inline | split | queue_split | parallel | queue_parallel | sorted | cm_only
+//     MODE  inline | split | queue_split | parallel | queue_parallel | sorted | two_phase | cm_only
 //     STYLE code-block style bits of the parallel modes (default 6 =
 //           RESET|TERMALL; 7 adds BYPASS)
 //     DIR   holds q.bin (int32 coefficient planes) and desc.bin (rg_cb_desc
@@ -19,6 +19,8 @@
 //   sorted       parallel styles with the chains in the longest-first lists
 //                (built here with the kernels' chain_key) coded 64 per
 //                wavefront by t1_chain_lane, as chain_code_kernel does
+//   two_phase    sorted, with context modelling split at pass 10: passes
+//                0-9 modelled and coded, then the rest from the saved state
 //   cm_only      context modelling alone; writes cm.bin and poff.bin
 // The parallel modes also write seglen.bin (MAXP int32 per block).
 // Exit status 3: an emulator check failed (message on stderr).
@@ -146,6 +148,8 @@ uint8_t g_ring[RING_BYTES]; // and the lane MQ coders' output rings
 
 void inline_body(void *a) { t1_block<false>(*static_cast<T1Job *>(a), g_lut); }
 void cm_body(void *a) { t1_block<true>(*static_cast<T1Job *>(a), g_lut); }
+CMPhase g_phase;
+void cm_phase_body(void *a) { t1_block<true>(*static_cast<T1Job *>(a), g_lut, g_phase); }
 
 struct MQFlat {
     int i0, ncb, symcap, slot;
@@ -252,6 +256,7 @@ int main(int argc, char **argv)
     std::vector<rg_cb_result> res(ncb);
     uint8_t *syms = static_cast<uint8_t *>(aligned_alloc(16, (size_t)symcap * (ncb ? ncb : 1)));
     std::vector<i32x4> cm(ncb);
+    std::vector<u32x4> cms((size_t)64 * (ncb ? ncb : 1));
     std::vector<T1Job> jobs(ncb);
     const size_t astride = (size_t)seg_bytes(symcap);
     uint8_t *arena = static_cast<uint8_t *>(aligned_alloc(16, astride * (ncb ? ncb : 1)));
@@ -266,11 +271,12 @@ int main(int argc, char **argv)
         j.sym = syms + (size_t)k * symcap;
         j.symcap = symcap;
         j.cm = &cm[k];
-        const bool par = mode == "parallel" || mode == "queue_parallel" || mode == "sorted";
+        const bool par = mode == "parallel" || mode == "queue_parallel" || mode == "sorted" || mode == "two_phase";
         j.style = par ? style : 0;
         j.poff = par ? &poffs[(size_t)k * MAXP] : nullptr;
         j.seg = arena + (size_t)k * astride;
         j.seglen = &seglens[(size_t)k * MAXP];
+        j.cms = &cms[(size_t)k * 64];
     }
     if (mode == "inline") {
         for (int k = 0; k < ncb; ++k) emu::run(inline_body, &jobs[k]);
@@ -292,29 +298,42 @@ int main(int argc, char **argv)
         for (int k = 0; k < ncb; ++k) emu::run(cm_body, &jobs[k]);
         spill(dir + "/cm.bin", cm.data(), sizeof(i32x4) * cm.size());
         spill(dir + "/poff.bin", poffs.data(), sizeof(int32_t) * poffs.size());
-    } else if (mode == "sorted") {
-        for (int k = 0; k < ncb; ++k) emu::run(cm_body, &jobs[k]);
-        // counting sort by chain_key: MQ buckets (longest first), then raw ones
-        std::vector<std::vector<int32_t>> bucket(CH_HIST);
-        for (int k = 0; k < ncb; ++k)
-            for (int q = 0; q < MAXP; ++q) {
-                const int key = chain_key(cm[k], jobs[k].poff, q, style);
-                if (key >= 0) bucket[key].push_back(k * MAXP + q);
+    } else if (mode == "sorted" || mode == "two_phase") {
+        // counting sort by chain_key over passes [qlo, qhi): MQ buckets
+        // (longest first), then raw ones; 64 list entries per wavefront
+        auto code_chains = [&](int qlo, int qhi) {
+            std::vector<std::vector<int32_t>> bucket(CH_HIST);
+            for (int k = 0; k < ncb; ++k)
+                for (int q = 0; q < MAXP; ++q) {
+                    const int key = chain_key(cm[k], jobs[k].poff, q, style, qlo, qhi);
+                    if (key >= 0) bucket[key].push_back(k * MAXP + q);
+                }
+            std::vector<int32_t> list;
+            int nmq = 0;
+            for (int key = 0; key < CH_HIST; ++key) {
+                list.insert(list.end(), bucket[key].begin(), bucket[key].end());
+                if (key == CH_BUCKETS - 1) nmq = (int)list.size();
             }
-        std::vector<int32_t> list;
-        int nmq = 0;
-        for (int key = 0; key < CH_HIST; ++key) {
-            list.insert(list.end(), bucket[key].begin(), bucket[key].end());
-            if (key == CH_BUCKETS - 1) nmq = (int)list.size();
-        }
-        const int ntot = (int)list.size();
-        for (int base = 0; base < nmq; base += 64) {
-            ChainRun c{jobs.data(), list.data(), base, nmq, false};
-            emu::run(chain_body, &c);
-        }
-        for (int base = nmq; base < ntot; base += 64) {
-            ChainRun c{jobs.data(), list.data(), base, ntot, true};
-            emu::run(chain_body, &c);
+            const int ntot = (int)list.size();
+            for (int base = 0; base < nmq; base += 64) {
+                ChainRun c{jobs.data(), list.data(), base, nmq, false};
+                emu::run(chain_body, &c);
+            }
+            for (int base = nmq; base < ntot; base += 64) {
+                ChainRun c{jobs.data(), list.data(), base, ntot, true};
+                emu::run(chain_body, &c);
+            }
+        };
+        if (mode == "sorted") {
+            for (int k = 0; k < ncb; ++k) emu::run(cm_body, &jobs[k]);
+            code_chains(0, MAXP);
+        } else {
+            g_phase.qbeg = 0; g_phase.qend = 10;
+            for (int k = 0; k < ncb; ++k) emu::run(cm_phase_body, &jobs[k]);
+            code_chains(0, 10);
+            g_phase.qbeg = 10; g_phase.qend = 1 << 30;
+            for (int k = 0; k < ncb; ++k) emu::run(cm_phase_body, &jobs[k]);
+            code_chains(10, MAXP);
         }
         for (int k = 0; k < ncb; ++k) emu::run(pack_body, &jobs[k]);
         spill(dir + "/seglen.bin", seglens.data(), sizeof(int32_t) * seglens.size());
