@@ -217,30 +217,45 @@ def main():
                                      d_src=d_src, w=W, h=H, ncomp=NC, style=st)
     t_enc_par = timed(enc_style(RG_STYLE_PARALLEL))
     t_enc_byp = timed(enc_style(RG_STYLE_PARALLEL_BYPASS))
-    # reverse path on device-resident standard-style codewords of the same image
-    if STYLE:
-        enc.encode_u8(None, levels=LEVELS, reversible=False, cbw=CB, cbh=CB, delta=DELTA, d_src=d_src, w=W, h=H,
-                      ncomp=NC, style=0)
-    _, res0, blobs0 = enc.codeblocks()
-    data0, offs0 = R.pack_codewords(blobs0)
-    d_data0, d_offs0 = ctx.upload(data0), ctx.upload(offs0)
-    d_res0 = ctx.upload(np.ascontiguousarray(res0).view(np.int32))
+    # reverse path on device-resident codewords of the same image: the
+    # headline style's (with its per-pass segment lengths) and the standard
+    # style's
+    from roger_amd._lib import check as check_rc
     d_qd = ctx.alloc(NC * W * H * 4)
     d_rec = ctx.alloc(NC * W * H * 4)
     d_u8 = ctx.alloc(NC * W * H)
     step_f = R.band_steps(LEVELS, DELTA)[0]
 
-    def decode_once():
-        check_rc(lib().rg_t1_decode(ctx.handle, ctypes.c_void_p(d_data0.ptr), ctypes.c_void_p(d_offs0.ptr),
-                                    ctypes.c_void_p(d_res0.ptr), ctypes.c_void_p(d_desc.ptr), len(desc),
-                                    ctypes.c_void_p(d_qd.ptr), W, W * H))
-        R.dwt_inv(ctx, RG_FILTER_97, d_qd, W, H, NC, LEVELS, step_f, d_rec, d_s)
-        R.colour_inv(ctx, d_rec, W, H, NC, RG_COLOUR_ICT, 1, d_u8)
+    def decode_rate(style):
+        enc.encode_u8(None, levels=LEVELS, reversible=False, cbw=CB, cbh=CB, delta=DELTA, d_src=d_src, w=W, h=H,
+                      ncomp=NC, style=style)
+        _, res_s, blobs_s = enc.codeblocks()
+        data_s, offs_s = R.pack_codewords(blobs_s)
+        d_data, d_offs = ctx.upload(data_s), ctx.upload(offs_s)
+        d_res_s = ctx.upload(np.ascontiguousarray(res_s).view(np.int32))
+        d_seg = ctx.upload(np.ascontiguousarray(enc.pass_lengths(), np.int32)) if style else None
 
-    from roger_amd._lib import check as check_rc
-    t_dec = timed(decode_once)
-    rec = d_u8.read(np.uint8, img.shape)
-    dec_maxerr = int(np.abs(rec.astype(int) - img.astype(int)).max())
+        def decode_once():
+            check_rc(lib().rg_t1_decode_style(ctx.handle, ctypes.c_void_p(d_data.ptr), ctypes.c_void_p(d_offs.ptr),
+                                              ctypes.c_void_p(d_res_s.ptr), ctypes.c_void_p(d_desc.ptr), len(desc),
+                                              ctypes.c_void_p(d_qd.ptr), W, W * H, style,
+                                              ctypes.c_void_p(d_seg.ptr if d_seg is not None else 0)))
+            R.dwt_inv(ctx, RG_FILTER_97, d_qd, W, H, NC, LEVELS, step_f, d_rec, d_s)
+            R.colour_inv(ctx, d_rec, W, H, NC, RG_COLOUR_ICT, 1, d_u8)
+
+        t_t1d = timed(lambda: check_rc(lib().rg_t1_decode_style(
+            ctx.handle, ctypes.c_void_p(d_data.ptr), ctypes.c_void_p(d_offs.ptr), ctypes.c_void_p(d_res_s.ptr),
+            ctypes.c_void_p(d_desc.ptr), len(desc), ctypes.c_void_p(d_qd.ptr), W, W * H, style,
+            ctypes.c_void_p(d_seg.ptr if d_seg is not None else 0))))
+        t = timed(decode_once)
+        rec = d_u8.read(np.uint8, img.shape)
+        for b in (d_data, d_offs, d_res_s) + ((d_seg,) if d_seg is not None else ()):
+            b.free()
+        return {"ms": round(t, 4), "mpix_s": round(mpix / (t * 1e-3), 1), "t1_decode_ms": round(t_t1d, 4),
+                "max_abs_err_vs_input": int(np.abs(rec.astype(int) - img.astype(int)).max())}
+
+    dec_main = decode_rate(STYLE)
+    dec_std = decode_rate(0) if STYLE else dec_main
     # measured device-to-device copy ceiling (read + write of the same bytes as the level-1 DWT)
     cp_n = 4 * W * H * NC
     d_cp = ctx.alloc(cp_n)
@@ -283,10 +298,9 @@ def main():
                       "encode_parallel_style": round(t_enc_par, 4),
                       "encode_bypass_style": round(t_enc_byp, 4)},
         "styles": styles,
-        "decode": {"what": "device-resident standard-style codewords -> tier-1 decode -> dequantisation + "
-                           "inverse 9/7 -> inverse ICT -> uint8 RGB",
-                   "ms": round(t_dec, 4), "mpix_s": round(mpix / (t_dec * 1e-3), 1),
-                   "max_abs_err_vs_input": dec_maxerr},
+        "decode": dict(what="device-resident codewords of the same code-block style -> tier-1 decode -> "
+                            "dequantisation + inverse 9/7 -> inverse ICT -> uint8 RGB", **dec_main,
+                       standard_style=dec_std),
     }
 
     if not args.no_batch:

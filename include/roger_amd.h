@@ -259,6 +259,14 @@ int rg_decoder_unmap_out(rg_decoder *dec, void *host_ptr);
  * its decoder never decodes bits, src/OCLDecoder.cpp:35-38). */
 int rg_t1_decode(rg_context *ctx, const uint8_t *d_data, const long *d_offsets, const rg_cb_result *d_res,
                  const rg_cb_desc *d_desc, int ncb, int32_t *d_q, int pitch, size_t plane_stride);
+/* rg_t1_decode for a code-block style (0, RG_STYLE_PARALLEL or
+ * RG_STYLE_PARALLEL_BYPASS); with TERMALL, d_seglen (device) holds each
+ * block's pass segment lengths, RG_MAX_PASSES per block, as
+ * rg_encoder_get_pass_lengths returns them.  No counterpart in the
+ * reference (see rg_t1_decode). */
+int rg_t1_decode_style(rg_context *ctx, const uint8_t *d_data, const long *d_offsets, const rg_cb_result *d_res,
+                       const rg_cb_desc *d_desc, int ncb, int32_t *d_q, int pitch, size_t plane_stride, int style,
+                       const int32_t *d_seglen);
 /* Full native decode of one image from its code-blocks (host arrays, the
  * layout rg_encoder_get_codeblocks returns): tier-1 decode -> dequantisation
  * + inverse DWT -> inverse RCT/ICT -> interleaved uint8 on the device.

@@ -114,6 +114,7 @@ SIGNATURES = {
     "rg_decoder_idwt": (I, [P, I]),
     "rg_decoder_map_out": (I, [P, P]),
     "rg_t1_decode": (I, [P, P, P, P, P, I, P, I, SZ]),
+    "rg_t1_decode_style": (I, [P, P, P, P, P, I, P, I, SZ, I, P]),
     "rg_decoder_decode_codeblocks": (I, [P, P, P, P, P, L, I, I, I, I, P, P]),
     "rg_decoder_image_u8": (I, [P, P]),
     "rg_decoder_coefficients": (I, [P, P]),

@@ -872,6 +872,19 @@ extern "C" int rg_t1_decode(rg_context *c, const uint8_t *d_data, const long *d_
     return rg_launch_t1_decode(c->stream, d_data, d_offsets, d_res, d_desc, ncb, d_q, pitch, plane_stride, 0, nullptr);
 }
 
+extern "C" int rg_t1_decode_style(rg_context *c, const uint8_t *d_data, const long *d_offsets,
+                                  const rg_cb_result *d_res, const rg_cb_desc *d_desc, int ncb, int32_t *d_q, int pitch,
+                                  size_t plane_stride, int style, const int32_t *d_seglen)
+{
+    RG_CHECK(c && d_data && d_offsets && d_res && d_desc && d_q, "rg_t1_decode_style: null argument");
+    RG_CHECK(ncb >= 0 && pitch > 0, "rg_t1_decode_style: bad sizes");
+    RG_CHECK(style == 0 || style == RG_STYLE_PARALLEL || style == RG_STYLE_PARALLEL_BYPASS,
+             "code-block style %d: only 0, RG_STYLE_PARALLEL and RG_STYLE_PARALLEL_BYPASS", style);
+    RG_CHECK(!(style & RG_STYLE_TERMALL) || d_seglen, "TERMALL style needs the per-pass segment lengths");
+    return rg_launch_t1_decode(c->stream, d_data, d_offsets, d_res, d_desc, ncb, d_q, pitch, plane_stride, style,
+                               (style & RG_STYLE_TERMALL) ? d_seglen : nullptr);
+}
+
 extern "C" int rg_decoder_decode_codeblocks(rg_decoder *d, const rg_cb_desc *desc, const rg_cb_result *res,
                                             const long *offsets, const uint8_t *data, long total, int ncb, int w,
                                             int h, int ncomp, const rg_encode_params *p, const int32_t *seglen)

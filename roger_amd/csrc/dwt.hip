@@ -9,12 +9,14 @@
 // passes through, 9/7 low band x 1/K and high band x K, every 9/7 lifting
 // step x[i] = fma(c, x[i-1] + x[i+1], x[i]).
 //
-// Kernel design (gfx950): one 256-thread workgroup per 64x64 tile of the
-// level's input region.  Forward:
+// Kernel design (gfx950): one 320-thread (five-wavefront) workgroup per 64x64
+// tile of the level's input region.  Forward:
 //   1. vertical pass straight from HBM: work item (column, 16-row segment)
 //      loads its segment plus a 4-row (9/7) / 2-row (5/3) halo with coalesced
 //      row-contiguous loads, runs all lifting steps in registers, and stores
 //      the 16 results into an LDS tile (64 rows x 72 columns);
+//      -- the 72 (9/7) or 68 (5/3) halo'd columns x 4 segments are one round
+//      of the 320 work items, so every load of the tile is issued at once;
 //   2. one barrier;
 //   3. horizontal pass from LDS: work item (row, 16-column segment) lifts in
 //      registers and writes its 8 low + 8 high outputs straight to their
@@ -31,6 +33,9 @@ namespace {
 constexpr int TILE = 64;
 constexpr int SEG = 16;
 constexpr int NSEG = TILE / SEG;
+constexpr int WG = 320;                    // >= (TILE + 2 * 4) * NSEG: one round of column segments
+constexpr int ROWT = TILE * NSEG;          // work items of the row passes (the first 256 threads)
+static_assert(WG >= (TILE + 8) * NSEG && WG >= ROWT, "one round of halo'd column segments");
 
 constexpr float A97 = (float)-1.586134342059924;
 constexpr float B97 = (float)-0.052980118572961;
@@ -181,8 +186,9 @@ __device__ __forceinline__ void fwd_tile(const rg_fwd_level_args &a, int plane, 
 
     // ---- horizontal pass from LDS: (segment k, row r), k fastest ----
     T o[SEG];
-    const int kk = threadIdx.x % NSEG, rr = threadIdx.x / NSEG;   // TILE * NSEG == blockDim.x
-    {
+    const int kk = threadIdx.x % NSEG, rr = threadIdx.x / NSEG;   // rr < TILE for the first ROWT threads
+    const bool rowt = threadIdx.x < ROWT;
+    if (rowt) {
         T e[N];
 #pragma unroll
         for (int j = 0; j < N; ++j) e[j] = tile[rr * LP + kk * SEG + j];
@@ -194,8 +200,10 @@ __device__ __forceinline__ void fwd_tile(const rg_fwd_level_args &a, int plane, 
     }
     __syncthreads();
     // deinterleave the row into LDS: low outputs to [0, 32), high to [32, 64)
+    if (rowt) {
 #pragma unroll
-    for (int j = 0; j < SEG; ++j) tile[rr * LP + (j & 1) * (TILE / 2) + kk * (SEG / 2) + (j >> 1)] = o[j];
+        for (int j = 0; j < SEG; ++j) tile[rr * LP + (j & 1) * (TILE / 2) + kk * (SEG / 2) + (j >> 1)] = o[j];
+    }
     __syncthreads();
 
     // ---- write out: each wavefront stores one row: 32 low + 32 high values ----
@@ -204,7 +212,8 @@ __device__ __forceinline__ void fwd_tile(const rg_fwd_level_args &a, int plane, 
     if (interior) {
         // a thread keeps one column c and rows r0, r0+4, ...: its band, its
         // step and the parity of its rows never change (y0 is even)
-        const int c = threadIdx.x % TILE, r0 = threadIdx.x / TILE;   // 256 threads: r0 in 0..3
+        if (!rowt) return;
+        const int c = threadIdx.x % TILE, r0 = threadIdx.x / TILE;   // r0 in 0..3
         const bool high = c >= TILE / 2, lowrow = !(r0 & 1);
         const int xo = (x0 >> 1) + (c & (TILE / 2 - 1));
         const int yh = (y0 + r0) >> 1;                                  // row inside the half band
@@ -363,7 +372,7 @@ __device__ __forceinline__ void xcd_tile(int &plane, int &bx, int &by)
 }
 
 template <int F>
-__global__ void __launch_bounds__(256) fwd_level_kernel(rg_fwd_level_args a)
+__global__ void __launch_bounds__(WG) fwd_level_kernel(rg_fwd_level_args a)
 {
     __shared__ __attribute__((aligned(16))) char lds[FWD_LDS_BYTES];
     int plane, bx, by;
@@ -372,7 +381,7 @@ __global__ void __launch_bounds__(256) fwd_level_kernel(rg_fwd_level_args a)
 }
 
 template <int F>
-__global__ void __launch_bounds__(256) inv_level_kernel(rg_inv_level_args a)
+__global__ void __launch_bounds__(WG) inv_level_kernel(rg_inv_level_args a)
 {
     __shared__ __attribute__((aligned(16))) char lds[INV_LDS_BYTES];
     int plane, bx, by;
@@ -382,7 +391,7 @@ __global__ void __launch_bounds__(256) inv_level_kernel(rg_inv_level_args a)
 
 // Batched: one workgroup per job {image args index, plane, tile x, tile y};
 // images of both filters share the launch (the filter is uniform per job).
-__global__ void __launch_bounds__(256) fwd_level_batch_kernel(const rg_fwd_level_args *__restrict__ args,
+__global__ void __launch_bounds__(WG) fwd_level_batch_kernel(const rg_fwd_level_args *__restrict__ args,
                                                               const int4 *__restrict__ jobs)
 {
     __shared__ __attribute__((aligned(16))) char lds[FWD_LDS_BYTES];
@@ -392,7 +401,7 @@ __global__ void __launch_bounds__(256) fwd_level_batch_kernel(const rg_fwd_level
     else fwd_tile<RG_FILTER_97>(a, j.y, j.z, j.w, lds);
 }
 
-__global__ void __launch_bounds__(256) inv_level_batch_kernel(const rg_inv_level_args *__restrict__ args,
+__global__ void __launch_bounds__(WG) inv_level_batch_kernel(const rg_inv_level_args *__restrict__ args,
                                                               const int4 *__restrict__ jobs)
 {
     __shared__ __attribute__((aligned(16))) char lds[INV_LDS_BYTES];
@@ -407,7 +416,7 @@ __global__ void __launch_bounds__(256) inv_level_batch_kernel(const rg_inv_level
 int rg_launch_fwd_level_batch(hipStream_t s, const rg_fwd_level_args *args, const int4 *jobs, int njobs)
 {
     if (njobs <= 0) return RG_SUCCESS;
-    fwd_level_batch_kernel<<<dim3((unsigned)njobs), dim3(256), 0, s>>>(args, jobs);
+    fwd_level_batch_kernel<<<dim3((unsigned)njobs), dim3(WG), 0, s>>>(args, jobs);
     RG_LAUNCH_CHECK();
     return RG_SUCCESS;
 }
@@ -415,7 +424,7 @@ int rg_launch_fwd_level_batch(hipStream_t s, const rg_fwd_level_args *args, cons
 int rg_launch_inv_level_batch(hipStream_t s, const rg_inv_level_args *args, const int4 *jobs, int njobs)
 {
     if (njobs <= 0) return RG_SUCCESS;
-    inv_level_batch_kernel<<<dim3((unsigned)njobs), dim3(256), 0, s>>>(args, jobs);
+    inv_level_batch_kernel<<<dim3((unsigned)njobs), dim3(WG), 0, s>>>(args, jobs);
     RG_LAUNCH_CHECK();
     return RG_SUCCESS;
 }
@@ -423,7 +432,7 @@ int rg_launch_inv_level_batch(hipStream_t s, const rg_inv_level_args *args, cons
 int rg_launch_fwd53_level(hipStream_t s, const rg_fwd_level_args &a)
 {
     dim3 grid((a.cw + TILE - 1) / TILE, (a.ch + TILE - 1) / TILE, a.nplanes);
-    fwd_level_kernel<RG_FILTER_53><<<grid, dim3(256), 0, s>>>(a);
+    fwd_level_kernel<RG_FILTER_53><<<grid, dim3(WG), 0, s>>>(a);
     RG_LAUNCH_CHECK();
     return RG_SUCCESS;
 }
@@ -431,7 +440,7 @@ int rg_launch_fwd53_level(hipStream_t s, const rg_fwd_level_args &a)
 int rg_launch_fwd97_level(hipStream_t s, const rg_fwd_level_args &a)
 {
     dim3 grid((a.cw + TILE - 1) / TILE, (a.ch + TILE - 1) / TILE, a.nplanes);
-    fwd_level_kernel<RG_FILTER_97><<<grid, dim3(256), 0, s>>>(a);
+    fwd_level_kernel<RG_FILTER_97><<<grid, dim3(WG), 0, s>>>(a);
     RG_LAUNCH_CHECK();
     return RG_SUCCESS;
 }
@@ -439,7 +448,7 @@ int rg_launch_fwd97_level(hipStream_t s, const rg_fwd_level_args &a)
 int rg_launch_inv53_level(hipStream_t s, const rg_inv_level_args &a)
 {
     dim3 grid((a.cw + TILE - 1) / TILE, (a.ch + TILE - 1) / TILE, a.nplanes);
-    inv_level_kernel<RG_FILTER_53><<<grid, dim3(256), 0, s>>>(a);
+    inv_level_kernel<RG_FILTER_53><<<grid, dim3(WG), 0, s>>>(a);
     RG_LAUNCH_CHECK();
     return RG_SUCCESS;
 }
@@ -447,7 +456,7 @@ int rg_launch_inv53_level(hipStream_t s, const rg_inv_level_args &a)
 int rg_launch_inv97_level(hipStream_t s, const rg_inv_level_args &a)
 {
     dim3 grid((a.cw + TILE - 1) / TILE, (a.ch + TILE - 1) / TILE, a.nplanes);
-    inv_level_kernel<RG_FILTER_97><<<grid, dim3(256), 0, s>>>(a);
+    inv_level_kernel<RG_FILTER_97><<<grid, dim3(WG), 0, s>>>(a);
     RG_LAUNCH_CHECK();
     return RG_SUCCESS;
 }

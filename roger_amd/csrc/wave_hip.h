@@ -61,6 +61,17 @@ RG_DEV uint32_t wv_perm(uint32_t hi, uint32_t lo, uint32_t s) { return __builtin
 // Compiler scheduling fence: no instruction moves across it (used to issue
 // a memory read before independent work that should hide its latency).
 RG_DEV void wv_sched_fence() { __builtin_amdgcn_sched_barrier(0); }
+// Byte p of read-only global memory through the scalar cache (p wave-
+// uniform): the aligned dword holding it is loaded, so the read never leaves
+// the page of a valid byte.
+RG_DEV uint32_t wv_sload_u8(const uint8_t *p)
+{
+    const uintptr_t a = (uintptr_t)p;
+    const uint32_t w = *(const __attribute__((address_space(4))) uint32_t *)(a & ~(uintptr_t)3);
+    return (w >> (8 * (a & 3))) & 0xFFu;
+}
+// Atomic OR into workgroup memory.
+RG_DEV void wv_lds_or(uint32_t *p, uint32_t v) { atomicOr(p, v); }
 // Number of set bits of a 64-bit lane mask below this lane.
 RG_DEV int wv_mbcnt(uint64_t m)
 {

@@ -23,6 +23,12 @@
 //                0-9 modelled and coded, then the rest from the saved state
 //   cm_only      context modelling alone; writes cm.bin and poff.bin
 // The parallel modes also write seglen.bin (MAXP int32 per block).
+//
+//   t1_emu decode DIR PITCH PLANE_STRIDE NPLANES [STYLE]
+// runs the decoder core (t1_dec_core.h) per block, as t1_decode_kernel does:
+// DIR holds desc.bin, res.bin, cw.bin (codewords), offs.bin (int64 offset of
+// each block's codeword) and, for TERMALL styles, seglen.bin; the decoded
+// coefficients are written to dq.bin (NPLANES planes of int32).
 // Exit status 3: an emulator check failed (message on stderr).
 #include <stdint.h>
 #include <stdio.h>
@@ -34,6 +40,7 @@
 
 #include "wave_emu.h"
 #include "../../roger_amd/csrc/t1_core.h"
+#include "../../roger_amd/csrc/t1_dec_core.h"
 
 using namespace rg_t1;
 
@@ -208,6 +215,9 @@ void pack_body(void *a)
     const T1Job &j = *static_cast<T1Job *>(a);
     t1_pack_block(*j.cm, j.poff, j.seg, j.seglen, j.out, j.cap, j.res);
 }
+uint8_t g_dlut[LUT_BYTES];
+uint32_t g_rawbuf[DEC_RAW_WORDS];
+void dec_body(void *a) { t1_dec_block(*static_cast<DecJob *>(a), g_dlut, g_rawbuf); }
 void mq_queue_body(void *a)
 {
     const MQQueue &m = *static_cast<MQQueue *>(a);
@@ -241,10 +251,37 @@ int main(int argc, char **argv)
         fprintf(stderr, "usage: t1_emu MODE DIR PITCH PLANE_STRIDE SLOT [STYLE]\n");
         return 2;
     }
-    const int style = argc == 7 ? atoi(argv[6]) : STYLE_RESET | STYLE_TERMALL;
     const std::string mode = argv[1], dir = argv[2];
     const int pitch = atoi(argv[3]);
     const long plane_stride = atol(argv[4]);
+    if (mode == "decode") {
+        const int nplanes = atoi(argv[5]);
+        const int dstyle = argc == 7 ? atoi(argv[6]) : 0;
+        std::vector<char> draw = slurp(dir + "/desc.bin"), rraw = slurp(dir + "/res.bin");
+        std::vector<char> cw = slurp(dir + "/cw.bin"), oraw = slurp(dir + "/offs.bin");
+        const int ncb = (int)(draw.size() / sizeof(rg_cb_desc));
+        const rg_cb_desc *desc = reinterpret_cast<const rg_cb_desc *>(draw.data());
+        const rg_cb_result *res = reinterpret_cast<const rg_cb_result *>(rraw.data());
+        const int64_t *offs = reinterpret_cast<const int64_t *>(oraw.data());
+        std::vector<char> sraw;
+        if (dstyle & STYLE_TERMALL) sraw = slurp(dir + "/seglen.bin");
+        cw.resize(cw.size() + 16);
+        std::vector<int32_t> dq((size_t)nplanes * plane_stride, 0);
+        for (int k = 0; k < ncb; ++k) {
+            DecJob j{};
+            j.data = reinterpret_cast<const uint8_t *>(cw.data()) + offs[k];
+            j.len = res[k].len; j.npasses = res[k].npasses; j.nbp = res[k].nbp;
+            j.w = desc[k].w; j.h = desc[k].h; j.orient = desc[k].orient; j.pitch = pitch;
+            j.dst = dq.data() + (size_t)desc[k].comp * plane_stride + (size_t)desc[k].y0 * pitch + desc[k].x0;
+            j.style = dstyle;
+            j.seglen = (dstyle & STYLE_TERMALL) ? reinterpret_cast<const int32_t *>(sraw.data()) + (size_t)k * MAXP
+                                                : nullptr;
+            emu::run(dec_body, &j);
+        }
+        spill(dir + "/dq.bin", dq.data(), sizeof(int32_t) * dq.size());
+        return 0;
+    }
+    const int style = argc == 7 ? atoi(argv[6]) : STYLE_RESET | STYLE_TERMALL;
     const int slot = atoi(argv[5]);
     std::vector<char> qraw = slurp(dir + "/q.bin"), draw = slurp(dir + "/desc.bin");
     const int32_t *q = reinterpret_cast<const int32_t *>(qraw.data());

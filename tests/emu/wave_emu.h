@@ -95,6 +95,8 @@ static inline uint32_t wv_from_right(uint32_t x)
     return emu::lane() == 63 ? 0u : (uint32_t)v[emu::lane() + 1];
 }
 static inline void wv_sched_fence() {}
+static inline uint32_t wv_sload_u8(const uint8_t *p) { return *p; }
+static inline void wv_lds_or(uint32_t *p, uint32_t v) { *p |= v; }
 static inline uint32_t wv_perm(uint32_t hi, uint32_t lo, uint32_t s)
 {
     const uint64_t v = ((uint64_t)hi << 32) | lo;
