@@ -37,6 +37,24 @@ __global__ void __launch_bounds__(256) chain_zero_kernel(int *ws)
     for (int i = threadIdx.x; i < CH_LIST; i += 256) ws[i] = 0;
 }
 
+// (block, pass) items per workgroup of the list kernels: 256 threads x
+// CH_ITEMS, so that each workgroup's LDS histogram covers enough items to keep
+// the global atomics per bucket few.
+constexpr int CH_ITEMS = 16;
+
+// Key of item t of this thread (items of a workgroup are consecutive, the
+// threads interleaved for coalesced pass-offset loads); -1 if none.
+template <class J>
+__device__ __forceinline__ int chain_item_key(const J &job, int nblocks, int style, int qlo, int qhi, int t, long &i)
+{
+    i = ((long)blockIdx.x * CH_ITEMS + t) * 256 + threadIdx.x;
+    if (i >= (long)nblocks * MAXP) return -1;
+    const int q = (int)(i % MAXP);
+    if (q < qlo || q >= qhi) return -1;
+    const T1Job &jb = job((int)(i / MAXP));
+    return chain_key(*jb.cm, jb.poff, q, style, qlo, qhi);
+}
+
 // Histogram of chain keys, privatised per workgroup in LDS.
 template <class J>
 __global__ void __launch_bounds__(256) chain_hist_kernel(J job, int nblocks, int style, int qlo, int qhi, int *ws)
@@ -44,10 +62,10 @@ __global__ void __launch_bounds__(256) chain_hist_kernel(J job, int nblocks, int
     __shared__ int h[CH_HIST];
     for (int k = threadIdx.x; k < CH_HIST; k += 256) h[k] = 0;
     __syncthreads();
-    const long i = (long)blockIdx.x * 256 + threadIdx.x;
-    if (i < (long)nblocks * MAXP) {
-        const T1Job &jb = job((int)(i / MAXP));
-        const int key = chain_key(*jb.cm, jb.poff, (int)(i % MAXP), style, qlo, qhi);
+#pragma unroll 4
+    for (int t = 0; t < CH_ITEMS; ++t) {
+        long i;
+        const int key = chain_item_key(job, nblocks, style, qlo, qhi, t, i);
         if (key >= 0) atomicAdd(&h[key], 1);
     }
     __syncthreads();
@@ -82,18 +100,20 @@ __global__ void __launch_bounds__(256) chain_scatter_kernel(J job, int nblocks, 
     __shared__ int h[CH_HIST];
     for (int k = threadIdx.x; k < CH_HIST; k += 256) h[k] = 0;
     __syncthreads();
-    const long i = (long)blockIdx.x * 256 + threadIdx.x;
-    int key = -1, local = 0;
-    if (i < (long)nblocks * MAXP) {
-        const T1Job &jb = job((int)(i / MAXP));
-        key = chain_key(*jb.cm, jb.poff, (int)(i % MAXP), style, qlo, qhi);
-        if (key >= 0) local = atomicAdd(&h[key], 1);
+    int key[CH_ITEMS], local[CH_ITEMS];
+    long id[CH_ITEMS];
+#pragma unroll
+    for (int t = 0; t < CH_ITEMS; ++t) {
+        key[t] = chain_item_key(job, nblocks, style, qlo, qhi, t, id[t]);
+        local[t] = key[t] >= 0 ? atomicAdd(&h[key[t]], 1) : 0;
     }
     __syncthreads();
     for (int k = threadIdx.x; k < CH_HIST; k += 256)
         if (h[k]) h[k] = atomicAdd(&ws[CH_HIST + k], h[k]);
     __syncthreads();
-    if (key >= 0) ws[CH_LIST + h[key] + local] = (int)i;
+#pragma unroll
+    for (int t = 0; t < CH_ITEMS; ++t)
+        if (key[t] >= 0) ws[CH_LIST + h[key[t]] + local[t]] = (int)id[t];
 }
 
 // Codes the sorted lists: MQ wavefronts first (longest chains first), then
@@ -131,7 +151,7 @@ template <class J>
 int launch_chains(hipStream_t s, const J &job, int nblocks, int style, int *ws, int qlo = 0, int qhi = MAXP)
 {
     const long n = (long)nblocks * MAXP;
-    const unsigned g = (unsigned)((n + 255) / 256);
+    const unsigned g = (unsigned)((n + 256 * CH_ITEMS - 1) / (256 * CH_ITEMS));
     chain_zero_kernel<<<1, 256, 0, s>>>(ws);
     RG_LAUNCH_CHECK();
     chain_hist_kernel<<<g, 256, 0, s>>>(job, nblocks, style, qlo, qhi, ws);

@@ -570,37 +570,11 @@ RG_DEV void t1_block(const T1Job &job, uint8_t *lut, CMPhase ph = CMPhase())
 
 // ---- split path, second half: one coder per lane ----
 //
-// Symbols arrive 16 at a time from an unconditional (index-clamped) 16-byte
-// load issued one chunk ahead, so the memory latency stays off a chain's
-// dependency path.  The raw (bypass) coder collects its output bytes in four
-// registers that leave as one 16-byte store per 16 bytes (a byte store per
-// output byte would make every later wait on the symbol load also wait for
-// that store); output regions are 16-byte aligned.
-struct LaneMQ {
-    uint32_t A, C, B;
-    int CT, bp, ovf;
-    uint32_t o0, o1, o2, o3;   // codeword bytes [bp & ~15, bp) not yet stored
-};
-
-// Places byte v at codeword position pos (bytes at or beyond cap only set the
-// overflow flag); a completed 16-byte group is stored.
-RG_DEV void lane_put(LaneMQ &m, uint32_t v, int pos, uint8_t *out, int cap)
-{
-    if (pos >= cap) {
-        m.ovf = 1;
-        return;
-    }
-    const uint32_t x = v << ((pos & 3) * 8);
-    const int w = (pos >> 2) & 3;
-    m.o0 |= w == 0 ? x : 0u;
-    m.o1 |= w == 1 ? x : 0u;
-    m.o2 |= w == 2 ? x : 0u;
-    m.o3 |= w == 3 ? x : 0u;
-    if ((pos & 15) == 15) {
-        wv_store16(out + (pos & ~15), m.o0, m.o1, m.o2, m.o3);
-        m.o0 = m.o1 = m.o2 = m.o3 = 0;
-    }
-}
+// Symbols arrive 16 at a time from unconditional (index-clamped) 16-byte
+// loads issued ahead of use, so the memory latency stays off a chain's
+// dependency path.  Output bytes go to a per-lane ring in workgroup memory
+// (an LDS store per byte, whose waits never hold up a global load) and leave
+// for global memory in 16-byte groups; output regions are 16-byte aligned.
 
 // ---- lane MQ coder, pipelined form ----
 //
@@ -869,68 +843,84 @@ RG_DEV void t1_mq_pass_lane(uint32_t *tab, uint8_t *ring, bool active, const uin
 // decision bits of symbols [s0, s1) go out as they are, MSB first, with 7
 // bits in the byte after an 0xFF (its MSB a stuffed 0); an unfinished byte,
 // including the 7-bit byte owed after a final 0xFF, is padded with 0,1,0,...
-// so the segment never ends in 0xFF.  Bits are taken 16 symbols at a time:
-// the low bit of each symbol byte is gathered with one multiply per word.
-// The coder reuses LaneMQ: B holds the bits of the byte being filled, CT the
-// free bits left in it, bp the next byte index.
-RG_DEV void raw_put_bits(LaneMQ &m, uint32_t bits, int nb, uint8_t *out, int cap)
-{
-    // at most three bytes complete from 16 bits (bytes hold >= 7)
-#pragma unroll
-    for (int it = 0; it < 3; ++it) {
-        if (nb >= m.CT) {
-            nb -= m.CT;
-            const uint32_t byte = ((m.B << m.CT) | (bits >> nb)) & 0xFFu;
-            bits &= (1u << nb) - 1u;
-            lane_put(m, byte, m.bp, out, cap);
-            m.bp++;
-            m.CT = byte == 0xFFu ? 7 : 8;
-            m.B = 0;
-        }
-    }
-    m.B = (m.B << nb) | bits;
-    m.CT -= nb;
-}
-
-RG_DEV void t1_raw_pass_lane(bool active, const uint8_t *sym, int s0, int s1, uint8_t *seg, int cap,
+// so the segment never ends in 0xFF.  Bits are taken 16 symbols at a time
+// (the low bit of each symbol byte gathered with one multiply per word) into
+// a bit accumulator; each step then completes up to three bytes (16 + 7
+// pending bits, >= 7 bits a byte) with selects, writing every candidate byte
+// to the lane's ring slot unconditionally (an unfinished one is rewritten
+// before it is flushed).  `ring`: RING_BYTES of workgroup memory.
+RG_DEV void t1_raw_pass_lane(uint8_t *ring, bool active, const uint8_t *sym, int s0, int s1, uint8_t *seg, int cap,
                              int32_t *len_out)
 {
+    uint8_t *rb = ring + wv_lane() * RING_STRIDE;
     const int c0 = active ? s0 >> 4 : 0, c1 = active ? (s1 + 15) >> 4 : 0;
     int nch = c1 - c0, nmax = nch;
     for (int o = 32; o >= 1; o >>= 1) {
         const int t = wv_shfl_xor(nmax, o);
         nmax = nmax > t ? nmax : t;
     }
-    LaneMQ m;
-    m.A = 0; m.C = 0; m.B = 0; m.CT = 8; m.bp = 0; m.ovf = 0;
-    m.o0 = m.o1 = m.o2 = m.o3 = 0;
+    RingMQ m;
+    m.A = 0; m.C = 0; m.B = 0; m.CT = 0; m.bp = 0; m.fl = 0; m.ovf = 0;
+    uint32_t acc = 0;   // the low nacc bits are pending, oldest most significant
+    int nacc = 0, need = 8;   // need: bits of the next byte (7 after an 0xFF)
     const int last = nch > 0 ? c1 - 1 : c0;
-    u32x4 nxt = wv_load16(sym + 16 * c0);
-    for (int i = 0; i < nmax; ++i) {
-        const u32x4 cur = nxt;
-        const int cn = c0 + i + 1;
-        nxt = wv_load16(sym + 16 * (cn < last ? cn : last));
-        // symbol k of the chunk -> bit 15 - k (first symbol most significant)
-        const uint32_t g0 = ((cur.x & 0x01010101u) * 0x08040201u) >> 24;
-        const uint32_t g1 = ((cur.y & 0x01010101u) * 0x08040201u) >> 24;
-        const uint32_t g2 = ((cur.z & 0x01010101u) * 0x08040201u) >> 24;
-        const uint32_t g3 = ((cur.w & 0x01010101u) * 0x08040201u) >> 24;
-        const uint32_t b16 = (g0 << 12) | (g1 << 8) | (g2 << 4) | g3;
-        const int base = 16 * (c0 + i);
-        const int lo = s0 > base ? s0 - base : 0;
-        const int hi = s1 - base < 16 ? s1 - base : 16;
-        const int nb = i < nch && hi > lo ? hi - lo : 0;
-        raw_put_bits(m, (b16 >> (16 - hi < 16 ? 16 - hi : 0)) & ((1u << nb) - 1u), nb, seg, cap);
+    // a step is short next to a load from HBM (the stream was written by the
+    // context-modelling kernel and is not cached), so chunk c is requested
+    // RAW_AHEAD steps before it is coded; the loop is unrolled by RAW_AHEAD
+    // and each slot reloaded right after its chunk's bits are taken
+    constexpr int RAW_AHEAD = 4;
+    u32x4 pre[RAW_AHEAD];
+#pragma unroll
+    for (int a = 0; a < RAW_AHEAD; ++a) pre[a] = wv_load16(sym + 16 * (c0 + a < last ? c0 + a : last));
+    for (int i = 0; i < nmax; i += RAW_AHEAD) {
+#pragma unroll
+        for (int a = 0; a < RAW_AHEAD; ++a) {
+            const int step = i + a;
+            // symbol k of the chunk -> bit 15 - k (first symbol most significant)
+            const uint32_t g0 = ((pre[a].x & 0x01010101u) * 0x08040201u) >> 24;
+            const uint32_t g1 = ((pre[a].y & 0x01010101u) * 0x08040201u) >> 24;
+            const uint32_t g2 = ((pre[a].z & 0x01010101u) * 0x08040201u) >> 24;
+            const uint32_t g3 = ((pre[a].w & 0x01010101u) * 0x08040201u) >> 24;
+            const uint32_t b16 = (g0 << 12) | (g1 << 8) | (g2 << 4) | g3;
+            const int cn = c0 + step + RAW_AHEAD;
+            wv_sched_fence();   // the slot's old chunk is dead before its reload is issued
+            pre[a] = wv_load16(sym + 16 * (cn < last ? cn : last));
+            if (step < nmax) {
+                const int base = 16 * (c0 + step);
+                const int lo = s0 > base ? s0 - base : 0;
+                const int hi = s1 - base < 16 ? s1 - base : 16;
+                const int nb = step < nch && hi > lo ? hi - lo : 0;
+                const uint32_t bits = (b16 >> (16 - hi < 16 ? 16 - hi : 0)) & ((1u << nb) - 1u);
+                acc = (acc << nb) | bits;
+                nacc += nb;
+#pragma unroll
+                for (int e = 0; e < 3; ++e) {
+                    const bool emit = nacc >= need;
+                    const uint32_t byte = (acc >> (emit ? nacc - need : 0)) & ((1u << need) - 1u);
+                    rb[m.bp & 63] = (uint8_t)byte;
+                    nacc -= emit ? need : 0;
+                    acc &= (1u << nacc) - 1u;
+                    m.bp += emit ? 1 : 0;
+                    need = emit ? (byte == 0xFFu ? 7 : 8) : need;
+                }
+            }
+        }
+        // at most 4 x 3 new bytes since the last flush left < 16 pending
+        ring_flush(m, rb, seg, cap);
     }
-    if (!active) return;
-    if (m.CT < 8) {
-        const int n = m.CT;
-        const uint32_t byte = ((m.B << n) | (0x55u >> (8 - n))) & 0xFFu;
-        lane_put(m, byte, m.bp, seg, cap);
+    // termination: pad the byte being filled (need - nacc free bits) with 0,1,0,...
+    const int padn = need - nacc;
+    if (padn < 8) {
+        rb[m.bp & 63] = (uint8_t)(((acc << padn) | (0x55u >> (8 - padn))) & 0xFFu);
         m.bp++;
     }
     const int len = m.bp;
-    if ((len & 15) != 0 && (len & ~15) < cap) wv_store16(seg + (len & ~15), m.o0, m.o1, m.o2, m.o3);
+    ring_flush(m, rb, seg, cap);
+    if (!active) return;
+    if ((len & 15) != 0 && (len & ~15) < cap) {
+        const uint32_t *w = reinterpret_cast<const uint32_t *>(rb + ((len & ~15) & 63));
+        wv_store16(seg + (len & ~15), w[0], w[1], w[2], w[3]);
+    }
     *len_out = (m.ovf || len > cap) ? -1 : len;
 }
 
@@ -949,23 +939,63 @@ RG_DEV bool pass_range(const i32x4 &c, const int32_t *poff, int q, int &p, int &
     return true;
 }
 
+// Workgroup memory of t1_pack_block: the output offset of every pass
+// (MAXP + 1 words) and its segment's arena offset (MAXP words).
+constexpr int PACK_WORDS = 2 * MAXP + 1;
+
 // One wavefront per block: concatenates the pass segments into the block's
-// output slot (lanes copy consecutive bytes) and writes the block result.
+// output slot and writes the block result.  The passes' lengths are read at
+// once and prefix-summed across the lanes, so the copy does not wait on a
+// chain of per-pass loads: lane l copies output bytes l, l + 64, ... (four
+// loads in flight), finding each byte's pass with a cursor that only moves
+// forward.  Passes from the first overflowed segment (length < 0) on are
+// dropped and the block flagged.  `sh`: PACK_WORDS words of workgroup memory.
 RG_DEV void t1_pack_block(const i32x4 &c, const int32_t *poff, const uint8_t *seg, const int32_t *seglen,
-                          uint8_t *out, int cap, rg_cb_result *res)
+                          uint8_t *out, int cap, rg_cb_result *res, int32_t *sh)
 {
+    static_assert(MAXP <= 128, "two passes per lane");
     const int lane = wv_lane();
-    int ovf = c.w, pos = 0;
-    for (int p = 0; p < c.y; ++p) {
-        const int len = seglen[p];
-        if (len < 0) { ovf = 1; break; }
-        const uint8_t *src = seg + seg_at(poff[p], p);
-        for (int k = lane; k < len; k += 64)
-            if (pos + k < cap) out[pos + k] = src[k];
-        pos += len;
+    const int np = c.y;
+    int32_t *pos = sh, *soff = sh + MAXP + 1;
+    const int q1 = lane + 64;
+    const int l0 = lane < np ? seglen[lane] : 0, l1 = q1 < np ? seglen[q1] : 0;
+    const uint64_t bad0 = wv_ballot(l0 < 0), bad1 = wv_ballot(l1 < 0);
+    const int pf = bad0 ? __builtin_ctzll(bad0) : (bad1 ? 64 + __builtin_ctzll(bad1) : np);
+    const int n0 = lane < pf ? l0 : 0, n1 = q1 < pf ? l1 : 0;
+    int t0, t1;
+    const int e0 = wave_excl_count(n0, t0);
+    const int e1 = wave_excl_count(n1, t1);
+    const int total = t0 + t1;
+    pos[lane] = e0;
+    soff[lane] = lane < pf ? seg_at(poff[lane], lane) : 0;
+    if (q1 < MAXP) {
+        pos[q1] = t0 + e1;
+        soff[q1] = q1 < pf ? seg_at(poff[q1], q1) : 0;
     }
-    if (pos > cap) ovf = 1;
-    if (lane == 0) *res = rg_cb_result{c.z == 0 ? 0 : pos, c.y, c.z, ovf ? 1 : 0};
+    if (lane == 0) pos[MAXP] = total;
+    wv_sync();
+    const int lim = total < cap ? total : cap;
+    int p = 0;   // this lane's current pass: pos[p] <= k < pos[p + 1]
+    for (int k0 = 0; k0 < lim; k0 += 256) {
+        uint32_t v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = k0 + 64 * u + lane;
+            v[u] = 0;
+            if (k < lim) {
+                while (pos[p + 1] <= k) ++p;
+                v[u] = seg[soff[p] + (k - pos[p])];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = k0 + 64 * u + lane;
+            if (k < lim) out[k] = (uint8_t)v[u];
+        }
+    }
+    const bool ovf = c.w || pf < np || total > cap;
+    if (lane == 0) *res = rg_cb_result{c.z == 0 ? 0 : total, c.y, c.z, ovf ? 1 : 0};
+    wv_sync();
 }
 
 // ---- parallel styles, sorted form: chains ordered longest first ----
@@ -1013,7 +1043,7 @@ RG_DEV void t1_chain_lane(uint32_t *tab, uint8_t *ring, bool raw, const int32_t 
     int p = 0, s0 = 0, s1 = 0, at = 0, cap = 0;
     if (active) active = pass_range(*jb.cm, jb.poff, q, p, s0, s1, at, cap);
     if (raw)
-        t1_raw_pass_lane(active, jb.sym, s0, s1, jb.seg + at, cap, jb.seglen + p);
+        t1_raw_pass_lane(ring, active, jb.sym, s0, s1, jb.seg + at, cap, jb.seglen + p);
     else
         t1_mq_pass_lane(tab, ring, active, jb.sym, s0, s1, jb.seg + at, cap, jb.seglen + p);
 }

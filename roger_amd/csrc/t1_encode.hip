@@ -125,15 +125,17 @@ __global__ void __launch_bounds__(64) t1_pack_kernel(const int4 *__restrict__ cm
                                                      const int32_t *__restrict__ seglens, uint8_t *__restrict__ slots,
                                                      int slot, rg_cb_result *__restrict__ res)
 {
+    __shared__ int32_t sh[PACK_WORDS];
     const size_t b = blockIdx.x;
     t1_pack_block(cm[b], poffs + b * MAXP, arena + b * arena_stride, seglens + b * MAXP, slots + b * slot, slot,
-                  res + b);
+                  res + b, sh);
 }
 
 __global__ void __launch_bounds__(64) t1_pack_jobs_kernel(const T1Job *__restrict__ jobs)
 {
+    __shared__ int32_t sh[PACK_WORDS];
     const T1Job &j = jobs[blockIdx.x];
-    t1_pack_block(*j.cm, j.poff, j.seg, j.seglen, j.out, j.cap, j.res);
+    t1_pack_block(*j.cm, j.poff, j.seg, j.seglen, j.out, j.cap, j.res, sh);
 }
 
 }  // namespace

@@ -191,7 +191,7 @@ void mq_pass_body(void *a)
     const T1Job &j = m.jobs[active ? b : 0];
     if (active) active = pass_range(*j.cm, j.poff, m.p, p, s0, s1, at, cap);
     if (pass_is_raw(m.style, m.p))
-        t1_raw_pass_lane(active, j.sym, s0, s1, j.seg + at, cap, j.seglen + p);
+        t1_raw_pass_lane(g_ring, active, j.sym, s0, s1, j.seg + at, cap, j.seglen + p);
     else
         t1_mq_pass_lane(g_tab, g_ring, active, j.sym, s0, s1, j.seg + at, cap, j.seglen + p);
 }
@@ -213,7 +213,8 @@ void chain_body(void *a)
 void pack_body(void *a)
 {
     const T1Job &j = *static_cast<T1Job *>(a);
-    t1_pack_block(*j.cm, j.poff, j.seg, j.seglen, j.out, j.cap, j.res);
+    static int32_t sh[PACK_WORDS];
+    t1_pack_block(*j.cm, j.poff, j.seg, j.seglen, j.out, j.cap, j.res, sh);
 }
 uint8_t g_dlut[LUT_BYTES];
 uint32_t g_rawbuf[DEC_RAW_WORDS];
