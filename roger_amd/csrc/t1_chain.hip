@@ -122,7 +122,7 @@ template <class J>
 __global__ void __launch_bounds__(64) chain_code_kernel(J job, const int *ws)
 {
     __shared__ uint32_t tab[48];
-    __shared__ uint8_t ring[RING_BYTES];
+    __shared__ __attribute__((aligned(16))) uint8_t ring[MQ_LDS_BYTES];
     const int nmq = ws[CH_COUNTS], ntot = nmq + ws[CH_COUNTS + 1];
     const int mqw = (nmq + 63) >> 6;
     const int w = (int)blockIdx.x;

@@ -578,55 +578,45 @@ RG_DEV void t1_block(const T1Job &job, uint8_t *lut, CMPhase ph = CMPhase())
 
 // ---- lane MQ coder, pipelined form ----
 //
-// One MQ chain per lane.  A chain's serial dependency runs through each
-// symbol's probability-table read (workgroup memory) and the interval
-// arithmetic; the renormalisation shifts, BYTEOUT and output can overlap the
-// next symbol's table read.  So a step updates the interval and the context
-// state (ring_code), then reads the NEXT symbol's table entry -- its context
-// state is final at that point -- and only then does the current symbol's
-// renormalisation and BYTEOUT (ring_renorm).  Output bytes go to a per-lane
-// 64-byte ring in workgroup memory with one unconditional store per BYTEOUT
-// (a step without BYTEOUT rewrites the pending byte's own slot with its
-// current value), so a step has no store branch; completed 16-byte groups
-// leave for global memory once per 16-symbol chunk (ring_flush).  A chunk
-// adds at most 48 bytes (three BYTEOUTs per step at most) to fewer than 16
-// unflushed ones, so the ring never overwrites an unflushed byte.  Chunks
-// wholly inside a lane's symbol range run without per-symbol range tests.
+// One MQ chain per lane.  A context's state lives in workgroup memory as its
+// probability-table entry (Qe | NMPS << 16 | NLPS << 22 | SWITCH << 28) with
+// the MPS in bit 31, one word per (context, lane), so a symbol reads its
+// context with one LDS load and no table lookup, and writes the context back
+// with one store.  A chain's serial dependency runs through that entry: the
+// next state's table entry is one more LDS read.  So a step issues, before
+// its interval arithmetic, both that read and the read of the NEXT symbol's
+// context (forwarded from the store when the next symbol has the same
+// context, since the store is issued later), and renormalisation and BYTEOUT
+// overlap the reads.  Output bytes go to a per-lane 64-byte ring in workgroup
+// memory with one unconditional store per BYTEOUT (a step without BYTEOUT
+// rewrites the pending byte's own slot with its current value), so a step
+// has no store branch; completed 16-byte groups leave for global memory once
+// per 16-symbol chunk (ring_flush).  A chunk adds at most 48 bytes (three
+// BYTEOUTs per step at most) to fewer than 16 unflushed ones, so the ring
+// never overwrites an unflushed byte.  Chunks wholly inside a lane's symbol
+// range run without per-symbol range tests.
 constexpr int RING_STRIDE = 68;          // bytes per lane: 17 dwords, so lanes start in distinct LDS banks
 constexpr int RING_BYTES = 64 * RING_STRIDE;
+constexpr int MQ_CTX = 19;
+// workgroup memory of the lane MQ coders: the rings, then the context words
+// (context cx of lane l at word cx * 64 + l, so a wave's accesses hit 64
+// distinct banks)
+constexpr int MQ_LDS_BYTES = RING_BYTES + MQ_CTX * 64 * 4;
+
+// Initial context entries (Table D.7: context 0 in state 4, RL (17) in 3,
+// UNIFORM (18) in 46, all others in 0; MPS 0) for this lane (ce = its word
+// of context 0).
+RG_DEV void ce_init(uint32_t *ce)
+{
+#pragma unroll
+    for (int cx = 0; cx < MQ_CTX; ++cx) ce[64 * cx] = MQT[cx == 0 ? 4 : (cx == CTX_RL ? 3 : (cx == CTX_UNI ? 46 : 0))];
+}
 
 struct RingMQ {
     uint32_t A, C, B;
     int CT, bp;    // bp: position of the pending byte B (-1: the byte before the codeword)
     int fl;        // bytes [0, fl) are in global memory
     int ovf;
-};
-
-// The 19 context states (8 bits each: I << 1 | MPS), byte cx of the 20-byte
-// value c4:c3:c2:c1:c0.  A read picks the byte out of each register pair with
-// one byte permute and chooses among the three; a write inserts the byte
-// under a mask that is zero for the four other registers.  (A compare/select
-// chain over the registers is turned by the compiler into an indexed access
-// to scratch memory.)
-struct CtxStates {
-    uint32_t c0, c1, c2, c3, c4;
-    // initial states (Table D.7): ctx 0 -> 4, RL (17) -> 3, UNIFORM (18) -> 46
-    RG_DEV void init() { c0 = 4u << 1; c1 = c2 = c3 = 0; c4 = (3u << 1) << 8 | (46u << 1) << 16; }
-    RG_DEV uint32_t get(uint32_t cx) const
-    {
-        const uint32_t sel = (cx & 7u) | 0x0C0C0C00u;   // byte cx & 7, upper bytes zero
-        const uint32_t a = wv_perm(c1, c0, sel), b = wv_perm(c3, c2, sel), c = wv_perm(c4, c4, sel);
-        return cx < 8 ? a : (cx < 16 ? b : c);
-    }
-    RG_DEV void set(uint32_t cx, uint32_t st)
-    {
-        const uint32_t sh = (cx & 3u) * 8u, m = 0xFFu << sh, v = st << sh, w = cx >> 2;
-        c0 = (c0 & ~(w == 0 ? m : 0u)) | (w == 0 ? v : 0u);
-        c1 = (c1 & ~(w == 1 ? m : 0u)) | (w == 1 ? v : 0u);
-        c2 = (c2 & ~(w == 2 ? m : 0u)) | (w == 2 ? v : 0u);
-        c3 = (c3 & ~(w == 3 ? m : 0u)) | (w == 3 ? v : 0u);
-        c4 = (c4 & ~(w == 4 ? m : 0u)) | (w == 4 ? v : 0u);
-    }
 };
 
 // BYTEOUT (C.2.7) when `need`, as predicated code: a carry into B, then 8 or
@@ -645,29 +635,38 @@ RG_DEV void ring_byteout(RingMQ &m, bool need, uint8_t *rb)
     m.CT = need ? (big ? 7 : 8) : m.CT;
 }
 
-// ENCODE's interval and state update (C.2.2-C.2.5) of one (context << 1 | bit)
-// symbol whose context is in state st with table entry t; A is renormalised
-// (shifted) here, and the returned shift count is C's share for ring_renorm.
-RG_DEV int ring_code(RingMQ &m, CtxStates &cs, uint32_t sym, uint32_t st, uint32_t t)
+// ENCODE's interval update (C.2.2-C.2.5) of decision d in a context with
+// entry e; A is renormalised (shifted) here and the returned shift count is
+// C's share for ring_renorm.  *enew: the context's entry after the symbol
+// (tnew = the table entry of the state it moves to if it renormalises).
+RG_DEV int ring_code_e(RingMQ &m, uint32_t d, uint32_t e, uint32_t tnew, uint32_t lps_nmps, bool renorm,
+                       uint32_t &enew)
 {
-    const uint32_t cx = sym >> 1, d = sym & 1u, mps = st & 1u;
-    const uint32_t qe = t & 0xFFFFu;
+    const uint32_t qe = e & 0xFFFFu;
     const uint32_t a = m.A - qe;
-    const bool lps = d != mps;
-    const bool renorm = lps || !(a & 0x8000u);
+    const bool lps = d != (e >> 31);
     // C.2.4 / C.2.5: the interval kept is the upper one (C += Qe) for an MPS
     // unless A < Qe (conditional exchange), and the reverse for an LPS
     const bool small = a < qe;
     const bool upper = !renorm || (lps ? small : !small);
     m.C += upper ? qe : 0u;
     m.A = upper ? a : qe;
-    // next state: NLPS (bits 22..27) with MPS ^ SWITCH after an LPS, NMPS
-    // (bits 16..21) after an MPS
-    const uint32_t nst = (((t >> (lps ? 22u : 16u)) & 63u) << 1) | (mps ^ (lps ? (t >> 28) : 0u));
-    cs.set(cx, renorm ? nst : st);
+    enew = renorm ? (tnew | (lps_nmps << 31)) : e;
     const int n = wv_clz(m.A) - 16;
     m.A <<= n;
     return n;
+}
+
+// The next state of a context with entry e after decision d: its table index
+// (NLPS after an LPS, NMPS after an MPS), the new MPS (switched after an LPS
+// when SWITCH is set), and whether the step renormalises.
+RG_DEV uint32_t ring_next(uint32_t d, uint32_t e, uint32_t A, uint32_t &nmps, bool &renorm)
+{
+    const uint32_t mps = e >> 31;
+    const bool lps = d != mps;
+    renorm = lps || !((A - (e & 0xFFFFu)) & 0x8000u);
+    nmps = mps ^ (lps ? ((e >> 28) & 1u) : 0u);
+    return (e >> (lps ? 22u : 16u)) & 63u;
 }
 
 // RENORME's C side (C.2.6): n shifts with BYTEOUT whenever CT reaches 0.  C
@@ -706,15 +705,17 @@ RG_DEV void ring_flush(RingMQ &m, const uint8_t *rb, uint8_t *out, int cap)
 
 // Codes symbols [s0, s1) of the 16-byte-aligned stream `sym` as one MQ
 // codeword (INITENC, ENCODE per symbol, FLUSH C.2.9) into `out` (16-byte
-// aligned, capacity cap), from the context states cs.  Every lane of the
+// aligned, capacity cap), from the initial context states.  Every lane of the
 // wavefront calls it; idle lanes pass active = false.  Returns the codeword
 // length; ovf is set if it did not fit (out then holds its first complete
 // 16-byte groups).  `tab`: the probability table in workgroup memory; `ring`:
-// RING_BYTES of workgroup memory.
+// MQ_LDS_BYTES of workgroup memory (4-byte aligned).
 RG_DEV int ring_chain(const uint32_t *tab, uint8_t *ring, bool active, const uint8_t *sym, int s0, int s1,
-                      uint8_t *out, int cap, CtxStates &cs, bool &ovf)
+                      uint8_t *out, int cap, bool &ovf)
 {
     uint8_t *rb = ring + wv_lane() * RING_STRIDE;
+    uint32_t *ce = reinterpret_cast<uint32_t *>(ring + RING_BYTES) + wv_lane();
+    ce_init(ce);
     const int c0 = active ? s0 >> 4 : 0, c1 = active ? (s1 + 15) >> 4 : 0;
     const int nch = c1 - c0;
     int nmax = nch;
@@ -734,25 +735,42 @@ RG_DEV int ring_chain(const uint32_t *tab, uint8_t *ring, bool active, const uin
         const int base = 16 * (c0 + i);
         const bool full = i < nch && base >= s0 && base + 16 <= s1;
         if (full) {
-            uint32_t sy = w4[0] & 0xFFu, st = cs.get(sy >> 1), t = tab[st >> 1];
+            uint32_t sy = w4[0] & 0xFFu;
+            uint32_t e = ce[64 * (sy >> 1)];
 #pragma unroll
             for (int k = 0; k < 16; ++k) {
-                const int n = ring_code(m, cs, sy, st, t);
-                if (k < 15) {   // the next symbol's table read before this one's byte work
-                    sy = (w4[(k + 1) >> 2] >> (8 * ((k + 1) & 3))) & 0xFFu;
-                    st = cs.get(sy >> 1);
-                    t = tab[st >> 1];
-                    wv_sched_fence();
+                const uint32_t cx = sy >> 1, d = sy & 1u;
+                uint32_t nmps;
+                bool renorm;
+                const uint32_t nidx = ring_next(d, e, m.A, nmps, renorm);
+                const uint32_t tnew = tab[nidx];
+                uint32_t sy2 = 0, e2 = 0;
+                if (k < 15) {   // the next symbol's context, read before this one's store
+                    sy2 = (w4[(k + 1) >> 2] >> (8 * ((k + 1) & 3))) & 0xFFu;
+                    e2 = ce[64 * (sy2 >> 1)];
                 }
+                wv_sched_fence();
+                uint32_t enew;
+                const int n = ring_code_e(m, d, e, tnew, nmps, renorm, enew);
+                ce[64 * cx] = enew;
+                if (k < 15) e = (sy2 >> 1) == cx ? enew : e2;
                 ring_renorm(m, n, rb);
+                sy = sy2;
             }
         } else if (i < nch) {
 #pragma unroll
             for (int k = 0; k < 16; ++k)
                 if (base + k >= s0 && base + k < s1) {
                     const uint32_t sy = (w4[k >> 2] >> (8 * (k & 3))) & 0xFFu;
-                    const uint32_t st = cs.get(sy >> 1);
-                    ring_renorm(m, ring_code(m, cs, sy, st, tab[st >> 1]), rb);
+                    const uint32_t cx = sy >> 1, d = sy & 1u;
+                    const uint32_t e = ce[64 * cx];
+                    uint32_t nmps;
+                    bool renorm;
+                    const uint32_t nidx = ring_next(d, e, m.A, nmps, renorm);
+                    uint32_t enew;
+                    const int n = ring_code_e(m, d, e, tab[nidx], nmps, renorm, enew);
+                    ce[64 * cx] = enew;
+                    ring_renorm(m, n, rb);
                 }
         }
         ring_flush(m, rb, out, cap);
@@ -782,7 +800,7 @@ RG_DEV int ring_chain(const uint32_t *tab, uint8_t *ring, bool active, const uin
 // MQ-codes one block per lane from the streams the context-modelling pass
 // wrote.  Block `i` (i >= njobs: idle lane) is either jobs[i] (QUEUE) or
 // entry i of the flat arrays.  `tab` is 48 words of workgroup-shared memory,
-// `ring` RING_BYTES of it.
+// `ring` MQ_LDS_BYTES of it.
 template <bool QUEUE>
 RG_DEV void t1_mq_lanes(uint32_t *tab, uint8_t *ring, const T1Job *jobs, int i, int njobs, const uint8_t *syms,
                         int symcap, const i32x4 *cm, uint8_t *slots, int slot, rg_cb_result *res)
@@ -805,11 +823,9 @@ RG_DEV void t1_mq_lanes(uint32_t *tab, uint8_t *ring, const T1Job *jobs, int i, 
             c = cm[i]; src = syms + (size_t)i * symcap; out = slots + (size_t)i * slot; cap = slot; dst = res + i;
         }
     }
-    CtxStates cs;
-    cs.init();
     const bool coded = active && c.z > 0;   // a block without bit-planes has no codeword
     bool ovf;
-    const int len = ring_chain(tab, ring, coded, src, 0, c.x, out, cap, cs, ovf);
+    const int len = ring_chain(tab, ring, coded, src, 0, c.x, out, cap, ovf);
     if (!active) return;
     if (!coded) {
         *dst = rg_cb_result{0, 0, 0, 0};
@@ -832,10 +848,8 @@ RG_DEV void t1_mq_pass_lane(uint32_t *tab, uint8_t *ring, bool active, const uin
     const int lane = wv_lane();
     if (lane < 47) tab[lane] = MQT[lane];
     wv_sync();
-    CtxStates cs;
-    cs.init();
     bool ovf;
-    const int len = ring_chain(tab, ring, active, sym, s0, s1, seg, cap, cs, ovf);
+    const int len = ring_chain(tab, ring, active, sym, s0, s1, seg, cap, ovf);
     if (active) *len_out = ovf ? -1 : len;
 }
 

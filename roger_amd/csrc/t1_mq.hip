@@ -15,14 +15,14 @@ __global__ void __launch_bounds__(64) t1_mq_kernel(const uint8_t *__restrict__ s
                                                    int slot, rg_cb_result *__restrict__ res)
 {
     __shared__ uint32_t tab[48];
-    __shared__ uint8_t ring[RING_BYTES];
+    __shared__ __attribute__((aligned(16))) uint8_t ring[MQ_LDS_BYTES];
     t1_mq_lanes<false>(tab, ring, nullptr, blockIdx.x * 64 + threadIdx.x, ncb, syms, symcap, cm, slots, slot, res);
 }
 
 __global__ void __launch_bounds__(64) t1_mq_jobs_kernel(const T1Job *__restrict__ jobs, int njobs)
 {
     __shared__ uint32_t tab[48];
-    __shared__ uint8_t ring[RING_BYTES];
+    __shared__ __attribute__((aligned(16))) uint8_t ring[MQ_LDS_BYTES];
     t1_mq_lanes<true>(tab, ring, jobs, blockIdx.x * 64 + threadIdx.x, njobs, nullptr, 0, nullptr, nullptr, 0,
                       nullptr);
 }
