@@ -152,6 +152,10 @@ struct rg_t1_flat {
 };
 // Chain lists and coding of passes [qlo, qhi) of every block (t1_chain.hip).
 int rg_launch_chains_flat(hipStream_t s, const rg_t1_flat &f, int nblocks, int style, int32_t *ws, int qlo, int qhi);
+// The two halves of rg_launch_chains_flat: the sorted lists, then the coding.
+int rg_launch_chain_lists_flat(hipStream_t s, const rg_t1_flat &f, int nblocks, int style, int32_t *ws, int qlo,
+                               int qhi);
+int rg_launch_chain_code_flat(hipStream_t s, const rg_t1_flat &f, int nblocks, const int32_t *ws);
 int rg_launch_chains_jobs(hipStream_t s, const void *jobs, int njobs, int style, int32_t *ws, int qlo, int qhi);
 // Standard-style lane MQ coders (t1_mq.hip).
 int rg_launch_t1_mq(hipStream_t s, const uint8_t *syms, int symcap, const int4 *cm, int ncb, uint8_t *out,

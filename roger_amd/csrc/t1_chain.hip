@@ -146,9 +146,9 @@ __global__ void __launch_bounds__(64) chain_code_kernel(J job, const int *ws)
     t1_chain_lane(tab, ring, raw, ws + CH_LIST, base, end, job);
 }
 
-// Chain lists and coding of the passes [qlo, qhi) of every block.
+// Chain lists of the passes [qlo, qhi) of every block (counting sort).
 template <class J>
-int launch_chains(hipStream_t s, const J &job, int nblocks, int style, int *ws, int qlo = 0, int qhi = MAXP)
+int launch_lists(hipStream_t s, const J &job, int nblocks, int style, int *ws, int qlo, int qhi)
 {
     const long n = (long)nblocks * MAXP;
     const unsigned g = (unsigned)((n + 256 * CH_ITEMS - 1) / (256 * CH_ITEMS));
@@ -160,11 +160,24 @@ int launch_chains(hipStream_t s, const J &job, int nblocks, int style, int *ws, 
     RG_LAUNCH_CHECK();
     chain_scatter_kernel<<<g, 256, 0, s>>>(job, nblocks, style, qlo, qhi, ws);
     RG_LAUNCH_CHECK();
-    // at most nblocks * (MAXP - 2) chains; +1 wavefront for the raw list's
-    // unaligned start
+    return RG_SUCCESS;
+}
+
+// Coding of the listed chains: at most nblocks * (MAXP - 2) of them; +1
+// wavefront for the raw list's unaligned start.
+template <class J>
+int launch_code(hipStream_t s, const J &job, int nblocks, const int *ws)
+{
     chain_code_kernel<<<(unsigned)(((long)nblocks * (MAXP - 2) + 63) / 64 + 2), 64, 0, s>>>(job, ws);
     RG_LAUNCH_CHECK();
     return RG_SUCCESS;
+}
+
+template <class J>
+int launch_chains(hipStream_t s, const J &job, int nblocks, int style, int *ws, int qlo = 0, int qhi = MAXP)
+{
+    if (int rc = launch_lists(s, job, nblocks, style, ws, qlo, qhi)) return rc;
+    return launch_code(s, job, nblocks, ws);
 }
 
 }  // namespace
@@ -175,6 +188,19 @@ int rg_launch_chains_flat(hipStream_t s, const rg_t1_flat &f, int nblocks, int s
 {
     if (nblocks <= 0) return RG_SUCCESS;
     return launch_chains(s, FlatJobs{f}, nblocks, style, (int *)ws, qlo, qhi);
+}
+
+int rg_launch_chain_lists_flat(hipStream_t s, const rg_t1_flat &f, int nblocks, int style, int32_t *ws, int qlo,
+                               int qhi)
+{
+    if (nblocks <= 0) return RG_SUCCESS;
+    return launch_lists(s, FlatJobs{f}, nblocks, style, (int *)ws, qlo, qhi);
+}
+
+int rg_launch_chain_code_flat(hipStream_t s, const rg_t1_flat &f, int nblocks, const int32_t *ws)
+{
+    if (nblocks <= 0) return RG_SUCCESS;
+    return launch_code(s, FlatJobs{f}, nblocks, (const int *)ws);
 }
 
 int rg_launch_chains_jobs(hipStream_t s, const void *jobs, int njobs, int style, int32_t *ws, int qlo, int qhi)

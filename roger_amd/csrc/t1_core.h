@@ -138,14 +138,15 @@ RG_DEV void t1_lut_fill(uint8_t *lut, int orient)
 
 // Sign symbol for window row i with neighbour windows: own column (cur state
 // above, pre state below), left (Lw), right (rw); sign windows negw/neglw/negrw.
-// raw (bypass pass): the bit is the sign itself, without the XOR prediction.
+// (In a raw pass under BYPASS the sign goes out as itself, with no context:
+// the context modelling pushes the bit alone.)
 RG_DEV void push_sign(SymList &sl, const uint8_t *lut, int i, uint32_t cur, uint32_t sw, uint32_t Lw, uint32_t rw,
-                      uint32_t negw, uint32_t neglw, uint32_t negrw, bool raw = false)
+                      uint32_t negw, uint32_t neglw, uint32_t negrw)
 {
     const uint32_t x = b1(Lw, i) | (b1(neglw, i) << 1) | (b1(rw, i) << 2) | (b1(negrw, i) << 3) |
                        (b1(cur, i - 1) << 4) | (b1(negw, i - 1) << 5) | (b1(sw, i + 1) << 6) | (b1(negw, i + 1) << 7);
     const uint32_t sc = lut[256 + x];
-    sl.push_sym((sc & ~1u) | (b1(negw, i) ^ (raw ? 0u : (sc & 1u))));
+    sl.push_sym((sc & ~1u) | (b1(negw, i) ^ (sc & 1u)));
 }
 
 // ZC context of window row i (own column: cur above, sw below).
@@ -426,7 +427,8 @@ RG_DEV void t1_block(const T1Job &job, uint8_t *lut, CMPhase ph = CMPhase())
 
                 if (pass == 0) {
                     // ---- significance propagation ----
-                    // (a raw pass under BYPASS sends signs unpredicted; this pass is pidx - 1)
+                    // (under BYPASS a raw pass -- this pass is pidx - 1 -- sends its
+                    // decisions and signs as bits, without contexts)
                     const bool rawp = SPLIT && pass_is_raw(job.style, pidx - 1);
                     // Candidates: insignificant samples.  hn: rows with a neighbour that
                     // is significant before the pass (own column above and below, rows
@@ -463,16 +465,28 @@ RG_DEV void t1_block(const T1Job &job, uint8_t *lut, CMPhase ph = CMPhase())
                         if (!wv_any(((grown | (grown << 1) | (grown >> 1)) & cand0 & ~P) != 0)) break;
                     }
                     Lw = lw | (NL << 1);   // the left column's final state for the contexts
-                    uint32_t cur = sw;
+                    if (rawp) {
+                        // raw pass: the decision bits alone (the raw coder reads bit 0
+                        // of each symbol), so no context lookups
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        const int i = k + 1;
-                        if (!((P >> k) & 1u)) continue;
-                        const int bit = (int)b1(bw, i);
-                        sl.push(zc_at(lut, i, cur, sw, Lw, rw), bit);
-                        if (bit) {
-                            push_sign(sl, lut, i, cur, sw, Lw, rw, negw, neglw, negrw, rawp);
-                            cur |= 1u << i;
+                        for (int k = 0; k < 4; ++k) {
+                            if (!((P >> k) & 1u)) continue;
+                            const uint32_t bit = b1(bw, k + 1);
+                            sl.push_sym(bit);
+                            if (bit) sl.push_sym(b1(negw, k + 1));
+                        }
+                    } else {
+                        uint32_t cur = sw;
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            const int i = k + 1;
+                            if (!((P >> k) & 1u)) continue;
+                            const int bit = (int)b1(bw, i);
+                            sl.push(zc_at(lut, i, cur, sw, Lw, rw), bit);
+                            if (bit) {
+                                push_sign(sl, lut, i, cur, sw, Lw, rw, negw, neglw, negrw);
+                                cur |= 1u << i;
+                            }
                         }
                     }
                     S |= (uint64_t)N << r0;
@@ -483,6 +497,14 @@ RG_DEV void t1_block(const T1Job &job, uint8_t *lut, CMPhase ph = CMPhase())
                     const uint32_t rf = (uint32_t)(R >> r0) & 0xFu;
                     const uint32_t M = (sw >> 1) & ~vw & vrow;
                     if (!wv_any(M != 0)) continue;
+                    if (SPLIT && pass_is_raw(job.style, pidx - 1)) {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k)
+                            if ((M >> k) & 1u) sl.push_sym(b1(bw, k + 1));
+                        R |= (uint64_t)M << r0;
+                        emit_stripe(sl, job, spos, sovf, lane);
+                        continue;
+                    }
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
                         const int i = k + 1;

@@ -155,15 +155,19 @@ int rg_launch_t1_parallel(hipStream_t s, const int32_t *q, int pitch, size_t pla
     const rg_t1_flat fj{syms, symcap, cm, poffs, arena, astride, seglens};
     if (ov) {
         // Two phases overlapped: passes 0..9 (the first four bit-planes, where
-        // every code-block's longest MQ chains are) are modelled first and
-        // coded on the second stream while the first models the remaining
-        // passes and codes their (raw or short) chains; packing joins both.
+        // every code-block's longest MQ chains are) are modelled and listed
+        // first and coded on the second stream while the first models the
+        // remaining passes and codes their (raw or short) chains; packing joins
+        // both.  The lists are built before the second modelling launch: built
+        // beside it, their 256-thread workgroups waited for free SIMD slots
+        // until its whole grid was dispatched.
         t1_cm_kernel<<<g, b, 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm, poffs, style, 0,
                                      RG_T1_PHASE_SPLIT, (uint4 *)ov->cms);
         RG_LAUNCH_CHECK();
+        if (int rc = rg_launch_chain_lists_flat(s, fj, ncb, style, ov->chain_ws2, 0, RG_T1_PHASE_SPLIT)) return rc;
         RG_HIP(hipEventRecord(ov->ev_a, s));
         RG_HIP(hipStreamWaitEvent(ov->s2, ov->ev_a, 0));
-        if (int rc = rg_launch_chains_flat(ov->s2, fj, ncb, style, ov->chain_ws2, 0, RG_T1_PHASE_SPLIT)) return rc;
+        if (int rc = rg_launch_chain_code_flat(ov->s2, fj, ncb, ov->chain_ws2)) return rc;
         RG_HIP(hipEventRecord(ov->ev_b, ov->s2));
         t1_cm_kernel<<<g, b, 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm, poffs, style,
                                      RG_T1_PHASE_SPLIT, 1 << 30, (uint4 *)ov->cms);

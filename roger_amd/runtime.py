@@ -283,8 +283,10 @@ def pack_codewords(blobs):
     return np.frombuffer(b"".join(blobs) + b"\0", np.uint8), offs
 
 
-def t1_decode(ctx, descs, results, blobs, q_shape):
-    """Tier-1 decodes code-blocks into zeroed int32 planes of shape (ncomp, h, w)."""
+def t1_decode(ctx, descs, results, blobs, q_shape, style=0, seglen=None):
+    """Tier-1 decodes code-blocks into zeroed int32 planes of shape (ncomp, h, w).
+    style: code-block style (0, RG_STYLE_PARALLEL, RG_STYLE_PARALLEL_BYPASS);
+    TERMALL styles take seglen, [ncb, RG_MAX_PASSES] pass segment lengths."""
     descs = np.ascontiguousarray(descs).view(CB_DESC_DTYPE)
     res = np.ascontiguousarray(results).view(CB_RES_DTYPE)
     n = len(descs)
@@ -298,8 +300,13 @@ def t1_decode(ctx, descs, results, blobs, q_shape):
     nc, h, w = q_shape
     d_q = ctx.alloc(nc * h * w * 4)
     d_q.zero()
-    check(lib().rg_t1_decode(ctx.handle, _vp(d_data.ptr), _vp(d_offs.ptr), _vp(d_res.ptr), _vp(d_desc.ptr), n,
-                             _vp(d_q.ptr), w, h * w))
+    if style:
+        d_seg = ctx.upload(np.ascontiguousarray(seglen, np.int32))
+        check(lib().rg_t1_decode_style(ctx.handle, _vp(d_data.ptr), _vp(d_offs.ptr), _vp(d_res.ptr),
+                                       _vp(d_desc.ptr), n, _vp(d_q.ptr), w, h * w, int(style), _vp(d_seg.ptr)))
+    else:
+        check(lib().rg_t1_decode(ctx.handle, _vp(d_data.ptr), _vp(d_offs.ptr), _vp(d_res.ptr), _vp(d_desc.ptr), n,
+                                 _vp(d_q.ptr), w, h * w))
     return d_q.read(np.int32, q_shape)
 
 

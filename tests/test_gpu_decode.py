@@ -44,6 +44,47 @@ def test_t1_decode_truncated_passes(ctx):
         np.testing.assert_array_equal(got, ref)
 
 
+@pytest.mark.parametrize("style", [O.STYLE_PARALLEL, O.STYLE_PARALLEL_BYPASS])
+def test_t1_decode_style_blocks_vs_oracle(ctx, style):
+    """rg_t1_decode_style on device-resident codewords of the parallel styles
+    (per-pass segments, raw passes under BYPASS): every block of every shape
+    and orientation decodes to the oracle's input, and decoding only a
+    block's first passes gives what the oracle decoder gives."""
+    rng = np.random.default_rng(78)
+    pw = 64
+    blocks, descs, blobs, res, y = [], [], [], [], 0
+    seg = []
+    for (w, h, kind) in CASES:
+        for orient in range(4):
+            b = _block(kind, w, h, rng)
+            data, npass, nbp, passlen = O.t1_encode(b, orient, style=style)
+            blocks.append(b)
+            descs.append((0, y, w, h, 0, orient, 0, 0))
+            res.append((len(data), npass, nbp, 0))
+            blobs.append(data)
+            row = np.zeros(O.MAX_PASSES, np.int32)
+            row[:npass] = passlen
+            seg.append(row)
+            y += h
+    descs = np.array(descs, dtype=R.CB_DESC_DTYPE)
+    res = np.array(res, dtype=R.CB_RES_DTYPE)
+    q = R.t1_decode(ctx, descs, res, blobs, (1, y, pw), style=style, seglen=np.array(seg))[0]
+    for b, d in zip(blocks, descs):
+        np.testing.assert_array_equal(q[d["y0"]:d["y0"] + d["h"], :d["w"]], b)
+    # truncation: the first `keep` passes (their segments) of a dense block
+    b = rng.laplace(0, 400, size=(64, 64)).astype(np.int32)
+    data, npass, nbp, passlen = O.t1_encode(b, 1, style=style)
+    for keep in (1, 4, 10, 11, 12, npass - 1):
+        n = int(passlen[:keep].sum())
+        ref = O.t1_decode(data[:n], 64, 64, 1, nbp, keep, style=style, passlen=passlen[:keep])
+        row = np.zeros((1, O.MAX_PASSES), np.int32)
+        row[0, :keep] = passlen[:keep]
+        got = R.t1_decode(ctx, np.array([(0, 0, 64, 64, 0, 1, 0, 0)], R.CB_DESC_DTYPE),
+                          np.array([(n, keep, nbp, 0)], R.CB_RES_DTYPE), [data[:n]], (1, 64, 64), style=style,
+                          seglen=row)[0]
+        np.testing.assert_array_equal(got, ref, err_msg="keep %d" % keep)
+
+
 @pytest.mark.parametrize("w,h,nc,rev,levels,cb", [
     (512, 512, 1, True, 5, 64), (300, 257, 3, True, 4, 64), (257, 190, 3, False, 6, 64), (128, 96, 1, False, 3, 32),
 ])
