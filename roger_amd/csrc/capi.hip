@@ -436,11 +436,11 @@ struct rg_encoder {
     double delta = 1.0 / 64;
     float step[97], inv[97];
     DevBuf input, planes, q, fcoeff, scratch, desc, res, slots, chan16, syms, cm, poffs, arena, seglens, chains,
-        chains2, cms;
+        chains_e[2], cms;
     HostBuf staging, map;
-    // second stream and events of the overlapped two-phase tier-1 (created on first use)
-    hipStream_t s2 = nullptr;
-    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    // streams and events of the overlapped phased tier-1 (created on first use)
+    hipStream_t se[2] = {nullptr, nullptr};
+    hipEvent_t ev_in[2] = {nullptr, nullptr}, ev_out[2] = {nullptr, nullptr};
     std::vector<rg_cb_desc> h_desc;
     int desc_key[6] = {0, 0, 0, 0, 0, 0};
 };
@@ -459,13 +459,17 @@ extern "C" int rg_encoder_destroy(rg_encoder *e)
     if (!e) return RG_SUCCESS;
     hipSetDevice(e->ctx->device);
     hipStreamSynchronize(e->ctx->stream);
-    if (e->s2) hipStreamSynchronize(e->s2);
+    for (hipStream_t st : e->se)
+        if (st) hipStreamSynchronize(st);
     for (DevBuf *b : {&e->input, &e->planes, &e->q, &e->fcoeff, &e->scratch, &e->desc, &e->res, &e->slots, &e->chan16,
-                      &e->syms, &e->cm, &e->poffs, &e->arena, &e->seglens, &e->chains, &e->chains2, &e->cms})
+                      &e->syms, &e->cm, &e->poffs, &e->arena, &e->seglens, &e->chains, &e->chains_e[0], &e->chains_e[1],
+                      &e->cms})
         b->release();
-    if (e->ev_a) hipEventDestroy(e->ev_a);
-    if (e->ev_b) hipEventDestroy(e->ev_b);
-    if (e->s2) hipStreamDestroy(e->s2);
+    for (int k = 0; k < 2; ++k) {
+        if (e->ev_in[k]) hipEventDestroy(e->ev_in[k]);
+        if (e->ev_out[k]) hipEventDestroy(e->ev_out[k]);
+        if (e->se[k]) hipStreamDestroy(e->se[k]);
+    }
     e->staging.release();
     e->map.release();
     delete e;
@@ -509,15 +513,20 @@ static int run_t1(rg_encoder *e)
             (rc = e->chains.ensure(rg_t1_chain_ws_bytes(e->ncb))))
             return rc;
         rg_t1_overlap ov{};
-        const bool two = rg_t1_two_phase();
+        const int phases = rg_t1_phases();
+        const bool two = phases > 1;
         if (two) {
-            if ((rc = e->chains2.ensure(rg_t1_chain_ws_bytes(e->ncb))) || (rc = e->cms.ensure((size_t)e->ncb * 64 * 16)))
-                return rc;
-            if (!e->s2) RG_HIP(hipStreamCreateWithFlags(&e->s2, hipStreamNonBlocking));
-            if (!e->ev_a) RG_HIP(hipEventCreateWithFlags(&e->ev_a, hipEventDisableTiming));
-            if (!e->ev_b) RG_HIP(hipEventCreateWithFlags(&e->ev_b, hipEventDisableTiming));
-            ov.s2 = e->s2; ov.ev_a = e->ev_a; ov.ev_b = e->ev_b;
-            ov.chain_ws2 = (int32_t *)e->chains2.p; ov.cms = e->cms.p;
+            ov.nearly = phases - 1;
+            if ((rc = e->cms.ensure((size_t)e->ncb * 64 * 16))) return rc;
+            for (int k = 0; k < ov.nearly; ++k) {
+                if ((rc = e->chains_e[k].ensure(rg_t1_chain_ws_bytes(e->ncb)))) return rc;
+                if (!e->se[k]) RG_HIP(hipStreamCreateWithFlags(&e->se[k], hipStreamNonBlocking));
+                if (!e->ev_in[k]) RG_HIP(hipEventCreateWithFlags(&e->ev_in[k], hipEventDisableTiming));
+                if (!e->ev_out[k]) RG_HIP(hipEventCreateWithFlags(&e->ev_out[k], hipEventDisableTiming));
+                ov.se[k] = e->se[k]; ov.ev_in[k] = e->ev_in[k]; ov.ev_out[k] = e->ev_out[k];
+                ov.ws[k] = (int32_t *)e->chains_e[k].p;
+            }
+            ov.cms = e->cms.p;
         }
         rc = rg_launch_t1_parallel(e->ctx->stream, (const int32_t *)e->q.p, e->w, (size_t)e->w * e->h,
                                    (const rg_cb_desc *)e->desc.p, e->ncb, (uint8_t *)e->syms.p, symcap,

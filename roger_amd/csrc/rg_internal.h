@@ -127,16 +127,22 @@ int rg_launch_t1_jobs_split(hipStream_t s, const void *jobs, int njobs);
 int rg_t1_maxp();
 size_t rg_t1_seg_bytes(int symcap);
 size_t rg_t1_chain_ws_bytes(int nblocks);
-// Two-phase overlapped tier-1 (parallel styles, plain path): context
-// modelling split before pass RG_T1_PHASE_SPLIT; the first phase's chains are
-// coded on s2 while s continues.  cms: 64 x 16 bytes per block of modelling
-// state kept between the phases; chain_ws2: a second chain workspace.
+// Overlapped tier-1 (parallel styles, plain path): context modelling runs in
+// phases that end before pass RG_T1_PHASE_SPLIT1 (three-phase form only) and
+// before pass RG_T1_PHASE_SPLIT; the chains of each early phase are listed on
+// s and coded on their own stream (se[k], with a chain workspace ws[k] and the
+// events ev_in[k] / ev_out[k]) while s models the next passes.  cms: 64 x 16
+// bytes per block of modelling state kept between the phases.
+#define RG_T1_PHASE_SPLIT1 7
 #define RG_T1_PHASE_SPLIT 10
-bool rg_t1_two_phase();
+// number of modelling phases: 1, 2 or 3 (RG_T1_PHASES; RG_T1_TWO_PHASE=0 also
+// selects 1)
+int rg_t1_phases();
 struct rg_t1_overlap {
-    hipStream_t s2;
-    hipEvent_t ev_a, ev_b;
-    int32_t *chain_ws2;
+    int nearly;              // early phases: 1 or 2
+    hipStream_t se[2];
+    hipEvent_t ev_in[2], ev_out[2];
+    int32_t *ws[2];
     void *cms;
 };
 // The encoder's flat per-block tier-1 arrays, for the chain kernels

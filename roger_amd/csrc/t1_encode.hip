@@ -154,26 +154,31 @@ int rg_launch_t1_parallel(hipStream_t s, const int32_t *q, int pitch, size_t pla
     const dim3 g((unsigned)ncb), b(64);
     const rg_t1_flat fj{syms, symcap, cm, poffs, arena, astride, seglens};
     if (ov) {
-        // Two phases overlapped: passes 0..9 (the first four bit-planes, where
+        // Phases overlapped: the first passes (the first four bit-planes, where
         // every code-block's longest MQ chains are) are modelled and listed
-        // first and coded on the second stream while the first models the
-        // remaining passes and codes their (raw or short) chains; packing joins
-        // both.  The lists are built before the second modelling launch: built
-        // beside it, their 256-thread workgroups waited for free SIMD slots
-        // until its whole grid was dispatched.
-        t1_cm_kernel<<<g, b, 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm, poffs, style, 0,
-                                     RG_T1_PHASE_SPLIT, (uint4 *)ov->cms);
-        RG_LAUNCH_CHECK();
-        if (int rc = rg_launch_chain_lists_flat(s, fj, ncb, style, ov->chain_ws2, 0, RG_T1_PHASE_SPLIT)) return rc;
-        RG_HIP(hipEventRecord(ov->ev_a, s));
-        RG_HIP(hipStreamWaitEvent(ov->s2, ov->ev_a, 0));
-        if (int rc = rg_launch_chain_code_flat(ov->s2, fj, ncb, ov->chain_ws2)) return rc;
-        RG_HIP(hipEventRecord(ov->ev_b, ov->s2));
+        // first, in one or two phases, and each early phase's chains are coded
+        // on its own stream while the first stream models the next passes; the
+        // last phase codes its (raw or short) chains on the first stream and
+        // packing joins them all.  The lists are built on the first stream
+        // before the next modelling launch: built beside it, their 256-thread
+        // workgroups waited for SIMD slots until its whole grid was dispatched.
+        const int bounds[3] = {0, ov->nearly == 2 ? RG_T1_PHASE_SPLIT1 : RG_T1_PHASE_SPLIT, RG_T1_PHASE_SPLIT};
+        for (int k = 0; k < ov->nearly; ++k) {
+            const int qlo = bounds[k], qhi = bounds[k + 1];
+            t1_cm_kernel<<<g, b, 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm, poffs, style, qlo, qhi,
+                                         (uint4 *)ov->cms);
+            RG_LAUNCH_CHECK();
+            if (int rc = rg_launch_chain_lists_flat(s, fj, ncb, style, ov->ws[k], qlo, qhi)) return rc;
+            RG_HIP(hipEventRecord(ov->ev_in[k], s));
+            RG_HIP(hipStreamWaitEvent(ov->se[k], ov->ev_in[k], 0));
+            if (int rc = rg_launch_chain_code_flat(ov->se[k], fj, ncb, ov->ws[k])) return rc;
+            RG_HIP(hipEventRecord(ov->ev_out[k], ov->se[k]));
+        }
         t1_cm_kernel<<<g, b, 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm, poffs, style,
                                      RG_T1_PHASE_SPLIT, 1 << 30, (uint4 *)ov->cms);
         RG_LAUNCH_CHECK();
         if (int rc = rg_launch_chains_flat(s, fj, ncb, style, chain_ws, RG_T1_PHASE_SPLIT, MAXP)) return rc;
-        RG_HIP(hipStreamWaitEvent(s, ov->ev_b, 0));
+        for (int k = 0; k < ov->nearly; ++k) RG_HIP(hipStreamWaitEvent(s, ov->ev_out[k], 0));
         t1_pack_kernel<<<g, b, 0, s>>>(cm, poffs, arena, astride, seglens, out, slot_bytes, res);
         RG_LAUNCH_CHECK();
         return RG_SUCCESS;
@@ -270,14 +275,16 @@ void rg_t1_job_fill(void *dst, const int32_t *src, int pitch, int w, int h, int 
 }
 
 // Overlapped two-phase tier-1 for the parallel styles unless RG_T1_TWO_PHASE=0.
-bool rg_t1_two_phase()
+int rg_t1_phases()
 {
     static int v = -1;
     if (v < 0) {
-        const char *e = getenv("RG_T1_TWO_PHASE");
-        v = (e && e[0] == '0') ? 0 : 1;
+        const char *two = getenv("RG_T1_TWO_PHASE"), *n = getenv("RG_T1_PHASES");
+        v = 3;
+        if (n && (n[0] == '1' || n[0] == '2' || n[0] == '3')) v = n[0] - '0';
+        if (two && two[0] == '0') v = 1;
     }
-    return v == 1;
+    return v;
 }
 
 // Tier-1 variant: the split context-model + lane-MQ path unless RG_T1_INLINE=1

@@ -3,7 +3,8 @@
 // (not a restatement) can be checked against the oracle without a GPU.
 //
 //   t1_emu MODE DIR PITCH PLANE_STRIDE SLOT [STYLE]
-//     MODE  inline | split | queue_split | parallel | queue_parallel | sorted | two_phase | cm_only
+//     MODE  inline | split | queue_split | parallel | queue_parallel | sorted | two_phase | three_phase |
+//           cm_only
 //     STYLE code-block style bits of the parallel modes (default 6 =
 //           RESET|TERMALL; 7 adds BYPASS)
 //     DIR   holds q.bin (int32 coefficient planes) and desc.bin (rg_cb_desc
@@ -21,6 +22,7 @@
 //                wavefront by t1_chain_lane, as chain_code_kernel does
 //   two_phase    sorted, with context modelling split at pass 10: passes
 //                0-9 modelled and coded, then the rest from the saved state
+//   three_phase  the same with phases [0, 7), [7, 10) and [10, ...)
 //   cm_only      context modelling alone; writes cm.bin and poff.bin
 // The parallel modes also write seglen.bin (MAXP int32 per block).
 //
@@ -309,7 +311,8 @@ int main(int argc, char **argv)
         j.sym = syms + (size_t)k * symcap;
         j.symcap = symcap;
         j.cm = &cm[k];
-        const bool par = mode == "parallel" || mode == "queue_parallel" || mode == "sorted" || mode == "two_phase";
+        const bool par = mode == "parallel" || mode == "queue_parallel" || mode == "sorted" || mode == "two_phase" ||
+                         mode == "three_phase";
         j.style = par ? style : 0;
         j.poff = par ? &poffs[(size_t)k * MAXP] : nullptr;
         j.seg = arena + (size_t)k * astride;
@@ -336,7 +339,7 @@ int main(int argc, char **argv)
         for (int k = 0; k < ncb; ++k) emu::run(cm_body, &jobs[k]);
         spill(dir + "/cm.bin", cm.data(), sizeof(i32x4) * cm.size());
         spill(dir + "/poff.bin", poffs.data(), sizeof(int32_t) * poffs.size());
-    } else if (mode == "sorted" || mode == "two_phase") {
+    } else if (mode == "sorted" || mode == "two_phase" || mode == "three_phase") {
         // counting sort by chain_key over passes [qlo, qhi): MQ buckets
         // (longest first), then raw ones; 64 list entries per wavefront
         auto code_chains = [&](int qlo, int qhi) {
@@ -366,12 +369,14 @@ int main(int argc, char **argv)
             for (int k = 0; k < ncb; ++k) emu::run(cm_body, &jobs[k]);
             code_chains(0, MAXP);
         } else {
-            g_phase.qbeg = 0; g_phase.qend = 10;
-            for (int k = 0; k < ncb; ++k) emu::run(cm_phase_body, &jobs[k]);
-            code_chains(0, 10);
-            g_phase.qbeg = 10; g_phase.qend = 1 << 30;
-            for (int k = 0; k < ncb; ++k) emu::run(cm_phase_body, &jobs[k]);
-            code_chains(10, MAXP);
+            // phase boundaries: passes [0, 10), [10, ...) or [0, 7), [7, 10), [10, ...)
+            const std::vector<int> bounds =
+                mode == "two_phase" ? std::vector<int>{0, 10, 1 << 30} : std::vector<int>{0, 7, 10, 1 << 30};
+            for (size_t k = 0; k + 1 < bounds.size(); ++k) {
+                g_phase.qbeg = bounds[k]; g_phase.qend = bounds[k + 1];
+                for (int b = 0; b < ncb; ++b) emu::run(cm_phase_body, &jobs[b]);
+                code_chains(bounds[k], bounds[k + 1] < MAXP ? bounds[k + 1] : MAXP);
+            }
         }
         for (int k = 0; k < ncb; ++k) emu::run(pack_body, &jobs[k]);
         spill(dir + "/seglen.bin", seglens.data(), sizeof(int32_t) * seglens.size());

@@ -156,7 +156,7 @@ def _run_parallel(emu, tmp_path, mode, q, desc, pitch, plane_stride, slot, style
 
 
 @pytest.mark.parametrize("style", [O.STYLE_PARALLEL, O.STYLE_PARALLEL_BYPASS])
-@pytest.mark.parametrize("mode", ["parallel", "queue_parallel", "sorted", "two_phase"])
+@pytest.mark.parametrize("mode", ["parallel", "queue_parallel", "sorted", "two_phase", "three_phase"])
 @pytest.mark.parametrize("w,h,nc,rev,levels,seed,cb", [
     (97, 61, 1, False, 3, 52, 64),
     (33, 1, 1, False, 1, 56, 64),
@@ -186,7 +186,7 @@ def test_parallel_style_matches_oracle(emu, tmp_path, mode, w, h, nc, rev, level
     assert blobs == data
 
 
-@pytest.mark.parametrize("mode", ["parallel", "sorted", "two_phase"])
+@pytest.mark.parametrize("mode", ["parallel", "sorted", "two_phase", "three_phase"])
 @pytest.mark.parametrize("style", [O.STYLE_PARALLEL, O.STYLE_PARALLEL_BYPASS])
 def test_parallel_style_dense_blocks(emu, tmp_path, style, mode):
     rng = np.random.default_rng(23)
