@@ -108,6 +108,26 @@ void orc_t1_encode(const int32_t *q, int w, int h, int stride, int orient,
  * of pass i's codeword segment (the codeword is their concatenation). */
 void orc_t1_encode_style(const int32_t *q, int w, int h, int stride, int orient, int style,
                          uint8_t *out, int cap, orc_cb_result *res, int32_t *passlen);
+/* orc_t1_encode_style that also writes, per coding pass, the distortion
+ * reduction of the pass (dist[i]; units of orc_t1_dsig / orc_t1_dref). */
+void orc_t1_encode_rd(const int32_t *q, int w, int h, int stride, int orient, int style,
+                      uint8_t *out, int cap, orc_cb_result *res, int32_t *passlen, int64_t *dist);
+/* Squared-error reductions, in doubled index units (integers), of a magnitude
+ * m becoming significant at plane p / of refining its bit p; the decoder
+ * reconstructs the known bits plus half the remaining interval. */
+int64_t orc_t1_dsig(uint32_t m, int p);
+int64_t orc_t1_dref(uint32_t m, int p);
+/* Rate control (PCRD-opt): the upper convex hull of one block's (rate,
+ * weighted distortion reduction) truncation points; hn/hr/hs get the pass
+ * count, cumulative bytes and slope of each hull point (npasses + 1 entries
+ * at most).  Returns the number of hull points. */
+int orc_rd_hull(int npasses, const int32_t *passlen, const int64_t *dist, double w, int32_t *hn, int64_t *hr,
+                double *hs);
+/* Truncation of nblocks code-blocks (per-pass arrays with row stride `stride`)
+ * at one slope threshold: the smallest whose total rate is <= target.
+ * ntrunc / rtrunc: passes and bytes kept per block.  Returns the total. */
+long orc_rate_control(int nblocks, const int32_t *npasses, const int32_t *passlen, const int64_t *dist, int stride,
+                      const double *weight, long target, int32_t *ntrunc, int32_t *rtrunc);
 /* Decodes all `npasses` passes of a code-block back into q (row stride). */
 int orc_t1_decode(const uint8_t *data, int len, int w, int h, int orient,
                   int nbp, int npasses, int32_t *q, int stride);

@@ -91,6 +91,10 @@ def _declare(L):
         "orc_t2_write": [P, P, P, P, P, I, P, ctypes.c_long],
         "orc_encode_image": [P, I, I, P, P, P, I, I, P, P],
         "orc_round_f16": [ctypes.c_float],
+        "orc_t1_encode_rd": [P, I, I, I, I, I, P, I, P, P, P],
+        "orc_t1_dsig": [ctypes.c_uint32, I], "orc_t1_dref": [ctypes.c_uint32, I],
+        "orc_rd_hull": [I, P, P, ctypes.c_double, P, P, P],
+        "orc_rate_control": [I, P, P, P, I, P, ctypes.c_long, P, P],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -98,6 +102,9 @@ def _declare(L):
     L.orc_step_from_expmant.argtypes = [I, I, I]
     L.orc_step_from_expmant.restype = ctypes.c_double
     L.orc_t2_write.restype = ctypes.c_long
+    L.orc_t1_dsig.restype = ctypes.c_int64
+    L.orc_t1_dref.restype = ctypes.c_int64
+    L.orc_rate_control.restype = ctypes.c_long
     L.orc_round_f16.restype = ctypes.c_float
 
 
@@ -296,6 +303,55 @@ def t1_encode(block, orient, cap=1 << 17, style=0):
     if style:
         return bytes(out[:r.len]), r.npasses, r.nbp, passlen[:r.npasses].copy()
     return bytes(out[:r.len]), r.npasses, r.nbp
+
+
+def t1_encode_rd(block, orient, style, cap=1 << 17):
+    """(codeword bytes, npasses, nbp, per-pass segment lengths, per-pass
+    distortion reductions as int64 in doubled index units)."""
+    b = np.ascontiguousarray(block, dtype=np.int32)
+    h, w = b.shape
+    out = np.zeros(cap, np.uint8)
+    r = CbResult()
+    passlen = np.zeros(MAX_PASSES, np.int32)
+    dist = np.zeros(MAX_PASSES, np.int64)
+    lib().orc_t1_encode_rd(_p(b), w, h, w, orient, style, _p(out), cap, ctypes.byref(r), _p(passlen), _p(dist))
+    if r.flags:
+        raise RuntimeError("oracle t1 output overflow")
+    return bytes(out[:r.len]), r.npasses, r.nbp, passlen[:r.npasses].copy(), dist[:r.npasses].copy()
+
+
+def t1_dsig(m, p):
+    return lib().orc_t1_dsig(int(m), int(p))
+
+
+def t1_dref(m, p):
+    return lib().orc_t1_dref(int(m), int(p))
+
+
+def rd_hull(passlen, dist, weight):
+    """Convex-hull truncation points of one block: (passes, bytes, slope) arrays."""
+    pl = np.ascontiguousarray(passlen, np.int32)
+    d = np.ascontiguousarray(dist, np.int64)
+    n = len(pl)
+    hn = np.zeros(n + 1, np.int32)
+    hr = np.zeros(n + 1, np.int64)
+    hs = np.zeros(n + 1, np.float64)
+    m = lib().orc_rd_hull(n, _p(pl), _p(d), float(weight), _p(hn), _p(hr), _p(hs))
+    return hn[:m].copy(), hr[:m].copy(), hs[:m].copy()
+
+
+def rate_control(npasses, passlen, dist, weights, target):
+    """PCRD-opt truncation of many blocks (passlen / dist: [nblocks,
+    MAX_PASSES]) for a byte target: (passes kept, bytes kept, total)."""
+    npasses = np.ascontiguousarray(npasses, np.int32)
+    pl = np.ascontiguousarray(passlen, np.int32)
+    d = np.ascontiguousarray(dist, np.int64)
+    w = np.ascontiguousarray(weights, np.float64)
+    nb = len(npasses)
+    nt = np.zeros(nb, np.int32)
+    rt = np.zeros(nb, np.int32)
+    total = lib().orc_rate_control(nb, _p(npasses), _p(pl), _p(d), pl.shape[1], _p(w), int(target), _p(nt), _p(rt))
+    return nt, rt, int(total)
 
 
 def t1_decode(data, w, h, orient, nbp, npasses, style=0, passlen=None):

@@ -325,6 +325,7 @@ typedef struct {
     uint8_t *f;   /* (w+2) x (h+2) with a zero border */
     uint32_t *m;  /* magnitudes, w x h */
     int w, h, fs, orient;
+    int64_t dacc; /* distortion reduction of the current pass (orc_t1_dsig / orc_t1_dref units) */
 } cbstate;
 
 #define FL(s, x, y) ((s)->f[((y) + 1) * (s)->fs + (x) + 1])
@@ -359,6 +360,29 @@ static inline int any_sig_nbr(const cbstate *s, int x, int y)
     return ((c[-1] | c[1] | c[-fs] | c[fs] | c[-fs - 1] | c[-fs + 1] | c[fs - 1] | c[fs + 1]) & F_SIG) != 0;
 }
 
+/* ---------------- distortion (rate control) ----------------
+ * Squared error of a magnitude m against the decoder's reconstruction, in
+ * index units doubled (so every value is an integer): with the bits of m at
+ * and above plane p known, the reconstruction is their value plus half the
+ * remaining interval, 2^(p-1) (the exact m at p = 0), and an insignificant
+ * sample reconstructs as 0. */
+int64_t orc_t1_dsig(uint32_t m, int p)
+{
+    /* m becomes significant at plane p (m >> p == 1): error 2m -> 2m - 3*2^p */
+    const int64_t e0 = 2 * (int64_t)m;
+    const int64_t e1 = p > 0 ? e0 - (3ll << p) : 0;
+    return e0 * e0 - e1 * e1;
+}
+
+int64_t orc_t1_dref(uint32_t m, int p)
+{
+    /* bit p of a sample significant above p is refined */
+    const int64_t m2 = 2 * (int64_t)m;
+    const int64_t eb = m2 - ((((int64_t)(m >> (p + 1))) << (p + 2)) + (2ll << p));
+    const int64_t ea = p > 0 ? m2 - ((((int64_t)(m >> p)) << (p + 1)) + (1ll << p)) : 0;
+    return eb * eb - ea * ea;
+}
+
 /* ---------------- encoder passes ---------------- */
 /* raw != NULL: the pass is raw-coded (bypass), contexts are not used */
 static void enc_spp(cbstate *s, mq_enc *e, raw_enc *raw, int p)
@@ -376,6 +400,7 @@ static void enc_spp(cbstate *s, mq_enc *e, raw_enc *raw, int p)
                 *f |= F_VIS;
                 if (bit) {
                     const int neg = (*f & F_NEG) ? 1 : 0;
+                    s->dacc += orc_t1_dsig(s->m[y * s->w + x], p);
                     if (raw) {
                         rawe_put(raw, neg);
                     } else {
@@ -395,6 +420,7 @@ static void enc_mrp(cbstate *s, mq_enc *e, raw_enc *raw, int p)
                 uint8_t *f = &FL(s, x, y);
                 if ((*f & (F_SIG | F_VIS)) != F_SIG) continue;
                 const int bit = (s->m[y * s->w + x] >> p) & 1;
+                s->dacc += orc_t1_dref(s->m[y * s->w + x], p);
                 if (raw) {
                     rawe_put(raw, bit);
                 } else {
@@ -425,6 +451,7 @@ static void enc_cup(cbstate *s, mq_enc *e, int p)
                     int xr, sc = sc_of(s, x, y0 + k, &xr);
                     mqe_encode(e, sc, ((*f & F_NEG) ? 1 : 0) ^ xr);
                     *f |= F_SIG;
+                    s->dacc += orc_t1_dsig(s->m[(y0 + k) * s->w + x], p);
                     y = y0 + k + 1;
                 }
             }
@@ -437,6 +464,7 @@ static void enc_cup(cbstate *s, mq_enc *e, int p)
                     int xr, sc = sc_of(s, x, y, &xr);
                     mqe_encode(e, sc, ((*f & F_NEG) ? 1 : 0) ^ xr);
                     *f |= F_SIG;
+                    s->dacc += orc_t1_dsig(s->m[y * s->w + x], p);
                 }
             }
             for (int yy = y0; yy < yend; ++yy) FL(s, x, yy) &= (uint8_t)~F_VIS;
@@ -452,8 +480,14 @@ static void enc_cup(cbstate *s, mq_enc *e, int p)
 void orc_t1_encode_style(const int32_t *q, int w, int h, int stride, int orient, int style,
                          uint8_t *out, int cap, orc_cb_result *res, int32_t *passlen)
 {
+    orc_t1_encode_rd(q, w, h, stride, orient, style, out, cap, res, passlen, NULL);
+}
+
+void orc_t1_encode_rd(const int32_t *q, int w, int h, int stride, int orient, int style,
+                      uint8_t *out, int cap, orc_cb_result *res, int32_t *passlen, int64_t *dist)
+{
     cbstate s;
-    s.w = w; s.h = h; s.fs = w + 2; s.orient = orient;
+    s.w = w; s.h = h; s.fs = w + 2; s.orient = orient; s.dacc = 0;
     s.f = (uint8_t *)calloc((size_t)(w + 2) * (h + 2), 1);
     s.m = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)w * h + 4);
     uint32_t maxm = 0;
@@ -508,6 +542,8 @@ void orc_t1_encode_style(const int32_t *q, int w, int h, int stride, int orient,
                 }
             }
             if (style & ORC_STYLE_RESET) mq_reset_ctx(e.I, e.MPS);
+            if (dist) dist[pass] = s.dacc;
+            s.dacc = 0;
             ++pass;
         }
     res->npasses = 3 * nbp - 2;
