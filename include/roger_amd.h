@@ -97,7 +97,7 @@ const char *rg_last_error(void);
 /* ---- context: device + stream (replaces ocl_args_d_t, src/OCLUtil.h:31-39,
  *      and OCLDeviceManager::init/getInfo, src/OCLDeviceManager.h:27-39) ---- */
 int rg_context_create(int device, rg_context **out);
-int rg_context_destroy(rg_context *ctx);
+int rg_context_destroy(rg_context *ctx);             /* after every encoder, decoder, batch and buffer made on it */
 int rg_context_sync(rg_context *ctx);                 /* OCLEncodeDecode::finish, src/OCLEncodeDecode.cpp:33-36 */
 int rg_context_device_name(rg_context *ctx, char *buf, int cap);
 /* 1 when all work queued on the context's stream has finished, 0 if not. */
@@ -224,6 +224,21 @@ int rg_encoder_get_codeblocks(rg_encoder *enc, rg_cb_desc *desc, rg_cb_result *r
  * RG_MAX_PASSES entries per code-block (ncb * RG_MAX_PASSES, host).  Waits.
  * No counterpart in the reference (its bit-plane coder has no output). */
 int rg_encoder_get_pass_lengths(rg_encoder *enc, int32_t *seglen);
+/* Rate control (PCRD-opt).  No counterpart in the reference (its bit-plane
+ * coder writes no bytes, src/CL/oclbpc.cl:279-746); T.800 leaves truncation
+ * to the encoder.  enable_rate_control(enc, 1) before tier-1 runs makes the
+ * context modeller record which samples each significance-propagation pass
+ * made significant (512 bytes per bit-plane and code-block).  Then, for a
+ * parallel style, rate_control cuts every code-block after the coding pass
+ * that the slope threshold of the smallest total distortion within
+ * `target_bytes` of code-block data selects (the codestream adds its
+ * headers); weights: one positive factor per code-block scaling its
+ * distortion (NULL: step^2 times the band's synthesis energy).  Results,
+ * codewords and write_j2k reflect the cut; calling it again with another
+ * target recuts from the full codewords.  *kept = code-block bytes kept.
+ * Needs coefficient magnitudes below 2^24.  Waits. */
+int rg_encoder_enable_rate_control(rg_encoder *enc, int on);
+int rg_encoder_rate_control(rg_encoder *enc, long target_bytes, const double *weights, long *kept);
 /* Tier-2: the JPEG 2000 codestream (.j2k: one tile, one layer, LRCP) of the
  * last encode.  out == NULL only reports the size.  The reference has no
  * tier-2 (it stops after its bit-plane coder, src/OCLEncoder.cpp:43-52). */

@@ -436,7 +436,8 @@ struct rg_encoder {
     double delta = 1.0 / 64;
     float step[97], inv[97];
     DevBuf input, planes, q, fcoeff, scratch, desc, res, slots, chan16, syms, cm, poffs, arena, seglens, chains,
-        chains_e[2], cms;
+        chains_e[2], cms, sppm, rdws, rdw;
+    int rd = 0;   // rate control enabled: the context modeller records significance-pass masks
     HostBuf staging, map;
     // streams and events of the overlapped phased tier-1 (created on first use)
     hipStream_t se[2] = {nullptr, nullptr};
@@ -463,7 +464,7 @@ extern "C" int rg_encoder_destroy(rg_encoder *e)
         if (st) hipStreamSynchronize(st);
     for (DevBuf *b : {&e->input, &e->planes, &e->q, &e->fcoeff, &e->scratch, &e->desc, &e->res, &e->slots, &e->chan16,
                       &e->syms, &e->cm, &e->poffs, &e->arena, &e->seglens, &e->chains, &e->chains_e[0], &e->chains_e[1],
-                      &e->cms})
+                      &e->cms, &e->sppm, &e->rdws, &e->rdw})
         b->release();
     for (int k = 0; k < 2; ++k) {
         if (e->ev_in[k]) hipEventDestroy(e->ev_in[k]);
@@ -528,11 +529,13 @@ static int run_t1(rg_encoder *e)
             }
             ov.cms = e->cms.p;
         }
+        if (e->rd && (rc = e->sppm.ensure(rg_rd_sppm_bytes(e->ncb)))) return rc;
         rc = rg_launch_t1_parallel(e->ctx->stream, (const int32_t *)e->q.p, e->w, (size_t)e->w * e->h,
                                    (const rg_cb_desc *)e->desc.p, e->ncb, (uint8_t *)e->syms.p, symcap,
                                    (int4 *)e->cm.p, (int32_t *)e->poffs.p, (uint8_t *)e->arena.p,
                                    (int32_t *)e->seglens.p, (uint8_t *)e->slots.p, e->slot_bytes,
-                                   (rg_cb_result *)e->res.p, e->style, (int32_t *)e->chains.p, two ? &ov : nullptr);
+                                   (rg_cb_result *)e->res.p, e->style, (int32_t *)e->chains.p, two ? &ov : nullptr,
+                                   e->rd ? (uint64_t *)e->sppm.p : nullptr);
         if (rc == RG_SUCCESS) e->t1_ran = 1;
         return rc;
     }
@@ -680,6 +683,93 @@ extern "C" int rg_encoder_bpc(rg_encoder *e, int cbw, int cbh)
     if (rg_codeblocks(e->w, e->h, e->levels, cbw, cbh, nullptr, 0) < 0) return RG_ERR_INVALID;
     e->cbw = cbw; e->cbh = cbh;
     return run_t1(e);
+}
+
+// ---- rate control ----
+
+// Energy of the 1-D synthesis basis function of a coefficient of the lowpass
+// (hi = 0) or highpass band at decomposition level `level`: the band's
+// synthesis filter, then level-1 lowpass synthesis steps (upsample by two,
+// filter).  Filters: T.800's 5/3 (g0 = [1 2 1]/2, g1 = [-1 -2 6 -2 -1]/8) and
+// CDF 9/7 synthesis pairs with unit DC gain of the lowpass.
+static double synth_energy(int filter, int level, int hi)
+{
+    static const double g0_53[] = {0.5, 1.0, 0.5};
+    static const double g1_53[] = {-0.125, -0.25, 0.75, -0.25, -0.125};
+    static const double g0_97[] = {-0.091271763114250, -0.057543526228500, 0.591271763114250, 1.115087052457000,
+                                   0.591271763114250, -0.057543526228500, -0.091271763114250};
+    static const double g1_97[] = {0.026748757410810, 0.016864118442875, -0.078223266528990, -0.266864118442875,
+                                   0.602949018236360, -0.266864118442875, -0.078223266528990, 0.016864118442875,
+                                   0.026748757410810};
+    const double *g0 = filter == RG_FILTER_97 ? g0_97 : g0_53, *g1 = filter == RG_FILTER_97 ? g1_97 : g1_53;
+    const int n0 = filter == RG_FILTER_97 ? 7 : 3, n1 = filter == RG_FILTER_97 ? 9 : 5;
+    std::vector<double> v(hi ? g1 : g0, (hi ? g1 : g0) + (hi ? n1 : n0));
+    for (int l = 1; l < level; ++l) {
+        std::vector<double> u(2 * v.size() - 1, 0.0), o(2 * v.size() - 1 + n0 - 1, 0.0);
+        for (size_t i = 0; i < v.size(); ++i) u[2 * i] = v[i];
+        for (size_t i = 0; i < u.size(); ++i)
+            for (int k = 0; k < n0; ++k) o[i + k] += u[i] * g0[k];
+        v.swap(o);
+    }
+    double e = 0;
+    for (double x : v) e += x * x;
+    return e;
+}
+
+// Default weight of each code-block: squared-error contribution in the image
+// domain of one squared doubled index unit, step^2 * (horizontal synthesis
+// energy) * (vertical synthesis energy) / 4 (components weighted equally).
+static void default_rd_weights(const rg_encoder *e, std::vector<double> &w)
+{
+    w.resize(e->ncb);
+    const int filter = e->filter;
+    for (int i = 0; i < e->ncb; ++i) {
+        const rg_cb_desc &d = e->h_desc[i];
+        const int level = d.res == 0 ? e->levels : e->levels + 1 - d.res;
+        const int hx = d.orient == 1 || d.orient == 3, hy = d.orient == 2 || d.orient == 3;
+        double g = 1.0;
+        if (e->levels > 0) g = synth_energy(filter, level, hx) * synth_energy(filter, level, hy);
+        const double st = e->lossy ? (double)e->step[d.band] : 1.0;
+        w[i] = st * st * g * 0.25;
+    }
+}
+
+extern "C" int rg_encoder_enable_rate_control(rg_encoder *e, int on)
+{
+    RG_CHECK(e, "null encoder");
+    e->rd = on ? 1 : 0;
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_encoder_rate_control(rg_encoder *e, long target_bytes, const double *weights, long *kept)
+{
+    RG_CHECK(e, "null encoder");
+    if (!e->t1_ran) return rg_fail(RG_ERR_NOT_READY, "tier-1 has not run");
+    RG_CHECK(e->style & RG_STYLE_TERMALL, "rate control needs a parallel code-block style (TERMALL)");
+    if (!e->rd) return rg_fail(RG_ERR_NOT_READY, "rate control was not enabled before tier-1 ran");
+    RG_CHECK(target_bytes >= 0, "negative rate target %ld", target_bytes);
+    std::vector<double> w;
+    if (weights) {
+        w.assign(weights, weights + e->ncb);
+        for (int i = 0; i < e->ncb; ++i)
+            RG_CHECK(std::isfinite(w[i]) && w[i] > 0, "block weight %d is not a positive finite number", i);
+    } else {
+        default_rd_weights(e, w);
+    }
+    int rc;
+    if ((rc = e->rdws.ensure(rg_rd_ws_bytes(e->ncb))) || (rc = e->rdw.ensure(sizeof(double) * e->ncb))) return rc;
+    RG_HIP(hipSetDevice(e->ctx->device));
+    RG_HIP(hipMemcpyAsync(e->rdw.p, w.data(), sizeof(double) * e->ncb, hipMemcpyHostToDevice, e->ctx->stream));
+    long k = 0;
+    int err = 0;
+    rc = rg_launch_rate_control(e->ctx->stream, (const int32_t *)e->q.p, e->w, (size_t)e->w * e->h,
+                                (const rg_cb_desc *)e->desc.p, e->ncb, (const int4 *)e->cm.p,
+                                (const uint64_t *)e->sppm.p, (const int32_t *)e->seglens.p, (const double *)e->rdw.p,
+                                target_bytes, e->rdws.p, (rg_cb_result *)e->res.p, &k, &err);
+    if (rc) return rc;
+    if (err) return rg_fail(RG_ERR_INVALID, "rate control needs coefficient magnitudes below 2^24");
+    if (kept) *kept = k;
+    return RG_SUCCESS;
 }
 
 int rg_launch_sat_i16(hipStream_t s, const int32_t *src, long n, int16_t *dst);

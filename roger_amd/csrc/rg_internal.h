@@ -170,7 +170,17 @@ int rg_launch_t1_mq_jobs(hipStream_t s, const void *jobs, int njobs);
 int rg_launch_t1_parallel(hipStream_t s, const int32_t *q, int pitch, size_t plane_stride, const rg_cb_desc *desc,
                           int ncb, uint8_t *syms, int symcap, int4 *cm, int32_t *poffs, uint8_t *arena,
                           int32_t *seglens, uint8_t *out, int slot_bytes, rg_cb_result *res, int style,
-                          int32_t *chain_ws, const rg_t1_overlap *ov);
+                          int32_t *chain_ws, const rg_t1_overlap *ov, uint64_t *sppm = nullptr);
+// Rate control (t1_rate.hip): per-pass distortions from the coefficients and
+// the context modeller's significance-pass masks (sppm, rg_rd_sppm_bytes),
+// convex hulls, a slope threshold for `target` code-block bytes, and the
+// results cut at it.  kept_out / err_out (optional) wait for the stream.
+size_t rg_rd_ws_bytes(int ncb);
+size_t rg_rd_sppm_bytes(int ncb);
+int rg_launch_rate_control(hipStream_t s, const int32_t *q, int pitch, size_t plane_stride, const rg_cb_desc *desc,
+                           int ncb, const int4 *cm, const uint64_t *sppm, const int32_t *seglens,
+                           const double *weights, long target, void *ws, rg_cb_result *res, long *kept_out,
+                           int *err_out);
 int rg_launch_t1_jobs_parallel(hipStream_t s, const void *jobs, int njobs, int style, int32_t *chain_ws);
 // Persistent batched tier-1 (any style): nwaves workgroups pull job indices
 // from *counter (reset by a kernel inside the launch sequence).

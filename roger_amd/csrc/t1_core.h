@@ -271,7 +271,11 @@ struct T1Job {
     uint8_t *seg;   // parallel style: this block's segment arena (seg_bytes(symcap) bytes)
     int32_t *seglen;// parallel style: codeword segment length of each pass (MAXP)
     u32x4 *cms;     // two-phase context modelling: 64 per-lane state words kept between the phases
+    uint64_t *sppm = nullptr;  // split path (may be null): per bit-plane p, word p*64+lane = the rows of
+                               // column `lane` that became significant in plane p's significance pass
 };
+
+constexpr int RD_PLANES = 32;   // bit-planes per block in the significance-pass mask records
 
 // Context modelling of coding passes [qbeg, qend) only (split path).  A phase
 // ends before a significance-propagation pass (qend = 1 mod 3), where the
@@ -416,6 +420,7 @@ RG_DEV void t1_block(const T1Job &job, uint8_t *lut, CMPhase ph = CMPhase())
                 return;
             }
             ++pidx;
+            const uint64_t S0 = S;
             for (int s = 0; s < nstripes; ++s) {
                 const int r0 = 4 * s;
                 const uint32_t vrow = colok ? (uint32_t)(rowmask >> r0) & 0xFu : 0u;
@@ -568,6 +573,7 @@ RG_DEV void t1_block(const T1Job &job, uint8_t *lut, CMPhase ph = CMPhase())
                 else mq_stripe(m, ctxv, mqtv, sl, lane);
             }
             if (pass == 2) V = 0;
+            if (SPLIT && pass == 0 && job.sppm) job.sppm[p * 64 + lane] = S & ~S0;
         }
     }
 

@@ -95,7 +95,8 @@ __global__ void reset_counter_kernel(int *counter) { *counter = 0; }
 __global__ void __launch_bounds__(64) t1_cm_kernel(const int32_t *__restrict__ q, int pitch, size_t plane_stride,
                                                    const rg_cb_desc *__restrict__ desc, uint8_t *__restrict__ syms,
                                                    int symcap, int4 *__restrict__ cm, int32_t *__restrict__ poffs,
-                                                   int style, int qbeg, int qend, uint4 *__restrict__ cms)
+                                                   int style, int qbeg, int qend, uint4 *__restrict__ cms,
+                                                   uint64_t *__restrict__ sppm)
 {
     const rg_cb_desc d = desc[blockIdx.x];
     T1Job job;
@@ -107,6 +108,7 @@ __global__ void __launch_bounds__(64) t1_cm_kernel(const int32_t *__restrict__ q
     job.poff = poffs ? poffs + (size_t)blockIdx.x * MAXP : nullptr;
     job.style = style;
     job.cms = cms ? cms + (size_t)blockIdx.x * 64 : nullptr;
+    job.sppm = sppm ? sppm + (size_t)blockIdx.x * RD_PLANES * 64 : nullptr;
     __shared__ uint8_t lut[LUT_BYTES];
     CMPhase ph;
     ph.qbeg = qbeg;
@@ -146,7 +148,7 @@ size_t rg_t1_seg_bytes(int symcap) { return (size_t)seg_bytes(symcap); }
 int rg_launch_t1_parallel(hipStream_t s, const int32_t *q, int pitch, size_t plane_stride, const rg_cb_desc *desc,
                           int ncb, uint8_t *syms, int symcap, int4 *cm, int32_t *poffs, uint8_t *arena,
                           int32_t *seglens, uint8_t *out, int slot_bytes, rg_cb_result *res, int style,
-                          int32_t *chain_ws, const rg_t1_overlap *ov)
+                          int32_t *chain_ws, const rg_t1_overlap *ov, uint64_t *sppm)
 {
     if (ncb <= 0) return RG_SUCCESS;
     if (!chain_ws) return rg_fail(RG_ERR_INVALID, "parallel code-block styles need the chain workspace");
@@ -166,7 +168,7 @@ int rg_launch_t1_parallel(hipStream_t s, const int32_t *q, int pitch, size_t pla
         for (int k = 0; k < ov->nearly; ++k) {
             const int qlo = bounds[k], qhi = bounds[k + 1];
             t1_cm_kernel<<<g, b, 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm, poffs, style, qlo, qhi,
-                                         (uint4 *)ov->cms);
+                                         (uint4 *)ov->cms, sppm);
             RG_LAUNCH_CHECK();
             if (int rc = rg_launch_chain_lists_flat(s, fj, ncb, style, ov->ws[k], qlo, qhi)) return rc;
             RG_HIP(hipEventRecord(ov->ev_in[k], s));
@@ -175,7 +177,7 @@ int rg_launch_t1_parallel(hipStream_t s, const int32_t *q, int pitch, size_t pla
             RG_HIP(hipEventRecord(ov->ev_out[k], ov->se[k]));
         }
         t1_cm_kernel<<<g, b, 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm, poffs, style,
-                                     RG_T1_PHASE_SPLIT, 1 << 30, (uint4 *)ov->cms);
+                                     RG_T1_PHASE_SPLIT, 1 << 30, (uint4 *)ov->cms, sppm);
         RG_LAUNCH_CHECK();
         if (int rc = rg_launch_chains_flat(s, fj, ncb, style, chain_ws, RG_T1_PHASE_SPLIT, MAXP)) return rc;
         for (int k = 0; k < ov->nearly; ++k) RG_HIP(hipStreamWaitEvent(s, ov->ev_out[k], 0));
@@ -184,7 +186,7 @@ int rg_launch_t1_parallel(hipStream_t s, const int32_t *q, int pitch, size_t pla
         return RG_SUCCESS;
     }
     t1_cm_kernel<<<g, b, 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm, poffs, style, 0, 1 << 30,
-                                 nullptr);
+                                 nullptr, sppm);
     RG_LAUNCH_CHECK();
     if (int rc = rg_launch_chains_flat(s, fj, ncb, style, chain_ws, 0, MAXP)) return rc;
     t1_pack_kernel<<<dim3((unsigned)ncb), dim3(64), 0, s>>>(cm, poffs, arena, astride, seglens, out, slot_bytes, res);
@@ -212,7 +214,7 @@ int rg_launch_t1_split(hipStream_t s, const int32_t *q, int pitch, size_t plane_
 {
     if (ncb <= 0) return RG_SUCCESS;
     t1_cm_kernel<<<dim3((unsigned)ncb), dim3(64), 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm, nullptr,
-                                                          0, 0, 1 << 30, nullptr);
+                                                          0, 0, 1 << 30, nullptr, nullptr);
     RG_LAUNCH_CHECK();
     return rg_launch_t1_mq(s, syms, symcap, cm, ncb, out, slot_bytes, res);
 }

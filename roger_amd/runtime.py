@@ -2,6 +2,7 @@
 stage entry points, taking numpy arrays on the host and DeviceBuffer objects
 on the device.  No torch: device memory comes from rg_malloc."""
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -21,6 +22,10 @@ class Context:
         check(lib().rg_context_create(device, ctypes.byref(h)))
         self.handle = h
         self.device = device
+        self._owned = weakref.WeakSet()   # encoders, decoders, batches, buffers made on this context
+
+    def _own(self, obj):
+        self._owned.add(obj)
 
     def sync(self):
         check(lib().rg_context_sync(self.handle))
@@ -52,7 +57,15 @@ class Context:
         return ms.value
 
     def close(self):
+        """Releases what was made on this context (still alive), then the
+        context: the library needs its objects destroyed before their
+        context."""
         if self.handle:
+            for o in list(self._owned):
+                try:
+                    o.close()
+                except Exception:
+                    pass
             lib().rg_context_destroy(self.handle)
             self.handle = None
 
@@ -72,6 +85,7 @@ class DeviceBuffer:
         p = ctypes.c_void_p()
         check(lib().rg_malloc(ctx.handle, self.nbytes, ctypes.byref(p)))
         self.ptr = p.value
+        ctx._own(self)
 
     def write(self, arr):
         arr = np.ascontiguousarray(arr)
@@ -91,6 +105,8 @@ class DeviceBuffer:
         if self.ptr:
             lib().rg_free(self.ctx.handle, _vp(self.ptr))
             self.ptr = None
+
+    close = free
 
     def __del__(self):
         try:
@@ -202,6 +218,7 @@ class Batch:
         h = ctypes.c_void_p()
         check(lib().rg_batch_create(ctx.handle, ctypes.byref(h)))
         self.handle = h
+        ctx._own(self)
 
     def plan(self, images, cbw=64, cbh=64, use_graph=True, style=0, waves_per_cu=0):
         """images: list of dicts {buf: DeviceBuffer, w, h, ncomp, reversible, levels, delta}."""
@@ -319,6 +336,7 @@ class Decoder:
         h = ctypes.c_void_p()
         check(lib().rg_decoder_create(ctx.handle, ctypes.byref(h)))
         self.handle = h
+        ctx._own(self)
 
     def decode(self, descs, results, blobs, w, h, ncomp, levels=5, reversible=True, cbw=64, cbh=64,
                delta=1.0 / 64, style=0, seglen=None):
@@ -370,6 +388,7 @@ class Encoder:
         h = ctypes.c_void_p()
         check(lib().rg_encoder_create(ctx.handle, ctypes.byref(h)))
         self.handle = h
+        ctx._own(self)
 
     def encode_u8(self, img, levels=5, reversible=True, cbw=64, cbh=64, delta=1.0 / 64, ll_fp16=False,
                   d_src=None, w=None, h=None, ncomp=None, style=0):
@@ -421,6 +440,21 @@ class Encoder:
         out = np.zeros((n, RG_MAX_PASSES), np.int32)
         check(lib().rg_encoder_get_pass_lengths(self.handle, out.ctypes.data_as(ctypes.c_void_p)))
         return out
+
+    def enable_rate_control(self, on=True):
+        """Before encoding: record what rate control needs (parallel styles)."""
+        check(lib().rg_encoder_enable_rate_control(self.handle, int(bool(on))))
+
+    def rate_control(self, target_bytes, weights=None):
+        """Cuts the code-blocks at the PCRD-opt slope threshold for at most
+        target_bytes of code-block data; returns the bytes kept."""
+        kept = ctypes.c_long()
+        wp = None
+        if weights is not None:
+            weights = np.ascontiguousarray(weights, np.float64)
+            wp = weights.ctypes.data_as(ctypes.c_void_p)
+        check(lib().rg_encoder_rate_control(self.handle, int(target_bytes), wp, ctypes.byref(kept)))
+        return kept.value
 
     def j2k(self):
         """The JPEG 2000 codestream of the last encode (bytes)."""
