@@ -22,6 +22,8 @@ Also reported:
                   codestream bytes).
   decode        : the reverse path on the device-resident codewords (tier-1
                   decode, dequantisation + inverse 9/7, inverse ICT).
+  rate_control  : PCRD-opt cuts of the same encode to 2, 1 and 0.25 bits per
+                  pixel: time of the cut and PSNR of its native decode.
   cpu_baseline  : the CPU restatement (oracle) of the same encode on the host
                   cores, on the full image.
 """
@@ -226,14 +228,18 @@ def main():
     d_u8 = ctx.alloc(NC * W * H)
     step_f = R.band_steps(LEVELS, DELTA)[0]
 
-    def decode_rate(style):
-        enc.encode_u8(None, levels=LEVELS, reversible=False, cbw=CB, cbh=CB, delta=DELTA, d_src=d_src, w=W, h=H,
-                      ncomp=NC, style=style)
-        _, res_s, blobs_s = enc.codeblocks()
+    def decode_rate(style, e=None):
+        # e: an encoder already holding (possibly rate-cut) codewords; default:
+        # encode again in `style`
+        if e is None:
+            e = enc
+            e.encode_u8(None, levels=LEVELS, reversible=False, cbw=CB, cbh=CB, delta=DELTA, d_src=d_src, w=W, h=H,
+                        ncomp=NC, style=style)
+        _, res_s, blobs_s = e.codeblocks()
         data_s, offs_s = R.pack_codewords(blobs_s)
         d_data, d_offs = ctx.upload(data_s), ctx.upload(offs_s)
         d_res_s = ctx.upload(np.ascontiguousarray(res_s).view(np.int32))
-        d_seg = ctx.upload(np.ascontiguousarray(enc.pass_lengths(), np.int32)) if style else None
+        d_seg = ctx.upload(np.ascontiguousarray(e.pass_lengths(), np.int32)) if style else None
 
         def decode_once():
             check_rc(lib().rg_t1_decode_style(ctx.handle, ctypes.c_void_p(d_data.ptr), ctypes.c_void_p(d_offs.ptr),
@@ -251,11 +257,34 @@ def main():
         rec = d_u8.read(np.uint8, img.shape)
         for b in (d_data, d_offs, d_res_s) + ((d_seg,) if d_seg is not None else ()):
             b.free()
+        mse = float(np.mean((rec.astype(np.float64) - img.astype(np.float64)) ** 2))
         return {"ms": round(t, 4), "mpix_s": round(mpix / (t * 1e-3), 1), "t1_decode_ms": round(t_t1d, 4),
-                "max_abs_err_vs_input": int(np.abs(rec.astype(int) - img.astype(int)).max())}
+                "max_abs_err_vs_input": int(np.abs(rec.astype(int) - img.astype(int)).max()),
+                "psnr_db": round(10 * np.log10(255.0 ** 2 / mse), 2) if mse > 0 else 99.0}
 
     dec_main = decode_rate(STYLE)
     dec_std = decode_rate(0) if STYLE else dec_main
+    # rate control (PCRD-opt) on the same image: the encode recording the
+    # significance-pass masks, then cuts to code-block data budgets in bits per
+    # pixel (default band weights), each decoded by the native reverse path
+    rate = None
+    if STYLE:
+        enc_rc = R.Encoder(ctx)
+        enc_rc.enable_rate_control()
+        t_enc_rc = timed(lambda: enc_rc.encode_u8(None, levels=LEVELS, reversible=False, cbw=CB, cbh=CB,
+                                                  delta=DELTA, d_src=d_src, w=W, h=H, ncomp=NC, style=STYLE))
+        full_rc = int(enc_rc.codeblocks()[1]["len"].sum())
+        rate = {"what": "encode with masks recorded, then rg_encoder_rate_control (distortions, hulls, "
+                        "threshold, cut; host-synchronous) to each budget; PSNR of the native decode of the cut",
+                "encode_with_masks_ms": round(t_enc_rc, 4), "full_bytes": full_rc, "points": []}
+        for bpp in (2.0, 1.0, 0.25):
+            target = int(W * H * bpp / 8)
+            kept = enc_rc.rate_control(target)
+            t_rc = timed(lambda: enc_rc.rate_control(target))
+            dd = decode_rate(STYLE, enc_rc)
+            rate["points"].append({"bpp": bpp, "target_bytes": target, "kept_bytes": kept,
+                                   "rate_control_ms": round(t_rc, 4), "psnr_db": dd["psnr_db"]})
+        enc_rc.close()
     # measured device-to-device copy ceiling (read + write of the same bytes as the level-1 DWT)
     cp_n = 4 * W * H * NC
     d_cp = ctx.alloc(cp_n)
@@ -301,6 +330,7 @@ def main():
         "decode": dict(what="device-resident codewords of the same code-block style -> tier-1 decode -> "
                             "dequantisation + inverse 9/7 -> inverse ICT -> uint8 RGB", **dec_main,
                        standard_style=dec_std),
+        "rate_control": rate,
     }
 
     if not args.no_batch:

@@ -65,6 +65,8 @@ extern "C" int rg_context_destroy(rg_context *c)
     if (!c) return RG_SUCCESS;
     hipSetDevice(c->device);
     if (c->stream) { hipStreamSynchronize(c->stream); hipStreamDestroy(c->stream); }
+    for (hipStream_t st : c->side)
+        if (st) { hipStreamSynchronize(st); hipStreamDestroy(st); }
     if (c->ev_start) hipEventDestroy(c->ev_start);
     if (c->ev_stop) hipEventDestroy(c->ev_stop);
     delete c;
@@ -439,8 +441,8 @@ struct rg_encoder {
         chains_e[2], cms, sppm, rdws, rdw;
     int rd = 0;   // rate control enabled: the context modeller records significance-pass masks
     HostBuf staging, map;
-    // streams and events of the overlapped phased tier-1 (created on first use)
-    hipStream_t se[2] = {nullptr, nullptr};
+    // events of the overlapped phased tier-1 (created on first use; its side
+    // streams belong to the context)
     hipEvent_t ev_in[2] = {nullptr, nullptr}, ev_out[2] = {nullptr, nullptr};
     std::vector<rg_cb_desc> h_desc;
     int desc_key[6] = {0, 0, 0, 0, 0, 0};
@@ -460,7 +462,7 @@ extern "C" int rg_encoder_destroy(rg_encoder *e)
     if (!e) return RG_SUCCESS;
     hipSetDevice(e->ctx->device);
     hipStreamSynchronize(e->ctx->stream);
-    for (hipStream_t st : e->se)
+    for (hipStream_t st : e->ctx->side)
         if (st) hipStreamSynchronize(st);
     for (DevBuf *b : {&e->input, &e->planes, &e->q, &e->fcoeff, &e->scratch, &e->desc, &e->res, &e->slots, &e->chan16,
                       &e->syms, &e->cm, &e->poffs, &e->arena, &e->seglens, &e->chains, &e->chains_e[0], &e->chains_e[1],
@@ -469,7 +471,6 @@ extern "C" int rg_encoder_destroy(rg_encoder *e)
     for (int k = 0; k < 2; ++k) {
         if (e->ev_in[k]) hipEventDestroy(e->ev_in[k]);
         if (e->ev_out[k]) hipEventDestroy(e->ev_out[k]);
-        if (e->se[k]) hipStreamDestroy(e->se[k]);
     }
     e->staging.release();
     e->map.release();
@@ -521,10 +522,10 @@ static int run_t1(rg_encoder *e)
             if ((rc = e->cms.ensure((size_t)e->ncb * 64 * 16))) return rc;
             for (int k = 0; k < ov.nearly; ++k) {
                 if ((rc = e->chains_e[k].ensure(rg_t1_chain_ws_bytes(e->ncb)))) return rc;
-                if (!e->se[k]) RG_HIP(hipStreamCreateWithFlags(&e->se[k], hipStreamNonBlocking));
+                if (!e->ctx->side[k]) RG_HIP(hipStreamCreateWithFlags(&e->ctx->side[k], hipStreamNonBlocking));
                 if (!e->ev_in[k]) RG_HIP(hipEventCreateWithFlags(&e->ev_in[k], hipEventDisableTiming));
                 if (!e->ev_out[k]) RG_HIP(hipEventCreateWithFlags(&e->ev_out[k], hipEventDisableTiming));
-                ov.se[k] = e->se[k]; ov.ev_in[k] = e->ev_in[k]; ov.ev_out[k] = e->ev_out[k];
+                ov.se[k] = e->ctx->side[k]; ov.ev_in[k] = e->ev_in[k]; ov.ev_out[k] = e->ev_out[k];
                 ov.ws[k] = (int32_t *)e->chains_e[k].p;
             }
             ov.cms = e->cms.p;
@@ -722,16 +723,18 @@ static double synth_energy(int filter, int level, int hi)
 static void default_rd_weights(const rg_encoder *e, std::vector<double> &w)
 {
     w.resize(e->ncb);
-    const int filter = e->filter;
-    for (int i = 0; i < e->ncb; ++i) {
-        const rg_cb_desc &d = e->h_desc[i];
-        const int level = d.res == 0 ? e->levels : e->levels + 1 - d.res;
-        const int hx = d.orient == 1 || d.orient == 3, hy = d.orient == 2 || d.orient == 3;
+    // one weight per band (3 * levels + 1 of them), then per block
+    std::vector<double> band_w(3 * e->levels + 1, 0.0);
+    for (int band = 0; band <= 3 * e->levels; ++band) {
+        const int orient = band == 0 ? 0 : 1 + (band - 1) % 3;
+        const int level = band == 0 ? e->levels : e->levels - (band - 1) / 3;
+        const int hx = orient == 1 || orient == 3, hy = orient == 2 || orient == 3;
         double g = 1.0;
-        if (e->levels > 0) g = synth_energy(filter, level, hx) * synth_energy(filter, level, hy);
-        const double st = e->lossy ? (double)e->step[d.band] : 1.0;
-        w[i] = st * st * g * 0.25;
+        if (e->levels > 0) g = synth_energy(e->filter, level, hx) * synth_energy(e->filter, level, hy);
+        const double st = e->lossy ? (double)e->step[band] : 1.0;
+        band_w[band] = st * st * g * 0.25;
     }
+    for (int i = 0; i < e->ncb; ++i) w[i] = band_w[e->h_desc[i].band];
 }
 
 extern "C" int rg_encoder_enable_rate_control(rg_encoder *e, int on)

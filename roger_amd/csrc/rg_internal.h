@@ -8,6 +8,10 @@ struct rg_context {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    // side streams of the overlapped phased tier-1, shared by the context's
+    // encoders (created on first use): streams per encoder would outnumber
+    // the hardware queues and serialise the overlap
+    hipStream_t side[2] = {nullptr, nullptr};
 };
 
 // Thread-local last-error message; set by rg_fail().

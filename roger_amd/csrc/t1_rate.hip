@@ -94,11 +94,17 @@ __global__ void rd_init_kernel(RdState *st, unsigned long long *hist, unsigned l
     }
 }
 
+// The sums over the low bits go through the block's bit-planes: with B_j the
+// column's rows whose magnitude has bit j set (bit r = row r), the sum of
+// g = m mod 2^p over a row set X is sum_{j<p} 2^j * popcount(X & B_j).  The
+// planes are built once per block into workgroup memory (lane-private words,
+// so no barrier), then each plane p reads its lower planes back.
 __global__ void __launch_bounds__(64) rd_dist_kernel(const int32_t *__restrict__ q, int pitch, size_t plane_stride,
                                                      const rg_cb_desc *__restrict__ desc, const int4 *__restrict__ cm,
                                                      const uint64_t *__restrict__ sppm, int64_t *__restrict__ dist,
                                                      RdState *st)
 {
+    __shared__ uint64_t Bs[RD_MAX_NBP * 64];
     const int lane = wv_lane();
     const size_t b = blockIdx.x;
     const rg_cb_desc d = desc[b];
@@ -109,43 +115,52 @@ __global__ void __launch_bounds__(64) rd_dist_kernel(const int32_t *__restrict__
         if (lane == 0) atomicOr(&st->err, 1u);
         return;
     }
-    const int32_t *src = q + (size_t)d.comp * plane_stride + (size_t)d.y0 * pitch + d.x0;
-    const bool colok = lane < d.w;
-    uint32_t mag[64];
-#pragma unroll
-    for (int r = 0; r < 64; ++r) {
-        const int v = (colok && r < d.h) ? src[(size_t)r * pitch + lane] : 0;
-        mag[r] = (uint32_t)(v < 0 ? -v : v);
-    }
-    const uint64_t *M = sppm + b * RD_PLANES * 64;
-    for (int p = nbp - 1; p >= 0; --p) {
-        const uint64_t nm = p == nbp - 1 ? 0 : M[p * 64 + lane];
-        const uint32_t low = (1u << p) - 1u;
-        // g sums below 64 * 2^23: 32 bits suffice
-        uint32_t gS = 0, gC = 0, g1 = 0, g0 = 0, cS = 0, cC = 0, c1 = 0, c0 = 0;
+    {
+        const int32_t *src = q + (size_t)d.comp * plane_stride + (size_t)d.y0 * pitch + d.x0;
+        const bool colok = lane < d.w;
+        uint32_t mag[64];
 #pragma unroll
         for (int r = 0; r < 64; ++r) {
-            const uint32_t m = mag[r], qq = m >> p, g = m & low;
-            const bool sig = qq == 1, ref = qq >= 2, spp = (nm >> r) & 1u, one = qq & 1u;
-            gS += (sig && spp) ? g : 0u;
-            cS += (sig && spp) ? 1u : 0u;
-            gC += (sig && !spp) ? g : 0u;
-            cC += (sig && !spp) ? 1u : 0u;
-            g1 += (ref && one) ? g : 0u;
-            c1 += (ref && one) ? 1u : 0u;
-            g0 += (ref && !one) ? g : 0u;
-            c0 += (ref && !one) ? 1u : 0u;
+            const int v = (colok && r < d.h) ? src[(size_t)r * pitch + lane] : 0;
+            mag[r] = (uint32_t)(v < 0 ? -v : v);
         }
+        for (int j = 0; j < nbp; ++j) {
+            uint32_t lo = 0, hi = 0;
+#pragma unroll
+            for (int r = 0; r < 32; ++r) lo |= ((mag[r] >> j) & 1u) << r;
+#pragma unroll
+            for (int r = 0; r < 32; ++r) hi |= ((mag[r + 32] >> j) & 1u) << r;
+            Bs[j * 64 + lane] = ((uint64_t)hi << 32) | lo;
+        }
+    }
+    const uint64_t *M = sppm + b * RD_PLANES * 64;
+    uint64_t SIG = 0;   // rows significant above the current plane
+    for (int p = nbp - 1; p >= 0; --p) {
+        const uint64_t Bp = Bs[p * 64 + lane];
+        const uint64_t nm = p == nbp - 1 ? 0 : M[p * 64 + lane];
+        const uint64_t Q1 = Bp & ~SIG;                 // significant at p
+        const uint64_t XS = Q1 & nm, XC = Q1 & ~nm;     // ... in the propagation / the cleanup pass
+        const uint64_t X1 = Bp & SIG, X0 = SIG & ~Bp;   // refined with a 1 / a 0
+        // g sums below 64 * 2^23: 32 bits suffice
+        uint32_t gS = 0, gC = 0, g1 = 0, g0 = 0;
+        for (int j = 0; j < p; ++j) {
+            const uint64_t Bj = Bs[j * 64 + lane];
+            gS += (uint32_t)__popcll(XS & Bj) << j;
+            gC += (uint32_t)__popcll(XC & Bj) << j;
+            g1 += (uint32_t)__popcll(X1 & Bj) << j;
+            g0 += (uint32_t)__popcll(X0 & Bj) << j;
+        }
+        const int64_t cS = __popcll(XS), cC = __popcll(XC), c1 = __popcll(X1), c0 = __popcll(X0);
         int64_t dS, dC, dR;
         if (p == 0) {
-            dS = 4 * (int64_t)cS;
-            dC = 4 * (int64_t)cC;
-            dR = 4 * (int64_t)c0;
+            dS = 4 * cS;
+            dC = 4 * cC;
+            dR = 4 * c0;
         } else {
             const int64_t P = (int64_t)1 << p;
-            dS = 3 * P * ((int64_t)cS * P + 4 * (int64_t)gS);
-            dC = 3 * P * ((int64_t)cC * P + 4 * (int64_t)gC);
-            dR = P * (4 * (int64_t)g1 - (int64_t)c1 * P + 3 * (int64_t)c0 * P - 4 * (int64_t)g0);
+            dS = 3 * P * (cS * P + 4 * (int64_t)gS);
+            dC = 3 * P * (cC * P + 4 * (int64_t)gC);
+            dR = P * (4 * (int64_t)g1 - c1 * P + 3 * c0 * P - 4 * (int64_t)g0);
         }
         dS = wave_sum_i64(dS);
         dC = wave_sum_i64(dC);
@@ -160,6 +175,7 @@ __global__ void __launch_bounds__(64) rd_dist_kernel(const int32_t *__restrict__
                 D[k + 2] = dC;
             }
         }
+        SIG |= Bp;
     }
 }
 
