@@ -169,6 +169,9 @@ int orc_band_mb(const orc_j2k_params *p, int band);
 long orc_t2_write(const orc_j2k_params *p, const orc_cb_result *cbres,
                   const long *cboff, const uint8_t *cbdata, const int32_t *passlen,
                   int ncb_per_comp, uint8_t *out, long cap);
+long orc_t2_write_layers(const orc_j2k_params *p, const orc_cb_result *cbres,
+                         const long *cboff, const uint8_t *cbdata, const int32_t *passlen,
+                         int ncb_per_comp, int nlayers, const int32_t *layer_np, uint8_t *out, long cap);
 
 /* ---------------- synthetic inputs ---------------- */
 uint64_t orc_splitmix64(uint64_t *state);

@@ -89,6 +89,7 @@ def _declare(L):
         "orc_codeblocks": [I, I, I, I, I, P, I],
         "orc_band_range": [I, I],
         "orc_t2_write": [P, P, P, P, P, I, P, ctypes.c_long],
+        "orc_t2_write_layers": [P, P, P, P, P, I, I, P, P, ctypes.c_long],
         "orc_encode_image": [P, I, I, P, P, P, I, I, P, P],
         "orc_round_f16": [ctypes.c_float],
         "orc_t1_encode_rd": [P, I, I, I, I, I, P, I, P, P, P],
@@ -102,6 +103,7 @@ def _declare(L):
     L.orc_step_from_expmant.argtypes = [I, I, I]
     L.orc_step_from_expmant.restype = ctypes.c_double
     L.orc_t2_write.restype = ctypes.c_long
+    L.orc_t2_write_layers.restype = ctypes.c_long
     L.orc_t1_dsig.restype = ctypes.c_int64
     L.orc_t1_dref.restype = ctypes.c_int64
     L.orc_rate_control.restype = ctypes.c_long
@@ -422,9 +424,11 @@ def j2k_params(w, h, ncomp, levels, reversible, cbw=64, cbh=64, delta=1.0 / 64, 
     return p
 
 
-def t2_write(params, results, datas, passlen=None):
+def t2_write(params, results, datas, passlen=None, layers=None):
     """results: int array [ncomp*ncb, 4]; datas: list of bytes (same order);
-    passlen: [ncomp*ncb, MAX_PASSES] segment lengths (TERMALL style)."""
+    passlen: [ncomp*ncb, MAX_PASSES] segment lengths (TERMALL style);
+    layers: [ncomp*ncb, nlayers] passes of each block after each quality
+    layer (None: one layer)."""
     n = len(datas)
     ncb = n // params.ncomp
     res = (CbResult * n)()
@@ -438,8 +442,15 @@ def t2_write(params, results, datas, passlen=None):
     cap = len(blob) + 65536 + 64 * n
     out = np.zeros(cap, np.uint8)
     pl = None if passlen is None else np.ascontiguousarray(passlen, np.int32)
-    sz = lib().orc_t2_write(ctypes.byref(params), ctypes.cast(res, P), _p(offs), _p(blob_np),
-                            None if pl is None else _p(pl), ncb, _p(out), cap)
+    if layers is None:
+        sz = lib().orc_t2_write(ctypes.byref(params), ctypes.cast(res, P), _p(offs), _p(blob_np),
+                                None if pl is None else _p(pl), ncb, _p(out), cap)
+    else:
+        ly = np.ascontiguousarray(layers, np.int32)
+        cap += 8 * ly.size
+        out = np.zeros(cap, np.uint8)
+        sz = lib().orc_t2_write_layers(ctypes.byref(params), ctypes.cast(res, P), _p(offs), _p(blob_np),
+                                       None if pl is None else _p(pl), ncb, ly.shape[1], _p(ly), _p(out), cap)
     if sz < 0:
         raise RuntimeError("t2 write failed")
     return bytes(out[:sz])

@@ -134,6 +134,23 @@ long orc_t2_write(const orc_j2k_params *p, const orc_cb_result *cbres,
                   const long *cboff, const uint8_t *cbdata, const int32_t *passlen,
                   int ncb_per_comp, uint8_t *out, long cap)
 {
+    return orc_t2_write_layers(p, cbres, cboff, cbdata, passlen, ncb_per_comp, 1, NULL, out, cap);
+}
+
+/* Quality layers (B.9-B.10, LRCP): layer_np[blk * nlayers + l] is the number
+ * of coding passes block blk has after layer l (non-decreasing in l; NULL:
+ * one layer with every block's npasses).  A packet of layer l carries each
+ * block's passes beyond the previous layer's.  Inclusion: the tag tree holds
+ * the first layer with a pass (nlayers if none) and is coded against l + 1
+ * until the block is included, then one bit per layer says whether the
+ * block contributes; the zero-bit-plane tag tree is coded at first
+ * inclusion; Lblock (3 at first inclusion) persists across layers.  Several
+ * layers need the TERMALL style (segment lengths at every pass end). */
+long orc_t2_write_layers(const orc_j2k_params *p, const orc_cb_result *cbres,
+                         const long *cboff, const uint8_t *cbdata, const int32_t *passlen,
+                         int ncb_per_comp, int nlayers, const int32_t *layer_np, uint8_t *out, long cap)
+{
+    if (nlayers < 1 || (nlayers > 1 && (!layer_np || !passlen || !(p->cblk_style & ORC_STYLE_TERMALL)))) return -1;
     wbuf w = { out, 0, cap, 0 };
     int nbands = 3 * p->levels + 1;
     /* SOC, SIZ */
@@ -145,7 +162,7 @@ long orc_t2_write(const orc_j2k_params *p, const orc_cb_result *cbres,
     for (int c = 0; c < p->ncomp; ++c) { put8(&w, p->bitdepth - 1); put8(&w, 1); put8(&w, 1); }
     /* COD */
     put16(&w, 0xFF52); put16(&w, 12); put8(&w, 0);
-    put8(&w, 0); put16(&w, 1); put8(&w, p->mct);
+    put8(&w, 0); put16(&w, nlayers); put8(&w, p->mct);
     put8(&w, p->levels); put8(&w, p->cbw_log2 - 2); put8(&w, p->cbh_log2 - 2); put8(&w, p->cblk_style);
     put8(&w, p->reversible ? 1 : 0);
     /* QCD */
@@ -175,69 +192,105 @@ long orc_t2_write(const orc_j2k_params *p, const orc_cb_result *cbres,
     }
     if (acc != ncb_per_comp) return -1;
 
-    for (int r = 0; r <= p->levels; ++r) {
-        int b0 = r == 0 ? 0 : 1 + 3 * (r - 1), b1 = r == 0 ? 1 : b0 + 3;
-        for (int c = 0; c < p->ncomp; ++c) {
-            const orc_cb_result *cr = cbres + (long)c * ncb_per_comp;
-            const long *co = cboff + (long)c * ncb_per_comp;
-            const int32_t *pl = passlen ? passlen + (long)c * ncb_per_comp * ORC_MAX_PASSES : NULL;
-            int any = 0;
-            for (int k = b0; k < b1; ++k)
-                for (int i = 0; i < bnx[k] * bny[k]; ++i) if (cr[bstart[k] + i].npasses > 0) any = 1;
-            bitw b;
-            bw_init(&b, &w);
-            if (!any) { bw_bit(&b, 0); bw_flush(&b); continue; }
-            bw_bit(&b, 1);
-            for (int k = b0; k < b1; ++k) {
-                if (bnx[k] == 0) continue;
-                tagtree incl, imsb;
-                tt_init(&incl, bnx[k], bny[k]);
-                tt_init(&imsb, bnx[k], bny[k]);
-                int Mb = orc_band_mb(p, k);
-                for (int j = 0; j < bny[k]; ++j)
-                    for (int i = 0; i < bnx[k]; ++i) {
-                        const orc_cb_result *q = &cr[bstart[k] + j * bnx[k] + i];
-                        tt_set(&incl, i, j, q->npasses > 0 ? 0 : 1);
-                        tt_set(&imsb, i, j, Mb - q->nbp);
-                    }
-                for (int j = 0; j < bny[k]; ++j)
-                    for (int i = 0; i < bnx[k]; ++i) {
-                        const orc_cb_result *q = &cr[bstart[k] + j * bnx[k] + i];
-                        tt_encode(&incl, &b, i, j, 1);
-                        if (q->npasses == 0) continue;
-                        tt_encode(&imsb, &b, i, j, 999);
-                        put_npasses(&b, q->npasses);
-                        /* B.10.7: one Lblock increment, then the length of each
-                         * codeword segment (every pass with TERMALL, else one
-                         * segment of npasses passes) in Lblock + log2(passes) bits */
-                        const int32_t *segs = pl ? pl + (long)(bstart[k] + j * bnx[k] + i) * ORC_MAX_PASSES : NULL;
-                        const int termall = (p->cblk_style & ORC_STYLE_TERMALL) && segs;
-                        const int nseg = termall ? q->npasses : 1;
-                        const int seg_passes_log2 = termall ? 0 : floorlog2(q->npasses);
-                        int lblock = 3, inc = 0;
-                        for (int sgi = 0; sgi < nseg; ++sgi) {
-                            int len = termall ? segs[sgi] : q->len;
-                            int nb = floorlog2(len) + 1, have = lblock + seg_passes_log2;
-                            if (nb - have > inc) inc = nb - have;
-                        }
-                        for (int t = 0; t < inc; ++t) bw_bit(&b, 1);
-                        bw_bit(&b, 0);
-                        lblock += inc;
-                        for (int sgi = 0; sgi < nseg; ++sgi)
-                            bw_bits(&b, (uint32_t)(termall ? segs[sgi] : q->len), lblock + seg_passes_log2);
-                    }
-                tt_free(&incl); tt_free(&imsb);
-            }
-            bw_flush(&b);
-            for (int k = b0; k < b1; ++k)
-                for (int i = 0; i < bnx[k] * bny[k]; ++i) {
-                    const orc_cb_result *q = &cr[bstart[k] + i];
-                    if (q->npasses == 0) continue;
-                    const uint8_t *src = cbdata + co[bstart[k] + i];
-                    for (int t = 0; t < q->len; ++t) put8(&w, src[t]);
+    const long ncb_all = (long)ncb_per_comp * p->ncomp;
+    /* passes of block i after layer l */
+#define NP(i, l) (layer_np ? layer_np[(long)(i) * nlayers + (l)] : cbres[i].npasses)
+    /* per (component, band) tag trees, per block: passes sent, Lblock */
+    tagtree *incl = (tagtree *)calloc((size_t)p->ncomp * nbands, sizeof(tagtree));
+    tagtree *imsb = (tagtree *)calloc((size_t)p->ncomp * nbands, sizeof(tagtree));
+    int *sent = (int *)calloc((size_t)ncb_all, sizeof(int));
+    int *lblk = (int *)malloc(sizeof(int) * (size_t)ncb_all);
+    for (long i = 0; i < ncb_all; ++i) lblk[i] = 3;
+    for (int c = 0; c < p->ncomp; ++c)
+        for (int k = 0; k < nbands; ++k) {
+            if (bnx[k] == 0) continue;
+            tagtree *ti = &incl[c * nbands + k], *tm = &imsb[c * nbands + k];
+            tt_init(ti, bnx[k], bny[k]);
+            tt_init(tm, bnx[k], bny[k]);
+            const int Mb = orc_band_mb(p, k);
+            for (int j = 0; j < bny[k]; ++j)
+                for (int i = 0; i < bnx[k]; ++i) {
+                    const long blk = (long)c * ncb_per_comp + bstart[k] + j * bnx[k] + i;
+                    int first = nlayers;
+                    for (int l = nlayers - 1; l >= 0; --l) if (NP(blk, l) > 0) first = l;
+                    tt_set(ti, i, j, first);
+                    tt_set(tm, i, j, Mb - cbres[blk].nbp);
                 }
         }
-    }
+    for (int l = 0; l < nlayers; ++l)
+        for (int r = 0; r <= p->levels; ++r) {
+            int b0 = r == 0 ? 0 : 1 + 3 * (r - 1), b1 = r == 0 ? 1 : b0 + 3;
+            for (int c = 0; c < p->ncomp; ++c) {
+                const long cb0 = (long)c * ncb_per_comp;
+                int any = 0;
+                for (int k = b0; k < b1; ++k)
+                    for (int i = 0; i < bnx[k] * bny[k]; ++i)
+                        if (NP(cb0 + bstart[k] + i, l) > sent[cb0 + bstart[k] + i]) any = 1;
+                bitw b;
+                bw_init(&b, &w);
+                if (!any) { bw_bit(&b, 0); bw_flush(&b); continue; }
+                bw_bit(&b, 1);
+                for (int k = b0; k < b1; ++k) {
+                    if (bnx[k] == 0) continue;
+                    for (int j = 0; j < bny[k]; ++j)
+                        for (int i = 0; i < bnx[k]; ++i) {
+                            const long blk = cb0 + bstart[k] + j * bnx[k] + i;
+                            const orc_cb_result *q = &cbres[blk];
+                            const int prev = sent[blk], now = NP(blk, l), nnew = now - prev;
+                            if (prev == 0) {
+                                tt_encode(&incl[c * nbands + k], &b, i, j, l + 1);
+                                if (nnew <= 0) continue;
+                                tt_encode(&imsb[c * nbands + k], &b, i, j, 999);
+                            } else {
+                                bw_bit(&b, nnew > 0);
+                                if (nnew <= 0) continue;
+                            }
+                            put_npasses(&b, nnew);
+                            /* B.10.7: one Lblock increment, then the length of each
+                             * codeword segment (every new pass with TERMALL, else one
+                             * segment of npasses passes) in Lblock + log2(passes) bits */
+                            const int32_t *segs = passlen ? passlen + blk * ORC_MAX_PASSES : NULL;
+                            const int termall = (p->cblk_style & ORC_STYLE_TERMALL) && segs;
+                            const int nseg = termall ? nnew : 1;
+                            const int seg_passes_log2 = termall ? 0 : floorlog2(nnew);
+                            int inc = 0;
+                            for (int sgi = 0; sgi < nseg; ++sgi) {
+                                int len = termall ? segs[prev + sgi] : q->len;
+                                int nb = floorlog2(len) + 1, have = lblk[blk] + seg_passes_log2;
+                                if (nb - have > inc) inc = nb - have;
+                            }
+                            for (int t = 0; t < inc; ++t) bw_bit(&b, 1);
+                            bw_bit(&b, 0);
+                            lblk[blk] += inc;
+                            for (int sgi = 0; sgi < nseg; ++sgi)
+                                bw_bits(&b, (uint32_t)(termall ? segs[prev + sgi] : q->len),
+                                        lblk[blk] + seg_passes_log2);
+                        }
+                }
+                bw_flush(&b);
+                /* body: the new passes' bytes, block order */
+                for (int k = b0; k < b1; ++k)
+                    for (int i = 0; i < bnx[k] * bny[k]; ++i) {
+                        const long blk = cb0 + bstart[k] + i;
+                        const int prev = sent[blk], now = NP(blk, l);
+                        if (now <= prev) continue;
+                        long from = 0, to = cbres[blk].len;
+                        if (passlen && (p->cblk_style & ORC_STYLE_TERMALL)) {
+                            const int32_t *segs = passlen + blk * ORC_MAX_PASSES;
+                            to = 0;
+                            for (int t = 0; t < now; ++t) { if (t == prev) from = to; to += segs[t]; }
+                        }
+                        const uint8_t *src = cbdata + cboff[blk];
+                        for (long t = from; t < to; ++t) put8(&w, src[t]);
+                        sent[blk] = now;
+                    }
+            }
+        }
+#undef NP
+    for (int c = 0; c < p->ncomp; ++c)
+        for (int k = 0; k < nbands; ++k)
+            if (bnx[k]) { tt_free(&incl[c * nbands + k]); tt_free(&imsb[c * nbands + k]); }
+    free(incl); free(imsb); free(sent); free(lblk);
     long psot = w.pos - sot;
     if (sot + 10 <= cap) {
         out[sot + 6] = (uint8_t)(psot >> 24); out[sot + 7] = (uint8_t)(psot >> 16);

@@ -189,3 +189,69 @@ def test_threshold_is_the_smallest_that_fits():
         if prev is not None:
             assert (nt >= prev).all()                        # monotone in the target
         prev = nt
+
+
+def _oracle_image_rd(img, levels, rev, style):
+    """Oracle encode of an image plus every block's per-pass distortions."""
+    h, w = img.shape[:2]
+    nc = 1 if img.ndim == 2 else img.shape[2]
+    rs, data, q, passlen = O.encode_image(img, levels=levels, reversible=rev, cbw=64, cbh=64, style=style,
+                                          return_passlen=True)
+    cbs = O.codeblocks(w, h, levels, 64, 64)
+    dists = np.zeros((len(rs), O.MAX_PASSES), np.int64)
+    for c in range(nc):
+        for i, d in enumerate(cbs):
+            k = c * len(cbs) + i
+            blk = q[c, d["y0"]:d["y0"] + d["h"], d["x0"]:d["x0"] + d["w"]]
+            _, npass, _, pl, dist = O.t1_encode_rd(blk, d["orient"], style)
+            assert npass == rs[k, 1] and (pl == passlen[k, :npass]).all()
+            dists[k, :npass] = dist
+    return rs, data, passlen, dists
+
+
+def test_quality_layers_decode_like_single_cuts():
+    """Multi-layer codestreams from the oracle writer: one layer is the
+    single-layer writer's bytes; OpenJPEG decoding the first l layers of a
+    three-layer stream gives exactly the image of the single-layer stream cut
+    at layer l's target."""
+    import io
+    from PIL import Image
+    from roger_amd import synth
+    img = synth.image(192, 160, 3, 12)
+    style = O.STYLE_PARALLEL_BYPASS
+    rs, data, passlen, dists = _oracle_image_rd(img, 4, False, style)
+    params = O.j2k_params(192, 160, 3, 4, False, 64, 64, 1.0 / 64, style=style)
+    weights = np.ones(len(rs))
+    full = int(rs[:, 0].sum())
+    targets = [full // 30, full // 8, full // 3]
+    cuts, layers = [], []
+    for t in targets:
+        nt, rt, _ = O.rate_control(rs[:, 1], passlen, dists, weights, t)
+        r2 = rs.copy()
+        r2[:, 0], r2[:, 1] = rt, nt
+        cuts.append(O.t2_write(params, r2, [data[i][:rt[i]] for i in range(len(data))], passlen))
+        layers.append(nt)
+    layers = np.stack(layers, 1)
+    assert (np.diff(layers, axis=1) >= 0).all()
+    # one layer == the single-layer writer
+    assert O.t2_write(params, rs, data, passlen, layers=rs[:, 1:2]) == O.t2_write(params, rs, data, passlen)
+    r3 = rs.copy()
+    r3[:, 1] = layers[:, -1]
+    r3[:, 0] = [int(passlen[i, :layers[i, -1]].sum()) for i in range(len(rs))]
+    cs = O.t2_write(params, r3, data, passlen, layers=layers)
+    for l in range(3):
+        got = decode_layers(cs, l + 1)
+        ref = np.array(Image.open(io.BytesIO(cuts[l])))
+        np.testing.assert_array_equal(got, ref, err_msg="layer %d" % l)
+
+
+def decode_layers(cs, n):
+    """OpenJPEG (Pillow) decode of the first n quality layers.  Pillow reads
+    its `layers` setting only together with a resolution reduction, so the
+    decoder tile's arguments are set directly."""
+    import io
+    from PIL import Image, ImageFile
+    im = Image.open(io.BytesIO(cs))
+    t = im.tile[0]
+    im.tile = [ImageFile._Tile(t[0], t[1], t[2], (t[3][0], 0, n, t[3][3], t[3][4]))]
+    return np.array(im)
