@@ -239,6 +239,13 @@ int rg_encoder_get_pass_lengths(rg_encoder *enc, int32_t *seglen);
  * Needs coefficient magnitudes below 2^24.  Waits. */
 int rg_encoder_enable_rate_control(rg_encoder *enc, int on);
 int rg_encoder_rate_control(rg_encoder *enc, long target_bytes, const double *weights, long *kept);
+/* Quality layers: targets[0..nlayers) non-decreasing budgets of code-block
+ * data; layer l holds every block's passes up to layer l's threshold beyond
+ * those of layer l-1, and write_j2k writes nlayers layers (a decoder that
+ * stops after layer l gets the cut at targets[l]).  *kept: the last layer's
+ * bytes.  A new encode goes back to one layer. */
+int rg_encoder_rate_control_layers(rg_encoder *enc, int nlayers, const long *targets, const double *weights,
+                                   long *kept);
 /* Tier-2: the JPEG 2000 codestream (.j2k: one tile, one layer, LRCP) of the
  * last encode.  out == NULL only reports the size.  The reference has no
  * tier-2 (it stops after its bit-plane coder, src/OCLEncoder.cpp:43-52). */

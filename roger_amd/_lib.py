@@ -106,6 +106,7 @@ SIGNATURES = {
     "rg_encoder_get_pass_lengths": (I, [P, P]),
     "rg_encoder_enable_rate_control": (I, [P, I]),
     "rg_encoder_rate_control": (I, [P, L, P, P]),
+    "rg_encoder_rate_control_layers": (I, [P, I, P, P, P]),
     "rg_encoder_write_j2k": (I, [P, P, L, P]),
     "rg_encoder_map_dwt_out": (I, [P, P]),
     "rg_encoder_unmap_dwt_out": (I, [P, P]),

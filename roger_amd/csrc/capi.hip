@@ -440,6 +440,9 @@ struct rg_encoder {
     DevBuf input, planes, q, fcoeff, scratch, desc, res, slots, chan16, syms, cm, poffs, arena, seglens, chains,
         chains_e[2], cms, sppm, rdws, rdw;
     int rd = 0;   // rate control enabled: the context modeller records significance-pass masks
+    int nlayers = 1;               // quality layers of the last rate control (1 after a new encode)
+    std::vector<int32_t> layer_np; // per block, passes after each layer (nlayers > 1)
+    DevBuf d_layer_np;
     HostBuf staging, map;
     // events of the overlapped phased tier-1 (created on first use; its side
     // streams belong to the context)
@@ -466,7 +469,7 @@ extern "C" int rg_encoder_destroy(rg_encoder *e)
         if (st) hipStreamSynchronize(st);
     for (DevBuf *b : {&e->input, &e->planes, &e->q, &e->fcoeff, &e->scratch, &e->desc, &e->res, &e->slots, &e->chan16,
                       &e->syms, &e->cm, &e->poffs, &e->arena, &e->seglens, &e->chains, &e->chains_e[0], &e->chains_e[1],
-                      &e->cms, &e->sppm, &e->rdws, &e->rdw})
+                      &e->cms, &e->sppm, &e->rdws, &e->rdw, &e->d_layer_np})
         b->release();
     for (int k = 0; k < 2; ++k) {
         if (e->ev_in[k]) hipEventDestroy(e->ev_in[k]);
@@ -501,6 +504,8 @@ static int prepare_codeblocks(rg_encoder *e)
 
 static int run_t1(rg_encoder *e)
 {
+    e->nlayers = 1;
+    e->layer_np.clear();
     int rc = prepare_codeblocks(e);
     if (rc) return rc;
     e->slot_bytes = std::max(256, e->cbw * e->cbh * 3);
@@ -746,11 +751,20 @@ extern "C" int rg_encoder_enable_rate_control(rg_encoder *e, int on)
 
 extern "C" int rg_encoder_rate_control(rg_encoder *e, long target_bytes, const double *weights, long *kept)
 {
+    return rg_encoder_rate_control_layers(e, 1, &target_bytes, weights, kept);
+}
+
+extern "C" int rg_encoder_rate_control_layers(rg_encoder *e, int nlayers, const long *targets, const double *weights,
+                                              long *kept)
+{
     RG_CHECK(e, "null encoder");
     if (!e->t1_ran) return rg_fail(RG_ERR_NOT_READY, "tier-1 has not run");
     RG_CHECK(e->style & RG_STYLE_TERMALL, "rate control needs a parallel code-block style (TERMALL)");
     if (!e->rd) return rg_fail(RG_ERR_NOT_READY, "rate control was not enabled before tier-1 ran");
-    RG_CHECK(target_bytes >= 0, "negative rate target %ld", target_bytes);
+    RG_CHECK(nlayers >= 1 && nlayers <= 65535 && targets, "bad layer count %d", nlayers);
+    for (int l = 0; l < nlayers; ++l)
+        RG_CHECK(targets[l] >= 0 && (l == 0 || targets[l] >= targets[l - 1]),
+                 "rate targets must be non-negative and non-decreasing");
     std::vector<double> w;
     if (weights) {
         w.assign(weights, weights + e->ncb);
@@ -760,7 +774,9 @@ extern "C" int rg_encoder_rate_control(rg_encoder *e, long target_bytes, const d
         default_rd_weights(e, w);
     }
     int rc;
-    if ((rc = e->rdws.ensure(rg_rd_ws_bytes(e->ncb))) || (rc = e->rdw.ensure(sizeof(double) * e->ncb))) return rc;
+    if ((rc = e->rdws.ensure(rg_rd_ws_bytes(e->ncb))) || (rc = e->rdw.ensure(sizeof(double) * e->ncb)) ||
+        (nlayers > 1 && (rc = e->d_layer_np.ensure(sizeof(int32_t) * (size_t)e->ncb * nlayers))))
+        return rc;
     RG_HIP(hipSetDevice(e->ctx->device));
     RG_HIP(hipMemcpyAsync(e->rdw.p, w.data(), sizeof(double) * e->ncb, hipMemcpyHostToDevice, e->ctx->stream));
     long k = 0;
@@ -768,9 +784,17 @@ extern "C" int rg_encoder_rate_control(rg_encoder *e, long target_bytes, const d
     rc = rg_launch_rate_control(e->ctx->stream, (const int32_t *)e->q.p, e->w, (size_t)e->w * e->h,
                                 (const rg_cb_desc *)e->desc.p, e->ncb, (const int4 *)e->cm.p,
                                 (const uint64_t *)e->sppm.p, (const int32_t *)e->seglens.p, (const double *)e->rdw.p,
-                                target_bytes, e->rdws.p, (rg_cb_result *)e->res.p, &k, &err);
+                                nlayers, targets, e->rdws.p, nlayers > 1 ? (int32_t *)e->d_layer_np.p : nullptr,
+                                (rg_cb_result *)e->res.p, &k, &err);
     if (rc) return rc;
     if (err) return rg_fail(RG_ERR_INVALID, "rate control needs coefficient magnitudes below 2^24");
+    e->nlayers = nlayers;
+    e->layer_np.clear();
+    if (nlayers > 1) {
+        e->layer_np.resize((size_t)e->ncb * nlayers);
+        RG_HIP(hipMemcpy(e->layer_np.data(), e->d_layer_np.p, sizeof(int32_t) * e->layer_np.size(),
+                         hipMemcpyDeviceToHost));
+    }
     if (kept) *kept = k;
     return RG_SUCCESS;
 }
@@ -907,7 +931,8 @@ extern "C" int rg_encoder_write_j2k(rg_encoder *e, uint8_t *out, long cap, long 
                    e->ncomp == 3 && e->colour != RG_COLOUR_NONE, 2, e->bitdepth, e->delta, e->style};
     std::vector<uint8_t> cs;
     if ((rc = rg_j2k_assemble(im, e->h_desc.data(), res.data(), ptr.data(), seglen.empty() ? nullptr : seglen.data(),
-                              RG_MAX_PASSES, e->ncb_comp, cs)))
+                              RG_MAX_PASSES, e->ncb_comp, cs, e->nlayers,
+                              e->nlayers > 1 ? e->layer_np.data() : nullptr)))
         return rc;
     if (size) *size = (long)cs.size();
     if (out) {

@@ -183,8 +183,8 @@ size_t rg_rd_ws_bytes(int ncb);
 size_t rg_rd_sppm_bytes(int ncb);
 int rg_launch_rate_control(hipStream_t s, const int32_t *q, int pitch, size_t plane_stride, const rg_cb_desc *desc,
                            int ncb, const int4 *cm, const uint64_t *sppm, const int32_t *seglens,
-                           const double *weights, long target, void *ws, rg_cb_result *res, long *kept_out,
-                           int *err_out);
+                           const double *weights, int nlayers, const long *targets, void *ws, int32_t *layer_np,
+                           rg_cb_result *res, long *kept_out, int *err_out);
 int rg_launch_t1_jobs_parallel(hipStream_t s, const void *jobs, int njobs, int style, int32_t *chain_ws);
 // Persistent batched tier-1 (any style): nwaves workgroups pull job indices
 // from *counter (reset by a kernel inside the launch sequence).

@@ -15,6 +15,9 @@ struct rg_t2_image {
 // components, component-major; seglen (TERMALL style): per block, the length
 // of each pass's codeword segment (maxp entries per block).  Appends the
 // codestream to out.
+// layer_np (optional, TERMALL styles): per block, the passes it has after
+// each of nlayers quality layers (non-decreasing); null: one layer.
 int rg_j2k_assemble(const rg_t2_image &im, const rg_cb_desc *desc, const rg_cb_result *res, const uint8_t *const *data,
-                    const int32_t *seglen, int maxp, int ncb_per_comp, std::vector<uint8_t> &out);
+                    const int32_t *seglen, int maxp, int ncb_per_comp, std::vector<uint8_t> &out, int nlayers = 1,
+                    const int32_t *layer_np = nullptr);
 void rg_band_expmant(int Rb, double delta, int *eps, int *mu);

@@ -84,13 +84,15 @@ RG_DEV int64_t wave_sum_i64(int64_t v)
     return v;
 }
 
-__global__ void rd_init_kernel(RdState *st, unsigned long long *hist, unsigned long long target)
+// Selection state for a new target (and, first, the error flag).
+__global__ void rd_init_kernel(RdState *st, unsigned long long *hist, unsigned long long target, int first)
 {
     const int t = threadIdx.x;
     hist[t] = 0;
     if (t == 0) {
         st->prefix = 0; st->above = 0; st->target = target; st->K = 0; st->kept = 0;
-        st->done = 0; st->err = 0;
+        st->done = 0;
+        if (first) st->err = 0;
     }
 }
 
@@ -287,7 +289,10 @@ __global__ void __launch_bounds__(256) rd_pick_kernel(int round, RdWs w)
     }
 }
 
-__global__ void rd_apply_kernel(int ncb, RdWs w, rg_cb_result *__restrict__ res)
+// Layer `layer` of nlayers: the block's pass count into layer_np (if
+// given); the last layer also cuts the block's result.
+__global__ void rd_apply_kernel(int ncb, RdWs w, rg_cb_result *__restrict__ res, int32_t *__restrict__ layer_np,
+                                int nlayers, int layer)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= ncb) return;
@@ -298,8 +303,11 @@ __global__ void rd_apply_kernel(int ncb, RdWs w, rg_cb_result *__restrict__ res)
     while (keep < c && w.hk[keep * S + b] >= K) ++keep;
     const int nt = keep ? w.hn[(keep - 1) * S + b] : 0;
     const int32_t rt = keep ? w.hr[(keep - 1) * S + b] : 0;
-    res[b].npasses = nt;
-    res[b].len = rt;
+    if (layer_np) layer_np[(size_t)b * nlayers + layer] = nt;
+    if (layer == nlayers - 1) {
+        res[b].npasses = nt;
+        res[b].len = rt;
+    }
     if (rt) atomicAdd(&w.st->kept, (unsigned long long)rt);
 }
 
@@ -315,27 +323,36 @@ size_t rg_rd_sppm_bytes(int ncb) { return (size_t)ncb * RD_PLANES * 64 * 8; }
 
 int rg_launch_rate_control(hipStream_t s, const int32_t *q, int pitch, size_t plane_stride, const rg_cb_desc *desc,
                            int ncb, const int4 *cm, const uint64_t *sppm, const int32_t *seglens,
-                           const double *weights, long target, void *ws, rg_cb_result *res, long *kept_out,
-                           int *err_out)
+                           const double *weights, int nlayers, const long *targets, void *ws, int32_t *layer_np,
+                           rg_cb_result *res, long *kept_out, int *err_out)
 {
     if (ncb <= 0) return RG_SUCCESS;
-    RG_CHECK(target >= 0, "rate control: negative target");
+    RG_CHECK(nlayers >= 1 && targets, "rate control: no targets");
+    for (int l = 0; l < nlayers; ++l)
+        RG_CHECK(targets[l] >= 0 && (l == 0 || targets[l] >= targets[l - 1]),
+                 "rate control: targets must be non-negative and non-decreasing");
     const RdWs w = rd_ws(ws, ncb);
-    rd_init_kernel<<<1, 256, 0, s>>>(w.st, w.hist, (unsigned long long)target);
+    rd_init_kernel<<<1, 256, 0, s>>>(w.st, w.hist, (unsigned long long)targets[0], 1);
     RG_LAUNCH_CHECK();
     rd_dist_kernel<<<dim3((unsigned)ncb), dim3(64), 0, s>>>(q, pitch, plane_stride, desc, cm, sppm, w.dist, w.st);
     RG_LAUNCH_CHECK();
     const dim3 g((unsigned)((ncb + 255) / 256)), b(256);
     rd_hull_kernel<<<g, b, 0, s>>>(ncb, cm, seglens, weights, w);
     RG_LAUNCH_CHECK();
-    for (int round = 0; round < 8; ++round) {
-        rd_hist_kernel<<<g, b, 0, s>>>(ncb, round, w);
-        RG_LAUNCH_CHECK();
-        rd_pick_kernel<<<1, 256, 0, s>>>(round, w);
+    for (int l = 0; l < nlayers; ++l) {
+        if (l) {
+            rd_init_kernel<<<1, 256, 0, s>>>(w.st, w.hist, (unsigned long long)targets[l], 0);
+            RG_LAUNCH_CHECK();
+        }
+        for (int round = 0; round < 8; ++round) {
+            rd_hist_kernel<<<g, b, 0, s>>>(ncb, round, w);
+            RG_LAUNCH_CHECK();
+            rd_pick_kernel<<<1, 256, 0, s>>>(round, w);
+            RG_LAUNCH_CHECK();
+        }
+        rd_apply_kernel<<<g, b, 0, s>>>(ncb, w, res, layer_np, nlayers, l);
         RG_LAUNCH_CHECK();
     }
-    rd_apply_kernel<<<g, b, 0, s>>>(ncb, w, res);
-    RG_LAUNCH_CHECK();
     if (kept_out || err_out) {
         RdState h;
         RG_HIP(hipMemcpyAsync(&h, w.st, sizeof h, hipMemcpyDeviceToHost, s));

@@ -137,10 +137,14 @@ void rg_band_expmant(int Rb, double delta, int *eps, int *mu)
 }
 
 int rg_j2k_assemble(const rg_t2_image &im, const rg_cb_desc *desc, const rg_cb_result *res, const uint8_t *const *data,
-                    const int32_t *seglen, int maxp, int ncb_per_comp, std::vector<uint8_t> &out)
+                    const int32_t *seglen, int maxp, int ncb_per_comp, std::vector<uint8_t> &out, int nlayers,
+                    const int32_t *layer_np)
 {
     const bool termall = (im.style & RG_STYLE_TERMALL) != 0;
     if (termall && !seglen) return rg_fail(RG_ERR_INVALID, "TERMALL style needs the per-pass segment lengths");
+    if (!layer_np) nlayers = 1;
+    if (nlayers < 1 || nlayers > 65535) return rg_fail(RG_ERR_INVALID, "bad layer count %d", nlayers);
+    if (nlayers > 1 && !termall) return rg_fail(RG_ERR_INVALID, "quality layers need a TERMALL code-block style");
     const int nb = 3 * im.levels + 1;
     int eps[97], mu[97];
     for (int b = 0; b < nb; ++b) {
@@ -158,7 +162,7 @@ int rg_j2k_assemble(const rg_t2_image &im, const rg_cb_desc *desc, const rg_cb_r
     s.u16(im.ncomp);
     for (int c = 0; c < im.ncomp; ++c) { s.u8(im.bitdepth - 1); s.u8(1); s.u8(1); }
     s.u16(0xFF52); s.u16(12); s.u8(0);                        // COD
-    s.u8(0); s.u16(1); s.u8(im.mct);
+    s.u8(0); s.u16(nlayers); s.u8(im.mct);
     s.u8(im.levels); s.u8(ilog2(im.cbw) - 2); s.u8(ilog2(im.cbh) - 2); s.u8(im.style); s.u8(im.reversible ? 1 : 0);
     s.u16(0xFF5C);                                            // QCD
     if (im.reversible) {
@@ -192,54 +196,96 @@ int rg_j2k_assemble(const rg_t2_image &im, const rg_cb_desc *desc, const rg_cb_r
         bfirst[nb] = k;
         if (k != ncb_per_comp) return rg_fail(RG_ERR_INVALID, "code-block list out of band order");
     }
-    for (int r = 0; r <= im.levels; ++r) {
-        const int b0 = r == 0 ? 0 : 1 + 3 * (r - 1), b1 = r == 0 ? 1 : b0 + 3;
-        for (int c = 0; c < im.ncomp; ++c) {
-            const int cbase = c * ncb_per_comp;
-            bool any = false;
-            for (int i = bfirst[b0]; i < bfirst[b1]; ++i) any |= res[cbase + i].npasses > 0;
-            HeaderBits hb(s);
-            if (!any) { hb.bit(0); hb.finish(); continue; }
-            hb.bit(1);
-            for (int b = b0; b < b1; ++b) {
-                if (!bnx[b]) continue;
-                TagTree incl(bnx[b], bny[b]), msb(bnx[b], bny[b]);
-                const int Mb = im.guard_bits + eps[b] - 1;
-                for (int i = 0; i < bnx[b] * bny[b]; ++i) {
-                    const rg_cb_result &q = res[cbase + bfirst[b] + i];
-                    if (q.nbp > Mb) return rg_fail(RG_ERR_INVALID, "block needs %d bit-planes, band allows %d", q.nbp, Mb);
-                    incl.set(i % bnx[b], i / bnx[b], q.npasses > 0 ? 0 : 1);
-                    msb.set(i % bnx[b], i / bnx[b], Mb - q.nbp);
-                }
-                for (int i = 0; i < bnx[b] * bny[b]; ++i) {
-                    const rg_cb_result &q = res[cbase + bfirst[b] + i];
-                    const int x = i % bnx[b], y = i / bnx[b];
-                    incl.code(hb, x, y, 1);
-                    if (q.npasses == 0) continue;
-                    msb.code(hb, x, y, 1 << 20);
-                    pass_count(hb, q.npasses);
-                    // B.10.7: one Lblock increment, then each codeword segment's
-                    // length in Lblock + log2(passes in the segment) bits (one
-                    // segment per pass with TERMALL, else one for all passes)
-                    const int32_t *sl = termall ? seglen + (size_t)(cbase + bfirst[b] + i) * maxp : nullptr;
-                    const int nseg = termall ? q.npasses : 1, lp = termall ? 0 : ilog2(q.npasses);
-                    int inc = 0;
-                    for (int g = 0; g < nseg; ++g) {
-                        const int need = ilog2(termall ? sl[g] : q.len) + 1 - (3 + lp);
-                        inc = need > inc ? need : inc;
-                    }
-                    for (int t = 0; t < inc; ++t) hb.bit(1);
-                    hb.bit(0);
-                    for (int g = 0; g < nseg; ++g) hb.bits((uint32_t)(termall ? sl[g] : q.len), 3 + inc + lp);
-                }
-            }
-            hb.finish();
-            for (int i = bfirst[b0]; i < bfirst[b1]; ++i) {
-                const rg_cb_result &q = res[cbase + i];
-                if (q.npasses) out.insert(out.end(), data[cbase + i], data[cbase + i] + q.len);
+    // Packets in LRCP order.  Per block across the layers: passes sent so
+    // far and Lblock; per (component, band): the inclusion tag tree (value:
+    // the first layer with a pass, nlayers if none) and the missing-MSB tree.
+    const int ncb_all = ncb_per_comp * im.ncomp;
+    auto np_at = [&](int blk, int l) { return layer_np ? layer_np[(size_t)blk * nlayers + l] : res[blk].npasses; };
+    std::vector<int> sent(ncb_all, 0), lblock(ncb_all, 3);
+    std::vector<TagTree> incl, msb;
+    incl.reserve((size_t)im.ncomp * nb);
+    msb.reserve((size_t)im.ncomp * nb);
+    for (int c = 0; c < im.ncomp; ++c)
+        for (int b = 0; b < nb; ++b) {
+            incl.emplace_back(bnx[b] ? bnx[b] : 1, bny[b] ? bny[b] : 1);
+            msb.emplace_back(bnx[b] ? bnx[b] : 1, bny[b] ? bny[b] : 1);
+            const int Mb = im.guard_bits + eps[b] - 1;
+            for (int i = 0; i < bnx[b] * bny[b]; ++i) {
+                const int blk = c * ncb_per_comp + bfirst[b] + i;
+                const rg_cb_result &q = res[blk];
+                if (q.nbp > Mb) return rg_fail(RG_ERR_INVALID, "block needs %d bit-planes, band allows %d", q.nbp, Mb);
+                int first = nlayers;
+                for (int l = nlayers - 1; l >= 0; --l)
+                    if (np_at(blk, l) > 0) first = l;
+                for (int l = 1; l < nlayers; ++l)
+                    if (np_at(blk, l) < np_at(blk, l - 1)) return rg_fail(RG_ERR_INVALID, "layer passes decrease");
+                incl.back().set(i % bnx[b], i / bnx[b], first);
+                msb.back().set(i % bnx[b], i / bnx[b], Mb - q.nbp);
             }
         }
-    }
+    for (int l = 0; l < nlayers; ++l)
+        for (int r = 0; r <= im.levels; ++r) {
+            const int b0 = r == 0 ? 0 : 1 + 3 * (r - 1), b1 = r == 0 ? 1 : b0 + 3;
+            for (int c = 0; c < im.ncomp; ++c) {
+                const int cbase = c * ncb_per_comp;
+                bool any = false;
+                for (int i = bfirst[b0]; i < bfirst[b1]; ++i) any |= np_at(cbase + i, l) > sent[cbase + i];
+                HeaderBits hb(s);
+                if (!any) { hb.bit(0); hb.finish(); continue; }
+                hb.bit(1);
+                for (int b = b0; b < b1; ++b) {
+                    if (!bnx[b]) continue;
+                    TagTree &ti = incl[(size_t)c * nb + b], &tm = msb[(size_t)c * nb + b];
+                    for (int i = 0; i < bnx[b] * bny[b]; ++i) {
+                        const int blk = cbase + bfirst[b] + i;
+                        const rg_cb_result &q = res[blk];
+                        const int x = i % bnx[b], y = i / bnx[b];
+                        const int prev = sent[blk], nnew = np_at(blk, l) - prev;
+                        if (prev == 0) {
+                            ti.code(hb, x, y, l + 1);
+                            if (nnew <= 0) continue;
+                            tm.code(hb, x, y, 1 << 20);
+                        } else {
+                            hb.bit(nnew > 0 ? 1 : 0);
+                            if (nnew <= 0) continue;
+                        }
+                        pass_count(hb, nnew);
+                        // B.10.7: one Lblock increment, then each codeword segment's
+                        // length in Lblock + log2(passes in the segment) bits (one
+                        // segment per new pass with TERMALL, else one for all passes)
+                        const int32_t *sl = termall ? seglen + (size_t)blk * maxp + prev : nullptr;
+                        const int nseg = termall ? nnew : 1, lp = termall ? 0 : ilog2(nnew);
+                        int inc = 0;
+                        for (int g = 0; g < nseg; ++g) {
+                            const int need = ilog2(termall ? sl[g] : q.len) + 1 - (lblock[blk] + lp);
+                            inc = need > inc ? need : inc;
+                        }
+                        for (int t = 0; t < inc; ++t) hb.bit(1);
+                        hb.bit(0);
+                        lblock[blk] += inc;
+                        for (int g = 0; g < nseg; ++g) hb.bits((uint32_t)(termall ? sl[g] : q.len), lblock[blk] + lp);
+                    }
+                }
+                hb.finish();
+                // body: each contributing block's new passes, in block order
+                for (int i = bfirst[b0]; i < bfirst[b1]; ++i) {
+                    const int blk = cbase + i;
+                    const int prev = sent[blk], now = np_at(blk, l);
+                    if (now <= prev) continue;
+                    long from = 0, to = res[blk].len;
+                    if (termall) {
+                        const int32_t *sl = seglen + (size_t)blk * maxp;
+                        to = 0;
+                        for (int t = 0; t < now; ++t) {
+                            if (t == prev) from = to;
+                            to += sl[t];
+                        }
+                    }
+                    out.insert(out.end(), data[blk] + from, data[blk] + to);
+                    sent[blk] = now;
+                }
+            }
+        }
     const uint32_t psot = (uint32_t)(out.size() - sot);
     out[sot + 6] = (uint8_t)(psot >> 24); out[sot + 7] = (uint8_t)(psot >> 16);
     out[sot + 8] = (uint8_t)(psot >> 8); out[sot + 9] = (uint8_t)psot;

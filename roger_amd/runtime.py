@@ -456,6 +456,19 @@ class Encoder:
         check(lib().rg_encoder_rate_control(self.handle, int(target_bytes), wp, ctypes.byref(kept)))
         return kept.value
 
+    def rate_control_layers(self, targets, weights=None):
+        """Quality layers cut at the non-decreasing byte budgets `targets`;
+        returns the bytes kept by the last layer."""
+        kept = ctypes.c_long()
+        t = np.ascontiguousarray(targets, np.int64)
+        wp = None
+        if weights is not None:
+            weights = np.ascontiguousarray(weights, np.float64)
+            wp = weights.ctypes.data_as(ctypes.c_void_p)
+        check(lib().rg_encoder_rate_control_layers(self.handle, len(t), t.ctypes.data_as(ctypes.c_void_p), wp,
+                                                   ctypes.byref(kept)))
+        return kept.value
+
     def j2k(self):
         """The JPEG 2000 codestream of the last encode (bytes)."""
         size = ctypes.c_long()

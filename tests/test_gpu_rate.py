@@ -128,3 +128,52 @@ def test_rate_control_errors(ctx):
     kept = enc.rate_control(1 << 30)
     assert 0 < kept == int(enc.codeblocks()[1]["len"].sum())
     enc.close()
+
+
+@pytest.mark.parametrize("PAR", STYLES)
+def test_quality_layers_vs_oracle(ctx, PAR):
+    """Three quality layers: every block's passes per layer equal the oracle's
+    cut at each layer's target, the codestream equals the oracle's
+    multi-layer writer, and OpenJPEG stopping after layer l decodes exactly
+    the single-layer cut at layer l's target."""
+    from PIL import Image
+    from test_rate import decode_layers
+    w, h, nc, levels = 300, 257, 3, 4
+    img = synth.image(w, h, nc, 61)
+    enc = R.Encoder(ctx)
+    enc.enable_rate_control()
+    enc.encode_u8(img, levels=levels, reversible=False, cbw=64, cbh=64, delta=1.0 / 64, style=PAR)
+    desc, res, blobs = enc.codeblocks()
+    rs, data, q, passlen = O.encode_image(img, levels=levels, reversible=False, cbw=64, cbh=64, delta=1.0 / 64,
+                                          style=PAR, return_passlen=True)
+    dists = _oracle_dists(desc, q, PAR, passlen, rs)
+    weights = np.random.default_rng(9).uniform(0.2, 3.0, len(desc))
+    full = int(rs[:, 0].sum())
+    targets = [full // 30, full // 8, full // 3]
+    kept = enc.rate_control_layers(targets, weights)
+    cuts = [O.rate_control(rs[:, 1], passlen, dists, weights, t) for t in targets]
+    layers = np.stack([c[0] for c in cuts], 1)
+    nt, rt, total = cuts[-1]
+    assert kept == total
+    _, res2, blobs2 = enc.codeblocks()
+    np.testing.assert_array_equal(res2["npasses"], nt)
+    np.testing.assert_array_equal(res2["len"], rt)
+    params = O.j2k_params(w, h, nc, levels, False, 64, 64, 1.0 / 64, style=PAR)
+    r3 = rs.copy()
+    r3[:, 0], r3[:, 1] = rt, nt
+    cs = O.t2_write(params, r3, [data[i][:rt[i]] for i in range(len(data))], passlen, layers=layers)
+    assert enc.j2k() == cs
+    for l, (nl, rl, _) in enumerate(cuts):
+        r1 = rs.copy()
+        r1[:, 0], r1[:, 1] = rl, nl
+        single = O.t2_write(params, r1, [data[i][:rl[i]] for i in range(len(data))], passlen)
+        np.testing.assert_array_equal(decode_layers(cs, l + 1), np.array(Image.open(io.BytesIO(single))),
+                                      err_msg="layer %d" % l)
+    # a single cut afterwards, and a new encode, go back to one layer
+    enc.rate_control(targets[1], weights)
+    r1 = rs.copy()
+    r1[:, 0], r1[:, 1] = cuts[1][1], cuts[1][0]
+    assert enc.j2k() == O.t2_write(params, r1, [data[i][:cuts[1][1][i]] for i in range(len(data))], passlen)
+    with pytest.raises(RogerError):
+        enc.rate_control_layers([targets[1], targets[0]], weights)
+    enc.close()
