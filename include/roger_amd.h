@@ -297,6 +297,31 @@ int rg_t1_decode_style(rg_context *ctx, const uint8_t *d_data, const long *d_off
 int rg_decoder_decode_codeblocks(rg_decoder *dec, const rg_cb_desc *desc, const rg_cb_result *res,
                                  const long *offsets, const uint8_t *data, long total, int ncb, int w, int h,
                                  int ncomp, const rg_encode_params *p, const int32_t *seglen);
+/* Tier-2 reader (host only, no device work).  The reference has no tier-2
+ * (src/OCLEncoder.cpp:43-52).  Reads a .j2k codestream: one tile in one
+ * tile-part, LRCP, any number of quality layers, maximal precincts, unsigned
+ * 8-bit samples without subsampling, 1-4 components, code-block styles 0,
+ * RG_STYLE_PARALLEL, RG_STYLE_PARALLEL_BYPASS, optional SOP/EPH markers.
+ * Call with only `info` to learn ncb and data_bytes, then again with desc /
+ * res / offsets (ncb each), data (data_bytes) and, for TERMALL styles,
+ * seglen (ncb * RG_MAX_PASSES); blocks come in the encoder's order and the
+ * result feeds rg_decoder_decode_codeblocks. */
+typedef struct {
+    int32_t w, h, ncomp, levels, cbw, cbh;
+    int32_t style;        /* code-block style byte of COD */
+    int32_t filter;       /* RG_FILTER_53 / RG_FILTER_97 */
+    int32_t colour;       /* RG_COLOUR_* from the COD component transform */
+    int32_t nlayers, bitdepth, guard_bits;
+    int32_t ncb;          /* code-blocks, all components */
+    int64_t data_bytes;   /* codeword bytes, all blocks */
+    float step[97];       /* quantisation step per band (3*levels+1; 1 for 5/3) */
+} rg_j2k_info;
+int rg_j2k_parse(const uint8_t *cs, long size, rg_j2k_info *info, rg_cb_desc *desc, rg_cb_result *res,
+                 long *offsets, uint8_t *data, int32_t *seglen);
+/* Whole decode of a codestream (host bytes) into the decoder's uint8 image:
+ * rg_j2k_parse, then tier-1 decode, dequantisation with the codestream's
+ * steps, inverse DWT and colour transform on the device. */
+int rg_decoder_decode_j2k(rg_decoder *dec, const uint8_t *cs, long size, rg_j2k_info *info);
 int rg_decoder_image_u8(rg_decoder *dec, uint8_t **d_img);
 int rg_decoder_coefficients(rg_decoder *dec, int32_t **d_q);
 

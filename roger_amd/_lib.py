@@ -58,6 +58,12 @@ class EncodeParams(ctypes.Structure):
                 ("ll_fp16", ctypes.c_int32), ("bitdepth", ctypes.c_int32), ("style", ctypes.c_int32)]
 
 
+class J2KInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ("w", "h", "ncomp", "levels", "cbw", "cbh", "style", "filter", "colour",
+                                              "nlayers", "bitdepth", "guard_bits", "ncb")] + \
+               [("data_bytes", ctypes.c_int64), ("step", ctypes.c_float * 97)]
+
+
 P = ctypes.c_void_p
 I = ctypes.c_int
 SZ = ctypes.c_size_t
@@ -107,6 +113,8 @@ SIGNATURES = {
     "rg_encoder_enable_rate_control": (I, [P, I]),
     "rg_encoder_rate_control": (I, [P, L, P, P]),
     "rg_encoder_rate_control_layers": (I, [P, I, P, P, P]),
+    "rg_j2k_parse": (I, [P, L, P, P, P, P, P, P]),
+    "rg_decoder_decode_j2k": (I, [P, P, L, P]),
     "rg_encoder_write_j2k": (I, [P, P, L, P]),
     "rg_encoder_map_dwt_out": (I, [P, P]),
     "rg_encoder_unmap_dwt_out": (I, [P, P]),

@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libroger_amd.so")
 OBJ = os.path.join(HERE, "_obj")
 SOURCES = ["capi.hip", "colour.hip", "dwt.hip", "t1_encode.hip", "t1_mq.hip", "t1_chain.hip", "t1_decode.hip",
-           "t1_rate.hip", "batch.hip", "t2_write.hip"]
+           "t1_rate.hip", "batch.hip", "t2_write.hip", "t2_read.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

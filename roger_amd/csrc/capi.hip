@@ -1012,9 +1012,39 @@ extern "C" int rg_t1_decode_style(rg_context *c, const uint8_t *d_data, const lo
                                (style & RG_STYLE_TERMALL) ? d_seglen : nullptr);
 }
 
+static int decode_codeblocks(rg_decoder *d, const rg_cb_desc *desc, const rg_cb_result *res, const long *offsets,
+                             const uint8_t *data, long total, int ncb, int w, int h, int ncomp,
+                             const rg_encode_params *p, const int32_t *seglen, const float *steps);
+
 extern "C" int rg_decoder_decode_codeblocks(rg_decoder *d, const rg_cb_desc *desc, const rg_cb_result *res,
                                             const long *offsets, const uint8_t *data, long total, int ncb, int w,
                                             int h, int ncomp, const rg_encode_params *p, const int32_t *seglen)
+{
+    return decode_codeblocks(d, desc, res, offsets, data, total, ncb, w, h, ncomp, p, seglen, nullptr);
+}
+
+extern "C" int rg_decoder_decode_j2k(rg_decoder *d, const uint8_t *cs, long size, rg_j2k_info *info_out)
+{
+    RG_CHECK(d && cs, "rg_decoder_decode_j2k: null argument");
+    rg_j2k_info info;
+    std::vector<rg_cb_desc> desc;
+    std::vector<rg_cb_result> res;
+    std::vector<long> offs;
+    std::vector<uint8_t> data;
+    std::vector<int32_t> seg;
+    if (int rc = rg_j2k_read(cs, size, info, desc, res, offs, data, seg)) return rc;
+    rg_encode_params p{};
+    p.levels = info.levels; p.filter = info.filter; p.colour = info.colour; p.cbw = info.cbw; p.cbh = info.cbh;
+    p.delta = 1.0 / 64; p.ll_fp16 = 0; p.bitdepth = info.bitdepth; p.style = info.style;
+    if (info_out) *info_out = info;
+    return decode_codeblocks(d, desc.data(), res.data(), offs.data(), data.empty() ? nullptr : data.data(),
+                             (long)data.size(), info.ncb, info.w, info.h, info.ncomp, &p,
+                             seg.empty() ? nullptr : seg.data(), info.step);
+}
+
+static int decode_codeblocks(rg_decoder *d, const rg_cb_desc *desc, const rg_cb_result *res, const long *offsets,
+                             const uint8_t *data, long total, int ncb, int w, int h, int ncomp,
+                             const rg_encode_params *p, const int32_t *seglen, const float *steps)
 {
     RG_CHECK(d && desc && res && offsets && (data || total == 0), "rg_decoder_decode_codeblocks: null argument");
     int rc = check_params(p, w, h, ncomp);
@@ -1074,6 +1104,7 @@ extern "C" int rg_decoder_decode_codeblocks(rg_decoder *d, const rg_cb_desc *des
         return rc;
     float step[97], inv[97];
     band_steps(p->levels, p->bitdepth, p->delta, step, inv);
+    if (steps) memcpy(step, steps, sizeof(float) * (3 * p->levels + 1));
     if ((rc = rg_dwt_inv(d->ctx, p->filter, d->coeff.p, w, h, ncomp, p->levels,
                          p->filter == RG_FILTER_97 ? step : nullptr, d->out.p, d->scratch.p)))
         return rc;

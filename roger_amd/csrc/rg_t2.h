@@ -21,3 +21,10 @@ int rg_j2k_assemble(const rg_t2_image &im, const rg_cb_desc *desc, const rg_cb_r
                     const int32_t *seglen, int maxp, int ncb_per_comp, std::vector<uint8_t> &out, int nlayers = 1,
                     const int32_t *layer_np = nullptr);
 void rg_band_expmant(int Rb, double delta, int *eps, int *mu);
+
+// Tier-2 reader: the codestream's parameters (info) and its code-blocks in the
+// encoder's order (desc / res / offsets into data; seglen for TERMALL styles,
+// RG_MAX_PASSES per block).
+int rg_j2k_read(const uint8_t *cs, long size, rg_j2k_info &info, std::vector<rg_cb_desc> &desc,
+                std::vector<rg_cb_result> &res, std::vector<long> &offsets, std::vector<uint8_t> &data,
+                std::vector<int32_t> &seglen);

@@ -360,6 +360,16 @@ class Decoder:
             ncomp, ctypes.byref(p), None if sl is None else sl.ctypes.data_as(ctypes.c_void_p)))
         self.shape = (h, w) if ncomp == 1 else (h, w, ncomp)
 
+    def decode_j2k(self, cs):
+        """Decodes a .j2k codestream (bytes); returns its parameters."""
+        from ._lib import J2KInfo
+        buf = np.frombuffer(bytes(cs), np.uint8)
+        info = J2KInfo()
+        check(lib().rg_decoder_decode_j2k(self.handle, buf.ctypes.data_as(ctypes.c_void_p), buf.size,
+                                          ctypes.byref(info)))
+        self.shape = (info.h, info.w) if info.ncomp == 1 else (info.h, info.w, info.ncomp)
+        return _info_dict(info)
+
     def image(self):
         dp = ctypes.c_void_p()
         check(lib().rg_decoder_image_u8(self.handle, ctypes.byref(dp)))
@@ -488,3 +498,31 @@ class Encoder:
             self.close()
         except Exception:
             pass
+
+
+def _info_dict(info):
+    d = {n: getattr(info, n) for n, _ in info._fields_ if n != "step"}
+    d["step"] = list(info.step[:3 * info.levels + 1])
+    return d
+
+
+def j2k_parse(cs):
+    """Tier-2 read of a .j2k codestream (host only): (info, descs, results,
+    blobs, seglen or None), blocks in the encoder's order."""
+    from ._lib import J2KInfo
+    buf = np.frombuffer(bytes(cs), np.uint8)
+    info = J2KInfo()
+    vp = buf.ctypes.data_as(ctypes.c_void_p)
+    check(lib().rg_j2k_parse(vp, buf.size, ctypes.byref(info), None, None, None, None, None))
+    n = info.ncb
+    desc = np.zeros(n, CB_DESC_DTYPE)
+    res = np.zeros(n, CB_RES_DTYPE)
+    offs = np.zeros(n, np.int64)
+    data = np.zeros(max(info.data_bytes, 1), np.uint8)
+    seg = np.zeros((n, RG_MAX_PASSES), np.int32) if info.style & 4 else None
+    check(lib().rg_j2k_parse(vp, buf.size, ctypes.byref(info), desc.ctypes.data_as(ctypes.c_void_p),
+                             res.ctypes.data_as(ctypes.c_void_p), offs.ctypes.data_as(ctypes.c_void_p),
+                             data.ctypes.data_as(ctypes.c_void_p),
+                             None if seg is None else seg.ctypes.data_as(ctypes.c_void_p)))
+    blobs = [bytes(data[offs[i]:offs[i] + res["len"][i]]) for i in range(n)]
+    return _info_dict(info), desc, res, blobs, seg
