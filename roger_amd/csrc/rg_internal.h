@@ -139,6 +139,7 @@ size_t rg_t1_chain_ws_bytes(int nblocks);
 // bytes per block of modelling state kept between the phases.
 #define RG_T1_PHASE_SPLIT1 7
 #define RG_T1_PHASE_SPLIT 10
+void rg_t1_splits(int *s1, int *s2);
 // number of modelling phases: 1, 2 or 3 (RG_T1_PHASES; RG_T1_TWO_PHASE=0 also
 // selects 1)
 int rg_t1_phases();

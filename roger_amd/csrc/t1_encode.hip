@@ -12,6 +12,7 @@
 // The lane coders live in t1_mq.hip (one MQ chain per code-block) and
 // t1_chain.hip (the sorted pass chains of the parallel styles); the three
 // files compile in parallel.
+#include <cstdio>
 #include <cstdlib>
 #include "rg_internal.h"
 #include "wave_hip.h"
@@ -164,7 +165,9 @@ int rg_launch_t1_parallel(hipStream_t s, const int32_t *q, int pitch, size_t pla
         // packing joins them all.  The lists are built on the first stream
         // before the next modelling launch: built beside it, their 256-thread
         // workgroups waited for SIMD slots until its whole grid was dispatched.
-        const int bounds[3] = {0, ov->nearly == 2 ? RG_T1_PHASE_SPLIT1 : RG_T1_PHASE_SPLIT, RG_T1_PHASE_SPLIT};
+        int sp1, sp2;
+        rg_t1_splits(&sp1, &sp2);
+        const int bounds[3] = {0, ov->nearly == 2 ? sp1 : sp2, sp2};
         for (int k = 0; k < ov->nearly; ++k) {
             const int qlo = bounds[k], qhi = bounds[k + 1];
             t1_cm_kernel<<<g, b, 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm, poffs, style, qlo, qhi,
@@ -177,9 +180,9 @@ int rg_launch_t1_parallel(hipStream_t s, const int32_t *q, int pitch, size_t pla
             RG_HIP(hipEventRecord(ov->ev_out[k], ov->se[k]));
         }
         t1_cm_kernel<<<g, b, 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm, poffs, style,
-                                     RG_T1_PHASE_SPLIT, 1 << 30, (uint4 *)ov->cms, sppm);
+                                     sp2, 1 << 30, (uint4 *)ov->cms, sppm);
         RG_LAUNCH_CHECK();
-        if (int rc = rg_launch_chains_flat(s, fj, ncb, style, chain_ws, RG_T1_PHASE_SPLIT, MAXP)) return rc;
+        if (int rc = rg_launch_chains_flat(s, fj, ncb, style, chain_ws, sp2, MAXP)) return rc;
         for (int k = 0; k < ov->nearly; ++k) RG_HIP(hipStreamWaitEvent(s, ov->ev_out[k], 0));
         t1_pack_kernel<<<g, b, 0, s>>>(cm, poffs, arena, astride, seglens, out, slot_bytes, res);
         RG_LAUNCH_CHECK();
@@ -287,6 +290,27 @@ int rg_t1_phases()
         if (two && two[0] == '0') v = 1;
     }
     return v;
+}
+
+// Phase boundaries (coding-pass indices, each 1 mod 3 so that a phase ends
+// before a significance-propagation pass): RG_T1_PHASE_SPLIT1 and
+// RG_T1_PHASE_SPLIT unless RG_T1_SPLITS="a,b" names others (4 <= a < b <= 94).
+void rg_t1_splits(int *s1, int *s2)
+{
+    static int a = -1, b = -1;
+    if (a < 0) {
+        a = RG_T1_PHASE_SPLIT1;
+        b = RG_T1_PHASE_SPLIT;
+        if (const char *e = getenv("RG_T1_SPLITS")) {
+            int x = 0, y = 0;
+            if (sscanf(e, "%d,%d", &x, &y) == 2 && x >= 4 && x < y && y <= 94 && x % 3 == 1 && y % 3 == 1) {
+                a = x;
+                b = y;
+            }
+        }
+    }
+    *s1 = a;
+    *s2 = b;
 }
 
 // Tier-1 variant: the split context-model + lane-MQ path unless RG_T1_INLINE=1
