@@ -148,6 +148,7 @@ int rg_j2k_read(const uint8_t *cs, long size, rg_j2k_info &info, std::vector<rg_
             const int nc = b.u16();
             RG_CHECK(xo == 0 && yo == 0 && txo == 0 && tyo == 0, "image or tile offsets are not supported");
             RG_CHECK(txs >= xs && tys >= ys, "more than one tile is not supported");
+            RG_CHECK(xs >= 1 && ys >= 1 && xs <= 65536 && ys <= 65536, "image size %ux%u out of range", xs, ys);
             RG_CHECK(nc >= 1 && nc <= 4, "%d components (1-4 supported)", nc);
             for (int c = 0; c < nc; ++c) {
                 const int ss = b.u8(), dx = b.u8(), dy = b.u8();
@@ -162,8 +163,10 @@ int rg_j2k_read(const uint8_t *cs, long size, rg_j2k_info &info, std::vector<rg_
             info.nlayers = b.u16();
             mct = b.u8();
             info.levels = b.u8();
-            info.cbw = 1 << (b.u8() + 2);
-            info.cbh = 1 << (b.u8() + 2);
+            const int xcb = b.u8(), ycb = b.u8();
+            RG_CHECK(xcb <= 4 && ycb <= 4, "code-block size exponents %d, %d (at most 64x64)", xcb + 2, ycb + 2);
+            info.cbw = 1 << (xcb + 2);
+            info.cbh = 1 << (ycb + 2);
             info.style = b.u8();
             xform = b.u8();
             cod = true;

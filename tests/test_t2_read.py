@@ -123,3 +123,29 @@ def test_reader_refuses_unsupported():
         R.j2k_parse(buf.getvalue())
     with pytest.raises(RogerError):
         R.j2k_parse(b"\xff\x4f\xff\x51\x00")
+
+
+def test_reader_survives_damaged_codestreams():
+    """Truncations and byte flips of valid codestreams: the reader either
+    reads them or refuses them with an error, never reads out of bounds."""
+    img = synth.image(150, 120, 3, 9)
+    streams = []
+    for style in STYLES:
+        rs, data, q, passlen = O.encode_image(img, levels=3, reversible=False, style=style, return_passlen=True)
+        streams.append(O.t2_write(O.j2k_params(150, 120, 3, 3, False, style=style), rs, data,
+                                  passlen if style else None))
+    rng = np.random.default_rng(3)
+    outcomes = {"ok": 0, "refused": 0}
+    for k in range(300):
+        cs = bytearray(streams[k % len(streams)])
+        if k % 3 == 0:
+            cs = cs[:int(rng.integers(2, len(cs)))]
+        else:
+            for _ in range(int(rng.integers(1, 6))):
+                cs[int(rng.integers(0, len(cs)))] = int(rng.integers(0, 256))
+        try:
+            R.j2k_parse(bytes(cs))
+            outcomes["ok"] += 1
+        except RogerError:
+            outcomes["refused"] += 1
+    assert outcomes["ok"] + outcomes["refused"] == 300 and outcomes["refused"] > 0
