@@ -438,7 +438,7 @@ struct rg_encoder {
     double delta = 1.0 / 64;
     float step[97], inv[97];
     DevBuf input, planes, q, fcoeff, scratch, desc, res, slots, chan16, syms, cm, poffs, arena, seglens, chains,
-        chains_e[2], cms, sppm, rdws, rdw;
+        chains_e[3], cms, sppm, rdws, rdw;
     int rd = 0;   // rate control enabled: the context modeller records significance-pass masks
     int nlayers = 1;               // quality layers of the last rate control (1 after a new encode)
     std::vector<int32_t> layer_np; // per block, passes after each layer (nlayers > 1)
@@ -446,7 +446,7 @@ struct rg_encoder {
     HostBuf staging, map;
     // events of the overlapped phased tier-1 (created on first use; its side
     // streams belong to the context)
-    hipEvent_t ev_in[2] = {nullptr, nullptr}, ev_out[2] = {nullptr, nullptr};
+    hipEvent_t ev_in[3] = {nullptr, nullptr, nullptr}, ev_out[3] = {nullptr, nullptr, nullptr};
     std::vector<rg_cb_desc> h_desc;
     int desc_key[6] = {0, 0, 0, 0, 0, 0};
 };
@@ -468,10 +468,10 @@ extern "C" int rg_encoder_destroy(rg_encoder *e)
     for (hipStream_t st : e->ctx->side)
         if (st) hipStreamSynchronize(st);
     for (DevBuf *b : {&e->input, &e->planes, &e->q, &e->fcoeff, &e->scratch, &e->desc, &e->res, &e->slots, &e->chan16,
-                      &e->syms, &e->cm, &e->poffs, &e->arena, &e->seglens, &e->chains, &e->chains_e[0], &e->chains_e[1],
+                      &e->syms, &e->cm, &e->poffs, &e->arena, &e->seglens, &e->chains, &e->chains_e[0], &e->chains_e[1], &e->chains_e[2],
                       &e->cms, &e->sppm, &e->rdws, &e->rdw, &e->d_layer_np})
         b->release();
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < 3; ++k) {
         if (e->ev_in[k]) hipEventDestroy(e->ev_in[k]);
         if (e->ev_out[k]) hipEventDestroy(e->ev_out[k]);
     }

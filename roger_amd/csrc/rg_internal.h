@@ -11,7 +11,7 @@ struct rg_context {
     // side streams of the overlapped phased tier-1, shared by the context's
     // encoders (created on first use): streams per encoder would outnumber
     // the hardware queues and serialise the overlap
-    hipStream_t side[2] = {nullptr, nullptr};
+    hipStream_t side[3] = {nullptr, nullptr, nullptr};
 };
 
 // Thread-local last-error message; set by rg_fail().
@@ -137,17 +137,19 @@ size_t rg_t1_chain_ws_bytes(int nblocks);
 // s and coded on their own stream (se[k], with a chain workspace ws[k] and the
 // events ev_in[k] / ev_out[k]) while s models the next passes.  cms: 64 x 16
 // bytes per block of modelling state kept between the phases.
+#define RG_T1_PHASE_SPLIT0 4   // four-phase form only
 #define RG_T1_PHASE_SPLIT1 7
 #define RG_T1_PHASE_SPLIT 10
 void rg_t1_splits(int *s1, int *s2);
 // number of modelling phases: 1, 2 or 3 (RG_T1_PHASES; RG_T1_TWO_PHASE=0 also
 // selects 1)
 int rg_t1_phases();
+constexpr int RG_T1_MAX_EARLY = 3;   // early phases (each with a side stream)
 struct rg_t1_overlap {
-    int nearly;              // early phases: 1 or 2
-    hipStream_t se[2];
-    hipEvent_t ev_in[2], ev_out[2];
-    int32_t *ws[2];
+    int nearly;              // early phases: 1 .. RG_T1_MAX_EARLY
+    hipStream_t se[RG_T1_MAX_EARLY];
+    hipEvent_t ev_in[RG_T1_MAX_EARLY], ev_out[RG_T1_MAX_EARLY];
+    int32_t *ws[RG_T1_MAX_EARLY];
     void *cms;
 };
 // The encoder's flat per-block tier-1 arrays, for the chain kernels

@@ -167,7 +167,13 @@ int rg_launch_t1_parallel(hipStream_t s, const int32_t *q, int pitch, size_t pla
         // workgroups waited for SIMD slots until its whole grid was dispatched.
         int sp1, sp2;
         rg_t1_splits(&sp1, &sp2);
-        const int bounds[3] = {0, ov->nearly == 2 ? sp1 : sp2, sp2};
+        int bounds[RG_T1_MAX_EARLY + 1] = {0, sp2, sp2, sp2};
+        if (ov->nearly == 2) bounds[1] = sp1;
+        if (ov->nearly == 3) {
+            // [0, 4), [4, sp1), [sp1, sp2); with sp1 <= 4 the third early phase is empty
+            bounds[1] = sp1 > RG_T1_PHASE_SPLIT0 ? RG_T1_PHASE_SPLIT0 : sp1;
+            bounds[2] = sp1 > RG_T1_PHASE_SPLIT0 ? sp1 : sp2;
+        }
         for (int k = 0; k < ov->nearly; ++k) {
             const int qlo = bounds[k], qhi = bounds[k + 1];
             t1_cm_kernel<<<g, b, 0, s>>>(q, pitch, plane_stride, desc, syms, symcap, cm, poffs, style, qlo, qhi,
@@ -286,7 +292,7 @@ int rg_t1_phases()
     if (v < 0) {
         const char *two = getenv("RG_T1_TWO_PHASE"), *n = getenv("RG_T1_PHASES");
         v = 3;
-        if (n && (n[0] == '1' || n[0] == '2' || n[0] == '3')) v = n[0] - '0';
+        if (n && n[0] >= '1' && n[0] <= '4') v = n[0] - '0';
         if (two && two[0] == '0') v = 1;
     }
     return v;
