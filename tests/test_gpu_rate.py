@@ -177,3 +177,27 @@ def test_quality_layers_vs_oracle(ctx, PAR):
     with pytest.raises(RogerError):
         enc.rate_control_layers([targets[1], targets[0]], weights)
     enc.close()
+
+
+@pytest.mark.parametrize("shape", [(64, 64, 1), (33, 1, 1), (130, 70, 3)])
+def test_rate_control_flat_and_tiny_images(ctx, shape):
+    """Blocks with no bit-planes (a flat image has only its LL block coded),
+    single-row images: the cut matches the oracle for every target."""
+    w, h, nc = shape
+    img = np.full((h, w, nc) if nc > 1 else (h, w), 77, np.uint8)
+    img.reshape(-1)[::97] = 200
+    enc = R.Encoder(ctx)
+    enc.enable_rate_control()
+    enc.encode_u8(img, levels=3 if h > 1 else 1, reversible=True, cbw=64, cbh=64, style=RG_STYLE_PARALLEL)
+    desc, res, blobs = enc.codeblocks()
+    rs, data, q, passlen = O.encode_image(img, levels=3 if h > 1 else 1, reversible=True, cbw=64, cbh=64,
+                                          style=RG_STYLE_PARALLEL, return_passlen=True)
+    dists = _oracle_dists(desc, q, RG_STYLE_PARALLEL, passlen, rs)
+    weights = np.ones(len(desc))
+    full = int(rs[:, 0].sum())
+    for target in (0, 1, full // 3, full):
+        kept = enc.rate_control(target, weights)
+        nt, rt, total = O.rate_control(rs[:, 1], passlen, dists, weights, target)
+        assert kept == total <= target
+        np.testing.assert_array_equal(enc.codeblocks()[1]["npasses"], nt)
+    enc.close()
