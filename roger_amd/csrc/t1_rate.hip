@@ -196,23 +196,30 @@ __global__ void rd_hull_kernel(int ncb, const int4 *__restrict__ cm, const int32
     int m = 0;
     long r = 0;
     int64_t d = 0;
+    // the hull end (rate, distortion, key) stays in registers; memory is
+    // read back only when a point is dropped
+    long re = 0;
+    int64_t de = 0;
+    unsigned long long ke = 0;
     for (int n = 1; n <= npass; ++n) {
         r += pl[n - 1];
         d += dd[n - 1];
         for (;;) {
-            const long r0 = m ? (long)w.hr[(m - 1) * S + b] : 0;
-            const int64_t d0 = m ? w.hd[(m - 1) * S + b] : 0;
-            if (d <= d0) break;                                   // no gain over the hull end
-            const double s = r > r0 ? (wb * (double)(d - d0)) / (double)(r - r0) : __builtin_huge_val();
+            if (d <= de) break;                                   // no gain over the hull end
+            const double s = r > re ? (wb * (double)(d - de)) / (double)(r - re) : __builtin_huge_val();
             const unsigned long long k = key_of(s);
-            if (m && k >= w.hk[(m - 1) * S + b]) {                 // the hull end lies under the chord
+            if (m && k >= ke) {                                   // the hull end lies under the chord
                 --m;
+                re = m ? (long)w.hr[(m - 1) * S + b] : 0;
+                de = m ? w.hd[(m - 1) * S + b] : 0;
+                ke = m ? w.hk[(m - 1) * S + b] : 0;
                 continue;
             }
             w.hn[m * S + b] = n;
             w.hr[m * S + b] = (int32_t)r;
             w.hd[m * S + b] = d;
             w.hk[m * S + b] = k;
+            re = r; de = d; ke = k;
             ++m;
             break;
         }
