@@ -78,7 +78,7 @@ __global__ void __launch_bounds__(64) t1_encode_queue_kernel(const T1Job *__rest
     t1_queue_loop<false>(jobs, njobs, counter);
 }
 
-__global__ void __launch_bounds__(64) t1_cm_queue_kernel(const T1Job *__restrict__ jobs, int njobs,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) t1_cm_queue_kernel(const T1Job *__restrict__ jobs, int njobs,
                                                          int *__restrict__ counter)
 {
     t1_queue_loop<true>(jobs, njobs, counter);
@@ -96,7 +96,8 @@ __global__ void reset_counter_kernel(int *counter) { *counter = 0; }
 // Four waves per SIMD: the register budget drops from 146 to 128 VGPRs at
 // the price of 60 bytes per lane of spills, and the context modelling (issue
 // bound, one wave per block) gains more from the extra wave than the spills
-// cost: the 4096x4096 RGB encode went from 2.83 to 2.66 ms.
+// cost: the 4096x4096 RGB encode went from 2.83 to 2.66 ms.  The batched
+// forms below take the same budget (256 mixed images: 43.8 -> 39.0 ms).
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) t1_cm_kernel(const int32_t *__restrict__ q, int pitch, size_t plane_stride,
                                                    const rg_cb_desc *__restrict__ desc, uint8_t *__restrict__ syms,
                                                    int symcap, int4 *__restrict__ cm, int32_t *__restrict__ poffs,
@@ -121,7 +122,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
     t1_block<true>(job, lut, ph);
 }
 
-__global__ void __launch_bounds__(64) t1_cm_jobs_kernel(const T1Job *__restrict__ jobs)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) t1_cm_jobs_kernel(const T1Job *__restrict__ jobs)
 {
     __shared__ uint8_t lut[LUT_BYTES];
     t1_block<true>(jobs[blockIdx.x], lut);
