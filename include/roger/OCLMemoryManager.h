@@ -50,6 +50,15 @@ public:
         return rg_encoder_channel_i16(enc, (int)channel, &p) == RG_SUCCESS ? (tDeviceMem)p : nullptr;
     }
 
+    // Input of DWT level `level` (reference src/OCLMemoryManager.h:55-59):
+    // level 0 the transform's input planes, level k the LL band after k
+    // levels; nullptr for levels the transform does not have.
+    tDeviceMem getDwtIn(size_t level)
+    {
+        void *p = nullptr;
+        return rg_encoder_dwt_in(enc, (int)level, &p, nullptr, nullptr) == RG_SUCCESS ? (tDeviceMem)p : nullptr;
+    }
+
     void init(std::vector<T *> components, size_t w, size_t h, size_t levels, size_t precision)
     {
         width = w; height = h; _levels = levels; _precision = precision; numComponents = components.size();
@@ -68,8 +77,28 @@ public:
         if (rc != RG_SUCCESS) LogError("mapImage: %s\n", rg_last_error());
         return rc;
     }
+    // Blocking read of a w x h int16 device buffer (e.g. a getDWTOutByChannel
+    // plane) into host memory (reference src/OCLMemoryManager.cpp:195-214,
+    // which maps width*height*sizeof(short) bytes); valid until unmapMemory.
+    tDeviceRC mapBuffer(tDeviceMem buffer, void **mappedPtr)
+    {
+        if (!mappedPtr || !buffer) return RG_ERR_INVALID;
+        bufMap.resize(width * height);
+        int rc = rg_context_sync(ocl->context);
+        if (rc == RG_SUCCESS)
+            rc = rg_memcpy_d2h(ocl->context, bufMap.data(), (const void *)buffer, bufMap.size() * sizeof(short));
+        if (rc != RG_SUCCESS) {
+            LogError("mapBuffer: %s\n", rg_last_error());
+            *mappedPtr = nullptr;
+            return rc;
+        }
+        *mappedPtr = bufMap.data();
+        return DeviceSuccess;
+    }
     tDeviceRC unmapMemory(tDeviceMem, void *mappedPtr)
     {
+        if (!mappedPtr) return RG_ERR_INVALID;
+        if (!bufMap.empty() && mappedPtr == (void *)bufMap.data()) return DeviceSuccess;
         return role == ENCODE ? rg_encoder_unmap_dwt_out(enc, mappedPtr) : rg_decoder_unmap_out(dec, mappedPtr);
     }
 
@@ -94,4 +123,5 @@ private:
     rg_encoder *enc;
     rg_decoder *dec;
     double quantStep;
+    std::vector<short> bufMap;   // host copy behind mapBuffer
 };

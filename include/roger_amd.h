@@ -214,6 +214,13 @@ int rg_encoder_bpc(rg_encoder *enc, int cbw, int cbh);
 /* Device pointers of the last run (valid until the next run / destroy). */
 int rg_encoder_coefficients(rg_encoder *enc, int32_t **d_q, int *ncomp, int *w, int *h);
 int rg_encoder_float_coefficients(rg_encoder *enc, float **d_coeff);
+/* Input of DWT level `level` (OCLMemoryManager::getDwtIn,
+ * src/OCLMemoryManager.h:55-59): level 0 is the transform's input planes
+ * (float for 9/7, int32 for 5/3), level k in 1..levels-1 the LL band after k
+ * levels (float, or FP16 when the encode stored inter-level LL as FP16;
+ * int32 for 5/3), ncomp contiguous planes of *w x *h samples each.  Levels
+ * beyond the transform return RG_ERR_INVALID and a null pointer. */
+int rg_encoder_dwt_in(rg_encoder *enc, int level, void **d_ptr, int *w, int *h);
 /* Number of code-blocks of the last run (all components). */
 int rg_encoder_num_codeblocks(rg_encoder *enc);
 /* Copies per-block results (ncb) and codewords to host; data gets the
@@ -331,7 +338,7 @@ typedef struct rg_batch rg_batch;
 typedef struct {
     const uint8_t *d_src;  /* interleaved uint8 image in device memory, 16-byte aligned */
     int32_t w, h, ncomp;   /* ncomp 1 or 3 */
-    int32_t filter, colour, levels;
+    int32_t filter, colour, levels;  /* levels 1..15 */
     double delta;          /* 9/7 base quantisation step */
 } rg_batch_image;
 int rg_batch_create(rg_context *ctx, rg_batch **out);

@@ -107,6 +107,7 @@ SIGNATURES = {
     "rg_encoder_bpc": (I, [P, I, I]),
     "rg_encoder_coefficients": (I, [P, P, P, P, P]),
     "rg_encoder_float_coefficients": (I, [P, P]),
+    "rg_encoder_dwt_in": (I, [P, I, P, P, P]),
     "rg_encoder_num_codeblocks": (I, [P]),
     "rg_encoder_get_codeblocks": (I, [P, P, P, P, P, L, P]),
     "rg_encoder_get_pass_lengths": (I, [P, P]),

@@ -137,14 +137,15 @@ extern "C" int rg_batch_plan(rg_batch *b, const rg_batch_image *imgs, int n, int
     for (int i = 0; i < n; ++i) {
         const rg_batch_image &m = imgs[i];
         RG_CHECK(m.d_src && m.w > 0 && m.h > 0 && (m.ncomp == 1 || m.ncomp == 3), "image %d: bad shape", i);
-        RG_CHECK(m.levels >= 0 && m.levels <= 15, "image %d: bad levels", i);
+        // every image needs a DWT level: the batch has no level-0 copy/quantise job,
+        // so a 0-level image's coefficient plane would never be written
+        RG_CHECK(m.levels >= 1 && m.levels <= 15, "image %d: levels %d out of range 1..15", i, m.levels);
         RG_CHECK(m.filter == RG_FILTER_53 || m.filter == RG_FILTER_97, "image %d: bad filter", i);
         RG_CHECK(m.colour == RG_COLOUR_NONE || m.ncomp == 3, "image %d: colour transform needs 3 components", i);
         RG_CHECK(m.filter == RG_FILTER_53 || m.delta > 0, "image %d: bad quantisation step", i);
         RG_CHECK(((uintptr_t)m.d_src & 15) == 0, "image %d: source must be 16-byte aligned", i);
         b->maxlev = std::max(b->maxlev, m.levels);
     }
-    RG_CHECK(b->maxlev >= 1, "rg_batch_plan: needs at least one DWT level");
     // ---- buffer layout (element offsets, 64-element aligned) ----
     b->plane_off.resize(n + 1);
     b->scratch_off.resize(n + 1);
@@ -185,14 +186,19 @@ extern "C" int rg_batch_plan(rg_batch *b, const rg_batch_image *imgs, int n, int
             float step[97], inv[97];
             steps_for(m.levels, 8, m.filter == RG_FILTER_97 ? m.delta : 1.0, step, inv);
             const size_t np = (size_t)m.w * m.h;
-            char *A = (char *)b->scratch.p + b->scratch_off[i];
-            char *B = A + (((size_t)d.w[1] * d.h[1] * m.ncomp * 4 + 255) / 256) * 256;
+            char *sc = (char *)b->scratch.p + b->scratch_off[i];
             rg_fwd_level_args a{};
             a.filter = m.filter;
             a.cw = d.w[l - 1]; a.ch = d.h[l - 1]; a.nplanes = m.ncomp;
             if (l == 1) { a.in = (char *)b->planes.p + b->plane_off[i] * 4; a.in_pitch = m.w; a.in_ps = (long)np; }
-            else { a.in = (l - 1) & 1 ? A : B; a.in_pitch = d.w[l - 1]; a.in_ps = (long)d.w[l - 1] * d.h[l - 1]; }
-            if (l < m.levels) { a.ll_out = l & 1 ? A : B; a.ll_pitch = d.w[l]; a.ll_ps = (long)d.w[l] * d.h[l]; }
+            else {
+                a.in = sc + rg_ll_offset(m.w, m.h, m.ncomp, l - 1);
+                a.in_pitch = d.w[l - 1]; a.in_ps = (long)d.w[l - 1] * d.h[l - 1];
+            }
+            if (l < m.levels) {
+                a.ll_out = sc + rg_ll_offset(m.w, m.h, m.ncomp, l);
+                a.ll_pitch = d.w[l]; a.ll_ps = (long)d.w[l] * d.h[l];
+            }
             a.q = (char *)b->q.p + b->plane_off[i] * 4; a.qp = m.w; a.q_ps = (long)np;
             if (m.filter == RG_FILTER_97) {
                 const int bi = 1 + 3 * (m.levels - l);

@@ -205,8 +205,8 @@ static int check_levels(int w, int h, int levels)
 extern "C" size_t rg_dwt_scratch_bytes(int w, int h, int nplanes, int filter)
 {
     (void)filter;
-    size_t w1 = (w + 1) / 2, h1 = (h + 1) / 2, w2 = (w1 + 1) / 2, h2 = (h1 + 1) / 2;
-    return (w1 * h1 + w2 * h2) * (size_t)nplanes * 4 + 512;
+    if (w <= 0 || h <= 0 || nplanes <= 0) return 512;
+    return rg_ll_offset(w, h, nplanes, 33) + 512;   // the LL regions of up to 32 levels
 }
 
 // Level-0 (no transform) copy / quantisation: plane in -> q.
@@ -228,9 +228,8 @@ static int dwt_fwd_impl(rg_context *c, int filter, const void *in, int w, int h,
                                     (filter == RG_FILTER_97 && !q_float) ? inv_step[0] : 1.0f, q_float, dq);
     RG_CHECK(scratch || levels == 1, "scratch buffer required");
     rg_level_dims d = rg_dims(w, h, levels);
-    size_t sizeA = (size_t)d.w[1] * d.h[1] * nplanes;
-    char *bufA = reinterpret_cast<char *>(scratch);
-    char *bufB = bufA ? bufA + ((sizeA * 4 + 255) / 256) * 256 : nullptr;
+    char *sc = reinterpret_cast<char *>(scratch);
+    auto ll = [&](int l) { return sc + rg_ll_offset(w, h, nplanes, l); };
     const bool half = filter == RG_FILTER_97 && ll_fp16;
     for (int l = 1; l <= levels; ++l) {
         rg_fwd_level_args a{};
@@ -239,11 +238,11 @@ static int dwt_fwd_impl(rg_context *c, int filter, const void *in, int w, int h,
         if (l == 1) {
             a.in = in; a.in_pitch = w; a.in_ps = (long)w * h; a.in_half = 0;
         } else {
-            a.in = (l - 1) & 1 ? bufA : bufB;
+            a.in = ll(l - 1);
             a.in_pitch = d.w[l - 1]; a.in_ps = (long)d.w[l - 1] * d.h[l - 1]; a.in_half = half;
         }
         if (l < levels) {
-            a.ll_out = l & 1 ? bufA : bufB;
+            a.ll_out = ll(l);
             a.ll_pitch = d.w[l]; a.ll_ps = (long)d.w[l] * d.h[l]; a.ll_half = half;
         } else {
             a.ll_out = nullptr;
@@ -287,9 +286,8 @@ extern "C" int rg_dwt_inv(rg_context *c, int filter, const void *dq, int w, int 
         return rg_launch_copy_dequant(c->stream, filter, dq, (long)w * h, nplanes, step ? step[0] : 1.0f, q_float, out);
     RG_CHECK(scratch || levels == 1, "scratch buffer required");
     rg_level_dims d = rg_dims(w, h, levels);
-    size_t sizeA = (size_t)d.w[1] * d.h[1] * nplanes;
-    char *bufA = reinterpret_cast<char *>(scratch);
-    char *bufB = bufA ? bufA + ((sizeA * 4 + 255) / 256) * 256 : nullptr;
+    char *sc = reinterpret_cast<char *>(scratch);
+    auto ll = [&](int l) { return sc + rg_ll_offset(w, h, nplanes, l); };
     for (int l = levels; l >= 1; --l) {
         rg_inv_level_args a{};
         a.filter = filter;
@@ -298,7 +296,7 @@ extern "C" int rg_dwt_inv(rg_context *c, int filter, const void *dq, int w, int 
             a.ll_is_q = 1;
         } else {
             a.ll_is_q = 0;
-            a.ll_in = (l + 1) % 2 == 0 ? bufA : bufB;   // output of level l+1
+            a.ll_in = ll(l);   // LL after level l: the output of level l+1
             a.ll_pitch = d.w[l]; a.ll_ps = (long)d.w[l] * d.h[l];
         }
         a.q = dq; a.qp = w; a.q_ps = (long)w * h; a.q_float = q_float;
@@ -307,7 +305,7 @@ extern "C" int rg_dwt_inv(rg_context *c, int filter, const void *dq, int w, int 
             a.s_ll = step[0]; a.s_hl = step[b]; a.s_lh = step[b + 1]; a.s_hh = step[b + 2];
         }
         if (l > 1) {
-            a.out = l % 2 == 0 ? bufA : bufB;
+            a.out = ll(l - 1);
             a.out_pitch = d.w[l - 1]; a.out_ps = (long)d.w[l - 1] * d.h[l - 1];
         } else {
             a.out = out; a.out_pitch = w; a.out_ps = (long)w * h;
@@ -440,6 +438,7 @@ struct rg_encoder {
     DevBuf input, planes, q, fcoeff, scratch, desc, res, slots, chan16, syms, cm, poffs, arena, seglens, chains,
         chains_e[3], cms, sppm, rdws, rdw;
     int rd = 0;   // rate control enabled: the context modeller records significance-pass masks
+    int rd_ran = 0;  // the last tier-1 run recorded those masks (sppm holds them for e->ncb blocks)
     int nlayers = 1;               // quality layers of the last rate control (1 after a new encode)
     std::vector<int32_t> layer_np; // per block, passes after each layer (nlayers > 1)
     DevBuf d_layer_np;
@@ -506,6 +505,7 @@ static int run_t1(rg_encoder *e)
 {
     e->nlayers = 1;
     e->layer_np.clear();
+    e->rd_ran = 0;
     int rc = prepare_codeblocks(e);
     if (rc) return rc;
     e->slot_bytes = std::max(256, e->cbw * e->cbh * 3);
@@ -542,7 +542,10 @@ static int run_t1(rg_encoder *e)
                                    (int32_t *)e->seglens.p, (uint8_t *)e->slots.p, e->slot_bytes,
                                    (rg_cb_result *)e->res.p, e->style, (int32_t *)e->chains.p, two ? &ov : nullptr,
                                    e->rd ? (uint64_t *)e->sppm.p : nullptr);
-        if (rc == RG_SUCCESS) e->t1_ran = 1;
+        if (rc == RG_SUCCESS) {
+            e->t1_ran = 1;
+            e->rd_ran = e->rd;
+        }
         return rc;
     }
     if (!rg_t1_inline()) {
@@ -589,7 +592,7 @@ extern "C" int rg_encoder_encode_u8(rg_encoder *e, const uint8_t *src, int on_de
     e->w = w; e->h = h; e->ncomp = ncomp; e->levels = p->levels; e->cbw = p->cbw; e->cbh = p->cbh;
     e->filter = p->filter; e->lossy = p->filter == RG_FILTER_97; e->output_dwt = 0; e->bitdepth = p->bitdepth;
     e->colour = p->colour; e->delta = p->delta; e->style = p->style;
-    e->ran = 0; e->t1_ran = 0; e->loaded = 1; e->i16_ran = 0;
+    e->ran = 0; e->t1_ran = 0; e->rd_ran = 0; e->loaded = 1; e->i16_ran = 0;
     band_steps(p->levels, p->bitdepth, p->delta, e->step, e->inv);
     const uint8_t *d_src = src;
     if (!on_device) {
@@ -650,7 +653,8 @@ extern "C" int rg_encoder_load_planar(rg_encoder *e, const void *const *comps, i
     e->w = w; e->h = h; e->ncomp = ncomp; e->lossy = lossy; e->bitdepth = precision > 0 ? precision : 8;
     e->filter = lossy ? RG_FILTER_97 : RG_FILTER_53;
     e->colour = RG_COLOUR_NONE;
-    e->loaded = 1; e->ran = 0; e->t1_ran = 0; e->i16_ran = 0;
+    e->style = 0;   // the stage path (OCLEncoder::run) codes one codeword per block
+    e->loaded = 1; e->ran = 0; e->t1_ran = 0; e->rd_ran = 0; e->i16_ran = 0;
     return RG_SUCCESS;
 }
 
@@ -666,7 +670,7 @@ extern "C" int rg_encoder_dwt(rg_encoder *e, int levels, int output_dwt, double 
     if ((rc = e->scratch.ensure(rg_dwt_scratch_bytes(e->w, e->h, e->ncomp, e->filter)))) return rc;
     e->levels = levels;
     e->output_dwt = e->lossy && output_dwt;
-    e->t1_ran = 0; e->i16_ran = 0;
+    e->t1_ran = 0; e->rd_ran = 0; e->i16_ran = 0;
     band_steps(levels, e->bitdepth, delta, e->step, e->inv);
     e->delta = delta;
     if (e->output_dwt) {
@@ -760,7 +764,9 @@ extern "C" int rg_encoder_rate_control_layers(rg_encoder *e, int nlayers, const 
     RG_CHECK(e, "null encoder");
     if (!e->t1_ran) return rg_fail(RG_ERR_NOT_READY, "tier-1 has not run");
     RG_CHECK(e->style & RG_STYLE_TERMALL, "rate control needs a parallel code-block style (TERMALL)");
-    if (!e->rd) return rg_fail(RG_ERR_NOT_READY, "rate control was not enabled before tier-1 ran");
+    // the masks must come from the tier-1 run whose codewords are cut: enabling
+    // rate control after an encode leaves none (or a stale, smaller set)
+    if (!e->rd_ran) return rg_fail(RG_ERR_NOT_READY, "rate control was not enabled before tier-1 ran");
     RG_CHECK(nlayers >= 1 && nlayers <= 65535 && targets, "bad layer count %d", nlayers);
     for (int l = 0; l < nlayers; ++l)
         RG_CHECK(targets[l] >= 0 && (l == 0 || targets[l] >= targets[l - 1]),
@@ -851,6 +857,25 @@ extern "C" int rg_encoder_float_coefficients(rg_encoder *e, float **d)
     RG_CHECK(e && d, "null argument");
     if (!e->ran || !e->output_dwt) return rg_fail(RG_ERR_NOT_READY, "no float coefficients (output_dwt not set)");
     *d = (float *)e->fcoeff.p;
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_encoder_dwt_in(rg_encoder *e, int level, void **d_ptr, int *w, int *h)
+{
+    RG_CHECK(e && d_ptr, "null argument");
+    *d_ptr = nullptr;
+    if (!e->loaded) return rg_fail(RG_ERR_NOT_READY, "no image loaded");
+    RG_CHECK(level >= 0, "level %d is negative", level);
+    if (level > 0) {
+        if (!e->ran) return rg_fail(RG_ERR_NOT_READY, "the transform has not run");
+        if (level >= e->levels)
+            return rg_fail(RG_ERR_INVALID, "level %d: the %d-level transform has inputs 0..%d", level, e->levels,
+                           e->levels - 1);
+    }
+    const rg_level_dims d = rg_dims(e->w, e->h, level);
+    *d_ptr = level == 0 ? e->planes.p : (char *)e->scratch.p + rg_ll_offset(e->w, e->h, e->ncomp, level);
+    if (w) *w = d.w[level];
+    if (h) *h = d.h[level];
     return RG_SUCCESS;
 }
 

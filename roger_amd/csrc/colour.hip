@@ -5,18 +5,17 @@
 // de-interleave kernel (src/CL/oclplanar.cl:23-39).  T.800 G.2 (RCT) and
 // G.3 (ICT); FP32 operation order identical to the CPU oracle (fmaf chains).
 //
-// Streaming design, forward: a thread handles 4-pixel quads strided by the
-// workgroup, so lane k of a wavefront reads bytes [12k, 12k+12) (RGB) of one
-// contiguous span and writes bytes [16k, 16k+16) of each plane's span: every
-// memory instruction is a fully coalesced, contiguous wavefront access.
-// Inverse: each thread owns 16 consecutive pixels.  No LDS.
+// Streaming design, both directions: a thread handles 4-pixel quads strided
+// by the workgroup, so lane k of a wavefront reads bytes [12k, 12k+12) (RGB)
+// of one contiguous span and writes bytes [16k, 16k+16) of each plane's span
+// (the inverse the other way round): every memory instruction is a fully
+// coalesced, contiguous wavefront access.  No LDS.
 #include <algorithm>
 #include <type_traits>
 #include "rg_internal.h"
 
 namespace {
 
-constexpr int PX = 16;               // inverse: pixels per thread
 constexpr int QPT = 4;               // forward: 4-pixel quads per thread
 constexpr int SPAN = 256 * QPT * 4;  // forward: pixels per 256-thread workgroup
 
@@ -125,48 +124,84 @@ __global__ void __launch_bounds__(256) colour_fwd_kernel(const uint8_t *__restri
     colour_fwd_span<NC, MODE, FLT>(src, npix, planes, (long)blockIdx.x * SPAN);
 }
 
+// Inverse transform of one pixel: planes' values in[0..NC) -> NC bytes.
+template <int NC, int MODE, bool FLT, class T>
+__device__ __forceinline__ void px_inv(const T (&v)[NC], uint8_t (&o)[NC])
+{
+    if constexpr (FLT && MODE == 2) {
+        const float y = v[0], cb = v[1], cr = v[2];
+        o[0] = out_u8f(__builtin_fmaf((float)1.402, cr, y));
+        o[1] = out_u8f(__builtin_fmaf((float)-0.71414, cr, __builtin_fmaf((float)-0.34413, cb, y)));
+        o[2] = out_u8f(__builtin_fmaf((float)1.772, cb, y));
+    } else if constexpr (!FLT && MODE == 1) {
+        const int g = v[0] - ((v[1] + v[2]) >> 2);
+        o[0] = clamp_u8(v[2] + g + 128);
+        o[1] = clamp_u8(g + 128);
+        o[2] = clamp_u8(v[1] + g + 128);
+    } else {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            if constexpr (FLT) o[c] = out_u8f(v[c]);
+            else o[c] = clamp_u8(v[c] + 128);
+        }
+    }
+}
+
+// One quad of the inverse: pixels [p, p+4).  A full quad reads one 16-byte
+// vector per plane and writes NC dwords (4 pixels x NC bytes, p*NC is a
+// multiple of 4); a quad cut by the image end goes pixel by pixel.
+template <int NC, int MODE, bool FLT>
+__device__ __forceinline__ void colour_inv_quad(const void *__restrict__ planes, long npix, uint8_t *__restrict__ dst,
+                                                long p)
+{
+    using T = typename std::conditional<FLT, float, int32_t>::type;
+    using V = typename std::conditional<FLT, float4, int4>::type;
+    const T *in = reinterpret_cast<const T *>(planes);
+    if (p + 4 <= npix) {
+        V vv[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) vv[c] = *reinterpret_cast<const V *>(in + (long)c * npix + p);
+        uint32_t w[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) w[k] = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            T v[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) v[c] = i == 0 ? vv[c].x : (i == 1 ? vv[c].y : (i == 2 ? vv[c].z : vv[c].w));
+            uint8_t o[NC];
+            px_inv<NC, MODE, FLT>(v, o);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) w[(NC * i + c) >> 2] |= (uint32_t)o[c] << (((NC * i + c) & 3) * 8);
+        }
+        uint32_t *d32 = reinterpret_cast<uint32_t *>(dst + p * NC);
+#pragma unroll
+        for (int k = 0; k < NC; ++k) d32[k] = w[k];
+    } else {
+        for (long q = p; q < npix; ++q) {
+            T v[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) v[c] = in[(long)c * npix + q];
+            uint8_t o[NC];
+            px_inv<NC, MODE, FLT>(v, o);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) dst[q * NC + c] = o[c];
+        }
+    }
+}
+
+// Same span layout as the forward: quads t, t+256, ... of the workgroup's
+// SPAN pixels, so each wavefront load is 1 KB contiguous per plane and each
+// store NC x 256 B contiguous.
 template <int NC, int MODE, bool FLT>
 __global__ void __launch_bounds__(256) colour_inv_kernel(const void *__restrict__ planes, long npix,
                                                           uint8_t *__restrict__ dst)
 {
-    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long p0 = g * PX;
-    if (p0 >= npix) return;
-    int n = (int)min((long)PX, npix - p0);
-    uint8_t b[PX * NC];
+    const long p_wg = (long)blockIdx.x * SPAN;
 #pragma unroll
-    for (int i = 0; i < PX; ++i) {
-        if (i >= n) break;
-        if (FLT) {
-            const float *in = reinterpret_cast<const float *>(planes);
-            if (MODE == 2) {
-                float y = in[p0 + i], cb = in[npix + p0 + i], cr = in[2 * npix + p0 + i];
-                float r = __builtin_fmaf((float)1.402, cr, y);
-                float gg = __builtin_fmaf((float)-0.71414, cr, __builtin_fmaf((float)-0.34413, cb, y));
-                float bb = __builtin_fmaf((float)1.772, cb, y);
-                b[3 * i] = out_u8f(r); b[3 * i + 1] = out_u8f(gg); b[3 * i + 2] = out_u8f(bb);
-            } else {
-#pragma unroll
-                for (int c = 0; c < NC; ++c) b[NC * i + c] = out_u8f(in[(long)c * npix + p0 + i]);
-            }
-        } else {
-            const int32_t *in = reinterpret_cast<const int32_t *>(planes);
-            if (MODE == 1) {
-                int y = in[p0 + i], u = in[npix + p0 + i], v = in[2 * npix + p0 + i];
-                int gg = y - ((u + v) >> 2);
-                b[3 * i] = clamp_u8(v + gg + 128); b[3 * i + 1] = clamp_u8(gg + 128); b[3 * i + 2] = clamp_u8(u + gg + 128);
-            } else {
-#pragma unroll
-                for (int c = 0; c < NC; ++c) b[NC * i + c] = clamp_u8(in[(long)c * npix + p0 + i] + 128);
-            }
-        }
-    }
-    if (n == PX) {
-        uint4 *d4 = reinterpret_cast<uint4 *>(dst + p0 * NC);
-#pragma unroll
-        for (int k = 0; k < NC; ++k) d4[k] = *reinterpret_cast<const uint4 *>(&b[16 * k]);
-    } else {
-        for (int i = 0; i < n * NC; ++i) dst[p0 * NC + i] = b[i];
+    for (int u = 0; u < QPT; ++u) {
+        const long p = p_wg + 4 * ((long)u * 256 + threadIdx.x);
+        if (p < npix) colour_inv_quad<NC, MODE, FLT>(planes, npix, dst, p);
     }
 }
 
@@ -236,8 +271,9 @@ int rg_launch_colour_inv(hipStream_t s, const void *planes, int w, int h, int nc
 {
     long npix = (long)w * h;
     if (npix == 0) return RG_SUCCESS;
-    long nthr = (npix + PX - 1) / PX;
-    dim3 grid((unsigned)((nthr + 255) / 256)), block(256);
+    RG_CHECK(((uintptr_t)dst & 3) == 0 && ((uintptr_t)planes & 15) == 0,
+             "colour: destination must be 4-byte and planes 16-byte aligned");
+    dim3 grid((unsigned)((npix + SPAN - 1) / SPAN)), block(256);
     if (ncomp == 3 && colour == RG_COLOUR_RCT)
         colour_inv_kernel<3, 1, false><<<grid, block, 0, s>>>(planes, npix, dst);
     else if (ncomp == 3 && colour == RG_COLOUR_ICT)

@@ -57,6 +57,24 @@ inline rg_level_dims rg_dims(int w, int h, int levels)
     return d;
 }
 
+// Inter-level LL buffers of the multi-level DWT (forward and inverse): the LL
+// band after level l (l >= 1; the input of level l+1, ceil(w/2^l) x
+// ceil(h/2^l) samples per plane, nplanes planes, 4 bytes a sample) has its own
+// 256-byte-aligned region of the scratch buffer, so every level's LL stays
+// readable after the transform (the reference's dwtIn pyramid,
+// src/OCLMemoryManager.cpp:124-134).  About a third of the input's size.
+inline size_t rg_ll_offset(int w, int h, int nplanes, int level)
+{
+    size_t off = 0;
+    long lw = w, lh = h;
+    for (int l = 1; l < level; ++l) {
+        lw = (lw + 1) / 2;
+        lh = (lh + 1) / 2;
+        off += ((size_t)lw * lh * nplanes * 4 + 255) / 256 * 256;
+    }
+    return off;
+}
+
 // Kernel launchers (defined in the .hip files).
 int rg_launch_colour_fwd(hipStream_t s, const uint8_t *src, int w, int h, int ncomp, int colour,
                          int out_float, void *planes);

@@ -18,6 +18,8 @@
 #include <cmath>
 #include <cstring>
 #include <vector>
+#include <new>
+#include <stdexcept>
 #include "rg_internal.h"
 #include "rg_t2.h"
 
@@ -122,9 +124,16 @@ struct Block {
 
 }  // namespace
 
-int rg_j2k_read(const uint8_t *cs, long size, rg_j2k_info &info, std::vector<rg_cb_desc> &desc,
-                std::vector<rg_cb_result> &res, std::vector<long> &offsets, std::vector<uint8_t> &data,
-                std::vector<int32_t> &seglen)
+// Caps on what a header may ask the reader to hold: samples of all components
+// (the decode needs about 10 bytes of HBM per sample) and code-blocks (about
+// 100 bytes of host state each).  A codestream beyond them is refused before
+// anything is allocated.
+constexpr long RG_J2K_MAX_SAMPLES = 1L << 30;
+constexpr long RG_J2K_MAX_BLOCKS = 1L << 22;
+
+static int j2k_read_impl(const uint8_t *cs, long size, rg_j2k_info &info, std::vector<rg_cb_desc> &desc,
+                         std::vector<rg_cb_result> &res, std::vector<long> &offsets, std::vector<uint8_t> &data,
+                         std::vector<int32_t> &seglen)
 {
     RG_CHECK(cs && size > 0, "empty codestream");
     Bytes b{cs, size};
@@ -240,8 +249,12 @@ int rg_j2k_read(const uint8_t *cs, long size, rg_j2k_info &info, std::vector<rg_
         }
     }
     // ---- code-block layout (one component's list in band order, raster in a band) ----
+    RG_CHECK((long)info.w * info.h * info.ncomp <= RG_J2K_MAX_SAMPLES, "image of %dx%dx%d samples is beyond the "
+             "reader's limit of %ld", info.w, info.h, info.ncomp, RG_J2K_MAX_SAMPLES);
     const int per = rg_codeblocks(info.w, info.h, info.levels, info.cbw, info.cbh, nullptr, 0);
     RG_CHECK(per > 0, "bad image geometry");
+    RG_CHECK((long)per * info.ncomp <= RG_J2K_MAX_BLOCKS, "%ld code-blocks are beyond the reader's limit of %ld",
+             (long)per * info.ncomp, RG_J2K_MAX_BLOCKS);
     std::vector<rg_cb_desc> d1(per);
     rg_codeblocks(info.w, info.h, info.levels, info.cbw, info.cbh, d1.data(), per);
     std::vector<int> bfirst(nb + 1, 0), bnx(nb, 0), bny(nb, 0);
@@ -370,6 +383,20 @@ done:
     info.ncb = ncb;
     info.data_bytes = (int64_t)data.size();
     return RG_SUCCESS;
+}
+
+int rg_j2k_read(const uint8_t *cs, long size, rg_j2k_info &info, std::vector<rg_cb_desc> &desc,
+                std::vector<rg_cb_result> &res, std::vector<long> &offsets, std::vector<uint8_t> &data,
+                std::vector<int32_t> &seglen)
+{
+    // no C++ exception may cross the C ABI: allocation failures become error codes
+    try {
+        return j2k_read_impl(cs, size, info, desc, res, offsets, data, seglen);
+    } catch (const std::bad_alloc &) {
+        return rg_fail(RG_ERR_OOM, "tier-2 reader: out of host memory");
+    } catch (const std::exception &ex) {
+        return rg_fail(RG_ERR_INVALID, "tier-2 reader: %s", ex.what());
+    }
 }
 
 extern "C" int rg_j2k_parse(const uint8_t *cs, long size, rg_j2k_info *info, rg_cb_desc *desc, rg_cb_result *res,

@@ -455,14 +455,22 @@ class Encoder:
         """Before encoding: record what rate control needs (parallel styles)."""
         check(lib().rg_encoder_enable_rate_control(self.handle, int(bool(on))))
 
+    def _weights(self, weights):
+        """(array, pointer) of per-block weights; the library reads exactly one
+        double per code-block of the last encode, so the length is checked here."""
+        if weights is None:
+            return None, None
+        weights = np.ascontiguousarray(weights, np.float64).ravel()
+        n = check(lib().rg_encoder_num_codeblocks(self.handle))
+        if weights.size != n:
+            raise ValueError("rate control weights: %d given, the encode has %d code-blocks" % (weights.size, n))
+        return weights, weights.ctypes.data_as(ctypes.c_void_p)
+
     def rate_control(self, target_bytes, weights=None):
         """Cuts the code-blocks at the PCRD-opt slope threshold for at most
         target_bytes of code-block data; returns the bytes kept."""
         kept = ctypes.c_long()
-        wp = None
-        if weights is not None:
-            weights = np.ascontiguousarray(weights, np.float64)
-            wp = weights.ctypes.data_as(ctypes.c_void_p)
+        weights, wp = self._weights(weights)
         check(lib().rg_encoder_rate_control(self.handle, int(target_bytes), wp, ctypes.byref(kept)))
         return kept.value
 
@@ -471,10 +479,7 @@ class Encoder:
         returns the bytes kept by the last layer."""
         kept = ctypes.c_long()
         t = np.ascontiguousarray(targets, np.int64)
-        wp = None
-        if weights is not None:
-            weights = np.ascontiguousarray(weights, np.float64)
-            wp = weights.ctypes.data_as(ctypes.c_void_p)
+        weights, wp = self._weights(weights)
         check(lib().rg_encoder_rate_control_layers(self.handle, len(t), t.ctypes.data_as(ctypes.c_void_p), wp,
                                                    ctypes.byref(kept)))
         return kept.value

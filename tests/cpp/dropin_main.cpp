@@ -64,8 +64,30 @@ int main(int argc, char **argv)
         if (enc.getCodeblocks(d, r, o, data) != DeviceSuccess) return 6;
         if (!write_file(out + "/lossy_cb.res", r.data(), r.size() * sizeof(rg_cb_result))) return 5;
         if (!write_file(out + "/lossy_cb.bin", data.data(), data.size())) return 5;
-        // per-channel int16 planes exist after the planar stage
-        if (!enc.getMemoryManager()->getDWTOutByChannel(3)) return 7;
+        // per-channel int16 planes of the planar stage, read through mapBuffer
+        OCLMemoryManager<float> *mm = enc.getMemoryManager();
+        for (int c = 0; c < 4; ++c) {
+            tDeviceMem plane = mm->getDWTOutByChannel(c);
+            short *hp = nullptr;
+            if (!plane || mm->mapBuffer(plane, (void **)&hp) != DeviceSuccess || !hp) return 7;
+            if (!write_file(out + "/planar_c" + std::to_string(c) + ".i16", hp, n * sizeof(short))) return 5;
+            if (mm->unmapMemory(plane, hp) != DeviceSuccess) return 7;
+        }
+        rg_quiet() = true;
+        if (mm->getDWTOutByChannel(4)) return 7;
+        // the dwtIn pyramid: inputs of levels 0 .. levels-1, nothing beyond
+        for (size_t l = 0; l < levels; ++l)
+            if (!mm->getDwtIn(l)) return 13;
+        if (mm->getDwtIn(levels)) return 13;
+        rg_quiet() = false;
+        // LL after one level (input of level 2), component 0, float
+        {
+            const size_t w1 = (w + 1) / 2, h1 = (h + 1) / 2;
+            std::vector<float> ll(w1 * h1);
+            if (rg_memcpy_d2h(ocl->context, ll.data(), mm->getDwtIn(1), ll.size() * sizeof(float)) != RG_SUCCESS)
+                return 13;
+            if (!write_file(out + "/dwtin1_c0.f32", ll.data(), ll.size() * sizeof(float))) return 5;
+        }
     }
     // lossy float DWT out, then the decoder reconstructs the samples
     {

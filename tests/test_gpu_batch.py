@@ -49,3 +49,16 @@ def test_batch_matches_oracle(ctx, use_graph, wpc, cb):
             np.testing.assert_array_equal(res["npasses"], rs[:, 1])
             assert blobs == data, "image %d rep %d" % (i, rep)
     b.close()
+
+
+def test_batch_refuses_zero_levels(ctx):
+    """The batch has no level-0 (copy / quantise) job, so an image with 0 DWT
+    levels beside others with levels is refused on the host at plan time
+    rather than tier-1 coding an unwritten coefficient plane."""
+    from roger_amd._lib import RogerError
+    imgs = [dict(buf=synth.device_image(ctx, w, h, 1, 60 + i), w=w, h=h, ncomp=1, reversible=True, levels=lv)
+            for i, (w, h, lv) in enumerate([(96, 80, 3), (64, 64, 0)])]
+    b = R.Batch(ctx)
+    with pytest.raises(RogerError):
+        b.plan(imgs)
+    b.close()

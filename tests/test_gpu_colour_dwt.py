@@ -69,6 +69,41 @@ def test_colour_rgb_shift_only(ctx, w, h):
     np.testing.assert_array_equal(d_pl.read(np.float32, (3, h, w)), want.astype(np.float32))
 
 
+@pytest.mark.parametrize("w,h", [(1, 1), (5, 3), (17, 5), (64, 64), (257, 300)])
+@pytest.mark.parametrize("nc", [1, 2, 3, 4])
+def test_planar_split_i16(ctx, w, h, nc):
+    """rg_planar_split_i16 (the reference's OCLRGBtoPlanar de-interleave,
+    src/CL/oclplanar.cl:23-39): interleaved int16 channels -> one int16 plane
+    each, over the whole int16 range."""
+    rng = np.random.default_rng(w * 31 + h * 7 + nc)
+    src = rng.integers(-32768, 32768, size=(h, w, nc), dtype=np.int16)
+    d_src = ctx.upload(src)
+    d_pl = ctx.alloc(nc * w * h * 2)
+    R.planar_split_i16(ctx, d_src, w, h, nc, d_pl)
+    np.testing.assert_array_equal(d_pl.read(np.int16, (nc, h, w)), np.moveaxis(src, 2, 0))
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (3, 1), (5, 7), (33, 31), (1031, 65)])
+def test_colour_inv_out_of_range(ctx, w, h):
+    """Inverse colour on planes a decoder can produce (values beyond the
+    8-bit range, fractional float samples): rounding and clamping equal the
+    oracle's for ICT, RCT and the plain level shift."""
+    rng = np.random.default_rng(w + 5 * h)
+    f = (rng.uniform(-400.0, 400.0, size=(3, h, w)).astype(np.float32))
+    f[:, ::3] = np.round(f[:, ::3]) + 0.5          # exact halves
+    i = rng.integers(-700, 700, size=(3, h, w)).astype(np.int32)
+    d_f, d_i = ctx.upload(f), ctx.upload(i)
+    d_out = ctx.alloc(3 * w * h)
+    R.colour_inv(ctx, d_f, w, h, 3, RG_COLOUR_ICT, 1, d_out)
+    np.testing.assert_array_equal(d_out.read(np.uint8, (h, w, 3)), O.ict_inv_f32(f))
+    R.colour_inv(ctx, d_i, w, h, 3, RG_COLOUR_RCT, 0, d_out)
+    np.testing.assert_array_equal(d_out.read(np.uint8, (h, w, 3)), O.rct_inv(i))
+    R.colour_inv(ctx, d_f, w, h, 3, RG_COLOUR_NONE, 1, d_out)
+    np.testing.assert_array_equal(d_out.read(np.uint8, (h, w, 3)), np.stack([O.unshift(f[c]) for c in range(3)], 2))
+    R.colour_inv(ctx, d_i, w, h, 1, RG_COLOUR_NONE, 0, d_out)
+    np.testing.assert_array_equal(d_out.read(np.uint8, (h, w)), O.unshift(i[0]))
+
+
 def _levels_for(w, h):
     return [0, 1, 3, 6]
 

@@ -42,6 +42,14 @@ def test_dropin_encoder_decoder(dropin_bin, tmp_path, w, h):
     qref = [O.quantize(O.dwt97_fwd(c, levels), levels, inv) for c in chans]
     for c in range(4):
         np.testing.assert_array_equal(q[..., c], np.clip(qref[c], -32768, 32767))
+    # OCLRGBtoPlanar: one int16 plane per channel, read back with mapBuffer
+    for c in range(4):
+        plane = np.fromfile(tmp_path / ("planar_c%d.i16" % c), np.int16).reshape(h, w)
+        np.testing.assert_array_equal(plane, np.clip(qref[c], -32768, 32767), err_msg="channel %d" % c)
+    # getDwtIn(1): the LL band after one level, component 0
+    ll1 = np.fromfile(tmp_path / "dwtin1_c0.f32", np.float32).reshape((h + 1) // 2, (w + 1) // 2)
+    np.testing.assert_array_equal(ll1.view(np.uint32),
+                                  O.dwt97_fwd(chans[0], 1)[:(h + 1) // 2, :(w + 1) // 2].view(np.uint32))
     # tier-1 on 32x32 code-blocks, all four components
     res = np.fromfile(tmp_path / "lossy_cb.res", np.int32).reshape(-1, 4)
     blob = np.fromfile(tmp_path / "lossy_cb.bin", np.uint8).tobytes()

@@ -125,8 +125,28 @@ def test_rate_control_errors(ctx):
         enc.rate_control(-1)
     with pytest.raises(RogerError):
         enc.rate_control(1000, np.zeros(enc.codeblocks()[0].shape[0]))
+    with pytest.raises(ValueError):
+        enc.rate_control(1000, np.ones(3))   # one weight per code-block
     kept = enc.rate_control(1 << 30)
     assert 0 < kept == int(enc.codeblocks()[1]["len"].sum())
+    enc.close()
+    # enabled only after the encode: no masks were recorded for these codewords
+    # (the check is on the host, before any launch)
+    enc = R.Encoder(ctx)
+    enc.encode_u8(img, levels=2, reversible=True, style=RG_STYLE_PARALLEL)
+    enc.enable_rate_control()
+    with pytest.raises(RogerError):
+        enc.rate_control(1000)
+    # masks recorded for a small image, then off for a larger encode, then on
+    # again: the stale masks must not be used
+    enc.encode_u8(img, levels=2, reversible=True, style=RG_STYLE_PARALLEL)
+    enc.rate_control(1000)
+    enc.enable_rate_control(False)
+    big = synth.image(256, 192, 1, 4)
+    enc.encode_u8(big, levels=3, reversible=True, style=RG_STYLE_PARALLEL)
+    enc.enable_rate_control(True)
+    with pytest.raises(RogerError):
+        enc.rate_control(1000)
     enc.close()
 
 

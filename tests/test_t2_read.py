@@ -149,3 +149,32 @@ def test_reader_survives_damaged_codestreams():
         except RogerError:
             outcomes["refused"] += 1
     assert outcomes["ok"] + outcomes["refused"] == 300 and outcomes["refused"] > 0
+
+
+def _header_only_stream(w, h, ncomp=1, levels=0, xcb=0, ycb=0):
+    """SOC, SIZ, COD (LRCP, one layer, 5/3), QCD, one empty tile-part, EOC:
+    a codestream whose headers alone describe a w x h image."""
+    import struct
+    siz = struct.pack(">HHHIIIIIIIIH", 0xFF51, 38 + 3 * ncomp, 0, w, h, 0, 0, w, h, 0, 0, ncomp)
+    siz += bytes([7, 1, 1]) * ncomp
+    cod = struct.pack(">HHBBHBBBBBB", 0xFF52, 12, 0, 0, 1, 0, levels, xcb, ycb, 0, 1)
+    nb = 3 * levels + 1
+    qcd = struct.pack(">HHB", 0xFF5C, 3 + nb, 2 << 5) + bytes([9 << 3] * nb)
+    body = b"\x00" * (levels + 1)        # one empty packet per resolution
+    sot = struct.pack(">HHHIBB", 0xFF90, 10, 0, 12 + 2 + len(body), 0, 1)
+    return b"\xff\x4f" + siz + cod + qcd + sot + b"\xff\x93" + body + b"\xff\xd9"
+
+
+def test_reader_refuses_oversized_headers():
+    """An 82-byte codestream whose SIZ/COD ask for 65536x65536 samples in 4x4
+    code-blocks (268M blocks) is refused before any allocation, with an error
+    code rather than an exception or an abort; a small stream with the same
+    headers reads."""
+    huge = _header_only_stream(65536, 65536)
+    assert len(huge) < 100
+    with pytest.raises(RogerError):
+        R.j2k_parse(huge)
+    with pytest.raises(RogerError):
+        R.j2k_parse(_header_only_stream(40000, 40000, ncomp=1, levels=5, xcb=4, ycb=4))
+    info = R.j2k_parse(_header_only_stream(64, 48, levels=2, xcb=2, ycb=2))[0]
+    assert (info["w"], info["h"], info["levels"], info["cbw"]) == (64, 48, 2, 16)
