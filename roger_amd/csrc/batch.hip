@@ -5,7 +5,7 @@
 // one launch per level and per channel; here the plan step lays out every
 // image's buffers in a few large HBM allocations and builds job tables:
 //   colour : one workgroup per {image, 4096-pixel chunk};
-//   DWT    : per level, one workgroup per {image, plane, 64x64 tile}, both
+//   DWT    : per level, one workgroup per {image, plane, 128x16 tile}, both
 //            filters in the same launch;
 //   tier-1 : code-blocks sorted largest first, either one workgroup per
 //            block or a persistent grid (waves_per_cu per CU) pulling block
@@ -206,7 +206,7 @@ extern "C" int rg_batch_plan(rg_batch *b, const rg_batch_image *imgs, int n, int
             }
             const int ai = (int)args.size();
             args.push_back(a);
-            const int tx = (a.cw + 63) / 64, ty = (a.ch + 63) / 64;
+            const int tx = (a.cw + RG_DWT_FWD_TW - 1) / RG_DWT_FWD_TW, ty = (a.ch + RG_DWT_FWD_TH - 1) / RG_DWT_FWD_TH;
             for (int p = 0; p < m.ncomp; ++p)
                 for (int y = 0; y < ty; ++y)
                     for (int x = 0; x < tx; ++x) jobs.push_back(make_int4(ai, p, x, y));

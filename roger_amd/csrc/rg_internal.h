@@ -120,6 +120,11 @@ struct rg_inv_level_args {
 int rg_launch_inv53_level(hipStream_t s, const rg_inv_level_args &a);
 int rg_launch_inv97_level(hipStream_t s, const rg_inv_level_args &a);
 
+// Tile shapes of the level kernels (the batched job tables use them too).
+#define RG_DWT_FWD_TW 128
+#define RG_DWT_FWD_TH 16
+#define RG_DWT_INV_TW 64
+#define RG_DWT_INV_TH 64
 int rg_launch_fwd_level_batch(hipStream_t s, const rg_fwd_level_args *args, const int4 *jobs, int njobs);
 int rg_launch_inv_level_batch(hipStream_t s, const rg_inv_level_args *args, const int4 *jobs, int njobs);
 
