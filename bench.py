@@ -151,11 +151,12 @@ def main():
     STYLE_NAMES = {"bypass": "RESET|TERMALL|BYPASS", "parallel": "RESET|TERMALL", "standard": "standard"}
     STYLE = STYLES[args.style]
     print("generating %dx%d RGB input" % (W, H), file=sys.stderr, flush=True)
-    img = synth.rgb(W, H, 1)
     import hashlib
-    img_sha = hashlib.sha256(img.tobytes()).hexdigest()
     ctx = R.Context(0)
-    d_src = ctx.upload(img)
+    # the seeded image generated on the device (bit-identical to synth.rgb), then read back
+    d_src = synth.device_image(ctx, W, H, 3, 1)
+    img = d_src.read(np.uint8, (H, W, 3))
+    img_sha = hashlib.sha256(img.tobytes()).hexdigest()
     enc = R.Encoder(ctx)
 
     def encode_rate(style, steps, warmup):
@@ -262,6 +263,50 @@ def main():
                 "max_abs_err_vs_input": int(np.abs(rec.astype(int) - img.astype(int)).max()),
                 "psnr_db": round(10 * np.log10(255.0 ** 2 / mse), 2) if mse > 0 else 99.0}
 
+    # reverse-path HBM kernels on their own: inverse 9/7 (dequantisation
+    # fused) level 1 and all levels, inverse ICT to uint8
+    R.dwt_fwd(ctx, RG_FILTER_97, d_pl, W, H, NC, LEVELS, inv, 0, d_q, d_s)
+    d_rec0 = ctx.alloc(NC * W * H * 4)
+    d_u80 = ctx.alloc(NC * W * H)
+    t_idwt_l1 = timed(lambda: R.dwt_inv(ctx, RG_FILTER_97, d_q, W, H, NC, 1, R.band_steps(1, DELTA)[0], d_rec0,
+                                        d_s))
+    t_idwt = timed(lambda: R.dwt_inv(ctx, RG_FILTER_97, d_q, W, H, NC, LEVELS, step, d_rec0, d_s))
+    t_cinv = timed(lambda: R.colour_inv(ctx, d_rec0, W, H, NC, RG_COLOUR_ICT, 1, d_u80))
+    d_rec0.free()
+    d_u80.free()
+
+    # ingest: the same encode from a pinned host image (upload on the upload
+    # stream, double-buffered, overlapping the previous encode's kernels), and
+    # the reference's protocol of a blocking upload inside every iteration
+    pin = ctx.host_alloc(img.nbytes)
+    pin_view = pin.array(np.uint8, img.shape)
+    pin_view[...] = img
+    n_h2d = max(5, args.steps // 2)
+
+    def host_encode():
+        enc.encode_u8(pin_view, levels=LEVELS, reversible=False, cbw=CB, cbh=CB, delta=DELTA, style=STYLE)
+
+    host_encode()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(n_h2d):
+        host_encode()
+    ctx.sync()
+    t_h2d_pipe = (time.perf_counter() - t0) / n_h2d
+    t0 = time.perf_counter()
+    for _ in range(n_h2d):
+        host_encode()
+        ctx.sync()
+    t_h2d_serial = (time.perf_counter() - t0) / n_h2d
+    t_up = timed(lambda: R.h2d_async(ctx, d_src, pin_view))
+    ingest = {"what": "4096x4096 RGB uint8 from pinned host memory, H2D included: 'pipelined' = back-to-back "
+                      "encodes (the upload of image i+1 overlaps the kernels of image i), 'serial' = wait after "
+                      "every encode (upload, then kernels, fully exposed)",
+              "pipelined_mpix_s": round(mpix / t_h2d_pipe, 1), "serial_mpix_s": round(mpix / t_h2d_serial, 1),
+              "upload_ms": round(t_up, 4), "upload_GBps": round(img.nbytes / (t_up * 1e-3) / 1e9, 1)}
+    pin_view = None
+    pin.free()
+
     dec_main = decode_rate(STYLE)
     dec_std = decode_rate(0) if STYLE else dec_main
     # rate control (PCRD-opt) on the same image: the encode recording the
@@ -326,6 +371,11 @@ def main():
                       "t1_standard_codeblocks_per_s": round(len(desc) / (t_t1 * 1e-3), 1),
                       "encode_parallel_style": round(t_enc_par, 4),
                       "encode_bypass_style": round(t_enc_byp, 4)},
+        "reverse_stages_ms": {"dwt97_inv_level1": round(t_idwt_l1, 4),
+                              "dwt97_inv_level1_GBps": round(l1_bytes / (t_idwt_l1 * 1e-3) / 1e9, 1),
+                              "dwt97_inv_6lv": round(t_idwt, 4), "colour_inv_ict": round(t_cinv, 4),
+                              "colour_inv_GBps": round(15.0 * W * H / (t_cinv * 1e-3) / 1e9, 1)},
+        "ingest": ingest,
         "styles": styles,
         "decode": dict(what="device-resident codewords of the same code-block style -> tier-1 decode -> "
                             "dequantisation + inverse 9/7 -> inverse ICT -> uint8 RGB", **dec_main,
@@ -338,31 +388,56 @@ def main():
     if not args.no_serve:
         out["serving"] = bench_serving(ctx, args)
 
+    out["parity"] = None
     if not args.no_cpu:
-        # the oracle (CPU restatement) on the host cores: 1 warm-up + 5 timed
-        # encodes of the full image on up to 16 threads (median reported), and
-        # one single-thread encode of the image's top-left 2048x2048 quarter
+        # the oracle (CPU restatement) on the host cores, protocol of the
+        # reference's timing loop (src/OCLTest.cpp:96-106): 1 warm-up, then 20
+        # timed encodes of the full image on every host thread (median and
+        # mean), and 1 warm-up + 20 single-thread encodes of the image's
+        # top-left 512x512 corner; the full-image encode is also compared with
+        # the GPU's codewords of the same image and style
         from oracle import oracle as O
-        threads = min(os.cpu_count() or 1, 16)
+        import platform
+        import subprocess
+        threads = os.cpu_count() or 1
+        log = lambda m: print(m, file=sys.stderr, flush=True)
 
         def cpu_encode(a, nt):
             t = time.perf_counter()
-            O.encode_image(a, levels=LEVELS, reversible=False, cbw=CB, cbh=CB, delta=DELTA, nthreads=nt,
-                           style=STYLE)
-            return time.perf_counter() - t
+            r = O.encode_image(a, levels=LEVELS, reversible=False, cbw=CB, cbh=CB, delta=DELTA, nthreads=nt,
+                               style=STYLE)
+            return time.perf_counter() - t, r
 
-        cpu_encode(img, threads)
-        runs = sorted(cpu_encode(img, threads) for _ in range(5))
-        quarter = np.ascontiguousarray(img[:H // 2, :W // 2])
-        t1 = cpu_encode(quarter, 1)
-        out["cpu_baseline"] = {"value": round(mpix / runs[2], 3), "unit": "Mpix/s", "cores": threads, "kind": "port",
+        log("cpu baseline: %d threads, full image" % threads)
+        _, (rs, data, q) = cpu_encode(img, threads)
+        out["parity"] = {"what": "GPU codewords, pass counts and bit-planes of the timed encode vs the oracle's "
+                                 "encode of the same image and style",
+                         "codeblocks": len(blobs),
+                         "codewords_equal": bool(data == blobs),
+                         "npasses_equal": bool(np.array_equal(rs[:, 1], res["npasses"])),
+                         "nbp_equal": bool(np.array_equal(rs[:, 2], res["nbp"]))}
+        out["parity"]["ok"] = all(out["parity"][k] for k in ("codewords_equal", "npasses_equal", "nbp_equal"))
+        del rs, data, q
+        runs = sorted(cpu_encode(img, threads)[0] for _ in range(20))
+        corner = np.ascontiguousarray(img[:512, :512])
+        log("cpu baseline: 1 thread, 512x512 corner")
+        cpu_encode(corner, 1)
+        runs1 = sorted(cpu_encode(corner, 1)[0] for _ in range(20))
+        model = platform.processor()
+        try:
+            lines = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout.splitlines()
+            model = next((l.split(":", 1)[1].strip() for l in lines if l.startswith("Model name")), model)
+        except Exception:
+            pass
+        out["cpu_baseline"] = {"value": round(mpix / runs[10], 3), "unit": "Mpix/s", "cores": threads, "kind": "port",
                                "sample": "full %dx%d RGB image, same encode and code-block style (ICT, 9/7 6 "
-                                         "levels, quantisation, tier-1), OpenMP over code-blocks; median of 5 "
-                                         "after 1 warm-up" % (W, H),
-                               "mean": round(mpix * len(runs) / sum(runs), 3),
-                               "single_thread": {"value": round(quarter.shape[0] * quarter.shape[1] / 1e6 / t1, 3),
-                                                 "sample": "top-left %dx%d quarter, one encode, 1 thread"
-                                                           % (W // 2, H // 2)}}
+                                         "levels, quantisation, tier-1), OpenMP over code-blocks on every host "
+                                         "thread; median of 20 after 1 warm-up" % (W, H),
+                               "mean": round(mpix * len(runs) / sum(runs), 3), "cpu_model": model, "nproc": threads,
+                               "single_thread": {"value": round(0.512 * 0.512 / runs1[10], 3),
+                                                 "mean": round(0.512 * 0.512 * len(runs1) / sum(runs1), 3),
+                                                 "sample": "top-left 512x512 corner, 1 thread, median of 20 "
+                                                           "after 1 warm-up"}}
     print(json.dumps(out))
 
 

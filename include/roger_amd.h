@@ -107,9 +107,16 @@ int rg_context_query(rg_context *ctx);
 int rg_malloc(rg_context *ctx, size_t bytes, void **d_ptr);
 int rg_free(rg_context *ctx, void *d_ptr);
 int rg_memcpy_h2d(rg_context *ctx, void *d_dst, const void *src, size_t bytes);   /* stream-ordered, waits */
+/* Stream-ordered and asynchronous (the source must stay unchanged until the
+ * stream reaches the copy; pinned memory makes it asynchronous to the host). */
+int rg_memcpy_h2d_async(rg_context *ctx, void *d_dst, const void *src, size_t bytes);
 int rg_memcpy_d2h(rg_context *ctx, void *dst, const void *d_src, size_t bytes);   /* stream-ordered, waits */
 int rg_memset(rg_context *ctx, void *d_dst, int value, size_t bytes);
 int rg_memcpy_d2d(rg_context *ctx, void *d_dst, const void *d_src, size_t bytes);  /* asynchronous */
+/* Pinned (page-locked) host memory, for host sources whose upload should run
+ * asynchronously at full PCIe rate (rg_encoder_encode_u8 with a host source). */
+int rg_host_alloc(rg_context *ctx, size_t bytes, void **host_ptr);
+int rg_host_free(rg_context *ctx, void *host_ptr);
 /* Streaming device copy kernel (16-byte accesses): the measured HBM ceiling. */
 int rg_stream_copy(rg_context *ctx, void *d_dst, const void *d_src, size_t bytes);
 
@@ -185,8 +192,15 @@ int rg_t1_encode(rg_context *ctx, const int32_t *d_q, int pitch, size_t plane_st
 int rg_encoder_create(rg_context *ctx, rg_encoder **out);
 int rg_encoder_destroy(rg_encoder *enc);
 /* Native path: interleaved uint8 image (device pointer if src_on_device,
- * else host) -> colour -> DWT (+quantisation) -> tier-1.  Asynchronous
- * when the source is on the device. */
+ * else host) -> colour -> DWT (+quantisation) -> tier-1, asynchronous on the
+ * context's stream.  A host source is uploaded on the context's upload
+ * stream into one of two device input buffers used in turn, so the upload
+ * of one image overlaps the kernels of the encode before it (double-buffered
+ * ingest, the design of the reference's unbuilt
+ * src/OCLDataTransferManager.cpp:18-30, replacing the blocking upload of
+ * src/OCLMemoryManager.cpp:150-160); the host memory must stay unchanged
+ * until the encode has completed (rg_context_sync / rg_context_query).
+ * Pinned memory (rg_host_alloc) makes the copy asynchronous to the host. */
 int rg_encoder_encode_u8(rg_encoder *enc, const uint8_t *src, int src_on_device, int w, int h,
                          int ncomp, const rg_encode_params *p);
 /* Drop-in path of OCLEncoder<T>::run(components, w, h, levels, precision):

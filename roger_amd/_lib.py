@@ -81,9 +81,12 @@ SIGNATURES = {
     "rg_malloc": (I, [P, SZ, P]),
     "rg_free": (I, [P, P]),
     "rg_memcpy_h2d": (I, [P, P, P, SZ]),
+    "rg_memcpy_h2d_async": (I, [P, P, P, SZ]),
     "rg_memcpy_d2h": (I, [P, P, P, SZ]),
     "rg_memset": (I, [P, P, I, SZ]),
     "rg_memcpy_d2d": (I, [P, P, P, SZ]),
+    "rg_host_alloc": (I, [P, SZ, P]),
+    "rg_host_free": (I, [P, P]),
     "rg_stream_copy": (I, [P, P, P, SZ]),
     "rg_timer_start": (I, [P]),
     "rg_timer_stop": (I, [P, P]),

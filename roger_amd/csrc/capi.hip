@@ -67,6 +67,7 @@ extern "C" int rg_context_destroy(rg_context *c)
     if (c->stream) { hipStreamSynchronize(c->stream); hipStreamDestroy(c->stream); }
     for (hipStream_t st : c->side)
         if (st) { hipStreamSynchronize(st); hipStreamDestroy(st); }
+    if (c->upload) { hipStreamSynchronize(c->upload); hipStreamDestroy(c->upload); }
     if (c->ev_start) hipEventDestroy(c->ev_start);
     if (c->ev_stop) hipEventDestroy(c->ev_stop);
     delete c;
@@ -125,6 +126,13 @@ extern "C" int rg_memcpy_h2d(rg_context *c, void *dst, const void *src, size_t b
     return RG_SUCCESS;
 }
 
+extern "C" int rg_memcpy_h2d_async(rg_context *c, void *dst, const void *src, size_t bytes)
+{
+    RG_CHECK(c, "null context");
+    RG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    return RG_SUCCESS;
+}
+
 extern "C" int rg_memcpy_d2h(rg_context *c, void *dst, const void *src, size_t bytes)
 {
     RG_CHECK(c, "null context");
@@ -144,6 +152,25 @@ extern "C" int rg_memset(rg_context *c, void *dst, int value, size_t bytes)
 {
     RG_CHECK(c, "null context");
     RG_HIP(hipMemsetAsync(dst, value, bytes, c->stream));
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_host_alloc(rg_context *c, size_t bytes, void **p)
+{
+    RG_CHECK(c && p, "bad arguments");
+    RG_HIP(hipSetDevice(c->device));
+    if (hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return rg_fail(RG_ERR_OOM, "hipHostMalloc(%zu) failed", bytes);
+    }
+    return RG_SUCCESS;
+}
+
+extern "C" int rg_host_free(rg_context *c, void *p)
+{
+    RG_CHECK(c, "null context");
+    if (p) RG_HIP(hipHostFree(p));
     return RG_SUCCESS;
 }
 
@@ -435,7 +462,7 @@ struct rg_encoder {
     int ncb_comp = 0, ncb = 0, slot_bytes = 0, colour = 0, style = 0;
     double delta = 1.0 / 64;
     float step[97], inv[97];
-    DevBuf input, planes, q, fcoeff, scratch, desc, res, slots, chan16, syms, cm, poffs, arena, seglens, chains,
+    DevBuf input[2], planes, q, fcoeff, scratch, desc, res, slots, chan16, syms, cm, poffs, arena, seglens, chains,
         chains_e[3], cms, sppm, rdws, rdw;
     int rd = 0;   // rate control enabled: the context modeller records significance-pass masks
     int rd_ran = 0;  // the last tier-1 run recorded those masks (sppm holds them for e->ncb blocks)
@@ -443,6 +470,11 @@ struct rg_encoder {
     std::vector<int32_t> layer_np; // per block, passes after each layer (nlayers > 1)
     DevBuf d_layer_np;
     HostBuf staging, map;
+    // double-buffered ingest of host sources: input[k] is filled on the
+    // context's upload stream (event up[k]) and released by the colour kernel
+    // that reads it (event used[k]); consecutive encodes alternate k
+    hipEvent_t ev_up[2] = {nullptr, nullptr}, ev_used[2] = {nullptr, nullptr};
+    int in_slot = 0, used_rec[2] = {0, 0};
     // events of the overlapped phased tier-1 (created on first use; its side
     // streams belong to the context)
     hipEvent_t ev_in[3] = {nullptr, nullptr, nullptr}, ev_out[3] = {nullptr, nullptr, nullptr};
@@ -466,13 +498,17 @@ extern "C" int rg_encoder_destroy(rg_encoder *e)
     hipStreamSynchronize(e->ctx->stream);
     for (hipStream_t st : e->ctx->side)
         if (st) hipStreamSynchronize(st);
-    for (DevBuf *b : {&e->input, &e->planes, &e->q, &e->fcoeff, &e->scratch, &e->desc, &e->res, &e->slots, &e->chan16,
+    for (DevBuf *b : {&e->input[0], &e->input[1], &e->planes, &e->q, &e->fcoeff, &e->scratch, &e->desc, &e->res, &e->slots, &e->chan16,
                       &e->syms, &e->cm, &e->poffs, &e->arena, &e->seglens, &e->chains, &e->chains_e[0], &e->chains_e[1], &e->chains_e[2],
                       &e->cms, &e->sppm, &e->rdws, &e->rdw, &e->d_layer_np})
         b->release();
     for (int k = 0; k < 3; ++k) {
         if (e->ev_in[k]) hipEventDestroy(e->ev_in[k]);
         if (e->ev_out[k]) hipEventDestroy(e->ev_out[k]);
+    }
+    for (int k = 0; k < 2; ++k) {
+        if (e->ev_up[k]) hipEventDestroy(e->ev_up[k]);
+        if (e->ev_used[k]) hipEventDestroy(e->ev_used[k]);
     }
     e->staging.release();
     e->map.release();
@@ -595,16 +631,35 @@ extern "C" int rg_encoder_encode_u8(rg_encoder *e, const uint8_t *src, int on_de
     e->ran = 0; e->t1_ran = 0; e->rd_ran = 0; e->loaded = 1; e->i16_ran = 0;
     band_steps(p->levels, p->bitdepth, p->delta, e->step, e->inv);
     const uint8_t *d_src = src;
+    int slot = -1;
     if (!on_device) {
-        if ((rc = e->input.ensure(npix * ncomp))) return rc;
-        RG_HIP(hipMemcpyAsync(e->input.p, src, npix * ncomp, hipMemcpyHostToDevice, e->ctx->stream));
-        d_src = (const uint8_t *)e->input.p;
+        // host source: upload on the context's upload stream into the input
+        // buffer the encode before last used, so this copy overlaps the
+        // previous encode's kernels; the main stream waits for it
+        slot = e->in_slot;
+        e->in_slot ^= 1;
+        if ((rc = e->input[slot].ensure(npix * ncomp))) return rc;
+        rg_context *c = e->ctx;
+        if (!c->upload) RG_HIP(hipStreamCreateWithFlags(&c->upload, hipStreamNonBlocking));
+        for (int k = 0; k < 2; ++k) {
+            if (!e->ev_up[k]) RG_HIP(hipEventCreateWithFlags(&e->ev_up[k], hipEventDisableTiming));
+            if (!e->ev_used[k]) RG_HIP(hipEventCreateWithFlags(&e->ev_used[k], hipEventDisableTiming));
+        }
+        if (e->used_rec[slot]) RG_HIP(hipStreamWaitEvent(c->upload, e->ev_used[slot], 0));
+        RG_HIP(hipMemcpyAsync(e->input[slot].p, src, npix * ncomp, hipMemcpyHostToDevice, c->upload));
+        RG_HIP(hipEventRecord(e->ev_up[slot], c->upload));
+        RG_HIP(hipStreamWaitEvent(c->stream, e->ev_up[slot], 0));
+        d_src = (const uint8_t *)e->input[slot].p;
     }
     if ((rc = e->planes.ensure(npix * ncomp * 4))) return rc;
     if ((rc = e->q.ensure(npix * ncomp * 4))) return rc;
     if ((rc = e->scratch.ensure(rg_dwt_scratch_bytes(w, h, ncomp, p->filter)))) return rc;
     rc = rg_launch_colour_fwd(e->ctx->stream, d_src, w, h, ncomp, p->colour, p->filter == RG_FILTER_97, e->planes.p);
     if (rc) return rc;
+    if (slot >= 0) {
+        RG_HIP(hipEventRecord(e->ev_used[slot], e->ctx->stream));
+        e->used_rec[slot] = 1;
+    }
     rc = dwt_fwd_impl(e->ctx, p->filter, e->planes.p, w, h, ncomp, p->levels, e->inv, p->ll_fp16, e->q.p, 0,
                       e->scratch.p);
     if (rc) return rc;

@@ -12,6 +12,9 @@ struct rg_context {
     // encoders (created on first use): streams per encoder would outnumber
     // the hardware queues and serialise the overlap
     hipStream_t side[3] = {nullptr, nullptr, nullptr};
+    // host-to-device copies of the encoders' host sources (created on first
+    // use): an upload there overlaps the kernels of the previous encode
+    hipStream_t upload = nullptr;
 };
 
 // Thread-local last-error message; set by rg_fail().

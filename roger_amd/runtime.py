@@ -48,6 +48,10 @@ class Context:
         b.write(arr)
         return b
 
+    def host_alloc(self, nbytes):
+        """Pinned host memory (rg_host_alloc) as a PinnedBuffer."""
+        return PinnedBuffer(self, nbytes)
+
     def timer_start(self):
         check(lib().rg_timer_start(self.handle))
 
@@ -115,6 +119,39 @@ class DeviceBuffer:
             pass
 
 
+class PinnedBuffer:
+    """Page-locked host memory (rg_host_alloc / rg_host_free); .array(dtype,
+    shape) is a numpy view of it."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        check(lib().rg_host_alloc(ctx.handle, self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+        ctx._own(self)
+
+    def array(self, dtype, shape):
+        dt = np.dtype(dtype)
+        n = int(np.prod(shape)) * dt.itemsize
+        assert n <= self.nbytes, "view larger than the pinned buffer"
+        raw = (ctypes.c_uint8 * n).from_address(self.ptr)
+        return np.frombuffer(raw, dtype=dt).reshape(shape)
+
+    def free(self):
+        if self.ptr:
+            lib().rg_host_free(self.ctx.handle, _vp(self.ptr))
+            self.ptr = None
+
+    close = free
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 # ---------------------------------------------------------------- band steps
 def band_steps(levels, delta, bitdepth=8):
     """Per-band quantisation steps (T.800 E.1 exponent/mantissa closest to delta):
@@ -153,6 +190,13 @@ def planar_split_i16(ctx, d_src, w, h, ncomp, d_planes):
 
 def scratch_bytes(w, h, nplanes, filt):
     return lib().rg_dwt_scratch_bytes(w, h, nplanes, filt)
+
+
+def h2d_async(ctx, d_dst, host):
+    """Queued copy of a (pinned) host array into device memory on the context
+    stream (for timing uploads)."""
+    host = np.ascontiguousarray(host)
+    check(lib().rg_memcpy_h2d_async(ctx.handle, _vp(d_dst.ptr), host.ctypes.data_as(ctypes.c_void_p), host.nbytes))
 
 
 def dwt_fwd(ctx, filt, d_in, w, h, nplanes, levels, inv_step, ll_fp16, d_q, d_scratch):
@@ -407,6 +451,9 @@ class Encoder:
             h, w = img.shape[:2]
             ncomp = 1 if img.ndim == 2 else img.shape[2]
             src, on_dev = img.ctypes.data_as(ctypes.c_void_p), 0
+            # the upload is asynchronous: keep the host array alive until a
+            # later call (every later call that reads results waits for it)
+            self._host_src = img
         else:
             src, on_dev = _vp(d_src.ptr), 1
         p = EncodeParams()
