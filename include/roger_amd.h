@@ -159,6 +159,18 @@ size_t rg_dwt_scratch_bytes(int w, int h, int nplanes, int filter);
  * src/CL/ocldwt97.cl:487-717. */
 int rg_dwt_fwd(rg_context *ctx, int filter, const void *d_in, int w, int h, int nplanes,
                int levels, const float *inv_step, int ll_fp16, int32_t *d_q, void *d_scratch);
+/* rg_dwt_fwd with no allocation and no workspace argument.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The inter-level LL scratch is a static device array of 96 MiB
+ * that every GPU the library runs on holds until the process ends (enough for
+ * 4096x4096 x 3 planes or 8192x8192 x 1 plane); rg_dwt_scratch_bytes larger
+ * than that is refused with RG_ERR_INVALID before any work is queued.  All
+ * calls on one GPU share that GPU's array: calls that overlap on one GPU (two
+ * streams, two threads) race and must be serialised by the caller.  The
+ * array keeps the last call's contents; nothing needs to start at zero. */
+int rg_dwt_fwd_noalloc(rg_context *ctx, int filter, const void *d_in, int w, int h, int nplanes,
+                       int levels, const float *inv_step, int ll_fp16, int32_t *d_q);
 /* 9/7 only: forward DWT without quantisation, float Mallat output
  * (the reference's outputDwt path, src/CL/ocldwt97.cl:196-420). */
 int rg_dwt97_fwd_float(rg_context *ctx, const float *d_in, int w, int h, int nplanes, int levels,

@@ -96,6 +96,7 @@ SIGNATURES = {
     "rg_dwt_scratch_bytes": (SZ, [I, I, I, I]),
     "rg_dwt_fwd": (I, [P, I, P, I, I, I, I, P, I, P, P]),
     "rg_dwt97_fwd_float": (I, [P, P, I, I, I, I, P, P]),
+    "rg_dwt_fwd_noalloc": (I, [P, I, P, I, I, I, I, P, I, P]),
     "rg_dwt_inv": (I, [P, I, P, I, I, I, I, P, P, P]),
     "rg_codeblocks": (I, [I, I, I, I, I, P, I]),
     "rg_t1_encode": (I, [P, P, I, SZ, P, I, P, I, P]),

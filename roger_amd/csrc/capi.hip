@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <cmath>
+#include <mutex>
 #include <vector>
 #include <algorithm>
 #include "rg_internal.h"
@@ -39,6 +40,34 @@ extern "C" const char *rg_error_string(int code)
     }
 }
 
+// ------------------------------------------------------------ static DWT scratch
+// Scratch of rg_dwt_fwd_noalloc: the inter-level LL bands live in this
+// file-scope device array instead of caller- or library-allocated memory.
+// Every GPU the library runs on holds its own copy until the process ends;
+// host code uses only the per-device address from hipGetSymbolAddress (the
+// array's name is a host-side stand-in, not a device address).
+#define RG_NOALLOC_SCRATCH_BYTES ((size_t)96 << 20)
+static __device__ __align__(256) unsigned char scratch[RG_NOALLOC_SCRATCH_BYTES];
+static constexpr int RG_MAX_DEVICES = 64;
+static void *g_noalloc_scratch[RG_MAX_DEVICES];
+static std::mutex g_noalloc_mu;
+
+// Looks up this device's copy of the array once, with the device current
+// (called from rg_context_create, the library's set-up call).
+static int noalloc_scratch_setup(int device)
+{
+    RG_CHECK(device >= 0 && device < RG_MAX_DEVICES, "device %d beyond the library's limit of %d", device,
+             RG_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (g_noalloc_scratch[device]) return RG_SUCCESS;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return rg_fail(RG_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    g_noalloc_scratch[device] = p;
+    return RG_SUCCESS;
+}
+
 // ------------------------------------------------------------ context
 extern "C" int rg_context_create(int device, rg_context **out)
 {
@@ -47,6 +76,7 @@ extern "C" int rg_context_create(int device, rg_context **out)
     RG_HIP(hipGetDeviceCount(&n));
     RG_CHECK(device >= 0 && device < n, "rg_context_create: device %d out of range (%d devices)", device, n);
     RG_HIP(hipSetDevice(device));
+    if (int rc = noalloc_scratch_setup(device)) return rc;
     rg_context *c = new rg_context();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -291,6 +321,22 @@ extern "C" int rg_dwt_fwd(rg_context *c, int filter, const void *in, int w, int 
 {
     RG_CHECK(c, "null context");
     return dwt_fwd_impl(c, filter, in, w, h, nplanes, levels, inv_step, ll_fp16, dq, 0, scratch);
+}
+
+extern "C" int rg_dwt_fwd_noalloc(rg_context *c, int filter, const void *in, int w, int h, int nplanes, int levels,
+                                  const float *inv_step, int ll_fp16, int32_t *dq)
+{
+    RG_CHECK(c, "null context");
+    int rc = check_levels(w, h, levels);
+    if (rc) return rc;
+    RG_CHECK(nplanes >= 1 && nplanes <= 4, "nplanes %d out of range 1..4", nplanes);
+    const size_t need = rg_dwt_scratch_bytes(w, h, nplanes, filter);
+    RG_CHECK(need <= RG_NOALLOC_SCRATCH_BYTES,
+             "rg_dwt_fwd_noalloc: %dx%d x %d planes needs %zu bytes of scratch, the static buffer holds %zu", w, h,
+             nplanes, need, RG_NOALLOC_SCRATCH_BYTES);
+    void *sc = c->device >= 0 && c->device < RG_MAX_DEVICES ? g_noalloc_scratch[c->device] : nullptr;
+    if (!sc) return rg_fail(RG_ERR_NOT_READY, "static scratch of device %d was not set up", c->device);
+    return dwt_fwd_impl(c, filter, in, w, h, nplanes, levels, inv_step, ll_fp16, dq, 0, sc);
 }
 
 extern "C" int rg_dwt97_fwd_float(rg_context *c, const float *in, int w, int h, int nplanes, int levels, float *out,

@@ -206,6 +206,14 @@ def dwt_fwd(ctx, filt, d_in, w, h, nplanes, levels, inv_step, ll_fp16, d_q, d_sc
                            _vp(d_q.ptr), _vp(d_scratch.ptr) if d_scratch else None))
 
 
+def dwt_fwd_noalloc(ctx, filt, d_in, w, h, nplanes, levels, inv_step, ll_fp16, d_q):
+    """rg_dwt_fwd with the library's static per-GPU scratch (no allocation)."""
+    inv = None if inv_step is None else np.ascontiguousarray(inv_step, np.float32)
+    check(lib().rg_dwt_fwd_noalloc(ctx.handle, filt, _vp(d_in.ptr), w, h, nplanes, levels,
+                                   None if inv is None else inv.ctypes.data_as(ctypes.c_void_p), int(ll_fp16),
+                                   _vp(d_q.ptr)))
+
+
 def dwt97_fwd_float(ctx, d_in, w, h, nplanes, levels, d_out, d_scratch):
     check(lib().rg_dwt97_fwd_float(ctx.handle, _vp(d_in.ptr), w, h, nplanes, levels, _vp(d_out.ptr),
                                    _vp(d_scratch.ptr) if d_scratch else None))
