@@ -95,30 +95,38 @@ def bench_batched(ctx, args, style):
             "speedup": round(t_plain / t_batch, 3)}
 
 
-def bench_serving(ctx, args):
-    """Serving loop: Poisson arrivals of mixed encode/decode requests (sides
-    log-uniform 128..8192, 70% 9/7 / 30% 5/3) at ~70% of the plain path's
-    measured capacity; continuous batching (4 slots) vs the plain path (1 slot)."""
+def bench_serving(ctx, args, style):
+    """Serving loop: mixed encode/decode requests (sides log-uniform 128..8192,
+    70% 9/7 / 30% 5/3, code-block style `style`), served by the plain path (1
+    slot, one request at a time), by 4 concurrent slots, and by 4 slots that
+    merge waiting encodes into one batch plan (up to 8 per launch); once with
+    every request waiting at t=0 (saturation) and once with Poisson arrivals
+    at 70% of the plain path's measured capacity."""
     from roger_amd import serve
     log = lambda s: print(s, file=sys.stderr, flush=True)
-    reqs = serve.make_requests(ctx, args.serve_requests, seed=42)
+    reqs = serve.make_requests(ctx, args.serve_requests, seed=42, style=style)
     log("serving: %d requests, %.1f Mpix" % (len(reqs), sum(r.mpix for r in reqs)))
-    out = {}
-    plain = serve.Server(slots=1, hbm_budget=96 << 30)
-    plain.warm(reqs)
-    for r in reqs:
-        r.arrival = 0.0
-    cap = plain.run(reqs)
-    rate = 0.7 * len(reqs) / cap["span_s"]
-    serve.poisson_arrivals(reqs, rate, seed=42)
-    out["plain"] = plain.run(reqs)
-    plain.close()
-    cb = serve.Server(slots=4, hbm_budget=96 << 30)
-    cb.warm(reqs)
-    out["continuous"] = cb.run(reqs)
-    cb.close()
+    configs = {"plain": dict(slots=1, merge=1), "slots4": dict(slots=4, merge=1),
+               "slots4_merge8": dict(slots=4, merge=8)}
+    out = {"requests": len(reqs), "mpix": round(sum(r.mpix for r in reqs), 1), "codeblock_style": style,
+           "saturated": {}, "poisson": {}}
+    rate = None
+    # one server at a time, so only its slots' streams are live
+    for name, c in configs.items():
+        srv = serve.Server(slots=c["slots"], merge=c["merge"], hbm_budget=96 << 30)
+        srv.warm(reqs)
+        for r in reqs:
+            r.arrival = 0.0
+        out["saturated"][name] = srv.run(reqs)
+        if rate is None:
+            rate = 0.7 * len(reqs) / out["saturated"]["plain"]["span_s"]
+        serve.poisson_arrivals(reqs, rate, seed=42)
+        out["poisson"][name] = srv.run(reqs)
+        srv.close()
+        log("serving %s: saturated %.1f Mpix/s, poisson p50 %.1f ms" % (name, out["saturated"][name]["mpix_s"],
+                                                                        out["poisson"][name]["p50_ms"]))
     out["arrival_rate_per_s"] = round(rate, 2)
-    out["plain_capacity_req_s"] = round(len(reqs) / cap["span_s"], 2)
+    out["plain_capacity_req_s"] = round(len(reqs) / out["saturated"]["plain"]["span_s"], 2)
     return out
 
 
@@ -386,7 +394,7 @@ def main():
     if not args.no_batch:
         out["batched"] = bench_batched(ctx, args, STYLE)
     if not args.no_serve:
-        out["serving"] = bench_serving(ctx, args)
+        out["serving"] = bench_serving(ctx, args, STYLE)
 
     out["parity"] = None
     if not args.no_cpu:

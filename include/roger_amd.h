@@ -237,6 +237,12 @@ int rg_encoder_dwt(rg_encoder *enc, int levels, int output_dwt, double delta);
 int rg_encoder_planar_i16(rg_encoder *enc);
 int rg_encoder_channel_i16(rg_encoder *enc, int channel, int16_t **d_plane);
 int rg_encoder_bpc(rg_encoder *enc, int cbw, int cbh);
+/* Parallel-style tier-1 of this encoder in `phases` modelling phases: 1 runs
+ * every kernel on the context's stream; 2-4 overlap the early passes' chain
+ * coding with the later passes' modelling on 1-3 side streams of the context
+ * (the default, 0, is three phases).  Callers that run several contexts at
+ * once (a serving loop) use 1 to stay within the GPU's hardware queues. */
+int rg_encoder_set_t1_phases(rg_encoder *enc, int phases);
 /* Device pointers of the last run (valid until the next run / destroy). */
 int rg_encoder_coefficients(rg_encoder *enc, int32_t **d_q, int *ncomp, int *w, int *h);
 int rg_encoder_float_coefficients(rg_encoder *enc, float **d_coeff);

@@ -116,6 +116,7 @@ SIGNATURES = {
     "rg_encoder_get_codeblocks": (I, [P, P, P, P, P, L, P]),
     "rg_encoder_get_pass_lengths": (I, [P, P]),
     "rg_encoder_enable_rate_control": (I, [P, I]),
+    "rg_encoder_set_t1_phases": (I, [P, I]),
     "rg_encoder_rate_control": (I, [P, L, P, P]),
     "rg_encoder_rate_control_layers": (I, [P, I, P, P, P]),
     "rg_j2k_parse": (I, [P, L, P, P, P, P, P, P]),

@@ -10,7 +10,9 @@ import time
 
 CACHE = os.environ.get("RG_AUTOTUNE_CACHE", os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                           ".autotune.json"))
-CANDIDATES = [(64, 0), (64, 4), (64, 8), (32, 0), (32, 8)]   # (code-block size, waves per CU; 0 = per-block grid)
+# (code-block size, waves per CU; 0 = per-block grid).  The context modeller
+# runs at four waves per SIMD, i.e. up to 16 per CU.
+CANDIDATES = [(64, 0), (64, 8), (64, 12), (64, 16), (32, 0), (32, 12), (32, 16)]
 
 
 def signature(images):

@@ -512,6 +512,7 @@ struct rg_encoder {
         chains_e[3], cms, sppm, rdws, rdw;
     int rd = 0;   // rate control enabled: the context modeller records significance-pass masks
     int rd_ran = 0;  // the last tier-1 run recorded those masks (sppm holds them for e->ncb blocks)
+    int t1_phases = 0;   // modelling phases of the parallel-style tier-1 (0: the library default)
     int nlayers = 1;               // quality layers of the last rate control (1 after a new encode)
     std::vector<int32_t> layer_np; // per block, passes after each layer (nlayers > 1)
     DevBuf d_layer_np;
@@ -524,8 +525,20 @@ struct rg_encoder {
     // events of the overlapped phased tier-1 (created on first use; its side
     // streams belong to the context)
     hipEvent_t ev_in[3] = {nullptr, nullptr, nullptr}, ev_out[3] = {nullptr, nullptr, nullptr};
-    std::vector<rg_cb_desc> h_desc;
-    int desc_key[6] = {0, 0, 0, 0, 0, 0};
+    // code-block descriptor tables, one per image shape seen (key: w, h,
+    // levels, cbw, cbh, ncomp), kept in device memory so that a change of
+    // shape queues no copy and waits for nothing; the least recently used
+    // table is dropped beyond RG_DESC_TABLES
+    struct DescTable {
+        int key[6];
+        int ncb_comp;
+        std::vector<rg_cb_desc> h;
+        void *d = nullptr;
+        uint64_t used = 0;
+    };
+    std::vector<DescTable *> dtabs;
+    DescTable *cur = nullptr;
+    uint64_t dclock = 0;
 };
 
 extern "C" int rg_encoder_create(rg_context *c, rg_encoder **out)
@@ -544,7 +557,7 @@ extern "C" int rg_encoder_destroy(rg_encoder *e)
     hipStreamSynchronize(e->ctx->stream);
     for (hipStream_t st : e->ctx->side)
         if (st) hipStreamSynchronize(st);
-    for (DevBuf *b : {&e->input[0], &e->input[1], &e->planes, &e->q, &e->fcoeff, &e->scratch, &e->desc, &e->res, &e->slots, &e->chan16,
+    for (DevBuf *b : {&e->input[0], &e->input[1], &e->planes, &e->q, &e->fcoeff, &e->scratch, &e->res, &e->slots, &e->chan16,
                       &e->syms, &e->cm, &e->poffs, &e->arena, &e->seglens, &e->chains, &e->chains_e[0], &e->chains_e[1], &e->chains_e[2],
                       &e->cms, &e->sppm, &e->rdws, &e->rdw, &e->d_layer_np})
         b->release();
@@ -556,30 +569,68 @@ extern "C" int rg_encoder_destroy(rg_encoder *e)
         if (e->ev_up[k]) hipEventDestroy(e->ev_up[k]);
         if (e->ev_used[k]) hipEventDestroy(e->ev_used[k]);
     }
+    for (rg_encoder::DescTable *t : e->dtabs) {
+        if (t->d) hipFree(t->d);
+        delete t;
+    }
     e->staging.release();
     e->map.release();
     delete e;
     return RG_SUCCESS;
 }
 
+constexpr size_t RG_DESC_TABLES = 512;
+
 static int prepare_codeblocks(rg_encoder *e)
 {
-    int key[6] = {e->w, e->h, e->levels, e->cbw, e->cbh, e->ncomp};
-    if (memcmp(key, e->desc_key, sizeof(key)) == 0 && e->ncb > 0) return RG_SUCCESS;
-    int n = rg_codeblocks(e->w, e->h, e->levels, e->cbw, e->cbh, nullptr, 0);
-    if (n < 0) return n;
-    e->h_desc.resize((size_t)n * e->ncomp);
-    rg_codeblocks(e->w, e->h, e->levels, e->cbw, e->cbh, e->h_desc.data(), n);
-    for (int c = 1; c < e->ncomp; ++c)
-        for (int i = 0; i < n; ++i) { e->h_desc[(size_t)c * n + i] = e->h_desc[i]; e->h_desc[(size_t)c * n + i].comp = c; }
-    e->ncb_comp = n;
-    e->ncb = n * e->ncomp;
-    int rc = e->desc.ensure(sizeof(rg_cb_desc) * e->h_desc.size());
-    if (rc) return rc;
-    RG_HIP(hipMemcpyAsync(e->desc.p, e->h_desc.data(), sizeof(rg_cb_desc) * e->h_desc.size(), hipMemcpyHostToDevice,
-                          e->ctx->stream));
-    RG_HIP(hipStreamSynchronize(e->ctx->stream));
-    memcpy(e->desc_key, key, sizeof(key));
+    const int key[6] = {e->w, e->h, e->levels, e->cbw, e->cbh, e->ncomp};
+    rg_encoder::DescTable *t = nullptr;
+    if (e->cur && memcmp(key, e->cur->key, sizeof(key)) == 0) {
+        t = e->cur;
+    } else {
+        for (rg_encoder::DescTable *x : e->dtabs)
+            if (memcmp(key, x->key, sizeof(key)) == 0) { t = x; break; }
+    }
+    if (!t) {
+        const int n = rg_codeblocks(e->w, e->h, e->levels, e->cbw, e->cbh, nullptr, 0);
+        if (n < 0) return n;
+        if (e->dtabs.size() >= RG_DESC_TABLES) {
+            // drop the least recently used table once queued work is done with it
+            auto lru = std::min_element(e->dtabs.begin(), e->dtabs.end(),
+                                        [](const rg_encoder::DescTable *a, const rg_encoder::DescTable *b) {
+                                            return a->used < b->used;
+                                        });
+            RG_HIP(hipStreamSynchronize(e->ctx->stream));
+            if ((*lru)->d) RG_HIP(hipFree((*lru)->d));
+            if (e->cur == *lru) e->cur = nullptr;
+            delete *lru;
+            e->dtabs.erase(lru);
+        }
+        t = new rg_encoder::DescTable();
+        memcpy(t->key, key, sizeof(key));
+        t->ncb_comp = n;
+        t->h.resize((size_t)n * e->ncomp);
+        rg_codeblocks(e->w, e->h, e->levels, e->cbw, e->cbh, t->h.data(), n);
+        for (int c = 1; c < e->ncomp; ++c)
+            for (int i = 0; i < n; ++i) { t->h[(size_t)c * n + i] = t->h[i]; t->h[(size_t)c * n + i].comp = c; }
+        const size_t bytes = sizeof(rg_cb_desc) * t->h.size();
+        if (hipMalloc(&t->d, bytes ? bytes : 1) != hipSuccess) {
+            (void)hipGetLastError();
+            delete t;
+            return rg_fail(RG_ERR_OOM, "hipMalloc(%zu) failed", bytes);
+        }
+        // t->h stays unchanged while the table lives, so the copy needs no wait
+        if (hipMemcpyAsync(t->d, t->h.data(), bytes, hipMemcpyHostToDevice, e->ctx->stream) != hipSuccess) {
+            hipFree(t->d);
+            delete t;
+            return rg_fail(RG_ERR_DEVICE, "hipMemcpyAsync of the code-block table failed");
+        }
+        e->dtabs.push_back(t);
+    }
+    t->used = ++e->dclock;
+    e->cur = t;
+    e->ncb_comp = t->ncb_comp;
+    e->ncb = t->ncb_comp * e->ncomp;
     return RG_SUCCESS;
 }
 
@@ -602,7 +653,7 @@ static int run_t1(rg_encoder *e)
             (rc = e->chains.ensure(rg_t1_chain_ws_bytes(e->ncb))))
             return rc;
         rg_t1_overlap ov{};
-        const int phases = rg_t1_phases();
+        const int phases = e->t1_phases ? e->t1_phases : rg_t1_phases();
         const bool two = phases > 1;
         if (two) {
             ov.nearly = phases - 1;
@@ -619,7 +670,7 @@ static int run_t1(rg_encoder *e)
         }
         if (e->rd && (rc = e->sppm.ensure(rg_rd_sppm_bytes(e->ncb)))) return rc;
         rc = rg_launch_t1_parallel(e->ctx->stream, (const int32_t *)e->q.p, e->w, (size_t)e->w * e->h,
-                                   (const rg_cb_desc *)e->desc.p, e->ncb, (uint8_t *)e->syms.p, symcap,
+                                   (const rg_cb_desc *)e->cur->d, e->ncb, (uint8_t *)e->syms.p, symcap,
                                    (int4 *)e->cm.p, (int32_t *)e->poffs.p, (uint8_t *)e->arena.p,
                                    (int32_t *)e->seglens.p, (uint8_t *)e->slots.p, e->slot_bytes,
                                    (rg_cb_result *)e->res.p, e->style, (int32_t *)e->chains.p, two ? &ov : nullptr,
@@ -634,13 +685,13 @@ static int run_t1(rg_encoder *e)
         const int symcap = rg_t1_symcap(e->cbw, e->cbh);
         if ((rc = e->syms.ensure((size_t)symcap * e->ncb)) || (rc = e->cm.ensure(sizeof(int4) * e->ncb))) return rc;
         rc = rg_launch_t1_split(e->ctx->stream, (const int32_t *)e->q.p, e->w, (size_t)e->w * e->h,
-                                (const rg_cb_desc *)e->desc.p, e->ncb, (uint8_t *)e->syms.p, symcap, (int4 *)e->cm.p,
+                                (const rg_cb_desc *)e->cur->d, e->ncb, (uint8_t *)e->syms.p, symcap, (int4 *)e->cm.p,
                                 (uint8_t *)e->slots.p, e->slot_bytes, (rg_cb_result *)e->res.p);
         if (rc == RG_SUCCESS) e->t1_ran = 1;
         return rc;
     }
     rc = rg_launch_t1_encode(e->ctx->stream, (const int32_t *)e->q.p, e->w, (size_t)e->w * e->h,
-                             (const rg_cb_desc *)e->desc.p, e->ncb, (uint8_t *)e->slots.p, e->slot_bytes,
+                             (const rg_cb_desc *)e->cur->d, e->ncb, (uint8_t *)e->slots.p, e->slot_bytes,
                              (rg_cb_result *)e->res.p);
     if (rc == RG_SUCCESS) e->t1_ran = 1;
     return rc;
@@ -844,7 +895,16 @@ static void default_rd_weights(const rg_encoder *e, std::vector<double> &w)
         const double st = e->lossy ? (double)e->step[band] : 1.0;
         band_w[band] = st * st * g * 0.25;
     }
-    for (int i = 0; i < e->ncb; ++i) w[i] = band_w[e->h_desc[i].band];
+    for (int i = 0; i < e->ncb; ++i) w[i] = band_w[e->cur->h[i].band];
+}
+
+extern "C" int rg_encoder_set_t1_phases(rg_encoder *e, int phases)
+{
+    RG_CHECK(e, "null encoder");
+    RG_CHECK(phases >= 0 && phases <= RG_T1_MAX_EARLY + 1, "phases %d out of range 0..%d", phases,
+             RG_T1_MAX_EARLY + 1);
+    e->t1_phases = phases;
+    return RG_SUCCESS;
 }
 
 extern "C" int rg_encoder_enable_rate_control(rg_encoder *e, int on)
@@ -889,7 +949,7 @@ extern "C" int rg_encoder_rate_control_layers(rg_encoder *e, int nlayers, const 
     long k = 0;
     int err = 0;
     rc = rg_launch_rate_control(e->ctx->stream, (const int32_t *)e->q.p, e->w, (size_t)e->w * e->h,
-                                (const rg_cb_desc *)e->desc.p, e->ncb, (const int4 *)e->cm.p,
+                                (const rg_cb_desc *)e->cur->d, e->ncb, (const int4 *)e->cm.p,
                                 (const uint64_t *)e->sppm.p, (const int32_t *)e->seglens.p, (const double *)e->rdw.p,
                                 nlayers, targets, e->rdws.p, nlayers > 1 ? (int32_t *)e->d_layer_np.p : nullptr,
                                 (rg_cb_result *)e->res.p, &k, &err);
@@ -1001,7 +1061,7 @@ extern "C" int rg_encoder_get_codeblocks(rg_encoder *e, rg_cb_desc *desc, rg_cb_
         sum += r[i].len;
     }
     if (total) *total = sum;
-    if (desc) memcpy(desc, e->h_desc.data(), sizeof(rg_cb_desc) * e->ncb);
+    if (desc) memcpy(desc, e->cur->h.data(), sizeof(rg_cb_desc) * e->ncb);
     if (res) memcpy(res, r.data(), sizeof(rg_cb_result) * e->ncb);
     if (data) {
         RG_CHECK(cap >= sum, "data buffer too small (%ld < %ld)", cap, sum);
@@ -1056,7 +1116,7 @@ extern "C" int rg_encoder_write_j2k(rg_encoder *e, uint8_t *out, long cap, long 
     rg_t2_image im{e->w, e->h, e->ncomp, e->levels, e->cbw, e->cbh, e->filter == RG_FILTER_53,
                    e->ncomp == 3 && e->colour != RG_COLOUR_NONE, 2, e->bitdepth, e->delta, e->style};
     std::vector<uint8_t> cs;
-    if ((rc = rg_j2k_assemble(im, e->h_desc.data(), res.data(), ptr.data(), seglen.empty() ? nullptr : seglen.data(),
+    if ((rc = rg_j2k_assemble(im, e->cur->h.data(), res.data(), ptr.data(), seglen.empty() ? nullptr : seglen.data(),
                               RG_MAX_PASSES, e->ncb_comp, cs, e->nlayers,
                               e->nlayers > 1 ? e->layer_np.data() : nullptr)))
         return rc;

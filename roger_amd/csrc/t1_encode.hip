@@ -55,18 +55,21 @@ __global__ void __launch_bounds__(64) t1_encode_jobs_kernel(const T1Job *__restr
 
 // Persistent form of the batched tier-1: a grid sized to the machine pulls job
 // indices from a device counter (jobs sorted largest block first) until the
-// list is exhausted.  Every lane executes the atomic (lane 0 adds 1, the
-// others 0) and takes lane 0's result with readlane, and the block body has no
-// lane-dependent statement, so the loop's exit is wave-uniform by
-// construction: the index never comes from a broadcast that assumes the lanes
-// are converged.
+// list is exhausted.  One atomic per job: lane 0 alone takes the next index
+// and leaves it in workgroup memory; after the barrier every lane reads the
+// same word, so all lanes see the same index and leave the loop in the same
+// iteration (no lane ever depends on a broadcast that assumes convergence).
+// The second barrier keeps lane 0's next write behind every lane's read.
 template <bool SPLIT>
 __device__ __forceinline__ void t1_queue_loop(const T1Job *__restrict__ jobs, int njobs, int *__restrict__ counter)
 {
     __shared__ uint8_t lut[LUT_BYTES];
+    __shared__ int next;
     for (;;) {
-        const int got = atomicAdd(counter, threadIdx.x == 0 ? 1 : 0);
-        const int i = __builtin_amdgcn_readlane(got, 0);
+        if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+        __syncthreads();
+        const int i = next;
+        __syncthreads();
         if (i >= njobs) break;
         t1_block<SPLIT>(jobs[i], lut);
     }

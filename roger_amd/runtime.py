@@ -506,6 +506,10 @@ class Encoder:
         check(lib().rg_encoder_get_pass_lengths(self.handle, out.ctypes.data_as(ctypes.c_void_p)))
         return out
 
+    def set_t1_phases(self, phases):
+        """Modelling phases of the parallel-style tier-1 (1: one stream; 0: default)."""
+        check(lib().rg_encoder_set_t1_phases(self.handle, int(phases)))
+
     def enable_rate_control(self, on=True):
         """Before encoding: record what rate control needs (parallel styles)."""
         check(lib().rg_encoder_enable_rate_control(self.handle, int(bool(on))))
