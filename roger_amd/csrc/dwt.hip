@@ -27,12 +27,15 @@
 // Short tiles keep every load of a workgroup in one round and give 12K+
 // workgroups per 4096x4096 level; the vertical halo (8 rows per 16) is read
 // again by the neighbouring tile, which the XCD-aware tile order places on
-// the same L2.  Inverse: one 320-thread workgroup per 64x64 tile; the tile
+// the same L2.  Inverse: one 512-thread workgroup per 64x64 tile; the tile
 // and its halo are first staged into LDS, interleaved and dequantised, with
-// every load of a thread issued before any is used and one load instruction
-// per sample (band pointer chosen per lane, no divergent branch); then the
-// horizontal synthesis (T.800 2D_SR: rows first) and the vertical one, whose
-// column items store 64-wide row runs (non-temporal).
+// every load of a thread issued before any is used: interior tiles read each
+// band row in aligned 16-byte quads (four samples of one band per load; per
+// sample, the per-element address work and dequantisation made the staging
+// the bottleneck), border tiles sample by sample with the band pointer
+// chosen per lane; then the horizontal synthesis (T.800 2D_SR: rows first)
+// and the vertical one, whose column items store 64-wide row runs
+// (non-temporal).
 #include <hip/hip_fp16.h>
 #include "rg_internal.h"
 
@@ -40,7 +43,7 @@ namespace {
 
 constexpr int SEG = 16;                    // samples a work item lifts in one run
 constexpr int FTW = RG_DWT_FWD_TW, FTH = RG_DWT_FWD_TH, FWG = 192;   // forward tile and workgroup
-constexpr int ITW = RG_DWT_INV_TW, ITH = RG_DWT_INV_TH, IWG = 320;   // inverse tile and workgroup
+constexpr int ITW = RG_DWT_INV_TW, ITH = RG_DWT_INV_TH, IWG = 512;   // inverse tile and workgroup
 
 // LDS column of tile column c: one pad word after every 16 columns
 __host__ __device__ constexpr int lcol(int c) { return c + (c >> 4); }
@@ -297,37 +300,84 @@ __device__ __forceinline__ void inv_tile(const rg_inv_level_args &a, int plane, 
     const int32_t *qb = reinterpret_cast<const int32_t *>(a.q) + (long)plane * a.q_ps;
     const int32_t *lb = a.ll_is_q ? qb : reinterpret_cast<const int32_t *>(a.ll_in) + (long)plane * a.ll_ps;
 
-    // ---- stage: every load first (raw 32-bit words), then convert ----
-    int32_t raw[NI];
-    float stp[NI];      // 9/7: dequantisation step of an index, 0 for a float word
+    // ---- stage: every load first, then convert ----
+    // Interior tiles read each band row as aligned 16-byte quads: band indices
+    // [bq, bq + 4*NQ), bq = x0/2 - 4, band index b landing in interleaved
+    // column 2*(b - (x0-H)/2) + (high band); other tiles go sample by sample.
+    const int32_t *lrow_base = a.ll_is_q ? qb : lb;
+    const long lpitch = a.ll_is_q ? a.qp : a.ll_pitch;
+    const bool quads = x0 >= 8 && y0 >= H && (x0 >> 1) + TW / 2 + 4 <= wl && y0 + TH + H <= ch &&
+                       ((a.qp | lpitch | wl) & 3) == 0 && (((uintptr_t)qb | (uintptr_t)lrow_base) & 15) == 0;
+    if (quads) {
+        constexpr int NQ = (TW / 2 + 8) / 4;            // quads per band and row
+        constexpr int NI4 = (2 * NQ * RH + WGS - 1) / WGS;
+        int4 raw[NI4];
+        float stp[NI4];                                  // 9/7: step of an index, 0 for a float word
+        const int bq = (x0 >> 1) - 4;
 #pragma unroll
-    for (int i = 0; i < NI; ++i) {
-        const int item = threadIdx.x + i * WGS;
-        const int it = item < CW * RH ? item : 0;
-        const int c = it % CW, rr = it / CW;
-        const int y = simple ? mirror1(y0 - H + rr, ch) : rg_mirror(y0 - H + rr, ch);
-        const int x = simple ? mirror1(x0 - H + c, cw) : rg_mirror(x0 - H + c, cw);
-        const bool ly = !(y & 1), lx = !(x & 1);
-        const int byy = y >> 1, bxx = x >> 1;
-        const bool from_ll = ly && lx && !a.ll_is_q;
-        const int32_t *p = from_ll ? lb + (long)byy * a.ll_pitch + bxx
-                                   : qb + (long)(ly ? byy : hl + byy) * a.qp + (lx ? bxx : wl + bxx);
-        raw[i] = *p;
-        if constexpr (F == RG_FILTER_97)
-            stp[i] = (from_ll || a.q_float) ? 0.0f : (ly ? (lx ? a.s_ll : a.s_hl) : (lx ? a.s_lh : a.s_hh));
-    }
-#pragma unroll
-    for (int i = 0; i < NI; ++i) {
-        const int item = threadIdx.x + i * WGS;
-        if (item >= CW * RH) break;
-        const int c = item % CW, rr = item / CW;
-        T v;
-        if constexpr (F == RG_FILTER_53) {
-            v = raw[i];
-        } else {
-            v = stp[i] != 0.0f ? dequant(raw[i], stp[i]) : __int_as_float(raw[i]);
+        for (int i = 0; i < NI4; ++i) {
+            const int item = threadIdx.x + i * WGS;
+            const int it = item < 2 * NQ * RH ? item : 0;
+            const int j = it % (2 * NQ), rr = it / (2 * NQ);
+            const int high = j >= NQ, m4 = j - high * NQ;
+            const int y = y0 - H + rr;
+            const bool ly = !(y & 1);
+            const int byy = y >> 1;
+            const bool from_ll = ly && !high;
+            const int32_t *p = from_ll ? lrow_base + (long)byy * lpitch + bq + 4 * m4
+                                       : qb + (long)(ly ? byy : hl + byy) * a.qp + (high ? wl : 0) + bq + 4 * m4;
+            raw[i] = *reinterpret_cast<const int4 *>(p);
+            if constexpr (F == RG_FILTER_97)
+                stp[i] = ((from_ll && !a.ll_is_q) || a.q_float) ? 0.0f
+                                                                 : (ly ? (high ? a.s_hl : a.s_ll) : (high ? a.s_hh : a.s_lh));
         }
-        tile[rr * LP + lcol(c)] = v;
+#pragma unroll
+        for (int i = 0; i < NI4; ++i) {
+            const int item = threadIdx.x + i * WGS;
+            if (item >= 2 * NQ * RH) break;
+            const int j = item % (2 * NQ), rr = item / (2 * NQ);
+            const int high = j >= NQ, m4 = j - high * NQ;
+            const int c0 = 2 * (4 * m4 + H / 2 - 4) + high;
+            const int32_t v4[4] = {raw[i].x, raw[i].y, raw[i].z, raw[i].w};
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int c = c0 + 2 * u;
+                if (c < 0 || c >= CW) continue;
+                T v;
+                if constexpr (F == RG_FILTER_53) v = v4[u];
+                else v = stp[i] != 0.0f ? dequant(v4[u], stp[i]) : __int_as_float(v4[u]);
+                tile[rr * LP + lcol(c)] = v;
+            }
+        }
+    } else {
+        int32_t raw[NI];
+        float stp[NI];      // 9/7: dequantisation step of an index, 0 for a float word
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+            const int item = threadIdx.x + i * WGS;
+            const int it = item < CW * RH ? item : 0;
+            const int c = it % CW, rr = it / CW;
+            const int y = simple ? mirror1(y0 - H + rr, ch) : rg_mirror(y0 - H + rr, ch);
+            const int x = simple ? mirror1(x0 - H + c, cw) : rg_mirror(x0 - H + c, cw);
+            const bool ly = !(y & 1), lx = !(x & 1);
+            const int byy = y >> 1, bxx = x >> 1;
+            const bool from_ll = ly && lx && !a.ll_is_q;
+            const int32_t *p = from_ll ? lb + (long)byy * a.ll_pitch + bxx
+                                       : qb + (long)(ly ? byy : hl + byy) * a.qp + (lx ? bxx : wl + bxx);
+            raw[i] = *p;
+            if constexpr (F == RG_FILTER_97)
+                stp[i] = (from_ll || a.q_float) ? 0.0f : (ly ? (lx ? a.s_ll : a.s_hl) : (lx ? a.s_lh : a.s_hh));
+        }
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+            const int item = threadIdx.x + i * WGS;
+            if (item >= CW * RH) break;
+            const int c = item % CW, rr = item / CW;
+            T v;
+            if constexpr (F == RG_FILTER_53) v = raw[i];
+            else v = stp[i] != 0.0f ? dequant(raw[i], stp[i]) : __int_as_float(raw[i]);
+            tile[rr * LP + lcol(c)] = v;
+        }
     }
     __syncthreads();
 
