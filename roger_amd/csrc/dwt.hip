@@ -209,6 +209,7 @@ __device__ __forceinline__ void fwd_tile(const rg_fwd_level_args &a, int plane, 
     const bool simple = cw > H && ch > H;      // one reflection suffices
     // the tile and its halo lie inside the level: no extension, no bounds tests
     const bool interior = x0 >= H && y0 >= H && x0 + TW + H <= cw && y0 + TH + H <= ch;
+    const bool small_plane = (long)ch * a.in_pitch * 4 < (1L << 31);   // byte offsets fit 32 bits
 
     // ---- vertical pass straight from HBM: (column c of CW, segment k) ----
     for (int item = threadIdx.x; item < CW * VS; item += WGS) {
@@ -218,8 +219,20 @@ __device__ __forceinline__ void fwd_tile(const rg_fwd_level_args &a, int plane, 
         T e[N];
         if (interior) {
             const long first = (long)ybase * a.in_pitch + x;
+            if (F == RG_FILTER_97 && a.in_half) {
 #pragma unroll
-            for (int j = 0; j < N; ++j) e[j] = load_in<F>(in, first + (long)j * a.in_pitch, a.in_half);
+                for (int j = 0; j < N; ++j) e[j] = load_in<F>(in, first + (long)j * a.in_pitch, 1);
+            } else if (small_plane) {
+                // 32-bit byte offsets from the uniform plane base: one add per load
+                const char *pb = reinterpret_cast<const char *>(in);
+                const uint32_t b0 = (uint32_t)first * (uint32_t)sizeof(T), pitch_b = (uint32_t)a.in_pitch * sizeof(T);
+#pragma unroll
+                for (int j = 0; j < N; ++j) e[j] = *reinterpret_cast<const T *>(pb + (b0 + (uint32_t)j * pitch_b));
+            } else {
+                const T *ip = reinterpret_cast<const T *>(in) + first;
+#pragma unroll
+                for (int j = 0; j < N; ++j) e[j] = ip[(long)j * a.in_pitch];
+            }
         } else if (simple) {
 #pragma unroll
             for (int j = 0; j < N; ++j) e[j] = load_in<F>(in, (long)mirror1(ybase + j, ch) * a.in_pitch + x, a.in_half);
