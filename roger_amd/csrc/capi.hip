@@ -1169,20 +1169,22 @@ extern "C" int rg_encoder_unmap_dwt_out(rg_encoder *e, void *host)
 struct rg_decoder {
     rg_context *ctx = nullptr;
     int w = 0, h = 0, ncomp = 0, lossy = 0, ran = 0, loaded = 0, u8_ran = 0;
-    DevBuf coeff, out, scratch, cbdata, cboffs, cbres, cbdesc, u8, cbseg;
+    DevBuf coeff, out, scratch, cbdata, cboffs, cbres, cbdesc, u8, cbseg, t1ws;
     HostBuf staging, map;
 };
 
 int rg_launch_t1_decode(hipStream_t s, const uint8_t *data, const long *offs, const rg_cb_result *res,
                         const rg_cb_desc *desc, int ncb, int32_t *q, int pitch, size_t plane_stride, int style,
-                        const int32_t *seglens);
+                        const int32_t *seglens, void *ws);
+size_t rg_t1_decode_ws_bytes(int ncb);
 
 extern "C" int rg_t1_decode(rg_context *c, const uint8_t *d_data, const long *d_offsets, const rg_cb_result *d_res,
                             const rg_cb_desc *d_desc, int ncb, int32_t *d_q, int pitch, size_t plane_stride)
 {
     RG_CHECK(c && d_data && d_offsets && d_res && d_desc && d_q, "rg_t1_decode: null argument");
     RG_CHECK(ncb >= 0 && pitch > 0, "rg_t1_decode: bad sizes");
-    return rg_launch_t1_decode(c->stream, d_data, d_offsets, d_res, d_desc, ncb, d_q, pitch, plane_stride, 0, nullptr);
+    return rg_launch_t1_decode(c->stream, d_data, d_offsets, d_res, d_desc, ncb, d_q, pitch, plane_stride, 0, nullptr,
+                               nullptr);
 }
 
 extern "C" int rg_t1_decode_style(rg_context *c, const uint8_t *d_data, const long *d_offsets,
@@ -1195,7 +1197,7 @@ extern "C" int rg_t1_decode_style(rg_context *c, const uint8_t *d_data, const lo
              "code-block style %d: only 0, RG_STYLE_PARALLEL and RG_STYLE_PARALLEL_BYPASS", style);
     RG_CHECK(!(style & RG_STYLE_TERMALL) || d_seglen, "TERMALL style needs the per-pass segment lengths");
     return rg_launch_t1_decode(c->stream, d_data, d_offsets, d_res, d_desc, ncb, d_q, pitch, plane_stride, style,
-                               (style & RG_STYLE_TERMALL) ? d_seglen : nullptr);
+                               (style & RG_STYLE_TERMALL) ? d_seglen : nullptr, nullptr);
 }
 
 static int decode_codeblocks(rg_decoder *d, const rg_cb_desc *desc, const rg_cb_result *res, const long *offsets,
@@ -1262,7 +1264,8 @@ static int decode_codeblocks(rg_decoder *d, const rg_cb_desc *desc, const rg_cb_
         (rc = d->cbres.ensure(sizeof(rg_cb_result) * ncb)) || (rc = d->cbdesc.ensure(sizeof(rg_cb_desc) * ncb)) ||
         (rc = d->coeff.ensure(npix * ncomp * 4)) || (rc = d->out.ensure(npix * ncomp * 4)) ||
         (rc = d->scratch.ensure(rg_dwt_scratch_bytes(w, h, ncomp, p->filter))) || (rc = d->u8.ensure(npix * ncomp)) ||
-        (termall && (rc = d->cbseg.ensure(sizeof(int32_t) * RG_MAX_PASSES * ncb))))
+        (termall && (rc = d->cbseg.ensure(sizeof(int32_t) * RG_MAX_PASSES * ncb))) ||
+        (rc = d->t1ws.ensure(rg_t1_decode_ws_bytes(ncb))))
         return rc;
     hipStream_t s = d->ctx->stream;
     // stage the host arrays in pinned memory so the uploads are truly
@@ -1286,7 +1289,8 @@ static int decode_codeblocks(rg_decoder *d, const rg_cb_desc *desc, const rg_cb_
     RG_HIP(hipMemcpyAsync(d->cbdesc.p, st + o3, sizeof(rg_cb_desc) * ncb, hipMemcpyHostToDevice, s));
     if ((rc = rg_launch_t1_decode(s, (const uint8_t *)d->cbdata.p, (const long *)d->cboffs.p,
                                   (const rg_cb_result *)d->cbres.p, (const rg_cb_desc *)d->cbdesc.p, ncb,
-                                  (int32_t *)d->coeff.p, w, npix, p->style, termall ? (const int32_t *)d->cbseg.p : nullptr)))
+                                  (int32_t *)d->coeff.p, w, npix, p->style, termall ? (const int32_t *)d->cbseg.p : nullptr,
+                                  d->t1ws.p)))
         return rc;
     float step[97], inv[97];
     band_steps(p->levels, p->bitdepth, p->delta, step, inv);
@@ -1330,7 +1334,8 @@ extern "C" int rg_decoder_destroy(rg_decoder *d)
     if (!d) return RG_SUCCESS;
     hipSetDevice(d->ctx->device);
     hipStreamSynchronize(d->ctx->stream);
-    for (DevBuf *b : {&d->coeff, &d->out, &d->scratch, &d->cbdata, &d->cboffs, &d->cbres, &d->cbdesc, &d->u8, &d->cbseg})
+    for (DevBuf *b : {&d->coeff, &d->out, &d->scratch, &d->cbdata, &d->cboffs, &d->cbres, &d->cbdesc, &d->u8, &d->cbseg,
+                      &d->t1ws})
         b->release();
     d->staging.release(); d->map.release();
     delete d;

@@ -43,6 +43,7 @@
 #include "wave_emu.h"
 #include "../../roger_amd/csrc/t1_core.h"
 #include "../../roger_amd/csrc/t1_dec_core.h"
+#include "../../roger_amd/csrc/t1_dec_lane.h"
 
 using namespace rg_t1;
 
@@ -257,6 +258,69 @@ int main(int argc, char **argv)
     const std::string mode = argv[1], dir = argv[2];
     const int pitch = atoi(argv[3]);
     const long plane_stride = atol(argv[4]);
+    if (mode == "decode_lane") {
+        // the lane decoder (t1_dec_lane.h) per block with plain host state,
+        // then the magnitude fold of the epilogue
+        const int nplanes = atoi(argv[5]);
+        const int dstyle = argc == 7 ? atoi(argv[6]) : 0;
+        std::vector<char> draw = slurp(dir + "/desc.bin"), rraw = slurp(dir + "/res.bin");
+        std::vector<char> cw = slurp(dir + "/cw.bin"), oraw = slurp(dir + "/offs.bin");
+        const int ncb = (int)(draw.size() / sizeof(rg_cb_desc));
+        const rg_cb_desc *desc = reinterpret_cast<const rg_cb_desc *>(draw.data());
+        const rg_cb_result *res = reinterpret_cast<const rg_cb_result *>(rraw.data());
+        const int64_t *offs = reinterpret_cast<const int64_t *>(oraw.data());
+        std::vector<char> sraw;
+        if (dstyle & STYLE_TERMALL) sraw = slurp(dir + "/seglen.bin");
+        std::vector<int32_t> dq((size_t)nplanes * plane_stride, 0);
+        struct HostMem {
+            uint64_t s[64], n[64], v[64], r[64], b[64];
+            int cx[19];
+            int orient;
+            uint64_t &S(int c) { return s[c]; }
+            uint64_t &N(int c) { return n[c]; }
+            uint64_t &V(int c) { return v[c]; }
+            uint64_t &R(int c) { return r[c]; }
+            uint64_t &B(int c) { return b[c]; }
+            int ctx(int k) const { return cx[k]; }
+            void set_ctx(int k, int x) { cx[k] = x; }
+            uint32_t mqt(int k) const { return MQT[k]; }
+            uint32_t lut(int x) const { return lut_entry(orient, x); }
+        };
+        struct HostRd {
+            const uint8_t *p;
+            int n;
+            uint32_t get(int i) const
+            {
+                if (i < 0 || i >= n) { fprintf(stderr, "lane decoder read byte %d of %d\n", i, n); exit(3); }
+                return p[i];
+            }
+        };
+        for (int k = 0; k < ncb; ++k) {
+            HostMem mem;
+            mem.orient = desc[k].orient;
+            std::vector<uint64_t> planes(64 * 32, 0), negs(64, 0);
+            LaneJob j{};
+            j.len = res[k].len; j.npasses = res[k].npasses; j.nbp = res[k].nbp;
+            j.w = desc[k].w; j.h = desc[k].h; j.orient = desc[k].orient;
+            j.style = dstyle;
+            j.seglen = (dstyle & STYLE_TERMALL) ? reinterpret_cast<const int32_t *>(sraw.data()) + (size_t)k * MAXP
+                                                : nullptr;
+            j.planes = planes.data();
+            j.negs = negs.data();
+            HostRd rd{reinterpret_cast<const uint8_t *>(cw.data()) + offs[k], res[k].len};
+            t1_dec_lane(j, mem, rd);
+            int32_t *dst = dq.data() + (size_t)desc[k].comp * plane_stride + (size_t)desc[k].y0 * pitch + desc[k].x0;
+            for (int c = 0; c < desc[k].w; ++c)
+                for (int r = 0; r < desc[k].h; ++r) {
+                    uint32_t mag = 0;
+                    for (int p = j.nbp - 1; p >= 0; --p) mag = (mag << 1) | (uint32_t)((planes[p * 64 + c] >> r) & 1);
+                    const int v = (int)mag;
+                    dst[(size_t)r * pitch + c] = ((negs[c] >> r) & 1) ? -v : v;
+                }
+        }
+        spill(dir + "/dq.bin", dq.data(), sizeof(int32_t) * dq.size());
+        return 0;
+    }
     if (mode == "decode") {
         const int nplanes = atoi(argv[5]);
         const int dstyle = argc == 7 ? atoi(argv[6]) : 0;

@@ -12,7 +12,15 @@ from test_gpu_t1 import CASES, _block
 pytestmark = pytest.mark.gpu
 
 
-def test_t1_decode_blocks_vs_oracle(ctx):
+@pytest.fixture(params=["wave", "lane"])
+def t1dec(request, monkeypatch):
+    """Runs a test under each tier-1 decoder (the library reads RG_T1_DEC on
+    every launch)."""
+    monkeypatch.setenv("RG_T1_DEC", request.param)
+    return request.param
+
+
+def test_t1_decode_blocks_vs_oracle(ctx, t1dec):
     rng = np.random.default_rng(77)
     pw = 64
     blocks, descs, blobs, res, y = [], [], [], [], 0
@@ -32,7 +40,7 @@ def test_t1_decode_blocks_vs_oracle(ctx):
         np.testing.assert_array_equal(q[d["y0"]:d["y0"] + d["h"], :d["w"]], b)
 
 
-def test_t1_decode_truncated_passes(ctx):
+def test_t1_decode_truncated_passes(ctx, t1dec):
     """Decoding only the first passes gives what the oracle decoder gives."""
     rng = np.random.default_rng(5)
     b = rng.laplace(0, 40, size=(64, 64)).astype(np.int32)
@@ -45,7 +53,7 @@ def test_t1_decode_truncated_passes(ctx):
 
 
 @pytest.mark.parametrize("style", [O.STYLE_PARALLEL, O.STYLE_PARALLEL_BYPASS])
-def test_t1_decode_style_blocks_vs_oracle(ctx, style):
+def test_t1_decode_style_blocks_vs_oracle(ctx, style, t1dec):
     """rg_t1_decode_style on device-resident codewords of the parallel styles
     (per-pass segments, raw passes under BYPASS): every block of every shape
     and orientation decodes to the oracle's input, and decoding only a
@@ -88,7 +96,7 @@ def test_t1_decode_style_blocks_vs_oracle(ctx, style):
 @pytest.mark.parametrize("w,h,nc,rev,levels,cb", [
     (512, 512, 1, True, 5, 64), (300, 257, 3, True, 4, 64), (257, 190, 3, False, 6, 64), (128, 96, 1, False, 3, 32),
 ])
-def test_full_decode(ctx, w, h, nc, rev, levels, cb):
+def test_full_decode(ctx, w, h, nc, rev, levels, cb, t1dec):
     img = synth.image(w, h, nc, 13)
     enc = R.Encoder(ctx)
     enc.encode_u8(img, levels=levels, reversible=rev, cbw=cb, cbh=cb)

@@ -215,7 +215,8 @@ def test_parallel_style_dense_blocks(emu, tmp_path, style, mode):
 
 
 # ---------------- decoder core (t1_dec_core.h) ----------------
-def _run_decode(emu, tmp_path, desc, res_rows, blobs, pitch, plane_stride, nplanes, style, passlen=None):
+def _run_decode(emu, tmp_path, desc, res_rows, blobs, pitch, plane_stride, nplanes, style, passlen=None,
+                mode="decode"):
     d = str(tmp_path)
     np.ascontiguousarray(desc, CB_DESC_DTYPE).tofile(os.path.join(d, "desc.bin"))
     res = np.zeros(len(desc), CB_RES_DTYPE)
@@ -228,15 +229,19 @@ def _run_decode(emu, tmp_path, desc, res_rows, blobs, pitch, plane_stride, nplan
         f.write(b"".join(blobs))
     if passlen is not None:
         np.ascontiguousarray(passlen, np.int32).tofile(os.path.join(d, "seglen.bin"))
-    r = subprocess.run([emu, "decode", d, str(pitch), str(plane_stride), str(nplanes), str(style)],
+    r = subprocess.run([emu, mode, d, str(pitch), str(plane_stride), str(nplanes), str(style)],
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     return np.fromfile(os.path.join(d, "dq.bin"), np.int32).reshape(nplanes, -1)
 
 
 DEC_STYLES = [0, O.STYLE_PARALLEL, O.STYLE_PARALLEL_BYPASS]
+# decode: the wavefront decoder core (t1_dec_core.h) under the emulator;
+# decode_lane: the lane decoder (t1_dec_lane.h) with plain host state
+DEC_MODES = ["decode", "decode_lane"]
 
 
+@pytest.mark.parametrize("mode", DEC_MODES)
 @pytest.mark.parametrize("style", DEC_STYLES)
 @pytest.mark.parametrize("w,h,nc,rev,levels,seed,cb", [
     (97, 61, 1, False, 3, 52, 64),
@@ -244,7 +249,7 @@ DEC_STYLES = [0, O.STYLE_PARALLEL, O.STYLE_PARALLEL_BYPASS]
     (64, 64, 3, False, 2, 54, 32),
     (129, 57, 1, True, 4, 55, 64),
 ])
-def test_decoder_core_inverts_oracle(emu, tmp_path, style, w, h, nc, rev, levels, seed, cb):
+def test_decoder_core_inverts_oracle(emu, tmp_path, mode, style, w, h, nc, rev, levels, seed, cb):
     """The decoder kernel's core on the oracle's codewords of whole images, in
     every code-block style: exactly the quantisation indices."""
     img = synth.image(w, h, nc, seed)
@@ -257,12 +262,13 @@ def test_decoder_core_inverts_oracle(emu, tmp_path, style, w, h, nc, rev, levels
         for d in one:
             desc[k] = (d["x0"], d["y0"], d["w"], d["h"], c, d["orient"], d["band"], d["res"])
             k += 1
-    dq = _run_decode(emu, tmp_path, desc, rs, data, w, w * h, nc, style, passlen if style else None)
+    dq = _run_decode(emu, tmp_path, desc, rs, data, w, w * h, nc, style, passlen if style else None, mode)
     np.testing.assert_array_equal(dq.reshape(q.shape), q)
 
 
+@pytest.mark.parametrize("mode", DEC_MODES)
 @pytest.mark.parametrize("style", DEC_STYLES)
-def test_decoder_core_dense_blocks(emu, tmp_path, style):
+def test_decoder_core_dense_blocks(emu, tmp_path, mode, style):
     """Dense and sparse 64x64 blocks (many bit-planes, run-length paths,
     skewed LPS states, raw segments full of 0xFF bytes) and a ragged 64x7
     block, every orientation: decoded back exactly."""
@@ -293,7 +299,7 @@ def test_decoder_core_dense_blocks(emu, tmp_path, style):
         if style:
             pls[i, :r[1]] = r[3]
     dq = _run_decode(emu, tmp_path, desc, np.array(rows, np.int32), blobs, plane.shape[1], plane.size, 1, style,
-                     pls if style else None)
+                     pls if style else None, mode)
     dq = dq.reshape(plane.shape)
     for i, b in enumerate(blocks):
         np.testing.assert_array_equal(dq[:hs[i], 64 * i:64 * (i + 1)], b[:hs[i]], err_msg="block %d" % i)
