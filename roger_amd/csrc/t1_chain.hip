@@ -1,7 +1,7 @@
 // Parallel code-block styles (RESET | TERMALL, optionally | BYPASS), second
 // half of the split tier-1: every (code-block, pass) chain is listed longest
-// first by a device counting sort and coded one per lane, 64 per wavefront
-// (t1_chain_lane in t1_core.h).  Replaces, with t1_encode.hip, the
+// first by a device counting sort and coded one per lane, 64 per two-wavefront
+// workgroup (t1_chain_lane in t1_core.h).  Replaces, with t1_encode.hip, the
 // reference's bit-plane coder launch src/OCLBPC.cpp:38-51.
 #include "rg_internal.h"
 #include "wave_hip.h"
@@ -116,18 +116,22 @@ __global__ void __launch_bounds__(256) chain_scatter_kernel(J job, int nblocks, 
         if (key[t] >= 0) ws[CH_LIST + h[key[t]] + local[t]] = (int)id[t];
 }
 
-// Codes the sorted lists: MQ wavefronts first (longest chains first), then
-// raw ones; workgroups past the end of the lists exit.
+// Codes the sorted lists with two-wavefront workgroups: MQ workgroups first
+// (longest chains first), 64 chains each, every chain coded by lane l of both
+// wavefronts (interval side and code side, ring_chain_wg); then raw ones, 64
+// chains per wavefront; workgroups and raw wavefronts past the end of the
+// lists exit (a raw wavefront has no barrier to hold the other up).
 template <class J>
-__global__ void __launch_bounds__(64) chain_code_kernel(J job, const int *ws)
+__global__ void __launch_bounds__(128) chain_code_kernel(J job, const int *ws)
 {
     __shared__ uint32_t tab[48];
-    __shared__ __attribute__((aligned(16))) uint8_t ring[MQ_LDS_BYTES];
+    __shared__ __attribute__((aligned(16))) uint8_t ring[MQ2_LDS_BYTES];
+    static_assert(MQ2_LDS_BYTES >= 2 * RING_BYTES, "one ring per raw wavefront");
     const int nmq = ws[CH_COUNTS], ntot = nmq + ws[CH_COUNTS + 1];
     const int mqw = (nmq + 63) >> 6;
     const int w = (int)blockIdx.x;
     const bool raw = w >= mqw;
-    const int base = raw ? nmq + (w - mqw) * 64 : w * 64;
+    const int base = raw ? nmq + (w - mqw) * 128 + 64 * wv_wave() : w * 64;
     const int end = raw ? ntot : nmq;
     if (base >= end) return;
     // The kernel lasts as long as its longest chain, whose wavefront shares a
@@ -143,7 +147,8 @@ __global__ void __launch_bounds__(64) chain_code_kernel(J job, const int *ws)
         else if (len >= 1024) __builtin_amdgcn_s_setprio(2);
         else if (len >= 256) __builtin_amdgcn_s_setprio(1);
     }
-    t1_chain_lane(tab, ring, raw, ws + CH_LIST, base, end, job);
+    if (raw) t1_chain_lane<J, false>(tab, ring + wv_wave() * RING_BYTES, true, ws + CH_LIST, base, end, job);
+    else t1_chain_lane<J, true>(tab, ring, false, ws + CH_LIST, base, end, job);
 }
 
 // Chain lists of the passes [qlo, qhi) of every block (counting sort).
@@ -168,7 +173,7 @@ int launch_lists(hipStream_t s, const J &job, int nblocks, int style, int *ws, i
 template <class J>
 int launch_code(hipStream_t s, const J &job, int nblocks, const int *ws)
 {
-    chain_code_kernel<<<(unsigned)(((long)nblocks * (MAXP - 2) + 63) / 64 + 2), 64, 0, s>>>(job, ws);
+    chain_code_kernel<<<(unsigned)(((long)nblocks * (MAXP - 2) + 63) / 64 + 2), 128, 0, s>>>(job, ws);
     RG_LAUNCH_CHECK();
     return RG_SUCCESS;
 }

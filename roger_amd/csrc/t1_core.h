@@ -663,26 +663,27 @@ RG_DEV void ring_byteout(RingMQ &m, bool need, uint8_t *rb)
     m.CT = need ? (big ? 7 : 8) : m.CT;
 }
 
-// ENCODE's interval update (C.2.2-C.2.5) of decision d in a context with
-// entry e; A is renormalised (shifted) here and the returned shift count is
-// C's share for ring_renorm.  *enew: the context's entry after the symbol
-// (tnew = the table entry of the state it moves to if it renormalises).
-RG_DEV int ring_code_e(RingMQ &m, uint32_t d, uint32_t e, uint32_t tnew, uint32_t lps_nmps, bool renorm,
-                       uint32_t &enew)
+// Interval side of ENCODE (C.2.2-C.2.5) for decision d in a context with
+// entry e (tnew: the table entry of the state it moves to if it
+// renormalises): A is updated and renormalised, *enew is the context's entry
+// after the symbol.  Returns the symbol's record for the code side: the
+// amount added to C (Qe when the upper interval is kept, else 0) | the
+// renormalisation shift << 16.
+RG_DEV uint32_t mq_interval(uint32_t &A, uint32_t d, uint32_t e, uint32_t tnew, uint32_t lps_nmps, bool renorm,
+                            uint32_t &enew)
 {
     const uint32_t qe = e & 0xFFFFu;
-    const uint32_t a = m.A - qe;
+    const uint32_t a = A - qe;
     const bool lps = d != (e >> 31);
     // C.2.4 / C.2.5: the interval kept is the upper one (C += Qe) for an MPS
     // unless A < Qe (conditional exchange), and the reverse for an LPS
     const bool small = a < qe;
     const bool upper = !renorm || (lps ? small : !small);
-    m.C += upper ? qe : 0u;
-    m.A = upper ? a : qe;
+    A = upper ? a : qe;
     enew = renorm ? (tnew | (lps_nmps << 31)) : e;
-    const int n = wv_clz(m.A) - 16;
-    m.A <<= n;
-    return n;
+    const int n = wv_clz(A) - 16;
+    A <<= n;
+    return (upper ? qe : 0u) | ((uint32_t)n << 16);
 }
 
 // The next state of a context with entry e after decision d: its table index
@@ -714,6 +715,14 @@ RG_DEV void ring_renorm(RingMQ &m, int n, uint8_t *rb)
     m.C <<= n; m.CT -= n;
 }
 
+// Code side of ENCODE for one symbol's record (mq_interval): C takes the
+// addition, then RENORME's shifts with their BYTEOUTs.
+RG_DEV void mq_code(RingMQ &m, uint32_t rec, uint8_t *rb)
+{
+    m.C += rec & 0xFFFFu;
+    ring_renorm(m, (int)(rec >> 16), rb);
+}
+
 // Moves the ring's completed 16-byte groups [fl, bp) to out (bytes at or
 // beyond cap only set the overflow flag).
 RG_DEV void ring_flush(RingMQ &m, const uint8_t *rb, uint8_t *out, int cap)
@@ -731,19 +740,92 @@ RG_DEV void ring_flush(RingMQ &m, const uint8_t *rb, uint8_t *out, int cap)
     }
 }
 
+// ---- the coder as two chains ----
+//
+// ENCODE is two serial chains.  The interval side (context entries, the A
+// register, the interval kept and the renormalisation shift) never reads the
+// code register, and the code side (C, CT, BYTEOUT and the output bytes)
+// needs from each symbol only its record from mq_interval.  ring_chain runs
+// both sides in one wavefront (ROLE_BOTH: the records of a 16-symbol chunk in
+// registers), or one side per wavefront of a two-wavefront workgroup, with the
+// same chain in lane l of both (ROLE_INTERVAL in wavefront 0, ROLE_CODE in
+// wavefront 1): each wavefront then runs only its side's instructions, one
+// chunk apart, and a chain's time per symbol drops from the sum of the two
+// sides to the longer one.  The records of chunk i pass through workgroup
+// memory (buffer i & 1, 16 x 64 words); one barrier per chunk orders the
+// writes of chunk i before its reads and the reads of chunk i - 1 before the
+// buffer's reuse.  The final A follows the last chunk the same way.
+constexpr int ROLE_BOTH = 0, ROLE_INTERVAL = 1, ROLE_CODE = 2;
+constexpr int MQ_HAND_WORDS = 2 * 16 * 64 + 64;
+// workgroup memory of the two-wavefront form: MQ_LDS_BYTES, then the records
+constexpr int MQ2_LDS_BYTES = MQ_LDS_BYTES + MQ_HAND_WORDS * 4;
+
+// Interval side of the chunk w4 (16 symbols from symbol `base`) into rec:
+// the pipelined form when the whole chunk lies in [s0, s1) (`full`),
+// otherwise symbol by symbol with records 0 (no change on the code side)
+// outside the range.
+RG_DEV void mq_interval_chunk(const uint32_t *tab, uint32_t *ce, uint32_t &A, const uint32_t (&w4)[4], bool full,
+                              bool any, int base, int s0, int s1, uint32_t (&rec)[16])
+{
+    if (full) {
+        uint32_t sy = w4[0] & 0xFFu;
+        uint32_t e = ce[64 * (sy >> 1)];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const uint32_t cx = sy >> 1, d = sy & 1u;
+            uint32_t nmps;
+            bool renorm;
+            const uint32_t nidx = ring_next(d, e, A, nmps, renorm);
+            const uint32_t tnew = tab[nidx];
+            uint32_t sy2 = 0, e2 = 0;
+            if (k < 15) {   // the next symbol's context, read before this one's store
+                sy2 = (w4[(k + 1) >> 2] >> (8 * ((k + 1) & 3))) & 0xFFu;
+                e2 = ce[64 * (sy2 >> 1)];
+            }
+            wv_sched_fence();
+            uint32_t enew;
+            rec[k] = mq_interval(A, d, e, tnew, nmps, renorm, enew);
+            ce[64 * cx] = enew;
+            if (k < 15) e = (sy2 >> 1) == cx ? enew : e2;
+            sy = sy2;
+        }
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        rec[k] = 0;
+        if (any && base + k >= s0 && base + k < s1) {
+            const uint32_t sy = (w4[k >> 2] >> (8 * (k & 3))) & 0xFFu;
+            const uint32_t cx = sy >> 1, d = sy & 1u;
+            const uint32_t e = ce[64 * cx];
+            uint32_t nmps;
+            bool renorm;
+            const uint32_t nidx = ring_next(d, e, A, nmps, renorm);
+            uint32_t enew;
+            rec[k] = mq_interval(A, d, e, tab[nidx], nmps, renorm, enew);
+            ce[64 * cx] = enew;
+        }
+    }
+}
+
 // Codes symbols [s0, s1) of the 16-byte-aligned stream `sym` as one MQ
 // codeword (INITENC, ENCODE per symbol, FLUSH C.2.9) into `out` (16-byte
-// aligned, capacity cap), from the initial context states.  Every lane of the
-// wavefront calls it; idle lanes pass active = false.  Returns the codeword
-// length; ovf is set if it did not fit (out then holds its first complete
-// 16-byte groups).  `tab`: the probability table in workgroup memory; `ring`:
-// MQ_LDS_BYTES of workgroup memory (4-byte aligned).
+// aligned, capacity cap), from the initial context states, in role ROLE (the
+// two-wavefront roles are called by both wavefronts, each with its role).
+// Every lane of a wavefront calls it; idle lanes pass active = false.
+// Returns the codeword length (0 in ROLE_INTERVAL); ovf is set if it did not
+// fit (out then holds its first complete 16-byte groups).  `tab`: the
+// probability table in workgroup memory; `ring`: MQ_LDS_BYTES (ROLE_BOTH) or
+// MQ2_LDS_BYTES of workgroup memory (4-byte aligned).
+template <int ROLE>
 RG_DEV int ring_chain(const uint32_t *tab, uint8_t *ring, bool active, const uint8_t *sym, int s0, int s1,
                       uint8_t *out, int cap, bool &ovf)
 {
-    uint8_t *rb = ring + wv_lane() * RING_STRIDE;
-    uint32_t *ce = reinterpret_cast<uint32_t *>(ring + RING_BYTES) + wv_lane();
-    ce_init(ce);
+    constexpr bool IV = ROLE != ROLE_CODE, CD = ROLE != ROLE_INTERVAL;
+    const int lane = wv_lane();
+    uint8_t *rb = ring + lane * RING_STRIDE;
+    uint32_t *ce = reinterpret_cast<uint32_t *>(ring + RING_BYTES) + lane;
+    uint32_t *hand = reinterpret_cast<uint32_t *>(ring + MQ_LDS_BYTES) + lane;
     const int c0 = active ? s0 >> 4 : 0, c1 = active ? (s1 + 15) >> 4 : 0;
     const int nch = c1 - c0;
     int nmax = nch;
@@ -751,59 +833,54 @@ RG_DEV int ring_chain(const uint32_t *tab, uint8_t *ring, bool active, const uin
         const int t = wv_shfl_xor(nmax, o);
         nmax = nmax > t ? nmax : t;
     }
+    uint32_t A = 0x8000;
     RingMQ m;
     m.A = 0x8000; m.C = 0; m.B = 0; m.CT = 12; m.bp = -1; m.fl = 0; m.ovf = 0;
     const int last = nch > 0 ? c1 - 1 : c0;
-    u32x4 nxt = wv_load16(sym + 16 * c0);
+    u32x4 nxt = mk_u32x4(0, 0, 0, 0);
+    if (IV) {
+        ce_init(ce);
+        nxt = wv_load16(sym + 16 * c0);
+    }
     for (int i = 0; i < nmax; ++i) {
-        const u32x4 cur = nxt;
-        const int cn = c0 + i + 1;
-        nxt = wv_load16(sym + 16 * (cn < last ? cn : last));
-        const uint32_t w4[4] = {cur.x, cur.y, cur.z, cur.w};
-        const int base = 16 * (c0 + i);
-        const bool full = i < nch && base >= s0 && base + 16 <= s1;
-        if (full) {
-            uint32_t sy = w4[0] & 0xFFu;
-            uint32_t e = ce[64 * (sy >> 1)];
-#pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                const uint32_t cx = sy >> 1, d = sy & 1u;
-                uint32_t nmps;
-                bool renorm;
-                const uint32_t nidx = ring_next(d, e, m.A, nmps, renorm);
-                const uint32_t tnew = tab[nidx];
-                uint32_t sy2 = 0, e2 = 0;
-                if (k < 15) {   // the next symbol's context, read before this one's store
-                    sy2 = (w4[(k + 1) >> 2] >> (8 * ((k + 1) & 3))) & 0xFFu;
-                    e2 = ce[64 * (sy2 >> 1)];
-                }
-                wv_sched_fence();
-                uint32_t enew;
-                const int n = ring_code_e(m, d, e, tnew, nmps, renorm, enew);
-                ce[64 * cx] = enew;
-                if (k < 15) e = (sy2 >> 1) == cx ? enew : e2;
-                ring_renorm(m, n, rb);
-                sy = sy2;
-            }
-        } else if (i < nch) {
-#pragma unroll
-            for (int k = 0; k < 16; ++k)
-                if (base + k >= s0 && base + k < s1) {
-                    const uint32_t sy = (w4[k >> 2] >> (8 * (k & 3))) & 0xFFu;
-                    const uint32_t cx = sy >> 1, d = sy & 1u;
-                    const uint32_t e = ce[64 * cx];
-                    uint32_t nmps;
-                    bool renorm;
-                    const uint32_t nidx = ring_next(d, e, m.A, nmps, renorm);
-                    uint32_t enew;
-                    const int n = ring_code_e(m, d, e, tab[nidx], nmps, renorm, enew);
-                    ce[64 * cx] = enew;
-                    ring_renorm(m, n, rb);
-                }
+        uint32_t rec[16];
+        if (IV) {
+            const u32x4 cur = nxt;
+            const int cn = c0 + i + 1;
+            nxt = wv_load16(sym + 16 * (cn < last ? cn : last));
+            const uint32_t w4[4] = {cur.x, cur.y, cur.z, cur.w};
+            const int base = 16 * (c0 + i);
+            const bool full = i < nch && base >= s0 && base + 16 <= s1;
+            mq_interval_chunk(tab, ce, A, w4, full, i < nch, base, s0, s1, rec);
         }
-        ring_flush(m, rb, out, cap);
+        if (ROLE == ROLE_INTERVAL) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) hand[((i & 1) * 16 + k) * 64] = rec[k];
+            wg_sync();
+        }
+        if (ROLE == ROLE_CODE) {
+            wg_sync();
+#pragma unroll
+            for (int k = 0; k < 16; ++k) rec[k] = hand[((i & 1) * 16 + k) * 64];
+        }
+        if (CD) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) mq_code(m, rec[k], rb);
+            ring_flush(m, rb, out, cap);
+        }
+    }
+    if (ROLE == ROLE_INTERVAL) {
+        hand[2 * 16 * 64] = A;
+        wg_sync();
+        ovf = false;
+        return 0;
+    }
+    if (ROLE == ROLE_CODE) {
+        wg_sync();
+        A = hand[2 * 16 * 64];
     }
     // FLUSH (C.2.9); idle lanes run it on their untouched coder, unused
+    m.A = A;
     const uint32_t tmp = m.C + m.A;
     m.C |= 0xFFFF;
     if (m.C >= tmp) m.C -= 0x8000;
@@ -824,18 +901,30 @@ RG_DEV int ring_chain(const uint32_t *tab, uint8_t *ring, bool active, const uin
     return len;
 }
 
+// The coder in the workgroup's form: ROLE_BOTH with one wavefront (SPLIT
+// false), else the role of this wavefront of two.
+template <bool SPLIT>
+RG_DEV int ring_chain_wg(const uint32_t *tab, uint8_t *ring, bool active, const uint8_t *sym, int s0, int s1,
+                         uint8_t *out, int cap, bool &ovf)
+{
+    if (!SPLIT) return ring_chain<ROLE_BOTH>(tab, ring, active, sym, s0, s1, out, cap, ovf);
+    if (wv_wave() == 0) return ring_chain<ROLE_INTERVAL>(tab, ring, active, sym, s0, s1, out, cap, ovf);
+    return ring_chain<ROLE_CODE>(tab, ring, active, sym, s0, s1, out, cap, ovf);
+}
 
 // MQ-codes one block per lane from the streams the context-modelling pass
 // wrote.  Block `i` (i >= njobs: idle lane) is either jobs[i] (QUEUE) or
 // entry i of the flat arrays.  `tab` is 48 words of workgroup-shared memory,
-// `ring` MQ_LDS_BYTES of it.
-template <bool QUEUE>
+// `ring` MQ_LDS_BYTES (MQ2_LDS_BYTES with SPLIT: two wavefronts, each lane's
+// block coded by lane l of both, ring_chain_wg) of it; the code side writes
+// the results.
+template <bool QUEUE, bool SPLIT = false>
 RG_DEV void t1_mq_lanes(uint32_t *tab, uint8_t *ring, const T1Job *jobs, int i, int njobs, const uint8_t *syms,
                         int symcap, const i32x4 *cm, uint8_t *slots, int slot, rg_cb_result *res)
 {
     const int lane = wv_lane();
-    if (lane < 47) tab[lane] = MQT[lane];
-    wv_sync();
+    if (wv_wave() == 0 && lane < 47) tab[lane] = MQT[lane];
+    wg_sync();
     const bool active = i < njobs;
     i32x4 c = mk_i32x4(0, 0, 0, 0);
     // idle lanes read block 0's stream (always allocated) and store nothing
@@ -853,8 +942,8 @@ RG_DEV void t1_mq_lanes(uint32_t *tab, uint8_t *ring, const T1Job *jobs, int i, 
     }
     const bool coded = active && c.z > 0;   // a block without bit-planes has no codeword
     bool ovf;
-    const int len = ring_chain(tab, ring, coded, src, 0, c.x, out, cap, ovf);
-    if (!active) return;
+    const int len = ring_chain_wg<SPLIT>(tab, ring, coded, src, 0, c.x, out, cap, ovf);
+    if (!active || (SPLIT && wv_wave() == 0)) return;
     if (!coded) {
         *dst = rg_cb_result{0, 0, 0, 0};
         return;
@@ -870,15 +959,18 @@ RG_DEV void t1_mq_lanes(uint32_t *tab, uint8_t *ring, const T1Job *jobs, int i, 
 // [poff[p], poff[p+1]) of its stream into the block's segment arena, and
 // records the segment length; t1_pack_block then concatenates the segments.
 // `len_out` < 0 marks a segment that did not fit its arena region.
+// With SPLIT the workgroup's two wavefronts code the chain together
+// (ring_chain_wg; `ring` MQ2_LDS_BYTES) and the code side writes the length.
+template <bool SPLIT = false>
 RG_DEV void t1_mq_pass_lane(uint32_t *tab, uint8_t *ring, bool active, const uint8_t *sym, int s0, int s1,
                             uint8_t *seg, int cap, int32_t *len_out)
 {
     const int lane = wv_lane();
-    if (lane < 47) tab[lane] = MQT[lane];
-    wv_sync();
+    if (wv_wave() == 0 && lane < 47) tab[lane] = MQT[lane];
+    wg_sync();
     bool ovf;
-    const int len = ring_chain(tab, ring, active, sym, s0, s1, seg, cap, ovf);
-    if (active) *len_out = ovf ? -1 : len;
+    const int len = ring_chain_wg<SPLIT>(tab, ring, active, sym, s0, s1, seg, cap, ovf);
+    if (active && !(SPLIT && wv_wave() == 0)) *len_out = ovf ? -1 : len;
 }
 
 // Raw (bypass) pass of the parallel style with BYPASS (T.800 D.6): the
@@ -1072,8 +1164,9 @@ RG_DEV int chain_key(const i32x4 &c, const int32_t *poff, int q, int style, int 
 
 // Chain coder of the sorted form for list entry j of a wavefront whose
 // entries are [base, end) (end <= base: idle lane); `raw` is uniform.
-// J(b) returns block b's job record.
-template <class J>
+// J(b) returns block b's job record.  SPLIT: MQ chains coded by the two
+// wavefronts of the workgroup together (t1_mq_pass_lane).
+template <class J, bool SPLIT = false>
 RG_DEV void t1_chain_lane(uint32_t *tab, uint8_t *ring, bool raw, const int32_t *list, int base, int end,
                           const J &job)
 {
@@ -1087,7 +1180,7 @@ RG_DEV void t1_chain_lane(uint32_t *tab, uint8_t *ring, bool raw, const int32_t 
     if (raw)
         t1_raw_pass_lane(ring, active, jb.sym, s0, s1, jb.seg + at, cap, jb.seglen + p);
     else
-        t1_mq_pass_lane(tab, ring, active, jb.sym, s0, s1, jb.seg + at, cap, jb.seglen + p);
+        t1_mq_pass_lane<SPLIT>(tab, ring, active, jb.sym, s0, s1, jb.seg + at, cap, jb.seglen + p);
 }
 
 }  // namespace rg_t1

@@ -4,8 +4,10 @@
 // (tests/emu/wave_emu.h) that checks the kernels' logic on the CPU.
 //
 // Contract shared by both mappings: one wavefront of 64 lanes per workgroup
-// (the tier-1 kernels launch 64 threads), and every cross-lane call is made
-// by all 64 lanes at the same program point (wave-uniform control flow).
+// (the tier-1 kernels launch 64 threads; the MQ coders' two-wavefront form
+// launches 128, wv_wave() telling the two apart), and every cross-lane call
+// is made by all 64 lanes of a wavefront at the same program point
+// (wave-uniform control flow).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -17,8 +19,13 @@ using i32x4 = int4;
 using u32x4 = uint4;
 
 RG_DEV i32x4 mk_i32x4(int x, int y, int z, int w) { return make_int4(x, y, z, w); }
-RG_DEV int wv_lane() { return (int)threadIdx.x; }
+RG_DEV u32x4 mk_u32x4(uint32_t x, uint32_t y, uint32_t z, uint32_t w) { return make_uint4(x, y, z, w); }
+RG_DEV int wv_lane() { return (int)(threadIdx.x & 63); }
+// wavefront of the workgroup (0 in the one-wavefront kernels)
+RG_DEV int wv_wave() { return (int)(threadIdx.x >> 6); }
 RG_DEV void wv_sync() { __syncthreads(); }
+// barrier of the whole workgroup (every wavefront must reach it)
+RG_DEV void wg_sync() { __syncthreads(); }
 RG_DEV int wv_clz(uint32_t x) { return __clz((int)x); }
 RG_DEV bool wv_any(bool p) { return __any(p); }
 RG_DEV uint64_t wv_ballot(bool p) { return __ballot(p); }

@@ -28,6 +28,7 @@ struct alignas(16) u32x4 {
     uint32_t x, y, z, w;
 };
 static inline i32x4 mk_i32x4(int x, int y, int z, int w) { return i32x4{x, y, z, w}; }
+static inline u32x4 mk_u32x4(uint32_t x, uint32_t y, uint32_t z, uint32_t w) { return u32x4{x, y, z, w}; }
 static inline u32x4 wv_load16(const uint8_t *p)
 {
     if (((uintptr_t)p & 15) != 0) { fprintf(stderr, "wave emulator: misaligned 16-byte load\n"); exit(3); }
@@ -50,6 +51,9 @@ const uint64_t *exchange(uint64_t v, int check_uniform);
 
 static inline int wv_lane() { return emu::lane(); }
 static inline void wv_sync() { emu::exchange(0, 0); }
+// one emulated wavefront: the workgroup is that wavefront
+static inline int wv_wave() { return 0; }
+static inline void wg_sync() { emu::exchange(0, 0); }
 static inline int wv_clz(uint32_t x) { return x ? __builtin_clz(x) : 32; }
 static inline bool wv_any(bool p)
 {
