@@ -209,10 +209,11 @@ def main():
             fn()
         return ctx.timer_stop() / reps
 
-    t_colour = timed(lambda: R.colour_fwd(ctx, d_src, W, H, NC, RG_COLOUR_ICT, 1, d_pl))
+    # warm: back-to-back launches on the same buffers (cold figures below)
+    t_colour_warm = timed(lambda: R.colour_fwd(ctx, d_src, W, H, NC, RG_COLOUR_ICT, 1, d_pl))
     inv1 = R.band_steps(1, DELTA)[1]
-    t_dwt_l1 = timed(lambda: R.dwt_fwd(ctx, RG_FILTER_97, d_pl, W, H, NC, 1, inv1, 0, d_q, d_s))
-    t_dwt = timed(lambda: R.dwt_fwd(ctx, RG_FILTER_97, d_pl, W, H, NC, LEVELS, inv, 0, d_q, d_s))
+    t_dwt_l1_warm = timed(lambda: R.dwt_fwd(ctx, RG_FILTER_97, d_pl, W, H, NC, 1, inv1, 0, d_q, d_s))
+    R.dwt_fwd(ctx, RG_FILTER_97, d_pl, W, H, NC, LEVELS, inv, 0, d_q, d_s)
     desc = R.codeblocks(W, H, LEVELS, CB, CB, NC)
     d_desc = ctx.upload(desc)
     slot = CB * CB * 3
@@ -276,10 +277,9 @@ def main():
     R.dwt_fwd(ctx, RG_FILTER_97, d_pl, W, H, NC, LEVELS, inv, 0, d_q, d_s)
     d_rec0 = ctx.alloc(NC * W * H * 4)
     d_u80 = ctx.alloc(NC * W * H)
-    t_idwt_l1 = timed(lambda: R.dwt_inv(ctx, RG_FILTER_97, d_q, W, H, NC, 1, R.band_steps(1, DELTA)[0], d_rec0,
-                                        d_s))
-    t_idwt = timed(lambda: R.dwt_inv(ctx, RG_FILTER_97, d_q, W, H, NC, LEVELS, step, d_rec0, d_s))
-    t_cinv = timed(lambda: R.colour_inv(ctx, d_rec0, W, H, NC, RG_COLOUR_ICT, 1, d_u80))
+    t_idwt_l1_warm = timed(lambda: R.dwt_inv(ctx, RG_FILTER_97, d_q, W, H, NC, 1, R.band_steps(1, DELTA)[0],
+                                             d_rec0, d_s))
+    t_cinv_warm = timed(lambda: R.colour_inv(ctx, d_rec0, W, H, NC, RG_COLOUR_ICT, 1, d_u80))
     d_rec0.free()
     d_u80.free()
 
@@ -341,10 +341,50 @@ def main():
     # measured device-to-device copy ceiling (read + write of the same bytes as the level-1 DWT)
     cp_n = 4 * W * H * NC
     d_cp = ctx.alloc(cp_n)
-    t_copy = timed(lambda: lib().rg_stream_copy(ctx.handle, ctypes.c_void_p(d_cp.ptr), ctypes.c_void_p(d_pl.ptr),
-                                                  cp_n))
-    copy_gbs = 2.0 * cp_n / (t_copy * 1e-3) / 1e9
+    t_copy_warm = timed(lambda: lib().rg_stream_copy(ctx.handle, ctypes.c_void_p(d_cp.ptr),
+                                                       ctypes.c_void_p(d_pl.ptr), cp_n))
     d_cp.free()
+
+    # ---- HBM-bound stages with cold caches ----
+    # The device's 256 MB last-level cache keeps much of a 200 MB plane set
+    # between back-to-back launches on the same buffers, which flatters an
+    # HBM-bound kernel (level-1 9/7: about 68 us warm against 84 us cold).  In
+    # the encode each stage reads what the one before wrote, so the figures
+    # below rotate over ROT buffer sets: every launch reads data that no launch
+    # of the last ROT - 1 touched (ROT x 0.5 GB, beyond that cache).
+    ROT = 4
+    sets = []
+    for _k in range(ROT):
+        src_k = ctx.alloc(NC * W * H)
+        check_rc(lib().rg_stream_copy(ctx.handle, ctypes.c_void_p(src_k.ptr), ctypes.c_void_p(d_src.ptr), NC * W * H))
+        sets.append({"src": src_k, "pl": ctx.alloc(NC * W * H * 4), "q": ctx.alloc(NC * W * H * 4),
+                     "s": ctx.alloc(R.scratch_bytes(W, H, NC, RG_FILTER_97))})
+    n_rot = ROT * max(5, (reps + ROT - 1) // ROT)
+
+    def timed_rot(fn):
+        for k in range(ROT):
+            fn(sets[k])
+        ctx.timer_start()
+        for i in range(n_rot):
+            fn(sets[i % ROT])
+        return ctx.timer_stop() / n_rot
+
+    t_colour = timed_rot(lambda b: R.colour_fwd(ctx, b["src"], W, H, NC, RG_COLOUR_ICT, 1, b["pl"]))
+    t_dwt_l1 = timed_rot(lambda b: R.dwt_fwd(ctx, RG_FILTER_97, b["pl"], W, H, NC, 1, inv1, 0, b["q"], b["s"]))
+    t_dwt = timed_rot(lambda b: R.dwt_fwd(ctx, RG_FILTER_97, b["pl"], W, H, NC, LEVELS, inv, 0, b["q"], b["s"]))
+    t_copy = timed_rot(lambda b: lib().rg_stream_copy(ctx.handle, ctypes.c_void_p(b["q"].ptr),
+                                                      ctypes.c_void_p(b["pl"].ptr), cp_n))
+    for b in sets:   # indices of every set for the inverse
+        R.dwt_fwd(ctx, RG_FILTER_97, b["pl"], W, H, NC, LEVELS, inv, 0, b["q"], b["s"])
+    step1 = R.band_steps(1, DELTA)[0]
+    t_idwt_l1 = timed_rot(lambda b: R.dwt_inv(ctx, RG_FILTER_97, b["q"], W, H, NC, 1, step1, b["pl"], b["s"]))
+    t_idwt = timed_rot(lambda b: R.dwt_inv(ctx, RG_FILTER_97, b["q"], W, H, NC, LEVELS, step, b["pl"], b["s"]))
+    t_cinv = timed_rot(lambda b: R.colour_inv(ctx, b["pl"], W, H, NC, RG_COLOUR_ICT, 1, b["src"]))
+    ctx.sync()
+    for b in sets:
+        for v in b.values():
+            v.free()
+    copy_gbs = 2.0 * cp_n / (t_copy * 1e-3) / 1e9
     l1_bytes = 8.0 * W * H * NC
     achieved = l1_bytes / (t_dwt_l1 * 1e-3) / 1e9
     # HBM bytes per level-1 launch measured with rocprofv3 counters (FETCH_SIZE
@@ -371,9 +411,15 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "fwd_level_kernel<9/7> (level 1, 3 planes)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes": l1_bytes, "avg_launch_ms": round(t_dwt_l1, 4)},
+                     "algorithmic_bytes": l1_bytes, "avg_launch_ms": round(t_dwt_l1, 4),
+                     "timing": "%d launches rotating over %d buffer sets (cold last-level cache)" % (n_rot, ROT),
+                     "warm_avg_launch_ms": round(t_dwt_l1_warm, 4),
+                     "warm_frac": round(l1_bytes / (t_dwt_l1_warm * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "hbm_copy_ceiling_GBps": round(copy_gbs, 1),
-        "stages_ms": {"colour_ict": round(t_colour, 4), "colour_GBps": round(colour_gbs, 1),
+        "hbm_copy_warm_GBps": round(2.0 * cp_n / (t_copy_warm * 1e-3) / 1e9, 1),
+        "stages_ms": {"cache": "HBM-bound stages cold (rotating buffer sets); *_warm: same buffers",
+                      "colour_ict": round(t_colour, 4), "colour_GBps": round(colour_gbs, 1),
+                      "colour_ict_warm": round(t_colour_warm, 4), "dwt97_level1_warm": round(t_dwt_l1_warm, 4),
                       "dwt97_quant_6lv": round(t_dwt, 4), "dwt97_level1": round(t_dwt_l1, 4),
                       "t1_encode_standard_style": round(t_t1, 4),
                       "t1_standard_codeblocks_per_s": round(len(desc) / (t_t1 * 1e-3), 1),
@@ -382,7 +428,9 @@ def main():
         "reverse_stages_ms": {"dwt97_inv_level1": round(t_idwt_l1, 4),
                               "dwt97_inv_level1_GBps": round(l1_bytes / (t_idwt_l1 * 1e-3) / 1e9, 1),
                               "dwt97_inv_6lv": round(t_idwt, 4), "colour_inv_ict": round(t_cinv, 4),
-                              "colour_inv_GBps": round(15.0 * W * H / (t_cinv * 1e-3) / 1e9, 1)},
+                              "colour_inv_GBps": round(15.0 * W * H / (t_cinv * 1e-3) / 1e9, 1),
+                              "dwt97_inv_level1_warm": round(t_idwt_l1_warm, 4),
+                              "colour_inv_ict_warm": round(t_cinv_warm, 4)},
         "ingest": ingest,
         "styles": styles,
         "decode": dict(what="device-resident codewords of the same code-block style -> tier-1 decode -> "
