@@ -375,7 +375,9 @@ __device__ __forceinline__ void fwd_tile_plain97(const rg_fwd_level_args &a, int
 // y0-H .. y0+TH+H, columns x0-H .. x0+TW+H, symmetric extension) from the
 // four bands, dequantised, into LDS in interleaved order; then rows, then
 // columns.
-template <int F, int TW, int TH, int WGS>
+// PLAIN: the common 9/7 level (LL from its FP32 buffer, quantised bands,
+// both sides longer than the halo), its format choices fixed at compile time.
+template <int F, int TW, int TH, int WGS, bool PLAIN = false>
 __device__ __forceinline__ void inv_tile(const rg_inv_level_args &a, int plane, int bx, int by, char *lds)
 {
     using T = typename Filt<F>::T;
@@ -391,17 +393,18 @@ __device__ __forceinline__ void inv_tile(const rg_inv_level_args &a, int plane, 
     const int x0 = bx * TW, y0 = by * TH;
     const int cw = a.cw, ch = a.ch;
     const int wl = (cw + 1) >> 1, hl = (ch + 1) >> 1;
-    const bool vlift = ch > 1, hlift = cw > 1;
-    const bool simple = cw > H && ch > H;
+    const bool vlift = PLAIN || ch > 1, hlift = PLAIN || cw > 1;
+    const bool simple = PLAIN || (cw > H && ch > H);
+    const bool ll_is_q = PLAIN ? false : a.ll_is_q != 0, q_float = PLAIN ? false : a.q_float != 0;
     const int32_t *qb = reinterpret_cast<const int32_t *>(a.q) + (long)plane * a.q_ps;
-    const int32_t *lb = a.ll_is_q ? qb : reinterpret_cast<const int32_t *>(a.ll_in) + (long)plane * a.ll_ps;
+    const int32_t *lb = ll_is_q ? qb : reinterpret_cast<const int32_t *>(a.ll_in) + (long)plane * a.ll_ps;
 
     // ---- stage: every load first, then convert ----
     // Interior tiles read each band row as aligned 16-byte quads: band indices
     // [bq, bq + 4*NQ), bq = x0/2 - 4, band index b landing in interleaved
     // column 2*(b - (x0-H)/2) + (high band); other tiles go sample by sample.
-    const int32_t *lrow_base = a.ll_is_q ? qb : lb;
-    const long lpitch = a.ll_is_q ? a.qp : a.ll_pitch;
+    const int32_t *lrow_base = ll_is_q ? qb : lb;
+    const long lpitch = ll_is_q ? a.qp : a.ll_pitch;
     const bool quads = x0 >= 8 && y0 >= H && (x0 >> 1) + TW / 2 + 4 <= wl && y0 + TH + H <= ch &&
                        ((a.qp | lpitch | wl) & 3) == 0 && (((uintptr_t)qb | (uintptr_t)lrow_base) & 15) == 0;
     if (quads) {
@@ -424,7 +427,7 @@ __device__ __forceinline__ void inv_tile(const rg_inv_level_args &a, int plane, 
                                        : qb + (long)(ly ? byy : hl + byy) * a.qp + (high ? wl : 0) + bq + 4 * m4;
             raw[i] = *reinterpret_cast<const int4 *>(p);
             if constexpr (F == RG_FILTER_97)
-                stp[i] = ((from_ll && !a.ll_is_q) || a.q_float) ? 0.0f
+                stp[i] = ((from_ll && !ll_is_q) || q_float) ? 0.0f
                                                                  : (ly ? (high ? a.s_hl : a.s_ll) : (high ? a.s_hh : a.s_lh));
         }
 #pragma unroll
@@ -457,12 +460,12 @@ __device__ __forceinline__ void inv_tile(const rg_inv_level_args &a, int plane, 
             const int x = simple ? mirror1(x0 - H + c, cw) : rg_mirror(x0 - H + c, cw);
             const bool ly = !(y & 1), lx = !(x & 1);
             const int byy = y >> 1, bxx = x >> 1;
-            const bool from_ll = ly && lx && !a.ll_is_q;
+            const bool from_ll = ly && lx && !ll_is_q;
             const int32_t *p = from_ll ? lb + (long)byy * a.ll_pitch + bxx
                                        : qb + (long)(ly ? byy : hl + byy) * a.qp + (lx ? bxx : wl + bxx);
             raw[i] = *p;
             if constexpr (F == RG_FILTER_97)
-                stp[i] = (from_ll || a.q_float) ? 0.0f : (ly ? (lx ? a.s_ll : a.s_hl) : (lx ? a.s_lh : a.s_hh));
+                stp[i] = (from_ll || q_float) ? 0.0f : (ly ? (lx ? a.s_ll : a.s_hl) : (lx ? a.s_lh : a.s_hh));
         }
 #pragma unroll
         for (int i = 0; i < NI; ++i) {
@@ -547,13 +550,13 @@ __global__ void __launch_bounds__(FWG) fwd_level_kernel(rg_fwd_level_args a)
     else fwd_tile<F, FTW, FTH, FWG>(a, plane, bx, by, lds);
 }
 
-template <int F>
+template <int F, bool PLAIN = false>
 __global__ void __launch_bounds__(IWG) inv_level_kernel(rg_inv_level_args a)
 {
     __shared__ __attribute__((aligned(16))) char lds[INV_LDS_BYTES];
     int plane, bx, by;
     xcd_tile(plane, bx, by);
-    inv_tile<F, ITW, ITH, IWG>(a, plane, bx, by, lds);
+    inv_tile<F, ITW, ITH, IWG, PLAIN>(a, plane, bx, by, lds);
 }
 
 // Batched: one workgroup per job {image args index, plane, tile x, tile y};
@@ -626,7 +629,10 @@ int rg_launch_inv53_level(hipStream_t s, const rg_inv_level_args &a)
 int rg_launch_inv97_level(hipStream_t s, const rg_inv_level_args &a)
 {
     dim3 grid((a.cw + ITW - 1) / ITW, (a.ch + ITH - 1) / ITH, a.nplanes);
-    inv_level_kernel<RG_FILTER_97><<<grid, dim3(IWG), 0, s>>>(a);
+    if (!a.ll_is_q && !a.q_float && a.cw > 4 && a.ch > 4)
+        inv_level_kernel<RG_FILTER_97, true><<<grid, dim3(IWG), 0, s>>>(a);
+    else
+        inv_level_kernel<RG_FILTER_97><<<grid, dim3(IWG), 0, s>>>(a);
     RG_LAUNCH_CHECK();
     return RG_SUCCESS;
 }
