@@ -75,30 +75,33 @@ RG_DEV void mqd_start(MQDec &m, const uint8_t *p, int len)
     m.C <<= 7; m.CT -= 7; m.A = 0x8000;
 }
 
-// DECODE (C.3.2) of one decision in context cx; lane c of ctxv holds context
-// c's state (index << 1 | MPS), lane k of mqtv table entry k.  RENORMD
-// (C.3.3) shifts in one go up to the next BYTEIN.
+// DECODE (C.3.2) of one decision in context cx.  Lane c of ctxv holds
+// context c's state as its probability-table entry (Qe | NMPS << 16 | NLPS
+// << 22 | SWITCH << 28) with the MPS in bit 31, so a decision reads its state
+// with one readlane; lane k of mqtv holds table entry k, read only when the
+// state changes.  RENORMD (C.3.3) shifts in one go up to the next BYTEIN.
 RG_DEV int mqd_decode(MQDec &m, int &ctxv, int mqtv, int cx)
 {
-    const int st = wv_readlane(ctxv, cx);
-    const int mps = st & 1;
-    const uint32_t t = (uint32_t)wv_readlane(mqtv, st >> 1);
-    const uint32_t qe = t & 0xFFFFu;
-    int d, ns;
+    const uint32_t e = (uint32_t)wv_readlane(ctxv, cx);
+    const int mps = (int)(e >> 31);
+    const uint32_t qe = e & 0xFFFFu;
+    int d, idx, nm;
     m.A -= qe;
     if ((m.C >> 16) < qe) {
         const bool cond = m.A < qe;   // LPS exchange: the MPS if A < Qe
         m.A = qe;
         d = cond ? mps : 1 - mps;
-        ns = cond ? (int)(((t >> 16) & 63u) << 1) | mps : (int)(((t >> 22) & 63u) << 1) | (mps ^ (int)(t >> 28));
+        idx = (int)((e >> (cond ? 16 : 22)) & 63u);
+        nm = cond ? mps : mps ^ (int)((e >> 28) & 1u);
     } else {
         m.C -= qe << 16;
-        if (m.A & 0x8000u) return mps;
+        if (m.A & 0x8000) return mps;
         const bool cond = m.A < qe;   // MPS exchange: the LPS if A < Qe
         d = cond ? 1 - mps : mps;
-        ns = cond ? (int)(((t >> 22) & 63u) << 1) | (mps ^ (int)(t >> 28)) : (int)(((t >> 16) & 63u) << 1) | mps;
+        idx = (int)((e >> (cond ? 22 : 16)) & 63u);
+        nm = cond ? mps ^ (int)((e >> 28) & 1u) : mps;
     }
-    ctxv = wv_lane() == cx ? ns : ctxv;
+    ctxv = wv_writelane(ctxv, cx, (int)((uint32_t)wv_readlane(mqtv, idx) | ((uint32_t)nm << 31)));
     int n = wv_clz(m.A) - 16;
     m.A <<= n;
     while (n > 0) {
@@ -232,7 +235,8 @@ RG_DEV void t1_dec_block(const DecJob &job, uint8_t *lut, uint32_t *rawbuf)
     const bool col_ok = lane < w;
     const uint64_t rowmask = h >= 64 ? ~0ull : ((1ull << h) - 1);
     const int nstripes = (h + 3) >> 2;
-    const int ctx0 = lane == 0 ? (4 << 1) : lane == CTX_RL ? (3 << 1) : lane == CTX_UNI ? (46 << 1) : 0;
+    // initial states (Table D.7) as table entries, MPS 0
+    const int ctx0 = (int)MQT[lane == 0 ? 4 : lane == CTX_RL ? 3 : lane == CTX_UNI ? 46 : 0];
     int ctxv = ctx0;
     const int mqtv = lane < 47 ? (int)MQT[lane] : 0;
     const bool termall = (job.style & STYLE_TERMALL) && job.seglen;

@@ -31,6 +31,8 @@ RG_DEV bool wv_any(bool p) { return __any(p); }
 RG_DEV uint64_t wv_ballot(bool p) { return __ballot(p); }
 // value of lane j (j uniform)
 RG_DEV int wv_readlane(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
+// v with lane j's value replaced by x (j and x uniform)
+RG_DEV int wv_writelane(int v, int j, int x) { return wv_lane() == j ? x : v; }
 // a value the caller knows to be wave-uniform, moved to a scalar register
 RG_DEV int wv_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 template <class T> RG_DEV T wv_shfl_up(T v, int d) { return __shfl_up(v, d, 64); }

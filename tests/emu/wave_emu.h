@@ -70,6 +70,7 @@ static inline uint64_t wv_ballot(bool p)
     return m;
 }
 static inline int wv_readlane(int x, int j) { return (int)(uint32_t)emu::exchange((uint32_t)x, 0)[j]; }
+static inline int wv_writelane(int v, int j, int x) { return emu::lane() == j ? x : v; }
 static inline int wv_uniform(int x) { return (int)(uint32_t)emu::exchange((uint32_t)x, 1)[0]; }
 template <class T> static inline T wv_shfl_up(T x, int d)
 {
