@@ -46,70 +46,104 @@ struct DecJob {
 };
 
 struct MQDec {
-    uint32_t A, C;
-    int CT, bp, len;
+    // MQ: C.3's C register in bits 63..32 (the compared half in 63..48) with
+    // the codeword bits already read in below it, zero below bit E.  Raw: the
+    // current byte.
+    uint64_t C;
+    uint32_t A;
+    int E;            // MQ: lowest bit holding codeword data
+    int CT;           // raw: unread bits of the current byte
+    int bp, len;      // next byte to read; segment length
+    int ff, marker;   // MQ: the last byte read was 0xFF / a marker was reached
     const uint8_t *buf;
 };
 
 // Bytes past the segment end read as 0xFF (C.3.4).
 RG_DEV uint32_t mqd_byte(const MQDec &m, int i) { return i < m.len ? wv_sload_u8(m.buf + i) : 0xFFu; }
 
-// BYTEIN (C.3.4).
-RG_DEV void mqd_bytein(MQDec &m)
+// Reads codeword bytes in below bit E while E > 32, so a renormalisation's
+// (at most 15) shifts always find bits, each byte where BYTEIN (C.3.4) puts
+// it: 8 bits below the previous byte, or 7 after an 0xFF (its first bit a
+// carry into the 0xFF).  A byte after an 0xFF is added only once the 0xFF's
+// last bit has passed bit 48, which is when BYTEIN adds it, since its carry
+// can reach the compared bits; other bytes land on zero bits, so adding them
+// early changes nothing that DECODE compares.  After a marker (0xFF, then a
+// byte above 0x8F) the reader supplies 0xFF bytes for ever.
+RG_DEV void mqd_fill(MQDec &m)
 {
-    const uint32_t b = mqd_byte(m, m.bp), b1 = mqd_byte(m, m.bp + 1);
-    if (b == 0xFF) {
-        if (b1 > 0x8F) { m.C += 0xFF00; m.CT = 8; }
-        else { m.bp++; m.C += b1 << 9; m.CT = 7; }
-    } else {
-        m.bp++; m.C += b1 << 8; m.CT = 8;
+    while (m.E > 32) {
+        if (m.marker) {
+            m.C += 0xFFull << (m.E - 8);
+            m.E -= 8;
+            continue;
+        }
+        if (m.ff) {
+            if (m.E <= 48) break;
+            const uint32_t b1 = mqd_byte(m, m.bp);
+            if (b1 > 0x8F) {
+                m.marker = 1;
+                continue;
+            }
+            m.bp++;
+            m.C += (uint64_t)b1 << (m.E - 7);
+            m.E -= 7;
+            m.ff = 0;
+            continue;
+        }
+        const uint32_t b = mqd_byte(m, m.bp);
+        m.bp++;
+        m.C += (uint64_t)b << (m.E - 8);
+        m.E -= 8;
+        m.ff = b == 0xFFu;
     }
 }
 
-// INITDEC (C.3.5) on the segment [p, p + len).
+// INITDEC (C.3.5) on the segment [p, p + len): the first byte, BYTEIN and
+// the shift by 7.
 RG_DEV void mqd_start(MQDec &m, const uint8_t *p, int len)
 {
-    m.buf = p; m.len = len; m.bp = 0;
-    m.C = mqd_byte(m, 0) << 16;
-    mqd_bytein(m);
-    m.C <<= 7; m.CT -= 7; m.A = 0x8000;
+    m.buf = p; m.len = len;
+    const uint32_t b0 = mqd_byte(m, 0);
+    m.bp = 1; m.ff = b0 == 0xFFu; m.marker = 0;
+    m.C = (uint64_t)b0 << 55;
+    m.E = 55;
+    mqd_fill(m);
+    m.A = 0x8000;
 }
 
 // DECODE (C.3.2) of one decision in context cx.  Lane c of ctxv holds
 // context c's state as its probability-table entry (Qe | NMPS << 16 | NLPS
 // << 22 | SWITCH << 28) with the MPS in bit 31, so a decision reads its state
 // with one readlane; lane k of mqtv holds table entry k, read only when the
-// state changes.  RENORMD (C.3.3) shifts in one go up to the next BYTEIN.
+// state changes.  The exchanges are written as selects: the decision is the
+// LPS when the lower sub-interval is kept with A - Qe >= Qe or the upper one
+// with A - Qe < Qe.  RENORMD (C.3.3) shifts A and C in one go.
 RG_DEV int mqd_decode(MQDec &m, int &ctxv, int mqtv, int cx)
 {
     const uint32_t e = (uint32_t)wv_readlane(ctxv, cx);
-    const int mps = (int)(e >> 31);
+    const uint32_t mps = e >> 31;
     const uint32_t qe = e & 0xFFFFu;
-    int d, idx, nm;
-    m.A -= qe;
-    if ((m.C >> 16) < qe) {
-        const bool cond = m.A < qe;   // LPS exchange: the MPS if A < Qe
-        m.A = qe;
-        d = cond ? mps : 1 - mps;
-        idx = (int)((e >> (cond ? 16 : 22)) & 63u);
-        nm = cond ? mps : mps ^ (int)((e >> 28) & 1u);
-    } else {
-        m.C -= qe << 16;
-        if (m.A & 0x8000) return mps;
-        const bool cond = m.A < qe;   // MPS exchange: the LPS if A < Qe
-        d = cond ? 1 - mps : mps;
-        idx = (int)((e >> (cond ? 22 : 16)) & 63u);
-        nm = cond ? mps ^ (int)((e >> 28) & 1u) : mps;
+    const uint32_t a = m.A - qe;
+    const uint32_t chigh = (uint32_t)(m.C >> 48);
+    if (chigh >= qe) {
+        m.C -= (uint64_t)qe << 48;
+        if (a & 0x8000u) {
+            m.A = a;
+            return (int)mps;
+        }
     }
-    ctxv = wv_writelane(ctxv, cx, (int)((uint32_t)wv_readlane(mqtv, idx) | ((uint32_t)nm << 31)));
-    int n = wv_clz(m.A) - 16;
+    const uint32_t lower = chigh < qe ? 1u : 0u;
+    const uint32_t lps = lower ^ (a < qe ? 1u : 0u);
+    m.A = lower ? qe : a;
+    const uint32_t idx = (e >> (lps ? 22 : 16)) & 63u;
+    const uint32_t nm = mps ^ (lps & (e >> 28) & 1u);
+    ctxv = wv_writelane(ctxv, cx, (int)((uint32_t)wv_readlane(mqtv, (int)idx) | (nm << 31)));
+    const int n = wv_clz(m.A) - 16;
     m.A <<= n;
-    while (n > 0) {
-        if (m.CT == 0) mqd_bytein(m);
-        const int s = n < m.CT ? n : m.CT;
-        m.C <<= s; m.CT -= s; n -= s;
-    }
-    return d;
+    m.C <<= n;
+    m.E += n;
+    if (m.E > 32) mqd_fill(m);
+    return (int)(mps ^ lps);
 }
 
 // Raw segment reader (D.6): bits MSB first, 7 in the byte after an 0xFF,
