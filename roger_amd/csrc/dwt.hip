@@ -37,6 +37,7 @@
 // and the vertical one, whose column items store 64-wide row runs
 // (non-temporal).
 #include <hip/hip_fp16.h>
+#include <type_traits>
 #include "rg_internal.h"
 
 namespace {
@@ -287,11 +288,12 @@ __device__ __forceinline__ void fwd_tile(const rg_fwd_level_args &a, int plane, 
         fwd_put<F>(a, plane, qbase, y0 + r, xo, high, wl, hl, tile[r * LP + lcol(c)]);
 }
 
-// The common 9/7 level (FP32 input, FP32 LL output or LL quantised with the
-// other bands, quantised indices, both sides longer than the 4-sample halo):
-// fwd_tile's passes without the per-launch format choices.  Its leaner code measured 8-10% faster on
-// 4096x4096x3 level 1 with inputs not in any cache (92 -> 85 us).
-template <int TW, int TH, int WGS>
+// The common 9/7 levels (FP32 or FP16 input, FP32 or FP16 LL output or LL
+// quantised with the other bands, quantised indices, both sides longer than
+// the 4-sample halo): fwd_tile's passes with the formats fixed at compile
+// time.  Its leaner code measured 8-10% faster on 4096x4096x3 level 1 with
+// inputs not in any cache (92 -> 85 us).
+template <int TW, int TH, int WGS, bool IN_HALF, bool LL_HALF>
 __device__ __forceinline__ void fwd_tile_plain97(const rg_fwd_level_args &a, int plane, int bx, int by, char *lds)
 {
     using T = float;
@@ -305,7 +307,8 @@ __device__ __forceinline__ void fwd_tile_plain97(const rg_fwd_level_args &a, int
     T *tile = reinterpret_cast<T *>(lds);
     const int x0 = bx * TW, y0 = by * TH;
     const int cw = a.cw, ch = a.ch;
-    const float *in = reinterpret_cast<const float *>(a.in) + (long)plane * a.in_ps;
+    using In = typename std::conditional<IN_HALF, __half, float>::type;
+    const In *in = reinterpret_cast<const In *>(a.in) + (long)plane * a.in_ps;
     const bool interior = x0 >= H && y0 >= H && x0 + TW + H <= cw && y0 + TH + H <= ch;
     for (int item = threadIdx.x; item < CW * VS; item += WGS) {
         // items [0, TW): the tile's own columns, so that a wavefront's loads
@@ -319,10 +322,10 @@ __device__ __forceinline__ void fwd_tile_plain97(const rg_fwd_level_args &a, int
         if (interior) {
             const long first = (long)ybase * a.in_pitch + x;
 #pragma unroll
-            for (int j = 0; j < N; ++j) e[j] = in[first + (long)j * a.in_pitch];
+            for (int j = 0; j < N; ++j) e[j] = (float)in[first + (long)j * a.in_pitch];
         } else {
 #pragma unroll
-            for (int j = 0; j < N; ++j) e[j] = in[(long)mirror1(ybase + j, ch) * a.in_pitch + x];
+            for (int j = 0; j < N; ++j) e[j] = (float)in[(long)mirror1(ybase + j, ch) * a.in_pitch + x];
         }
         fwd97(e, true);
 #pragma unroll
@@ -348,7 +351,8 @@ __device__ __forceinline__ void fwd_tile_plain97(const rg_fwd_level_args &a, int
     __syncthreads();
     const int wl = (cw + 1) >> 1, hl = (ch + 1) >> 1;
     int *qbase = reinterpret_cast<int *>(a.q) + (long)plane * a.q_ps;
-    float *llbase = a.ll_out ? reinterpret_cast<float *>(a.ll_out) + (long)plane * a.ll_ps : nullptr;
+    using Ll = typename std::conditional<LL_HALF, __half, float>::type;
+    Ll *llbase = a.ll_out ? reinterpret_cast<Ll *>(a.ll_out) + (long)plane * a.ll_ps : nullptr;
     if (threadIdx.x >= RS * TW) return;
     const int c = threadIdx.x % TW, r0 = threadIdx.x / TW;
     const bool high = c >= TW / 2;
@@ -540,13 +544,17 @@ __device__ __forceinline__ void xcd_tile(int &plane, int &bx, int &by)
     bx = t % tx;
 }
 
-template <int F, bool PLAIN = false>
+// PLAIN: 0 the general tile; 1-3 the plain 9/7 tile with FP32 input and LL
+// (1), FP32 input and FP16 LL (2), FP16 input and LL (3)
+template <int F, int PLAIN = 0>
 __global__ void __launch_bounds__(FWG) fwd_level_kernel(rg_fwd_level_args a)
 {
     __shared__ __attribute__((aligned(16))) char lds[FWD_LDS_BYTES];
     int plane, bx, by;
     xcd_tile(plane, bx, by);
-    if constexpr (PLAIN) fwd_tile_plain97<FTW, FTH, FWG>(a, plane, bx, by, lds);
+    if constexpr (PLAIN == 1) fwd_tile_plain97<FTW, FTH, FWG, false, false>(a, plane, bx, by, lds);
+    else if constexpr (PLAIN == 2) fwd_tile_plain97<FTW, FTH, FWG, false, true>(a, plane, bx, by, lds);
+    else if constexpr (PLAIN == 3) fwd_tile_plain97<FTW, FTH, FWG, true, true>(a, plane, bx, by, lds);
     else fwd_tile<F, FTW, FTH, FWG>(a, plane, bx, by, lds);
 }
 
@@ -610,8 +618,15 @@ int rg_launch_fwd53_level(hipStream_t s, const rg_fwd_level_args &a)
 int rg_launch_fwd97_level(hipStream_t s, const rg_fwd_level_args &a)
 {
     dim3 grid((a.cw + FTW - 1) / FTW, (a.ch + FTH - 1) / FTH, a.nplanes);
-    if (!a.in_half && !a.ll_half && !a.q_float && a.cw > 4 && a.ch > 4)
-        fwd_level_kernel<RG_FILTER_97, true><<<grid, dim3(FWG), 0, s>>>(a);
+    const bool plain = !a.q_float && a.cw > 4 && a.ch > 4;
+    // the LL format only matters when the level writes an LL buffer
+    const bool llh = a.ll_out && a.ll_half;
+    if (plain && !a.in_half && !llh)
+        fwd_level_kernel<RG_FILTER_97, 1><<<grid, dim3(FWG), 0, s>>>(a);
+    else if (plain && !a.in_half)
+        fwd_level_kernel<RG_FILTER_97, 2><<<grid, dim3(FWG), 0, s>>>(a);
+    else if (plain && (llh || !a.ll_out))
+        fwd_level_kernel<RG_FILTER_97, 3><<<grid, dim3(FWG), 0, s>>>(a);
     else
         fwd_level_kernel<RG_FILTER_97><<<grid, dim3(FWG), 0, s>>>(a);
     RG_LAUNCH_CHECK();
