@@ -653,7 +653,7 @@ static int run_t1(rg_encoder *e)
             (rc = e->chains.ensure(rg_t1_chain_ws_bytes(e->ncb))))
             return rc;
         rg_t1_overlap ov{};
-        const int phases = e->t1_phases ? e->t1_phases : rg_t1_phases();
+        const int phases = e->t1_phases ? e->t1_phases : rg_t1_phases(e->style);
         const bool two = phases > 1;
         if (two) {
             ov.nearly = phases - 1;

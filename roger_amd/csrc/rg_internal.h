@@ -165,11 +165,11 @@ size_t rg_t1_chain_ws_bytes(int nblocks);
 // bytes per block of modelling state kept between the phases.
 #define RG_T1_PHASE_SPLIT0 4   // four-phase form only
 #define RG_T1_PHASE_SPLIT1 7
-#define RG_T1_PHASE_SPLIT 10
+#define RG_T1_PHASE_SPLIT 13
 void rg_t1_splits(int *s1, int *s2);
-// number of modelling phases: 1, 2 or 3 (RG_T1_PHASES; RG_T1_TWO_PHASE=0 also
-// selects 1)
-int rg_t1_phases();
+// number of modelling phases of a parallel code-block style: RG_T1_PHASES
+// (1-4) if set, RG_T1_TWO_PHASE=0 one; else two with BYPASS and one without
+int rg_t1_phases(int style);
 constexpr int RG_T1_MAX_EARLY = 3;   // early phases (each with a side stream)
 struct rg_t1_overlap {
     int nearly;              // early phases: 1 .. RG_T1_MAX_EARLY

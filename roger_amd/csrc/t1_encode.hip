@@ -293,17 +293,24 @@ void rg_t1_job_fill(void *dst, const int32_t *src, int pitch, int w, int h, int 
     *reinterpret_cast<T1Job *>(dst) = j;
 }
 
-// Overlapped two-phase tier-1 for the parallel styles unless RG_T1_TWO_PHASE=0.
-int rg_t1_phases()
+// Phases of the overlapped tier-1 of the parallel styles.  Measured on the
+// 4096x4096 RGB encode with the two-wavefront chain coder: RESET|TERMALL|
+// BYPASS 2.71 ms in one phase, 2.49 in two (split before pass 13; 2.52 at 10,
+// 2.51 at 16), 2.67 in three (7, 10); RESET|TERMALL 3.02 / 3.15 / 3.36 ms --
+// with every pass an MQ chain, the early chains' coding does not hide behind
+// the rest of the modelling.  RG_T1_PHASES=1..4 overrides (RG_T1_TWO_PHASE=0:
+// one).
+int rg_t1_phases(int style)
 {
     static int v = -1;
     if (v < 0) {
         const char *two = getenv("RG_T1_TWO_PHASE"), *n = getenv("RG_T1_PHASES");
-        v = 3;
+        v = 0;
         if (n && n[0] >= '1' && n[0] <= '4') v = n[0] - '0';
         if (two && two[0] == '0') v = 1;
     }
-    return v;
+    if (v) return v;
+    return (style & STYLE_BYPASS) ? 2 : 1;
 }
 
 // Phase boundaries (coding-pass indices, each 1 mod 3 so that a phase ends
