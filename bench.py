@@ -145,8 +145,9 @@ def main():
                     help="code-block style of the headline encode")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (single-GPU hot path; see DESIGN.md "
-              "for how it would shard); --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        print("bench.py: the hot path runs on one GPU only (one image's stages on one device; images "
+              "and code-blocks would shard across GPUs by image); --gpus %d is not supported" % args.gpus,
+              file=sys.stderr)
         sys.exit(2)
 
     from roger_amd import runtime as R

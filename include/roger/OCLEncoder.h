@@ -34,7 +34,7 @@ public:
             bpc->run(codeblockX, codeblockY);
         }
     }
-    // Extensions of this build: code-block size and the tier-1 output.
+    // Beyond the reference class: code-block size and the tier-1 output.
     void setCodeblockSize(size_t cbx, size_t cby) { codeblockX = cbx; codeblockY = cby; }
     tDeviceRC getCodeblocks(std::vector<rg_cb_desc> &desc, std::vector<rg_cb_result> &res,
                             std::vector<long> &offsets, std::vector<uint8_t> &data)

@@ -102,7 +102,7 @@ public:
         return role == ENCODE ? rg_encoder_unmap_dwt_out(enc, mappedPtr) : rg_decoder_unmap_out(dec, mappedPtr);
     }
 
-    // Extensions of this build.
+    // Beyond the reference class.
     void setRole(Role r) { role = r; }
     void setQuantStep(double delta) { quantStep = delta; }
     double getQuantStep() const { return quantStep; }

@@ -17,7 +17,6 @@ from roger_amd import synth
 from roger_amd._lib import RG_FILTER_97
 
 pytestmark = pytest.mark.gpu
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _psnr(a, b):
@@ -92,7 +91,7 @@ def test_precision_study_8192(ctx):
     ll_px = sum(((W + (1 << l) - 1) >> l) * ((H + (1 << l) - 1) >> l) for l in range(1, L))
     report["interlevel_bytes_fp32"] = 8 * ll_px
     report["interlevel_bytes_fp16"] = 4 * ll_px
-    out = os.path.join(ROOT, "gpurun_out")
+    out = os.environ.get("RG_REPORT_DIR") or __import__("tempfile").gettempdir()
     os.makedirs(out, exist_ok=True)
     with open(os.path.join(out, "precision_study.json"), "w") as f:
         json.dump(report, f, indent=1)
