@@ -13,11 +13,13 @@ code-block style chosen by --style:
 value = image Mpix/s over the timed steps.
 
 Also reported:
-  roofline      : the level-1 9/7 DWT+quantisation kernel (the DWT's dominant
-                  launch), algorithmic bytes 8 B/px/plane (FP32 in, LL FP32 +
-                  int32 indices out) over its HIP-event-timed launch, vs 8 TB/s.
-  stages_ms     : per-stage times (colour, DWT all levels, tier-1 in both
-                  styles).
+  roofline      : the encode's dominant HBM-bound launch, colour + level-1
+                  9/7 DWT + quantisation in one kernel: algorithmic bytes
+                  15 B/px (3 B uint8 RGB in; per component LL FP32 + int32
+                  indices, 4 B/px, out) over its HIP-event-timed launch, vs
+                  8 TB/s.
+  stages_ms     : per-stage times (fused front end, the separate colour and
+                  level kernels, tier-1 in each style).
   styles        : the same encode in each code-block style (rate and
                   codestream bytes).
   decode        : the reverse path on the device-resident codewords (tier-1
@@ -214,6 +216,8 @@ def main():
     t_colour_warm = timed(lambda: R.colour_fwd(ctx, d_src, W, H, NC, RG_COLOUR_ICT, 1, d_pl))
     inv1 = R.band_steps(1, DELTA)[1]
     t_dwt_l1_warm = timed(lambda: R.dwt_fwd(ctx, RG_FILTER_97, d_pl, W, H, NC, 1, inv1, 0, d_q, d_s))
+    assert R.dwt_fwd_u8_scratch_bytes(W, H, NC, RG_COLOUR_ICT, RG_FILTER_97) <= R.scratch_bytes(W, H, NC, RG_FILTER_97)
+    t_front1_warm = timed(lambda: R.dwt_fwd_u8(ctx, d_src, W, H, NC, RG_COLOUR_ICT, RG_FILTER_97, 1, inv1, d_q, d_s))
     R.dwt_fwd(ctx, RG_FILTER_97, d_pl, W, H, NC, LEVELS, inv, 0, d_q, d_s)
     desc = R.codeblocks(W, H, LEVELS, CB, CB, NC)
     d_desc = ctx.upload(desc)
@@ -370,6 +374,12 @@ def main():
             fn(sets[i % ROT])
         return ctx.timer_stop() / n_rot
 
+    # the encode's front end: colour + level 1 in one kernel (the image read
+    # once), and colour + all levels (rg_dwt_fwd_u8)
+    t_front1 = timed_rot(lambda b: R.dwt_fwd_u8(ctx, b["src"], W, H, NC, RG_COLOUR_ICT, RG_FILTER_97, 1, inv1,
+                                                b["q"], b["s"]))
+    t_front = timed_rot(lambda b: R.dwt_fwd_u8(ctx, b["src"], W, H, NC, RG_COLOUR_ICT, RG_FILTER_97, LEVELS, inv,
+                                               b["q"], b["s"]))
     t_colour = timed_rot(lambda b: R.colour_fwd(ctx, b["src"], W, H, NC, RG_COLOUR_ICT, 1, b["pl"]))
     t_dwt_l1 = timed_rot(lambda b: R.dwt_fwd(ctx, RG_FILTER_97, b["pl"], W, H, NC, 1, inv1, 0, b["q"], b["s"]))
     t_dwt = timed_rot(lambda b: R.dwt_fwd(ctx, RG_FILTER_97, b["pl"], W, H, NC, LEVELS, inv, 0, b["q"], b["s"]))
@@ -387,7 +397,9 @@ def main():
             v.free()
     copy_gbs = 2.0 * cp_n / (t_copy * 1e-3) / 1e9
     l1_bytes = 8.0 * W * H * NC
-    achieved = l1_bytes / (t_dwt_l1 * 1e-3) / 1e9
+    # colour + level 1: 3 B/px of uint8 in, level-1 outputs (4 B/px/plane) out
+    f1_bytes = (1.0 + 4.0) * W * H * NC
+    achieved = f1_bytes / (t_front1 * 1e-3) / 1e9
     # HBM bytes per level-1 launch measured with rocprofv3 counters (FETCH_SIZE
     # and WRITE_SIZE scaled by a copy kernel of known size), as recorded in the
     # committed counter summary for this workload
@@ -395,7 +407,7 @@ def main():
     prof = os.path.join(ROOT, "profiles", "encode_4096rgb_pmc_counters.md")
     if W == 4096 and H == 4096 and os.path.exists(prof):
         import re
-        mt = re.search(r"^traffic_bytes_per_launch: (\d+)$", open(prof).read(), re.M)
+        mt = re.search(r"^traffic_bytes_per_launch_fwd1_colour: (\d+)$", open(prof).read(), re.M)
         traffic = int(mt.group(1)) if mt else None
     colour_gbs = 15.0 * W * H / (t_colour * 1e-3) / 1e9
 
@@ -409,19 +421,23 @@ def main():
                    "codeblock_style": STYLE,
                    "image": [W, H, NC], "levels": LEVELS, "delta": DELTA, "codeblock": [CB, CB],
                    "codeblocks": int(ncb), "codestream_bytes": total_bytes, "input_sha256": img_sha},
-        "roofline": {"bound": "hbm", "kernel": "fwd_level_kernel<9/7> (level 1, 3 planes)",
+        "roofline": {"bound": "hbm",
+                     "kernel": "fwd1_colour_kernel<9/7, ICT> (colour + DWT level 1 + quantisation, 3 components)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes": l1_bytes, "avg_launch_ms": round(t_dwt_l1, 4),
+                     "algorithmic_bytes": f1_bytes, "avg_launch_ms": round(t_front1, 4),
                      "timing": "%d launches rotating over %d buffer sets (cold last-level cache)" % (n_rot, ROT),
-                     "warm_avg_launch_ms": round(t_dwt_l1_warm, 4),
-                     "warm_frac": round(l1_bytes / (t_dwt_l1_warm * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+                     "warm_avg_launch_ms": round(t_front1_warm, 4),
+                     "warm_frac": round(f1_bytes / (t_front1_warm * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "hbm_copy_ceiling_GBps": round(copy_gbs, 1),
         "hbm_copy_warm_GBps": round(2.0 * cp_n / (t_copy_warm * 1e-3) / 1e9, 1),
         "stages_ms": {"cache": "HBM-bound stages cold (rotating buffer sets); *_warm: same buffers",
+                      "colour_dwt97_level1_fused": round(t_front1, 4),
+                      "colour_dwt97_quant_6lv_fused": round(t_front, 4),
                       "colour_ict": round(t_colour, 4), "colour_GBps": round(colour_gbs, 1),
                       "colour_ict_warm": round(t_colour_warm, 4), "dwt97_level1_warm": round(t_dwt_l1_warm, 4),
                       "dwt97_quant_6lv": round(t_dwt, 4), "dwt97_level1": round(t_dwt_l1, 4),
+                      "dwt97_level1_GBps": round(l1_bytes / (t_dwt_l1 * 1e-3) / 1e9, 1),
                       "t1_encode_standard_style": round(t_t1, 4),
                       "t1_standard_codeblocks_per_s": round(len(desc) / (t_t1 * 1e-3), 1),
                       "encode_parallel_style": round(t_enc_par, 4),

@@ -171,6 +171,21 @@ int rg_dwt_fwd(rg_context *ctx, int filter, const void *d_in, int w, int h, int 
  * array keeps the last call's contents; nothing needs to start at zero. */
 int rg_dwt_fwd_noalloc(rg_context *ctx, int filter, const void *d_in, int w, int h, int nplanes,
                        int levels, const float *inv_step, int ll_fp16, int32_t *d_q);
+/* Colour stage and forward transform in one: interleaved uint8 image d_src
+ * (w x h x ncomp, 4-byte aligned; device memory) -> level shift, colour
+ * transform (RG_COLOUR_RCT with 5/3, RG_COLOUR_ICT with 9/7, or NONE) ->
+ * levels (>= 1) of DWT (+ 9/7 quantisation as rg_dwt_fwd) into d_q.  Where the
+ * shape allows (1 or 3 components, w * ncomp a multiple of 4, both sides
+ * longer than the filter halo: 4 samples for 9/7, 2 for 5/3) the first level
+ * reads the image itself and the colour planes never go to memory;
+ * otherwise the planes pass through d_scratch.  d_scratch holds
+ * rg_dwt_fwd_u8_scratch_bytes.  The native encoder's front end
+ * (rg_encoder_encode_u8); it replaces the reference's host level shift and
+ * upload (src/OCLTest.cpp:81-90, src/OCLMemoryManager.cpp:41-54,150-160)
+ * followed by OCLDWTForward::run (src/OCLDWTForward.cpp:124-137). */
+size_t rg_dwt_fwd_u8_scratch_bytes(int w, int h, int ncomp, int colour, int filter);
+int rg_dwt_fwd_u8(rg_context *ctx, const uint8_t *d_src, int w, int h, int ncomp, int colour, int filter,
+                  int levels, const float *inv_step, int32_t *d_q, void *d_scratch);
 /* 9/7 only: forward DWT without quantisation, float Mallat output
  * (the reference's outputDwt path, src/CL/ocldwt97.cl:196-420). */
 int rg_dwt97_fwd_float(rg_context *ctx, const float *d_in, int w, int h, int nplanes, int levels,

@@ -95,6 +95,8 @@ SIGNATURES = {
     "rg_planar_split_i16": (I, [P, P, I, I, I, P]),
     "rg_dwt_scratch_bytes": (SZ, [I, I, I, I]),
     "rg_dwt_fwd": (I, [P, I, P, I, I, I, I, P, I, P, P]),
+    "rg_dwt_fwd_u8_scratch_bytes": (ctypes.c_size_t, [I, I, I, I, I]),
+    "rg_dwt_fwd_u8": (I, [P, P, I, I, I, I, I, I, P, P, P]),
     "rg_dwt97_fwd_float": (I, [P, P, I, I, I, I, P, P]),
     "rg_dwt_fwd_noalloc": (I, [P, I, P, I, I, I, I, P, I, P]),
     "rg_dwt_inv": (I, [P, I, P, I, I, I, I, P, P, P]),

@@ -272,13 +272,22 @@ int rg_launch_copy_quant(hipStream_t s, int filter, const void *in, long n, int 
 int rg_launch_copy_dequant(hipStream_t s, int filter, const void *q, long n, int nplanes, float step, int q_float,
                            void *out);
 
+// Integer setting from the environment (`def` when unset).
+static int rg_env_int(const char *name, int def)
+{
+    const char *v = getenv(name);
+    return v && *v ? atoi(v) : def;
+}
+
+// Levels first..levels of the forward transform (first > 1: level first-1's
+// LL is already in the scratch buffer, written by the fused first level).
 static int dwt_fwd_impl(rg_context *c, int filter, const void *in, int w, int h, int nplanes, int levels,
-                        const float *inv_step, int ll_fp16, void *dq, int q_float, void *scratch)
+                        const float *inv_step, int ll_fp16, void *dq, int q_float, void *scratch, int first = 1)
 {
     int rc = check_levels(w, h, levels);
     if (rc) return rc;
     RG_CHECK(filter == RG_FILTER_53 || filter == RG_FILTER_97, "bad filter %d", filter);
-    RG_CHECK(in && dq && nplanes >= 1, "null buffer");
+    RG_CHECK((in || first > 1) && dq && nplanes >= 1, "null buffer");
     RG_CHECK(filter == RG_FILTER_53 || q_float || inv_step, "9/7 quantisation needs inv_step");
     if (levels == 0)
         return rg_launch_copy_quant(c->stream, filter, in, (long)w * h, nplanes,
@@ -288,7 +297,7 @@ static int dwt_fwd_impl(rg_context *c, int filter, const void *in, int w, int h,
     char *sc = reinterpret_cast<char *>(scratch);
     auto ll = [&](int l) { return sc + rg_ll_offset(w, h, nplanes, l); };
     const bool half = filter == RG_FILTER_97 && ll_fp16;
-    for (int l = 1; l <= levels; ++l) {
+    for (int l = first; l <= levels; ++l) {
         rg_fwd_level_args a{};
         a.filter = filter;
         a.cw = d.w[l - 1]; a.ch = d.h[l - 1]; a.nplanes = nplanes;
@@ -314,6 +323,68 @@ static int dwt_fwd_impl(rg_context *c, int filter, const void *in, int w, int h,
         if (rc) return rc;
     }
     return RG_SUCCESS;
+}
+
+// Colour stage + forward transform from the interleaved uint8 image: the
+// fused first level where rg_fwd1_colour_ok takes the shape (planes left
+// unwritten, *fused = true), else the colour kernel into `planes` and the
+// level kernels from there.  scratch: the LL pyramid (rg_dwt_scratch_bytes).
+// The choice depends on the shape alone, as rg_dwt_fwd_u8_scratch_bytes's
+// does: a caller's buffer sized by it always covers the path taken.
+static int dwt_fwd_u8_impl(rg_context *c, const uint8_t *src, int w, int h, int ncomp, int colour, int filter,
+                           int levels, const float *inv, void *dq, void *scratch, void *planes, bool *fused)
+{
+    int rc = check_levels(w, h, levels);
+    if (rc) return rc;
+    const bool f = levels >= 1 && rg_fwd1_colour_ok(w, h, ncomp, colour, filter);
+    if (fused) *fused = f;
+    if (!f) {
+        RG_CHECK(planes, "colour planes buffer required for this shape");
+        rc = rg_launch_colour_fwd(c->stream, src, w, h, ncomp, colour, filter == RG_FILTER_97, planes);
+        if (rc) return rc;
+        return dwt_fwd_impl(c, filter, planes, w, h, ncomp, levels, inv, 0, dq, 0, scratch);
+    }
+    RG_CHECK(filter == RG_FILTER_53 || inv, "9/7 quantisation needs inv_step");
+    RG_CHECK(scratch || levels == 1, "scratch buffer required");
+    const rg_level_dims d = rg_dims(w, h, 1);
+    rg_fwd1c_args a{};
+    a.src = src; a.w = w; a.h = h; a.ncomp = ncomp; a.colour = colour; a.filter = filter;
+    a.ll_out = levels > 1 ? (char *)scratch + rg_ll_offset(w, h, ncomp, 1) : nullptr;
+    a.ll_pitch = d.w[1]; a.ll_ps = (long)d.w[1] * d.h[1];
+    a.q = (int32_t *)dq; a.qp = w; a.q_ps = (long)w * h;
+    if (filter == RG_FILTER_97) {
+        const int b = 1 + 3 * (levels - 1);   // HL of level 1
+        a.inv_hl = inv[b]; a.inv_lh = inv[b + 1]; a.inv_hh = inv[b + 2]; a.inv_ll = inv[0];
+    }
+    rc = rg_launch_fwd1_colour(c->stream, a);
+    if (rc || levels == 1) return rc;
+    return dwt_fwd_impl(c, filter, nullptr, w, h, ncomp, levels, inv, 0, dq, 0, scratch, 2);
+}
+
+extern "C" size_t rg_dwt_fwd_u8_scratch_bytes(int w, int h, int ncomp, int colour, int filter)
+{
+    if (w <= 0 || h <= 0 || ncomp <= 0) return 512;
+    size_t n = rg_dwt_scratch_bytes(w, h, ncomp, filter);
+    if (!rg_fwd1_colour_ok(w, h, ncomp, colour, filter)) n += (size_t)w * h * ncomp * 4 + 256;
+    return n;
+}
+
+extern "C" int rg_dwt_fwd_u8(rg_context *c, const uint8_t *src, int w, int h, int ncomp, int colour, int filter,
+                             int levels, const float *inv_step, int32_t *dq, void *scratch)
+{
+    RG_CHECK(c && src && dq && scratch, "rg_dwt_fwd_u8: null argument");
+    RG_CHECK(w > 0 && h > 0 && ncomp >= 1 && ncomp <= 4, "rg_dwt_fwd_u8: bad shape %dx%dx%d", w, h, ncomp);
+    RG_CHECK(filter == RG_FILTER_53 || filter == RG_FILTER_97, "bad filter %d", filter);
+    RG_CHECK(colour == RG_COLOUR_NONE || (ncomp == 3 && ((colour == RG_COLOUR_RCT && filter == RG_FILTER_53) ||
+                                                         (colour == RG_COLOUR_ICT && filter == RG_FILTER_97))),
+             "colour transform %d does not pair with filter %d and %d components", colour, filter, ncomp);
+    RG_CHECK(((uintptr_t)src & 3) == 0, "rg_dwt_fwd_u8: source must be 4-byte aligned");
+    RG_CHECK(levels >= 1, "rg_dwt_fwd_u8: levels %d, at least 1", levels);
+    RG_HIP(hipSetDevice(c->device));
+    // the planes of an unfused shape sit after the LL pyramid, 256-byte aligned
+    const size_t ll_bytes = rg_dwt_scratch_bytes(w, h, ncomp, filter);
+    void *planes = (char *)scratch + ((ll_bytes + 255) & ~(size_t)255);
+    return dwt_fwd_u8_impl(c, src, w, h, ncomp, colour, filter, levels, inv_step, dq, scratch, planes, nullptr);
 }
 
 extern "C" int rg_dwt_fwd(rg_context *c, int filter, const void *in, int w, int h, int nplanes, int levels,
@@ -522,6 +593,12 @@ struct rg_encoder {
     // that reads it (event used[k]); consecutive encodes alternate k
     hipEvent_t ev_up[2] = {nullptr, nullptr}, ev_used[2] = {nullptr, nullptr};
     int in_slot = 0, used_rec[2] = {0, 0};
+    // level-0 planes: rg_encoder_encode_u8 fuses the colour stage into the
+    // first DWT level and leaves `planes` unwritten; they are computed from
+    // the encode's source (src_dev, input slot src_slot or -1: the caller's
+    // device buffer) only when asked for (ensure_planes)
+    int planes_valid = 0, src_slot = -1;
+    const uint8_t *src_dev = nullptr;
     // events of the overlapped phased tier-1 (created on first use; its side
     // streams belong to the context)
     hipEvent_t ev_in[3] = {nullptr, nullptr, nullptr}, ev_out[3] = {nullptr, nullptr, nullptr};
@@ -748,20 +825,60 @@ extern "C" int rg_encoder_encode_u8(rg_encoder *e, const uint8_t *src, int on_de
         RG_HIP(hipStreamWaitEvent(c->stream, e->ev_up[slot], 0));
         d_src = (const uint8_t *)e->input[slot].p;
     }
-    if ((rc = e->planes.ensure(npix * ncomp * 4))) return rc;
     if ((rc = e->q.ensure(npix * ncomp * 4))) return rc;
     if ((rc = e->scratch.ensure(rg_dwt_scratch_bytes(w, h, ncomp, p->filter)))) return rc;
-    rc = rg_launch_colour_fwd(e->ctx->stream, d_src, w, h, ncomp, p->colour, p->filter == RG_FILTER_97, e->planes.p);
-    if (rc) return rc;
-    if (slot >= 0) {
-        RG_HIP(hipEventRecord(e->ev_used[slot], e->ctx->stream));
-        e->used_rec[slot] = 1;
+    e->src_dev = d_src; e->src_slot = slot;
+    if (p->ll_fp16 || p->levels == 0 || !rg_fwd1_colour_ok(w, h, ncomp, p->colour, p->filter) ||
+        !rg_env_int("RG_DWT_FUSED", 1)) {
+        // FP16 inter-level LL, no transform, or a shape the fused first level
+        // does not take: colour planes first
+        if ((rc = e->planes.ensure(npix * ncomp * 4))) return rc;
+        rc = rg_launch_colour_fwd(e->ctx->stream, d_src, w, h, ncomp, p->colour, p->filter == RG_FILTER_97,
+                                  e->planes.p);
+        if (rc) return rc;
+        if (slot >= 0) {
+            RG_HIP(hipEventRecord(e->ev_used[slot], e->ctx->stream));
+            e->used_rec[slot] = 1;
+        }
+        e->planes_valid = 1;
+        rc = dwt_fwd_impl(e->ctx, p->filter, e->planes.p, w, h, ncomp, p->levels, e->inv, p->ll_fp16, e->q.p, 0,
+                          e->scratch.p);
+    } else {
+        // the colour stage inside the first DWT level: the image is read once
+        // and the planes are computed only if asked for (ensure_planes)
+        rc = dwt_fwd_u8_impl(e->ctx, d_src, w, h, ncomp, p->colour, p->filter, p->levels, e->inv, e->q.p,
+                             e->scratch.p, nullptr, nullptr);
+        if (slot >= 0) {   // the transform has read the input: a later upload into it may go ahead
+            RG_HIP(hipEventRecord(e->ev_used[slot], e->ctx->stream));
+            e->used_rec[slot] = 1;
+        }
+        e->planes_valid = 0;
     }
-    rc = dwt_fwd_impl(e->ctx, p->filter, e->planes.p, w, h, ncomp, p->levels, e->inv, p->ll_fp16, e->q.p, 0,
-                      e->scratch.p);
     if (rc) return rc;
     e->ran = 1;
     return run_t1(e);
+}
+
+// The level-0 planes of an rg_encoder_encode_u8 encode, computed on first
+// request from the image that encode read (its own input buffer for a host
+// source, which stays untouched until the encode after next; the caller's
+// device buffer otherwise, which must still hold the image).
+static int ensure_planes(rg_encoder *e)
+{
+    if (e->planes_valid) return RG_SUCCESS;
+    if (!e->src_dev) return rg_fail(RG_ERR_NOT_READY, "no level-0 planes and no source image");
+    const size_t npix = (size_t)e->w * e->h;
+    int rc;
+    if ((rc = e->planes.ensure(npix * e->ncomp * 4))) return rc;
+    rc = rg_launch_colour_fwd(e->ctx->stream, e->src_dev, e->w, e->h, e->ncomp, e->colour,
+                              e->filter == RG_FILTER_97, e->planes.p);
+    if (rc) return rc;
+    if (e->src_slot >= 0) {   // a later upload into this input waits for this read too
+        RG_HIP(hipEventRecord(e->ev_used[e->src_slot], e->ctx->stream));
+        e->used_rec[e->src_slot] = 1;
+    }
+    e->planes_valid = 1;
+    return RG_SUCCESS;
 }
 
 // Host conversion of T components into the kernels' sample type (float for
@@ -802,6 +919,7 @@ extern "C" int rg_encoder_load_planar(rg_encoder *e, const void *const *comps, i
     if (rc) return rc;
     if ((rc = e->planes.ensure(npix * ncomp * 4))) return rc;
     RG_HIP(hipMemcpyAsync(e->planes.p, e->staging.p, npix * ncomp * 4, hipMemcpyHostToDevice, e->ctx->stream));
+    e->planes_valid = 1; e->src_dev = nullptr; e->src_slot = -1;
     e->w = w; e->h = h; e->ncomp = ncomp; e->lossy = lossy; e->bitdepth = precision > 0 ? precision : 8;
     e->filter = lossy ? RG_FILTER_97 : RG_FILTER_53;
     e->colour = RG_COLOUR_NONE;
@@ -818,6 +936,7 @@ extern "C" int rg_encoder_dwt(rg_encoder *e, int levels, int output_dwt, double 
     RG_CHECK(!e->lossy || (delta > 0 && std::isfinite(delta)), "bad quantisation step");
     const size_t npix = (size_t)e->w * e->h;
     int rc;
+    if ((rc = ensure_planes(e))) return rc;
     if ((rc = e->q.ensure(npix * e->ncomp * 4))) return rc;
     if ((rc = e->scratch.ensure(rg_dwt_scratch_bytes(e->w, e->h, e->ncomp, e->filter)))) return rc;
     e->levels = levels;
@@ -1032,6 +1151,10 @@ extern "C" int rg_encoder_dwt_in(rg_encoder *e, int level, void **d_ptr, int *w,
         if (level >= e->levels)
             return rg_fail(RG_ERR_INVALID, "level %d: the %d-level transform has inputs 0..%d", level, e->levels,
                            e->levels - 1);
+    }
+    if (level == 0) {
+        const int rc = ensure_planes(e);
+        if (rc) return rc;
     }
     const rg_level_dims d = rg_dims(e->w, e->h, level);
     *d_ptr = level == 0 ? e->planes.p : (char *)e->scratch.p + rg_ll_offset(e->w, e->h, e->ncomp, level);

@@ -12,25 +12,13 @@
 // coalesced, contiguous wavefront access.  No LDS.
 #include <algorithm>
 #include <type_traits>
+#include "colour_px.h"
 #include "rg_internal.h"
 
 namespace {
 
 constexpr int QPT = 4;               // forward: 4-pixel quads per thread
 constexpr int SPAN = 256 * QPT * 4;  // forward: pixels per 256-thread workgroup
-
-__device__ __forceinline__ float ict_y(float r, float g, float b)
-{
-    return __builtin_fmaf((float)0.114, b, __builtin_fmaf((float)0.587, g, (float)0.299 * r));
-}
-__device__ __forceinline__ float ict_cb(float r, float g, float b)
-{
-    return __builtin_fmaf((float)0.5, b, __builtin_fmaf((float)-0.33126, g, (float)-0.16875 * r));
-}
-__device__ __forceinline__ float ict_cr(float r, float g, float b)
-{
-    return __builtin_fmaf((float)-0.08131, b, __builtin_fmaf((float)-0.41869, g, (float)0.5 * r));
-}
 
 // Streaming (non-temporal) 16-byte and 4-byte stores: the planes and images
 // these kernels write are read by the next kernel, long after they would
@@ -55,25 +43,7 @@ __device__ __forceinline__ uint8_t out_u8f(float x)
     return clamp_u8((int)floorf(s + 0.5f));
 }
 
-// MODE: 0 shift only, 1 RCT, 2 ICT.  FLT: float planes (else int32).
-// Forward transform of one level-shifted pixel v[0..NC) into o[0..NC).
-template <int NC, int MODE, bool FLT, class T>
-__device__ __forceinline__ void px_fwd(const int (&v)[NC], T (&o)[NC])
-{
-    if constexpr (MODE == 2) {
-        const float r = (float)v[0], g = (float)v[1], b = (float)v[2];
-        o[0] = ict_y(r, g, b);
-        o[1] = ict_cb(r, g, b);
-        o[2] = ict_cr(r, g, b);
-    } else if constexpr (MODE == 1) {
-        o[0] = (T)((v[0] + 2 * v[1] + v[2]) >> 2);
-        o[1] = (T)(v[2] - v[1]);
-        o[2] = (T)(v[0] - v[1]);
-    } else {
-#pragma unroll
-        for (int c = 0; c < NC; ++c) o[c] = (T)v[c];
-    }
-}
+using rg_px::px_fwd;
 
 // One quad: pixels [p, p+4) (p < npix, p a multiple of 4).  A full quad reads
 // NC aligned dwords (4 pixels x NC bytes) and writes one 16-byte vector per

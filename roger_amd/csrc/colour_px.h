@@ -1,0 +1,43 @@
+// Per-pixel forward colour transforms (T.800 G.2 RCT, G.3 ICT) after the DC
+// level shift, shared by the colour kernels and the fused colour + first DWT
+// level kernel so both compute identical values.  FP32 operation order is
+// the CPU oracle's (fmaf chains).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace rg_px {
+
+__device__ __forceinline__ float ict_y(float r, float g, float b)
+{
+    return __builtin_fmaf((float)0.114, b, __builtin_fmaf((float)0.587, g, (float)0.299 * r));
+}
+__device__ __forceinline__ float ict_cb(float r, float g, float b)
+{
+    return __builtin_fmaf((float)0.5, b, __builtin_fmaf((float)-0.33126, g, (float)-0.16875 * r));
+}
+__device__ __forceinline__ float ict_cr(float r, float g, float b)
+{
+    return __builtin_fmaf((float)-0.08131, b, __builtin_fmaf((float)-0.41869, g, (float)0.5 * r));
+}
+
+// MODE: 0 shift only, 1 RCT, 2 ICT.  Forward transform of one level-shifted
+// pixel v[0..NC) into o[0..NC) (T: float for 9/7 planes, int32 for 5/3).
+template <int NC, int MODE, bool FLT, class T>
+__device__ __forceinline__ void px_fwd(const int (&v)[NC], T (&o)[NC])
+{
+    if constexpr (MODE == 2) {
+        const float r = (float)v[0], g = (float)v[1], b = (float)v[2];
+        o[0] = ict_y(r, g, b);
+        o[1] = ict_cb(r, g, b);
+        o[2] = ict_cr(r, g, b);
+    } else if constexpr (MODE == 1) {
+        o[0] = (T)((v[0] + 2 * v[1] + v[2]) >> 2);
+        o[1] = (T)(v[2] - v[1]);
+        o[2] = (T)(v[0] - v[1]);
+    } else {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) o[c] = (T)v[c];
+    }
+}
+
+}  // namespace rg_px

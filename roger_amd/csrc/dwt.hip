@@ -36,8 +36,15 @@
 // chosen per lane; then the horizontal synthesis (T.800 2D_SR: rows first)
 // and the vertical one, whose column items store 64-wide row runs
 // (non-temporal).
+//
+// The native encoder's first level also takes the colour stage
+// (fwd1_colour_kernel): the workgroup reads its tile of the interleaved
+// uint8 image, level shifts and colour transforms each pixel once into LDS
+// and lifts from there, so the FP32 colour planes never go to HBM (3 B/px
+// read instead of 3 B/px + 12 B/px written + 12 B/px read again).
 #include <hip/hip_fp16.h>
 #include <type_traits>
+#include "colour_px.h"
 #include "rg_internal.h"
 
 namespace {
@@ -374,6 +381,198 @@ __device__ __forceinline__ void fwd_tile_plain97(const rg_fwd_level_args &a, int
     }
 }
 
+// ==================== first level fused with the colour stage ====================
+// One TW x SEG tile of the interleaved uint8 image, every component:
+//   1. staging: items (row, 4-pixel quad) over the tile's rows with the
+//      vertical halo (mirrored rows loaded in their place) and columns
+//      [x0 - 4, x0 + TW + 4) clipped to the image (quads start 4-pixel
+//      aligned, so a quad is NC aligned dwords); every load of a thread is
+//      issued before any is used; each pixel is level shifted and colour
+//      transformed once (rg_px::px_fwd, the colour kernels' arithmetic) and
+//      its components go to NC float / int planes in LDS; the mirrored halo
+//      columns at the image sides lie inside that span;
+//   2. column items (component, column): read the column from the staged
+//      plane, lift it in registers, and -- after a barrier, all reads done --
+//      write the SEG results over the staging as component k's tile;
+//   3. row items (component, row, 16-column segment) lift and de-interleave
+//      as fwd_tile does;
+//   4. stores: thread = one column of the de-interleaved tile and one row of
+//      every RS, over every component; a wavefront stores 64 consecutive
+//      outputs (non-temporal).
+// The level kernels are bound by VALU issue about as much as by HBM (lifting
+// alone is ~27 operations per sample and component with the halos), so the
+// items are laid out for full wavefronts and nothing is converted twice.
+// LDS: NC x (SEG + 2H) x LPS words.
+template <int TW> constexpr int fwd1c_lps() { return lcol(TW + 8) + 1; }
+template <int F, int NC, int TW> constexpr int fwd1c_lds_bytes() { return NC * (SEG + 2 * Filt<F>::H) * fwd1c_lps<TW>() * 4; }
+
+template <int F, int NC, int MODE, int TW, int WGS>
+__device__ __forceinline__ void fwd1_colour_tile(const rg_fwd1c_args &a, int bx, int by, char *lds)
+{
+    using T = typename Filt<F>::T;
+    constexpr int H = Filt<F>::H;
+    constexpr int TH = SEG;
+    constexpr int CW = TW + 2 * H, RH = TH + 2 * H;
+    constexpr int SW = TW + 8;                           // staged columns: image columns x0 - 4 .. x0 + TW + 4
+    constexpr int LPS = fwd1c_lps<TW>();                 // staged and tile row pitch (words)
+    constexpr int N = SEG + 2 * H;
+    constexpr int HS = TW / SEG;
+    constexpr int RS = WGS / TW;                         // rows per store round
+    static_assert(TW % 64 == 0 && WGS % TW == 0 && TH % RS == 0 && NC * CW <= WGS, "fused tile / workgroup shape");
+    T *fst = reinterpret_cast<T *>(lds);                 // component k, staged row r at (k * RH + r) * LPS
+    T *tile = fst;                                       // then component k, row j at (k * TH + j) * LPS
+    const int x0 = bx * TW, y0 = by * TH;
+    const int w = a.w, h = a.h;
+    const int xs = max(x0 - 4, 0), xe = min(x0 + TW + 4, w);   // staged span (both multiples of 4)
+    const long rowb = (long)w * NC;
+
+    // 1. staging: quads (row r, quad q), image column x0 - 4 + 4q; item i of
+    //    round t is threadIdx.x + t * WGS, its (r, q) stepped from round 0's
+    constexpr int NQ = SW / 4, NIT = RH * NQ, NI = (NIT + WGS - 1) / WGS;
+    constexpr int DR = WGS / NQ, DQ = WGS % NQ;
+    int rq[NI][2];
+    rq[0][0] = threadIdx.x / NQ;
+    rq[0][1] = threadIdx.x - rq[0][0] * NQ;
+#pragma unroll
+    for (int t = 1; t < NI; ++t) {
+        const int q = rq[t - 1][1] + DQ;
+        rq[t][0] = rq[t - 1][0] + DR + (q >= NQ ? 1 : 0);
+        rq[t][1] = q >= NQ ? q - NQ : q;
+    }
+    // byte offsets fit 32 bits for images below 2 GiB (the launcher checks)
+    const uint32_t rowb32 = (uint32_t)rowb;
+    uint32_t wq[NI][NC];
+#pragma unroll
+    for (int t = 0; t < NI; ++t) {
+        const int i = threadIdx.x + t * WGS;
+        const int r = rq[t][0], q = rq[t][1];
+        const int x = x0 - 4 + 4 * q;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) wq[t][k] = 0;
+        if (i < NIT && x >= xs && x < xe) {
+            const uint32_t off = (uint32_t)mirror1(y0 - H + r, h) * rowb32 + (uint32_t)(x * NC);
+            const uint32_t *p = reinterpret_cast<const uint32_t *>(a.src + off);
+#pragma unroll
+            for (int k = 0; k < NC; ++k) wq[t][k] = p[k];
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NI; ++t) {
+        const int i = threadIdx.x + t * WGS;
+        const int r = rq[t][0], q = rq[t][1];
+        const int x = x0 - 4 + 4 * q;
+        if (i < NIT && x >= xs && x < xe) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                int v[NC];
+#pragma unroll
+                for (int k = 0; k < NC; ++k) v[k] = (int)((wq[t][(NC * j + k) >> 2] >> (((NC * j + k) & 3) * 8)) & 0xFFu) - 128;
+                T o[NC];
+                rg_px::px_fwd<NC, MODE, F == RG_FILTER_97>(v, o);
+#pragma unroll
+                for (int k = 0; k < NC; ++k) fst[(k * RH + r) * LPS + lcol(4 * q + j)] = o[k];
+            }
+        }
+    }
+    __syncthreads();
+
+    // 2. columns: items (component, column), component slowest; one round
+    const int k2 = threadIdx.x / CW, c2 = threadIdx.x - k2 * CW;
+    const bool colt = threadIdx.x < NC * CW;
+    T e[N];
+    if (colt) {
+        // every column an output inside the image needs mirrors into [xs, xe);
+        // the clamp only keeps the reads of the other columns in the span
+        const int x = min(max(mirror1(x0 - H + c2, w), xs), xe - 1);
+        const T *col = fst + k2 * RH * LPS + lcol(x - (x0 - 4));
+#pragma unroll
+        for (int r = 0; r < N; ++r) e[r] = col[r * LPS];
+        if constexpr (F == RG_FILTER_53) fwd53(e); else fwd97(e, true);
+    }
+    __syncthreads();
+    if (colt) {
+#pragma unroll
+        for (int j = 0; j < SEG; ++j) tile[(k2 * TH + j) * LPS + lcol(c2)] = e[H + j];
+    }
+    __syncthreads();
+
+    // 3. rows: items (component x row, segment), segment fastest
+    constexpr int NROW = NC * TH * HS;
+    constexpr int RR = (NROW + WGS - 1) / WGS;
+    T o[RR][SEG];
+#pragma unroll
+    for (int u2 = 0; u2 < RR; ++u2) {
+        const int it = threadIdx.x + u2 * WGS;
+        if (it < NROW) {
+            const int kk = it % HS, rr = it / HS;
+            T er[N];
+#pragma unroll
+            for (int j = 0; j < N; ++j) er[j] = tile[rr * LPS + lcol(kk * SEG + j)];
+            if constexpr (F == RG_FILTER_53) fwd53(er); else fwd97(er, true);
+#pragma unroll
+            for (int j = 0; j < SEG; ++j) o[u2][j] = er[H + j];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u2 = 0; u2 < RR; ++u2) {
+        const int it = threadIdx.x + u2 * WGS;
+        if (it < NROW) {
+            const int kk = it % HS, rr = it / HS;
+#pragma unroll
+            for (int j = 0; j < SEG; ++j)
+                tile[rr * LPS + lcol((j & 1) * (TW / 2) + kk * (SEG / 2) + (j >> 1))] = o[u2][j];
+        }
+    }
+    __syncthreads();
+
+    // 4. stores.  Wavefront w takes half band (w & 1) of the de-interleaved
+    //    tile and rows of one parity ((w >> 1) & 1; y0 is even), so its band,
+    //    quantiser and destination are uniform; its lanes cover LPH = TW / 2
+    //    columns of HPW = 64 / LPH rows, so a store instruction writes one or
+    //    two runs of LPH consecutive outputs.  Round j covers RPR rows of each
+    //    half band.
+    constexpr int LPH = TW / 2, HPW = 64 / LPH, NW = WGS / 64;
+    constexpr int RPR = (NW / 2) * HPW;
+    static_assert(NW % 4 == 0 && TH % RPR == 0, "store mapping");
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const int half = wv & 1, grp = wv >> 1;
+    const int par = grp & 1, gsub = grp >> 1;        // row parity; group among the waves of that parity
+    const int cc = lane % LPH, rsub = lane / LPH;
+    const int wl = (w + 1) >> 1, hl = (h + 1) >> 1;
+    const int xo = (x0 >> 1) + cc;
+    const bool col_in = 2 * xo + half < w;
+    const bool lowrow = par == 0;
+    const bool to_ll = lowrow && !half && a.ll_out;
+    const long pitch = to_ll ? a.ll_pitch : a.qp;
+    const float inv = lowrow ? (half ? a.inv_hl : a.inv_ll) : (half ? a.inv_hh : a.inv_lh);
+    // output row of round j: (y0 + r) / 2 with r = par + 2 * (gsub * HPW + rsub + j * RPR / 2)
+    const int orow0 = (y0 >> 1) + gsub * HPW + rsub;
+    const long first = to_ll ? (long)orow0 * pitch + xo
+                             : (long)(lowrow ? orow0 : hl + orow0) * pitch + (half ? wl + xo : xo);
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        T *llk = to_ll ? reinterpret_cast<T *>(a.ll_out) + k * a.ll_ps + first : nullptr;
+        int32_t *qk = to_ll ? nullptr : a.q + k * a.q_ps + first;
+#pragma unroll
+        for (int j = 0; j < TH / RPR; ++j) {
+            const int r = par + 2 * (gsub * HPW + rsub + j * (RPR / 2));
+            if (!col_in || y0 + r >= h) continue;
+            const T v = tile[(k * TH + r) * LPS + lcol(half * (TW / 2) + cc)];
+            const long step = (long)j * (RPR / 2) * pitch;
+            if (to_ll) {
+                st_nt(llk + step, v);
+            } else if constexpr (F == RG_FILTER_53) {
+                st_nt(qk + step, v);
+            } else {
+                // sign(v) * floor(|v| * inv) == truncation of v * inv (IEEE products are sign-symmetric)
+                st_nt(qk + step, (int)(v * inv));
+            }
+        }
+    }
+}
+
 // ============================ inverse ============================
 // One TW x TH tile of the level region: stage the tile and its halo (rows
 // y0-H .. y0+TH+H, columns x0-H .. x0+TW+H, symmetric extension) from the
@@ -558,6 +757,16 @@ __global__ void __launch_bounds__(FWG) fwd_level_kernel(rg_fwd_level_args a)
     else fwd_tile<F, FTW, FTH, FWG>(a, plane, bx, by, lds);
 }
 
+// Fused colour + level 1, one workgroup per TW x 16 tile of the image.
+template <int F, int NC, int MODE, int TW, int WGS>
+__global__ void __launch_bounds__(WGS) fwd1_colour_kernel(rg_fwd1c_args a)
+{
+    __shared__ __attribute__((aligned(16))) char lds[fwd1c_lds_bytes<F, NC, TW>()];
+    int plane, bx, by;
+    xcd_tile(plane, bx, by);
+    fwd1_colour_tile<F, NC, MODE, TW, WGS>(a, bx, by, lds);
+}
+
 template <int F, bool PLAIN = false>
 __global__ void __launch_bounds__(IWG) inv_level_kernel(rg_inv_level_args a)
 {
@@ -629,6 +838,56 @@ int rg_launch_fwd97_level(hipStream_t s, const rg_fwd_level_args &a)
         fwd_level_kernel<RG_FILTER_97, 3><<<grid, dim3(FWG), 0, s>>>(a);
     else
         fwd_level_kernel<RG_FILTER_97><<<grid, dim3(FWG), 0, s>>>(a);
+    RG_LAUNCH_CHECK();
+    return RG_SUCCESS;
+}
+
+bool rg_fwd1_colour_ok(int w, int h, int ncomp, int colour, int filter)
+{
+    const int halo = filter == RG_FILTER_97 ? 4 : 2;
+    if (ncomp != 1 && ncomp != 3) return false;
+    if (colour != RG_COLOUR_NONE && !(ncomp == 3 && ((colour == RG_COLOUR_ICT && filter == RG_FILTER_97) ||
+                                                     (colour == RG_COLOUR_RCT && filter == RG_FILTER_53))))
+        return false;
+    return ((long)w * ncomp) % 4 == 0 && w > halo && h > halo && (long)w * h * ncomp < (1L << 31);
+}
+
+namespace {
+// Fused tile shapes: 64 x 16 with 256 threads (22 KB of LDS for three
+// components, 7 workgroups a CU) or 128 x 16 with 512 threads (42 KB, 3 a
+// CU; RG_DWT_FUSED_TW=128).  4096x4096 RGB ICT + 9/7 with caches cold: 66 us
+// against 85 us.
+template <int F, int NC, int MODE>
+void launch_fwd1c(hipStream_t s, const rg_fwd1c_args &a)
+{
+    static const int tw = [] { const char *e = getenv("RG_DWT_FUSED_TW"); return e && atoi(e) == 128 ? 128 : 64; }();
+    if (tw == 128) {
+        dim3 grid((a.w + 127) / 128, (a.h + SEG - 1) / SEG, 1);
+        fwd1_colour_kernel<F, NC, MODE, 128, 512><<<grid, dim3(512), 0, s>>>(a);
+    } else {
+        dim3 grid((a.w + 63) / 64, (a.h + SEG - 1) / SEG, 1);
+        fwd1_colour_kernel<F, NC, MODE, 64, 256><<<grid, dim3(256), 0, s>>>(a);
+    }
+}
+}  // namespace
+
+int rg_launch_fwd1_colour(hipStream_t s, const rg_fwd1c_args &a)
+{
+    RG_CHECK(rg_fwd1_colour_ok(a.w, a.h, a.ncomp, a.colour, a.filter),
+             "fused colour + DWT: unsupported shape %dx%dx%d (colour %d, filter %d)", a.w, a.h, a.ncomp, a.colour,
+             a.filter);
+    RG_CHECK(((uintptr_t)a.src & 3) == 0, "fused colour + DWT: source must be 4-byte aligned");
+    RG_CHECK((long)a.w * a.h * a.ncomp < (1L << 31), "fused colour + DWT: image of %dx%dx%d beyond 2 GiB", a.w, a.h,
+             a.ncomp);
+    if (a.filter == RG_FILTER_97) {
+        if (a.ncomp == 1) launch_fwd1c<RG_FILTER_97, 1, 0>(s, a);
+        else if (a.colour == RG_COLOUR_ICT) launch_fwd1c<RG_FILTER_97, 3, 2>(s, a);
+        else launch_fwd1c<RG_FILTER_97, 3, 0>(s, a);
+    } else {
+        if (a.ncomp == 1) launch_fwd1c<RG_FILTER_53, 1, 0>(s, a);
+        else if (a.colour == RG_COLOUR_RCT) launch_fwd1c<RG_FILTER_53, 3, 1>(s, a);
+        else launch_fwd1c<RG_FILTER_53, 3, 0>(s, a);
+    }
     RG_LAUNCH_CHECK();
     return RG_SUCCESS;
 }

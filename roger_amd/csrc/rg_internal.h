@@ -105,6 +105,25 @@ struct rg_fwd_level_args {
 int rg_launch_fwd53_level(hipStream_t s, const rg_fwd_level_args &a);
 int rg_launch_fwd97_level(hipStream_t s, const rg_fwd_level_args &a);
 
+// First forward level fused with the colour stage: the interleaved uint8
+// image (w x h x ncomp, 4-byte aligned rows of w * ncomp bytes) is level
+// shifted and colour transformed in registers and lifted at once; every
+// component's level-1 outputs go out as with rg_fwd_level_args (LL to
+// ll_out, or quantised with the bands into q when ll_out is null).
+struct rg_fwd1c_args {
+    const uint8_t *src;
+    int w, h, ncomp, colour, filter;
+    void *ll_out;           // float (9/7) / int32 (5/3)
+    long ll_pitch, ll_ps;
+    int32_t *q;
+    long qp, q_ps;
+    float inv_ll, inv_hl, inv_lh, inv_hh;  // 9/7 quantisation of level 1
+};
+// Shapes the fused kernel takes: 1 or 3 components, w * ncomp a multiple of
+// 4, both sides longer than the filter's halo (4 for 9/7, 2 for 5/3).
+bool rg_fwd1_colour_ok(int w, int h, int ncomp, int colour, int filter);
+int rg_launch_fwd1_colour(hipStream_t s, const rg_fwd1c_args &a);
+
 // One inverse level: reads the four bands of a cw x ch region (LL from ll_in,
 // pitch ll_pitch; high bands from the Mallat plane q), writes the region to out.
 struct rg_inv_level_args {

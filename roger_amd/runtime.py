@@ -206,6 +206,19 @@ def dwt_fwd(ctx, filt, d_in, w, h, nplanes, levels, inv_step, ll_fp16, d_q, d_sc
                            _vp(d_q.ptr), _vp(d_scratch.ptr) if d_scratch else None))
 
 
+def dwt_fwd_u8_scratch_bytes(w, h, ncomp, colour, filt):
+    return lib().rg_dwt_fwd_u8_scratch_bytes(w, h, ncomp, colour, filt)
+
+
+def dwt_fwd_u8(ctx, d_src, w, h, ncomp, colour, filt, levels, inv_step, d_q, d_scratch):
+    """Colour stage + forward transform from the interleaved uint8 image
+    (first level fused where the shape allows)."""
+    inv = None if inv_step is None else np.ascontiguousarray(inv_step, np.float32)
+    check(lib().rg_dwt_fwd_u8(ctx.handle, _vp(d_src.ptr), w, h, ncomp, colour, filt, levels,
+                              None if inv is None else inv.ctypes.data_as(ctypes.c_void_p), _vp(d_q.ptr),
+                              _vp(d_scratch.ptr)))
+
+
 def dwt_fwd_noalloc(ctx, filt, d_in, w, h, nplanes, levels, inv_step, ll_fp16, d_q):
     """rg_dwt_fwd with the library's static per-GPU scratch (no allocation)."""
     inv = None if inv_step is None else np.ascontiguousarray(inv_step, np.float32)
@@ -453,7 +466,7 @@ class Encoder:
         ctx._own(self)
 
     def encode_u8(self, img, levels=5, reversible=True, cbw=64, cbh=64, delta=1.0 / 64, ll_fp16=False,
-                  d_src=None, w=None, h=None, ncomp=None, style=0):
+                  d_src=None, w=None, h=None, ncomp=None, style=0, colour=None):
         if d_src is None:
             img = np.ascontiguousarray(img, np.uint8)
             h, w = img.shape[:2]
@@ -467,7 +480,9 @@ class Encoder:
         p = EncodeParams()
         p.levels = levels
         p.filter = RG_FILTER_53 if reversible else RG_FILTER_97
-        p.colour = RG_COLOUR_NONE if ncomp != 3 else (RG_COLOUR_RCT if reversible else RG_COLOUR_ICT)
+        if colour is None:
+            colour = RG_COLOUR_NONE if ncomp != 3 else (RG_COLOUR_RCT if reversible else RG_COLOUR_ICT)
+        p.colour = colour
         p.cbw, p.cbh, p.delta, p.ll_fp16, p.bitdepth = cbw, cbh, float(delta), int(ll_fp16), 8
         p.style = int(style)
         self.shape = (w, h, ncomp)
@@ -483,6 +498,22 @@ class Encoder:
         self.ctx.sync()
         check(lib().rg_memcpy_d2h(self.ctx.handle, out.ctypes.data_as(ctypes.c_void_p), dq, out.nbytes))
         return out
+
+    def dwt_in(self, level, dtype):
+        """Input of DWT level `level` (getDwtIn): (ncomp, h, w) array of
+        `dtype` (float32 for 9/7, int32 for 5/3)."""
+        d = ctypes.c_void_p()
+        w, h = ctypes.c_int(), ctypes.c_int()
+        check(lib().rg_encoder_dwt_in(self.handle, level, ctypes.byref(d), ctypes.byref(w), ctypes.byref(h)))
+        out = np.empty((self.shape[2], h.value, w.value), dtype)
+        self.ctx.sync()
+        check(lib().rg_memcpy_d2h(self.ctx.handle, out.ctypes.data_as(ctypes.c_void_p), d, out.nbytes))
+        return out
+
+    def dwt(self, levels, delta=1.0 / 64):
+        """Stage path: the forward transform of the loaded image again
+        (OCLDWTForward::run over all levels), quantised."""
+        check(lib().rg_encoder_dwt(self.handle, levels, 0, float(delta)))
 
     def codeblocks(self):
         n = check(lib().rg_encoder_num_codeblocks(self.handle))

@@ -1,0 +1,132 @@
+"""GPU parity of the first DWT level fused with the colour stage (the native
+encode reads the interleaved uint8 image once): quantised coefficients,
+tier-1 codewords, the level-1 LL band and the on-demand level-0 planes
+against the oracle, bit-exact, over shapes that take the fused kernel (rows
+of a multiple of 4 bytes, sides longer than the filter halo: border, partial
+and single tiles) and shapes that fall back to the separate colour kernel."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from roger_amd import runtime as R
+from roger_amd._lib import RG_COLOUR_NONE
+
+pytestmark = pytest.mark.gpu
+
+# (w, h, ncomp): fused unless noted
+SHAPES = [
+    (8, 5, 1),        # one partial tile, 9/7 halo + 1 rows
+    (4, 100, 1),      # 5/3 only (9/7 needs w > 4: falls back)
+    (12, 17, 3),
+    (128, 16, 3),     # exactly one tile
+    (132, 40, 3),     # a second tile of 4 columns
+    (260, 301, 3),
+    (516, 20, 1),
+    (257, 300, 3),    # rows of 771 bytes: separate colour kernel
+    (1000, 3, 3),     # 3 rows: fused for 5/3, separate colour kernel for 9/7
+    (5, 5, 1),        # 5-byte rows: separate colour kernel
+]
+
+
+def rand_img(w, h, nc, seed):
+    rng = np.random.default_rng(seed)
+    shape = (h, w) if nc == 1 else (h, w, nc)
+    return rng.integers(0, 256, size=shape, dtype=np.uint8)
+
+
+def level0_planes(img, reversible, colour_none=False):
+    nc = 1 if img.ndim == 2 else img.shape[2]
+    if nc == 1:
+        return O.shift(img, np.int32 if reversible else np.float32)[None]
+    if colour_none:
+        return np.moveaxis(img.astype(np.int32) - 128, 2, 0).astype(np.int32 if reversible else np.float32)
+    return O.rct_fwd(img) if reversible else O.ict_fwd(img)
+
+
+@pytest.mark.parametrize("w,h,nc", SHAPES)
+@pytest.mark.parametrize("reversible", [True, False])
+@pytest.mark.parametrize("levels", [1, 2, 5])
+def test_fused_first_level(ctx, w, h, nc, reversible, levels):
+    img = rand_img(w, h, nc, w * 13 + h * 3 + nc)
+    enc = R.Encoder(ctx)
+    enc.encode_u8(img, levels=levels, reversible=reversible, cbw=32, cbh=32, delta=0.4)
+    rs, data, q = O.encode_image(img, levels=levels, reversible=reversible, cbw=32, cbh=32, delta=0.4)
+    np.testing.assert_array_equal(enc.coefficients(), q)
+    desc, res, blobs = enc.codeblocks()
+    assert blobs == data, "tier-1 codewords differ from the oracle"
+    planes = level0_planes(img, reversible)
+    if levels > 1:
+        # level-1 LL band (the next level's input), FP32 bit patterns for 9/7
+        ll = enc.dwt_in(1, np.int32 if reversible else np.float32)
+        for c in range(nc):
+            full = O.dwt53_fwd(planes[c], 1) if reversible else O.dwt97_fwd(planes[c], 1)
+            want = full[:(h + 1) // 2, :(w + 1) // 2]
+            np.testing.assert_array_equal(ll[c].view(np.uint32), want.view(np.uint32))
+    # level-0 planes, computed on request from the encode's source
+    got0 = enc.dwt_in(0, np.int32 if reversible else np.float32)
+    np.testing.assert_array_equal(got0.view(np.uint32), planes.view(np.uint32))
+    # the stage path transforms those planes again: same coefficients
+    enc.dwt(levels, delta=0.4)
+    np.testing.assert_array_equal(enc.coefficients(), q)
+    enc.close()
+
+
+@pytest.mark.parametrize("reversible", [True, False])
+def test_fused_three_components_no_colour(ctx, reversible):
+    """Three components with no colour transform (level shift only)."""
+    w, h = 140, 36
+    img = rand_img(w, h, 3, 77)
+    enc = R.Encoder(ctx)
+    enc.encode_u8(img, levels=3, reversible=reversible, cbw=32, cbh=32, delta=0.4, colour=RG_COLOUR_NONE)
+    planes = level0_planes(img, reversible, colour_none=True)
+    _, inv, _, _ = R.band_steps(3, 0.4)
+    for c in range(3):
+        if reversible:
+            want = O.dwt53_fwd(planes[c], 3)
+        else:
+            want = O.quantize(O.dwt97_fwd(planes[c], 3), 3, inv)
+        np.testing.assert_array_equal(enc.coefficients()[c], want)
+    enc.close()
+
+
+def test_fused_matches_separate_colour(ctx, monkeypatch):
+    """The fused kernel and the separate colour + level kernels give the same
+    coefficients and codewords (1032 x 517 RGB, 9/7, 6 levels)."""
+    w, h = 1032, 517
+    img = rand_img(w, h, 3, 5)
+    enc = R.Encoder(ctx)
+    enc.encode_u8(img, levels=6, reversible=False, style=7)
+    q1, (_, _, b1) = enc.coefficients(), enc.codeblocks()
+    monkeypatch.setenv("RG_DWT_FUSED", "0")
+    enc.encode_u8(img, levels=6, reversible=False, style=7)
+    q2, (_, _, b2) = enc.coefficients(), enc.codeblocks()
+    np.testing.assert_array_equal(q1, q2)
+    assert b1 == b2
+    enc.close()
+
+
+@pytest.mark.parametrize("w,h,nc", [(132, 40, 3), (257, 300, 3), (516, 20, 1), (5, 5, 1)])
+@pytest.mark.parametrize("reversible", [True, False])
+@pytest.mark.parametrize("env", [None, "0"])
+def test_dwt_fwd_u8_entry(ctx, w, h, nc, reversible, env, monkeypatch):
+    """rg_dwt_fwd_u8 (colour + transform from the uint8 image) against the
+    oracle, on fused and unfused shapes, with its workspace sized by
+    rg_dwt_fwd_u8_scratch_bytes; the encoder's RG_DWT_FUSED setting does not
+    change which path the entry point takes (its workspace fits either way)."""
+    from roger_amd._lib import RG_COLOUR_ICT, RG_COLOUR_RCT, RG_FILTER_53, RG_FILTER_97
+    if env is not None:
+        monkeypatch.setenv("RG_DWT_FUSED", env)
+    img = rand_img(w, h, nc, w + 3 * h)
+    levels = 4
+    filt = RG_FILTER_53 if reversible else RG_FILTER_97
+    colour = RG_COLOUR_NONE if nc == 1 else (RG_COLOUR_RCT if reversible else RG_COLOUR_ICT)
+    _, inv, _, _ = R.band_steps(levels, 0.4)
+    d_src = ctx.upload(img)
+    d_q = ctx.alloc(nc * w * h * 4)
+    d_s = ctx.alloc(R.dwt_fwd_u8_scratch_bytes(w, h, nc, colour, filt))
+    R.dwt_fwd_u8(ctx, d_src, w, h, nc, colour, filt, levels, None if reversible else inv, d_q, d_s)
+    got = d_q.read(np.int32, (nc, h, w))
+    planes = level0_planes(img, reversible)
+    for c in range(nc):
+        want = O.dwt53_fwd(planes[c], levels) if reversible else O.quantize(O.dwt97_fwd(planes[c], levels), levels, inv)
+        np.testing.assert_array_equal(got[c], want)

@@ -98,3 +98,33 @@ def test_no_oracle_in_product():
                 text = open(os.path.join(dirpath, f)).read()
                 for pat in ("import oracle", "from oracle", "liboracle", "oracle/"):
                     assert pat not in text, (os.path.join(dirpath, f), pat)
+
+
+def test_dwt_fwd_u8_scratch_covers_the_path_taken():
+    """rg_dwt_fwd_u8's workspace: the LL pyramid when the first level fuses
+    with the colour stage (rows of a multiple of 4 bytes, sides beyond the
+    filter halo), the pyramid plus 256-byte aligned colour planes otherwise;
+    the choice depends on the shape alone (no environment setting)."""
+    from roger_amd._lib import RG_COLOUR_ICT, RG_COLOUR_NONE, RG_COLOUR_RCT, RG_FILTER_53, RG_FILTER_97
+    cases = [(4096, 4096, 3, RG_COLOUR_ICT, RG_FILTER_97, True), (132, 40, 3, RG_COLOUR_RCT, RG_FILTER_53, True),
+             (8, 5, 1, RG_COLOUR_NONE, RG_FILTER_97, True), (4, 100, 1, RG_COLOUR_NONE, RG_FILTER_97, False),
+             (257, 300, 3, RG_COLOUR_ICT, RG_FILTER_97, False), (1000, 3, 3, RG_COLOUR_RCT, RG_FILTER_53, True),
+             (1000, 3, 3, RG_COLOUR_ICT, RG_FILTER_97, False),
+             (5, 5, 1, RG_COLOUR_NONE, RG_FILTER_53, False)]
+    for w, h, nc, col, filt, fused in cases:
+        base = R.scratch_bytes(w, h, nc, filt)
+        for env in ("1", "0"):
+            os.environ["RG_DWT_FUSED"] = env
+            try:
+                got = R.dwt_fwd_u8_scratch_bytes(w, h, nc, col, filt)
+            finally:
+                del os.environ["RG_DWT_FUSED"]
+            if fused:
+                assert got == base, (w, h, nc)
+            else:
+                assert got >= ((base + 255) & ~255) + w * h * nc * 4, (w, h, nc)
+
+
+def test_dwt_fwd_u8_refuses_zero_levels():
+    ctx_free = _lib.lib().rg_dwt_fwd_u8(None, None, 8, 8, 1, 0, 1, 0, None, None, None)
+    assert ctx_free == _lib.RG_ERR_INVALID
