@@ -403,12 +403,15 @@ def main():
     # HBM bytes per level-1 launch measured with rocprofv3 counters (FETCH_SIZE
     # and WRITE_SIZE scaled by a copy kernel of known size), as recorded in the
     # committed counter summary for this workload
-    traffic = None
+    traffic = valu_busy = None
     prof = os.path.join(ROOT, "profiles", "encode_4096rgb_pmc_counters.md")
     if W == 4096 and H == 4096 and os.path.exists(prof):
         import re
-        mt = re.search(r"^traffic_bytes_per_launch_fwd1_colour: (\d+)$", open(prof).read(), re.M)
+        ptxt = open(prof).read()
+        mt = re.search(r"^traffic_bytes_per_launch_fwd1_colour: (\d+)$", ptxt, re.M)
         traffic = int(mt.group(1)) if mt else None
+        mv = re.search(r"^valu_busy_fwd1_colour: ([0-9.]+)$", ptxt, re.M)
+        valu_busy = float(mv.group(1)) if mv else None
     colour_gbs = 15.0 * W * H / (t_colour * 1e-3) / 1e9
 
     out = {
@@ -428,7 +431,10 @@ def main():
                      "algorithmic_bytes": f1_bytes, "avg_launch_ms": round(t_front1, 4),
                      "timing": "%d launches rotating over %d buffer sets (cold last-level cache)" % (n_rot, ROT),
                      "warm_avg_launch_ms": round(t_front1_warm, 4),
-                     "warm_frac": round(f1_bytes / (t_front1_warm * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+                     "warm_frac": round(f1_bytes / (t_front1_warm * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     # the same launch's VALU busy fraction (SQ_ACTIVE_INST_VALU per CU over its cycles, from
+                     # the committed counter summary): the 9/7 lifting keeps the vector unit about as busy as HBM
+                     "valu_busy": valu_busy},
         "hbm_copy_ceiling_GBps": round(copy_gbs, 1),
         "hbm_copy_warm_GBps": round(2.0 * cp_n / (t_copy_warm * 1e-3) / 1e9, 1),
         "stages_ms": {"cache": "HBM-bound stages cold (rotating buffer sets); *_warm: same buffers",
