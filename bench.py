@@ -390,6 +390,13 @@ def main():
     step1 = R.band_steps(1, DELTA)[0]
     t_idwt_l1 = timed_rot(lambda b: R.dwt_inv(ctx, RG_FILTER_97, b["q"], W, H, NC, 1, step1, b["pl"], b["s"]))
     t_idwt = timed_rot(lambda b: R.dwt_inv(ctx, RG_FILTER_97, b["q"], W, H, NC, LEVELS, step, b["pl"], b["s"]))
+    # the decoder's back end: last inverse level + inverse ICT in one kernel
+    # (uint8 out), and all levels + inverse ICT (rg_dwt_inv_u8)
+    assert R.dwt_inv_u8_scratch_bytes(W, H, NC, RG_COLOUR_ICT, RG_FILTER_97) <= R.scratch_bytes(W, H, NC, RG_FILTER_97)
+    t_iback1 = timed_rot(lambda b: R.dwt_inv_u8(ctx, b["q"], W, H, NC, RG_COLOUR_ICT, RG_FILTER_97, 1, step1, b["src"],
+                                                b["s"]))
+    t_iback = timed_rot(lambda b: R.dwt_inv_u8(ctx, b["q"], W, H, NC, RG_COLOUR_ICT, RG_FILTER_97, LEVELS, step,
+                                               b["src"], b["s"]))
     t_cinv = timed_rot(lambda b: R.colour_inv(ctx, b["pl"], W, H, NC, RG_COLOUR_ICT, 1, b["src"]))
     ctx.sync()
     for b in sets:
@@ -448,7 +455,10 @@ def main():
                       "t1_standard_codeblocks_per_s": round(len(desc) / (t_t1 * 1e-3), 1),
                       "encode_parallel_style": round(t_enc_par, 4),
                       "encode_bypass_style": round(t_enc_byp, 4)},
-        "reverse_stages_ms": {"dwt97_inv_level1": round(t_idwt_l1, 4),
+        "reverse_stages_ms": {"dwt97_inv_level1_colour_fused": round(t_iback1, 4),
+                              "dwt97_inv_level1_colour_fused_GBps": round(f1_bytes / (t_iback1 * 1e-3) / 1e9, 1),
+                              "dwt97_inv_6lv_colour_fused": round(t_iback, 4),
+                              "dwt97_inv_level1": round(t_idwt_l1, 4),
                               "dwt97_inv_level1_GBps": round(l1_bytes / (t_idwt_l1 * 1e-3) / 1e9, 1),
                               "dwt97_inv_6lv": round(t_idwt, 4), "colour_inv_ict": round(t_cinv, 4),
                               "colour_inv_GBps": round(15.0 * W * H / (t_cinv * 1e-3) / 1e9, 1),

@@ -196,6 +196,21 @@ int rg_dwt97_fwd_float(rg_context *ctx, const float *d_in, int w, int h, int npl
  *       (step == NULL: d_q is read as float coefficients), -> float samples. */
 int rg_dwt_inv(rg_context *ctx, int filter, const void *d_q, int w, int h, int nplanes,
                int levels, const float *step, void *d_out, void *d_scratch);
+/* Inverse transform and inverse colour stage in one: levels (>= 1) of 5/3
+ * coefficients or 9/7 quantisation indices d_q (dequantised with step, as
+ * rg_dwt_inv) -> inverse colour transform (RG_COLOUR_RCT with 5/3,
+ * RG_COLOUR_ICT with 9/7, or NONE) -> interleaved uint8 image d_dst
+ * (w x h x ncomp, 16-byte aligned; device memory).  Where the shape allows (as
+ * rg_dwt_fwd_u8) the last level synthesises every component of a tile in
+ * workgroup memory and writes the uint8 pixels itself, so the FP32 planes
+ * never go to memory; otherwise they pass through d_scratch.  d_scratch holds
+ * rg_dwt_inv_u8_scratch_bytes.  The native decoder's back end
+ * (rg_decoder_decode_codeblocks); it replaces the reference's inverse DWT
+ * (src/OCLDWTRev.cpp:39-59) and the host-side level shift back, which the
+ * reference never built. */
+size_t rg_dwt_inv_u8_scratch_bytes(int w, int h, int ncomp, int colour, int filter);
+int rg_dwt_inv_u8(rg_context *ctx, const int32_t *d_q, int w, int h, int ncomp, int colour, int filter,
+                  int levels, const float *step, uint8_t *d_dst, void *d_scratch);
 
 /* Code-block list for a w x h image with `levels` levels and cbw x cbh
  * blocks, per component in resolution / band / raster order.  Returns the

@@ -97,6 +97,8 @@ SIGNATURES = {
     "rg_dwt_fwd": (I, [P, I, P, I, I, I, I, P, I, P, P]),
     "rg_dwt_fwd_u8_scratch_bytes": (ctypes.c_size_t, [I, I, I, I, I]),
     "rg_dwt_fwd_u8": (I, [P, P, I, I, I, I, I, I, P, P, P]),
+    "rg_dwt_inv_u8_scratch_bytes": (ctypes.c_size_t, [I, I, I, I, I]),
+    "rg_dwt_inv_u8": (I, [P, P, I, I, I, I, I, I, P, P, P]),
     "rg_dwt97_fwd_float": (I, [P, P, I, I, I, I, P, P]),
     "rg_dwt_fwd_noalloc": (I, [P, I, P, I, I, I, I, P, I, P]),
     "rg_dwt_inv": (I, [P, I, P, I, I, I, I, P, P, P]),

@@ -417,14 +417,16 @@ extern "C" int rg_dwt97_fwd_float(rg_context *c, const float *in, int w, int h, 
     return dwt_fwd_impl(c, RG_FILTER_97, in, w, h, nplanes, levels, nullptr, 0, out, 1, scratch);
 }
 
-extern "C" int rg_dwt_inv(rg_context *c, int filter, const void *dq, int w, int h, int nplanes, int levels,
-                          const float *step, void *out, void *scratch)
+// Levels `levels` down to `last` of the inverse transform (last > 1: the
+// output is the LL band of level last - 1, left in the scratch buffer for
+// the fused last level).
+static int dwt_inv_impl(rg_context *c, int filter, const void *dq, int w, int h, int nplanes, int levels,
+                        const float *step, void *out, void *scratch, int last = 1)
 {
-    RG_CHECK(c, "null context");
     int rc = check_levels(w, h, levels);
     if (rc) return rc;
     RG_CHECK(filter == RG_FILTER_53 || filter == RG_FILTER_97, "bad filter %d", filter);
-    RG_CHECK(dq && out && nplanes >= 1, "null buffer");
+    RG_CHECK(dq && (out || last > 1) && nplanes >= 1, "null buffer");
     const int q_float = filter == RG_FILTER_97 && step == nullptr;
     if (levels == 0)
         return rg_launch_copy_dequant(c->stream, filter, dq, (long)w * h, nplanes, step ? step[0] : 1.0f, q_float, out);
@@ -432,7 +434,7 @@ extern "C" int rg_dwt_inv(rg_context *c, int filter, const void *dq, int w, int 
     rg_level_dims d = rg_dims(w, h, levels);
     char *sc = reinterpret_cast<char *>(scratch);
     auto ll = [&](int l) { return sc + rg_ll_offset(w, h, nplanes, l); };
-    for (int l = levels; l >= 1; --l) {
+    for (int l = levels; l >= last; --l) {
         rg_inv_level_args a{};
         a.filter = filter;
         a.cw = d.w[l - 1]; a.ch = d.h[l - 1]; a.nplanes = nplanes;
@@ -458,6 +460,73 @@ extern "C" int rg_dwt_inv(rg_context *c, int filter, const void *dq, int w, int 
         if (rc) return rc;
     }
     return RG_SUCCESS;
+}
+
+extern "C" int rg_dwt_inv(rg_context *c, int filter, const void *dq, int w, int h, int nplanes, int levels,
+                          const float *step, void *out, void *scratch)
+{
+    RG_CHECK(c, "null context");
+    return dwt_inv_impl(c, filter, dq, w, h, nplanes, levels, step, out, scratch);
+}
+
+// Inverse transform + inverse colour stage into the interleaved uint8 image:
+// levels down to 2 as usual, then the fused last level where
+// rg_inv1_colour_ok takes the shape (*fused = true), else level 1 into
+// `planes` and the inverse colour kernel from there.  9/7 needs `step`
+// (quantisation indices).  scratch: the LL pyramid.
+static int dwt_inv_u8_impl(rg_context *c, const int32_t *dq, int w, int h, int ncomp, int colour, int filter,
+                           int levels, const float *step, uint8_t *dst, void *scratch, void *planes, bool *fused)
+{
+    int rc = check_levels(w, h, levels);
+    if (rc) return rc;
+    RG_CHECK(filter == RG_FILTER_53 || step, "9/7 dequantisation needs the band steps");
+    const bool f = levels >= 1 && rg_inv1_colour_ok(w, h, ncomp, colour, filter);
+    if (fused) *fused = f;
+    if (!f) {
+        RG_CHECK(planes, "planes buffer required for this shape");
+        if ((rc = dwt_inv_impl(c, filter, dq, w, h, ncomp, levels, step, planes, scratch))) return rc;
+        return rg_launch_colour_inv(c->stream, planes, w, h, ncomp, colour, filter == RG_FILTER_97, dst);
+    }
+    RG_CHECK(scratch || levels == 1, "scratch buffer required");
+    if (levels > 1 && (rc = dwt_inv_impl(c, filter, dq, w, h, ncomp, levels, step, nullptr, scratch, 2))) return rc;
+    const rg_level_dims d = rg_dims(w, h, 1);
+    rg_inv1c_args a{};
+    a.ll_is_q = levels == 1;
+    a.ll_in = levels > 1 ? (const void *)((char *)scratch + rg_ll_offset(w, h, ncomp, 1)) : nullptr;
+    a.ll_pitch = d.w[1]; a.ll_ps = (long)d.w[1] * d.h[1];
+    a.q = dq; a.qp = w; a.q_ps = (long)w * h;
+    a.w = w; a.h = h; a.ncomp = ncomp; a.colour = colour; a.filter = filter;
+    if (filter == RG_FILTER_97) {
+        const int b = 1 + 3 * (levels - 1);   // HL of level 1
+        a.s_ll = step[0]; a.s_hl = step[b]; a.s_lh = step[b + 1]; a.s_hh = step[b + 2];
+    }
+    a.dst = dst;
+    return rg_launch_inv1_colour(c->stream, a);
+}
+
+extern "C" size_t rg_dwt_inv_u8_scratch_bytes(int w, int h, int ncomp, int colour, int filter)
+{
+    if (w <= 0 || h <= 0 || ncomp <= 0) return 512;
+    size_t n = rg_dwt_scratch_bytes(w, h, ncomp, filter);
+    if (!rg_inv1_colour_ok(w, h, ncomp, colour, filter)) n += (size_t)w * h * ncomp * 4 + 256;
+    return n;
+}
+
+extern "C" int rg_dwt_inv_u8(rg_context *c, const int32_t *dq, int w, int h, int ncomp, int colour, int filter,
+                             int levels, const float *step, uint8_t *dst, void *scratch)
+{
+    RG_CHECK(c && dq && dst && scratch, "rg_dwt_inv_u8: null argument");
+    RG_CHECK(w > 0 && h > 0 && ncomp >= 1 && ncomp <= 4, "rg_dwt_inv_u8: bad shape %dx%dx%d", w, h, ncomp);
+    RG_CHECK(filter == RG_FILTER_53 || filter == RG_FILTER_97, "bad filter %d", filter);
+    RG_CHECK(colour == RG_COLOUR_NONE || (ncomp == 3 && ((colour == RG_COLOUR_RCT && filter == RG_FILTER_53) ||
+                                                         (colour == RG_COLOUR_ICT && filter == RG_FILTER_97))),
+             "colour transform %d does not pair with filter %d and %d components", colour, filter, ncomp);
+    RG_CHECK(((uintptr_t)dst & 15) == 0, "rg_dwt_inv_u8: destination must be 16-byte aligned");
+    RG_CHECK(levels >= 1, "rg_dwt_inv_u8: levels %d, at least 1", levels);
+    RG_HIP(hipSetDevice(c->device));
+    const size_t ll_bytes = rg_dwt_scratch_bytes(w, h, ncomp, filter);
+    void *planes = (char *)scratch + ((ll_bytes + 255) & ~(size_t)255);
+    return dwt_inv_u8_impl(c, dq, w, h, ncomp, colour, filter, levels, step, dst, scratch, planes, nullptr);
 }
 
 // ------------------------------------------------------------ code-blocks
@@ -1294,6 +1363,11 @@ struct rg_decoder {
     int w = 0, h = 0, ncomp = 0, lossy = 0, ran = 0, loaded = 0, u8_ran = 0;
     DevBuf coeff, out, scratch, cbdata, cboffs, cbres, cbdesc, u8, cbseg, t1ws;
     HostBuf staging, map;
+    // a code-block decode ends in the fused last inverse level, which writes
+    // the uint8 image and no FP32 planes (`out`); those are computed from the
+    // coefficients (still in `coeff`) if rg_decoder_map_out asks for them
+    int out_valid = 0, levels = 0, filter = 0;
+    float step[97];
 };
 
 int rg_launch_t1_decode(hipStream_t s, const uint8_t *data, const long *offs, const rg_cb_result *res,
@@ -1418,12 +1492,19 @@ static int decode_codeblocks(rg_decoder *d, const rg_cb_desc *desc, const rg_cb_
     float step[97], inv[97];
     band_steps(p->levels, p->bitdepth, p->delta, step, inv);
     if (steps) memcpy(step, steps, sizeof(float) * (3 * p->levels + 1));
-    if ((rc = rg_dwt_inv(d->ctx, p->filter, d->coeff.p, w, h, ncomp, p->levels,
-                         p->filter == RG_FILTER_97 ? step : nullptr, d->out.p, d->scratch.p)))
-        return rc;
-    if ((rc = rg_launch_colour_inv(s, d->out.p, w, h, ncomp, p->colour, p->filter == RG_FILTER_97, (uint8_t *)d->u8.p)))
-        return rc;
+    // inverse transform and inverse colour stage; the last level writes the
+    // uint8 image itself where the shape allows (RG_DWT_FUSED=0: never)
+    bool fused = false;
+    if (p->levels >= 1 && rg_env_int("RG_DWT_FUSED", 1))
+        rc = dwt_inv_u8_impl(d->ctx, (const int32_t *)d->coeff.p, w, h, ncomp, p->colour, p->filter, p->levels,
+                             step, (uint8_t *)d->u8.p, d->scratch.p, d->out.p, &fused);
+    else if (!(rc = dwt_inv_impl(d->ctx, p->filter, d->coeff.p, w, h, ncomp, p->levels,
+                                 p->filter == RG_FILTER_97 ? step : nullptr, d->out.p, d->scratch.p)))
+        rc = rg_launch_colour_inv(s, d->out.p, w, h, ncomp, p->colour, p->filter == RG_FILTER_97, (uint8_t *)d->u8.p);
+    if (rc) return rc;
     d->w = w; d->h = h; d->ncomp = ncomp; d->lossy = p->filter == RG_FILTER_97; d->ran = 1; d->u8_ran = 1;
+    d->out_valid = !fused; d->levels = p->levels; d->filter = p->filter;
+    memcpy(d->step, step, sizeof(float) * (3 * p->levels + 1));
     return RG_SUCCESS;
 }
 
@@ -1496,6 +1577,7 @@ extern "C" int rg_decoder_idwt(rg_decoder *d, int levels)
                     d->out.p, d->scratch.p);
     if (rc) return rc;
     d->ran = 1;
+    d->out_valid = 1;
     return RG_SUCCESS;
 }
 
@@ -1512,6 +1594,12 @@ extern "C" int rg_decoder_map_out(rg_decoder *d, void **host)
     RG_CHECK(d && host, "null argument");
     if (!d->ran) return rg_fail(RG_ERR_NOT_READY, "decoder has not run");
     const size_t npix = (size_t)d->w * d->h, nc = d->ncomp;
+    if (!d->out_valid) {   // after a fused code-block decode: the planes from the coefficients
+        const int rc = dwt_inv_impl(d->ctx, d->filter, d->coeff.p, d->w, d->h, d->ncomp, d->levels,
+                                    d->filter == RG_FILTER_97 ? d->step : nullptr, d->out.p, d->scratch.p);
+        if (rc) return rc;
+        d->out_valid = 1;
+    }
     std::vector<uint8_t> tmp(npix * nc * 4);
     RG_HIP(hipMemcpyAsync(tmp.data(), d->out.p, tmp.size(), hipMemcpyDeviceToHost, d->ctx->stream));
     RG_HIP(hipStreamSynchronize(d->ctx->stream));

@@ -36,13 +36,6 @@ __device__ __forceinline__ void st_nt4(int32_t *p, int32_t a, int32_t b, int32_t
     __builtin_nontemporal_store(v, reinterpret_cast<rg_i4 *>(p));
 }
 
-__device__ __forceinline__ uint8_t clamp_u8(int x) { return (uint8_t)(x < 0 ? 0 : (x > 255 ? 255 : x)); }
-__device__ __forceinline__ uint8_t out_u8f(float x)
-{
-    float s = x + 128.0f;
-    return clamp_u8((int)floorf(s + 0.5f));
-}
-
 using rg_px::px_fwd;
 
 // One quad: pixels [p, p+4) (p < npix, p a multiple of 4).  A full quad reads
@@ -108,28 +101,9 @@ __global__ void __launch_bounds__(256) colour_fwd_kernel(const uint8_t *__restri
     colour_fwd_span<NC, MODE, FLT>(src, npix, planes, (long)blockIdx.x * SPAN);
 }
 
-// Inverse transform of one pixel: planes' values in[0..NC) -> NC bytes.
-template <int NC, int MODE, bool FLT, class T>
-__device__ __forceinline__ void px_inv(const T (&v)[NC], uint8_t (&o)[NC])
-{
-    if constexpr (FLT && MODE == 2) {
-        const float y = v[0], cb = v[1], cr = v[2];
-        o[0] = out_u8f(__builtin_fmaf((float)1.402, cr, y));
-        o[1] = out_u8f(__builtin_fmaf((float)-0.71414, cr, __builtin_fmaf((float)-0.34413, cb, y)));
-        o[2] = out_u8f(__builtin_fmaf((float)1.772, cb, y));
-    } else if constexpr (!FLT && MODE == 1) {
-        const int g = v[0] - ((v[1] + v[2]) >> 2);
-        o[0] = clamp_u8(v[2] + g + 128);
-        o[1] = clamp_u8(g + 128);
-        o[2] = clamp_u8(v[1] + g + 128);
-    } else {
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            if constexpr (FLT) o[c] = out_u8f(v[c]);
-            else o[c] = clamp_u8(v[c] + 128);
-        }
-    }
-}
+using rg_px::clamp_u8;
+using rg_px::out_u8f;
+using rg_px::px_inv;
 
 // One quad of the inverse: pixels [p, p+4).  A full quad reads one 16-byte
 // vector per plane and writes NC dwords (4 pixels x NC bytes, p*NC is a

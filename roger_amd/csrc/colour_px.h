@@ -40,4 +40,37 @@ __device__ __forceinline__ void px_fwd(const int (&v)[NC], T (&o)[NC])
     }
 }
 
+__device__ __forceinline__ uint8_t clamp_u8(int x) { return (uint8_t)(x < 0 ? 0 : (x > 255 ? 255 : x)); }
+// A reconstructed FP32 sample (level shifted) to uint8: + 128, rounded half
+// up, clamped.
+__device__ __forceinline__ uint8_t out_u8f(float x)
+{
+    float s = x + 128.0f;
+    return clamp_u8((int)floorf(s + 0.5f));
+}
+
+// Inverse transform of one pixel: the planes' values v[0..NC) -> NC bytes
+// (T.800 G.2 / G.3 inverse, then the inverse level shift).
+template <int NC, int MODE, bool FLT, class T>
+__device__ __forceinline__ void px_inv(const T (&v)[NC], uint8_t (&o)[NC])
+{
+    if constexpr (FLT && MODE == 2) {
+        const float y = v[0], cb = v[1], cr = v[2];
+        o[0] = out_u8f(__builtin_fmaf((float)1.402, cr, y));
+        o[1] = out_u8f(__builtin_fmaf((float)-0.71414, cr, __builtin_fmaf((float)-0.34413, cb, y)));
+        o[2] = out_u8f(__builtin_fmaf((float)1.772, cb, y));
+    } else if constexpr (!FLT && MODE == 1) {
+        const int g = v[0] - ((v[1] + v[2]) >> 2);
+        o[0] = clamp_u8(v[2] + g + 128);
+        o[1] = clamp_u8(g + 128);
+        o[2] = clamp_u8(v[1] + g + 128);
+    } else {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            if constexpr (FLT) o[c] = out_u8f(v[c]);
+            else o[c] = clamp_u8(v[c] + 128);
+        }
+    }
+}
+
 }  // namespace rg_px

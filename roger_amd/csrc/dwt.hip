@@ -41,7 +41,9 @@
 // (fwd1_colour_kernel): the workgroup reads its tile of the interleaved
 // uint8 image, level shifts and colour transforms each pixel once into LDS
 // and lifts from there, so the FP32 colour planes never go to HBM (3 B/px
-// read instead of 3 B/px + 12 B/px written + 12 B/px read again).
+// read instead of 3 B/px + 12 B/px written + 12 B/px read again).  The
+// decoder's last inverse level likewise ends in the inverse colour stage
+// (inv1_colour_kernel) and writes the uint8 image itself.
 #include <hip/hip_fp16.h>
 #include <type_traits>
 #include "colour_px.h"
@@ -723,6 +725,211 @@ __device__ __forceinline__ void inv_tile(const rg_inv_level_args &a, int plane, 
     }
 }
 
+// ==================== last inverse level fused with the inverse colour stage ====================
+// One TW x TH tile of the image, every component: the level-1 bands of each
+// component (dequantised) are staged interleaved in LDS exactly as inv_tile
+// stages one plane, synthesised by rows then columns (items over all
+// components at once), the columns' outputs written back over the staging,
+// and the pixels colour transformed back (rg_px::px_inv, the inverse colour
+// kernel's arithmetic) and stored as the interleaved uint8 image, four pixels
+// (NC aligned dwords) per item.  The FP32 planes never go to HBM.
+template <int F, int NC, int TW, int TH> constexpr int inv1c_lds_bytes()
+{
+    return NC * (TH + 2 * Filt<F>::H) * (lcol(TW + 2 * Filt<F>::H) + 1) * 4;
+}
+
+// Four pixels of row r (image row y, column x) from the synthesised tiles,
+// colour transformed back, as NC aligned dwords of the uint8 image.
+template <int F, int NC, int MODE, int TW, int H, int LP, int PS, class T>
+__device__ __forceinline__ void inv1c_store_quad(const rg_inv1c_args &a, const T *tile, int r, int q4, int y, int x)
+{
+    uint32_t wd[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) wd[k] = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        T v[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) v[k] = tile[k * PS + (H + r) * LP + lcol(H + 4 * q4 + i)];
+        uint8_t ob[NC];
+        rg_px::px_inv<NC, MODE, F == RG_FILTER_97>(v, ob);
+#pragma unroll
+        for (int k = 0; k < NC; ++k) wd[(NC * i + k) >> 2] |= (uint32_t)ob[k] << (((NC * i + k) & 3) * 8);
+    }
+    uint32_t *d32 = reinterpret_cast<uint32_t *>(a.dst + ((long)y * a.w + x) * NC);
+#pragma unroll
+    for (int k = 0; k < NC; ++k) __builtin_nontemporal_store(wd[k], d32 + k);
+}
+
+template <int F, int NC, int MODE, int TW, int TH, int WGS>
+__device__ __forceinline__ void inv1_colour_tile(const rg_inv1c_args &a, int bx, int by, char *lds)
+{
+    using T = typename Filt<F>::T;
+    constexpr int H = Filt<F>::H;
+    constexpr int CW = TW + 2 * H, RH = TH + 2 * H;
+    constexpr int LP = lcol(CW) + 1;
+    constexpr int PS = RH * LP;                      // words per component
+    constexpr int N = SEG + 2 * H;
+    constexpr int HS = TW / SEG, VS = TH / SEG;
+    static_assert(WGS % 64 == 0, "whole wavefronts");
+    T *tile = reinterpret_cast<T *>(lds);
+    const int x0 = bx * TW, y0 = by * TH;
+    const int w = a.w, h = a.h;
+    const int wl = (w + 1) >> 1, hl = (h + 1) >> 1;
+    const bool ll_is_q = a.ll_is_q != 0;
+
+    // ---- stage every component's bands, every load first ----
+    const int32_t *qb0 = a.q;
+    const int32_t *lb0 = ll_is_q ? a.q : reinterpret_cast<const int32_t *>(a.ll_in);
+    const long lpitch = ll_is_q ? a.qp : a.ll_pitch, lps = ll_is_q ? a.q_ps : a.ll_ps;
+    const bool quads = x0 >= 8 && y0 >= H && (x0 >> 1) + TW / 2 + 4 <= wl && y0 + TH + H <= h &&
+                       ((a.qp | lpitch | wl | a.q_ps | lps) & 3) == 0 && (((uintptr_t)qb0 | (uintptr_t)lb0) & 15) == 0;
+    if (quads) {
+        constexpr int NQ = (TW / 2 + 8) / 4;            // quads per band and row
+        constexpr int PI = 2 * NQ * RH;                 // items per component
+        constexpr int NI4 = (NC * PI + WGS - 1) / WGS;
+        int4 raw[NI4];
+        float stp[NI4];                                  // 9/7: step of an index, 0 for an FP32 LL word
+        const int bq = (x0 >> 1) - 4;
+#pragma unroll
+        for (int i = 0; i < NI4; ++i) {
+            const int item = threadIdx.x + i * WGS;
+            const int it = item < NC * PI ? item : 0;
+            const int k = it / PI, r2 = it - k * PI;
+            const int j = r2 % (2 * NQ), rr = r2 / (2 * NQ);
+            const int high = j >= NQ, m4 = j - high * NQ;
+            const int y = y0 - H + rr;
+            const bool ly = !(y & 1);
+            const int byy = y >> 1;
+            const bool from_ll = ly && !high;
+            const int32_t *p = from_ll ? lb0 + k * lps + (long)byy * lpitch + bq + 4 * m4
+                                       : qb0 + k * a.q_ps + (long)(ly ? byy : hl + byy) * a.qp + (high ? wl : 0) + bq +
+                                             4 * m4;
+            raw[i] = *reinterpret_cast<const int4 *>(p);
+            if constexpr (F == RG_FILTER_97)
+                stp[i] = (from_ll && !ll_is_q) ? 0.0f : (ly ? (high ? a.s_hl : a.s_ll) : (high ? a.s_hh : a.s_lh));
+        }
+#pragma unroll
+        for (int i = 0; i < NI4; ++i) {
+            const int item = threadIdx.x + i * WGS;
+            if (item >= NC * PI) break;
+            const int k = item / PI, r2 = item - k * PI;
+            const int j = r2 % (2 * NQ), rr = r2 / (2 * NQ);
+            const int high = j >= NQ, m4 = j - high * NQ;
+            const int c0 = 2 * (4 * m4 + H / 2 - 4) + high;
+            const int32_t v4[4] = {raw[i].x, raw[i].y, raw[i].z, raw[i].w};
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int c = c0 + 2 * u;
+                if (c < 0 || c >= CW) continue;
+                T v;
+                if constexpr (F == RG_FILTER_53) v = v4[u];
+                else v = stp[i] != 0.0f ? dequant(v4[u], stp[i]) : __int_as_float(v4[u]);
+                tile[k * PS + rr * LP + lcol(c)] = v;
+            }
+        }
+    } else {
+        constexpr int PI = CW * RH;
+        constexpr int NI = (NC * PI + WGS - 1) / WGS;
+        int32_t raw[NI];
+        float stp[NI];
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+            const int item = threadIdx.x + i * WGS;
+            const int it = item < NC * PI ? item : 0;
+            const int k = it / PI, r2 = it - k * PI;
+            const int c = r2 % CW, rr = r2 / CW;
+            const int y = mirror1(y0 - H + rr, h), x = mirror1(x0 - H + c, w);
+            const bool ly = !(y & 1), lx = !(x & 1);
+            const int byy = y >> 1, bxx = x >> 1;
+            const bool from_ll = ly && lx && !ll_is_q;
+            const int32_t *p = from_ll ? lb0 + k * lps + (long)byy * a.ll_pitch + bxx
+                                       : qb0 + k * a.q_ps + (long)(ly ? byy : hl + byy) * a.qp + (lx ? bxx : wl + bxx);
+            raw[i] = *p;
+            if constexpr (F == RG_FILTER_97)
+                stp[i] = from_ll ? 0.0f : (ly ? (lx ? a.s_ll : a.s_hl) : (lx ? a.s_lh : a.s_hh));
+        }
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+            const int item = threadIdx.x + i * WGS;
+            if (item >= NC * PI) break;
+            const int k = item / PI, r2 = item - k * PI;
+            const int c = r2 % CW, rr = r2 / CW;
+            T v;
+            if constexpr (F == RG_FILTER_53) v = raw[i];
+            else v = stp[i] != 0.0f ? dequant(raw[i], stp[i]) : __int_as_float(raw[i]);
+            tile[k * PS + rr * LP + lcol(c)] = v;
+        }
+    }
+    __syncthreads();
+
+    // ---- horizontal synthesis: items (component, row of RH, segment) ----
+    constexpr int NR = NC * RH * HS, RR = (NR + WGS - 1) / WGS;
+    T o[RR][SEG];
+#pragma unroll
+    for (int u = 0; u < RR; ++u) {
+        const int it = threadIdx.x + u * WGS;
+        if (it < NR) {
+            const int kk = it % HS, rk = it / HS;   // rk = component * RH + row
+            T e[N];
+#pragma unroll
+            for (int j = 0; j < N; ++j) e[j] = tile[rk * LP + lcol(kk * SEG + j)];
+            if constexpr (F == RG_FILTER_53) inv53(e); else inv97(e, true);
+#pragma unroll
+            for (int j = 0; j < SEG; ++j) o[u][j] = e[H + j];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < RR; ++u) {
+        const int it = threadIdx.x + u * WGS;
+        if (it < NR) {
+            const int kk = it % HS, rk = it / HS;
+#pragma unroll
+            for (int j = 0; j < SEG; ++j) tile[rk * LP + lcol(H + kk * SEG + j)] = o[u][j];
+        }
+    }
+    __syncthreads();
+
+    // ---- vertical synthesis: items (component, column, segment); outputs
+    //      written back over the staging (rows H .. H + TH) after every read ----
+    constexpr int NV = NC * TW * VS, RV = (NV + WGS - 1) / WGS;
+    T ov[RV][SEG];
+#pragma unroll
+    for (int u = 0; u < RV; ++u) {
+        const int it = threadIdx.x + u * WGS;
+        if (it < NV) {
+            const int vc = it % TW, vs = (it / TW) % VS, vk = it / (TW * VS);
+            T e[N];
+#pragma unroll
+            for (int j = 0; j < N; ++j) e[j] = tile[vk * PS + (vs * SEG + j) * LP + lcol(H + vc)];
+            if constexpr (F == RG_FILTER_53) inv53(e); else inv97(e, true);
+#pragma unroll
+            for (int j = 0; j < SEG; ++j) ov[u][j] = e[H + j];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < RV; ++u) {
+        const int it = threadIdx.x + u * WGS;
+        if (it < NV) {
+            const int vc = it % TW, vs = (it / TW) % VS, vk = it / (TW * VS);
+#pragma unroll
+            for (int j = 0; j < SEG; ++j) tile[vk * PS + (H + vs * SEG + j) * LP + lcol(H + vc)] = ov[u][j];
+        }
+    }
+    __syncthreads();
+
+    // ---- inverse colour: items (row, 4-pixel quad), NC aligned dwords out ----
+    for (int it = threadIdx.x; it < TH * TW / 4; it += WGS) {
+        const int r = it / (TW / 4), q4 = it % (TW / 4);
+        const int y = y0 + r, x = x0 + 4 * q4;
+        if (y >= h || x >= w) continue;
+        inv1c_store_quad<F, NC, MODE, TW, H, LP, PS>(a, tile, r, q4, y, x);
+    }
+}
+
+
 // XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
 // XCDs (each with its own L2), so in launch order neighbouring tiles land on
 // different XCDs and every halo column is fetched from HBM again.  The
@@ -765,6 +972,17 @@ __global__ void __launch_bounds__(WGS) fwd1_colour_kernel(rg_fwd1c_args a)
     int plane, bx, by;
     xcd_tile(plane, bx, by);
     fwd1_colour_tile<F, NC, MODE, TW, WGS>(a, bx, by, lds);
+}
+
+// Last inverse level with the inverse colour stage, one workgroup per
+// TW x TH tile of the image.
+template <int F, int NC, int MODE, int TW, int TH, int WGS>
+__global__ void __launch_bounds__(WGS) inv1_colour_kernel(rg_inv1c_args a)
+{
+    __shared__ __attribute__((aligned(16))) char lds[inv1c_lds_bytes<F, NC, TW, TH>()];
+    int plane, bx, by;
+    xcd_tile(plane, bx, by);
+    inv1_colour_tile<F, NC, MODE, TW, TH, WGS>(a, bx, by, lds);
 }
 
 template <int F, bool PLAIN = false>
@@ -887,6 +1105,48 @@ int rg_launch_fwd1_colour(hipStream_t s, const rg_fwd1c_args &a)
         if (a.ncomp == 1) launch_fwd1c<RG_FILTER_53, 1, 0>(s, a);
         else if (a.colour == RG_COLOUR_RCT) launch_fwd1c<RG_FILTER_53, 3, 1>(s, a);
         else launch_fwd1c<RG_FILTER_53, 3, 0>(s, a);
+    }
+    RG_LAUNCH_CHECK();
+    return RG_SUCCESS;
+}
+
+bool rg_inv1_colour_ok(int w, int h, int ncomp, int colour, int filter)
+{
+    return rg_fwd1_colour_ok(w, h, ncomp, colour, filter);
+}
+
+namespace {
+template <int F, int NC, int MODE, int TW, int TH, int WGS>
+void launch_inv1c_shape(hipStream_t s, const rg_inv1c_args &a)
+{
+    dim3 grid((a.w + TW - 1) / TW, (a.h + TH - 1) / TH, 1);
+    inv1_colour_kernel<F, NC, MODE, TW, TH, WGS><<<grid, dim3(WGS), 0, s>>>(a);
+}
+
+// Fused inverse tile: 64 x 32 with 512 threads (37 KB of LDS for RGB, three
+// workgroups a CU).  4096x4096 RGB, caches cold: 64 x 16 with 256 threads,
+// 64 x 32 with 256 and 64 x 64 with 512 measured 4-16% slower.
+template <int F, int NC, int MODE>
+void launch_inv1c(hipStream_t s, const rg_inv1c_args &a)
+{
+    launch_inv1c_shape<F, NC, MODE, 64, 32, 512>(s, a);
+}
+}  // namespace
+
+int rg_launch_inv1_colour(hipStream_t s, const rg_inv1c_args &a)
+{
+    RG_CHECK(rg_inv1_colour_ok(a.w, a.h, a.ncomp, a.colour, a.filter),
+             "fused inverse DWT + colour: unsupported shape %dx%dx%d (colour %d, filter %d)", a.w, a.h, a.ncomp,
+             a.colour, a.filter);
+    RG_CHECK(((uintptr_t)a.dst & 3) == 0, "fused inverse DWT + colour: destination must be 4-byte aligned");
+    if (a.filter == RG_FILTER_97) {
+        if (a.ncomp == 1) launch_inv1c<RG_FILTER_97, 1, 0>(s, a);
+        else if (a.colour == RG_COLOUR_ICT) launch_inv1c<RG_FILTER_97, 3, 2>(s, a);
+        else launch_inv1c<RG_FILTER_97, 3, 0>(s, a);
+    } else {
+        if (a.ncomp == 1) launch_inv1c<RG_FILTER_53, 1, 0>(s, a);
+        else if (a.colour == RG_COLOUR_RCT) launch_inv1c<RG_FILTER_53, 3, 1>(s, a);
+        else launch_inv1c<RG_FILTER_53, 3, 0>(s, a);
     }
     RG_LAUNCH_CHECK();
     return RG_SUCCESS;

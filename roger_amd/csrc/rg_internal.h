@@ -142,6 +142,24 @@ struct rg_inv_level_args {
 int rg_launch_inv53_level(hipStream_t s, const rg_inv_level_args &a);
 int rg_launch_inv97_level(hipStream_t s, const rg_inv_level_args &a);
 
+// Last inverse level fused with the inverse colour stage: the level-1 bands
+// of every component (LL from ll_in, or dequantised from q when ll_is_q) are
+// synthesised in workgroup memory, colour transformed back and written as
+// the interleaved uint8 image (w x h x ncomp, rows of w * ncomp bytes).
+struct rg_inv1c_args {
+    const void *ll_in;      // float (9/7) / int32 (5/3)
+    long ll_pitch, ll_ps;
+    int ll_is_q;
+    const int32_t *q;
+    long qp, q_ps;
+    int w, h, ncomp, colour, filter;
+    float s_ll, s_hl, s_lh, s_hh;   // 9/7 dequantisation steps of level 1
+    uint8_t *dst;
+};
+// Shapes the fused kernel takes: as rg_fwd1_colour_ok.
+bool rg_inv1_colour_ok(int w, int h, int ncomp, int colour, int filter);
+int rg_launch_inv1_colour(hipStream_t s, const rg_inv1c_args &a);
+
 // Tile shapes of the level kernels (the batched job tables use them too).
 #define RG_DWT_FWD_TW 128
 #define RG_DWT_FWD_TH 16

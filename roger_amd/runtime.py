@@ -219,6 +219,19 @@ def dwt_fwd_u8(ctx, d_src, w, h, ncomp, colour, filt, levels, inv_step, d_q, d_s
                               _vp(d_scratch.ptr)))
 
 
+def dwt_inv_u8_scratch_bytes(w, h, ncomp, colour, filt):
+    return lib().rg_dwt_inv_u8_scratch_bytes(w, h, ncomp, colour, filt)
+
+
+def dwt_inv_u8(ctx, d_q, w, h, ncomp, colour, filt, levels, step, d_dst, d_scratch):
+    """Inverse transform + inverse colour stage into the interleaved uint8
+    image (last level fused where the shape allows)."""
+    st = None if step is None else np.ascontiguousarray(step, np.float32)
+    check(lib().rg_dwt_inv_u8(ctx.handle, _vp(d_q.ptr), w, h, ncomp, colour, filt, levels,
+                              None if st is None else st.ctypes.data_as(ctypes.c_void_p), _vp(d_dst.ptr),
+                              _vp(d_scratch.ptr)))
+
+
 def dwt_fwd_noalloc(ctx, filt, d_in, w, h, nplanes, levels, inv_step, ll_fp16, d_q):
     """rg_dwt_fwd with the library's static per-GPU scratch (no allocation)."""
     inv = None if inv_step is None else np.ascontiguousarray(inv_step, np.float32)

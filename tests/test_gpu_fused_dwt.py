@@ -130,3 +130,74 @@ def test_dwt_fwd_u8_entry(ctx, w, h, nc, reversible, env, monkeypatch):
     for c in range(nc):
         want = O.dwt53_fwd(planes[c], levels) if reversible else O.quantize(O.dwt97_fwd(planes[c], levels), levels, inv)
         np.testing.assert_array_equal(got[c], want)
+
+
+def _oracle_recon(img, q, w, h, nc, levels, reversible, delta):
+    """The oracle's reconstruction of its own indices: uint8 image (9/7: FP32
+    inverse of the dequantised indices, inverse ICT) and the planes."""
+    if reversible:
+        planes = np.stack([O.dwt53_inv(q[c], levels) for c in range(nc)])
+        img8 = O.rct_inv(planes) if nc == 3 else O.unshift(planes[0])
+        return img8, planes
+    steps = np.array([s[2] for s in O.band_steps(w, h, levels, delta)], np.float32)
+    planes = np.stack([O.dwt97_inv(O.dequantize(q[c], levels, steps), levels) for c in range(nc)])
+    img8 = O.ict_inv_f32(planes) if nc == 3 else O.unshift(planes[0])
+    return img8, planes
+
+
+@pytest.mark.parametrize("w,h,nc", [(8, 5, 1), (12, 17, 3), (132, 40, 3), (260, 301, 3), (516, 20, 1),
+                                    (257, 300, 3), (1000, 3, 3), (5, 5, 1)])
+@pytest.mark.parametrize("reversible", [True, False])
+@pytest.mark.parametrize("levels", [1, 4])
+def test_dwt_inv_u8_entry(ctx, w, h, nc, reversible, levels):
+    """rg_dwt_inv_u8 (inverse transform + inverse colour into uint8) on the
+    oracle's indices, fused and unfused shapes: the oracle's reconstruction,
+    bit-exact (5/3 + RCT: the image itself)."""
+    from roger_amd._lib import RG_COLOUR_ICT, RG_COLOUR_RCT, RG_FILTER_53, RG_FILTER_97
+    img = rand_img(w, h, nc, w * 5 + h + levels)
+    delta = 0.4
+    _, _, q = O.encode_image(img, levels=levels, reversible=reversible, cbw=32, cbh=32, delta=delta)
+    filt = RG_FILTER_53 if reversible else RG_FILTER_97
+    colour = RG_COLOUR_NONE if nc == 1 else (RG_COLOUR_RCT if reversible else RG_COLOUR_ICT)
+    steps = None if reversible else np.array([s[2] for s in O.band_steps(w, h, levels, delta)], np.float32)
+    d_q = ctx.upload(q)
+    d_out = ctx.alloc(((w * h * nc + 15) // 16) * 16)
+    d_s = ctx.alloc(R.dwt_inv_u8_scratch_bytes(w, h, nc, colour, filt))
+    R.dwt_inv_u8(ctx, d_q, w, h, nc, colour, filt, levels, steps, d_out, d_s)
+    got = d_out.read(np.uint8, img.shape)
+    want, _ = _oracle_recon(img, q, w, h, nc, levels, reversible, delta)
+    np.testing.assert_array_equal(got, want)
+    if reversible:
+        np.testing.assert_array_equal(got, img)
+
+
+@pytest.mark.parametrize("w,h,nc,reversible", [(260, 301, 3, False), (132, 40, 3, True), (516, 20, 1, False)])
+def test_decoder_fused_back_end_planes_on_request(ctx, w, h, nc, reversible):
+    """A code-block decode ends in the fused last level (uint8 image, no FP32
+    planes); the decoder's host view of the planes (rg_decoder_map_out) is
+    then computed from the coefficients and equals the oracle's planes; the
+    image equals the oracle's reconstruction."""
+    import ctypes
+    from roger_amd._lib import lib
+    img = rand_img(w, h, nc, 11 * w + h)
+    levels, delta = 3, 1.0 / 64
+    enc = R.Encoder(ctx)
+    enc.encode_u8(img, levels=levels, reversible=reversible, cbw=32, cbh=32, delta=delta)
+    desc, res, blobs = enc.codeblocks()
+    q = enc.coefficients()
+    dec = R.Decoder(ctx)
+    dec.decode(desc, res, blobs, w, h, nc, levels=levels, reversible=reversible, cbw=32, cbh=32)
+    want, planes = _oracle_recon(img, q, w, h, nc, levels, reversible, delta)
+    np.testing.assert_array_equal(dec.image(), want)
+    hp = ctypes.c_void_p()
+    assert lib().rg_decoder_map_out(dec.handle, ctypes.byref(hp)) == 0
+    n = w * h * nc
+    if reversible:   # int16, interleaved by component
+        got = np.ctypeslib.as_array(ctypes.cast(hp, ctypes.POINTER(ctypes.c_int16)), (n,)).reshape(h, w, nc)
+        np.testing.assert_array_equal(np.moveaxis(got, 2, 0), planes.astype(np.int16))
+    else:
+        got = np.ctypeslib.as_array(ctypes.cast(hp, ctypes.POINTER(ctypes.c_float)), (n,)).reshape(h, w, nc)
+        np.testing.assert_array_equal(np.moveaxis(got, 2, 0).view(np.uint32), planes.view(np.uint32))
+    assert lib().rg_decoder_unmap_out(dec.handle, hp) == 0
+    enc.close()
+    dec.close()
