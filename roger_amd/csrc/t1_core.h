@@ -89,21 +89,21 @@ RG_DEV uint32_t win6(uint64_t m, int r0)
 
 RG_DEV uint32_t b1(uint32_t w, int i) { return (w >> i) & 1u; }
 
-// A lane's (context << 1 | bit) symbols of one stripe -- at most 10 (cleanup:
-// run-length, 2 uniform, sign, then 3 x (ZC, sign)) -- newest in the low
-// byte of a 96-bit shift register, so a push is a constant 8-bit shift.
-struct SymList {
-    uint32_t a0 = 0, a1 = 0, a2 = 0;
+// A lane's (context << 1 | bit) symbols of one stripe, oldest in the low
+// byte of f0: at most 10 (cleanup: run-length, 2 uniform, sign, then 3 x
+// (ZC, sign)).
+struct SymRun {
+    uint32_t f0 = 0, f1 = 0, f2 = 0;
     int n = 0;
-    RG_DEV void push_sym(uint32_t s)
-    {
-        a2 = (a2 << 8) | (a1 >> 24);
-        a1 = (a1 << 8) | (a0 >> 24);
-        a0 = (a0 << 8) | s;
-        ++n;
-    }
-    RG_DEV void push(int cx, int bit) { push_sym((uint32_t)((cx << 1) | bit)); }
 };
+
+// Byte k of spread(w) holds bits k, k+1, ... of w (w < 64): the four copies
+// of w shifted left by 7k do not overlap, so the product is their OR, and
+// bits 0..5-k of byte k are clean.  One 24-bit multiply turns a 6-bit row
+// window into the four rows' neighbourhoods, one row per byte.
+RG_DEV uint32_t spread(uint32_t w) { return wv_mul24(w, 0x00204081u); }
+// bit k of a 4-bit row mask -> bit 0 of byte k
+RG_DEV uint32_t spread4(uint32_t m) { return spread(m) & 0x01010101u; }
 
 // Context tables of one code-block in workgroup memory (filled by
 // t1_lut_fill for the block's orientation):
@@ -114,6 +114,32 @@ struct SymList {
 //            negative) pairs of the left (bits 0-1), right (2-3), above (4-5)
 //            and below (6-7) neighbours.
 constexpr int LUT_BYTES = 512;
+// The context modelling also keeps, after the context tables, the byte
+// selectors that pack a stripe's symbols in scan order: entry z | n << 4 (z:
+// rows with a ZC or refinement symbol, n: rows with a sign symbol) holds the
+// v_perm selectors (two words) that take, row by row, byte k of the first
+// word (k) and then byte k of the sign word (4 + k); unused bytes select zero.
+constexpr int SEL_BYTES = 256 * 8;
+constexpr int CM_LDS_BYTES = LUT_BYTES + SEL_BYTES;
+
+RG_DEV void t1_sel_fill(uint8_t *sel)
+{
+    const int lane = wv_lane();
+    uint32_t *w = reinterpret_cast<uint32_t *>(sel);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int x = lane * 4 + e, z = x & 15, n = x >> 4;
+        uint64_t v = 0x0C0C0C0C0C0C0C0Cull;
+        int j = 0;
+        for (int k = 0; k < 4; ++k) {
+            if ((z >> k) & 1) { v = (v & ~(0xFFull << (8 * j))) | ((uint64_t)k << (8 * j)); ++j; }
+            if ((n >> k) & 1) { v = (v & ~(0xFFull << (8 * j))) | ((uint64_t)(4 + k) << (8 * j)); ++j; }
+        }
+        w[2 * x] = (uint32_t)v;
+        w[2 * x + 1] = (uint32_t)(v >> 32);
+    }
+}
+
 
 RG_DEV void t1_lut_fill(uint8_t *lut, int orient)
 {
@@ -136,23 +162,48 @@ RG_DEV void t1_lut_fill(uint8_t *lut, int orient)
     wv_sync();
 }
 
-// Sign symbol for window row i with neighbour windows: own column (cur state
-// above, pre state below), left (Lw), right (rw); sign windows negw/neglw/negrw.
-// (In a raw pass under BYPASS the sign goes out as itself, with no context:
-// the context modelling pushes the bit alone.)
-RG_DEV void push_sign(SymList &sl, const uint8_t *lut, int i, uint32_t cur, uint32_t sw, uint32_t Lw, uint32_t rw,
-                      uint32_t negw, uint32_t neglw, uint32_t negrw)
+// ZC and sign symbols of the four rows of a stripe, one row per byte, in a
+// significance-propagation or cleanup pass.  Windows (bit 0 = the row above
+// the stripe): sw own column before the pass, Lw the left column's final
+// state, rw the right column before the pass, N the rows of this column that
+// become significant (so the row above row k is significant at its turn iff
+// bit k of sw | N << 1); negw/neglw/negrw the sign windows; bb bit k = row k's
+// magnitude bit.  Table indices as t1_lut_fill builds them: ZC bits 0-2 left
+// rows k-1..k+1, 3-5 right rows k-1..k+1, 6 above, 7 below; sign bits 0-1
+// left (significant, negative), 2-3 right, 4-5 above, 6-7 below.  Zs byte k =
+// ZC context << 1 | bit; Ss byte k = sign context << 1 | (sign XOR bit), built
+// only when want_sc (a wave-uniform flag: some lane has a new significance).
+RG_DEV void zc_sc_syms(const uint8_t *lut, uint32_t N, uint32_t sw, uint32_t Lw, uint32_t rw, uint32_t negw,
+                       uint32_t neglw, uint32_t negrw, uint32_t bb, bool want_sc, uint32_t &Zs, uint32_t &Ss)
 {
-    const uint32_t x = b1(Lw, i) | (b1(neglw, i) << 1) | (b1(rw, i) << 2) | (b1(negrw, i) << 3) |
-                       (b1(cur, i - 1) << 4) | (b1(negw, i - 1) << 5) | (b1(sw, i + 1) << 6) | (b1(negw, i + 1) << 7);
-    const uint32_t sc = lut[256 + x];
-    sl.push_sym((sc & ~1u) | (b1(negw, i) ^ (sc & 1u)));
+    const uint32_t mL = spread(Lw), mR = spread(rw), mC = spread(sw | (N << 1)), mS = spread(sw);
+    const uint32_t zi = (mL & 0x07070707u) | ((mR << 3) & 0x38383838u) | ((mC << 6) & 0x40404040u) |
+                        ((mS << 5) & 0x80808080u);
+    const uint32_t z0 = lut[zi & 0xFFu], z1 = lut[(zi >> 8) & 0xFFu], z2 = lut[(zi >> 16) & 0xFFu],
+                   z3 = lut[zi >> 24];
+    Zs = ((z0 | (z1 << 8) | (z2 << 16) | (z3 << 24)) << 1) | spread4(bb);
+    Ss = 0;
+    if (want_sc) {
+        const uint32_t mN = spread(negw);
+        const uint32_t si = ((mL >> 1) & 0x01010101u) | (spread(neglw) & 0x02020202u) | ((mR << 1) & 0x04040404u) |
+                            ((spread(negrw) << 2) & 0x08080808u) | ((mC << 4) & 0x10101010u) |
+                            ((mS << 4) & 0x40404040u) | ((mN << 5) & 0xA0A0A0A0u);
+        const uint8_t *sc = lut + 256;
+        const uint32_t s0 = sc[si & 0xFFu], s1 = sc[(si >> 8) & 0xFFu], s2 = sc[(si >> 16) & 0xFFu], s3 = sc[si >> 24];
+        Ss = (s0 | (s1 << 8) | (s2 << 16) | (s3 << 24)) ^ ((mN >> 1) & 0x01010101u);
+    }
 }
 
-// ZC context of window row i (own column: cur above, sw below).
-RG_DEV int zc_at(const uint8_t *lut, int i, uint32_t cur, uint32_t sw, uint32_t Lw, uint32_t rw)
+// A stripe's symbols in scan order: row by row, Zs byte k if bit k of Z,
+// then Ss byte k if bit k of Nm (sel: the selector table).
+RG_DEV SymRun pack_run(const uint8_t *sel, uint32_t Z, uint32_t Nm, uint32_t Zs, uint32_t Ss)
 {
-    return lut[((Lw >> (i - 1)) & 7u) | (((rw >> (i - 1)) & 7u) << 3) | (b1(cur, i - 1) << 6) | (b1(sw, i + 1) << 7)];
+    const uint32_t *w = reinterpret_cast<const uint32_t *>(sel) + 2 * (Z | (Nm << 4));
+    SymRun r;
+    r.f0 = wv_perm(Ss, Zs, w[0]);
+    r.f1 = wv_perm(Ss, Zs, w[1]);
+    r.n = __builtin_popcount(Z) + __builtin_popcount(Nm);
+    return r;
 }
 
 struct MQEnc {
@@ -227,18 +278,18 @@ RG_DEV void mq_encode(MQEnc &m, int &ctxv, int mqtv, int cx, int d, int lane)
 }
 
 // Codes one stripe's symbols in scan order: columns (lanes) left to right,
-// each lane's own list oldest first.  Lanes without symbols are skipped via
-// the ballot mask; a lane's list is read with three readlanes.
-RG_DEV void mq_stripe(MQEnc &m, int &ctxv, int mqtv, const SymList &sl, int lane)
+// each lane's own run oldest first.  Lanes without symbols are skipped via
+// the ballot mask; a lane's run is read with three readlanes.
+RG_DEV void mq_stripe(MQEnc &m, int &ctxv, int mqtv, const SymRun &r, int lane)
 {
-    uint64_t mask = wv_ballot(sl.n > 0);
+    uint64_t mask = wv_ballot(r.n > 0);
     while (mask) {
         const int j = __builtin_ctzll(mask);
         mask &= mask - 1;
-        const int n = wv_readlane(sl.n, j);
-        const uint32_t x0 = (uint32_t)wv_readlane((int)sl.a0, j), x1 = (uint32_t)wv_readlane((int)sl.a1, j),
-                       x2 = (uint32_t)wv_readlane((int)sl.a2, j);
-        for (int t = n - 1; t >= 0; --t) {     // byte t holds the symbol pushed (n - 1 - t)-th
+        const int n = wv_readlane(r.n, j);
+        const uint32_t x0 = (uint32_t)wv_readlane((int)r.f0, j), x1 = (uint32_t)wv_readlane((int)r.f1, j),
+                       x2 = (uint32_t)wv_readlane((int)r.f2, j);
+        for (int t = 0; t < n; ++t) {
             const uint32_t w = t < 4 ? x0 : (t < 8 ? x1 : x2);
             const int sym = (int)((w >> ((t & 3) * 8)) & 0xFFu);
             mq_encode(m, ctxv, mqtv, sym >> 1, sym & 1, lane);
@@ -302,26 +353,22 @@ RG_DEV int wave_excl_count(int n, int &total)
 }
 
 // Split path: appends one stripe's symbols (scan order) to the block's
-// stream.  The register holds them newest first, so they are stored from the
-// end of the lane's range backwards.
-RG_DEV void emit_stripe(const SymList &sl, const T1Job &job, int &pos, int &ovf, int lane)
+// stream, each lane's run at its exclusive prefix count.
+RG_DEV void emit_run(const SymRun &r, const T1Job &job, int &pos, int &ovf)
 {
     int total;
-    const int off = pos + wave_excl_count(sl.n, total);
+    const int off = pos + wave_excl_count(r.n, total);
     if (total == 0) return;
     if (pos + total > job.symcap) { ovf = 1; return; }
     // every lane's range must lie inside the stripe's [pos, pos + total)
-    const bool inside = off >= pos && off + sl.n <= pos + total;
+    const bool inside = off >= pos && off + r.n <= pos + total;
     if (wv_any(!inside)) { ovf = 1; return; }
-    uint8_t *last = job.sym + off + sl.n - 1;
-    uint32_t a0 = sl.a0, a1 = sl.a1, a2 = sl.a2;
+    uint8_t *d = job.sym + off;
 #pragma unroll
-    for (int t = 0; t < 10; ++t) {
-        if (!wv_any(t < sl.n)) break;
-        if (t < sl.n) last[-t] = (uint8_t)a0;
-        a0 = (a0 >> 8) | (a1 << 24);
-        a1 = (a1 >> 8) | (a2 << 24);
-        a2 >>= 8;
+    for (int t = 0; t < 12; ++t) {
+        if (!wv_any(t < r.n)) break;
+        const uint32_t w = t < 4 ? r.f0 : (t < 8 ? r.f1 : r.f2);
+        if (t < r.n) d[t] = (uint8_t)(w >> (8 * (t & 3)));
     }
     pos += total;
 }
@@ -364,7 +411,8 @@ RG_DEV void t1_block(const T1Job &job, uint8_t *lut, CMPhase ph = CMPhase())
     }
     const int npass = 3 * nbp - 2;
     if (SPLIT && ph.qbeg >= npass) return;   // a later phase with nothing left: the record is final
-    t1_lut_fill(lut, orient);
+    t1_sel_fill(lut + LUT_BYTES);
+    t1_lut_fill(lut, orient);   // (ends with the wavefront barrier both fills need)
     int spos = 0, sovf = 0;   // split path: stream position / overflow (uniform)
     int pidx = 0;             // split path: coding pass index
 
@@ -421,156 +469,126 @@ RG_DEV void t1_block(const T1Job &job, uint8_t *lut, CMPhase ph = CMPhase())
             }
             ++pidx;
             const uint64_t S0 = S;
+            // under BYPASS a raw pass (this pass is pidx - 1) sends its decisions
+            // and signs as bits, without contexts
+            const bool rawp = SPLIT && pass != 2 && pass_is_raw(job.style, pidx - 1);
             for (int s = 0; s < nstripes; ++s) {
                 const int r0 = 4 * s;
                 const uint32_t vrow = colok ? (uint32_t)(rowmask >> r0) & 0xFu : 0u;
                 const uint32_t sw = win6(S, r0);
                 const uint32_t lw = wv_from_left(sw), rw = wv_from_right(sw);
-                const uint32_t bw = win6(B, r0), negw = win6(NEG, r0);
-                const uint32_t neglw = win6(negl, r0), negrw = win6(negr, r0);
-                SymList sl;
+                const uint32_t bb = (win6(B, r0) >> 1) & 0xFu;   // bit k: row k's magnitude bit
+                const uint32_t own = ~(sw >> 1) & vrow & 0xFu;     // valid rows not yet significant
+                // Every pass reduces to: Zm rows with a ZC / refinement / raw-bit
+                // symbol (bytes of Zs), Nm rows with a sign symbol after it (bytes
+                // of Ss), and a cleanup run-length prefix of plen bytes.
+                uint32_t Zm, Nm = 0, Zs, Ss = 0, pre = 0;
+                int plen = 0;
 
                 if (pass == 0) {
                     // ---- significance propagation ----
-                    // (under BYPASS a raw pass -- this pass is pidx - 1 -- sends its
-                    // decisions and signs as bits, without contexts)
-                    const bool rawp = SPLIT && pass_is_raw(job.style, pidx - 1);
-                    // Candidates: insignificant samples.  hn: rows with a neighbour that
-                    // is significant before the pass (own column above and below, rows
-                    // i-1..i+1 of the left and right columns).  Joining the pass needs a
-                    // neighbour significant before it or newly significant in it, and the
-                    // first newly significant sample needs the former, so if no candidate
-                    // of any column has one the stripe has no pass symbols.
-                    const uint32_t cand0 = vrow & ~(sw >> 1) & 0xFu;
-                    const uint32_t hn = (sw | (sw >> 2) | lw | (lw >> 1) | (lw >> 2) | rw | (rw >> 1) | (rw >> 2)) & 0xFu;
-                    if (!wv_any((hn & cand0) != 0)) continue;
-                    uint32_t N = 0, P = 0, NL = 0, Lw = lw;
+                    // Row k joins the pass if a neighbour is significant at its turn:
+                    // before the pass (own column above and below, rows k-1..k+1 of
+                    // the right column and of the left one), or newly significant in
+                    // it (the row above; the left column, coded earlier).  If no
+                    // candidate of any column has one of the former, the stripe has
+                    // no pass symbols.
+                    const uint32_t hpre = (sw | (sw >> 2) | rw | (rw >> 1) | (rw >> 2)) & 0xFu;
+                    if (!wv_any(((hpre | lw | (lw >> 1) | (lw >> 2)) & own) != 0)) continue;
+                    // joining through the row above: it joined and its bit is 1
+                    const uint32_t q = own & (bb << 1);
+                    uint32_t P = 0, N = 0, NL = 0;
                     // monotone fix-point over the left-column dependency: converges within
                     // 65 rounds (a chain across <= 64 lanes).  Another round runs only when
                     // the left column's newly significant samples border a row of this
                     // column that is not yet in the pass (the only way membership grows).
                     for (int round = 0; round < 80; ++round) {
-                        Lw = lw | (NL << 1);
-                        uint32_t cur = sw, Nn = 0;
-                        P = 0;
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            const int i = k + 1;
-                            if (!((vrow >> k) & 1u) || b1(cur, i)) continue;
-                            uint32_t nb = b1(cur, i - 1) | b1(sw, i + 1) | ((Lw >> (i - 1)) & 7u) | ((rw >> (i - 1)) & 7u);
-                            if (nb) {
-                                P |= 1u << k;
-                                if (b1(bw, i)) { Nn |= 1u << k; cur |= 1u << i; }
-                            }
-                        }
-                        N = Nn;
+                        const uint32_t Lr = lw | (NL << 1);
+                        P = own & (hpre | Lr | (Lr >> 1) | (Lr >> 2));
+                        P |= q & (P << 1);   // a newly significant row pulls in the one below,
+                        P |= q & (P << 1);   // at most three times down the stripe
+                        P |= q & (P << 1);
+                        N = P & bb;
                         const uint32_t NLn = wv_from_left(N);
                         const uint32_t grown = NLn & ~NL;
                         NL = NLn;
-                        if (!wv_any(((grown | (grown << 1) | (grown >> 1)) & cand0 & ~P) != 0)) break;
+                        if (!wv_any(((grown | (grown << 1) | (grown >> 1)) & own & ~P) != 0)) break;
                     }
-                    Lw = lw | (NL << 1);   // the left column's final state for the contexts
+                    const uint32_t negw = win6(NEG, r0);
                     if (rawp) {
-                        // raw pass: the decision bits alone (the raw coder reads bit 0
-                        // of each symbol), so no context lookups
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            if (!((P >> k) & 1u)) continue;
-                            const uint32_t bit = b1(bw, k + 1);
-                            sl.push_sym(bit);
-                            if (bit) sl.push_sym(b1(negw, k + 1));
-                        }
+                        Zs = spread4(bb);
+                        Ss = (spread(negw) >> 1) & 0x01010101u;
                     } else {
-                        uint32_t cur = sw;
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            const int i = k + 1;
-                            if (!((P >> k) & 1u)) continue;
-                            const int bit = (int)b1(bw, i);
-                            sl.push(zc_at(lut, i, cur, sw, Lw, rw), bit);
-                            if (bit) {
-                                push_sign(sl, lut, i, cur, sw, Lw, rw, negw, neglw, negrw);
-                                cur |= 1u << i;
-                            }
-                        }
+                        // the left column's final state for the contexts
+                        zc_sc_syms(lut, N, sw, lw | (NL << 1), rw, negw, win6(negl, r0), win6(negr, r0), bb,
+                                   wv_any(N != 0), Zs, Ss);
                     }
+                    Zm = P;
+                    Nm = N;
                     S |= (uint64_t)N << r0;
                     V |= (uint64_t)P << r0;
                 } else if (pass == 1) {
-                    // ---- magnitude refinement ----
+                    // ---- magnitude refinement (no intra-pass dependency) ----
                     const uint32_t vw = (uint32_t)(V >> r0) & 0xFu;
                     const uint32_t rf = (uint32_t)(R >> r0) & 0xFu;
-                    const uint32_t M = (sw >> 1) & ~vw & vrow;
-                    if (!wv_any(M != 0)) continue;
-                    if (SPLIT && pass_is_raw(job.style, pidx - 1)) {
-#pragma unroll
-                        for (int k = 0; k < 4; ++k)
-                            if ((M >> k) & 1u) sl.push_sym(b1(bw, k + 1));
-                        R |= (uint64_t)M << r0;
-                        emit_stripe(sl, job, spos, sovf, lane);
-                        continue;
+                    Zm = (sw >> 1) & ~vw & vrow & 0xFu;
+                    if (!wv_any(Zm != 0)) continue;
+                    if (rawp) {
+                        Zs = spread4(bb);
+                    } else {
+                        // context 16 after the first refinement, else 15 with a
+                        // significant neighbour, else 14
+                        const uint32_t t = lw | rw;
+                        const uint32_t nb = (sw | (sw >> 2) | t | (t >> 1) | (t >> 2)) & ~rf & 0xFu;
+                        Zs = ((0x0E0E0E0Eu + spread4(nb) + (spread4(rf) << 1)) << 1) | spread4(bb);
                     }
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        const int i = k + 1;
-                        if (!((M >> k) & 1u)) continue;
-                        int cx;
-                        if ((rf >> k) & 1u) cx = 16;
-                        else {
-                            uint32_t nb = b1(sw, i - 1) | b1(sw, i + 1) | ((lw >> (i - 1)) & 7u) | ((rw >> (i - 1)) & 7u);
-                            cx = nb ? 15 : 14;
-                        }
-                        sl.push(cx, (int)b1(bw, i));
-                    }
-                    R |= (uint64_t)M << r0;
+                    R |= (uint64_t)Zm << r0;
                 } else {
                     // ---- cleanup ----
                     const uint32_t vw = (uint32_t)(V >> r0) & 0xFu;
-                    const uint32_t Cm = ~(sw >> 1) & ~vw & vrow & 0xFu;
+                    const uint32_t Cm = own & ~vw;
                     // every sample of the stripe significant or visited: no cleanup
                     // symbols (most stripes of the low bit-planes)
                     if (!wv_any(Cm != 0)) continue;
-                    const uint32_t N = Cm & (bw >> 1);
+                    const uint32_t N = Cm & bb;   // newly significant rows are known up front
                     const uint32_t NL = wv_from_left(N);
                     const uint32_t Lw = lw | (NL << 1);
                     const bool full = r0 + 4 <= h;
                     const bool rl = full && Cm == 0xFu && (Lw & 0x3Fu) == 0 && (rw & 0x3Fu) == 0 && (sw & 0x21u) == 0;
-                    uint32_t cur = sw;
-                    int kstart = 0;
-                    bool done = false;
+                    // run-length mode: (RL, 0) for an all-zero column stripe, else
+                    // (RL, 1), the first 1's row as two UNIFORM bits, its sign, and
+                    // ZC coding of the rows below it
+                    const int kf = __builtin_ctz(N | 0x10u);
+                    Zm = rl ? ((0x1Eu << kf) & 0xFu) : Cm;
+                    Nm = N;
                     if (rl) {
-                        if (N == 0) {
-                            sl.push(CTX_RL, 0);
-                            done = true;
-                        } else {
-                            const int k = __builtin_ctz(N);
-                            sl.push(CTX_RL, 1);
-                            sl.push(CTX_UNI, k >> 1);
-                            sl.push(CTX_UNI, k & 1);
-                            push_sign(sl, lut, k + 1, cur, sw, Lw, rw, negw, neglw, negrw);
-                            cur |= 1u << (k + 1);
-                            kstart = k + 1;
-                        }
+                        pre = N ? ((2u * CTX_RL + 1) | ((2u * CTX_UNI + (uint32_t)(kf >> 1)) << 8) |
+                                 ((2u * CTX_UNI + (uint32_t)(kf & 1)) << 16))
+                              : 2u * CTX_RL;
+                        plen = N ? 3 : 1;
                     }
-                    if (!done) {
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            const int i = k + 1;
-                            if (k < kstart || !((Cm >> k) & 1u)) continue;
-                            const int bit = (int)((N >> k) & 1u);
-                            sl.push(zc_at(lut, i, cur, sw, Lw, rw), bit);
-                            if (bit) {
-                                push_sign(sl, lut, i, cur, sw, Lw, rw, negw, neglw, negrw);
-                                cur |= 1u << i;
-                            }
-                        }
-                    }
+                    Zs = 0;
+                    if (wv_any((Zm | Nm) != 0))
+                        zc_sc_syms(lut, N, sw, Lw, rw, win6(NEG, r0), win6(negl, r0), win6(negr, r0), bb,
+                                   wv_any(N != 0), Zs, Ss);
                     S |= (uint64_t)N << r0;
                 }
 
+                SymRun run = pack_run(lut + LUT_BYTES, Zm, Nm, Zs, Ss);
+                if (pass == 2) {
+                    // the run-length prefix goes in front (plen 0, 1 or 3 bytes)
+                    const int sh = 8 * plen;
+                    const uint64_t body = ((uint64_t)run.f1 << 32) | run.f0;
+                    const uint64_t lo = body << sh;
+                    run.f2 = (uint32_t)(((uint64_t)run.f1 << sh) >> 32);
+                    run.f0 = (uint32_t)lo | pre;
+                    run.f1 = (uint32_t)(lo >> 32);
+                    run.n += plen;
+                }
+
                 // ---- MQ-code this stripe's symbols in scan order (or hand them on) ----
-                if constexpr (SPLIT) emit_stripe(sl, job, spos, sovf, lane);
-                else mq_stripe(m, ctxv, mqtv, sl, lane);
+                if constexpr (SPLIT) emit_run(run, job, spos, sovf);
+                else mq_stripe(m, ctxv, mqtv, run, lane);
             }
             if (pass == 2) V = 0;
             if (SPLIT && pass == 0 && job.sppm) job.sppm[p * 64 + lane] = S & ~S0;

@@ -34,7 +34,7 @@ __global__ void __launch_bounds__(64) t1_encode_kernel(const int32_t *__restrict
     job.out = out + (size_t)blockIdx.x * slot_bytes;
     job.res = res + blockIdx.x;
     job.style = 0;
-    __shared__ uint8_t lut[LUT_BYTES];
+    __shared__ __align__(16) uint8_t lut[CM_LDS_BYTES];
     t1_block<false>(job, lut);
 }
 
@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(64) t1_encode_kernel(const int32_t *__restrict
 // never leave the loop.)
 __global__ void __launch_bounds__(64) t1_encode_jobs_kernel(const T1Job *__restrict__ jobs)
 {
-    __shared__ uint8_t lut[LUT_BYTES];
+    __shared__ __align__(16) uint8_t lut[CM_LDS_BYTES];
     t1_block<false>(jobs[blockIdx.x], lut);
 }
 
@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(64) t1_encode_jobs_kernel(const T1Job *__restr
 template <bool SPLIT>
 __device__ __forceinline__ void t1_queue_loop(const T1Job *__restrict__ jobs, int njobs, int *__restrict__ counter)
 {
-    __shared__ uint8_t lut[LUT_BYTES];
+    __shared__ __align__(16) uint8_t lut[CM_LDS_BYTES];
     __shared__ int next;
     for (;;) {
         if (threadIdx.x == 0) next = atomicAdd(counter, 1);
@@ -118,7 +118,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
     job.style = style;
     job.cms = cms ? cms + (size_t)blockIdx.x * 64 : nullptr;
     job.sppm = sppm ? sppm + (size_t)blockIdx.x * RD_PLANES * 64 : nullptr;
-    __shared__ uint8_t lut[LUT_BYTES];
+    __shared__ __align__(16) uint8_t lut[CM_LDS_BYTES];
     CMPhase ph;
     ph.qbeg = qbeg;
     ph.qend = qend;
@@ -127,7 +127,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
 
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) t1_cm_jobs_kernel(const T1Job *__restrict__ jobs)
 {
-    __shared__ uint8_t lut[LUT_BYTES];
+    __shared__ __align__(16) uint8_t lut[CM_LDS_BYTES];
     t1_block<true>(jobs[blockIdx.x], lut);
 }
 

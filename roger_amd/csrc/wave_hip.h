@@ -64,6 +64,8 @@ RG_DEV int wv_incl_scan(int v)
     v += __builtin_amdgcn_update_dpp(0, v, 0x143 /* row_bcast:31 */, 0xC, 0xF, false);
     return v;
 }
+// 24-bit multiply (v_mul_u32_u24): low 32 bits of a * b for a, b < 2^24.
+RG_DEV uint32_t wv_mul24(uint32_t a, uint32_t b) { return __umul24(a, b); }
 // Byte permute (v_perm_b32): byte i of the result is byte s.byte[i] of the
 // 8-byte value hi:lo (selector 12 gives 0x00).
 RG_DEV uint32_t wv_perm(uint32_t hi, uint32_t lo, uint32_t s) { return __builtin_amdgcn_perm(hi, lo, s); }

@@ -153,7 +153,7 @@ void run(void (*fn)(void *), void *arg)
 
 namespace {
 uint32_t g_tab[48];
-uint8_t g_lut[LUT_BYTES];   // the kernels' workgroup-memory context tables
+alignas(16) uint8_t g_lut[CM_LDS_BYTES];   // the kernels' workgroup memory: context tables, symbol selectors
 alignas(16) uint8_t g_ring[MQ_LDS_BYTES]; // and the lane MQ coders' output rings and context words
 
 void inline_body(void *a) { t1_block<false>(*static_cast<T1Job *>(a), g_lut); }

@@ -102,6 +102,7 @@ static inline uint32_t wv_from_right(uint32_t x)
 static inline void wv_sched_fence() {}
 static inline uint32_t wv_sload_u8(const uint8_t *p) { return *p; }
 static inline void wv_lds_or(uint32_t *p, uint32_t v) { *p |= v; }
+static inline uint32_t wv_mul24(uint32_t a, uint32_t b) { return (a & 0xFFFFFFu) * (b & 0xFFFFFFu); }
 static inline uint32_t wv_perm(uint32_t hi, uint32_t lo, uint32_t s)
 {
     const uint64_t v = ((uint64_t)hi << 32) | lo;
