@@ -353,22 +353,24 @@ RG_DEV int wave_excl_count(int n, int &total)
 }
 
 // Split path: appends one stripe's symbols (scan order) to the block's
-// stream, each lane's run at its exclusive prefix count.
+// stream, each lane's run at its exclusive prefix count (so every lane's
+// range lies inside the stripe's [pos, pos + total), inside the stream).
+// Byte t of the run goes out in the t-th store; lanes whose run is shorter
+// aim it outside the stream, where the sink's range check drops it (no
+// exec-mask branch).
 RG_DEV void emit_run(const SymRun &r, const T1Job &job, int &pos, int &ovf)
 {
     int total;
     const int off = pos + wave_excl_count(r.n, total);
     if (total == 0) return;
     if (pos + total > job.symcap) { ovf = 1; return; }
-    // every lane's range must lie inside the stripe's [pos, pos + total)
-    const bool inside = off >= pos && off + r.n <= pos + total;
-    if (wv_any(!inside)) { ovf = 1; return; }
-    uint8_t *d = job.sym + off;
+    const ByteSink snk = wv_byte_sink(job.sym, job.symcap);
 #pragma unroll
     for (int t = 0; t < 12; ++t) {
-        if (!wv_any(t < r.n)) break;
+        const bool has = t < r.n;
+        if (!wv_any(has)) break;
         const uint32_t w = t < 4 ? r.f0 : (t < 8 ? r.f1 : r.f2);
-        if (t < r.n) d[t] = (uint8_t)(w >> (8 * (t & 3)));
+        wv_sink_store(snk, has ? off : SINK_DROP, t, w >> (8 * (t & 3)));
     }
     pos += total;
 }

@@ -89,6 +89,25 @@ RG_DEV int wv_mbcnt(uint64_t m)
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Byte stores into [base, base + size) through a raw buffer resource: the
+// hardware range check discards a store whose lane offset lies outside, so a
+// lane with nothing to write aims at SINK_DROP instead of branching on the
+// exec mask (base and size wave-uniform).  wv_sink_store writes byte v at
+// off + k, k a small constant added after the range check (the scalar offset
+// field), so off + k must lie inside whenever off does.
+struct ByteSink {
+    __amdgpu_buffer_rsrc_t r;
+};
+constexpr int SINK_DROP = 0x40000000;   // beyond every sink size
+RG_DEV ByteSink wv_byte_sink(uint8_t *base, int size)
+{
+    return ByteSink{__builtin_amdgcn_make_buffer_rsrc(base, (short)0, size, 0x00020000)};
+}
+RG_DEV void wv_sink_store(const ByteSink &s, int off, int k, uint32_t v)
+{
+    __builtin_amdgcn_raw_buffer_store_b8((uint8_t)v, s.r, off, k, 0);
+}
+
 // 16-byte global-memory load / store (p 16-byte aligned).  The explicit global
 // address space gives global_load/store (vector-memory counter only) where a
 // generic pointer would give flat instructions, whose completion every later

@@ -36,6 +36,20 @@ static inline u32x4 wv_load16(const uint8_t *p)
     memcpy(&v, p, 16);
     return v;
 }
+// Byte stores into [base, base + size); an offset outside is dropped (the
+// kernels' raw buffer range check).
+struct ByteSink {
+    uint8_t *base;
+    int size;
+};
+constexpr int SINK_DROP = 0x40000000;
+static inline ByteSink wv_byte_sink(uint8_t *base, int size) { return ByteSink{base, size}; }
+static inline void wv_sink_store(const ByteSink &s, int off, int k, uint32_t v)
+{
+    if (off < 0 || off >= s.size) return;
+    if (off + k >= s.size) { fprintf(stderr, "wave emulator: sink store past its range\n"); exit(3); }
+    s.base[off + k] = (uint8_t)v;
+}
 static inline void wv_store16(uint8_t *p, uint32_t a, uint32_t b, uint32_t c, uint32_t d)
 {
     if (((uintptr_t)p & 15) != 0) { fprintf(stderr, "wave emulator: misaligned 16-byte store\n"); exit(3); }
