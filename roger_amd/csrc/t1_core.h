@@ -743,6 +743,34 @@ RG_DEV void mq_code(RingMQ &m, uint32_t rec, uint8_t *rb)
     ring_renorm(m, (int)(rec >> 16), rb);
 }
 
+// mq_code for a shift of at most 7 (so at most one BYTEOUT: after one, CT is
+// 7 or 8 and fewer than 7 shifts remain), with the pending byte's class taken
+// before C is known: the C chain is add-shift, the carry bit, the next byte's
+// width, the mask and the remaining shift.  Same state updates as mq_code.
+RG_DEV void mq_code7(RingMQ &m, uint32_t rec, uint8_t *rb)
+{
+    const int n = (int)(rec >> 16);
+    const bool need = n >= m.CT;                     // CT reaches 0 inside this shift
+    const int sft = need ? m.CT : n;
+    const uint32_t isff = m.B == 0xFFu ? 1u : 0u, isfe = m.B == 0xFEu ? 1u : 0u;
+    const uint32_t cyok = need && !isff ? 1u : 0u;   // a carry bit would go into B
+    const uint32_t x = (m.C + (rec & 0xFFFFu)) << sft;
+    const uint32_t cy = (x >> 27) & cyok;
+    // after B = 0xFF (as it was, or by this carry) the next byte takes 7 bits
+    const uint32_t big = isff | (isfe & cy);
+    rb[m.bp & 63] = (uint8_t)(m.B + cy);   // the pending byte's slot (rewritten before use when !need)
+    const int rem = n - sft;
+    if (need) {
+        m.bp += 1;
+        m.B = (x - (cy << 27)) >> (19 + big);
+        m.C = (x & (0x7FFFFu | (big << 19))) << rem;
+        m.CT = 8 - (int)big - rem;
+    } else {
+        m.C = x;
+        m.CT -= n;
+    }
+}
+
 // Moves the ring's completed 16-byte groups [fl, bp) to out (bytes at or
 // beyond cap only set the overflow flag).
 RG_DEV void ring_flush(RingMQ &m, const uint8_t *rb, uint8_t *out, int cap)
@@ -884,8 +912,18 @@ RG_DEV int ring_chain(const uint32_t *tab, uint8_t *ring, bool active, const uin
             for (int k = 0; k < 16; ++k) rec[k] = hand[((i & 1) * 16 + k) * 64];
         }
         if (CD) {
+            // shifts of 8 or more (an LPS in a very skewed state) are rare: a
+            // chunk without one in any lane takes the short form
+            uint32_t big = 0;
 #pragma unroll
-            for (int k = 0; k < 16; ++k) mq_code(m, rec[k], rb);
+            for (int k = 0; k < 16; ++k) big |= rec[k];
+            if (wv_any((big & 0x80000u) != 0)) {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) mq_code(m, rec[k], rb);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) mq_code7(m, rec[k], rb);
+            }
             ring_flush(m, rb, out, cap);
         }
     }
