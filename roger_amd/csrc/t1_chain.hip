@@ -124,7 +124,7 @@ __global__ void __launch_bounds__(256) chain_scatter_kernel(J job, int nblocks, 
 template <class J>
 __global__ void __launch_bounds__(128) chain_code_kernel(J job, const int *ws)
 {
-    __shared__ uint32_t tab[48];
+    __shared__ uint32_t tab[MQ_TT_WORDS];
     __shared__ __attribute__((aligned(16))) uint8_t ring[MQ2_LDS_BYTES];
     static_assert(MQ2_LDS_BYTES >= 2 * RING_BYTES, "one ring per raw wavefront");
     const int nmq = ws[CH_COUNTS], ntot = nmq + ws[CH_COUNTS + 1];

@@ -654,10 +654,34 @@ constexpr int MQ_LDS_BYTES = RING_BYTES + MQ_CTX * 64 * 4;
 // Initial context entries (Table D.7: context 0 in state 4, RL (17) in 3,
 // UNIFORM (18) in 46, all others in 0; MPS 0) for this lane (ce = its word
 // of context 0).
+// A lane coder's context entry: Qe | MPS << 16 | state index << 17.  The
+// transition table (MQ_TT_WORDS of workgroup memory) holds at 2 * state + lps
+// the entry of the state a renormalising decision moves to, with bit 16 set
+// where that transition switches the MPS, so the entry after a renormalising
+// step is TT[((e >> 16) & 0x7F) ^ d] ^ (e & 0x10000) (d the decision).
+constexpr int MQ_TT_WORDS = 96;
+RG_DEV uint32_t ce_entry(int st) { return (MQT[st] & 0xFFFFu) | ((uint32_t)st << 17); }
+RG_DEV uint32_t tt_entry(int k)
+{
+    const uint32_t t = MQT[k >> 1], l = (uint32_t)k & 1u;
+    const int j = (int)((t >> (l ? 22 : 16)) & 63u);
+    return ce_entry(j) | ((l & (t >> 28)) << 16);
+}
+// Fills the transition table (every lane of the workgroup calls it; a
+// barrier must follow).
+RG_DEV void mq_tt_fill(uint32_t *tab)
+{
+    const int lane = wv_lane();
+    if (wv_wave() == 0) {
+        tab[lane] = tt_entry(lane);
+        if (lane < 94 - 64) tab[64 + lane] = tt_entry(64 + lane);
+    }
+}
+
 RG_DEV void ce_init(uint32_t *ce)
 {
 #pragma unroll
-    for (int cx = 0; cx < MQ_CTX; ++cx) ce[64 * cx] = MQT[cx == 0 ? 4 : (cx == CTX_RL ? 3 : (cx == CTX_UNI ? 46 : 0))];
+    for (int cx = 0; cx < MQ_CTX; ++cx) ce[64 * cx] = ce_entry(cx == 0 ? 4 : (cx == CTX_RL ? 3 : (cx == CTX_UNI ? 46 : 0)));
 }
 
 struct RingMQ {
@@ -684,39 +708,31 @@ RG_DEV void ring_byteout(RingMQ &m, bool need, uint8_t *rb)
 }
 
 // Interval side of ENCODE (C.2.2-C.2.5) for decision d in a context with
-// entry e (tnew: the table entry of the state it moves to if it
-// renormalises): A is updated and renormalised, *enew is the context's entry
-// after the symbol.  Returns the symbol's record for the code side: the
-// amount added to C (Qe when the upper interval is kept, else 0) | the
-// renormalisation shift << 16.
-RG_DEV uint32_t mq_interval(uint32_t &A, uint32_t d, uint32_t e, uint32_t tnew, uint32_t lps_nmps, bool renorm,
-                            uint32_t &enew)
+// entry e (tnew: its transition-table word for d).  A is kept as A << 16, so
+// the renormalisation shift is a plain count of leading zeros and "A - Qe
+// below 0x8000" is the sign bit.  A is updated and renormalised, *enew is the
+// context's entry after the symbol.  Returns the symbol's record for the code
+// side: the amount added to C (Qe when the upper interval is kept, else 0) |
+// the renormalisation shift << 16.
+RG_DEV uint32_t mq_interval(uint32_t &A, uint32_t d, uint32_t e, uint32_t tnew, uint32_t &enew)
 {
-    const uint32_t qe = e & 0xFFFFu;
-    const uint32_t a = A - qe;
-    const bool lps = d != (e >> 31);
+    const uint32_t qe = e & 0xFFFFu, qe32 = e << 16;
+    const bool lps = ((e >> 16) ^ d) & 1u;
+    const uint32_t a = A - qe32;
+    const bool renorm = lps || (int32_t)a >= 0;
     // C.2.4 / C.2.5: the interval kept is the upper one (C += Qe) for an MPS
     // unless A < Qe (conditional exchange), and the reverse for an LPS
-    const bool small = a < qe;
+    const bool small = a < qe32;
     const bool upper = !renorm || (lps ? small : !small);
-    A = upper ? a : qe;
-    enew = renorm ? (tnew | (lps_nmps << 31)) : e;
-    const int n = wv_clz(A) - 16;
+    A = upper ? a : qe32;
+    enew = renorm ? (tnew ^ (e & 0x10000u)) : e;
+    const uint32_t n = (uint32_t)__builtin_clz(A);   // A != 0
     A <<= n;
-    return (upper ? qe : 0u) | ((uint32_t)n << 16);
+    return (upper ? qe : 0u) | (n << 16);
 }
 
-// The next state of a context with entry e after decision d: its table index
-// (NLPS after an LPS, NMPS after an MPS), the new MPS (switched after an LPS
-// when SWITCH is set), and whether the step renormalises.
-RG_DEV uint32_t ring_next(uint32_t d, uint32_t e, uint32_t A, uint32_t &nmps, bool &renorm)
-{
-    const uint32_t mps = e >> 31;
-    const bool lps = d != mps;
-    renorm = lps || !((A - (e & 0xFFFFu)) & 0x8000u);
-    nmps = mps ^ (lps ? ((e >> 28) & 1u) : 0u);
-    return (e >> (lps ? 22u : 16u)) & 63u;
-}
+// Transition-table word index of decision d in a context with entry e.
+RG_DEV uint32_t tt_index(uint32_t d, uint32_t e) { return ((e >> 16) & 0x7Fu) ^ d; }
 
 // RENORME's C side (C.2.6): n shifts with BYTEOUT whenever CT reaches 0.  C
 // takes min(n, CT), the predicated BYTEOUT, then the rest -- fewer than the 7
@@ -821,10 +837,7 @@ RG_DEV void mq_interval_chunk(const uint32_t *tab, uint32_t *ce, uint32_t &A, co
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             const uint32_t cx = sy >> 1, d = sy & 1u;
-            uint32_t nmps;
-            bool renorm;
-            const uint32_t nidx = ring_next(d, e, A, nmps, renorm);
-            const uint32_t tnew = tab[nidx];
+            const uint32_t tnew = tab[tt_index(d, e)];
             uint32_t sy2 = 0, e2 = 0;
             if (k < 15) {   // the next symbol's context, read before this one's store
                 sy2 = (w4[(k + 1) >> 2] >> (8 * ((k + 1) & 3))) & 0xFFu;
@@ -832,7 +845,7 @@ RG_DEV void mq_interval_chunk(const uint32_t *tab, uint32_t *ce, uint32_t &A, co
             }
             wv_sched_fence();
             uint32_t enew;
-            rec[k] = mq_interval(A, d, e, tnew, nmps, renorm, enew);
+            rec[k] = mq_interval(A, d, e, tnew, enew);
             ce[64 * cx] = enew;
             if (k < 15) e = (sy2 >> 1) == cx ? enew : e2;
             sy = sy2;
@@ -846,11 +859,8 @@ RG_DEV void mq_interval_chunk(const uint32_t *tab, uint32_t *ce, uint32_t &A, co
             const uint32_t sy = (w4[k >> 2] >> (8 * (k & 3))) & 0xFFu;
             const uint32_t cx = sy >> 1, d = sy & 1u;
             const uint32_t e = ce[64 * cx];
-            uint32_t nmps;
-            bool renorm;
-            const uint32_t nidx = ring_next(d, e, A, nmps, renorm);
             uint32_t enew;
-            rec[k] = mq_interval(A, d, e, tab[nidx], nmps, renorm, enew);
+            rec[k] = mq_interval(A, d, e, tab[tt_index(d, e)], enew);
             ce[64 * cx] = enew;
         }
     }
@@ -881,7 +891,7 @@ RG_DEV int ring_chain(const uint32_t *tab, uint8_t *ring, bool active, const uin
         const int t = wv_shfl_xor(nmax, o);
         nmax = nmax > t ? nmax : t;
     }
-    uint32_t A = 0x8000;
+    uint32_t A = 0x8000u << 16;   // interval side: A << 16
     RingMQ m;
     m.A = 0x8000; m.C = 0; m.B = 0; m.CT = 12; m.bp = -1; m.fl = 0; m.ovf = 0;
     const int last = nch > 0 ? c1 - 1 : c0;
@@ -927,6 +937,7 @@ RG_DEV int ring_chain(const uint32_t *tab, uint8_t *ring, bool active, const uin
             ring_flush(m, rb, out, cap);
         }
     }
+    A >>= 16;
     if (ROLE == ROLE_INTERVAL) {
         hand[2 * 16 * 64] = A;
         wg_sync();
@@ -972,7 +983,7 @@ RG_DEV int ring_chain_wg(const uint32_t *tab, uint8_t *ring, bool active, const 
 
 // MQ-codes one block per lane from the streams the context-modelling pass
 // wrote.  Block `i` (i >= njobs: idle lane) is either jobs[i] (QUEUE) or
-// entry i of the flat arrays.  `tab` is 48 words of workgroup-shared memory,
+// entry i of the flat arrays.  `tab` is MQ_TT_WORDS of workgroup-shared memory,
 // `ring` MQ_LDS_BYTES (MQ2_LDS_BYTES with SPLIT: two wavefronts, each lane's
 // block coded by lane l of both, ring_chain_wg) of it; the code side writes
 // the results.
@@ -981,7 +992,7 @@ RG_DEV void t1_mq_lanes(uint32_t *tab, uint8_t *ring, const T1Job *jobs, int i, 
                         int symcap, const i32x4 *cm, uint8_t *slots, int slot, rg_cb_result *res)
 {
     const int lane = wv_lane();
-    if (wv_wave() == 0 && lane < 47) tab[lane] = MQT[lane];
+    mq_tt_fill(tab);
     wg_sync();
     const bool active = i < njobs;
     i32x4 c = mk_i32x4(0, 0, 0, 0);
@@ -1024,7 +1035,7 @@ RG_DEV void t1_mq_pass_lane(uint32_t *tab, uint8_t *ring, bool active, const uin
                             uint8_t *seg, int cap, int32_t *len_out)
 {
     const int lane = wv_lane();
-    if (wv_wave() == 0 && lane < 47) tab[lane] = MQT[lane];
+    mq_tt_fill(tab);
     wg_sync();
     bool ovf;
     const int len = ring_chain_wg<SPLIT>(tab, ring, active, sym, s0, s1, seg, cap, ovf);

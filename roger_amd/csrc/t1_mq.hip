@@ -16,7 +16,7 @@ __global__ void __launch_bounds__(128) t1_mq_kernel(const uint8_t *__restrict__ 
                                                     const int4 *__restrict__ cm, int ncb, uint8_t *__restrict__ slots,
                                                     int slot, rg_cb_result *__restrict__ res)
 {
-    __shared__ uint32_t tab[48];
+    __shared__ uint32_t tab[MQ_TT_WORDS];
     __shared__ __attribute__((aligned(16))) uint8_t ring[MQ2_LDS_BYTES];
     t1_mq_lanes<false, true>(tab, ring, nullptr, blockIdx.x * 64 + wv_lane(), ncb, syms, symcap, cm, slots, slot,
                              res);
@@ -24,7 +24,7 @@ __global__ void __launch_bounds__(128) t1_mq_kernel(const uint8_t *__restrict__ 
 
 __global__ void __launch_bounds__(128) t1_mq_jobs_kernel(const T1Job *__restrict__ jobs, int njobs)
 {
-    __shared__ uint32_t tab[48];
+    __shared__ uint32_t tab[MQ_TT_WORDS];
     __shared__ __attribute__((aligned(16))) uint8_t ring[MQ2_LDS_BYTES];
     t1_mq_lanes<true, true>(tab, ring, jobs, blockIdx.x * 64 + wv_lane(), njobs, nullptr, 0, nullptr, nullptr, 0,
                             nullptr);

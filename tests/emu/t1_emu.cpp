@@ -152,7 +152,7 @@ void run(void (*fn)(void *), void *arg)
 }  // namespace emu
 
 namespace {
-uint32_t g_tab[48];
+uint32_t g_tab[MQ_TT_WORDS];
 alignas(16) uint8_t g_lut[CM_LDS_BYTES];   // the kernels' workgroup memory: context tables, symbol selectors
 alignas(16) uint8_t g_ring[MQ_LDS_BYTES]; // and the lane MQ coders' output rings and context words
 
