@@ -712,11 +712,11 @@ RG_DEV void ring_byteout(RingMQ &m, bool need, uint8_t *rb)
 // the renormalisation shift is a plain count of leading zeros and "A - Qe
 // below 0x8000" is the sign bit.  A is updated and renormalised, *enew is the
 // context's entry after the symbol.  Returns the symbol's record for the code
-// side: the amount added to C (Qe when the upper interval is kept, else 0) |
-// the renormalisation shift << 16.
+// side: the amount added to C (Qe when the upper interval is kept, else 0) <<
+// 16 | the renormalisation shift.
 RG_DEV uint32_t mq_interval(uint32_t &A, uint32_t d, uint32_t e, uint32_t tnew, uint32_t &enew)
 {
-    const uint32_t qe = e & 0xFFFFu, qe32 = e << 16;
+    const uint32_t qe32 = e << 16;
     const bool lps = ((e >> 16) ^ d) & 1u;
     const uint32_t a = A - qe32;
     const bool renorm = lps || (int32_t)a >= 0;
@@ -728,7 +728,7 @@ RG_DEV uint32_t mq_interval(uint32_t &A, uint32_t d, uint32_t e, uint32_t tnew, 
     enew = renorm ? (tnew ^ (e & 0x10000u)) : e;
     const uint32_t n = (uint32_t)__builtin_clz(A);   // A != 0
     A <<= n;
-    return (upper ? qe : 0u) | (n << 16);
+    return (upper ? qe32 : 0u) | n;
 }
 
 // Transition-table word index of decision d in a context with entry e.
@@ -755,8 +755,8 @@ RG_DEV void ring_renorm(RingMQ &m, int n, uint8_t *rb)
 // addition, then RENORME's shifts with their BYTEOUTs.
 RG_DEV void mq_code(RingMQ &m, uint32_t rec, uint8_t *rb)
 {
-    m.C += rec & 0xFFFFu;
-    ring_renorm(m, (int)(rec >> 16), rb);
+    m.C += rec >> 16;
+    ring_renorm(m, (int)(rec & 0xFFu), rb);
 }
 
 // mq_code for a shift of at most 7 (so at most one BYTEOUT: after one, CT is
@@ -765,12 +765,12 @@ RG_DEV void mq_code(RingMQ &m, uint32_t rec, uint8_t *rb)
 // width, the mask and the remaining shift.  Same state updates as mq_code.
 RG_DEV void mq_code7(RingMQ &m, uint32_t rec, uint8_t *rb)
 {
-    const int n = (int)(rec >> 16);
+    const int n = (int)(rec & 0xFFu);
     const bool need = n >= m.CT;                     // CT reaches 0 inside this shift
     const int sft = need ? m.CT : n;
     const uint32_t isff = m.B == 0xFFu ? 1u : 0u, isfe = m.B == 0xFEu ? 1u : 0u;
     const uint32_t cyok = need && !isff ? 1u : 0u;   // a carry bit would go into B
-    const uint32_t x = (m.C + (rec & 0xFFFFu)) << sft;
+    const uint32_t x = (m.C + (rec >> 16)) << sft;
     const uint32_t cy = (x >> 27) & cyok;
     // after B = 0xFF (as it was, or by this carry) the next byte takes 7 bits
     const uint32_t big = isff | (isfe & cy);
@@ -833,22 +833,25 @@ RG_DEV void mq_interval_chunk(const uint32_t *tab, uint32_t *ce, uint32_t &A, co
 {
     if (full) {
         uint32_t sy = w4[0] & 0xFFu;
-        uint32_t e = ce[64 * (sy >> 1)];
+        uint32_t ci = 64 * (sy >> 1);   // the context's word in ce
+        uint32_t e = ce[ci];
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            const uint32_t cx = sy >> 1, d = sy & 1u;
+            const uint32_t d = sy & 1u;
             const uint32_t tnew = tab[tt_index(d, e)];
-            uint32_t sy2 = 0, e2 = 0;
+            uint32_t sy2 = 0, ci2 = 0, e2 = 0;
             if (k < 15) {   // the next symbol's context, read before this one's store
                 sy2 = (w4[(k + 1) >> 2] >> (8 * ((k + 1) & 3))) & 0xFFu;
-                e2 = ce[64 * (sy2 >> 1)];
+                ci2 = 64 * (sy2 >> 1);
+                e2 = ce[ci2];
             }
             wv_sched_fence();
             uint32_t enew;
             rec[k] = mq_interval(A, d, e, tnew, enew);
-            ce[64 * cx] = enew;
-            if (k < 15) e = (sy2 >> 1) == cx ? enew : e2;
+            ce[ci] = enew;
+            if (k < 15) e = ci2 == ci ? enew : e2;
             sy = sy2;
+            ci = ci2;
         }
         return;
     }
@@ -927,7 +930,7 @@ RG_DEV int ring_chain(const uint32_t *tab, uint8_t *ring, bool active, const uin
             uint32_t big = 0;
 #pragma unroll
             for (int k = 0; k < 16; ++k) big |= rec[k];
-            if (wv_any((big & 0x80000u) != 0)) {
+            if (wv_any((big & 0x8u) != 0)) {
 #pragma unroll
                 for (int k = 0; k < 16; ++k) mq_code(m, rec[k], rb);
             } else {
