@@ -1182,7 +1182,41 @@ RG_DEV void t1_pack_block(const i32x4 &c, const int32_t *poff, const uint8_t *se
     wv_sync();
     const int lim = total < cap ? total : cap;
     int p = 0;   // this lane's current pass: pos[p] <= k < pos[p + 1]
-    for (int k0 = 0; k0 < lim; k0 += 256) {
+    // Whole output dwords first (4-byte aligned slot): a dword inside one pass
+    // is two aligned source dwords funnel-shifted (segments start 16-byte
+    // aligned with room after their end, so the second read stays inside the
+    // arena); one across a pass boundary is gathered bytewise.
+    const int nd = ((uintptr_t)out & 3) == 0 ? lim >> 2 : 0;
+    for (int j0 = 0; j0 < nd; j0 += 256) {
+        uint32_t v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = j0 + 64 * u + lane, k = 4 * j;
+            v[u] = 0;
+            if (j < nd) {
+                while (pos[p + 1] <= k) ++p;
+                if (k + 3 < pos[p + 1]) {
+                    const int so = soff[p] + (k - pos[p]);
+                    const uint32_t *w = reinterpret_cast<const uint32_t *>(seg + (so & ~3));
+                    v[u] = (uint32_t)((((uint64_t)w[1] << 32) | w[0]) >> (8 * (so & 3)));
+                } else {
+                    int q = p;
+                    for (int b = 0; b < 4; ++b) {
+                        while (pos[q + 1] <= k + b) ++q;
+                        v[u] |= (uint32_t)seg[soff[q] + (k + b - pos[q])] << (8 * b);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = j0 + 64 * u + lane;
+            if (j < nd) reinterpret_cast<uint32_t *>(out)[j] = v[u];
+        }
+    }
+    // the remaining bytes (all of them if the slot is not 4-byte aligned)
+    p = 0;
+    for (int k0 = 4 * nd; k0 < lim; k0 += 256) {
         uint32_t v[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
