@@ -40,6 +40,22 @@ __device__ __forceinline__ void px_fwd(const int (&v)[NC], T (&o)[NC])
     }
 }
 
+// The FP32 modes from the level-shifted samples already as floats (exact:
+// integers below 2^24), the same arithmetic as px_fwd.
+template <int NC, int MODE>
+__device__ __forceinline__ void px_fwd_f(const float (&v)[NC], float (&o)[NC])
+{
+    static_assert(MODE == 0 || MODE == 2, "FP32 modes");
+    if constexpr (MODE == 2) {
+        o[0] = ict_y(v[0], v[1], v[2]);
+        o[1] = ict_cb(v[0], v[1], v[2]);
+        o[2] = ict_cr(v[0], v[1], v[2]);
+    } else {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) o[c] = v[c];
+    }
+}
+
 __device__ __forceinline__ uint8_t clamp_u8(int x) { return (uint8_t)(x < 0 ? 0 : (x > 255 ? 255 : x)); }
 // A reconstructed FP32 sample (level shifted) to uint8: + 128, rounded half
 // up, clamped.

@@ -406,21 +406,25 @@ __device__ __forceinline__ void fwd_tile_plain97(const rg_fwd_level_args &a, int
 // items are laid out for full wavefronts and nothing is converted twice.
 // LDS: NC x (SEG + 2H) x LPS words.
 template <int TW> constexpr int fwd1c_lps() { return lcol(TW + 8) + 1; }
-template <int F, int NC, int TW> constexpr int fwd1c_lds_bytes() { return NC * (SEG + 2 * Filt<F>::H) * fwd1c_lps<TW>() * 4; }
+template <int F, int NC, int TW, int TH> constexpr int fwd1c_lds_bytes()
+{
+    return NC * (TH + 2 * Filt<F>::H) * fwd1c_lps<TW>() * 4;
+}
 
-template <int F, int NC, int MODE, int TW, int WGS>
+template <int F, int NC, int MODE, int TW, int TH, int WGS>
 __device__ __forceinline__ void fwd1_colour_tile(const rg_fwd1c_args &a, int bx, int by, char *lds)
 {
     using T = typename Filt<F>::T;
     constexpr int H = Filt<F>::H;
-    constexpr int TH = SEG;
     constexpr int CW = TW + 2 * H, RH = TH + 2 * H;
     constexpr int SW = TW + 8;                           // staged columns: image columns x0 - 4 .. x0 + TW + 4
     constexpr int LPS = fwd1c_lps<TW>();                 // staged and tile row pitch (words)
-    constexpr int N = SEG + 2 * H;
+    constexpr int N = SEG + 2 * H;                       // a row item's window
+    constexpr int NV = TH + 2 * H;                       // a column item's window
     constexpr int HS = TW / SEG;
     constexpr int RS = WGS / TW;                         // rows per store round
-    static_assert(TW % 64 == 0 && WGS % TW == 0 && TH % RS == 0 && NC * CW <= WGS, "fused tile / workgroup shape");
+    static_assert(TW % 64 == 0 && WGS % TW == 0 && TH % RS == 0 && TH % SEG == 0 && NC * CW <= WGS,
+                  "fused tile / workgroup shape");
     T *fst = reinterpret_cast<T *>(lds);                 // component k, staged row r at (k * RH + r) * LPS
     T *tile = fst;                                       // then component k, row j at (k * TH + j) * LPS
     const int x0 = bx * TW, y0 = by * TH;
@@ -466,11 +470,21 @@ __device__ __forceinline__ void fwd1_colour_tile(const rg_fwd1c_args &a, int bx,
         if (i < NIT && x >= xs && x < xe) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                int v[NC];
-#pragma unroll
-                for (int k = 0; k < NC; ++k) v[k] = (int)((wq[t][(NC * j + k) >> 2] >> (((NC * j + k) & 3) * 8)) & 0xFFu) - 128;
                 T o[NC];
-                rg_px::px_fwd<NC, MODE, F == RG_FILTER_97>(v, o);
+                if constexpr (F == RG_FILTER_97) {
+                    // byte to float in one conversion, then the level shift (exact)
+                    float v[NC];
+#pragma unroll
+                    for (int k = 0; k < NC; ++k)
+                        v[k] = (float)((wq[t][(NC * j + k) >> 2] >> (((NC * j + k) & 3) * 8)) & 0xFFu) - 128.0f;
+                    rg_px::px_fwd_f<NC, MODE>(v, o);
+                } else {
+                    int v[NC];
+#pragma unroll
+                    for (int k = 0; k < NC; ++k)
+                        v[k] = (int)((wq[t][(NC * j + k) >> 2] >> (((NC * j + k) & 3) * 8)) & 0xFFu) - 128;
+                    rg_px::px_fwd<NC, MODE, false>(v, o);
+                }
 #pragma unroll
                 for (int k = 0; k < NC; ++k) fst[(k * RH + r) * LPS + lcol(4 * q + j)] = o[k];
             }
@@ -481,20 +495,20 @@ __device__ __forceinline__ void fwd1_colour_tile(const rg_fwd1c_args &a, int bx,
     // 2. columns: items (component, column), component slowest; one round
     const int k2 = threadIdx.x / CW, c2 = threadIdx.x - k2 * CW;
     const bool colt = threadIdx.x < NC * CW;
-    T e[N];
+    T e[NV];
     if (colt) {
         // every column an output inside the image needs mirrors into [xs, xe);
         // the clamp only keeps the reads of the other columns in the span
         const int x = min(max(mirror1(x0 - H + c2, w), xs), xe - 1);
         const T *col = fst + k2 * RH * LPS + lcol(x - (x0 - 4));
 #pragma unroll
-        for (int r = 0; r < N; ++r) e[r] = col[r * LPS];
+        for (int r = 0; r < NV; ++r) e[r] = col[r * LPS];
         if constexpr (F == RG_FILTER_53) fwd53(e); else fwd97(e, true);
     }
     __syncthreads();
     if (colt) {
 #pragma unroll
-        for (int j = 0; j < SEG; ++j) tile[(k2 * TH + j) * LPS + lcol(c2)] = e[H + j];
+        for (int j = 0; j < TH; ++j) tile[(k2 * TH + j) * LPS + lcol(c2)] = e[H + j];
     }
     __syncthreads();
 
@@ -964,14 +978,14 @@ __global__ void __launch_bounds__(FWG) fwd_level_kernel(rg_fwd_level_args a)
     else fwd_tile<F, FTW, FTH, FWG>(a, plane, bx, by, lds);
 }
 
-// Fused colour + level 1, one workgroup per TW x 16 tile of the image.
-template <int F, int NC, int MODE, int TW, int WGS>
+// Fused colour + level 1, one workgroup per TW x TH tile of the image.
+template <int F, int NC, int MODE, int TW, int TH, int WGS>
 __global__ void __launch_bounds__(WGS) fwd1_colour_kernel(rg_fwd1c_args a)
 {
-    __shared__ __attribute__((aligned(16))) char lds[fwd1c_lds_bytes<F, NC, TW>()];
+    __shared__ __attribute__((aligned(16))) char lds[fwd1c_lds_bytes<F, NC, TW, TH>()];
     int plane, bx, by;
     xcd_tile(plane, bx, by);
-    fwd1_colour_tile<F, NC, MODE, TW, WGS>(a, bx, by, lds);
+    fwd1_colour_tile<F, NC, MODE, TW, TH, WGS>(a, bx, by, lds);
 }
 
 // Last inverse level with the inverse colour stage, one workgroup per
@@ -1072,19 +1086,24 @@ bool rg_fwd1_colour_ok(int w, int h, int ncomp, int colour, int filter)
 
 namespace {
 // Fused tile shapes: 64 x 16 with 256 threads (22 KB of LDS for three
-// components, 7 workgroups a CU) or 128 x 16 with 512 threads (42 KB, 3 a
-// CU; RG_DWT_FUSED_TW=128).  4096x4096 RGB ICT + 9/7 with caches cold: 66 us
-// against 85 us.
+// components, 7 workgroups a CU), 64 x 32 with 256 threads (37 KB, 4 a CU;
+// RG_DWT_FUSED_TH=32: fewer halo rows staged and lifted, 63 -> 70 us) or
+// 128 x 16 with 512 threads (42 KB, 3 a CU; RG_DWT_FUSED_TW=128).  4096x4096
+// RGB ICT + 9/7 with caches cold: 66 us against 85 us for the last.
 template <int F, int NC, int MODE>
 void launch_fwd1c(hipStream_t s, const rg_fwd1c_args &a)
 {
     static const int tw = [] { const char *e = getenv("RG_DWT_FUSED_TW"); return e && atoi(e) == 128 ? 128 : 64; }();
+    static const int th = [] { const char *e = getenv("RG_DWT_FUSED_TH"); return e && atoi(e) == 32 ? 32 : 16; }();
     if (tw == 128) {
         dim3 grid((a.w + 127) / 128, (a.h + SEG - 1) / SEG, 1);
-        fwd1_colour_kernel<F, NC, MODE, 128, 512><<<grid, dim3(512), 0, s>>>(a);
+        fwd1_colour_kernel<F, NC, MODE, 128, SEG, 512><<<grid, dim3(512), 0, s>>>(a);
+    } else if (th == 16) {
+        dim3 grid((a.w + 63) / 64, (a.h + 15) / 16, 1);
+        fwd1_colour_kernel<F, NC, MODE, 64, 16, 256><<<grid, dim3(256), 0, s>>>(a);
     } else {
-        dim3 grid((a.w + 63) / 64, (a.h + SEG - 1) / SEG, 1);
-        fwd1_colour_kernel<F, NC, MODE, 64, 256><<<grid, dim3(256), 0, s>>>(a);
+        dim3 grid((a.w + 63) / 64, (a.h + 31) / 32, 1);
+        fwd1_colour_kernel<F, NC, MODE, 64, 32, 256><<<grid, dim3(256), 0, s>>>(a);
     }
 }
 }  // namespace
