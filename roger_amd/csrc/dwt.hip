@@ -567,26 +567,48 @@ __device__ __forceinline__ void fwd1_colour_tile(const rg_fwd1c_args &a, int bx,
     const int orow0 = (y0 >> 1) + gsub * HPW + rsub;
     const long first = to_ll ? (long)orow0 * pitch + xo
                              : (long)(lowrow ? orow0 : hl + orow0) * pitch + (half ? wl + xo : xo);
+    // every read of the tile first (all inside the LDS tile, so none needs a
+    // guard), then the stores: one branch on the destination for the whole
+    // wavefront, and no per-store bounds test on tiles inside the image
+    constexpr int NJ = TH / RPR;
+    T v[NC][NJ];
 #pragma unroll
-    for (int k = 0; k < NC; ++k) {
-        T *llk = to_ll ? reinterpret_cast<T *>(a.ll_out) + k * a.ll_ps + first : nullptr;
-        int32_t *qk = to_ll ? nullptr : a.q + k * a.q_ps + first;
+    for (int k = 0; k < NC; ++k)
 #pragma unroll
-        for (int j = 0; j < TH / RPR; ++j) {
-            const int r = par + 2 * (gsub * HPW + rsub + j * (RPR / 2));
-            if (!col_in || y0 + r >= h) continue;
-            const T v = tile[(k * TH + r) * LPS + lcol(half * (TW / 2) + cc)];
-            const long step = (long)j * (RPR / 2) * pitch;
-            if (to_ll) {
-                st_nt(llk + step, v);
-            } else if constexpr (F == RG_FILTER_53) {
-                st_nt(qk + step, v);
-            } else {
-                // sign(v) * floor(|v| * inv) == truncation of v * inv (IEEE products are sign-symmetric)
-                st_nt(qk + step, (int)(v * inv));
+        for (int j = 0; j < NJ; ++j)
+            v[k][j] = tile[(k * TH + par + 2 * (gsub * HPW + rsub + j * (RPR / 2))) * LPS + lcol(half * (TW / 2) + cc)];
+    const bool inner = x0 + TW <= w && y0 + TH <= h;
+    const int rlim = h - y0 - par - 2 * (gsub * HPW + rsub);   // round j stores while j * RPR < rlim
+    const long step = (long)(RPR / 2) * pitch;
+    auto put = [&](auto guarded) {
+        constexpr bool G = decltype(guarded)::value;
+        if (to_ll) {
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                T *llk = reinterpret_cast<T *>(a.ll_out) + k * a.ll_ps + first;
+#pragma unroll
+                for (int j = 0; j < NJ; ++j)
+                    if (!G || (col_in && j * RPR < rlim)) st_nt(llk + j * step, v[k][j]);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                int32_t *qk = a.q + k * a.q_ps + first;
+#pragma unroll
+                for (int j = 0; j < NJ; ++j) {
+                    if (G && !(col_in && j * RPR < rlim)) continue;
+                    if constexpr (F == RG_FILTER_53) {
+                        st_nt(qk + j * step, v[k][j]);
+                    } else {
+                        // sign(v) * floor(|v| * inv) == truncation of v * inv (IEEE products are sign-symmetric)
+                        st_nt(qk + j * step, (int)(v[k][j] * inv));
+                    }
+                }
             }
         }
-    }
+    };
+    if (inner) put(std::false_type{});
+    else put(std::true_type{});
 }
 
 // ============================ inverse ============================

@@ -348,11 +348,31 @@ RG_DEV void t1_dec_block(const DecJob &job, uint8_t *lut, uint32_t *rawbuf)
                 cand = col_ok ? cand & 0xFu : 0u;
                 uint64_t cols = wv_ballot(cand != 0);
                 const bool full = r0 + 4 <= h;
-                int lastj = -2;
+                // Cleanup: columns in run-length mode (D.3.4: four insignificant,
+                // unvisited rows and no significant sample in the 3 x 6
+                // neighbourhood) as of the stripe's start.  During the walk only
+                // the left neighbour of a column can change, and only when that
+                // column has just become significant, so a column of this mask
+                // is still in run-length mode unless column j - 1 was the last
+                // one to gain a significant sample (sigj), and no other column
+                // enters the mode.
+                const uint64_t rlm =
+                    pass == 2 && full ? wv_ballot(cand == 0xFu && ((swv | lw | rw) & 0x3Fu) == 0u) : 0ull;
+                int lastj = -2, sigj = -2;
                 uint32_t Lw = 0, Lneg = 0;
                 while (cols) {
                     const int j = __builtin_ctzll(cols);
                     cols &= cols - 1;
+                    int rl = 0;   // run-length decision 1: a significant row follows
+                    if (((rlm >> j) & 1u) && sigj != j - 1) {
+                        // the run-length decision, before any neighbourhood read:
+                        // a 0 leaves the column as it is (no window of it changes)
+                        rl = mqd_decode(m, ctxv, mqtv, CTX_RL);
+                        if (!rl) {
+                            lastj = -2;
+                            continue;
+                        }
+                    }
                     uint32_t Cw = (uint32_t)wv_readlane((int)swv, j);
                     uint32_t Cneg = (uint32_t)wv_readlane((int)negv, j);
                     if (j != lastj + 1) {   // else: carried from the column just decoded
@@ -388,34 +408,29 @@ RG_DEV void t1_dec_block(const DecJob &job, uint8_t *lut, uint32_t *rawbuf)
                     } else {
                         const uint32_t cj = (uint32_t)wv_readlane((int)cand, j);
                         int k0 = 0;
-                        bool done = false;
-                        if (full && cj == 0xFu && ((Lw | Rw) & 0x3Fu) == 0 && (Cw & 0x21u) == 0) {
-                            // run-length mode (D.3.4)
-                            if (!mqd_decode(m, ctxv, mqtv, CTX_RL)) {
-                                done = true;
-                            } else {
-                                int k = mqd_decode(m, ctxv, mqtv, CTX_UNI) << 1;
-                                k |= mqd_decode(m, ctxv, mqtv, CTX_UNI);
-                                const int neg = dec_sign(m, ctxv, mqtv, scv, k + 1, Lw, Lneg, Cw, Cneg, Rw, Rneg);
-                                Cw |= 1u << (k + 1);
-                                Cneg |= (uint32_t)neg << (k + 1);
-                                nsig |= 1u << k;
-                                k0 = k + 1;
-                            }
+                        if (rl) {
+                            // run-length mode, decision 1: two UNIFORM decisions give
+                            // the first significant row, then its sign
+                            int k = mqd_decode(m, ctxv, mqtv, CTX_UNI) << 1;
+                            k |= mqd_decode(m, ctxv, mqtv, CTX_UNI);
+                            const int neg = dec_sign(m, ctxv, mqtv, scv, k + 1, Lw, Lneg, Cw, Cneg, Rw, Rneg);
+                            Cw |= 1u << (k + 1);
+                            Cneg |= (uint32_t)neg << (k + 1);
+                            nsig |= 1u << k;
+                            k0 = k + 1;
                         }
-                        if (!done) {
-                            for (int k = k0; k < 4; ++k) {
-                                if (!b1(cj, k)) continue;
-                                const int i = k + 1;
-                                if (mqd_decode(m, ctxv, mqtv, dec_zc(zcv, i, Lw, Cw, Rw))) {
-                                    const int neg = dec_sign(m, ctxv, mqtv, scv, i, Lw, Lneg, Cw, Cneg, Rw, Rneg);
-                                    Cw |= 1u << i;
-                                    Cneg |= (uint32_t)neg << i;
-                                    nsig |= 1u << k;
-                                }
+                        for (int k = k0; k < 4; ++k) {
+                            if (!b1(cj, k)) continue;
+                            const int i = k + 1;
+                            if (mqd_decode(m, ctxv, mqtv, dec_zc(zcv, i, Lw, Cw, Rw))) {
+                                const int neg = dec_sign(m, ctxv, mqtv, scv, i, Lw, Lneg, Cw, Cneg, Rw, Rneg);
+                                Cw |= 1u << i;
+                                Cneg |= (uint32_t)neg << i;
+                                nsig |= 1u << k;
                             }
                         }
                     }
+                    if (nsig) sigj = j;
                     if (vis | nsig) {
                         if (lane == j) {
                             V |= (uint64_t)vis << r0;
