@@ -202,7 +202,7 @@ size_t rg_t1_chain_ws_bytes(int nblocks);
 // bytes per block of modelling state kept between the phases.
 #define RG_T1_PHASE_SPLIT0 4   // four-phase form only
 #define RG_T1_PHASE_SPLIT1 7
-#define RG_T1_PHASE_SPLIT 13
+#define RG_T1_PHASE_SPLIT 16
 void rg_t1_splits(int *s1, int *s2);
 // number of modelling phases of a parallel code-block style: RG_T1_PHASES
 // (1-4) if set, RG_T1_TWO_PHASE=0 one; else two with BYPASS and one without

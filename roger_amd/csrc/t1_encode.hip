@@ -298,8 +298,10 @@ void rg_t1_job_fill(void *dst, const int32_t *src, int pitch, int w, int h, int 
 // BYPASS 2.71 ms in one phase, 2.49 in two (split before pass 13; 2.52 at 10,
 // 2.51 at 16), 2.67 in three (7, 10); RESET|TERMALL 3.02 / 3.15 / 3.36 ms --
 // with every pass an MQ chain, the early chains' coding does not hide behind
-// the rest of the modelling.  RG_T1_PHASES=1..4 overrides (RG_T1_TWO_PHASE=0:
-// one).
+// the rest of the modelling.  With the faster coders since then (best of 4 x
+// 10 encodes, three runs each): two phases split before pass 10 1.83 ms, 13
+// 1.79-1.80, 16 1.78, 19 1.81, 22 1.82-1.83; one phase 1.82; three 2.02-2.07.
+// RG_T1_PHASES=1..4 overrides (RG_T1_TWO_PHASE=0: one).
 int rg_t1_phases(int style)
 {
     static int v = -1;
