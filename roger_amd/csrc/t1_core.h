@@ -1181,13 +1181,62 @@ RG_DEV void t1_pack_block(const i32x4 &c, const int32_t *poff, const uint8_t *se
     if (lane == 0) pos[MAXP] = total;
     wv_sync();
     const int lim = total < cap ? total : cap;
-    int p = 0;   // this lane's current pass: pos[p] <= k < pos[p + 1]
-    // Whole output dwords first (4-byte aligned slot): a dword inside one pass
-    // is two aligned source dwords funnel-shifted (segments start 16-byte
-    // aligned with room after their end, so the second read stays inside the
-    // arena); one across a pass boundary is gathered bytewise.
+    int p = 0;   // this lane's current pass: pos[p] <= k < pos[p + 1] (k only grows)
+    // Whole 16-byte output chunks first (16-byte aligned slot): a chunk inside
+    // one pass is five aligned source dwords byte-aligned into four (segments
+    // start 16-byte aligned with room after their end, so the reads past the
+    // segment stay inside the arena); one across a pass boundary is gathered
+    // bytewise.
+    const int n16 = ((uintptr_t)out & 15) == 0 ? lim >> 4 : 0;
+    constexpr int P16 = 4;   // chunks per lane in flight
+    for (int j0 = 0; j0 < n16; j0 += 64 * P16) {
+        uint32_t v[P16][4];
+#pragma unroll
+        for (int u = 0; u < P16; ++u) {
+            const int j = j0 + 64 * u + lane, k = 16 * j;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[u][i] = 0;
+            if (j < n16) {
+                while (pos[p + 1] <= k) ++p;
+                if (k + 15 < pos[p + 1]) {
+                    const int so = soff[p] + (k - pos[p]);
+                    const uint32_t *w = reinterpret_cast<const uint32_t *>(seg + (so & ~3));
+                    uint32_t a[5];
+#pragma unroll
+                    for (int i = 0; i < 5; ++i) a[i] = w[i];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        v[u][i] = (uint32_t)((((uint64_t)a[i + 1] << 32) | a[i]) >> (8 * (so & 3)));
+                } else {
+                    // every source offset first (workgroup memory only), then
+                    // the 16 byte loads together
+                    int q = p, src[16];
+#pragma unroll
+                    for (int b = 0; b < 16; ++b) {
+                        while (pos[q + 1] <= k + b) ++q;
+                        src[b] = soff[q] + (k + b - pos[q]);
+                    }
+                    uint32_t by[16];
+#pragma unroll
+                    for (int b = 0; b < 16; ++b) by[b] = seg[src[b]];
+#pragma unroll
+                    for (int b = 0; b < 16; ++b) v[u][b >> 2] |= by[b] << (8 * (b & 3));
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < P16; ++u) {
+            const int j = j0 + 64 * u + lane;
+            if (j < n16) {
+                *reinterpret_cast<u32x4 *>(out + 16 * j) = mk_u32x4(v[u][0], v[u][1], v[u][2], v[u][3]);
+            }
+        }
+    }
+    // Then whole output dwords (4-byte aligned slot): a dword inside one pass
+    // is two aligned source dwords funnel-shifted; one across a pass boundary
+    // is gathered bytewise.
     const int nd = ((uintptr_t)out & 3) == 0 ? lim >> 2 : 0;
-    for (int j0 = 0; j0 < nd; j0 += 256) {
+    for (int j0 = 4 * n16; j0 < nd; j0 += 256) {
         uint32_t v[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -1200,11 +1249,14 @@ RG_DEV void t1_pack_block(const i32x4 &c, const int32_t *poff, const uint8_t *se
                     const uint32_t *w = reinterpret_cast<const uint32_t *>(seg + (so & ~3));
                     v[u] = (uint32_t)((((uint64_t)w[1] << 32) | w[0]) >> (8 * (so & 3)));
                 } else {
-                    int q = p;
+                    int q = p, src[4];
+#pragma unroll
                     for (int b = 0; b < 4; ++b) {
                         while (pos[q + 1] <= k + b) ++q;
-                        v[u] |= (uint32_t)seg[soff[q] + (k + b - pos[q])] << (8 * b);
+                        src[b] = soff[q] + (k + b - pos[q]);
                     }
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) v[u] |= (uint32_t)seg[src[b]] << (8 * b);
                 }
             }
         }
@@ -1215,7 +1267,6 @@ RG_DEV void t1_pack_block(const i32x4 &c, const int32_t *poff, const uint8_t *se
         }
     }
     // the remaining bytes (all of them if the slot is not 4-byte aligned)
-    p = 0;
     for (int k0 = 4 * nd; k0 < lim; k0 += 256) {
         uint32_t v[4];
 #pragma unroll
