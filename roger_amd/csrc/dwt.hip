@@ -140,10 +140,13 @@ __device__ __forceinline__ int quant(float v, float inv)
     return v < 0.0f ? -m : m;
 }
 
+// sign(q) (|q| + 1/2) s for q != 0, written as (q + copysign(1/2, q)) s: the
+// conversion, the sum and the product of IEEE arithmetic round the same for
+// q and -q, so both forms give the same bits, with fewer instructions here
 __device__ __forceinline__ float dequant(int q, float s)
 {
-    float m = ((float)(q < 0 ? -q : q) + 0.5f) * s;
-    return q == 0 ? 0.0f : (q < 0 ? -m : m);
+    const float m = (float)q;
+    return q == 0 ? 0.0f : (m + __builtin_copysignf(0.5f, m)) * s;
 }
 
 // ============================ forward ============================
