@@ -386,13 +386,15 @@ RG_DEV void t1_dec_block(const DecJob &job, uint8_t *lut, uint32_t *rawbuf)
                     }
                     uint32_t vis = 0, nsig = 0;
                     if (pass == 0) {
+                        // window rows 1-4 in the pass: valid, insignificant, with a
+                        // significant neighbour (all four at once); a row that
+                        // becomes significant adds the row below it
+                        const uint32_t LR = Lw | Rw, vw = vrow << 1;
+                        uint32_t candm = (LR | (LR >> 1) | (LR << 1) | (Cw >> 1) | (Cw << 1)) & ~Cw & vw;
 #pragma unroll
                         for (int k = 0; k < 4; ++k) {
                             const int i = k + 1;
-                            if (!b1(vrow, k) || b1(Cw, i)) continue;
-                            const uint32_t nb = b1(Cw, i - 1) | b1(Cw, i + 1) | ((Lw >> (i - 1)) & 7u) |
-                                                ((Rw >> (i - 1)) & 7u);
-                            if (!nb) continue;
+                            if (!b1(candm, i)) continue;
                             vis |= 1u << k;
                             const int bit = raw ? raw_bit(m) : mqd_decode(m, ctxv, mqtv, dec_zc(zcv, i, Lw, Cw, Rw));
                             if (bit) {
@@ -401,6 +403,7 @@ RG_DEV void t1_dec_block(const DecJob &job, uint8_t *lut, uint32_t *rawbuf)
                                 Cw |= 1u << i;
                                 Cneg |= (uint32_t)neg << i;
                                 nsig |= 1u << k;
+                                candm |= (2u << i) & ~Cw & vw;
                             }
                         }
                         // a new significant sample can make the next column a candidate
