@@ -19,8 +19,8 @@
 //     adds the next column (whose candidates it can create);
 //   * a visited column's 6-row windows come from its lane and its neighbours'
 //     (readlane), the left one carried over when it was the column just
-//     decoded; ZC and SC contexts are looked up in 256-entry tables kept as
-//     bytes in a VGPR (readlane + shift);
+//     decoded; ZC and SC contexts are looked up in 256-entry tables of 4-bit
+//     entries kept in a VGPR (readlane + shift);
 //   * MQ refinement passes form their contexts in parallel (no intra-pass
 //     dependency) and only decode in order; raw refinement passes (BYPASS)
 //     are fully parallel: the segment is de-stuffed into a bit array in
@@ -166,28 +166,53 @@ RG_DEV int raw_bit(MQDec &m)
     return (int)((m.C >> m.CT) & 1u);
 }
 
-// Byte x of a 256-byte table held 4 bytes per lane.
-RG_DEV uint32_t lut_byte(int v, uint32_t x)
+// The decoder's context tables, 4 bits an entry, 8 entries per lane: entry
+// (lane l, nibble n) of a table held in one VGPR.  The fields are placed so
+// that the lane and the nibble come out of the windows with few scalar
+// operations (windows: bit 0 = the row above the stripe; row i):
+//   ZC: lane = right column rows i-1..i+1 (bits 0-2), own rows i-1 (bit 3)
+//       and i+1 (bit 5); nibble = left column rows i-1..i+1; entry = context;
+//   SC: lane = own row i-1 significant / negative (bits 0 / 1), row i+1
+//       (bits 2 / 3), left row i significant (bit 4); nibble = left row i
+//       negative, right row i significant, right row i negative; entry =
+//       (context - 9) << 1 | XOR bit.
+RG_DEV uint32_t lut_nib(int v, uint32_t l, uint32_t n)
 {
-    return ((uint32_t)wv_readlane(v, (int)(x >> 2)) >> (8 * (x & 3u))) & 0xFFu;
+    return ((uint32_t)wv_readlane(v, (int)l) >> (4 * n)) & 0xFu;
 }
 
-// ZC context of window row i of a column (windows: bit 0 = the row above the
-// stripe), from the table index of t1_lut_fill.
-RG_DEV int dec_zc(int zcv, int i, uint32_t Lw, uint32_t Cw, uint32_t Rw)
+// This lane's words of the two tables, from t1_lut_fill's byte tables.
+RG_DEV void dec_lut_pack(const uint8_t *lut, int &zcn, int &scn)
 {
-    return (int)lut_byte(zcv, ((Lw >> (i - 1)) & 7u) | (((Rw >> (i - 1)) & 7u) << 3) | (b1(Cw, i - 1) << 6) |
-                                  (b1(Cw, i + 1) << 7));
+    const uint32_t l = (uint32_t)wv_lane();
+    uint32_t z = 0, c = 0;
+#pragma unroll
+    for (uint32_t n = 0; n < 8; ++n) {
+        const uint32_t zx = n | ((l & 7u) << 3) | (((l >> 3) & 1u) << 6) | (((l >> 5) & 1u) << 7);
+        z |= (uint32_t)lut[zx] << (4 * n);
+        const uint32_t sx = ((l >> 4) & 1u) | ((n & 1u) << 1) | (((n >> 1) & 1u) << 2) | (((n >> 2) & 1u) << 3) |
+                            ((l & 1u) << 4) | (((l >> 1) & 1u) << 5) | (((l >> 2) & 1u) << 6) | (((l >> 3) & 1u) << 7);
+        const uint32_t sc = lut[256 + sx];
+        c |= (((((sc >> 1) - 9u) & 7u) << 1) | (sc & 1u)) << (4 * n);
+    }
+    zcn = (int)z;
+    scn = (int)c;
+}
+
+// ZC context of window row i of a column.
+RG_DEV int dec_zc(int zcn, int i, uint32_t Lw, uint32_t Cw, uint32_t Rw)
+{
+    return (int)lut_nib(zcn, ((Rw >> (i - 1)) & 7u) | (((Cw >> (i - 1)) & 5u) << 3), (Lw >> (i - 1)) & 7u);
 }
 
 // Sign of window row i: MQ-decoded with its SC context and XOR bit.
-RG_DEV int dec_sign(MQDec &m, int &ctxv, int mqtv, int scv, int i, uint32_t Lw, uint32_t Lneg, uint32_t Cw,
+RG_DEV int dec_sign(MQDec &m, int &ctxv, int mqtv, int scn, int i, uint32_t Lw, uint32_t Lneg, uint32_t Cw,
                     uint32_t Cneg, uint32_t Rw, uint32_t Rneg)
 {
-    const uint32_t x = b1(Lw, i) | (b1(Lneg, i) << 1) | (b1(Rw, i) << 2) | (b1(Rneg, i) << 3) | (b1(Cw, i - 1) << 4) |
-                       (b1(Cneg, i - 1) << 5) | (b1(Cw, i + 1) << 6) | (b1(Cneg, i + 1) << 7);
-    const uint32_t sc = lut_byte(scv, x);
-    return mqd_decode(m, ctxv, mqtv, (int)(sc >> 1)) ^ (int)(sc & 1u);
+    const uint32_t l = ((Cw >> (i - 1)) & 5u) | (((Cneg >> (i - 1)) & 5u) << 1) | (b1(Lw, i) << 4);
+    const uint32_t n = b1(Lneg, i) | (b1(Rw, i) << 1) | (b1(Rneg, i) << 2);
+    const uint32_t sc = lut_nib(scn, l, n);
+    return mqd_decode(m, ctxv, mqtv, 9 + (int)(sc >> 1)) ^ (int)(sc & 1u);
 }
 
 // Whole raw refinement pass: segment [p, p + n); every significant, not yet
@@ -257,10 +282,8 @@ RG_DEV void t1_dec_block(const DecJob &job, uint8_t *lut, uint32_t *rawbuf)
     const int lane = wv_lane();
     const int w = job.w, h = job.h, nbp = job.nbp;
     t1_lut_fill(lut, job.orient);
-    const int zcv = (int)((uint32_t)lut[4 * lane] | ((uint32_t)lut[4 * lane + 1] << 8) |
-                          ((uint32_t)lut[4 * lane + 2] << 16) | ((uint32_t)lut[4 * lane + 3] << 24));
-    const int scv = (int)((uint32_t)lut[256 + 4 * lane] | ((uint32_t)lut[256 + 4 * lane + 1] << 8) |
-                          ((uint32_t)lut[256 + 4 * lane + 2] << 16) | ((uint32_t)lut[256 + 4 * lane + 3] << 24));
+    int zcv, scv;
+    dec_lut_pack(lut, zcv, scv);
 
     uint32_t mag[64];
 #pragma unroll
